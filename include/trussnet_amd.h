@@ -1,0 +1,179 @@
+/*
+ * trussnet_amd.h -- C ABI of the MI355X (gfx950) truss relaxation.
+ *
+ * The relaxation moves mesh nodes inside a multi-label volume until the truss
+ * of their K nearest neighbours is in equilibrium, trapping nodes on the smooth
+ * label interfaces. It is the drop-in for v2mesh's relaxation entry points:
+ *
+ *   void tn::relax_cl (const Grid&, const RelaxParams&, Nodes&, RelaxStats&, int device)
+ *        (src/v2m_gpu.h:18, implemented at src/v2m_gpu.cpp:116-553)
+ *   void tn::relax_cpu(const Grid&, const RelaxParams&, Nodes&, RelaxStats&)
+ *        (src/v2m_particles.h:99, implemented at src/v2m_particles.cpp:728-976)
+ *
+ * The grid is read-only. The nodes are mutated in place: P, typ and part; lab
+ * and the node count are unchanged (src/v2m_gpu.cpp:514-517). Every entry point
+ * returns TN_OK or an error status; tn_last_error() gives the message of the
+ * calling thread's last failure. The C++ wrapper (trussnet_amd.hpp) turns a
+ * status into std::runtime_error, which is what relax_cl throws
+ * (src/v2m_gpu.h:16-17) and what the pipeline catches to fall back to the CPU
+ * (src/v2m_pipeline.cpp:497-526).
+ *
+ * Plain pointers and sizes only: no HIP or torch types cross this boundary.
+ */
+#ifndef TRUSSNET_AMD_H
+#define TRUSSNET_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TN_ABI_VERSION 1
+
+/* Sizes fixed by the node and field formats. */
+#define TN_K 24          /* truss neighbours per node: V2M_K, src/opencl/v2m_particle_body.cl:18 */
+#define TN_MAXLEV 8      /* hash levels: V2M_MAXLEV, src/opencl/v2m_particle_body.cl:19 */
+#define TN_BS 8          /* brick side: V2M_BS, src/opencl/v2m_grid_body.cl:20 */
+#define TN_SLOT 512      /* phi values per (brick, label) slot: V2M_SLOT, :21 */
+#define TN_BL 16         /* labels per brick: V2M_BL, :22 */
+#define TN_NOLAB 0xFFFF  /* no label: V2M_NOLAB, src/opencl/v2m_seed_body.cl:19 */
+
+/* Node types (src/opencl/v2m_seed_body.cl:15-18). */
+enum { TN_INTERIOR = 0, TN_INTERFACE = 1, TN_JUNCTION = 2, TN_CORNER = 3 };
+
+/* Status codes. */
+enum {
+    TN_OK = 0,
+    TN_ERR_ARG = 1,         /* a null pointer, a size mismatch, an out-of-range value */
+    TN_ERR_NODEVICE = 2,    /* no HIP device, or the device index is out of range */
+    TN_ERR_HIP = 3,         /* a HIP runtime call failed */
+    TN_ERR_UNSUPPORTED = 4, /* a mode this build does not run on the device (gray / shape fields, voxel trap) */
+    TN_ERR_INTERNAL = 5
+};
+
+/*
+ * The field the nodes move in: v2mesh's Grid (src/v2m_grid.h:45-81) as raw
+ * arrays, x fastest. Voxel (i,j,k) is centred at (i*vs[0], j*vs[1], k*vs[2])
+ * "grid mm". Bricks are 8^3 voxels; nb = nbx*nby*nbz.
+ */
+typedef struct tn_grid_view {
+    int32_t nx, ny, nz;       /* voxels */
+    int32_t nbx, nby, nbz;    /* bricks: ceil(n / 8) per axis */
+    float vs[3];              /* voxel size (mm) */
+    const uint16_t* L;        /* [nx*ny*nz] labels (Grid::L) */
+    const int32_t* bl_cnt;    /* [nb] labels within reach of the brick (TN_BL + 1 = overflow) */
+    const uint16_t* bl_lab;   /* [nb*TN_BL] those labels, ascending, TN_NOLAB padded */
+    const int32_t* bl_slot;   /* [nb] first slot of the brick, -1 = uniform */
+    const float* phi;         /* [nslot*TN_SLOT] smoothed indicators per (brick, label) slot */
+    int64_t nslot;
+    const float* h;           /* [nx*ny*nz] sizing field (mm) */
+    float hmin, hmax;         /* sizing range (Grid::hmin/hmax): the hash geometry */
+    const float* gI;          /* gray-scale intensity (gm > 0), else null */
+    const float* gTW;         /* gray thresholds / shape programs (gm != 0), else null */
+    int64_t ngTW;
+    int32_t gm;               /* 0: label volume (the device path); != 0: unsupported here */
+} tn_grid_view;
+
+/* RelaxParams (src/v2m_particles.h:21-41) plus the rebuild divisor that the
+ * reference reads from the environment (V2M_REBUILD_DIV, src/v2m_particles.cpp:731). */
+typedef struct tn_relax_params {
+    int32_t nseed;
+    float t, skin, fscale, fsurf, dt, maxstep, snap;
+    int32_t max_iters;
+    float dptol, jseed;
+    int32_t corners, voxel_trap;
+    float thin;
+    int32_t fire;
+    float fire_dtmax;
+    int32_t verbose;
+    int32_t rebuild_div;      /* <= 0: 300, the reference's default */
+} tn_relax_params;
+
+/* Nodes (src/v2m_particles.h:78-87). P is 3 floats per node (AoS). */
+typedef struct tn_nodes_view {
+    int64_t n;
+    float* P;                 /* [3n] in/out */
+    const uint16_t* lab;      /* [n] in */
+    uint8_t* typ;             /* [n] in/out */
+    uint16_t* part;           /* [2n] in/out */
+} tn_nodes_view;
+
+/* RelaxStats (src/v2m_particles.h:89-96). The ms_* fields are device times. */
+typedef struct tn_relax_stats {
+    int32_t iters, rebuilds;
+    float last_move, last_p99;
+    int32_t fire_resets;
+    float fire_dt;
+    int64_t n_interior, n_interface, n_junction, n_corner;
+    double ms_seed, ms_hash, ms_force, ms_move;
+} tn_relax_stats;
+
+/* The RelaxParams defaults (src/v2m_particles.h:22-39), rebuild_div 300. */
+void tn_relax_params_default(tn_relax_params* prm);
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* tn_last_error(void);
+
+/* Number of HIP devices (0 and TN_ERR_NODEVICE when there is none). */
+int tn_device_count(int32_t* count);
+
+/*
+ * Relax the nodes on HIP device `device` (-1 = the first). Same contract as
+ * relax_cl (src/v2m_gpu.h:16-18) and relax_cpu (src/v2m_particles.h:99):
+ * uploads the grid and nodes, runs the loop on the device, writes back P, typ
+ * and part, fills *st. On error the nodes are left unchanged.
+ */
+int tn_relax(const tn_grid_view* g, const tn_relax_params* prm, tn_nodes_view* nd, tn_relax_stats* st,
+             int32_t device);
+
+/*
+ * A relaxation plan: the grid and the seed nodes resident on one device, with
+ * every buffer the loop needs allocated once. tn_plan_run restores the seeds on
+ * the device and relaxes them, so repeated runs do the same work with no host
+ * traffic; the relaxed nodes stay on the device until tn_plan_download. This
+ * is relax_cl split at its host/device crossings (upload at
+ * src/v2m_gpu.cpp:187-204, loop at :303-405, read-back at :514-517).
+ */
+typedef struct tn_plan tn_plan;
+
+int tn_plan_create(const tn_grid_view* g, const tn_relax_params* prm, const tn_nodes_view* seeds, int32_t device,
+                   tn_plan** out);
+int tn_plan_run(tn_plan* plan, tn_relax_stats* st);
+int tn_plan_download(tn_plan* plan, tn_nodes_view* nd);
+int tn_plan_destroy(tn_plan* plan);
+
+/* The hash geometry of a plan (V2mHash, src/opencl/v2m_particle_body.cl:21-27;
+ * built at src/v2m_particles.cpp:738-761): b0, origin, levels, dims, offsets. */
+typedef struct tn_hash_info {
+    float b0, ox, oy, oz;
+    int32_t nlev;
+    int32_t dim[TN_MAXLEV][3];
+    int32_t off[TN_MAXLEV];
+    int32_t nkeys;
+} tn_hash_info;
+
+int tn_plan_hash_info(const tn_plan* plan, tn_hash_info* hi);
+
+/*
+ * Stages of one relaxation, for checking each against the CPU path. They act
+ * on the plan's current device state (the seeds after tn_plan_create or
+ * tn_plan_reset) and copy results to host arrays (any pointer may be null).
+ *   tn_plan_rebuild: keys, the stable sort, cell starts, the K-nearest lists
+ *     (the rebuild lambda, src/v2m_particles.cpp:771-820). key[n], sorted[n],
+ *     cstart[nkeys+1], nbr[n*TN_K], nnb[n].
+ *   tn_plan_force: the spring forces for the current lists (v2m_force,
+ *     src/opencl/v2m_particle_body.cl:161-184). F[4n].
+ *   tn_plan_step: `iters` whole iterations of the loop (src/v2m_particles.cpp:828-947),
+ *     rebuilds included; iteration numbering continues across calls.
+ */
+int tn_plan_reset(tn_plan* plan);
+int tn_plan_rebuild(tn_plan* plan, int32_t* key, int32_t* sorted, int32_t* cstart, int32_t* nbr, int32_t* nnb);
+int tn_plan_force(tn_plan* plan, float* F);
+int tn_plan_step(tn_plan* plan, int32_t iters, tn_relax_stats* st);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRUSSNET_AMD_H */

@@ -1,0 +1,1166 @@
+/*
+ * oracle.c -- CPU restatement of v2mesh's relaxation (see oracle.h).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker of the HIP path, never linked by it.
+ *
+ * Each function names the reference lines it restates. The arithmetic keeps
+ * the reference's association and operand order exactly (the results must be
+ * bit-identical), compiled with -ffp-contract=off like the reference
+ * (CMakeLists.txt:69-72). Only label volumes (gm == 0) with interface trapping
+ * are covered; gray-scale and shape fields and the voxel trap are refused.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static _Thread_local char g_err[256];
+
+const char* orc_last_error(void) {
+    return g_err;
+}
+
+static int fail(const char* msg) {
+    snprintf(g_err, sizeof g_err, "%s", msg);
+    return 1;
+}
+
+/* ---- the field ------------------------------------------------------------------ */
+
+typedef struct {
+    int nx, ny, nz, nbx, nby;
+    float vx, vy, vz;
+    const uint16_t* L;
+    const int32_t* bl_cnt;
+    const uint16_t* bl_lab;
+    const int32_t* bl_slot;
+    const float* phi;
+    const float* h;
+} Fld;
+
+static Fld fld_of(const tn_grid_view* g) {
+    Fld f;
+    f.nx = g->nx;
+    f.ny = g->ny;
+    f.nz = g->nz;
+    f.nbx = g->nbx;
+    f.nby = g->nby;
+    f.vx = g->vs[0];
+    f.vy = g->vs[1];
+    f.vz = g->vs[2];
+    f.L = g->L;
+    f.bl_cnt = g->bl_cnt;
+    f.bl_lab = g->bl_lab;
+    f.bl_slot = g->bl_slot;
+    f.phi = g->phi;
+    f.h = g->h;
+    return f;
+}
+
+static inline size_t vox(const Fld* f, int i, int j, int k) {
+    return (size_t)i + (size_t)f->nx * ((size_t)j + (size_t)f->ny * (size_t)k);
+}
+
+/* v2m_brick_of, src/opencl/v2m_grid_body.cl:116-118 */
+static inline int brick_at(const Fld* f, int i, int j, int k) {
+    return (i >> 3) + f->nbx * ((j >> 3) + f->nby * (k >> 3));
+}
+
+/* v2m_slot_of, src/opencl/v2m_grid_body.cl:122-134: the slot of label l in brick b
+ * (-1 if the brick is uniform or lacks l); *has = l is one of the brick's labels */
+static inline int slot_in(const Fld* f, int b, int l, int* has) {
+    const int cnt = f->bl_cnt[b];
+    *has = 0;
+    for (int s = 0; s < cnt && s < TN_BL; ++s) {
+        if (f->bl_lab[(size_t)b * TN_BL + s] == l) {
+            *has = 1;
+            return f->bl_slot[b] < 0 ? -1 : f->bl_slot[b] + s;
+        }
+    }
+    return -1;
+}
+
+/* v2m_phi_vox, src/opencl/v2m_grid_body.cl:137-153 */
+static float phi_voxel(const Fld* f, int l, int i, int j, int k) {
+    if (i < 0 || j < 0 || k < 0 || i >= f->nx || j >= f->ny || k >= f->nz) {
+        return l == 0 ? 1.0f : 0.0f;
+    }
+    int has;
+    const int s = slot_in(f, brick_at(f, i, j, k), l, &has);
+    if (s < 0) {
+        return (has && f->L[vox(f, i, j, k)] == l) ? 1.0f : 0.0f;
+    }
+    return f->phi[(size_t)s * TN_SLOT + (size_t)((i & 7) + TN_BS * ((j & 7) + TN_BS * (k & 7)))];
+}
+
+/* v2m_phi_cell, src/opencl/v2m_grid_body.cl:159-186: the 8 corners of the cell at
+ * (i0,j0,k0), corner bit 0 -> +i, bit 1 -> +j, bit 2 -> +k */
+static void phi_corners(const Fld* f, int l, int i0, int j0, int k0, float* c) {
+    if (i0 >= 0 && j0 >= 0 && k0 >= 0 && i0 + 1 < f->nx && j0 + 1 < f->ny && k0 + 1 < f->nz &&
+        (i0 >> 3) == ((i0 + 1) >> 3) && (j0 >> 3) == ((j0 + 1) >> 3) && (k0 >> 3) == ((k0 + 1) >> 3)) {
+        int has;
+        const int s = slot_in(f, brick_at(f, i0, j0, k0), l, &has);
+        if (s >= 0) {
+            const float* base = f->phi + (size_t)s * TN_SLOT + (size_t)((i0 & 7) + TN_BS * ((j0 & 7) + TN_BS * (k0 & 7)));
+            for (int m = 0; m < 8; ++m) {
+                c[m] = base[(m & 1) + TN_BS * ((m >> 1) & 1) + TN_BS * TN_BS * (m >> 2)];
+            }
+        } else {
+            for (int m = 0; m < 8; ++m) {
+                c[m] = (has && f->L[vox(f, i0 + (m & 1), j0 + ((m >> 1) & 1), k0 + (m >> 2))] == l) ? 1.0f : 0.0f;
+            }
+        }
+        return;
+    }
+    for (int m = 0; m < 8; ++m) {
+        c[m] = phi_voxel(f, l, i0 + (m & 1), j0 + ((m >> 1) & 1), k0 + ((m >> 2) & 1));
+    }
+}
+
+/* trilinear interpolation of 8 corner values (the shared tail of v2m_phi_at,
+ * src/opencl/v2m_grid_body.cl:196-199, and v2m_psi, src/opencl/v2m_seed_body.cl:201-204) */
+static inline float trilerp(const float* c, float fx, float fy, float fz) {
+    const float x00 = c[0] + fx * (c[1] - c[0]), x10 = c[2] + fx * (c[3] - c[2]);
+    const float x01 = c[4] + fx * (c[5] - c[4]), x11 = c[6] + fx * (c[7] - c[6]);
+    const float y0 = x00 + fy * (x10 - x00), y1 = x01 + fy * (x11 - x01);
+    return y0 + fz * (y1 - y0);
+}
+
+/* v2m_phi_at, src/opencl/v2m_grid_body.cl:189-200 (v2m_phi_f with gm == 0,
+ * src/opencl/v2m_seed_body.cl:33-39) */
+static float phi_point(const Fld* f, int l, float px, float py, float pz) {
+    const float u = px / f->vx, v = py / f->vy, w = pz / f->vz;
+    const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
+    float c[8];
+    phi_corners(f, l, i0, j0, k0, c);
+    return trilerp(c, u - (float)i0, v - (float)j0, w - (float)k0);
+}
+
+/* v2m_psi (gm == 0), src/opencl/v2m_seed_body.cl:179-205: phi_a - phi_b, trilinear */
+static float psi_point(const Fld* f, int a, int b, float px, float py, float pz) {
+    const float u = px / f->vx, v = py / f->vy, w = pz / f->vz;
+    const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
+    float c[8], e[8];
+    phi_corners(f, a, i0, j0, k0, c);
+    phi_corners(f, b, i0, j0, k0, e);
+    for (int m = 0; m < 8; ++m) {
+        c[m] -= e[m];
+    }
+    return trilerp(c, u - (float)i0, v - (float)j0, w - (float)k0);
+}
+
+/* v2m_psi_grad (gm == 0), src/opencl/v2m_seed_body.cl:211-247: psi and its analytic
+ * gradient from one trilinear cell */
+static float psi_grad_point(const Fld* f, int a, int b, float px, float py, float pz, float* g) {
+    const float u = px / f->vx, v = py / f->vy, w = pz / f->vz;
+    const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
+    const float fx = u - (float)i0, fy = v - (float)j0, fz = w - (float)k0;
+    float c[8], e[8];
+    phi_corners(f, a, i0, j0, k0, c);
+    phi_corners(f, b, i0, j0, k0, e);
+    for (int m = 0; m < 8; ++m) {
+        c[m] -= e[m];
+    }
+    const float x00 = c[0] + fx * (c[1] - c[0]), x10 = c[2] + fx * (c[3] - c[2]);
+    const float x01 = c[4] + fx * (c[5] - c[4]), x11 = c[6] + fx * (c[7] - c[6]);
+    const float y0 = x00 + fy * (x10 - x00), y1 = x01 + fy * (x11 - x01);
+    const float dx0 = (c[1] - c[0]) + fy * ((c[3] - c[2]) - (c[1] - c[0]));
+    const float dx1 = (c[5] - c[4]) + fy * ((c[7] - c[6]) - (c[5] - c[4]));
+    g[0] = (dx0 + fz * (dx1 - dx0)) / f->vx;
+    g[1] = ((x10 - x00) + fz * ((x11 - x01) - (x10 - x00))) / f->vy;
+    g[2] = (y1 - y0) / f->vz;
+    return y0 + fz * (y1 - y0);
+}
+
+/* v2m_labels_near, src/opencl/v2m_seed_body.cl:42-59: the labels of p's brick */
+static int brick_labels_at(const Fld* f, float px, float py, float pz, int* lab) {
+    const int i = (int)floorf(px / f->vx + 0.5f), j = (int)floorf(py / f->vy + 0.5f), k = (int)floorf(pz / f->vz + 0.5f);
+    if (i < 0 || j < 0 || k < 0 || i >= f->nx || j >= f->ny || k >= f->nz) {
+        lab[0] = 0;
+        return 1;
+    }
+    const int b = brick_at(f, i, j, k);
+    const int n = f->bl_cnt[b] < TN_BL ? f->bl_cnt[b] : TN_BL;
+    for (int s = 0; s < n; ++s) {
+        lab[s] = f->bl_lab[(size_t)b * TN_BL + s];
+    }
+    return n;
+}
+
+/* v2m_label_of (gm == 0, voxmode 0), src/opencl/v2m_seed_body.cl:116-175: argmax of
+ * phi over the brick's labels, plus the runner-up and the margin */
+static int owner_at(const Fld* f, float px, float py, float pz, int* second, float* margin) {
+    int lab[TN_BL];
+    const int n = brick_labels_at(f, px, py, pz, lab);
+    int best = lab[0], sec = TN_NOLAB;
+    float pb = -1.0f, ps = pb;
+    if (n == 1) {
+        *second = TN_NOLAB;
+        *margin = 1.0f;
+        return best;
+    }
+    for (int s = 0; s < n; ++s) {
+        const float v = phi_point(f, lab[s], px, py, pz);
+        if (v > pb) {
+            ps = pb;
+            sec = best;
+            pb = v;
+            best = lab[s];
+        } else if (v > ps) {
+            ps = v;
+            sec = lab[s];
+        }
+    }
+    *second = sec;
+    *margin = pb - ps;
+    return best;
+}
+
+/* v2m_bisect_dir, src/opencl/v2m_seed_body.cl:255-320: bracket psi_ab = 0 along
+ * direction u (doubling probes from the Newton guess tg), then Illinois regula falsi */
+static int bracket_root(const Fld* f, int a, int b, float* p, const float* u, float v0, float rad, float tg) {
+    if (v0 == 0.0f) {
+        return 1;
+    }
+    const float sg = v0 > 0.0f ? -1.0f : 1.0f;
+    float t0 = 0.0f, t1 = -1.0f, f0 = v0, f1 = 0.0f;
+    float t = (tg > 0.0f && tg < rad) ? tg : 0.25f * rad;
+    for (int k = 0; k < 8; ++k) {
+        const float v = psi_point(f, a, b, p[0] + sg * t * u[0], p[1] + sg * t * u[1], p[2] + sg * t * u[2]);
+        if ((v > 0.0f) != (v0 > 0.0f) || v == 0.0f) {
+            t1 = t;
+            f1 = v;
+            break;
+        }
+        t0 = t;
+        f0 = v;
+        if (t >= rad) {
+            break;
+        }
+        t = fminf(2.0f * t, rad);
+    }
+    if (t1 < 0.0f) {
+        return 0;
+    }
+    int side = 0;
+    for (int it = 0; it < 8 && f1 != 0.0f && fabsf(t1 - t0) > 1e-4f * rad; ++it) {
+        const float tm = (f0 != f1) ? t1 - f1 * (t1 - t0) / (f1 - f0) : 0.5f * (t0 + t1);
+        const float v = psi_point(f, a, b, p[0] + sg * tm * u[0], p[1] + sg * tm * u[1], p[2] + sg * tm * u[2]);
+        if ((v > 0.0f) == (f1 > 0.0f)) {
+            t1 = tm;
+            f1 = v;
+            if (side == -1) {
+                f0 *= 0.5f;
+            }
+            side = -1;
+        } else {
+            t0 = t1;
+            f0 = f1;
+            t1 = tm;
+            f1 = v;
+            side = 1;
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        p[k] += sg * t1 * u[k];
+    }
+    return 1;
+}
+
+/* v2m_project1, src/opencl/v2m_seed_body.cl:324-335 */
+static int onto_interface(const Fld* f, int a, int b, float* p, float rad) {
+    float g[3];
+    const float v = psi_grad_point(f, a, b, p[0], p[1], p[2], g);
+    const float gn = sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    if (gn < 1e-9f) {
+        return 0;
+    }
+    const float u[3] = { g[0] / gn, g[1] / gn, g[2] / gn };
+    return bracket_root(f, a, b, p, u, v, rad, 1.5f * fabsf(v) / gn);
+}
+
+/* v2m_project2 (gm == 0), src/opencl/v2m_seed_body.cl:342-420: alternate projections
+ * onto a|b along n_ab and onto a|c along n_ac's part orthogonal to n_ab */
+static int onto_junction(const Fld* f, int a, int b, int c, float* p, float rad) {
+    float q[3] = { p[0], p[1], p[2] };
+    for (int it = 0; it < 3; ++it) {
+        float g1[3], g2[3];
+        if (!onto_interface(f, a, b, q, rad)) {
+            return 0;
+        }
+        psi_grad_point(f, a, b, q[0], q[1], q[2], g1);
+        const float v2 = psi_grad_point(f, a, c, q[0], q[1], q[2], g2);
+        const float a11 = g1[0] * g1[0] + g1[1] * g1[1] + g1[2] * g1[2];
+        if (a11 < 1e-18f) {
+            return 0;
+        }
+        const float pr = (g1[0] * g2[0] + g1[1] * g2[1] + g1[2] * g2[2]) / a11;
+        float u[3] = { g2[0] - pr * g1[0], g2[1] - pr * g1[1], g2[2] - pr * g1[2] };
+        const float un = sqrtf(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        if (un < 1e-9f) {
+            return 0;
+        }
+        u[0] /= un;
+        u[1] /= un;
+        u[2] /= un;
+        if (!bracket_root(f, a, c, q, u, v2, rad, 1.5f * fabsf(v2) / un)) {
+            return 0;
+        }
+    }
+    if (!onto_interface(f, a, b, q, rad)) {
+        return 0;
+    }
+    const float m2 = (q[0] - p[0]) * (q[0] - p[0]) + (q[1] - p[1]) * (q[1] - p[1]) + (q[2] - p[2]) * (q[2] - p[2]);
+    if (m2 > rad * rad) {
+        return 0;
+    }
+    p[0] = q[0];
+    p[1] = q[1];
+    p[2] = q[2];
+    return 1;
+}
+
+/* ---- per-node bodies --------------------------------------------------------------- */
+
+typedef struct {
+    float b0;
+    int nlev;
+    float ox, oy, oz;
+    int dim[TN_MAXLEV][3];
+    int off[TN_MAXLEV];
+    int nkeys;
+} Hash;
+
+/* v2m_h_at, src/opencl/v2m_particle_body.cl:30-36: h at the nearest voxel, clamped */
+static inline float size_at(const Fld* f, float px, float py, float pz) {
+    int i = (int)floorf(px / f->vx + 0.5f), j = (int)floorf(py / f->vy + 0.5f), k = (int)floorf(pz / f->vz + 0.5f);
+    i = i < 0 ? 0 : (i >= f->nx ? f->nx - 1 : i);
+    j = j < 0 ? 0 : (j >= f->ny ? f->ny - 1 : j);
+    k = k < 0 ? 0 : (k >= f->nz ? f->nz - 1 : k);
+    return f->h[vox(f, i, j, k)];
+}
+
+/* v2m_level_of, src/opencl/v2m_particle_body.cl:38-46 */
+static inline int level_for(const Hash* H, float R) {
+    int L = 0;
+    while (L + 1 < H->nlev && H->b0 * (float)(1 << L) < R) {
+        ++L;
+    }
+    return L;
+}
+
+/* v2m_bin_coord, src/opencl/v2m_particle_body.cl:48-52 */
+static inline int cell_coord(const Hash* H, int L, float x, float o, int ax) {
+    const int c = (int)floorf((x - o) / (H->b0 * (float)(1 << L)));
+    const int n = H->dim[L][ax];
+    return c < 0 ? 0 : (c >= n ? n - 1 : c);
+}
+
+/* v2m_bin_key, src/opencl/v2m_particle_body.cl:54-58 */
+static inline int cell_key(const Hash* H, int L, float px, float py, float pz) {
+    const int cx = cell_coord(H, L, px, H->ox, 0), cy = cell_coord(H, L, py, H->oy, 1), cz = cell_coord(H, L, pz, H->oz, 2);
+    return H->off[L] + cx + H->dim[L][0] * (cy + H->dim[L][1] * cz);
+}
+
+/* the hash set-up of relax_cpu, src/v2m_particles.cpp:738-761 */
+static void make_hash(const tn_grid_view* g, const tn_relax_params* prm, Hash* H) {
+    memset(H, 0, sizeof *H);
+    H->b0 = (prm->t + prm->skin) * g->hmin;
+    H->ox = -0.5f * g->vs[0];
+    H->oy = -0.5f * g->vs[1];
+    H->oz = -0.5f * g->vs[2];
+    const float ext[3] = { (float)g->nx * g->vs[0], (float)g->ny * g->vs[1], (float)g->nz * g->vs[2] };
+    H->nlev = 1;
+    while (H->nlev < TN_MAXLEV && H->b0 * (float)(1 << (H->nlev - 1)) < (prm->t + prm->skin) * g->hmax) {
+        ++H->nlev;
+    }
+    int nkeys = 0;
+    for (int L = 0; L < H->nlev; ++L) {
+        H->off[L] = nkeys;
+        const float bL = H->b0 * (float)(1 << L);
+        for (int a = 0; a < 3; ++a) {
+            const int d = (int)ceilf(ext[a] / bL);
+            H->dim[L][a] = d > 1 ? d : 1;
+        }
+        nkeys += H->dim[L][0] * H->dim[L][1] * H->dim[L][2];
+    }
+    H->nkeys = nkeys;
+}
+
+void orc_hash_geometry(const tn_grid_view* g, const tn_relax_params* prm, tn_hash_info* out) {
+    Hash H;
+    make_hash(g, prm, &H);
+    memset(out, 0, sizeof *out);
+    out->b0 = H.b0;
+    out->ox = H.ox;
+    out->oy = H.oy;
+    out->oz = H.oz;
+    out->nlev = H.nlev;
+    memcpy(out->dim, H.dim, sizeof H.dim);
+    memcpy(out->off, H.off, sizeof H.off);
+    out->nkeys = H.nkeys;
+}
+
+typedef struct {
+    const float* hn;
+    const float* P;
+    const uint16_t* lab;
+    const uint8_t* typ;
+    const int32_t* cstart;
+    const int32_t* sorted;
+    const float* Ps;
+    float t, skin;
+} NbrIn;
+
+/* v2m_neighbors, src/opencl/v2m_particle_body.cl:73-151: the K nearest valid
+ * neighbours of node i in (distance, id) order, scanning levels Li-1..Li+1 */
+static void truss_of(const Hash* H, const NbrIn* in, int i, int32_t* nbr, int32_t* nnb) {
+    float ds[TN_K];
+    int ids[TN_K];
+    const float px = in->P[3 * i], py = in->P[3 * i + 1], pz = in->P[3 * i + 2];
+    const float hi = in->hn[i];
+    const float Ri = (in->t + in->skin) * hi;
+    const int ti_int = in->typ[i] == TN_INTERIOR;
+    const int li = in->lab[i];
+    const int Li = level_for(H, Ri);
+    int n = 0;
+    for (int L = Li - 1; L <= Li + 1; ++L) {
+        if (L < 0 || L >= H->nlev) {
+            continue;
+        }
+        const float bL = H->b0 * (float)(1 << L);
+        const float r = 0.5f * (Ri + bL);
+        const int x0 = cell_coord(H, L, px - r, H->ox, 0), x1 = cell_coord(H, L, px + r, H->ox, 0);
+        const int y0 = cell_coord(H, L, py - r, H->oy, 1), y1 = cell_coord(H, L, py + r, H->oy, 1);
+        const int z0 = cell_coord(H, L, pz - r, H->oz, 2), z1 = cell_coord(H, L, pz + r, H->oz, 2);
+        for (int cz = z0; cz <= z1; ++cz) {
+            for (int cy = y0; cy <= y1; ++cy) {
+                for (int cx = x0; cx <= x1; ++cx) {
+                    const int key = H->off[L] + cx + H->dim[L][0] * (cy + H->dim[L][1] * cz);
+                    for (int s = in->cstart[key]; s < in->cstart[key + 1]; ++s) {
+                        const int j = in->sorted[s];
+                        if (j == i) {
+                            continue;
+                        }
+                        const float ex = in->Ps[4 * (size_t)s] - px, ey = in->Ps[4 * (size_t)s + 1] - py,
+                                    ez = in->Ps[4 * (size_t)s + 2] - pz;
+                        const float d2 = ex * ex + ey * ey + ez * ez;
+                        const float hj = in->Ps[4 * (size_t)s + 3];
+                        const float rj = (in->t + in->skin) * 0.5f * (hi + hj);
+                        if (d2 >= rj * rj) {
+                            continue;
+                        }
+                        if (ti_int && in->typ[j] == TN_INTERIOR && in->lab[j] != li) {
+                            continue;
+                        }
+                        const float dist = sqrtf(d2);
+                        if (n == TN_K && (dist > ds[TN_K - 1] || (dist == ds[TN_K - 1] && j > ids[TN_K - 1]))) {
+                            continue;
+                        }
+                        int k = n < TN_K ? n++ : TN_K - 1;
+                        while (k > 0 && (ds[k - 1] > dist || (ds[k - 1] == dist && ids[k - 1] > j))) {
+                            ds[k] = ds[k - 1];
+                            ids[k] = ids[k - 1];
+                            --k;
+                        }
+                        ds[k] = dist;
+                        ids[k] = j;
+                    }
+                }
+            }
+        }
+    }
+    nnb[i] = n;
+    for (int k = 0; k < n; ++k) {
+        nbr[(size_t)i * TN_K + k] = ids[k];
+    }
+}
+
+/* v2m_force, src/opencl/v2m_particle_body.cl:161-184: repulsive bars in list order */
+static void bar_force(const float* hn, const float* P, const uint8_t* typ, const int32_t* nbr, const int32_t* nnb,
+                      float fscale, float fsurf, int i, float* F) {
+    const float px = P[3 * i], py = P[3 * i + 1], pz = P[3 * i + 2];
+    const float hi = hn[i];
+    F[0] = F[1] = F[2] = F[3] = 0.0f;
+    for (int k = 0; k < nnb[i]; ++k) {
+        const int j = nbr[(size_t)i * TN_K + k];
+        const float ex = px - P[3 * j], ey = py - P[3 * j + 1], ez = pz - P[3 * j + 2];
+        const float l = sqrtf(ex * ex + ey * ey + ez * ez);
+        const float hj = hn[j];
+        const float l0 = ((typ[i] != TN_INTERIOR && typ[j] != TN_INTERIOR) ? fsurf : fscale) * 0.5f * (hi + hj);
+        if (l >= l0 || l <= 0.0f) {
+            continue;
+        }
+        const float fr = (l0 - l) / l;
+        F[0] += fr * ex;
+        F[1] += fr * ey;
+        F[2] += fr * ez;
+        F[3] += 1.0f;
+    }
+}
+
+/* v2m_crossing, src/opencl/v2m_particle_body.cl:188-212 */
+static void segment_crossing(const Fld* f, int a, int b, const float* p, const float* q, float* c) {
+    float t0 = 0.0f, t1 = 1.0f;
+    float v0 = psi_point(f, a, b, p[0], p[1], p[2]);
+    float v1 = psi_point(f, a, b, q[0], q[1], q[2]);
+    for (int it = 0; it < 6; ++it) {
+        const float tm = 0.5f * (t0 + t1);
+        const float vm = psi_point(f, a, b, p[0] + tm * (q[0] - p[0]), p[1] + tm * (q[1] - p[1]), p[2] + tm * (q[2] - p[2]));
+        if (vm > 0.0f) {
+            t0 = tm;
+            v0 = vm;
+        } else {
+            t1 = tm;
+            v1 = vm;
+        }
+    }
+    const float t = (v0 - v1) != 0.0f ? t0 + (t1 - t0) * v0 / (v0 - v1) : 0.5f * (t0 + t1);
+    for (int k = 0; k < 3; ++k) {
+        c[k] = p[k] + t * (q[k] - p[k]);
+    }
+}
+
+/* v2m_tangential, src/opencl/v2m_particle_body.cl:215-229 */
+static void drop_normal(const Fld* f, int a, int b, const float* p, float* v) {
+    float g[3];
+    psi_grad_point(f, a, b, p[0], p[1], p[2], g);
+    const float gn2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+    if (gn2 < 1e-12f) {
+        return;
+    }
+    const float vn = (v[0] * g[0] + v[1] * g[1] + v[2] * g[2]) / gn2;
+    for (int k = 0; k < 3; ++k) {
+        v[k] -= vn * g[k];
+    }
+}
+
+/* v2m_third_label, src/opencl/v2m_particle_body.cl:232-260 */
+static int competing_label(const Fld* f, int a, int b, const float* q) {
+    int nl[TN_BL];
+    const int n = brick_labels_at(f, q[0], q[1], q[2], nl);
+    int other = 0;
+    for (int s = 0; s < n; ++s) {
+        other |= nl[s] != a && nl[s] != b;
+    }
+    if (!other) {
+        return TN_NOLAB;
+    }
+    const float pa = phi_point(f, a, q[0], q[1], q[2]);
+    int c = TN_NOLAB;
+    float pc = 0.5f * pa;
+    for (int s = 0; s < n; ++s) {
+        if (nl[s] != a && nl[s] != b) {
+            const float w = phi_point(f, nl[s], q[0], q[1], q[2]);
+            if (w > pc) {
+                pc = w;
+                c = nl[s];
+            }
+        }
+    }
+    return c;
+}
+
+/* v2m_valid_on, src/opencl/v2m_particle_body.cl:266-291 */
+static int labels_dominate(const Fld* f, int a, int b, int c, const float* q) {
+    int nl[TN_BL];
+    const int n = brick_labels_at(f, q[0], q[1], q[2], nl);
+    int other = 0;
+    for (int s = 0; s < n; ++s) {
+        other |= nl[s] != a && nl[s] != b && nl[s] != c;
+    }
+    if (!other) {
+        return 1;
+    }
+    float mine = fminf(phi_point(f, a, q[0], q[1], q[2]), phi_point(f, b, q[0], q[1], q[2]));
+    if (c != TN_NOLAB) {
+        mine = fminf(mine, phi_point(f, c, q[0], q[1], q[2]));
+    }
+    for (int s = 0; s < n; ++s) {
+        if (nl[s] != a && nl[s] != b && nl[s] != c && phi_point(f, nl[s], q[0], q[1], q[2]) > mine + 0.02f) {
+            return 0;
+        }
+    }
+    return 1;
+}
+
+typedef struct {
+    float* P;
+    const uint16_t* lab;
+    uint8_t* typ;
+    uint16_t* part;
+    float* hn;
+} NodeIO;
+
+static inline float dist2(const float* q, const float* p) {
+    return (q[0] - p[0]) * (q[0] - p[0]) + (q[1] - p[1]) * (q[1] - p[1]) + (q[2] - p[2]) * (q[2] - p[2]);
+}
+
+/* v2m_move_step (smooth mode), src/opencl/v2m_particle_body.cl:659-833: apply step s to
+ * node i with interface trapping; returns |dp| / h (0 = stayed) */
+static float trapped_step(const Fld* f, float h, float* s, float maxstep, float snap, int i, NodeIO* io) {
+    float p[3], q[3];
+    p[0] = io->P[3 * i];
+    p[1] = io->P[3 * i + 1];
+    p[2] = io->P[3 * i + 2];
+    const int a = io->lab[i];
+    int ty = io->typ[i];
+    int b = io->part[2 * i], c = io->part[2 * i + 1];
+    uint16_t* part = io->part;
+    uint8_t* typ = io->typ;
+
+    if (ty == TN_INTERIOR) {
+        for (int k = 0; k < 3; ++k) {
+            q[k] = p[k] + s[k];
+        }
+        int sec;
+        float mg;
+        const int l = owner_at(f, q[0], q[1], q[2], &sec, &mg);
+        if (l != a) {
+            b = l;
+            float x[3], r[3];
+            segment_crossing(f, a, b, p, q, x);
+            for (int k = 0; k < 3; ++k) {
+                r[k] = q[k] - x[k];
+            }
+            drop_normal(f, a, b, x, r);
+            for (int k = 0; k < 3; ++k) {
+                q[k] = x[k] + r[k];
+            }
+            if (!onto_interface(f, a, b, q, 0.5f * h)) {
+                q[0] = x[0];
+                q[1] = x[1];
+                q[2] = x[2];
+            }
+            ty = TN_INTERFACE;
+            part[2 * i] = (uint16_t)b;
+        } else if (sec != TN_NOLAB && snap > 0.0f) {
+            float gq[3];
+            const float v = psi_grad_point(f, a, sec, q[0], q[1], q[2], gq);
+            const float gn = sqrtf(gq[0] * gq[0] + gq[1] * gq[1] + gq[2] * gq[2]);
+            if (gn > 1e-9f && v / gn < snap * h && onto_interface(f, a, sec, q, 1.2f * snap * h)) {
+                ty = TN_INTERFACE;
+                b = sec;
+                part[2 * i] = (uint16_t)sec;
+            }
+        }
+    } else if (ty == TN_INTERFACE) {
+        drop_normal(f, a, b, p, s);
+        for (int k = 0; k < 3; ++k) {
+            q[k] = p[k] + s[k];
+        }
+        if (!onto_interface(f, a, b, q, 0.5f * h)) {
+            return 0.0f;
+        }
+    } else {
+        float g1[3], g2[3], tdir[3];
+        psi_grad_point(f, a, b, p[0], p[1], p[2], g1);
+        psi_grad_point(f, a, c, p[0], p[1], p[2], g2);
+        tdir[0] = g1[1] * g2[2] - g1[2] * g2[1];
+        tdir[1] = g1[2] * g2[0] - g1[0] * g2[2];
+        tdir[2] = g1[0] * g2[1] - g1[1] * g2[0];
+        const float tl2 = tdir[0] * tdir[0] + tdir[1] * tdir[1] + tdir[2] * tdir[2];
+        const float st = tl2 > 1e-20f ? (s[0] * tdir[0] + s[1] * tdir[1] + s[2] * tdir[2]) / tl2 : 0.0f;
+        for (int k = 0; k < 3; ++k) {
+            q[k] = p[k] + st * tdir[k];
+        }
+        if (!onto_junction(f, a, b, c, q, 0.5f * h)) {
+            return 0.0f;
+        }
+    }
+
+    if (ty == TN_INTERFACE) {
+        const int cc = competing_label(f, a, b, q);
+        if (cc != TN_NOLAB) {
+            float r[3] = { q[0], q[1], q[2] };
+            if (onto_junction(f, a, b, cc, r, 0.5f * h)) {
+                q[0] = r[0];
+                q[1] = r[1];
+                q[2] = r[2];
+                ty = TN_JUNCTION;
+                part[2 * i + 1] = (uint16_t)cc;
+            }
+        }
+    }
+
+    {
+        const float lim = (maxstep + fmaxf(snap, 0.5f)) * h;
+        const float m2 = dist2(q, p);
+        if (!(m2 <= lim * lim)) {
+            if (typ[i] == TN_INTERIOR) {
+                part[2 * i] = TN_NOLAB;
+            }
+            return 0.0f;
+        }
+    }
+
+    if (ty != TN_INTERIOR &&
+        !labels_dominate(f, a, part[2 * i], ty >= TN_JUNCTION ? part[2 * i + 1] : TN_NOLAB, q)) {
+        if (ty == TN_INTERFACE || typ[i] == TN_INTERIOR) {
+            const int cc = competing_label(f, a, part[2 * i], q);
+            float r[3] = { q[0], q[1], q[2] };
+            if (cc != TN_NOLAB && onto_junction(f, a, part[2 * i], cc, r, 0.5f * h) &&
+                labels_dominate(f, a, part[2 * i], cc, r)) {
+                q[0] = r[0];
+                q[1] = r[1];
+                q[2] = r[2];
+                ty = TN_JUNCTION;
+                part[2 * i + 1] = (uint16_t)cc;
+            } else {
+                typ[i] = (uint8_t)(typ[i] == TN_INTERIOR ? TN_INTERIOR : ty);
+                if (typ[i] == TN_INTERIOR) {
+                    part[2 * i] = TN_NOLAB;
+                }
+                return 0.0f;
+            }
+        } else {
+            return 0.0f;
+        }
+    }
+
+    const float mv = sqrtf(dist2(q, p));
+    io->P[3 * i] = q[0];
+    io->P[3 * i + 1] = q[1];
+    io->P[3 * i + 2] = q[2];
+    typ[i] = (uint8_t)ty;
+    io->hn[i] = size_at(f, q[0], q[1], q[2]);
+    return mv / h;
+}
+
+/* v2m_move, src/opencl/v2m_particle_body.cl:844-935: the step from the force (FIRE or
+ * Jacobi), the settle and cap tests, the trapped step, and FIRE's velocity */
+static float node_move(const Fld* f, const float* F, float dt, float maxstep, float snap, int i, NodeIO* io,
+                       int fire, float* V, float fdt, float falpha, float* pw) {
+    float s[3];
+    const float p0 = io->P[3 * i], p1 = io->P[3 * i + 1], p2 = io->P[3 * i + 2];
+    const float h = size_at(f, p0, p1, p2);
+    const int ty0 = io->typ[i];
+    if (fire) {
+        pw[i] = 0.0f;
+    }
+    if (ty0 == TN_CORNER) {
+        if (fire) {
+            V[3 * i] = V[3 * i + 1] = V[3 * i + 2] = 0.0f;
+        }
+        return 0.0f;
+    }
+    const float stiff = F[4 * i + 3] > 1.0f ? F[4 * i + 3] : 1.0f;
+    float ac[3];
+    for (int k = 0; k < 3; ++k) {
+        ac[k] = F[4 * i + k] / stiff;
+    }
+    if (fire) {
+        float v[3] = { V[3 * i], V[3 * i + 1], V[3 * i + 2] };
+        const float pv = ac[0] * v[0] + ac[1] * v[1] + ac[2] * v[2];
+        pw[i] = pv;
+        if (pv < 0.0f) {
+            v[0] = v[1] = v[2] = 0.0f;
+        } else {
+            const float vn = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            const float an = sqrtf(ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2]);
+            if (an > 0.0f) {
+                for (int k = 0; k < 3; ++k) {
+                    v[k] = (1.0f - falpha) * v[k] + falpha * vn * ac[k] / an;
+                }
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            v[k] += fdt * ac[k];
+            s[k] = fdt * v[k];
+        }
+    } else {
+        for (int k = 0; k < 3; ++k) {
+            s[k] = dt * ac[k];
+        }
+    }
+    const float sl = sqrtf(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    if (sl < 5e-4f * h) {
+        if (fire) {
+            V[3 * i] = V[3 * i + 1] = V[3 * i + 2] = 0.0f;
+        }
+        return 0.0f;
+    }
+    if (sl > maxstep * h) {
+        for (int k = 0; k < 3; ++k) {
+            s[k] *= maxstep * h / sl;
+        }
+    }
+    const float m = trapped_step(f, h, s, maxstep, snap, i, io);
+    if (fire) {
+        if (m > 0.0f && io->typ[i] == ty0) {
+            V[3 * i] = (io->P[3 * i] - p0) / fdt;
+            V[3 * i + 1] = (io->P[3 * i + 1] - p1) / fdt;
+            V[3 * i + 2] = (io->P[3 * i + 2] - p2) / fdt;
+        } else {
+            V[3 * i] = V[3 * i + 1] = V[3 * i + 2] = 0.0f;
+        }
+    }
+    return m;
+}
+
+/* ---- FIRE's global part: FireCtl, src/v2m_particles.h:47-76 ------------------------- */
+
+typedef struct {
+    float dt0, dt, dtmax, alpha;
+    int npos, resets;
+} Fire;
+
+static void fire_init(Fire* c, const tn_relax_params* p) {
+    c->dt0 = sqrtf(p->dt);
+    c->dt = c->dt0;
+    c->dtmax = p->fire_dtmax * c->dt0;
+    c->alpha = 0.1f;
+    c->npos = 0;
+    c->resets = 0;
+}
+
+static int fire_may_stop(const Fire* c) {
+    return c->dt >= 0.25f * c->dt0;
+}
+
+/* returns 1 when every velocity must be zeroed */
+static int fire_update(Fire* c, double P) {
+    if (P >= 0.0) {
+        if (++c->npos > 5) {
+            const float up = c->dt * 1.1f;
+            c->dt = up < c->dtmax ? up : c->dtmax;
+            c->alpha *= 0.99f;
+        }
+        return 0;
+    }
+    c->npos = 0;
+    const float down = c->dt * 0.5f, lo = 0.05f * c->dt0;
+    c->dt = down < lo ? lo : down;
+    c->alpha = 0.1f;
+    ++c->resets;
+    return 1;
+}
+
+/* the histogram bin of src/v2m_particles.cpp:881 (std::log2 on a float is log2f) */
+int32_t orc_hist_bin(float m) {
+    if (!(m > 0.0f)) {
+        return 0;
+    }
+    const int b = 20 + (int)floorf(log2f(m));
+    return b < 0 ? 0 : (b > 31 ? 31 : b);
+}
+
+/* ---- the driver: relax_cpu, src/v2m_particles.cpp:728-976 ---------------------------- */
+
+static int supported(const tn_grid_view* g, const tn_relax_params* prm) {
+    if (!g || !prm) {
+        return fail("oracle: null argument");
+    }
+    if (g->gm != 0) {
+        return fail("oracle: only label volumes (gm == 0)");
+    }
+    if (prm->voxel_trap) {
+        return fail("oracle: the voxel trap is not restated");
+    }
+    return 0;
+}
+
+typedef struct {
+    Hash H;
+    int n;
+    int32_t *key, *cstart, *sorted, *nbr, *nnb;
+    float* Ps;
+} Truss;
+
+static void truss_free(Truss* T) {
+    free(T->key);
+    free(T->cstart);
+    free(T->sorted);
+    free(T->nbr);
+    free(T->nnb);
+    free(T->Ps);
+}
+
+static int truss_alloc(Truss* T, int n) {
+    T->n = n;
+    T->key = malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    T->cstart = malloc(sizeof(int32_t) * ((size_t)T->H.nkeys + 1));
+    T->sorted = malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    T->nbr = calloc((size_t)(n > 0 ? n : 1) * TN_K, sizeof(int32_t));
+    T->nnb = malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    T->Ps = malloc(sizeof(float) * 4 * (size_t)(n > 0 ? n : 1));
+    if (!T->key || !T->cstart || !T->sorted || !T->nbr || !T->nnb || !T->Ps) {
+        truss_free(T);
+        return fail("oracle: out of memory");
+    }
+    return 0;
+}
+
+/* the rebuild lambda, src/v2m_particles.cpp:771-820 (without P0 = P) */
+static void truss_rebuild(Truss* T, const Fld* f, const tn_relax_params* prm, const float* P, const uint16_t* lab,
+                          const uint8_t* typ, const float* hn) {
+    const int n = T->n;
+    const Hash* H = &T->H;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        const float h = size_at(f, P[3 * i], P[3 * i + 1], P[3 * i + 2]);
+        T->key[i] = cell_key(H, level_for(H, (prm->t + prm->skin) * h), P[3 * i], P[3 * i + 1], P[3 * i + 2]);
+    }
+    memset(T->cstart, 0, sizeof(int32_t) * ((size_t)H->nkeys + 1));
+    for (int i = 0; i < n; ++i) {
+        ++T->cstart[T->key[i] + 1];
+    }
+    for (int k = 0; k < H->nkeys; ++k) {
+        T->cstart[k + 1] += T->cstart[k];
+    }
+    /* stable scatter: ascending id within a cell (src/v2m_particles.cpp:791-795) */
+    int32_t* cur = malloc(sizeof(int32_t) * ((size_t)H->nkeys + 1));
+    memcpy(cur, T->cstart, sizeof(int32_t) * ((size_t)H->nkeys + 1));
+    for (int i = 0; i < n; ++i) {
+        T->sorted[cur[T->key[i]]++] = i;
+    }
+    free(cur);
+#pragma omp parallel for schedule(static)
+    for (int s = 0; s < n; ++s) {
+        const int j = T->sorted[s];
+        T->Ps[4 * (size_t)s] = P[3 * j];
+        T->Ps[4 * (size_t)s + 1] = P[3 * j + 1];
+        T->Ps[4 * (size_t)s + 2] = P[3 * j + 2];
+        T->Ps[4 * (size_t)s + 3] = hn[j];
+    }
+    NbrIn in = { hn, P, lab, typ, T->cstart, T->sorted, T->Ps, prm->t, prm->skin };
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < n; ++i) {
+        truss_of(H, &in, i, T->nbr, T->nnb);
+    }
+}
+
+int orc_rebuild(const tn_grid_view* g, const tn_relax_params* prm, const tn_nodes_view* nd, int32_t* key,
+                int32_t* sorted, int32_t* cstart, int32_t* nbr, int32_t* nnb, float* hn) {
+    if (supported(g, prm)) {
+        return 1;
+    }
+    const Fld f = fld_of(g);
+    const int n = (int)nd->n;
+    Truss T;
+    make_hash(g, prm, &T.H);
+    if (truss_alloc(&T, n)) {
+        return 1;
+    }
+    for (int i = 0; i < n; ++i) {
+        hn[i] = size_at(&f, nd->P[3 * i], nd->P[3 * i + 1], nd->P[3 * i + 2]);
+    }
+    truss_rebuild(&T, &f, prm, nd->P, nd->lab, nd->typ, hn);
+    if (key) {
+        memcpy(key, T.key, sizeof(int32_t) * (size_t)n);
+    }
+    if (sorted) {
+        memcpy(sorted, T.sorted, sizeof(int32_t) * (size_t)n);
+    }
+    if (cstart) {
+        memcpy(cstart, T.cstart, sizeof(int32_t) * ((size_t)T.H.nkeys + 1));
+    }
+    if (nbr) {
+        memcpy(nbr, T.nbr, sizeof(int32_t) * (size_t)n * TN_K);
+    }
+    if (nnb) {
+        memcpy(nnb, T.nnb, sizeof(int32_t) * (size_t)n);
+    }
+    truss_free(&T);
+    return 0;
+}
+
+void orc_force(const tn_relax_params* prm, const tn_nodes_view* nd, const float* hn, const int32_t* nbr,
+               const int32_t* nnb, float* F) {
+    const int n = (int)nd->n;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        bar_force(hn, nd->P, nd->typ, nbr, nnb, prm->fscale, prm->fsurf, i, &F[4 * (size_t)i]);
+    }
+}
+
+int orc_move(const tn_grid_view* g, const tn_relax_params* prm, tn_nodes_view* nd, float* hn, const float* F,
+             float* V, float fdt, float falpha, float* pw, float* mv) {
+    if (supported(g, prm)) {
+        return 1;
+    }
+    const Fld f = fld_of(g);
+    const int n = (int)nd->n;
+    const int fire = prm->fire && !prm->voxel_trap ? 1 : 0;
+    NodeIO io = { nd->P, nd->lab, nd->typ, nd->part, hn };
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int i = 0; i < n; ++i) {
+        mv[i] = node_move(&f, F, prm->dt, prm->maxstep, prm->snap, i, &io, fire, V, fdt, falpha, pw);
+    }
+    return 0;
+}
+
+int orc_relax(const tn_grid_view* g, const tn_relax_params* prm, tn_nodes_view* nd, tn_relax_stats* st,
+              orc_iter_trace* trace, int32_t nthreads) {
+    if (supported(g, prm) || !nd || !st) {
+        return 1;
+    }
+#ifdef _OPENMP
+    const int nthr0 = omp_get_max_threads();
+    if (nthreads > 0) {
+        omp_set_num_threads(nthreads);
+    }
+#endif
+    const int rebuild_div = prm->rebuild_div > 0 ? prm->rebuild_div : 300;
+    const Fld f = fld_of(g);
+    const int n = (int)nd->n;
+    memset(st, 0, sizeof *st);
+    Truss T;
+    make_hash(g, prm, &T.H);
+    if (truss_alloc(&T, n)) {
+        return 1;
+    }
+    const size_t n1 = (size_t)(n > 0 ? n : 1);
+    float* F = malloc(sizeof(float) * 4 * n1);
+    float* P0 = malloc(sizeof(float) * 3 * n1);
+    float* mv = malloc(sizeof(float) * n1);
+    float* hn = malloc(sizeof(float) * n1);
+    const int fire = prm->fire && !prm->voxel_trap ? 1 : 0;
+    float* V = calloc(fire ? 3 * n1 : 1, sizeof(float));
+    float* pw = calloc(fire ? n1 : 1, sizeof(float));
+    if (!F || !P0 || !mv || !hn || !V || !pw) {
+        free(F), free(P0), free(mv), free(hn), free(V), free(pw);
+        truss_free(&T);
+        return fail("oracle: out of memory");
+    }
+    float* P = nd->P;
+    memcpy(P0, P, sizeof(float) * 3 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        hn[i] = size_at(&f, P[3 * i], P[3 * i + 1], P[3 * i + 2]);
+    }
+    NodeIO io = { nd->P, nd->lab, nd->typ, nd->part, hn };
+
+    /* rebuild: lists, then P0 = P */
+#define REBUILD()                                                          \
+    do {                                                                   \
+        truss_rebuild(&T, &f, prm, P, nd->lab, nd->typ, hn);              \
+        memcpy(P0, P, sizeof(float) * 3 * (size_t)n);                      \
+        ++st->rebuilds;                                                    \
+    } while (0)
+
+    REBUILD();
+    Fire fc;
+    fire_init(&fc, prm);
+    for (int it = 0; it < prm->max_iters; ++it) {
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int i = 0; i < n; ++i) {
+            bar_force(hn, P, nd->typ, T.nbr, T.nnb, prm->fscale, prm->fsurf, i, &F[4 * (size_t)i]);
+        }
+        float mmax = 0.0f;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(max : mmax)
+        for (int i = 0; i < n; ++i) {
+            mv[i] = node_move(&f, F, prm->dt, prm->maxstep, prm->snap, i, &io, fire, V, fc.dt, fc.alpha, pw);
+            mmax = mmax > mv[i] ? mmax : mv[i];
+        }
+        double pt = 0.0;
+        if (fire) {
+            for (int i = 0; i < n; ++i) { /* serial, index order */
+                pt += pw[i];
+            }
+            if (fire_update(&fc, pt)) {
+                memset(V, 0, sizeof(float) * 3 * (size_t)n);
+            }
+        }
+        st->iters = it + 1;
+        st->last_move = mmax;
+        st->fire_resets = fc.resets;
+        st->fire_dt = fire ? fc.dt : 0.0f;
+
+        int hist[32] = { 0 };
+        for (int i = 0; i < n; ++i) {
+            ++hist[orc_hist_bin(mv[i])];
+        }
+        int acc = 0, b99 = 31;
+        for (int b = 0; b < 32; ++b) {
+            acc += hist[b];
+            if (acc >= 0.99 * n) {
+                b99 = b;
+                break;
+            }
+        }
+        const float p99 = ldexpf(1.0f, b99 - 20 + 1);
+        st->last_p99 = p99;
+        orc_iter_trace* tr = trace ? &trace[it] : NULL;
+        if (tr) {
+            memset(tr, 0, sizeof *tr);
+            tr->it = it;
+            tr->mmax = mmax;
+            tr->p99 = p99;
+            tr->fdt = fc.dt;
+            tr->falpha = fc.alpha;
+            tr->power = pt;
+            memcpy(tr->hist, hist, sizeof hist);
+        }
+        if (p99 < prm->dptol && (!fire || fire_may_stop(&fc))) {
+            break;
+        }
+
+        /* Verlet bookkeeping, src/v2m_particles.cpp:917-937 */
+        int nhalf = 0, nref = 0;
+        NbrIn in = { hn, P, nd->lab, nd->typ, T.cstart, T.sorted, T.Ps, prm->t, prm->skin };
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : nhalf, nref)
+        for (int i = 0; i < n; ++i) {
+            const float ex = P[3 * i] - P0[3 * i], ey = P[3 * i + 1] - P0[3 * i + 1], ez = P[3 * i + 2] - P0[3 * i + 2];
+            const float h = size_at(&f, P[3 * i], P[3 * i + 1], P[3 * i + 2]);
+            const float m2 = ex * ex + ey * ey + ez * ez, s2 = prm->skin * prm->skin * h * h;
+            if (m2 > s2) {
+                truss_of(&T.H, &in, i, T.nbr, T.nnb);
+                P0[3 * i] = P[3 * i];
+                P0[3 * i + 1] = P[3 * i + 1];
+                P0[3 * i + 2] = P[3 * i + 2];
+                ++nref;
+            } else {
+                nhalf += m2 > 0.25f * s2;
+            }
+        }
+        const int endgame = it >= prm->max_iters * 4 / 5 || p99 < 4.0f * prm->dptol;
+        const int div = endgame ? (rebuild_div > 1000 ? rebuild_div : 1000) : rebuild_div;
+        if (tr) {
+            tr->nhalf = nhalf;
+            tr->nrefresh = nref;
+        }
+        if (nhalf > n / div) {
+            REBUILD();
+            if (tr) {
+                tr->rebuilt = 1;
+            }
+        }
+    }
+#undef REBUILD
+    st->n_interior = st->n_interface = st->n_junction = st->n_corner = 0;
+    for (int i = 0; i < n; ++i) {
+        const int t = nd->typ[i];
+        st->n_interior += t == TN_INTERIOR;
+        st->n_interface += t == TN_INTERFACE;
+        st->n_junction += t == TN_JUNCTION;
+        st->n_corner += t == TN_CORNER;
+    }
+    free(F), free(P0), free(mv), free(hn), free(V), free(pw);
+    truss_free(&T);
+#ifdef _OPENMP
+    if (nthreads > 0) {
+        omp_set_num_threads(nthr0);
+    }
+#endif
+    return 0;
+}
+
+void orc_field_points(const tn_grid_view* g, int32_t a, int32_t b, int64_t m, const float* X, float* phi_a,
+                      float* psi, float* grad, int32_t* label, int32_t* second) {
+    const Fld f = fld_of(g);
+    for (int64_t k = 0; k < m; ++k) {
+        const float* x = X + 3 * k;
+        phi_a[k] = phi_point(&f, a, x[0], x[1], x[2]);
+        psi[k] = psi_grad_point(&f, a, b, x[0], x[1], x[2], grad + 3 * k);
+        int sec;
+        float mg;
+        label[k] = owner_at(&f, x[0], x[1], x[2], &sec, &mg);
+        second[k] = sec;
+    }
+}

@@ -1,0 +1,124 @@
+"""ctypes loader of oracle/lib/liboracle.so, the CPU restatement of the relaxation.
+
+TEST INFRASTRUCTURE ONLY: the checker the HIP path is compared with. Built by
+``make -C oracle`` from oracle/oracle.c.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from trussnet_amd._abi import K, Grid, GridView, HashInfoC, Nodes, NodesView, RelaxParams, RelaxParamsC, RelaxStatsC
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "liboracle.so")
+
+
+class IterTrace(C.Structure):
+    _fields_ = [("it", C.c_int32), ("rebuilt", C.c_int32), ("nhalf", C.c_int32), ("nrefresh", C.c_int32),
+                ("mmax", C.c_float), ("p99", C.c_float), ("fdt", C.c_float), ("falpha", C.c_float),
+                ("power", C.c_double), ("hist", C.c_int32 * 32)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (make -C oracle)")
+        L = C.CDLL(LIB_PATH)
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_hash_geometry.argtypes = [C.POINTER(GridView), C.POINTER(RelaxParamsC), C.POINTER(HashInfoC)]
+        L.orc_rebuild.argtypes = [C.POINTER(GridView), C.POINTER(RelaxParamsC), C.POINTER(NodesView)] + [C.c_void_p] * 6
+        L.orc_force.argtypes = [C.POINTER(RelaxParamsC), C.POINTER(NodesView)] + [C.c_void_p] * 4
+        L.orc_move.argtypes = [C.POINTER(GridView), C.POINTER(RelaxParamsC), C.POINTER(NodesView), C.c_void_p,
+                               C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_void_p, C.c_void_p]
+        L.orc_relax.argtypes = [C.POINTER(GridView), C.POINTER(RelaxParamsC), C.POINTER(NodesView),
+                                C.POINTER(RelaxStatsC), C.POINTER(IterTrace), C.c_int32]
+        L.orc_field_points.argtypes = [C.POINTER(GridView), C.c_int32, C.c_int32, C.c_int64] + [C.c_void_p] * 6
+        L.orc_hist_bin.argtypes = [C.c_float]
+        L.orc_hist_bin.restype = C.c_int32
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+
+def hash_info(grid: Grid, prm: RelaxParams) -> HashInfoC:
+    hi = HashInfoC()
+    gv, pc = grid.view(), prm.to_c()
+    lib().orc_hash_geometry(C.byref(gv), C.byref(pc), C.byref(hi))
+    return hi
+
+
+def rebuild(grid: Grid, nodes: Nodes, prm: RelaxParams) -> dict:
+    n = nodes.n
+    nk = hash_info(grid, prm).nkeys
+    r = {"key": np.zeros(n, np.int32), "sorted": np.zeros(n, np.int32), "cstart": np.zeros(nk + 1, np.int32),
+         "nbr": np.zeros(n * K, np.int32), "nnb": np.zeros(n, np.int32), "hn": np.zeros(n, np.float32)}
+    gv, pc, nv = grid.view(), prm.to_c(), nodes.view()
+    _check(lib().orc_rebuild(C.byref(gv), C.byref(pc), C.byref(nv), _p(r["key"]), _p(r["sorted"]), _p(r["cstart"]),
+                             _p(r["nbr"]), _p(r["nnb"]), _p(r["hn"])))
+    return r
+
+
+def force(nodes: Nodes, prm: RelaxParams, hn, nbr, nnb) -> np.ndarray:
+    F = np.zeros(nodes.n * 4, np.float32)
+    pc, nv = prm.to_c(), nodes.view()
+    lib().orc_force(C.byref(pc), C.byref(nv), _p(hn), _p(nbr), _p(nnb), _p(F))
+    return F
+
+
+def move(grid: Grid, nodes: Nodes, prm: RelaxParams, hn, F, V, fdt, falpha):
+    out = nodes.copy()
+    hn = np.array(hn, np.float32)
+    V = np.array(V, np.float32)
+    pw = np.zeros(nodes.n, np.float32)
+    mv = np.zeros(nodes.n, np.float32)
+    gv, pc, nv = grid.view(), prm.to_c(), out.view()
+    _check(lib().orc_move(C.byref(gv), C.byref(pc), C.byref(nv), _p(hn), _p(F), _p(V), fdt, falpha, _p(pw), _p(mv)))
+    return out, hn, V, pw, mv
+
+
+def relax(grid: Grid, nodes: Nodes, prm: RelaxParams, *, trace: bool = False, nthreads: int = 0):
+    """relax_cpu's restatement on a copy of the nodes: (nodes', stats dict[, trace list])."""
+    out = nodes.copy()
+    st = RelaxStatsC()
+    tr = (IterTrace * max(prm.max_iters, 1))() if trace else None
+    gv, pc, nv = grid.view(), prm.to_c(), out.view()
+    _check(lib().orc_relax(C.byref(gv), C.byref(pc), C.byref(nv), C.byref(st),
+                           tr if tr is not None else None, nthreads))
+    if trace:
+        recs = [{"it": r.it, "rebuilt": r.rebuilt, "nhalf": r.nhalf, "nrefresh": r.nrefresh, "mmax": r.mmax,
+                 "p99": r.p99, "fdt": r.fdt, "falpha": r.falpha, "power": r.power, "hist": list(r.hist)}
+                for r in list(tr)[: st.iters]]
+        return out, st.as_dict(), recs
+    return out, st.as_dict()
+
+
+def field_points(grid: Grid, a: int, b: int, X: np.ndarray):
+    X = np.ascontiguousarray(X, np.float32)
+    m = X.shape[0]
+    phi = np.zeros(m, np.float32)
+    psi = np.zeros(m, np.float32)
+    grad = np.zeros((m, 3), np.float32)
+    lab = np.zeros(m, np.int32)
+    sec = np.zeros(m, np.int32)
+    gv = grid.view()
+    lib().orc_field_points(C.byref(gv), a, b, m, _p(X), _p(phi), _p(psi), _p(grad), _p(lab), _p(sec))
+    return phi, psi, grad, lab, sec
+
+
+def hist_bin(m: float) -> int:
+    return int(lib().orc_hist_bin(m))

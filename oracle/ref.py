@@ -1,0 +1,186 @@
+"""ctypes loader of oracle/_ref/libv2m_ref.so: the reference's own CPU path.
+
+TEST INFRASTRUCTURE ONLY (fixture generation, the CPU baseline). The library
+is compiled by ``make -C oracle ref`` from the reference sources
+(src/v2m_particles.cpp, src/v2m_grid.cpp, src/v2m_sdfshape.cpp, src/v2m_log.cpp)
+plus oracle/ref_harness.cpp; see the harness for what each call wraps.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from trussnet_amd._abi import (K, Grid, GridView, HashInfoC, Nodes, NodesView, RelaxParams, RelaxParamsC,
+                               RelaxStatsC)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_ref", "libv2m_ref.so")
+
+
+class RefGridParams(C.Structure):
+    _fields_ = [("sigma", C.c_float), ("hbase", C.c_float), ("hmin", C.c_float), ("hmax", C.c_float),
+                ("K", C.c_float), ("g", C.c_float)]
+
+
+_lib = None
+
+
+def available() -> bool:
+    return os.path.exists(LIB_PATH)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not available():
+            raise RuntimeError(f"reference library not built: {LIB_PATH} (make -C oracle ref)")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.ref_last_error.restype = C.c_char_p
+        L.ref_grid_build.restype = vp
+        L.ref_grid_build.argtypes = [C.POINTER(C.c_uint16), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float),
+                                     C.POINTER(RefGridParams), C.POINTER(C.c_float)]
+        L.ref_grid_from_view.restype = vp
+        L.ref_grid_from_view.argtypes = [C.POINTER(GridView)]
+        L.ref_grid_free.argtypes = [vp]
+        L.ref_grid_view.argtypes = [vp, C.POINTER(GridView), C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_float)]
+        L.ref_seed.restype = C.c_int64
+        L.ref_seed.argtypes = [vp, C.POINTER(RelaxParamsC)]
+        L.ref_seed_copy.argtypes = [vp] + [C.c_void_p] * 5
+        L.ref_relax.restype = C.c_int
+        L.ref_relax.argtypes = [vp, C.POINTER(RelaxParamsC), C.POINTER(NodesView), C.POINTER(RelaxStatsC)]
+        L.ref_hash.argtypes = [vp, C.POINTER(RelaxParamsC), C.POINTER(HashInfoC)]
+        L.ref_rebuild.argtypes = [vp, C.POINTER(RelaxParamsC), C.POINTER(NodesView)] + [C.c_void_p] * 6
+        L.ref_force.argtypes = [vp, C.POINTER(RelaxParamsC), C.POINTER(NodesView)] + [C.c_void_p] * 4
+        L.ref_move.argtypes = [vp, C.POINTER(RelaxParamsC), C.POINTER(NodesView), C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_float, C.c_float, C.c_void_p, C.c_void_p]
+        L.ref_field_points.argtypes = [vp, C.c_int, C.c_int, C.c_int64] + [C.c_void_p] * 6
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class RefGrid:
+    """A grid held by the reference (built by build_grid_cpu, or copied from arrays)."""
+
+    def __init__(self, handle, keep=None):
+        if not handle:
+            raise RuntimeError("reference: " + lib().ref_last_error().decode())
+        self.h = C.c_void_p(handle)
+        self._keep = keep
+
+    @staticmethod
+    def build(labels: np.ndarray, vs=(1.0, 1.0, 1.0), *, sigma=1.0, hbase=0.0, hmin=0.0, hmax=0.0, K_=3.0, g=0.3,
+              hvox: np.ndarray | None = None) -> "RefGrid":
+        """build_grid_cpu (src/v2m_grid.cpp:47) on a (nz, ny, nx) label array."""
+        nz, ny, nx = labels.shape
+        lab = np.ascontiguousarray(labels, np.uint16)
+        vsa = np.array(vs, np.float32)
+        gp = RefGridParams(sigma, hbase, hmin, hmax, K_, g)
+        hv = None if hvox is None else np.ascontiguousarray(hvox, np.float32)
+        h = lib().ref_grid_build(lab.ctypes.data_as(C.POINTER(C.c_uint16)), nx, ny, nz,
+                                 vsa.ctypes.data_as(C.POINTER(C.c_float)), C.byref(gp),
+                                 None if hv is None else hv.ctypes.data_as(C.POINTER(C.c_float)))
+        return RefGrid(h)
+
+    @staticmethod
+    def from_grid(grid: Grid) -> "RefGrid":
+        v = grid.view()
+        return RefGrid(lib().ref_grid_from_view(C.byref(v)), keep=grid)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.ref_grid_free(self.h)
+            self.h = None
+
+    def grid(self) -> Grid:
+        """A copy of the grid's arrays."""
+        v = GridView()
+        grade = C.POINTER(C.c_uint8)()
+        hb = C.c_float()
+        lib().ref_grid_view(self.h, C.byref(v), C.byref(grade), C.byref(hb))
+        nv = v.nx * v.ny * v.nz
+        nb = v.nbx * v.nby * v.nbz
+        cp = lambda ptr, cnt, dt: np.ctypeslib.as_array(ptr, shape=(cnt,)).astype(dt, copy=True)
+        gr = cp(grade, nv, np.uint8) if grade else None
+        return Grid(v.nx, v.ny, v.nz, tuple(v.vs), cp(v.L, nv, np.uint16), cp(v.bl_cnt, nb, np.int32),
+                    cp(v.bl_lab, nb * 16, np.uint16), cp(v.bl_slot, nb, np.int32),
+                    cp(v.phi, v.nslot * 512, np.float32) if v.nslot else np.zeros(0, np.float32),
+                    cp(v.h, nv, np.float32), float(v.hmin), float(v.hmax), gr)
+
+    def seed(self, prm: RelaxParams) -> Nodes:
+        """seed_cpu (src/v2m_particles.cpp:625)."""
+        pc = prm.to_c()
+        n = lib().ref_seed(self.h, C.byref(pc))
+        if n < 0:
+            raise RuntimeError("reference seed: " + lib().ref_last_error().decode())
+        P = np.zeros((n, 3), np.float32)
+        lab = np.zeros(n, np.uint16)
+        typ = np.zeros(n, np.uint8)
+        part = np.zeros((n, 2), np.uint16)
+        part3 = np.zeros(n, np.uint16)
+        lib().ref_seed_copy(self.h, _p(P), _p(lab), _p(typ), _p(part), _p(part3))
+        return Nodes(P, lab, typ, part)
+
+    def relax(self, nodes: Nodes, prm: RelaxParams) -> tuple[Nodes, dict]:
+        """relax_cpu (src/v2m_particles.cpp:728) on a copy of the nodes."""
+        out = nodes.copy()
+        v = out.view()
+        st = RelaxStatsC()
+        pc = prm.to_c()
+        if lib().ref_relax(self.h, C.byref(pc), C.byref(v), C.byref(st)) != 0:
+            raise RuntimeError("reference relax: " + lib().ref_last_error().decode())
+        return out, st.as_dict()
+
+    def hash_info(self, prm: RelaxParams) -> HashInfoC:
+        hi = HashInfoC()
+        pc = prm.to_c()
+        lib().ref_hash(self.h, C.byref(pc), C.byref(hi))
+        return hi
+
+    def rebuild(self, nodes: Nodes, prm: RelaxParams) -> dict:
+        """The first rebuild of relax_cpu on these nodes: key, sorted, cstart, nbr, nnb, hn."""
+        n = nodes.n
+        nk = self.hash_info(prm).nkeys
+        r = {"key": np.zeros(n, np.int32), "sorted": np.zeros(n, np.int32), "cstart": np.zeros(nk + 1, np.int32),
+             "nbr": np.zeros(n * K, np.int32), "nnb": np.zeros(n, np.int32), "hn": np.zeros(n, np.float32)}
+        v = nodes.view()
+        pc = prm.to_c()
+        lib().ref_rebuild(self.h, C.byref(pc), C.byref(v), _p(r["key"]), _p(r["sorted"]), _p(r["cstart"]),
+                          _p(r["nbr"]), _p(r["nnb"]), _p(r["hn"]))
+        return r
+
+    def force(self, nodes: Nodes, prm: RelaxParams, hn, nbr, nnb) -> np.ndarray:
+        F = np.zeros(nodes.n * 4, np.float32)
+        v = nodes.view()
+        pc = prm.to_c()
+        lib().ref_force(self.h, C.byref(pc), C.byref(v), _p(hn), _p(nbr), _p(nnb), _p(F))
+        return F
+
+    def move(self, nodes: Nodes, prm: RelaxParams, hn, F, V, fdt, falpha):
+        """One v2m_move per node; returns (nodes', hn', V', pw, mv)."""
+        out = nodes.copy()
+        hn = hn.copy()
+        V = V.copy()
+        pw = np.zeros(nodes.n, np.float32)
+        mv = np.zeros(nodes.n, np.float32)
+        v = out.view()
+        pc = prm.to_c()
+        lib().ref_move(self.h, C.byref(pc), C.byref(v), _p(hn), _p(F), _p(V), fdt, falpha, _p(pw), _p(mv))
+        return out, hn, V, pw, mv
+
+    def field_points(self, a: int, b: int, X: np.ndarray):
+        X = np.ascontiguousarray(X, np.float32)
+        m = X.shape[0]
+        phi = np.zeros(m, np.float32)
+        psi = np.zeros(m, np.float32)
+        grad = np.zeros((m, 3), np.float32)
+        lab = np.zeros(m, np.int32)
+        sec = np.zeros(m, np.int32)
+        lib().ref_field_points(self.h, a, b, m, _p(X), _p(phi), _p(psi), _p(grad), _p(lab), _p(sec))
+        return phi, psi, grad, lab, sec

@@ -1,0 +1,252 @@
+"""ctypes mirrors of the C ABI structs in include/trussnet_amd.h.
+
+The structs describe v2mesh's Grid (src/v2m_grid.h:45-81), RelaxParams
+(src/v2m_particles.h:21-41), Nodes (src/v2m_particles.h:78-87) and RelaxStats
+(src/v2m_particles.h:89-96) as raw arrays. ``Grid`` and ``Nodes`` below hold the
+numpy arrays a view points into, so a view stays valid while its owner lives.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field, fields
+
+import numpy as np
+
+K = 24
+MAXLEV = 8
+BS = 8
+SLOT = 512
+BL = 16
+NOLAB = 0xFFFF
+INTERIOR, INTERFACE, JUNCTION, CORNER = 0, 1, 2, 3
+
+TN_OK, TN_ERR_ARG, TN_ERR_NODEVICE, TN_ERR_HIP, TN_ERR_UNSUPPORTED, TN_ERR_INTERNAL = range(6)
+
+_f32p = C.POINTER(C.c_float)
+_i32p = C.POINTER(C.c_int32)
+_u16p = C.POINTER(C.c_uint16)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class GridView(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("nbx", C.c_int32), ("nby", C.c_int32), ("nbz", C.c_int32),
+        ("vs", C.c_float * 3),
+        ("L", _u16p), ("bl_cnt", _i32p), ("bl_lab", _u16p), ("bl_slot", _i32p),
+        ("phi", _f32p), ("nslot", C.c_int64), ("h", _f32p),
+        ("hmin", C.c_float), ("hmax", C.c_float),
+        ("gI", _f32p), ("gTW", _f32p), ("ngTW", C.c_int64), ("gm", C.c_int32),
+    ]
+
+
+class RelaxParamsC(C.Structure):
+    _fields_ = [
+        ("nseed", C.c_int32), ("t", C.c_float), ("skin", C.c_float), ("fscale", C.c_float),
+        ("fsurf", C.c_float), ("dt", C.c_float), ("maxstep", C.c_float), ("snap", C.c_float),
+        ("max_iters", C.c_int32), ("dptol", C.c_float), ("jseed", C.c_float),
+        ("corners", C.c_int32), ("voxel_trap", C.c_int32), ("thin", C.c_float),
+        ("fire", C.c_int32), ("fire_dtmax", C.c_float), ("verbose", C.c_int32),
+        ("rebuild_div", C.c_int32),
+    ]
+
+
+class NodesView(C.Structure):
+    _fields_ = [("n", C.c_int64), ("P", _f32p), ("lab", _u16p), ("typ", _u8p), ("part", _u16p)]
+
+
+class RelaxStatsC(C.Structure):
+    _fields_ = [
+        ("iters", C.c_int32), ("rebuilds", C.c_int32), ("last_move", C.c_float), ("last_p99", C.c_float),
+        ("fire_resets", C.c_int32), ("fire_dt", C.c_float),
+        ("n_interior", C.c_int64), ("n_interface", C.c_int64), ("n_junction", C.c_int64), ("n_corner", C.c_int64),
+        ("ms_seed", C.c_double), ("ms_hash", C.c_double), ("ms_force", C.c_double), ("ms_move", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class HashInfoC(C.Structure):
+    _fields_ = [
+        ("b0", C.c_float), ("ox", C.c_float), ("oy", C.c_float), ("oz", C.c_float),
+        ("nlev", C.c_int32), ("dim", (C.c_int32 * 3) * MAXLEV), ("off", C.c_int32 * MAXLEV),
+        ("nkeys", C.c_int32),
+    ]
+
+
+def _ptr(a: np.ndarray | None, ctype):
+    if a is None:
+        return C.cast(None, C.POINTER(ctype))
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+@dataclass
+class RelaxParams:
+    """RelaxParams with the reference defaults (src/v2m_particles.h:22-39)."""
+    nseed: int = 8
+    t: float = 1.3
+    skin: float = 0.3
+    fscale: float = 1.2
+    fsurf: float = 1.0
+    dt: float = 0.5
+    maxstep: float = 0.2
+    snap: float = 0.5
+    max_iters: int = 500
+    dptol: float = 2e-3
+    jseed: float = 0.8
+    corners: bool = True
+    voxel_trap: bool = False
+    thin: float = 0.0
+    fire: bool = True
+    fire_dtmax: float = 2.0
+    verbose: bool = False
+    rebuild_div: int = 300
+
+    def to_c(self) -> RelaxParamsC:
+        c = RelaxParamsC()
+        for f in fields(self):
+            v = getattr(self, f.name)
+            setattr(c, f.name, int(v) if isinstance(v, bool) else v)
+        return c
+
+
+@dataclass
+class Grid:
+    """The field the nodes move in (v2mesh Grid, src/v2m_grid.h:45-81), x fastest.
+
+    ``L`` and ``h`` are flat arrays of nx*ny*nz; ``phi`` holds nslot*512 values.
+    """
+    nx: int
+    ny: int
+    nz: int
+    vs: tuple
+    L: np.ndarray
+    bl_cnt: np.ndarray
+    bl_lab: np.ndarray
+    bl_slot: np.ndarray
+    phi: np.ndarray
+    h: np.ndarray
+    hmin: float
+    hmax: float
+    grade: np.ndarray | None = None
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def nbx(self):
+        return (self.nx + BS - 1) // BS
+
+    @property
+    def nby(self):
+        return (self.ny + BS - 1) // BS
+
+    @property
+    def nbz(self):
+        return (self.nz + BS - 1) // BS
+
+    @property
+    def nslot(self):
+        return self.phi.size // SLOT
+
+    def normalized(self) -> "Grid":
+        """The same grid with every array contiguous and of the ABI dtype."""
+        return Grid(self.nx, self.ny, self.nz, tuple(float(v) for v in self.vs),
+                    np.ascontiguousarray(self.L, np.uint16).ravel(),
+                    np.ascontiguousarray(self.bl_cnt, np.int32).ravel(),
+                    np.ascontiguousarray(self.bl_lab, np.uint16).ravel(),
+                    np.ascontiguousarray(self.bl_slot, np.int32).ravel(),
+                    np.ascontiguousarray(self.phi, np.float32).ravel(),
+                    np.ascontiguousarray(self.h, np.float32).ravel(),
+                    float(self.hmin), float(self.hmax),
+                    None if self.grade is None else np.ascontiguousarray(self.grade, np.uint8).ravel())
+
+    def check(self) -> None:
+        nv = self.nx * self.ny * self.nz
+        nb = self.nbx * self.nby * self.nbz
+        if self.L.size != nv or self.h.size != nv:
+            raise ValueError("grid: L and h need nx*ny*nz values")
+        if self.bl_cnt.size != nb or self.bl_slot.size != nb or self.bl_lab.size != nb * BL:
+            raise ValueError("grid: brick tables need nbx*nby*nbz entries")
+        if self.phi.size % SLOT:
+            raise ValueError("grid: phi must hold whole 512-value slots")
+
+    def view(self) -> GridView:
+        g = self.normalized() if not self._is_normal() else self
+        g.check()
+        v = GridView()
+        v.nx, v.ny, v.nz = g.nx, g.ny, g.nz
+        v.nbx, v.nby, v.nbz = g.nbx, g.nby, g.nbz
+        v.vs[:] = [float(x) for x in g.vs]
+        v.L = _ptr(g.L, C.c_uint16)
+        v.bl_cnt = _ptr(g.bl_cnt, C.c_int32)
+        v.bl_lab = _ptr(g.bl_lab, C.c_uint16)
+        v.bl_slot = _ptr(g.bl_slot, C.c_int32)
+        v.phi = _ptr(g.phi, C.c_float)
+        v.nslot = g.nslot
+        v.h = _ptr(g.h, C.c_float)
+        v.hmin, v.hmax = g.hmin, g.hmax
+        v.gI = _ptr(None, C.c_float)
+        v.gTW = _ptr(None, C.c_float)
+        v.ngTW = 0
+        v.gm = 0
+        if g is not self:
+            self._keep.append(g)
+        return v
+
+    def _is_normal(self) -> bool:
+        want = [(self.L, np.uint16), (self.bl_cnt, np.int32), (self.bl_lab, np.uint16), (self.bl_slot, np.int32),
+                (self.phi, np.float32), (self.h, np.float32)]
+        return all(a.dtype == t and a.flags["C_CONTIGUOUS"] for a, t in want)
+
+    def arrays(self) -> dict:
+        d = {"dims": np.array([self.nx, self.ny, self.nz], np.int32), "vs": np.array(self.vs, np.float32),
+             "hminmax": np.array([self.hmin, self.hmax], np.float32), "L": self.L, "bl_cnt": self.bl_cnt,
+             "bl_lab": self.bl_lab, "bl_slot": self.bl_slot, "phi": self.phi, "h": self.h}
+        return d
+
+    @staticmethod
+    def from_arrays(d, prefix: str = "") -> "Grid":
+        g = lambda k: d[prefix + k]
+        nx, ny, nz = (int(x) for x in g("dims"))
+        hmin, hmax = (float(x) for x in g("hminmax"))
+        return Grid(nx, ny, nz, tuple(float(x) for x in g("vs")), g("L"), g("bl_cnt"), g("bl_lab"), g("bl_slot"),
+                    g("phi"), g("h"), hmin, hmax).normalized()
+
+
+@dataclass
+class Nodes:
+    """Nodes (src/v2m_particles.h:78-87): P (n,3) float32, lab, typ, part (n,2)."""
+    P: np.ndarray
+    lab: np.ndarray
+    typ: np.ndarray
+    part: np.ndarray
+
+    def __post_init__(self):
+        self.P = np.ascontiguousarray(self.P, np.float32).reshape(-1, 3)
+        self.lab = np.ascontiguousarray(self.lab, np.uint16).ravel()
+        self.typ = np.ascontiguousarray(self.typ, np.uint8).ravel()
+        self.part = np.ascontiguousarray(self.part, np.uint16).reshape(-1, 2)
+        n = self.lab.size
+        if self.P.shape[0] != n or self.typ.size != n or self.part.shape[0] != n:
+            raise ValueError("nodes: P, lab, typ and part must describe the same node count")
+
+    @property
+    def n(self) -> int:
+        return int(self.lab.size)
+
+    def copy(self) -> "Nodes":
+        return Nodes(self.P.copy(), self.lab.copy(), self.typ.copy(), self.part.copy())
+
+    def view(self) -> NodesView:
+        v = NodesView()
+        v.n = self.n
+        v.P = _ptr(self.P, C.c_float)
+        v.lab = _ptr(self.lab, C.c_uint16)
+        v.typ = _ptr(self.typ, C.c_uint8)
+        v.part = _ptr(self.part, C.c_uint16)
+        return v
+
+    def equal(self, other: "Nodes") -> bool:
+        return (np.array_equal(self.P.view(np.uint32), other.P.view(np.uint32)) and np.array_equal(self.typ, other.typ)
+                and np.array_equal(self.part, other.part) and np.array_equal(self.lab, other.lab))

@@ -1,0 +1,98 @@
+"""Synthetic label volumes for the relaxation workloads (numpy, x fastest in memory).
+
+Arrays are (nz, ny, nx) uint16 so that ``.ravel()`` is the x-fastest layout of
+v2mesh's LabelVolume (src/v2m_volume.h:22-27). Label 0 is the exterior.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def _centres(n: int) -> np.ndarray:
+    return np.arange(n, dtype=np.float64)
+
+
+def two_spheres(n: int = 64, r1: float = 24.0, r2: float = 12.0) -> np.ndarray:
+    """Label 1 = ball of radius r1 about the volume centre, label 2 = concentric ball r2.
+
+    The 64^3 case with r 24/12 and a uniform 3-voxel size is the small plain workload.
+    """
+    c = (n - 1) / 2.0
+    z, y, x = np.meshgrid(_centres(n), _centres(n), _centres(n), indexing="ij")
+    r = np.sqrt((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2)
+    lab = np.zeros((n, n, n), np.uint16)
+    lab[r < r1] = 1
+    lab[r < r2] = 2
+    return lab
+
+
+def sectors(n: int = 48, r1: float = 18.0, r2: float = 10.0) -> np.ndarray:
+    """A ball (label 1) holding a smaller ball cut into three sectors (labels 2, 3, 4).
+
+    Triple lines run where the sectors meet each other and label 1; points where
+    four labels meet (0 excluded) sit where the sector axis leaves the inner ball,
+    so the seeds hold interface, junction and corner nodes.
+    """
+    c = (n - 1) / 2.0
+    z, y, x = np.meshgrid(_centres(n), _centres(n), _centres(n), indexing="ij")
+    r = np.sqrt((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2)
+    lab = np.zeros((n, n, n), np.uint16)
+    lab[r < r1] = 1
+    inner = r < r2
+    xs, ys = x - c - 0.3, y - c + 0.2
+    lab[inner & (xs < 0)] = 2
+    lab[inner & (xs >= 0) & (ys < 0)] = 3
+    lab[inner & (xs >= 0) & (ys >= 0)] = 4
+    return lab
+
+
+def _rotation(rng: np.random.Generator) -> np.ndarray:
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    a, b, c, d = q
+    return np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                     [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+                     [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+
+
+def gaussian_field(shape, sigma: float, rng: np.random.Generator) -> np.ndarray:
+    """White noise smoothed by a Gaussian of `sigma` voxels, scaled to unit std (float32)."""
+    from scipy.ndimage import gaussian_filter
+
+    f = gaussian_filter(rng.standard_normal(shape).astype(np.float32), sigma, mode="wrap")
+    f /= max(float(f.std()), 1e-12)
+    return f
+
+
+def nested_ellipsoids(n: int = 256, nlab: int = 6, seed: int = 0, axes=(20.0, 120.0), grf_sigma: float = 4.0,
+                      grf_amp: float = 3.0, centre_jitter: float = 6.0) -> np.ndarray:
+    """Nested, randomly oriented ellipsoids (labels 1..nlab), boundaries perturbed.
+
+    Semi-axes are drawn uniformly in `axes` voxels; ellipsoid k is painted over
+    k-1, largest first, so each later (smaller) one nests inside the earlier ones
+    as far as its axes allow. Every boundary is displaced by a smoothed Gaussian
+    random field (std 1 after smoothing with `grf_sigma`, times `grf_amp` voxels).
+    """
+    rng = np.random.default_rng(seed)
+    ax = rng.uniform(axes[0], axes[1], size=(nlab, 3))
+    order = np.argsort(-np.prod(ax, axis=1))
+    ax = ax[order]
+    rots = [_rotation(rng) for _ in range(nlab)]
+    cen = (n - 1) / 2.0 + rng.uniform(-centre_jitter, centre_jitter, size=(nlab, 3))
+    grf = gaussian_field((n, n, n), grf_sigma, rng) * grf_amp
+    lab = np.zeros((n, n, n), np.uint16)
+    zz = np.arange(n, dtype=np.float32)
+    for k in range(nlab):
+        R = rots[k].astype(np.float32)
+        a = ax[k].astype(np.float32)
+        for z0 in range(0, n, 32):
+            z1 = min(n, z0 + 32)
+            z, y, x = np.meshgrid(zz[z0:z1], zz, zz, indexing="ij")
+            d = np.stack([x - cen[k, 0], y - cen[k, 1], z - cen[k, 2]], axis=-1).astype(np.float32)
+            u = d @ R  # body coordinates
+            q = np.sqrt((u[..., 0] / a[0]) ** 2 + (u[..., 1] / a[1]) ** 2 + (u[..., 2] / a[2]) ** 2)
+            # signed distance (approximately, in voxels) to the ellipsoid surface, perturbed
+            s = (q - 1.0) * np.float32(a.mean()) + grf[z0:z1]
+            sl = lab[z0:z1]
+            sl[s < 0] = k + 1
+    return lab
