@@ -1,0 +1,104 @@
+"""The C ABI library: it loads, exports every declared entry point, its structs
+match the header, and without a device every call fails with a status (the
+caller's cue to fall back, as src/v2m_pipeline.cpp:497-526 does)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, Fixture
+
+HEADER = os.path.join(ROOT, "include", "trussnet_amd.h")
+
+
+@pytest.fixture(scope="module")
+def tnlib():
+    from trussnet_amd import relax
+
+    try:
+        relax.lib()
+    except RuntimeError:
+        from trussnet_amd import build
+
+        build.build()
+    return relax
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(tn_[a-z_]+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol(tnlib):
+    names = declared_functions()
+    assert len(names) >= 13
+    out = subprocess.run(["nm", "-D", "--defined-only", tnlib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r"\sT\s(tn_\w+)", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    assert sorted(tnlib.SYMBOLS) == names
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors and the C header agree on every struct size and offset."""
+    import ctypes as C
+
+    from trussnet_amd import _abi
+
+    structs = {"tn_grid_view": _abi.GridView, "tn_relax_params": _abi.RelaxParamsC, "tn_nodes_view": _abi.NodesView,
+               "tn_relax_stats": _abi.RelaxStatsC, "tn_hash_info": _abi.HashInfoC}
+    src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, cls in structs.items():
+        src.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            src.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    src.append("return 0; }")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(c)], check=True)
+    lines = dict(ln.split() for ln in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                     check=True).stdout.splitlines())
+    for cname, cls in structs.items():
+        assert int(lines[cname]) == C.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert int(lines[f"{cname}.{fname}"]) == getattr(cls, fname).offset, (cname, fname)
+
+
+def test_default_params_match_reference(tnlib):
+    from dataclasses import fields
+
+    from trussnet_amd._abi import RelaxParams
+
+    got, want = tnlib.default_params(), RelaxParams()
+    for f in fields(RelaxParams):
+        assert np.float32(getattr(got, f.name)) == np.float32(getattr(want, f.name)), f.name
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is visible here")
+def test_no_device_raises(tnlib):
+    """No HIP device: tn_relax fails with TN_ERR_NODEVICE and the nodes stay as they were."""
+    fx = Fixture("spheres64")
+    assert tnlib.device_count() == 0
+    with pytest.raises(tnlib.NoDevice):
+        tnlib.relax(fx.grid, fx.seeds)
+    with pytest.raises(tnlib.NoDevice):
+        tnlib.Plan(fx.grid, fx.seeds)
+
+
+def test_bad_arguments_rejected(tnlib):
+    """Malformed inputs fail on the host before any device work (any status but OK)."""
+    from trussnet_amd._abi import Grid, RelaxParams
+
+    fx = Fixture("spheres64")
+    g = fx.grid
+    bad = Grid(g.nx, g.ny, g.nz, g.vs, g.L, g.bl_cnt, g.bl_lab, g.bl_slot.copy(), g.phi, g.h, g.hmin, g.hmax)
+    bad.bl_slot[np.argmax(bad.bl_slot)] = g.nslot + 5  # a slot beyond phi
+    with pytest.raises(tnlib.DeviceError):
+        tnlib.relax(bad, fx.seeds)
+    with pytest.raises(tnlib.DeviceError):
+        tnlib.relax(g, fx.seeds, RelaxParams(voxel_trap=True))

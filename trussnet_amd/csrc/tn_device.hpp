@@ -1,0 +1,731 @@
+// tn_device.hpp -- per-node device bodies of the relaxation for gfx950.
+//
+// Field queries on the brick-sparse interface fields, the multi-level hash,
+// the K-nearest truss list, the bar forces and the trapped move. Each function
+// names the reference lines whose arithmetic it reproduces: the results are
+// compared bit for bit with the CPU path, so every expression keeps the
+// reference's association and operand order, and the library is compiled with
+// -ffp-contract=off, correctly rounded fp32 division and square root, and fp32
+// denormals kept (the CPU path's IEEE single precision).
+//
+// The interface queries of the move (psi, projections, label tests) are real
+// calls (__noinline__) that take and return points by value: inlined, the
+// projection tree multiplies into hundreds of copies of the 16-load trilinear
+// cell; as calls with value arguments the points stay in registers.
+//
+// Layout in HBM (device buffers of a plan):
+//   X[i]    float4 (x, y, z, h at the node)  -- positions and sizes, one load per gather
+//   typ[i]  u8, lab[i] u16, part[i] u32 (partner b | partner c << 16)
+//   nbrT[k*n + i]  int32, slot-major K lists (coalesced per slot), nnb[i] int32
+//   Ps[s]   float4 positions and sizes in cell order (the neighbour search's candidates)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/trussnet_amd.h"
+
+#pragma clang fp contract(off)
+
+namespace tnd {
+
+constexpr int kK = TN_K;
+constexpr int kBL = TN_BL;
+constexpr int kNoLab = TN_NOLAB;
+
+// The grid on the device (a tn_grid_view with device pointers).
+struct Field {
+    int nx, ny, nz, nbx, nby, nbz;
+    float vx, vy, vz;
+    const uint16_t* __restrict__ L;
+    const int32_t* __restrict__ bl_cnt;
+    const uint16_t* __restrict__ bl_lab;
+    const int32_t* __restrict__ bl_slot;
+    const float* __restrict__ phi;
+    const float* __restrict__ h;
+};
+
+// The multi-level hash geometry (V2mHash, src/opencl/v2m_particle_body.cl:21-27).
+struct Hash {
+    float b0;
+    int nlev;
+    float ox, oy, oz;
+    int dim[TN_MAXLEV][3];
+    int off[TN_MAXLEV];
+    int nkeys;
+};
+
+struct P3 {
+    float x, y, z;
+};
+
+// a projected point and whether the projection succeeded
+struct P3ok {
+    P3 p;
+    int ok;
+};
+
+__device__ __forceinline__ P3 p3(float x, float y, float z) {
+    P3 r;
+    r.x = x;
+    r.y = y;
+    r.z = z;
+    return r;
+}
+
+__device__ __forceinline__ size_t vox(const Field& f, int i, int j, int k) {
+    return (size_t)i + (size_t)f.nx * ((size_t)j + (size_t)f.ny * (size_t)k);
+}
+
+// v2m_brick_of, src/opencl/v2m_grid_body.cl:116-118
+__device__ __forceinline__ int brick_at(const Field& f, int i, int j, int k) {
+    return (i >> 3) + f.nbx * ((j >> 3) + f.nby * (k >> 3));
+}
+
+// v2m_slot_of, src/opencl/v2m_grid_body.cl:122-134
+__device__ __forceinline__ int slot_in(const Field& f, int b, int l, int* has) {
+    const int cnt = f.bl_cnt[b];
+    *has = 0;
+    const uint16_t* lb = f.bl_lab + (size_t)b * kBL;
+    for (int s = 0; s < cnt && s < kBL; ++s) {
+        if (lb[s] == l) {
+            *has = 1;
+            const int s0 = f.bl_slot[b];
+            return s0 < 0 ? -1 : s0 + s;
+        }
+    }
+    return -1;
+}
+
+// v2m_phi_vox, src/opencl/v2m_grid_body.cl:137-153
+__device__ __noinline__ float phi_voxel(const Field& f, int l, int i, int j, int k) {
+    if (i < 0 || j < 0 || k < 0 || i >= f.nx || j >= f.ny || k >= f.nz) {
+        return l == 0 ? 1.0f : 0.0f;
+    }
+    int has;
+    const int s = slot_in(f, brick_at(f, i, j, k), l, &has);
+    if (s < 0) {
+        return (has && f.L[vox(f, i, j, k)] == l) ? 1.0f : 0.0f;
+    }
+    return f.phi[(size_t)s * TN_SLOT + (size_t)((i & 7) + TN_BS * ((j & 7) + TN_BS * (k & 7)))];
+}
+
+// v2m_phi_cell, src/opencl/v2m_grid_body.cl:159-186 (corner bit 0 -> +i, 1 -> +j, 2 -> +k)
+__device__ __forceinline__ void phi_corners(const Field& f, int l, int i0, int j0, int k0, float* c) {
+    if (i0 >= 0 && j0 >= 0 && k0 >= 0 && i0 + 1 < f.nx && j0 + 1 < f.ny && k0 + 1 < f.nz &&
+        (i0 >> 3) == ((i0 + 1) >> 3) && (j0 >> 3) == ((j0 + 1) >> 3) && (k0 >> 3) == ((k0 + 1) >> 3)) {
+        int has;
+        const int s = slot_in(f, brick_at(f, i0, j0, k0), l, &has);
+        if (s >= 0) {
+            const float* base = f.phi + (size_t)s * TN_SLOT + (size_t)((i0 & 7) + TN_BS * ((j0 & 7) + TN_BS * (k0 & 7)));
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                c[m] = base[(m & 1) + TN_BS * ((m >> 1) & 1) + TN_BS * TN_BS * (m >> 2)];
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                c[m] = (has && f.L[vox(f, i0 + (m & 1), j0 + ((m >> 1) & 1), k0 + (m >> 2))] == l) ? 1.0f : 0.0f;
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        c[m] = phi_voxel(f, l, i0 + (m & 1), j0 + ((m >> 1) & 1), k0 + ((m >> 2) & 1));
+    }
+}
+
+// the trilinear tail of v2m_phi_at / v2m_psi (src/opencl/v2m_grid_body.cl:196-199)
+__device__ __forceinline__ float trilerp(const float* c, float fx, float fy, float fz) {
+    const float x00 = c[0] + fx * (c[1] - c[0]), x10 = c[2] + fx * (c[3] - c[2]);
+    const float x01 = c[4] + fx * (c[5] - c[4]), x11 = c[6] + fx * (c[7] - c[6]);
+    const float y0 = x00 + fy * (x10 - x00), y1 = x01 + fy * (x11 - x01);
+    return y0 + fz * (y1 - y0);
+}
+
+// v2m_phi_at (v2m_phi_f, gm == 0), src/opencl/v2m_grid_body.cl:189-200
+__device__ __noinline__ float phi_point(const Field& f, int l, P3 p) {
+    const float u = p.x / f.vx, v = p.y / f.vy, w = p.z / f.vz;
+    const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
+    float c[8];
+    phi_corners(f, l, i0, j0, k0, c);
+    return trilerp(c, u - (float)i0, v - (float)j0, w - (float)k0);
+}
+
+// v2m_psi (gm == 0), src/opencl/v2m_seed_body.cl:179-205
+__device__ __noinline__ float psi_point(const Field& f, int a, int b, P3 p) {
+    const float u = p.x / f.vx, v = p.y / f.vy, w = p.z / f.vz;
+    const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
+    float c[8], e[8];
+    phi_corners(f, a, i0, j0, k0, c);
+    phi_corners(f, b, i0, j0, k0, e);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        c[m] -= e[m];
+    }
+    return trilerp(c, u - (float)i0, v - (float)j0, w - (float)k0);
+}
+
+// v2m_psi_grad (gm == 0), src/opencl/v2m_seed_body.cl:211-247: (gx, gy, gz, psi)
+__device__ __noinline__ float4 psi_grad_point(const Field& f, int a, int b, P3 p) {
+    const float u = p.x / f.vx, v = p.y / f.vy, w = p.z / f.vz;
+    const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
+    const float fx = u - (float)i0, fy = v - (float)j0, fz = w - (float)k0;
+    float c[8], e[8];
+    phi_corners(f, a, i0, j0, k0, c);
+    phi_corners(f, b, i0, j0, k0, e);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        c[m] -= e[m];
+    }
+    const float x00 = c[0] + fx * (c[1] - c[0]), x10 = c[2] + fx * (c[3] - c[2]);
+    const float x01 = c[4] + fx * (c[5] - c[4]), x11 = c[6] + fx * (c[7] - c[6]);
+    const float y0 = x00 + fy * (x10 - x00), y1 = x01 + fy * (x11 - x01);
+    const float dx0 = (c[1] - c[0]) + fy * ((c[3] - c[2]) - (c[1] - c[0]));
+    const float dx1 = (c[5] - c[4]) + fy * ((c[7] - c[6]) - (c[5] - c[4]));
+    float4 g;
+    g.x = (dx0 + fz * (dx1 - dx0)) / f.vx;
+    g.y = ((x10 - x00) + fz * ((x11 - x01) - (x10 - x00))) / f.vy;
+    g.z = (y1 - y0) / f.vz;
+    g.w = y0 + fz * (y1 - y0);
+    return g;
+}
+
+// The labels of p's brick (v2m_labels_near, src/opencl/v2m_seed_body.cl:42-59) as
+// a brick index and a count: bl_lab[b*16 + s] for s < n (b = -1: outside the grid,
+// the one label 0).
+struct BrickLabels {
+    int b, n;
+};
+
+__device__ __forceinline__ BrickLabels brick_labels_at(const Field& f, P3 p) {
+    const int i = (int)floorf(p.x / f.vx + 0.5f), j = (int)floorf(p.y / f.vy + 0.5f), k = (int)floorf(p.z / f.vz + 0.5f);
+    BrickLabels r;
+    if (i < 0 || j < 0 || k < 0 || i >= f.nx || j >= f.ny || k >= f.nz) {
+        r.b = -1;
+        r.n = 1;
+        return r;
+    }
+    r.b = brick_at(f, i, j, k);
+    const int cnt = f.bl_cnt[r.b];
+    r.n = cnt < kBL ? cnt : kBL;
+    return r;
+}
+
+__device__ __forceinline__ int brick_label(const Field& f, BrickLabels bl, int s) {
+    return bl.b < 0 ? 0 : (int)f.bl_lab[(size_t)bl.b * kBL + s];
+}
+
+// v2m_label_of (gm == 0, voxmode 0), src/opencl/v2m_seed_body.cl:116-175: (best, second)
+__device__ __noinline__ int2 owner_at(const Field& f, P3 p) {
+    const BrickLabels bl = brick_labels_at(f, p);
+    int best = brick_label(f, bl, 0), sec = kNoLab;
+    float pb = -1.0f, ps = pb;
+    if (bl.n == 1) {
+        return make_int2(best, kNoLab);
+    }
+    for (int s = 0; s < bl.n; ++s) {
+        const int l = brick_label(f, bl, s);
+        const float v = phi_point(f, l, p);
+        if (v > pb) {
+            ps = pb;
+            sec = best;
+            pb = v;
+            best = l;
+        } else if (v > ps) {
+            ps = v;
+            sec = l;
+        }
+    }
+    return make_int2(best, sec);
+}
+
+__device__ __forceinline__ P3 along(P3 p, float sg, float t, P3 u) {
+    return p3(p.x + sg * t * u.x, p.y + sg * t * u.y, p.z + sg * t * u.z);
+}
+
+// v2m_bisect_dir, src/opencl/v2m_seed_body.cl:255-320 (doubling probes, then Illinois)
+__device__ __noinline__ P3ok bracket_root(const Field& f, int a, int b, P3 p, P3 u, float v0, float rad, float tg) {
+    P3ok r;
+    r.p = p;
+    r.ok = 1;
+    if (v0 == 0.0f) {
+        return r;
+    }
+    const float sg = v0 > 0.0f ? -1.0f : 1.0f;
+    float t0 = 0.0f, t1 = -1.0f, f0 = v0, f1 = 0.0f;
+    float t = (tg > 0.0f && tg < rad) ? tg : 0.25f * rad;
+    for (int k = 0; k < 8; ++k) {
+        const float v = psi_point(f, a, b, along(p, sg, t, u));
+        if ((v > 0.0f) != (v0 > 0.0f) || v == 0.0f) {
+            t1 = t;
+            f1 = v;
+            break;
+        }
+        t0 = t;
+        f0 = v;
+        if (t >= rad) {
+            break;
+        }
+        t = fminf(2.0f * t, rad);
+    }
+    if (t1 < 0.0f) {
+        r.ok = 0;
+        return r;
+    }
+    int side = 0;
+    for (int it = 0; it < 8 && f1 != 0.0f && fabsf(t1 - t0) > 1e-4f * rad; ++it) {
+        const float tm = (f0 != f1) ? t1 - f1 * (t1 - t0) / (f1 - f0) : 0.5f * (t0 + t1);
+        const float v = psi_point(f, a, b, along(p, sg, tm, u));
+        if ((v > 0.0f) == (f1 > 0.0f)) {
+            t1 = tm;
+            f1 = v;
+            if (side == -1) {
+                f0 *= 0.5f;
+            }
+            side = -1;
+        } else {
+            t0 = t1;
+            f0 = f1;
+            t1 = tm;
+            f1 = v;
+            side = 1;
+        }
+    }
+    r.p.x = p.x + sg * t1 * u.x;
+    r.p.y = p.y + sg * t1 * u.y;
+    r.p.z = p.z + sg * t1 * u.z;
+    return r;
+}
+
+// v2m_project1, src/opencl/v2m_seed_body.cl:324-335 (ok = 0: p unchanged)
+__device__ __noinline__ P3ok onto_interface(const Field& f, int a, int b, P3 p, float rad) {
+    const float4 g = psi_grad_point(f, a, b, p);
+    const float gn = sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
+    if (gn < 1e-9f) {
+        P3ok r;
+        r.p = p;
+        r.ok = 0;
+        return r;
+    }
+    const P3 u = p3(g.x / gn, g.y / gn, g.z / gn);
+    return bracket_root(f, a, b, p, u, g.w, rad, 1.5f * fabsf(g.w) / gn);
+}
+
+// v2m_project2 (gm == 0), src/opencl/v2m_seed_body.cl:342-420 (ok = 0: p unchanged)
+__device__ __noinline__ P3ok onto_junction(const Field& f, int a, int b, int c, P3 p, float rad) {
+    P3ok fail;
+    fail.p = p;
+    fail.ok = 0;
+    P3 q = p;
+    for (int it = 0; it < 3; ++it) {
+        const P3ok r1 = onto_interface(f, a, b, q, rad);
+        if (!r1.ok) {
+            return fail;
+        }
+        q = r1.p;
+        const float4 g1 = psi_grad_point(f, a, b, q);
+        const float4 g2 = psi_grad_point(f, a, c, q);
+        const float v2 = g2.w;
+        const float a11 = g1.x * g1.x + g1.y * g1.y + g1.z * g1.z;
+        if (a11 < 1e-18f) {
+            return fail;
+        }
+        const float pr = (g1.x * g2.x + g1.y * g2.y + g1.z * g2.z) / a11;
+        P3 u = p3(g2.x - pr * g1.x, g2.y - pr * g1.y, g2.z - pr * g1.z);
+        const float un = sqrtf(u.x * u.x + u.y * u.y + u.z * u.z);
+        if (un < 1e-9f) {
+            return fail;
+        }
+        u.x /= un;
+        u.y /= un;
+        u.z /= un;
+        const P3ok r2 = bracket_root(f, a, c, q, u, v2, rad, 1.5f * fabsf(v2) / un);
+        if (!r2.ok) {
+            return fail;
+        }
+        q = r2.p;
+    }
+    const P3ok r3 = onto_interface(f, a, b, q, rad);
+    if (!r3.ok) {
+        return fail;
+    }
+    q = r3.p;
+    const float m2 = (q.x - p.x) * (q.x - p.x) + (q.y - p.y) * (q.y - p.y) + (q.z - p.z) * (q.z - p.z);
+    if (m2 > rad * rad) {
+        return fail;
+    }
+    P3ok r;
+    r.p = q;
+    r.ok = 1;
+    return r;
+}
+
+// ---- hash ------------------------------------------------------------------------------
+
+// v2m_h_at, src/opencl/v2m_particle_body.cl:30-36
+__device__ __forceinline__ float size_at(const Field& f, float px, float py, float pz) {
+    int i = (int)floorf(px / f.vx + 0.5f), j = (int)floorf(py / f.vy + 0.5f), k = (int)floorf(pz / f.vz + 0.5f);
+    i = i < 0 ? 0 : (i >= f.nx ? f.nx - 1 : i);
+    j = j < 0 ? 0 : (j >= f.ny ? f.ny - 1 : j);
+    k = k < 0 ? 0 : (k >= f.nz ? f.nz - 1 : k);
+    return f.h[vox(f, i, j, k)];
+}
+
+// v2m_level_of, src/opencl/v2m_particle_body.cl:38-46
+__device__ __forceinline__ int level_for(const Hash& H, float R) {
+    int L = 0;
+    while (L + 1 < H.nlev && H.b0 * (float)(1 << L) < R) {
+        ++L;
+    }
+    return L;
+}
+
+// v2m_bin_coord, src/opencl/v2m_particle_body.cl:48-52
+__device__ __forceinline__ int cell_coord(const Hash& H, int L, float x, float o, int ax) {
+    const int c = (int)floorf((x - o) / (H.b0 * (float)(1 << L)));
+    const int n = H.dim[L][ax];
+    return c < 0 ? 0 : (c >= n ? n - 1 : c);
+}
+
+// v2m_bin_key, src/opencl/v2m_particle_body.cl:54-58
+__device__ __forceinline__ int cell_key(const Hash& H, int L, float px, float py, float pz) {
+    const int cx = cell_coord(H, L, px, H.ox, 0), cy = cell_coord(H, L, py, H.oy, 1), cz = cell_coord(H, L, pz, H.oz, 2);
+    return H.off[L] + cx + H.dim[L][0] * (cy + H.dim[L][1] * cz);
+}
+
+// ---- truss -----------------------------------------------------------------------------
+
+// The inputs of one K-nearest search (v2m_neighbors, src/opencl/v2m_particle_body.cl:73-151).
+struct TrussIn {
+    const float4* __restrict__ X;      // current positions + h
+    const uint16_t* __restrict__ lab;
+    const uint8_t* __restrict__ typ;
+    const int32_t* __restrict__ cstart;
+    const int32_t* __restrict__ sorted;
+    const float4* __restrict__ Ps;     // positions + h in cell order, as of the last rebuild
+    float t, skin;
+};
+
+// The K nearest valid neighbours of node i in (distance, id) order. ds / ids are
+// the node's K-list in LDS, lane-interleaved with stride `st` (dynamic indexing
+// into private arrays would go to scratch). Writes nbrT[k*n + i], nnb[i].
+// The cells x0..x1 of one row are consecutive keys, so a row is one contiguous
+// run of Ps, scanned in the reference's order (cell by cell, ascending slot).
+// The id is loaded only for candidates within reach.
+__device__ inline void truss_of(const Hash& H, const TrussIn& in, int i, int n_nodes, float* ds, int* ids, int st,
+                                int32_t* __restrict__ nbrT, int32_t* __restrict__ nnb) {
+    const float4 xi = in.X[i];
+    const float px = xi.x, py = xi.y, pz = xi.z;
+    const float hi = xi.w;
+    const float Ri = (in.t + in.skin) * hi;
+    const int ti_int = in.typ[i] == TN_INTERIOR;
+    const int li = in.lab[i];
+    const int Li = level_for(H, Ri);
+    int n = 0;
+    float dlast = 0.0f;   // the K-th distance and id once the list is full
+    int jlast = 0;
+    for (int L = Li - 1; L <= Li + 1; ++L) {
+        if (L < 0 || L >= H.nlev) {
+            continue;
+        }
+        const float bL = H.b0 * (float)(1 << L);
+        const float r = 0.5f * (Ri + bL);
+        const int x0 = cell_coord(H, L, px - r, H.ox, 0), x1 = cell_coord(H, L, px + r, H.ox, 0);
+        const int y0 = cell_coord(H, L, py - r, H.oy, 1), y1 = cell_coord(H, L, py + r, H.oy, 1);
+        const int z0 = cell_coord(H, L, pz - r, H.oz, 2), z1 = cell_coord(H, L, pz + r, H.oz, 2);
+        for (int cz = z0; cz <= z1; ++cz) {
+            for (int cy = y0; cy <= y1; ++cy) {
+                const int krow = H.off[L] + H.dim[L][0] * (cy + H.dim[L][1] * cz);
+                const int s0 = in.cstart[krow + x0], s1 = in.cstart[krow + x1 + 1];
+                for (int s = s0; s < s1; ++s) {
+                    const float4 c = in.Ps[s];
+                    const float ex = c.x - px, ey = c.y - py, ez = c.z - pz;
+                    const float d2 = ex * ex + ey * ey + ez * ez;
+                    const float rj = (in.t + in.skin) * 0.5f * (hi + c.w);
+                    if (d2 >= rj * rj) {
+                        continue;
+                    }
+                    const int j = in.sorted[s];
+                    if (j == i) {
+                        continue;
+                    }
+                    if (ti_int && in.typ[j] == TN_INTERIOR && in.lab[j] != li) {
+                        continue;
+                    }
+                    const float dist = sqrtf(d2);
+                    if (n == kK && (dist > dlast || (dist == dlast && j > jlast))) {
+                        continue;
+                    }
+                    int k = n < kK ? n++ : kK - 1;
+                    while (k > 0) {
+                        const float dp = ds[(k - 1) * st];
+                        const int jp = ids[(k - 1) * st];
+                        if (!(dp > dist || (dp == dist && jp > j))) {
+                            break;
+                        }
+                        ds[k * st] = dp;
+                        ids[k * st] = jp;
+                        --k;
+                    }
+                    ds[k * st] = dist;
+                    ids[k * st] = j;
+                    if (n == kK) {
+                        dlast = ds[(kK - 1) * st];
+                        jlast = ids[(kK - 1) * st];
+                    }
+                }
+            }
+        }
+    }
+    nnb[i] = n;
+    for (int k = 0; k < n; ++k) {
+        nbrT[(size_t)k * n_nodes + i] = ids[k * st];
+    }
+}
+
+// v2m_force, src/opencl/v2m_particle_body.cl:161-184: the repulsive bars of node i,
+// summed in list order (the fixed order that makes the sum reproducible)
+__device__ __forceinline__ float4 bar_force(const float4* __restrict__ X, const uint8_t* __restrict__ typ,
+                                            const int32_t* __restrict__ nbrT, const int32_t* __restrict__ nnb,
+                                            float fscale, float fsurf, int i, int n_nodes) {
+    const float4 xi = X[i];
+    const float px = xi.x, py = xi.y, pz = xi.z, hi = xi.w;
+    const int ti = typ[i];
+    float F0 = 0.0f, F1 = 0.0f, F2 = 0.0f, F3 = 0.0f;
+    const int m = nnb[i];
+    for (int k = 0; k < m; ++k) {
+        const int j = nbrT[(size_t)k * n_nodes + i];
+        const float4 xj = X[j];
+        const float ex = px - xj.x, ey = py - xj.y, ez = pz - xj.z;
+        const float l = sqrtf(ex * ex + ey * ey + ez * ez);
+        const float l0 = ((ti != TN_INTERIOR && typ[j] != TN_INTERIOR) ? fsurf : fscale) * 0.5f * (hi + xj.w);
+        if (l >= l0 || l <= 0.0f) {
+            continue;
+        }
+        const float fr = (l0 - l) / l;
+        F0 += fr * ex;
+        F1 += fr * ey;
+        F2 += fr * ez;
+        F3 += 1.0f;
+    }
+    return make_float4(F0, F1, F2, F3);
+}
+
+// ---- the trapped move --------------------------------------------------------------------
+
+// v2m_crossing, src/opencl/v2m_particle_body.cl:188-212
+__device__ __noinline__ P3 segment_crossing(const Field& f, int a, int b, P3 p, P3 q) {
+    float t0 = 0.0f, t1 = 1.0f;
+    float v0 = psi_point(f, a, b, p);
+    float v1 = psi_point(f, a, b, q);
+    for (int it = 0; it < 6; ++it) {
+        const float tm = 0.5f * (t0 + t1);
+        const float vm = psi_point(f, a, b, p3(p.x + tm * (q.x - p.x), p.y + tm * (q.y - p.y), p.z + tm * (q.z - p.z)));
+        if (vm > 0.0f) {
+            t0 = tm;
+            v0 = vm;
+        } else {
+            t1 = tm;
+            v1 = vm;
+        }
+    }
+    const float t = (v0 - v1) != 0.0f ? t0 + (t1 - t0) * v0 / (v0 - v1) : 0.5f * (t0 + t1);
+    return p3(p.x + t * (q.x - p.x), p.y + t * (q.y - p.y), p.z + t * (q.z - p.z));
+}
+
+// v2m_tangential, src/opencl/v2m_particle_body.cl:215-229: v less its part along n_ab at p
+__device__ __noinline__ P3 drop_normal(const Field& f, int a, int b, P3 p, P3 v) {
+    const float4 g = psi_grad_point(f, a, b, p);
+    const float gn2 = g.x * g.x + g.y * g.y + g.z * g.z;
+    if (gn2 < 1e-12f) {
+        return v;
+    }
+    const float vn = (v.x * g.x + v.y * g.y + v.z * g.z) / gn2;
+    return p3(v.x - vn * g.x, v.y - vn * g.y, v.z - vn * g.z);
+}
+
+// v2m_third_label, src/opencl/v2m_particle_body.cl:232-260
+__device__ __noinline__ int competing_label(const Field& f, int a, int b, P3 q) {
+    const BrickLabels bl = brick_labels_at(f, q);
+    int other = 0;
+    for (int s = 0; s < bl.n; ++s) {
+        const int l = brick_label(f, bl, s);
+        other |= l != a && l != b;
+    }
+    if (!other) {
+        return kNoLab;
+    }
+    const float pa = phi_point(f, a, q);
+    int c = kNoLab;
+    float pc = 0.5f * pa;
+    for (int s = 0; s < bl.n; ++s) {
+        const int l = brick_label(f, bl, s);
+        if (l != a && l != b) {
+            const float w = phi_point(f, l, q);
+            if (w > pc) {
+                pc = w;
+                c = l;
+            }
+        }
+    }
+    return c;
+}
+
+// v2m_valid_on, src/opencl/v2m_particle_body.cl:266-291
+__device__ __noinline__ int labels_dominate(const Field& f, int a, int b, int c, P3 q) {
+    const BrickLabels bl = brick_labels_at(f, q);
+    int other = 0;
+    for (int s = 0; s < bl.n; ++s) {
+        const int l = brick_label(f, bl, s);
+        other |= l != a && l != b && l != c;
+    }
+    if (!other) {
+        return 1;
+    }
+    float mine = fminf(phi_point(f, a, q), phi_point(f, b, q));
+    if (c != kNoLab) {
+        mine = fminf(mine, phi_point(f, c, q));
+    }
+    for (int s = 0; s < bl.n; ++s) {
+        const int l = brick_label(f, bl, s);
+        if (l != a && l != b && l != c && phi_point(f, l, q) > mine + 0.02f) {
+            return 0;
+        }
+    }
+    return 1;
+}
+
+__device__ __forceinline__ float dist2(P3 q, P3 p) {
+    return (q.x - p.x) * (q.x - p.x) + (q.y - p.y) * (q.y - p.y) + (q.z - p.z) * (q.z - p.z);
+}
+
+// A node's mutable state during one move (registers; written back by the caller).
+struct NodeState {
+    P3 p;           // position in, new position out
+    float hn;       // h at the position
+    int ty;         // type: typ[i] as stored
+    int b, c;       // partners as stored: part[2i], part[2i+1]
+};
+
+// v2m_move_step (smooth mode), src/opencl/v2m_particle_body.cl:659-833: apply the
+// capped step s with interface trapping. Returns |dp| / h (0: the node stayed;
+// its type and partners may still have changed, as in the reference).
+__device__ __noinline__ float trapped_step(const Field& f, float h, P3 s, float maxstep, float snap, int a,
+                                           NodeState& ns) {
+    const P3 p = ns.p;
+    P3 q;
+    const int typ0 = ns.ty;   // typ[i] as stored until the final write
+    int ty = ns.ty;
+    int b = ns.b;             // the locals b, c of the reference; ns.b / ns.c are the stored part[]
+    const int c = ns.c;
+
+    if (ty == TN_INTERIOR) {
+        q = p3(p.x + s.x, p.y + s.y, p.z + s.z);
+        const int2 own = owner_at(f, q);
+        const int l = own.x, sec = own.y;
+        if (l != a) {
+            b = l;
+            const P3 x = segment_crossing(f, a, b, p, q);
+            const P3 r = drop_normal(f, a, b, x, p3(q.x - x.x, q.y - x.y, q.z - x.z));
+            q = p3(x.x + r.x, x.y + r.y, x.z + r.z);
+            const P3ok pr = onto_interface(f, a, b, q, 0.5f * h);
+            q = pr.ok ? pr.p : x;
+            ty = TN_INTERFACE;
+            ns.b = b;
+        } else if (sec != kNoLab && snap > 0.0f) {
+            const float4 gq = psi_grad_point(f, a, sec, q);
+            const float v = gq.w;
+            const float gn = sqrtf(gq.x * gq.x + gq.y * gq.y + gq.z * gq.z);
+            if (gn > 1e-9f && v / gn < snap * h) {
+                const P3ok pr = onto_interface(f, a, sec, q, 1.2f * snap * h);
+                if (pr.ok) {
+                    q = pr.p;
+                    ty = TN_INTERFACE;
+                    b = sec;
+                    ns.b = sec;
+                }
+            }
+        }
+    } else if (ty == TN_INTERFACE) {
+        s = drop_normal(f, a, b, p, s);
+        q = p3(p.x + s.x, p.y + s.y, p.z + s.z);
+        const P3ok pr = onto_interface(f, a, b, q, 0.5f * h);
+        if (!pr.ok) {
+            return 0.0f;
+        }
+        q = pr.p;
+    } else {
+        const float4 g1 = psi_grad_point(f, a, b, p);
+        const float4 g2 = psi_grad_point(f, a, c, p);
+        const float t0 = g1.y * g2.z - g1.z * g2.y;
+        const float t1 = g1.z * g2.x - g1.x * g2.z;
+        const float t2 = g1.x * g2.y - g1.y * g2.x;
+        const float tl2 = t0 * t0 + t1 * t1 + t2 * t2;
+        const float st = tl2 > 1e-20f ? (s.x * t0 + s.y * t1 + s.z * t2) / tl2 : 0.0f;
+        q = p3(p.x + st * t0, p.y + st * t1, p.z + st * t2);
+        const P3ok pr = onto_junction(f, a, b, c, q, 0.5f * h);
+        if (!pr.ok) {
+            return 0.0f;
+        }
+        q = pr.p;
+    }
+
+    if (ty == TN_INTERFACE) {
+        const int cc = competing_label(f, a, b, q);
+        if (cc != kNoLab) {
+            const P3ok pr = onto_junction(f, a, b, cc, q, 0.5f * h);
+            if (pr.ok) {
+                q = pr.p;
+                ty = TN_JUNCTION;
+                ns.c = cc;
+            }
+        }
+    }
+
+    {
+        const float lim = (maxstep + fmaxf(snap, 0.5f)) * h;
+        const float m2 = dist2(q, p);
+        if (!(m2 <= lim * lim)) {
+            if (typ0 == TN_INTERIOR) {
+                ns.b = kNoLab;
+            }
+            return 0.0f;
+        }
+    }
+
+    if (ty != TN_INTERIOR && !labels_dominate(f, a, ns.b, ty >= TN_JUNCTION ? ns.c : kNoLab, q)) {
+        if (ty == TN_INTERFACE || typ0 == TN_INTERIOR) {
+            const int cc = competing_label(f, a, ns.b, q);
+            int joined = 0;
+            if (cc != kNoLab) {
+                const P3ok pr = onto_junction(f, a, ns.b, cc, q, 0.5f * h);
+                if (pr.ok && labels_dominate(f, a, ns.b, cc, pr.p)) {
+                    q = pr.p;
+                    joined = 1;
+                }
+            }
+            if (joined) {
+                ty = TN_JUNCTION;
+                ns.c = cc;
+            } else {
+                ns.ty = typ0 == TN_INTERIOR ? TN_INTERIOR : ty;
+                if (ns.ty == TN_INTERIOR) {
+                    ns.b = kNoLab;
+                }
+                return 0.0f;
+            }
+        } else {
+            return 0.0f;
+        }
+    }
+
+    const float mv = sqrtf(dist2(q, p));
+    ns.p = q;
+    ns.ty = ty;
+    ns.hn = size_at(f, q.x, q.y, q.z);
+    return mv / h;
+}
+
+}  // namespace tnd

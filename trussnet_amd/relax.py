@@ -1,0 +1,188 @@
+"""Python entry to the gfx950 relaxation: ctypes over lib/libtrussnet_amd.so.
+
+``relax(grid, nodes, params, device)`` has relax_cl's contract
+(src/v2m_gpu.h:16-18): it relaxes a copy of the nodes on the device and returns
+it with the RelaxStats; any failure raises :class:`DeviceError` (relax_cl
+throws std::runtime_error), and the caller decides whether to fall back, as
+v2mesh's pipeline does (src/v2m_pipeline.cpp:497-526). There is no silent CPU
+path: without the built library every call raises.
+
+``Plan`` keeps the grid and the seeds resident on one device and exposes the
+stages of a relaxation (rebuild, forces, single iterations) for checking.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from ._abi import (K, TN_ERR_NODEVICE, TN_ERR_UNSUPPORTED, TN_OK, Grid, GridView, HashInfoC, Nodes, NodesView,
+                   RelaxParams, RelaxParamsC, RelaxStatsC)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libtrussnet_amd.so")
+
+# every entry point of include/trussnet_amd.h
+SYMBOLS = ["tn_relax_params_default", "tn_last_error", "tn_device_count", "tn_relax", "tn_plan_create",
+           "tn_plan_run", "tn_plan_download", "tn_plan_destroy", "tn_plan_hash_info", "tn_plan_reset",
+           "tn_plan_rebuild", "tn_plan_force", "tn_plan_step"]
+
+
+class DeviceError(RuntimeError):
+    """A failed device call (status code and the library's message)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class UnsupportedMode(DeviceError):
+    pass
+
+
+class NoDevice(DeviceError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"trussnet_amd library not built: {LIB_PATH} (python -m trussnet_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.tn_last_error.restype = C.c_char_p
+        L.tn_relax_params_default.argtypes = [C.POINTER(RelaxParamsC)]
+        L.tn_device_count.argtypes = [C.POINTER(C.c_int32)]
+        L.tn_relax.argtypes = [C.POINTER(GridView), C.POINTER(RelaxParamsC), C.POINTER(NodesView),
+                               C.POINTER(RelaxStatsC), C.c_int32]
+        L.tn_plan_create.argtypes = [C.POINTER(GridView), C.POINTER(RelaxParamsC), C.POINTER(NodesView), C.c_int32,
+                                     C.POINTER(vp)]
+        L.tn_plan_run.argtypes = [vp, C.POINTER(RelaxStatsC)]
+        L.tn_plan_download.argtypes = [vp, C.POINTER(NodesView)]
+        L.tn_plan_destroy.argtypes = [vp]
+        L.tn_plan_hash_info.argtypes = [vp, C.POINTER(HashInfoC)]
+        L.tn_plan_reset.argtypes = [vp]
+        L.tn_plan_rebuild.argtypes = [vp] + [vp] * 5
+        L.tn_plan_force.argtypes = [vp, vp]
+        L.tn_plan_step.argtypes = [vp, C.c_int32, C.POINTER(RelaxStatsC)]
+        for name in SYMBOLS:
+            if name != "tn_last_error" and name != "tn_relax_params_default":
+                getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _raise(code: int):
+    msg = lib().tn_last_error().decode(errors="replace")
+    if code == TN_ERR_NODEVICE:
+        raise NoDevice(code, msg)
+    if code == TN_ERR_UNSUPPORTED:
+        raise UnsupportedMode(code, msg)
+    raise DeviceError(code, msg)
+
+
+def _check(code: int):
+    if code != TN_OK:
+        _raise(code)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def device_count() -> int:
+    c = C.c_int32(0)
+    rc = lib().tn_device_count(C.byref(c))
+    return int(c.value) if rc == TN_OK else 0
+
+
+def default_params() -> RelaxParams:
+    c = RelaxParamsC()
+    lib().tn_relax_params_default(C.byref(c))
+    return RelaxParams(**{name: (bool(getattr(c, name)) if isinstance(getattr(RelaxParams(), name), bool)
+                                 else getattr(c, name)) for name, _ in RelaxParamsC._fields_})
+
+
+def relax(grid: Grid, nodes: Nodes, prm: RelaxParams | None = None, device: int = -1) -> tuple[Nodes, dict]:
+    """Relax a copy of `nodes` on HIP device `device` (-1: the first). Returns (nodes, stats)."""
+    prm = prm or RelaxParams()
+    out = nodes.copy()
+    gv, pc, nv = grid.view(), prm.to_c(), out.view()
+    st = RelaxStatsC()
+    _check(lib().tn_relax(C.byref(gv), C.byref(pc), C.byref(nv), C.byref(st), device))
+    return out, st.as_dict()
+
+
+class Plan:
+    """A grid and its seed nodes resident on one device (tn_plan_* of the C ABI)."""
+
+    def __init__(self, grid: Grid, seeds: Nodes, prm: RelaxParams | None = None, device: int = -1):
+        self.prm = prm or RelaxParams()
+        self.grid = grid
+        self.seeds = seeds
+        self.n = seeds.n
+        self._h = C.c_void_p()
+        gv, pc, nv = grid.view(), self.prm.to_c(), seeds.view()
+        _check(lib().tn_plan_create(C.byref(gv), C.byref(pc), C.byref(nv), device, C.byref(self._h)))
+        self.hash = self.hash_info()
+
+    def close(self):
+        if self._h:
+            lib().tn_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def hash_info(self) -> HashInfoC:
+        hi = HashInfoC()
+        _check(lib().tn_plan_hash_info(self._h, C.byref(hi)))
+        return hi
+
+    def run(self) -> dict:
+        """Restore the seeds and relax them on the device (no host traffic but the control block)."""
+        st = RelaxStatsC()
+        _check(lib().tn_plan_run(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def reset(self):
+        _check(lib().tn_plan_reset(self._h))
+
+    def step(self, iters: int) -> dict:
+        st = RelaxStatsC()
+        _check(lib().tn_plan_step(self._h, iters, C.byref(st)))
+        return st.as_dict()
+
+    def download(self) -> Nodes:
+        out = self.seeds.copy()
+        nv = out.view()
+        _check(lib().tn_plan_download(self._h, C.byref(nv)))
+        return out
+
+    def rebuild(self) -> dict:
+        n = self.n
+        r = {"key": np.zeros(n, np.int32), "sorted": np.zeros(n, np.int32),
+             "cstart": np.zeros(self.hash.nkeys + 1, np.int32), "nbr": np.zeros(n * K, np.int32),
+             "nnb": np.zeros(n, np.int32)}
+        _check(lib().tn_plan_rebuild(self._h, _p(r["key"]), _p(r["sorted"]), _p(r["cstart"]), _p(r["nbr"]),
+                                     _p(r["nnb"])))
+        return r
+
+    def force(self) -> np.ndarray:
+        F = np.zeros(self.n * 4, np.float32)
+        _check(lib().tn_plan_force(self._h, _p(F)))
+        return F
