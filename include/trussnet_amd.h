@@ -99,7 +99,8 @@ typedef struct tn_nodes_view {
     uint16_t* part;           /* [2n] in/out */
 } tn_nodes_view;
 
-/* RelaxStats (src/v2m_particles.h:89-96). The ms_* fields are device times. */
+/* RelaxStats (src/v2m_particles.h:89-96). The ms_* fields are device times while
+ * kernel timing is on (tn_plan_set_timing), else 0. */
 typedef struct tn_relax_stats {
     int32_t iters, rebuilds;
     float last_move, last_p99;
@@ -126,6 +127,23 @@ int tn_device_count(int32_t* count);
  */
 int tn_relax(const tn_grid_view* g, const tn_relax_params* prm, tn_nodes_view* nd, tn_relax_stats* st,
              int32_t device);
+
+/*
+ * Seeding on the device: the graded HCP lattice seeds of seed_cpu
+ * (src/v2m_particles.h:98; its body at src/v2m_particles.cpp:637-669): each
+ * voxel claims the points of its seed level's lattice inside its box
+ * (v2m_seed_voxel, count -> scan -> fill, node order = voxel order) and every
+ * seed is classified and projected onto a nearby interface or junction curve
+ * (v2m_seed_classify). grade[nx*ny*nz] is Grid::grade (src/v2m_grid.h:77).
+ * Junction-line seeds and corner nodes are not generated: prm->jseed must be 0
+ * and prm->corners 0 (TN_ERR_UNSUPPORTED otherwise). tn_seed_count gives the
+ * seed count; tn_seed writes exactly that many nodes into the arrays
+ * (P[3n], lab[n], typ[n], part[2n]).
+ */
+int tn_seed_count(const tn_grid_view* g, const uint8_t* grade, const tn_relax_params* prm, int32_t device,
+                  int64_t* n);
+int tn_seed(const tn_grid_view* g, const uint8_t* grade, const tn_relax_params* prm, int32_t device, int64_t n,
+            float* P, uint16_t* lab, uint8_t* typ, uint16_t* part);
 
 /*
  * A relaxation plan: the grid and the seed nodes resident on one device, with
@@ -168,6 +186,24 @@ int tn_plan_hash_info(const tn_plan* plan, tn_hash_info* hi);
  *     rebuilds included; iteration numbering continues across calls.
  */
 int tn_plan_reset(tn_plan* plan);
+
+/*
+ * Per-kernel device times of a plan, from HIP event pairs recorded on the
+ * plan's stream around each launch (the OpenCL host's V2M_CL_PROFILE timings,
+ * src/v2m_gpu.cpp:250-258, without its extra synchronisations). Off by default
+ * (on with prm->verbose or TN_KERNEL_TIMING in the environment); switching
+ * clears the totals. Kernel classes: keys, scan, radix_sort, gather, neighbors,
+ * order (the rebuild), force, move, stats, refresh, control (an iteration).
+ * The tn_relax_stats ms_* fields are filled from these when timing is on.
+ */
+typedef struct tn_kernel_time {
+    char name[16];
+    int64_t launches;
+    double ms;
+} tn_kernel_time;
+
+int tn_plan_set_timing(tn_plan* plan, int32_t on);
+int tn_plan_timings(const tn_plan* plan, tn_kernel_time* out, int32_t cap, int32_t* count);
 int tn_plan_rebuild(tn_plan* plan, int32_t* key, int32_t* sorted, int32_t* cstart, int32_t* nbr, int32_t* nnb);
 int tn_plan_force(tn_plan* plan, float* F);
 int tn_plan_step(tn_plan* plan, int32_t iters, tn_relax_stats* st);

@@ -93,6 +93,10 @@ def make(name: str, spec: dict) -> str:
     for k, v in G.arrays().items():
         out["grid_" + k] = v
     out["seed_P"], out["seed_lab"], out["seed_typ"], out["seed_part"] = seeds.P, seeds.lab, seeds.typ, seeds.part
+    # the lattice seeds alone: seed_cpu without junction-line seeds and corners
+    out["grid_grade"] = G.grade
+    lat = rg.seed(RelaxParams(jseed=0.0, corners=False))
+    out["lat_P"], out["lat_lab"], out["lat_typ"], out["lat_part"] = lat.P, lat.lab, lat.typ, lat.part
     hi = rg.hash_info(prm)
     out["hash_nlev"] = np.int32(hi.nlev)
     out["hash_nkeys"] = np.int32(hi.nkeys)

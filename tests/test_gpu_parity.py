@@ -58,6 +58,34 @@ def test_relax_runs(tn, case):
             assert np.float32(st[k]) == np.float32(v), (case.name, prm.max_iters, k, st[k], v)
 
 
+def test_seeding(tn, case):
+    """Device seeding = the reference's seed_cpu without junction-line seeds and corners."""
+    from trussnet_amd._abi import Nodes
+
+    d = case.d
+    got = tn.seed(case.grid, RelaxParams(jseed=0.0, corners=False), device=0)
+    want = Nodes(d["lat_P"], d["lat_lab"], d["lat_typ"], d["lat_part"])
+    assert got.n == want.n
+    assert got.equal(want), int((got.P != want.P).any(axis=1).sum())
+
+
+def test_generated_workload_vs_oracle(tn, oracle_lib):
+    """A graded six-label workload built here, seeded on the device, relaxed on both sides."""
+    from trussnet_amd import volumes, workload
+
+    lab = volumes.nested_ellipsoids(96, nlab=6, seed=5, axes=(8.0, 45.0), grf_sigma=3.0, grf_amp=2.0)
+    grid = workload.build_grid(lab, workload.graded_size(lab, 1.5, 6.0, 0.05), hmin=1.5, hmax=6.0)
+    seeds = tn.seed(grid, RelaxParams(jseed=0.0, corners=False), device=0)
+    assert seeds.n > 20000
+    for prm in (RelaxParams(max_iters=30, dptol=0.0), RelaxParams(max_iters=300)):
+        got, st = tn.relax(grid, seeds, prm)
+        want, sw = oracle_lib.relax(grid, seeds, prm)
+        assert got.equal(want), (prm.max_iters, int((got.P != want.P).any(axis=1).sum()))
+        for k in ("iters", "rebuilds", "fire_resets", "n_interior", "n_interface", "n_junction", "n_corner"):
+            assert st[k] == sw[k], k
+        assert np.float32(st["last_move"]) == np.float32(sw["last_move"])
+
+
 def test_plan_rerun_is_identical(tn):
     """A plan restores its seeds on every run: repeated runs give the same bits."""
     case = Fixture("sectors48")

@@ -75,6 +75,10 @@ class HashInfoC(C.Structure):
     ]
 
 
+class KernelTimeC(C.Structure):
+    _fields_ = [("name", C.c_char * 16), ("launches", C.c_int64), ("ms", C.c_double)]
+
+
 def _ptr(a: np.ndarray | None, ctype):
     if a is None:
         return C.cast(None, C.POINTER(ctype))
@@ -210,8 +214,9 @@ class Grid:
         g = lambda k: d[prefix + k]
         nx, ny, nz = (int(x) for x in g("dims"))
         hmin, hmax = (float(x) for x in g("hminmax"))
+        grade = d[prefix + "grade"] if (prefix + "grade") in d else None
         return Grid(nx, ny, nz, tuple(float(x) for x in g("vs")), g("L"), g("bl_cnt"), g("bl_lab"), g("bl_slot"),
-                    g("phi"), g("h"), hmin, hmax).normalized()
+                    g("phi"), g("h"), hmin, hmax, grade).normalized()
 
 
 @dataclass

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "tn_kernels.hpp"
+#include "tn_seed.hpp"
 
 namespace {
 
@@ -128,6 +129,134 @@ __global__ void k_iota(int32_t* __restrict__ a, int n) {
     }
 }
 
+// Device buffers freed together (a plan's, or one seeding call's).
+struct DevMem {
+    std::vector<void*> owned;
+    int device = -1;
+    DevMem() = default;
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    ~DevMem() {
+        if (device >= 0) {
+            (void)hipSetDevice(device);
+        }
+        for (void* p : owned) {
+            (void)hipFree(p);
+        }
+    }
+    template <class T>
+    T* alloc(size_t count) {
+        void* p = nullptr;
+        TN_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+        owned.push_back(p);
+        return static_cast<T*>(p);
+    }
+    template <class T>
+    T* upload(const T* host, size_t count) {
+        T* d = alloc<T>(count);
+        if (count) {
+            TN_HIP(hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice));
+        }
+        return d;
+    }
+};
+
+// the grid's arrays on the device (Grid, src/v2m_grid.h:45-81)
+tnd::Field upload_grid(const tn_grid_view* g, DevMem& mem) {
+    const size_t nv = (size_t)g->nx * g->ny * g->nz;
+    const size_t nb = (size_t)g->nbx * g->nby * g->nbz;
+    tnd::Field f;
+    f.nx = g->nx;
+    f.ny = g->ny;
+    f.nz = g->nz;
+    f.nbx = g->nbx;
+    f.nby = g->nby;
+    f.nbz = g->nbz;
+    f.vx = g->vs[0];
+    f.vy = g->vs[1];
+    f.vz = g->vs[2];
+    f.L = mem.upload(g->L, nv);
+    f.bl_cnt = mem.upload(g->bl_cnt, nb);
+    f.bl_lab = mem.upload(g->bl_lab, nb * TN_BL);
+    f.bl_slot = mem.upload(g->bl_slot, nb);
+    f.phi = mem.upload(g->phi, (size_t)g->nslot * TN_SLOT);
+    f.h = mem.upload(g->h, nv);
+    return f;
+}
+
+// Kernel classes of the per-kernel device timings.
+enum KClass { KC_KEYS, KC_SCAN, KC_RADIX, KC_GATHER, KC_NEIGHBORS, KC_ORDER, KC_FORCE, KC_MOVE, KC_STATS, KC_REFRESH,
+              KC_CONTROL, KC_COUNT };
+const char* const kKernelNames[KC_COUNT] = { "keys", "scan", "radix_sort", "gather", "neighbors", "order",
+                                             "force", "move", "stats", "refresh", "control" };
+
+// Device time per kernel class from event pairs on one stream. Elapsed times are
+// read only after the stream has been synchronised (collect), so timing adds
+// event records to the stream but no host waits.
+struct KernelTimer {
+    bool on = false;
+    hipStream_t stream = nullptr;
+    std::vector<hipEvent_t> pool;
+    struct Rec {
+        int kc;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> pending;
+    double ms[KC_COUNT] = {};
+    int64_t launches[KC_COUNT] = {};
+
+    ~KernelTimer() {
+        for (const Rec& r : pending) {
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        for (hipEvent_t e : pool) {
+            (void)hipEventDestroy(e);
+        }
+    }
+    hipEvent_t take() {
+        if (pool.empty()) {
+            hipEvent_t e;
+            TN_HIP(hipEventCreate(&e));
+            return e;
+        }
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t begin() {
+        if (!on) {
+            return nullptr;
+        }
+        hipEvent_t e = take();
+        TN_HIP(hipEventRecord(e, stream));
+        return e;
+    }
+    void end(int kc, hipEvent_t a) {
+        if (!on || !a) {
+            return;
+        }
+        hipEvent_t b = take();
+        TN_HIP(hipEventRecord(b, stream));
+        pending.push_back({ kc, a, b });
+    }
+    void collect() {   // the stream has been synchronised
+        for (const Rec& r : pending) {
+            float t = 0;
+            TN_HIP(hipEventElapsedTime(&t, r.a, r.b));
+            ms[r.kc] += t;
+            launches[r.kc] += 1;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        pending.clear();
+    }
+    void clear() {
+        std::fill(ms, ms + KC_COUNT, 0.0);
+        std::fill(launches, launches + KC_COUNT, 0);
+    }
+};
+
 }  // namespace
 
 struct tn_plan {
@@ -139,9 +268,9 @@ struct tn_plan {
     tnd::Field f{};
     tnd::Hash H{};
     float tskin = 0.0f;
-    bool profile = false;
+    KernelTimer timer;
     // device buffers
-    std::vector<void*> owned;
+    DevMem mem;
     float4 *dX0 = nullptr, *dX = nullptr, *dV = nullptr, *dF = nullptr, *dP0 = nullptr, *dPs = nullptr;
     uint8_t *dTyp0 = nullptr, *dTyp = nullptr;
     uint32_t *dPart0 = nullptr, *dPart = nullptr;
@@ -158,34 +287,24 @@ struct tn_plan {
     // host-side loop state
     int rebuilds = 0;
     bool finished = false;
-    double ms_hash = 0, ms_force = 0, ms_move = 0;
-    hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };
 
     template <class T>
     T* alloc(size_t count) {
-        void* p = nullptr;
-        TN_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
-        owned.push_back(p);
-        return static_cast<T*>(p);
+        return mem.alloc<T>(count);
     }
 
     ~tn_plan() {
         if (device >= 0) {
-            hipSetDevice(device);
-        }
-        for (void* p : owned) {
-            hipFree(p);
-        }
-        if (hCtl) {
-            hipHostFree(hCtl);
-        }
-        for (hipEvent_t& e : ev) {
-            if (e) {
-                hipEventDestroy(e);
-            }
+            (void)hipSetDevice(device);
         }
         if (stream) {
-            hipStreamDestroy(stream);
+            (void)hipStreamSynchronize(stream);
+        }
+        if (hCtl) {
+            (void)hipHostFree(hCtl);
+        }
+        if (stream) {
+            (void)hipStreamDestroy(stream);
         }
     }
 
@@ -212,13 +331,15 @@ struct tn_plan {
     // the rebuild lambda (src/v2m_particles.cpp:771-820) and the type order
     // (src/v2m_gpu.cpp:288-302)
     void rebuild() {
-        if (profile) {
-            TN_HIP(hipEventRecord(ev[0], stream));
-        }
         const unsigned nb = blocks_for(n, tnk::kBlock);
+        hipEvent_t t = timer.begin();
         tnk::k_fill_i32<<<blocks_for(nkeys + 1, tnk::kBlock), tnk::kBlock, 0, stream>>>(dCstart, nkeys + 1, 0);
         tnk::k_keys<<<nb, tnk::kBlock, 0, stream>>>(f, H, dX, n, tskin, dKey, dCstart);
+        timer.end(KC_KEYS, t);
+        t = timer.begin();
         scan(dCstart, dCstart, nkeys + 1);
+        timer.end(KC_SCAN, t);
+        t = timer.begin();
         if (radix_passes == 0) {
             k_iota<<<nb, tnk::kBlock, 0, stream>>>(dSorted, n);
         }
@@ -232,19 +353,20 @@ struct tn_plan {
             tnk::k_radix_scatter<<<radix_nblk, tnk::kBlock, 0, stream>>>(kin, vin, n, 8 * p, radix_nblk, dBhist, kout,
                                                                           vout);
         }
+        timer.end(KC_RADIX, t);
+        t = timer.begin();
         tnk::k_gather<<<nb, tnk::kBlock, 0, stream>>>(dSorted, dX, n, dPs, dP0);
+        timer.end(KC_GATHER, t);
+        t = timer.begin();
         tnk::k_neighbors<<<blocks_for(n, tnk::kNbrBlock), tnk::kNbrBlock, 0, stream>>>(H, truss_in(), n, dNbrT, dNnb);
+        timer.end(KC_NEIGHBORS, t);
+        t = timer.begin();
         tnk::k_order_begin<<<1, 64, 0, stream>>>(dCtl);
         tnk::k_order_count<<<nb, tnk::kBlock, 0, stream>>>(dTyp, n, dCtl);
         tnk::k_order_scatter<<<nb, tnk::kBlock, 0, stream>>>(dTyp, n, dCtl, dOrder);
+        timer.end(KC_ORDER, t);
+        TN_HIP(hipGetLastError());
         ++rebuilds;
-        if (profile) {
-            TN_HIP(hipEventRecord(ev[1], stream));
-            TN_HIP(hipEventSynchronize(ev[1]));
-            float ms = 0;
-            TN_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
-            ms_hash += ms;
-        }
     }
 
     // device state back to the seeds, then the first rebuild
@@ -264,43 +386,39 @@ struct tn_plan {
         TN_HIP(hipMemcpyAsync(dCtl, hCtl, sizeof c, hipMemcpyHostToDevice, stream));
         rebuilds = 0;
         finished = false;
-        ms_hash = ms_force = ms_move = 0;
         if (n > 0) {
             rebuild();
         }
         TN_HIP(hipStreamSynchronize(stream));
+        timer.collect();
     }
 
     // one iteration (src/v2m_particles.cpp:828-946); returns false once converged
     bool iterate() {
         const unsigned nb = blocks_for(n, tnk::kBlock);
-        if (profile) {
-            TN_HIP(hipEventRecord(ev[0], stream));
-        }
         tnk::k_iter_begin<<<1, 64, 0, stream>>>(dCtl);
+        hipEvent_t t = timer.begin();
         tnk::k_force<<<nb, tnk::kBlock, 0, stream>>>(dX, dTyp, dNbrT, dNnb, prm.fscale, prm.fsurf, n, dF);
-        if (profile) {
-            TN_HIP(hipEventRecord(ev[1], stream));
-        }
+        timer.end(KC_FORCE, t);
+        t = timer.begin();
         tnk::MoveArgs ma{ prm.dt, prm.maxstep, prm.snap, fire };
         tnk::k_move<<<nb, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dOrder, n, dF, dX, dV, dTyp, dPart, dLab, dMv, dPw);
+        timer.end(KC_MOVE, t);
+        t = timer.begin();
         tnk::k_stats<<<nb, tnk::kBlock, 0, stream>>>(dX, dP0, dMv, dPw, n, prm.skin, fire, dThr, dCtl, dQueue,
                                                      dPartials);
+        timer.end(KC_STATS, t);
+        t = timer.begin();
         tnk::k_refresh<<<256, tnk::kNbrBlock, 0, stream>>>(H, truss_in(), n, dQueue, dCtl, dNbrT, dNnb, dP0);
+        timer.end(KC_REFRESH, t);
+        t = timer.begin();
         tnk::CtlArgs ca{ n, prm.max_iters, fire, rebuild_div, prm.dptol, nparts };
         tnk::k_control<<<1, tnk::kBlock, 0, stream>>>(ca, dCtl, dPartials, dPw);
+        timer.end(KC_CONTROL, t);
+        TN_HIP(hipGetLastError());
         TN_HIP(hipMemcpyAsync(hCtl, dCtl, sizeof(tnk::Ctl), hipMemcpyDeviceToHost, stream));
-        if (profile) {
-            TN_HIP(hipEventRecord(ev[2], stream));
-        }
         TN_HIP(hipStreamSynchronize(stream));
-        if (profile) {
-            float a = 0, b = 0;
-            TN_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
-            TN_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
-            ms_force += a;
-            ms_move += b;
-        }
+        timer.collect();
         if (hCtl->done) {
             finished = true;
             return false;
@@ -318,6 +436,7 @@ struct tn_plan {
             }
         }
         TN_HIP(hipStreamSynchronize(stream));
+        timer.collect();
     }
 
     void fill_stats(tn_relax_stats* st) {
@@ -332,18 +451,22 @@ struct tn_plan {
         st->last_p99 = c.last_p99;
         st->fire_resets = c.fire_resets;
         st->fire_dt = c.fire_dt;
-        st->ms_hash = ms_hash;
-        st->ms_force = ms_force;
-        st->ms_move = ms_move;
-        if (n > 0) {
-            std::vector<uint8_t> typ(n);
-            TN_HIP(hipMemcpy(typ.data(), dTyp, n, hipMemcpyDeviceToHost));
-            for (uint8_t t : typ) {
-                st->n_interior += t == TN_INTERIOR;
-                st->n_interface += t == TN_INTERFACE;
-                st->n_junction += t == TN_JUNCTION;
-                st->n_corner += t == TN_CORNER;
-            }
+        const double* ms = timer.ms;
+        st->ms_hash = ms[KC_KEYS] + ms[KC_SCAN] + ms[KC_RADIX] + ms[KC_GATHER] + ms[KC_NEIGHBORS] + ms[KC_ORDER];
+        st->ms_force = ms[KC_FORCE];
+        st->ms_move = ms[KC_MOVE] + ms[KC_STATS] + ms[KC_REFRESH] + ms[KC_CONTROL];
+        if (n > 0) {   // type counts (src/v2m_particles.cpp:968-975), counted on the device
+            tnk::k_order_begin<<<1, 64, 0, stream>>>(dCtl);
+            tnk::k_order_count<<<blocks_for(n, tnk::kBlock), tnk::kBlock, 0, stream>>>(dTyp, n, dCtl);
+            TN_HIP(hipGetLastError());
+            tnk::Ctl c2;
+            TN_HIP(hipMemcpyAsync(hCtl, dCtl, sizeof(tnk::Ctl), hipMemcpyDeviceToHost, stream));
+            TN_HIP(hipStreamSynchronize(stream));
+            c2 = *hCtl;
+            st->n_interface = c2.order_cnt[0];
+            st->n_junction = c2.order_cnt[1];
+            st->n_interior = c2.order_cnt[2];
+            st->n_corner = c2.order_cnt[3];
         }
     }
 };
@@ -400,6 +523,68 @@ void check_nodes(const tn_nodes_view* nd) {
     if (nd->n > 0) {
         arg_check(nd->P && nd->lab && nd->typ && nd->part, "nodes: null array");
     }
+}
+
+// The seeding of seed_cpu_body (src/v2m_particles.cpp:637-669): per-voxel counts,
+// their scan, the fill in voxel order and the classification, on one device.
+// The seeds stay in `mem`; returns their count.
+struct Seeds {
+    int64_t n = 0;
+    float* P = nullptr;
+    uint16_t* lab = nullptr;
+    uint8_t* typ = nullptr;
+    uint16_t* part = nullptr;
+};
+
+Seeds run_seeding(const tn_grid_view* g, const uint8_t* grade, const tn_relax_params* prm, int dev, DevMem& mem,
+                  bool fill) {
+    check_grid(g);
+    arg_check(prm != nullptr && grade != nullptr, "seeding: null argument");
+    if (prm->jseed > 0.0f || prm->corners) {
+        throw TnError(TN_ERR_UNSUPPORTED, "seeding: junction-line seeds (jseed > 0) and corner nodes are not "
+                                          "generated on the device; set jseed = 0 and corners = 0");
+    }
+    arg_check(prm->nseed >= 1 && prm->nseed <= 64, "seeding: nseed out of range");
+    TN_HIP(hipSetDevice(dev));
+    mem.device = dev;
+    const tnd::Field f = upload_grid(g, mem);
+    const int64_t nv = (int64_t)g->nx * g->ny * g->nz;
+    // v2m_seed_spacing (src/opencl/v2m_seed_body.cl:445-447), with the host's powf
+    std::vector<float> spacing(prm->nseed);
+    for (int L = 0; L < prm->nseed; ++L) {
+        spacing[L] = prm->nseed <= 1 ? g->hmin
+                                     : g->hmin * std::pow(g->hmax / g->hmin, (float)L / (float)(prm->nseed - 1));
+        arg_check(spacing[L] > 0.0f && std::isfinite(spacing[L]), "seeding: bad seed spacing");
+    }
+    tnk::SeedArgs sa;
+    sa.grade = mem.upload(grade, (size_t)nv);
+    sa.spacing = mem.upload(spacing.data(), spacing.size());
+    sa.nseed = prm->nseed;
+    int32_t* offs = mem.alloc<int32_t>((size_t)nv + 1);
+    int32_t* bsum = mem.alloc<int32_t>(blocks_for(nv + 1, tnk::kScanTile));
+    tnk::k_fill_i32<<<blocks_for(nv + 1, tnk::kBlock), tnk::kBlock>>>(offs, nv + 1, 0);
+    tnk::k_seed_count<<<blocks_for(nv, tnk::kBlock), tnk::kBlock>>>(f, sa, offs);
+    const unsigned nb = blocks_for(nv + 1, tnk::kScanTile);
+    tnk::k_scan_reduce<<<nb, tnk::kBlock>>>(offs, nv + 1, bsum);
+    tnk::k_scan_top<<<1, tnk::kBlock>>>(bsum, nb);
+    tnk::k_scan_down<<<nb, tnk::kBlock>>>(offs, offs, nv + 1, bsum);
+    TN_HIP(hipGetLastError());
+    int32_t total = 0;
+    TN_HIP(hipMemcpy(&total, offs + nv, sizeof total, hipMemcpyDeviceToHost));
+    Seeds s;
+    s.n = total;
+    if (!fill || total == 0) {
+        return s;
+    }
+    s.P = mem.alloc<float>((size_t)3 * total);
+    s.lab = mem.alloc<uint16_t>(total);
+    s.typ = mem.alloc<uint8_t>(total);
+    s.part = mem.alloc<uint16_t>((size_t)2 * total);
+    tnk::k_seed_fill<<<blocks_for(nv, tnk::kBlock), tnk::kBlock>>>(f, sa, offs, s.P, s.lab);
+    tnk::k_seed_classify<<<blocks_for(total, tnk::kBlock), tnk::kBlock>>>(f, total, s.P, s.lab, s.typ, s.part);
+    TN_HIP(hipGetLastError());
+    TN_HIP(hipDeviceSynchronize());
+    return s;
 }
 
 }  // namespace
@@ -466,46 +651,14 @@ int tn_plan_create(const tn_grid_view* g, const tn_relax_params* prm, const tn_n
         p->fire = prm->fire && !prm->voxel_trap ? 1 : 0;
         p->rebuild_div = prm->rebuild_div > 0 ? prm->rebuild_div : 300;
         p->tskin = prm->t + prm->skin;
-        p->profile = prm->verbose || std::getenv("TN_PROFILE") != nullptr;
         p->H = make_hash(g, prm, &p->nkeys);
         TN_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
-        for (hipEvent_t& e : p->ev) {
-            TN_HIP(hipEventCreate(&e));
-        }
+        p->timer.stream = p->stream;
+        p->timer.on = prm->verbose || std::getenv("TN_KERNEL_TIMING") != nullptr;
         const int n = p->n;
-        const size_t nv = (size_t)g->nx * g->ny * g->nz;
-        const size_t nb = (size_t)g->nbx * g->nby * g->nbz;
-        // the grid
-        uint16_t* dL = p->alloc<uint16_t>(nv);
-        int32_t* dCnt = p->alloc<int32_t>(nb);
-        uint16_t* dBl = p->alloc<uint16_t>(nb * TN_BL);
-        int32_t* dSlot = p->alloc<int32_t>(nb);
-        float* dPhi = p->alloc<float>((size_t)g->nslot * TN_SLOT);
-        float* dH = p->alloc<float>(nv);
-        TN_HIP(hipMemcpy(dL, g->L, nv * 2, hipMemcpyHostToDevice));
-        TN_HIP(hipMemcpy(dCnt, g->bl_cnt, nb * 4, hipMemcpyHostToDevice));
-        TN_HIP(hipMemcpy(dBl, g->bl_lab, nb * TN_BL * 2, hipMemcpyHostToDevice));
-        TN_HIP(hipMemcpy(dSlot, g->bl_slot, nb * 4, hipMemcpyHostToDevice));
-        if (g->nslot > 0) {
-            TN_HIP(hipMemcpy(dPhi, g->phi, (size_t)g->nslot * TN_SLOT * 4, hipMemcpyHostToDevice));
-        }
-        TN_HIP(hipMemcpy(dH, g->h, nv * 4, hipMemcpyHostToDevice));
-        tnd::Field& f = p->f;
-        f.nx = g->nx;
-        f.ny = g->ny;
-        f.nz = g->nz;
-        f.nbx = g->nbx;
-        f.nby = g->nby;
-        f.nbz = g->nbz;
-        f.vx = g->vs[0];
-        f.vy = g->vs[1];
-        f.vz = g->vs[2];
-        f.L = dL;
-        f.bl_cnt = dCnt;
-        f.bl_lab = dBl;
-        f.bl_slot = dSlot;
-        f.phi = dPhi;
-        f.h = dH;
+        p->mem.device = dev;
+        p->f = upload_grid(g, p->mem);
+        const tnd::Field& f = p->f;
         // node state
         p->dX0 = p->alloc<float4>(n);
         p->dX = p->alloc<float4>(n);
@@ -563,6 +716,30 @@ int tn_plan_create(const tn_grid_view* g, const tn_relax_params* prm, const tn_n
         }
         p->reset();
         *out = p.release();
+    });
+}
+
+int tn_plan_set_timing(tn_plan* p, int32_t on) {
+    return guarded([&] {
+        arg_check(p != nullptr, "null plan");
+        TN_HIP(hipSetDevice(p->device));
+        TN_HIP(hipStreamSynchronize(p->stream));
+        p->timer.collect();
+        p->timer.clear();
+        p->timer.on = on != 0;
+    });
+}
+
+int tn_plan_timings(const tn_plan* p, tn_kernel_time* out, int32_t cap, int32_t* count) {
+    return guarded([&] {
+        arg_check(p != nullptr && count != nullptr && cap >= 0 && (cap == 0 || out != nullptr), "bad timings call");
+        *count = KC_COUNT;
+        for (int k = 0; k < KC_COUNT && k < cap; ++k) {
+            std::memset(&out[k], 0, sizeof out[k]);
+            std::strncpy(out[k].name, kKernelNames[k], sizeof out[k].name - 1);
+            out[k].launches = p->timer.launches[k];
+            out[k].ms = p->timer.ms[k];
+        }
     });
 }
 
@@ -694,6 +871,33 @@ int tn_plan_force(tn_plan* p, float* F) {
         TN_HIP(hipGetLastError());
         TN_HIP(hipStreamSynchronize(p->stream));
         TN_HIP(hipMemcpy(F, p->dF, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    });
+}
+
+int tn_seed_count(const tn_grid_view* g, const uint8_t* grade, const tn_relax_params* prm, int32_t device,
+                  int64_t* n) {
+    return guarded([&] {
+        arg_check(n != nullptr, "null count");
+        DevMem mem;
+        *n = run_seeding(g, grade, prm, pick_device(device), mem, false).n;
+    });
+}
+
+int tn_seed(const tn_grid_view* g, const uint8_t* grade, const tn_relax_params* prm, int32_t device, int64_t n,
+            float* P, uint16_t* lab, uint8_t* typ, uint16_t* part) {
+    return guarded([&] {
+        arg_check(n >= 0 && (n == 0 || (P && lab && typ && part)), "seeding: null output array");
+        DevMem mem;
+        const Seeds s = run_seeding(g, grade, prm, pick_device(device), mem, true);
+        arg_check(s.n == n, "seeding: the output holds " + std::to_string(n) + " nodes, the seeding made " +
+                                std::to_string(s.n) + " (tn_seed_count)");
+        if (s.n == 0) {
+            return;
+        }
+        TN_HIP(hipMemcpy(P, s.P, sizeof(float) * 3 * s.n, hipMemcpyDeviceToHost));
+        TN_HIP(hipMemcpy(lab, s.lab, sizeof(uint16_t) * s.n, hipMemcpyDeviceToHost));
+        TN_HIP(hipMemcpy(typ, s.typ, s.n, hipMemcpyDeviceToHost));
+        TN_HIP(hipMemcpy(part, s.part, sizeof(uint16_t) * 2 * s.n, hipMemcpyDeviceToHost));
     });
 }
 

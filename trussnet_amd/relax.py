@@ -17,16 +17,17 @@ import os
 
 import numpy as np
 
-from ._abi import (K, TN_ERR_NODEVICE, TN_ERR_UNSUPPORTED, TN_OK, Grid, GridView, HashInfoC, Nodes, NodesView,
-                   RelaxParams, RelaxParamsC, RelaxStatsC)
+from ._abi import (K, TN_ERR_NODEVICE, TN_ERR_UNSUPPORTED, TN_OK, Grid, GridView, HashInfoC, KernelTimeC, Nodes,
+                   NodesView, RelaxParams, RelaxParamsC, RelaxStatsC)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libtrussnet_amd.so")
 
 # every entry point of include/trussnet_amd.h
-SYMBOLS = ["tn_relax_params_default", "tn_last_error", "tn_device_count", "tn_relax", "tn_plan_create",
-           "tn_plan_run", "tn_plan_download", "tn_plan_destroy", "tn_plan_hash_info", "tn_plan_reset",
-           "tn_plan_rebuild", "tn_plan_force", "tn_plan_step"]
+SYMBOLS = ["tn_relax_params_default", "tn_last_error", "tn_device_count", "tn_relax", "tn_seed_count", "tn_seed",
+           "tn_plan_create", "tn_plan_run", "tn_plan_download", "tn_plan_destroy", "tn_plan_hash_info",
+           "tn_plan_reset", "tn_plan_rebuild", "tn_plan_force", "tn_plan_step", "tn_plan_set_timing",
+           "tn_plan_timings"]
 
 
 class DeviceError(RuntimeError):
@@ -70,6 +71,12 @@ def lib():
         L.tn_plan_rebuild.argtypes = [vp] + [vp] * 5
         L.tn_plan_force.argtypes = [vp, vp]
         L.tn_plan_step.argtypes = [vp, C.c_int32, C.POINTER(RelaxStatsC)]
+        L.tn_plan_set_timing.argtypes = [vp, C.c_int32]
+        L.tn_plan_timings.argtypes = [vp, C.POINTER(KernelTimeC), C.c_int32, C.POINTER(C.c_int32)]
+        L.tn_seed_count.argtypes = [C.POINTER(GridView), vp, C.POINTER(RelaxParamsC), C.c_int32,
+                                    C.POINTER(C.c_int64)]
+        L.tn_seed.argtypes = [C.POINTER(GridView), vp, C.POINTER(RelaxParamsC), C.c_int32, C.c_int64, vp, vp, vp,
+                              vp]
         for name in SYMBOLS:
             if name != "tn_last_error" and name != "tn_relax_params_default":
                 getattr(L, name).restype = C.c_int
@@ -116,6 +123,29 @@ def relax(grid: Grid, nodes: Nodes, prm: RelaxParams | None = None, device: int 
     st = RelaxStatsC()
     _check(lib().tn_relax(C.byref(gv), C.byref(pc), C.byref(nv), C.byref(st), device))
     return out, st.as_dict()
+
+
+def seed(grid: Grid, prm: RelaxParams | None = None, device: int = -1) -> Nodes:
+    """The graded HCP lattice seeds of seed_cpu, classified, made on the device.
+
+    Needs ``grid.grade``; junction-line seeds and corner nodes are not made, so
+    ``prm.jseed`` must be 0 and ``prm.corners`` False.
+    """
+    prm = prm or RelaxParams(jseed=0.0, corners=False)
+    if grid.grade is None:
+        raise ValueError("seeding needs the grid's grades")
+    g = grid.normalized() if grid.grade.dtype != np.uint8 else grid
+    grade = np.ascontiguousarray(g.grade, np.uint8)
+    gv, pc = g.view(), prm.to_c()
+    n = C.c_int64(0)
+    _check(lib().tn_seed_count(C.byref(gv), _p(grade), C.byref(pc), device, C.byref(n)))
+    n = int(n.value)
+    P = np.zeros((n, 3), np.float32)
+    lab = np.zeros(n, np.uint16)
+    typ = np.zeros(n, np.uint8)
+    part = np.zeros((n, 2), np.uint16)
+    _check(lib().tn_seed(C.byref(gv), _p(grade), C.byref(pc), device, n, _p(P), _p(lab), _p(typ), _p(part)))
+    return Nodes(P, lab, typ, part)
 
 
 class Plan:
@@ -186,3 +216,14 @@ class Plan:
         F = np.zeros(self.n * 4, np.float32)
         _check(lib().tn_plan_force(self._h, _p(F)))
         return F
+
+    def set_timing(self, on: bool = True):
+        """Per-kernel device timing (HIP events on the plan's stream); clears the totals."""
+        _check(lib().tn_plan_set_timing(self._h, 1 if on else 0))
+
+    def timings(self) -> dict:
+        """{kernel class: (launches, total ms)} since timing was switched on."""
+        buf = (KernelTimeC * 32)()
+        cnt = C.c_int32(0)
+        _check(lib().tn_plan_timings(self._h, buf, 32, C.byref(cnt)))
+        return {buf[k].name.decode(): (int(buf[k].launches), float(buf[k].ms)) for k in range(min(cnt.value, 32))}

@@ -68,15 +68,15 @@ def nested_ellipsoids(n: int = 256, nlab: int = 6, seed: int = 0, axes=(20.0, 12
                       grf_amp: float = 3.0, centre_jitter: float = 6.0) -> np.ndarray:
     """Nested, randomly oriented ellipsoids (labels 1..nlab), boundaries perturbed.
 
-    Semi-axes are drawn uniformly in `axes` voxels; ellipsoid k is painted over
-    k-1, largest first, so each later (smaller) one nests inside the earlier ones
-    as far as its axes allow. Every boundary is displaced by a smoothed Gaussian
-    random field (std 1 after smoothing with `grf_sigma`, times `grf_amp` voxels).
+    Semi-axes are drawn uniformly in `axes` voxels and sorted per axis, so
+    ellipsoid k's axes are each at least ellipsoid k+1's; ellipsoid k is painted
+    over k-1, largest first, so each later one nests inside the earlier ones as
+    far as its orientation allows. Every boundary is displaced by a smoothed
+    Gaussian random field (std 1 after smoothing with `grf_sigma`, times
+    `grf_amp` voxels).
     """
     rng = np.random.default_rng(seed)
-    ax = rng.uniform(axes[0], axes[1], size=(nlab, 3))
-    order = np.argsort(-np.prod(ax, axis=1))
-    ax = ax[order]
+    ax = -np.sort(-rng.uniform(axes[0], axes[1], size=(nlab, 3)), axis=0)
     rots = [_rotation(rng) for _ in range(nlab)]
     cen = (n - 1) / 2.0 + rng.uniform(-centre_jitter, centre_jitter, size=(nlab, 3))
     grf = gaussian_field((n, n, n), grf_sigma, rng) * grf_amp
