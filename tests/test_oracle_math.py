@@ -1,0 +1,312 @@
+"""What the mathematics fixes about the relaxation, checked on the golden cases.
+
+These checks do not retype the restatement's formulas: each one derives the
+expected value another way (brute force over all pairs, an energy's gradient by
+finite differences, an independent trilinear interpolator, Bitzek's FIRE rules
+replayed, a correctly rounded log2), so a dropped term, a wrong sign, a
+transposed index or a missed hash cell fails one of them. They pin both the
+reference's outputs (the fixtures) and the oracle that restates it.
+"""
+import numpy as np
+import pytest
+from scipy import ndimage
+
+from conftest import bits
+from trussnet_amd._abi import K, RelaxParams
+
+INTERIOR = 0
+
+
+def _levels(R, b0, nlev):
+    """Smallest hash level whose cell edge b0 * 2^L is >= R (the top level at most)."""
+    L = np.zeros(R.shape, np.int64)
+    for lev in range(nlev - 1):
+        L += (np.float32(b0) * np.float32(1 << lev) < R)
+    return L
+
+
+def test_lists_equal_brute_force(golden):
+    """K nearest valid bars by (distance, id) over ALL node pairs, levels Li-1..Li+1
+    (src/opencl/v2m_particle_body.cl:73-151): no hash cell may be missed."""
+    d = golden.d
+    prm = RelaxParams()
+    P, lab, typ = golden.seeds.P, golden.seeds.lab, golden.seeds.typ
+    hn = d["rb_hn"]
+    n = P.shape[0]
+    ts = np.float32(prm.t) + np.float32(prm.skin)
+    R = ts * hn
+    lev = _levels(R, d["hash_b0"], int(d["hash_nlev"]))
+    nbr = d["rb_nbr"].reshape(n, K)
+    nnb = d["rb_nnb"]
+    half = ts * np.float32(0.5)
+    for i0 in range(0, n, 256):
+        rows = np.arange(i0, min(n, i0 + 256))
+        e = P[None, :, :] - P[rows, None, :]
+        d2 = e[..., 0] * e[..., 0] + e[..., 1] * e[..., 1] + e[..., 2] * e[..., 2]
+        rj = half * (hn[rows, None] + hn[None, :])
+        ok = d2 < rj * rj
+        ok &= np.arange(n)[None, :] != rows[:, None]
+        ok &= np.abs(lev[None, :] - lev[rows, None]) <= 1
+        cross = (typ[rows, None] == INTERIOR) & (typ[None, :] == INTERIOR) & (lab[rows, None] != lab[None, :])
+        ok &= ~cross
+        dist = np.sqrt(d2)
+        for r, i in enumerate(rows):
+            js = np.nonzero(ok[r])[0]
+            o = np.lexsort((js, dist[r, js]))[:K]
+            want = js[o]
+            assert nnb[i] == len(want), i
+            assert np.array_equal(nbr[i, : nnb[i]], want), i
+
+
+def test_sort_invariants(golden):
+    """Counting sort (src/v2m_particles.cpp:781-795): a permutation, keys ascending,
+    ids ascending within a cell, cell starts = the keys' cumulative counts."""
+    d = golden.d
+    key, srt, cs = d["rb_key"], d["rb_sorted"], d["rb_cstart"]
+    n = key.size
+    assert np.array_equal(np.sort(srt), np.arange(n))
+    ks = key[srt]
+    assert np.all(np.diff(ks) >= 0)
+    same = np.diff(ks) == 0
+    assert np.all(np.diff(srt)[same] > 0)
+    assert cs[0] == 0 and cs[-1] == n
+    assert np.array_equal(cs, np.searchsorted(ks, np.arange(cs.size), side="left"))
+    nlev, off, dim = int(d["hash_nlev"]), d["hash_off"], d["hash_dim"]
+    for L in range(nlev - 1):
+        assert off[L + 1] == off[L] + dim[L].prod()
+    assert cs.size == int(d["hash_nkeys"]) + 1
+
+
+def _energy(P, hn, typ, nbr, nnb, prm, x):
+    """E_i(x) = sum over i's listed bars of (l0 - l)_+^2 / 2, node i moved to x (float64)."""
+    n = P.shape[0]
+    E = np.zeros(n)
+    l0f = np.where(typ != INTERIOR, 1, 0)
+    for k in range(K):
+        act = k < nnb
+        j = np.where(act, nbr[:, k], 0)
+        e = x - P[j].astype(np.float64)
+        l = np.sqrt((e * e).sum(1))
+        surf = (l0f == 1) & (l0f[j] == 1)
+        l0 = np.where(surf, prm.fsurf, prm.fscale) * 0.5 * (hn.astype(np.float64) + hn[j])
+        E += np.where(act & (l < l0), 0.5 * (l0 - l) ** 2, 0.0)
+    return E
+
+
+def test_force_is_minus_energy_gradient(golden):
+    """The DistMesh bar law (src/opencl/v2m_particle_body.cl:161-184) is the
+    negative gradient of (l0 - l)_+^2 / 2 summed over the node's bars."""
+    d = golden.d
+    prm = RelaxParams()
+    P, typ = golden.seeds.P, golden.seeds.typ
+    n = P.shape[0]
+    nbr = d["rb_nbr"].reshape(n, K)
+    nnb = d["rb_nnb"]
+    hn = d["rb_hn"]
+    F = d["force_F"].reshape(n, 4).astype(np.float64)
+    x0 = P.astype(np.float64)
+    eps = 1e-4 * float(hn.min())
+    g = np.zeros((n, 3))
+    for a in range(3):
+        dx = np.zeros(3)
+        dx[a] = eps
+        g[:, a] = (_energy(P, hn, typ, nbr, nnb, prm, x0 + dx) - _energy(P, hn, typ, nbr, nnb, prm, x0 - dx)) / (2 * eps)
+    scale = np.abs(F[:, :3]).max()
+    # bars that switch on or off within +-eps make the energy non-smooth there
+    assert np.quantile(np.abs(-g - F[:, :3]).max(1), 0.995) < 1e-3 * scale
+    assert np.median(np.abs(-g - F[:, :3]).max(1)) < 1e-5 * scale
+    # F[3]: the number of compressed bars with l > 0
+    cnt = np.zeros(n)
+    for k in range(K):
+        act = k < nnb
+        j = np.where(act, nbr[:, k], 0)
+        l = np.sqrt(((x0 - P[j]) ** 2).sum(1))
+        surf = (typ != INTERIOR) & (typ[j] != INTERIOR)
+        l0 = np.where(surf, prm.fsurf, prm.fscale) * 0.5 * (hn.astype(np.float64) + hn[j])
+        cnt += act & (l < l0) & (l > 0)
+    near = np.abs(cnt - F[:, 3]) > 0
+    assert near.mean() < 1e-3
+
+
+def test_move_from_rest_follows_the_force(golden):
+    """From rest FIRE's first step is s = fdt^2 F / stiffness capped at maxstep h
+    (src/opencl/v2m_particle_body.cl:844-935): nodes that stay interior move along
+    F, never farther than the cap; pw = a . v = 0; V = displacement / fdt."""
+    d = golden.d
+    prm = RelaxParams()
+    P0 = golden.seeds.P.astype(np.float64)
+    n = P0.shape[0]
+    F = d["force_F"].reshape(n, 4).astype(np.float64)
+    P1 = d["move_P"].astype(np.float64)
+    h = d["rb_hn"].astype(np.float64)   # the size at the node before the step
+    fdt = float(d["move_fdt"])
+    assert np.float32(fdt) == np.sqrt(np.float32(prm.dt))
+    s = P1 - P0
+    sl = np.linalg.norm(s, axis=1)
+    assert np.all(d["move_pw"] == 0)
+    V = d["move_V"].reshape(n, 3).astype(np.float64)
+    moved = (sl > 0) & (d["move_typ"] == golden.seeds.typ)
+    assert np.allclose(V[moved], s[moved] / fdt, rtol=1e-5, atol=1e-6)
+    assert np.all(V[~moved] == 0)
+    free = moved & (golden.seeds.typ == INTERIOR) & (d["move_typ"] == INTERIOR)
+    a = F[:, :3] / np.maximum(F[:, 3:4], 1.0)
+    want = fdt * fdt * a
+    wl = np.linalg.norm(want, axis=1)
+    cap = prm.maxstep * h
+    want = np.where((wl > cap)[:, None], want * (cap / np.maximum(wl, 1e-30))[:, None], want)
+    assert free.sum() > 0.3 * n
+    # positions are float32: the displacement is known to a few units of their last place
+    ulp = float(np.spacing(np.float32(np.abs(P0).max())))
+    assert np.allclose(s[free], want[free], rtol=2e-5, atol=2 * ulp)
+    assert np.all(sl[free] <= cap[free] * (1 + 1e-5))
+
+
+def test_fire_control_follows_bitzek(golden, oracle_lib):
+    """FIRE's global update (Bitzek et al. 2006; src/v2m_particles.h:47-76) replayed
+    from the traced powers: N_min 5, f_inc 1.1, f_dec 0.5, alpha 0.1 * 0.99^k,
+    dt capped at fire_dtmax * dt0 and floored at 0.05 dt0."""
+    prm = RelaxParams(max_iters=60, dptol=0.0)
+    _, st, tr = oracle_lib.relax(golden.grid, golden.seeds, prm, trace=True)
+    f32 = np.float32
+    dt0 = np.sqrt(f32(prm.dt))
+    dt, alpha, npos, resets = dt0, f32(0.1), 0, 0
+    dtmax = f32(prm.fire_dtmax) * dt0
+    assert len(tr) == 60
+    for r in tr:
+        if r["power"] >= 0.0:
+            npos += 1
+            if npos > 5:
+                dt = min(f32(dt * f32(1.1)), dtmax)
+                alpha = f32(alpha * f32(0.99))
+        else:
+            npos = 0
+            dt = max(f32(dt * f32(0.5)), f32(f32(0.05) * dt0))
+            alpha = f32(0.1)
+            resets += 1
+        assert f32(r["fdt"]) == dt and f32(r["falpha"]) == alpha, r["it"]
+    assert st["fire_resets"] == resets
+    assert f32(st["fire_dt"]) == dt
+
+
+def test_histogram_and_p99(golden, oracle_lib):
+    """The convergence histogram (src/v2m_particles.cpp:877-895): n counts, the
+    largest move in the top non-empty bin, p99 the upper edge of the first bin
+    whose cumulative count reaches 0.99 n."""
+    prm = RelaxParams(max_iters=30, dptol=0.0)
+    _, st, tr = oracle_lib.relax(golden.grid, golden.seeds, prm, trace=True)
+    n = golden.seeds.n
+    for r in tr:
+        h = np.array(r["hist"])
+        assert h.sum() == n
+        top = int(np.nonzero(h)[0].max())
+        assert top == oracle_lib.hist_bin(r["mmax"])
+        b99 = int(np.argmax(np.cumsum(h) >= 0.99 * n))
+        assert np.float32(r["p99"]) == np.ldexp(np.float32(1), b99 - 19)
+
+
+def test_hist_bin_is_correctly_rounded_log2():
+    """bin = clamp(20 + floor(log2f(m)), 0, 31) for m > 0, else 0: the exponent of
+    m except just below a power of two, where log2f rounds up to the next integer.
+    Checked against log2 in double rounded once to float."""
+    from oracle import orc
+
+    xs = []
+    for e in range(-26, 14):
+        x = np.float32(2.0 ** e)
+        for s in (np.float32(0), np.float32(np.inf)):
+            y = x
+            for _ in range(48):
+                y = np.nextafter(y, s)
+                xs.append(y)
+        xs.append(x)
+        xs.append(np.float32(1.5 * 2.0 ** e))
+    xs = np.array(xs, np.float32)
+    want = np.clip(20 + np.floor(np.log2(xs.astype(np.float64)).astype(np.float32)), 0, 31).astype(int)
+    got = np.array([orc.hist_bin(float(x)) for x in xs])
+    assert np.array_equal(got, want)
+    assert orc.hist_bin(0.0) == 0 and orc.hist_bin(-1.0) == 0
+    assert orc.hist_bin(float(np.float32(3e-38))) == 0 and orc.hist_bin(1e30) == 31
+
+
+def _dense_phi(grid, l):
+    """phi_l on every voxel from the brick slots (exact indicator off the slots),
+    padded by one voxel of the exterior value (1 for label 0, else 0)."""
+    nx, ny, nz = grid.nx, grid.ny, grid.nz
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    b = (i >> 3) + grid.nbx * ((j >> 3) + grid.nby * (k >> 3))
+    cnt = np.minimum(grid.bl_cnt[b], 16)
+    labs = grid.bl_lab.reshape(-1, 16)[b]
+    hit = (labs == l) & (np.arange(16) < cnt[..., None])
+    has = hit.any(-1)
+    idx = hit.argmax(-1)
+    slot = grid.bl_slot[b]
+    Lv = grid.L.reshape(nz, ny, nx).transpose(2, 1, 0)
+    out = np.where(has & (Lv == l), 1.0, 0.0).astype(np.float32)
+    use = has & (slot >= 0)
+    s = slot + idx
+    loc = (i & 7) + 8 * ((j & 7) + 8 * (k & 7))
+    out[use] = grid.phi[(s * 512 + loc)[use]]
+    return np.pad(out, 1, constant_values=np.float32(1.0 if l == 0 else 0.0))
+
+
+def test_field_queries_are_trilinear(golden):
+    """phi_a and psi_ab = phi_a - phi_b are trilinear in the voxel values
+    (src/opencl/v2m_grid_body.cl:137-202, v2m_seed_body.cl:179-253), their
+    gradient is the interpolant's, and the owner is the argmax of phi over the
+    brick's labels (v2m_seed_body.cl:116-177)."""
+    d = golden.d
+    g = golden.grid
+    a, b = (int(x) for x in d["fq_ab"])
+    X = d["fq_X"].astype(np.float64)
+    vs = np.asarray(g.vs, np.float64)
+    u = (X / vs).T + 1.0
+    pa, pb = _dense_phi(g, a), _dense_phi(g, b)
+    phi = ndimage.map_coordinates(pa.astype(np.float64), u, order=1, mode="nearest")
+    psi = ndimage.map_coordinates((pa - pb).astype(np.float64), u, order=1, mode="nearest")
+    assert np.abs(phi - d["fq_phi"]).max() < 3e-6
+    assert np.abs(psi - d["fq_psi"]).max() < 6e-6
+    # the interpolant's gradient: central differences that stay inside the cell
+    frac = u - np.floor(u)
+    inside = np.all((frac > 1e-3) & (frac < 1 - 1e-3), axis=0)
+    G = np.zeros((3, X.shape[0]))
+    eps = 1e-4
+    for ax in range(3):
+        du = np.zeros((3, 1))
+        du[ax] = eps
+        G[ax] = (ndimage.map_coordinates((pa - pb).astype(np.float64), u + du, order=1, mode="nearest")
+                 - ndimage.map_coordinates((pa - pb).astype(np.float64), u - du, order=1, mode="nearest")) / (2 * eps)
+        G[ax] /= vs[ax]
+    got = d["fq_grad"].astype(np.float64).T
+    assert inside.sum() > 0.5 * X.shape[0]
+    assert np.abs(G[:, inside] - got[:, inside]).max() < 1e-4 * max(1.0, np.abs(got).max())
+    # owner: the argmax over the brick's labels where the margin is clear
+    labels = np.unique(g.L)
+    vals = np.stack([ndimage.map_coordinates(_dense_phi(g, int(l)).astype(np.float64), u, order=1, mode="nearest")
+                     for l in labels])
+    vox = np.floor(X / vs + 0.5).astype(int)
+    inb = np.all((vox >= 0) & (vox < np.array([g.nx, g.ny, g.nz])), axis=1)
+    bk = (vox[:, 0] >> 3) + g.nbx * ((vox[:, 1] >> 3) + g.nby * (vox[:, 2] >> 3))
+    checked = 0
+    for q in np.nonzero(inb)[0]:
+        c = min(int(g.bl_cnt[bk[q]]), 16)
+        cand = [int(x) for x in g.bl_lab[bk[q] * 16: bk[q] * 16 + c]]
+        if c == 1:
+            assert d["fq_label"][q] == cand[0]
+            continue
+        v = np.array([vals[np.searchsorted(labels, l), q] for l in cand])
+        o = np.argsort(-v, kind="stable")
+        if v[o[0]] - v[o[1]] > 1e-5:
+            assert d["fq_label"][q] == cand[o[0]], q
+            if c > 2 and v[o[1]] - v[o[2]] > 1e-5:
+                assert d["fq_second"][q] == cand[o[1]], q
+            checked += 1
+    assert checked > 0.3 * X.shape[0]
+
+
+def test_oracle_matches_fixture_lists(golden, oracle_lib):
+    """The brute-force lists above are the fixture's; the oracle's are the same bits."""
+    rb = oracle_lib.rebuild(golden.grid, golden.seeds, RelaxParams())
+    nbr = rb["nbr"].reshape(-1, K).copy()
+    nbr[np.arange(K)[None, :] >= rb["nnb"][:, None]] = -1
+    assert np.array_equal(nbr.ravel(), golden.d["rb_nbr"])
+    assert np.array_equal(bits(rb["hn"]), bits(golden.d["rb_hn"]))
