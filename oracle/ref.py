@@ -45,6 +45,7 @@ def lib():
         L.ref_grid_from_view.restype = vp
         L.ref_grid_from_view.argtypes = [C.POINTER(GridView)]
         L.ref_grid_free.argtypes = [vp]
+        L.ref_grid_set_grade.argtypes = [vp, C.c_void_p]
         L.ref_grid_view.argtypes = [vp, C.POINTER(GridView), C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_float)]
         L.ref_seed.restype = C.c_int64
         L.ref_seed.argtypes = [vp, C.POINTER(RelaxParamsC)]
@@ -91,7 +92,12 @@ class RefGrid:
     @staticmethod
     def from_grid(grid: Grid) -> "RefGrid":
         v = grid.view()
-        return RefGrid(lib().ref_grid_from_view(C.byref(v)), keep=grid)
+        r = RefGrid(lib().ref_grid_from_view(C.byref(v)), keep=grid)
+        if grid.grade is not None:
+            grade = np.ascontiguousarray(grid.normalized().grade if grid.grade.dtype != np.uint8 else grid.grade,
+                                         np.uint8)
+            lib().ref_grid_set_grade(r.h, _p(grade))
+        return r
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -213,6 +213,13 @@ void* ref_grid_from_view(const tn_grid_view* v) {
     return r;
 }
 
+// the per-voxel grades (Grid::grade) that seed_cpu reads, for a grid made from a view
+void ref_grid_set_grade(void* h, const uint8_t* grade) {
+    RefGrid* r = static_cast<RefGrid*>(h);
+    const size_t nv = static_cast<size_t>(r->g.nx) * r->g.ny * r->g.nz;
+    r->g.grade.assign(grade, grade + nv);
+}
+
 void ref_grid_free(void* h) {
     delete static_cast<RefGrid*>(h);
 }

@@ -102,3 +102,21 @@ def test_bad_arguments_rejected(tnlib):
         tnlib.relax(bad, fx.seeds)
     with pytest.raises(tnlib.DeviceError):
         tnlib.relax(g, fx.seeds, RelaxParams(voxel_trap=True))
+
+
+def test_cpp_wrapper_builds_and_links(tnlib, tmp_path):
+    """include/trussnet_amd.hpp compiles as C++17 and links against the library."""
+    src = tmp_path / "use.cpp"
+    src.write_text(
+        '#include "trussnet_amd.hpp"\n#include <cstdio>\n'
+        "int main() {\n"
+        "  tn_relax_params p = tnamd::default_params();\n"
+        "  try { tnamd::check(TN_ERR_ARG, \"probe\"); } catch (const std::runtime_error& e) {\n"
+        "    std::printf(\"%.3f %d %d\\n\", p.t, p.max_iters, static_cast<const tnamd::Error&>(e).code());\n"
+        "  }\n  return 0;\n}\n")
+    exe = tmp_path / "use"
+    libdir = os.path.dirname(tnlib.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe), f"-L{libdir}", "-ltrussnet_amd", f"-Wl,-rpath,{libdir}"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    assert out == ["1.300", "500", str(1)]

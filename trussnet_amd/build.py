@@ -37,10 +37,12 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(LIB_DIR, exist_ok=True)
-    tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc(), *HIP_FLAGS, "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
+def build(verbose: bool = False, out: str | None = None, defines: tuple = ()) -> str:
+    """Compile the library (to `out`, default lib/libtrussnet_amd.so) with extra -D defines."""
+    out = out or LIB_PATH
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    tmp = out + ".tmp"
+    cmd = [hipcc(), *HIP_FLAGS, *[f"-D{d}" for d in defines], "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -52,8 +54,8 @@ def build(verbose: bool = False) -> str:
         lines = [ln for ln in r.stderr.splitlines() if "packed-fp32-ops" not in ln]
         if lines:
             print("\n".join(lines))
-    os.replace(tmp, LIB_PATH)
-    return LIB_PATH
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -9,9 +9,9 @@
 // denormals kept (the CPU path's IEEE single precision).
 //
 // The interface queries of the move (psi, projections, label tests) are real
-// calls (__noinline__) that take and return points by value: inlined, the
-// projection tree multiplies into hundreds of copies of the 16-load trilinear
-// cell; as calls with value arguments the points stay in registers.
+// calls (__noinline__) only on their rare paths (junction projections, third
+// labels, cells across bricks), taking and returning points by value; the hot
+// paths are inline and share trilinear-cell caches (PsiCell).
 //
 // Layout in HBM (device buffers of a plan):
 //   X[i]    float4 (x, y, z, h at the node)  -- positions and sizes, one load per gather
@@ -21,6 +21,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <limits.h>
 #include <stdint.h>
 
 #include "../../include/trussnet_amd.h"
@@ -53,6 +54,7 @@ struct Hash {
     int dim[TN_MAXLEV][3];
     int off[TN_MAXLEV];
     int nkeys;
+    int prune;   // every node's reach fits its level's cell: the K-nearest search may skip far cells
 };
 
 struct P3 {
@@ -110,29 +112,91 @@ __device__ __noinline__ float phi_voxel(const Field& f, int l, int i, int j, int
     return f.phi[(size_t)s * TN_SLOT + (size_t)((i & 7) + TN_BS * ((j & 7) + TN_BS * (k & 7)))];
 }
 
+// v2m_phi_cell's single-brick case (src/opencl/v2m_grid_body.cl:161-180): the cell
+// (i0,j0,k0) lies in one brick inside the grid
+__device__ __forceinline__ bool cell_in_one_brick(const Field& f, int i0, int j0, int k0) {
+    return i0 >= 0 && j0 >= 0 && k0 >= 0 && i0 + 1 < f.nx && j0 + 1 < f.ny && k0 + 1 < f.nz &&
+           (i0 >> 3) == ((i0 + 1) >> 3) && (j0 >> 3) == ((j0 + 1) >> 3) && (k0 >> 3) == ((k0 + 1) >> 3);
+}
+
+// the 8 corners of phi_l in a single-brick cell, given l's slot s in that brick
+// (s < 0: uniform or absent, has = l among the brick's labels)
+__device__ __forceinline__ void corners_in_brick(const Field& f, int l, int s, int has, int i0, int j0, int k0,
+                                                 float* c) {
+    if (s >= 0) {
+        const float* base = f.phi + (size_t)s * TN_SLOT + (size_t)((i0 & 7) + TN_BS * ((j0 & 7) + TN_BS * (k0 & 7)));
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            c[m] = base[(m & 1) + TN_BS * ((m >> 1) & 1) + TN_BS * TN_BS * (m >> 2)];
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            c[m] = (has && f.L[vox(f, i0 + (m & 1), j0 + ((m >> 1) & 1), k0 + (m >> 2))] == l) ? 1.0f : 0.0f;
+        }
+    }
+}
+
 // v2m_phi_cell, src/opencl/v2m_grid_body.cl:159-186 (corner bit 0 -> +i, 1 -> +j, 2 -> +k)
 __device__ __forceinline__ void phi_corners(const Field& f, int l, int i0, int j0, int k0, float* c) {
-    if (i0 >= 0 && j0 >= 0 && k0 >= 0 && i0 + 1 < f.nx && j0 + 1 < f.ny && k0 + 1 < f.nz &&
-        (i0 >> 3) == ((i0 + 1) >> 3) && (j0 >> 3) == ((j0 + 1) >> 3) && (k0 >> 3) == ((k0 + 1) >> 3)) {
+    if (cell_in_one_brick(f, i0, j0, k0)) {
         int has;
         const int s = slot_in(f, brick_at(f, i0, j0, k0), l, &has);
-        if (s >= 0) {
-            const float* base = f.phi + (size_t)s * TN_SLOT + (size_t)((i0 & 7) + TN_BS * ((j0 & 7) + TN_BS * (k0 & 7)));
-#pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                c[m] = base[(m & 1) + TN_BS * ((m >> 1) & 1) + TN_BS * TN_BS * (m >> 2)];
-            }
-        } else {
-#pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                c[m] = (has && f.L[vox(f, i0 + (m & 1), j0 + ((m >> 1) & 1), k0 + (m >> 2))] == l) ? 1.0f : 0.0f;
-            }
-        }
+        corners_in_brick(f, l, s, has, i0, j0, k0, c);
         return;
     }
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
         c[m] = phi_voxel(f, l, i0 + (m & 1), j0 + ((m >> 1) & 1), k0 + ((m >> 2) & 1));
+    }
+}
+
+// A trilinear cell of psi_ab = phi_a - phi_b kept across evaluations of one label
+// pair: a bracketing or a crossing search probes a short segment, mostly within
+// one cell and one brick, so the corner differences (and the pair's slots in the
+// brick) are loaded once. The values are the ones v2m_psi computes, so every
+// result is unchanged.
+struct PsiCell {
+    int i0, j0, k0;           // the cell held in d (i0 = INT_MIN: none)
+    int bk, sa, sb, ha, hb;   // the brick whose slots for a and b are known (bk = -1: none)
+    float d[8];               // phi_a - phi_b at the 8 corners
+};
+
+__device__ __forceinline__ void psi_cell_init(PsiCell& pc) {
+    pc.i0 = INT_MIN;
+    pc.j0 = pc.k0 = 0;
+    pc.bk = -1;
+    pc.sa = pc.sb = -1;
+    pc.ha = pc.hb = 0;
+}
+
+__device__ __forceinline__ void psi_cell_load(const Field& f, int a, int b, int i0, int j0, int k0, PsiCell& pc) {
+    if (i0 == pc.i0 && j0 == pc.j0 && k0 == pc.k0) {
+        return;
+    }
+    pc.i0 = i0;
+    pc.j0 = j0;
+    pc.k0 = k0;
+    float c[8], e[8];
+    if (cell_in_one_brick(f, i0, j0, k0)) {
+        const int bk = brick_at(f, i0, j0, k0);
+        if (bk != pc.bk) {
+            pc.bk = bk;
+            pc.sa = slot_in(f, bk, a, &pc.ha);
+            pc.sb = slot_in(f, bk, b, &pc.hb);
+        }
+        corners_in_brick(f, a, pc.sa, pc.ha, i0, j0, k0, c);
+        corners_in_brick(f, b, pc.sb, pc.hb, i0, j0, k0, e);
+    } else {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            c[m] = phi_voxel(f, a, i0 + (m & 1), j0 + ((m >> 1) & 1), k0 + ((m >> 2) & 1));
+            e[m] = phi_voxel(f, b, i0 + (m & 1), j0 + ((m >> 1) & 1), k0 + ((m >> 2) & 1));
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        pc.d[m] = c[m] - e[m];
     }
 }
 
@@ -145,7 +209,7 @@ __device__ __forceinline__ float trilerp(const float* c, float fx, float fy, flo
 }
 
 // v2m_phi_at (v2m_phi_f, gm == 0), src/opencl/v2m_grid_body.cl:189-200
-__device__ __noinline__ float phi_point(const Field& f, int l, P3 p) {
+__device__ __forceinline__ float phi_point(const Field& f, int l, P3 p) {
     const float u = p.x / f.vx, v = p.y / f.vy, w = p.z / f.vz;
     const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
     float c[8];
@@ -153,32 +217,22 @@ __device__ __noinline__ float phi_point(const Field& f, int l, P3 p) {
     return trilerp(c, u - (float)i0, v - (float)j0, w - (float)k0);
 }
 
-// v2m_psi (gm == 0), src/opencl/v2m_seed_body.cl:179-205
-__device__ __noinline__ float psi_point(const Field& f, int a, int b, P3 p) {
+// v2m_psi (gm == 0), src/opencl/v2m_seed_body.cl:179-205, through a cell cache
+__device__ __forceinline__ float psi_at(const Field& f, int a, int b, P3 p, PsiCell& pc) {
     const float u = p.x / f.vx, v = p.y / f.vy, w = p.z / f.vz;
     const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
-    float c[8], e[8];
-    phi_corners(f, a, i0, j0, k0, c);
-    phi_corners(f, b, i0, j0, k0, e);
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-        c[m] -= e[m];
-    }
-    return trilerp(c, u - (float)i0, v - (float)j0, w - (float)k0);
+    psi_cell_load(f, a, b, i0, j0, k0, pc);
+    return trilerp(pc.d, u - (float)i0, v - (float)j0, w - (float)k0);
 }
 
-// v2m_psi_grad (gm == 0), src/opencl/v2m_seed_body.cl:211-247: (gx, gy, gz, psi)
-__device__ __noinline__ float4 psi_grad_point(const Field& f, int a, int b, P3 p) {
+// v2m_psi_grad (gm == 0), src/opencl/v2m_seed_body.cl:211-247: (gx, gy, gz, psi),
+// through a cell cache
+__device__ __forceinline__ float4 psi_grad_at(const Field& f, int a, int b, P3 p, PsiCell& pc) {
     const float u = p.x / f.vx, v = p.y / f.vy, w = p.z / f.vz;
     const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
     const float fx = u - (float)i0, fy = v - (float)j0, fz = w - (float)k0;
-    float c[8], e[8];
-    phi_corners(f, a, i0, j0, k0, c);
-    phi_corners(f, b, i0, j0, k0, e);
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-        c[m] -= e[m];
-    }
+    psi_cell_load(f, a, b, i0, j0, k0, pc);
+    const float* c = pc.d;
     const float x00 = c[0] + fx * (c[1] - c[0]), x10 = c[2] + fx * (c[3] - c[2]);
     const float x01 = c[4] + fx * (c[5] - c[4]), x11 = c[6] + fx * (c[7] - c[6]);
     const float y0 = x00 + fy * (x10 - x00), y1 = x01 + fy * (x11 - x01);
@@ -190,6 +244,13 @@ __device__ __noinline__ float4 psi_grad_point(const Field& f, int a, int b, P3 p
     g.z = (y1 - y0) / f.vz;
     g.w = y0 + fz * (y1 - y0);
     return g;
+}
+
+// psi_ab and its gradient at one point (no cache to share)
+__device__ __noinline__ float4 psi_grad_point(const Field& f, int a, int b, P3 p) {
+    PsiCell pc;
+    psi_cell_init(pc);
+    return psi_grad_at(f, a, b, p, pc);
 }
 
 // The labels of p's brick (v2m_labels_near, src/opencl/v2m_seed_body.cl:42-59) as
@@ -218,7 +279,7 @@ __device__ __forceinline__ int brick_label(const Field& f, BrickLabels bl, int s
 }
 
 // v2m_label_of (gm == 0, voxmode 0), src/opencl/v2m_seed_body.cl:116-175: (best, second)
-__device__ __noinline__ int2 owner_at(const Field& f, P3 p) {
+__device__ __forceinline__ int2 owner_at(const Field& f, P3 p) {
     const BrickLabels bl = brick_labels_at(f, p);
     int best = brick_label(f, bl, 0), sec = kNoLab;
     float pb = -1.0f, ps = pb;
@@ -245,8 +306,10 @@ __device__ __forceinline__ P3 along(P3 p, float sg, float t, P3 u) {
     return p3(p.x + sg * t * u.x, p.y + sg * t * u.y, p.z + sg * t * u.z);
 }
 
-// v2m_bisect_dir, src/opencl/v2m_seed_body.cl:255-320 (doubling probes, then Illinois)
-__device__ __noinline__ P3ok bracket_root(const Field& f, int a, int b, P3 p, P3 u, float v0, float rad, float tg) {
+// v2m_bisect_dir, src/opencl/v2m_seed_body.cl:255-320 (doubling probes, then Illinois).
+// On success the returned point is the last point probed, so pc then holds its cell.
+__device__ __forceinline__ P3ok bracket_along(const Field& f, int a, int b, P3 p, P3 u, float v0, float rad, float tg,
+                                              PsiCell& pc) {
     P3ok r;
     r.p = p;
     r.ok = 1;
@@ -257,7 +320,7 @@ __device__ __noinline__ P3ok bracket_root(const Field& f, int a, int b, P3 p, P3
     float t0 = 0.0f, t1 = -1.0f, f0 = v0, f1 = 0.0f;
     float t = (tg > 0.0f && tg < rad) ? tg : 0.25f * rad;
     for (int k = 0; k < 8; ++k) {
-        const float v = psi_point(f, a, b, along(p, sg, t, u));
+        const float v = psi_at(f, a, b, along(p, sg, t, u), pc);
         if ((v > 0.0f) != (v0 > 0.0f) || v == 0.0f) {
             t1 = t;
             f1 = v;
@@ -277,7 +340,7 @@ __device__ __noinline__ P3ok bracket_root(const Field& f, int a, int b, P3 p, P3
     int side = 0;
     for (int it = 0; it < 8 && f1 != 0.0f && fabsf(t1 - t0) > 1e-4f * rad; ++it) {
         const float tm = (f0 != f1) ? t1 - f1 * (t1 - t0) / (f1 - f0) : 0.5f * (t0 + t1);
-        const float v = psi_point(f, a, b, along(p, sg, tm, u));
+        const float v = psi_at(f, a, b, along(p, sg, tm, u), pc);
         if ((v > 0.0f) == (f1 > 0.0f)) {
             t1 = tm;
             f1 = v;
@@ -299,9 +362,10 @@ __device__ __noinline__ P3ok bracket_root(const Field& f, int a, int b, P3 p, P3
     return r;
 }
 
-// v2m_project1, src/opencl/v2m_seed_body.cl:324-335 (ok = 0: p unchanged)
-__device__ __noinline__ P3ok onto_interface(const Field& f, int a, int b, P3 p, float rad) {
-    const float4 g = psi_grad_point(f, a, b, p);
+// v2m_project1, src/opencl/v2m_seed_body.cl:324-335 (ok = 0: p unchanged), with the
+// a|b cell cache of the caller
+__device__ __forceinline__ P3ok onto_interface_at(const Field& f, int a, int b, P3 p, float rad, PsiCell& pc) {
+    const float4 g = psi_grad_at(f, a, b, p, pc);
     const float gn = sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
     if (gn < 1e-9f) {
         P3ok r;
@@ -310,7 +374,13 @@ __device__ __noinline__ P3ok onto_interface(const Field& f, int a, int b, P3 p, 
         return r;
     }
     const P3 u = p3(g.x / gn, g.y / gn, g.z / gn);
-    return bracket_root(f, a, b, p, u, g.w, rad, 1.5f * fabsf(g.w) / gn);
+    return bracket_along(f, a, b, p, u, g.w, rad, 1.5f * fabsf(g.w) / gn, pc);
+}
+
+__device__ __noinline__ P3ok onto_interface(const Field& f, int a, int b, P3 p, float rad) {
+    PsiCell pc;
+    psi_cell_init(pc);
+    return onto_interface_at(f, a, b, p, rad, pc);
 }
 
 // v2m_project2 (gm == 0), src/opencl/v2m_seed_body.cl:342-420 (ok = 0: p unchanged)
@@ -319,14 +389,17 @@ __device__ __noinline__ P3ok onto_junction(const Field& f, int a, int b, int c, 
     fail.p = p;
     fail.ok = 0;
     P3 q = p;
+    PsiCell pab, pac;   // the a|b and a|c cells: each projection ends where its cache points
+    psi_cell_init(pab);
+    psi_cell_init(pac);
     for (int it = 0; it < 3; ++it) {
-        const P3ok r1 = onto_interface(f, a, b, q, rad);
+        const P3ok r1 = onto_interface_at(f, a, b, q, rad, pab);
         if (!r1.ok) {
             return fail;
         }
         q = r1.p;
-        const float4 g1 = psi_grad_point(f, a, b, q);
-        const float4 g2 = psi_grad_point(f, a, c, q);
+        const float4 g1 = psi_grad_at(f, a, b, q, pab);
+        const float4 g2 = psi_grad_at(f, a, c, q, pac);
         const float v2 = g2.w;
         const float a11 = g1.x * g1.x + g1.y * g1.y + g1.z * g1.z;
         if (a11 < 1e-18f) {
@@ -341,13 +414,13 @@ __device__ __noinline__ P3ok onto_junction(const Field& f, int a, int b, int c, 
         u.x /= un;
         u.y /= un;
         u.z /= un;
-        const P3ok r2 = bracket_root(f, a, c, q, u, v2, rad, 1.5f * fabsf(v2) / un);
+        const P3ok r2 = bracket_along(f, a, c, q, u, v2, rad, 1.5f * fabsf(v2) / un, pac);
         if (!r2.ok) {
             return fail;
         }
         q = r2.p;
     }
-    const P3ok r3 = onto_interface(f, a, b, q, rad);
+    const P3ok r3 = onto_interface_at(f, a, b, q, rad, pab);
     if (!r3.ok) {
         return fail;
     }
@@ -408,83 +481,6 @@ struct TrussIn {
     float t, skin;
 };
 
-// The K nearest valid neighbours of node i in (distance, id) order. ds / ids are
-// the node's K-list in LDS, lane-interleaved with stride `st` (dynamic indexing
-// into private arrays would go to scratch). Writes nbrT[k*n + i], nnb[i].
-// The cells x0..x1 of one row are consecutive keys, so a row is one contiguous
-// run of Ps, scanned in the reference's order (cell by cell, ascending slot).
-// The id is loaded only for candidates within reach.
-__device__ inline void truss_of(const Hash& H, const TrussIn& in, int i, int n_nodes, float* ds, int* ids, int st,
-                                int32_t* __restrict__ nbrT, int32_t* __restrict__ nnb) {
-    const float4 xi = in.X[i];
-    const float px = xi.x, py = xi.y, pz = xi.z;
-    const float hi = xi.w;
-    const float Ri = (in.t + in.skin) * hi;
-    const int ti_int = in.typ[i] == TN_INTERIOR;
-    const int li = in.lab[i];
-    const int Li = level_for(H, Ri);
-    int n = 0;
-    float dlast = 0.0f;   // the K-th distance and id once the list is full
-    int jlast = 0;
-    for (int L = Li - 1; L <= Li + 1; ++L) {
-        if (L < 0 || L >= H.nlev) {
-            continue;
-        }
-        const float bL = H.b0 * (float)(1 << L);
-        const float r = 0.5f * (Ri + bL);
-        const int x0 = cell_coord(H, L, px - r, H.ox, 0), x1 = cell_coord(H, L, px + r, H.ox, 0);
-        const int y0 = cell_coord(H, L, py - r, H.oy, 1), y1 = cell_coord(H, L, py + r, H.oy, 1);
-        const int z0 = cell_coord(H, L, pz - r, H.oz, 2), z1 = cell_coord(H, L, pz + r, H.oz, 2);
-        for (int cz = z0; cz <= z1; ++cz) {
-            for (int cy = y0; cy <= y1; ++cy) {
-                const int krow = H.off[L] + H.dim[L][0] * (cy + H.dim[L][1] * cz);
-                const int s0 = in.cstart[krow + x0], s1 = in.cstart[krow + x1 + 1];
-                for (int s = s0; s < s1; ++s) {
-                    const float4 c = in.Ps[s];
-                    const float ex = c.x - px, ey = c.y - py, ez = c.z - pz;
-                    const float d2 = ex * ex + ey * ey + ez * ez;
-                    const float rj = (in.t + in.skin) * 0.5f * (hi + c.w);
-                    if (d2 >= rj * rj) {
-                        continue;
-                    }
-                    const int j = in.sorted[s];
-                    if (j == i) {
-                        continue;
-                    }
-                    if (ti_int && in.typ[j] == TN_INTERIOR && in.lab[j] != li) {
-                        continue;
-                    }
-                    const float dist = sqrtf(d2);
-                    if (n == kK && (dist > dlast || (dist == dlast && j > jlast))) {
-                        continue;
-                    }
-                    int k = n < kK ? n++ : kK - 1;
-                    while (k > 0) {
-                        const float dp = ds[(k - 1) * st];
-                        const int jp = ids[(k - 1) * st];
-                        if (!(dp > dist || (dp == dist && jp > j))) {
-                            break;
-                        }
-                        ds[k * st] = dp;
-                        ids[k * st] = jp;
-                        --k;
-                    }
-                    ds[k * st] = dist;
-                    ids[k * st] = j;
-                    if (n == kK) {
-                        dlast = ds[(kK - 1) * st];
-                        jlast = ids[(kK - 1) * st];
-                    }
-                }
-            }
-        }
-    }
-    nnb[i] = n;
-    for (int k = 0; k < n; ++k) {
-        nbrT[(size_t)k * n_nodes + i] = ids[k * st];
-    }
-}
-
 // v2m_force, src/opencl/v2m_particle_body.cl:161-184: the repulsive bars of node i,
 // summed in list order (the fixed order that makes the sum reproducible)
 __device__ __forceinline__ float4 bar_force(const float4* __restrict__ X, const uint8_t* __restrict__ typ,
@@ -516,13 +512,13 @@ __device__ __forceinline__ float4 bar_force(const float4* __restrict__ X, const 
 // ---- the trapped move --------------------------------------------------------------------
 
 // v2m_crossing, src/opencl/v2m_particle_body.cl:188-212
-__device__ __noinline__ P3 segment_crossing(const Field& f, int a, int b, P3 p, P3 q) {
+__device__ __forceinline__ P3 segment_crossing_at(const Field& f, int a, int b, P3 p, P3 q, PsiCell& pc) {
     float t0 = 0.0f, t1 = 1.0f;
-    float v0 = psi_point(f, a, b, p);
-    float v1 = psi_point(f, a, b, q);
+    float v0 = psi_at(f, a, b, p, pc);
+    float v1 = psi_at(f, a, b, q, pc);
     for (int it = 0; it < 6; ++it) {
         const float tm = 0.5f * (t0 + t1);
-        const float vm = psi_point(f, a, b, p3(p.x + tm * (q.x - p.x), p.y + tm * (q.y - p.y), p.z + tm * (q.z - p.z)));
+        const float vm = psi_at(f, a, b, p3(p.x + tm * (q.x - p.x), p.y + tm * (q.y - p.y), p.z + tm * (q.z - p.z)), pc);
         if (vm > 0.0f) {
             t0 = tm;
             v0 = vm;
@@ -536,8 +532,8 @@ __device__ __noinline__ P3 segment_crossing(const Field& f, int a, int b, P3 p, 
 }
 
 // v2m_tangential, src/opencl/v2m_particle_body.cl:215-229: v less its part along n_ab at p
-__device__ __noinline__ P3 drop_normal(const Field& f, int a, int b, P3 p, P3 v) {
-    const float4 g = psi_grad_point(f, a, b, p);
+__device__ __forceinline__ P3 drop_normal_at(const Field& f, int a, int b, P3 p, P3 v, PsiCell& pc) {
+    const float4 g = psi_grad_at(f, a, b, p, pc);
     const float gn2 = g.x * g.x + g.y * g.y + g.z * g.z;
     if (gn2 < 1e-12f) {
         return v;
@@ -546,17 +542,9 @@ __device__ __noinline__ P3 drop_normal(const Field& f, int a, int b, P3 p, P3 v)
     return p3(v.x - vn * g.x, v.y - vn * g.y, v.z - vn * g.z);
 }
 
-// v2m_third_label, src/opencl/v2m_particle_body.cl:232-260
-__device__ __noinline__ int competing_label(const Field& f, int a, int b, P3 q) {
-    const BrickLabels bl = brick_labels_at(f, q);
-    int other = 0;
-    for (int s = 0; s < bl.n; ++s) {
-        const int l = brick_label(f, bl, s);
-        other |= l != a && l != b;
-    }
-    if (!other) {
-        return kNoLab;
-    }
+// v2m_third_label, src/opencl/v2m_particle_body.cl:232-260. The brick test runs
+// inline; the field comparisons (a third label is near: rare) are a call.
+__device__ __noinline__ int competing_label_near(const Field& f, int a, int b, P3 q, BrickLabels bl) {
     const float pa = phi_point(f, a, q);
     int c = kNoLab;
     float pc = 0.5f * pa;
@@ -573,17 +561,21 @@ __device__ __noinline__ int competing_label(const Field& f, int a, int b, P3 q) 
     return c;
 }
 
-// v2m_valid_on, src/opencl/v2m_particle_body.cl:266-291
-__device__ __noinline__ int labels_dominate(const Field& f, int a, int b, int c, P3 q) {
+__device__ __forceinline__ int competing_label(const Field& f, int a, int b, P3 q) {
     const BrickLabels bl = brick_labels_at(f, q);
     int other = 0;
     for (int s = 0; s < bl.n; ++s) {
         const int l = brick_label(f, bl, s);
-        other |= l != a && l != b && l != c;
+        other |= l != a && l != b;
     }
     if (!other) {
-        return 1;
+        return kNoLab;
     }
+    return competing_label_near(f, a, b, q, bl);
+}
+
+// v2m_valid_on, src/opencl/v2m_particle_body.cl:266-291 (the same split)
+__device__ __noinline__ int labels_dominate_near(const Field& f, int a, int b, int c, P3 q, BrickLabels bl) {
     float mine = fminf(phi_point(f, a, q), phi_point(f, b, q));
     if (c != kNoLab) {
         mine = fminf(mine, phi_point(f, c, q));
@@ -595,6 +587,19 @@ __device__ __noinline__ int labels_dominate(const Field& f, int a, int b, int c,
         }
     }
     return 1;
+}
+
+__device__ __forceinline__ int labels_dominate(const Field& f, int a, int b, int c, P3 q) {
+    const BrickLabels bl = brick_labels_at(f, q);
+    int other = 0;
+    for (int s = 0; s < bl.n; ++s) {
+        const int l = brick_label(f, bl, s);
+        other |= l != a && l != b && l != c;
+    }
+    if (!other) {
+        return 1;
+    }
+    return labels_dominate_near(f, a, b, c, q, bl);
 }
 
 __device__ __forceinline__ float dist2(P3 q, P3 p) {
@@ -612,8 +617,8 @@ struct NodeState {
 // v2m_move_step (smooth mode), src/opencl/v2m_particle_body.cl:659-833: apply the
 // capped step s with interface trapping. Returns |dp| / h (0: the node stayed;
 // its type and partners may still have changed, as in the reference).
-__device__ __noinline__ float trapped_step(const Field& f, float h, P3 s, float maxstep, float snap, int a,
-                                           NodeState& ns) {
+__device__ __forceinline__ float trapped_step(const Field& f, float h, P3 s, float maxstep, float snap, int a,
+                                              NodeState& ns) {
     const P3 p = ns.p;
     P3 q;
     const int typ0 = ns.ty;   // typ[i] as stored until the final write
@@ -627,19 +632,23 @@ __device__ __noinline__ float trapped_step(const Field& f, float h, P3 s, float 
         const int l = own.x, sec = own.y;
         if (l != a) {
             b = l;
-            const P3 x = segment_crossing(f, a, b, p, q);
-            const P3 r = drop_normal(f, a, b, x, p3(q.x - x.x, q.y - x.y, q.z - x.z));
+            PsiCell pc;   // crossing, glide and projection stay near the crossing point
+            psi_cell_init(pc);
+            const P3 x = segment_crossing_at(f, a, b, p, q, pc);
+            const P3 r = drop_normal_at(f, a, b, x, p3(q.x - x.x, q.y - x.y, q.z - x.z), pc);
             q = p3(x.x + r.x, x.y + r.y, x.z + r.z);
-            const P3ok pr = onto_interface(f, a, b, q, 0.5f * h);
+            const P3ok pr = onto_interface_at(f, a, b, q, 0.5f * h, pc);
             q = pr.ok ? pr.p : x;
             ty = TN_INTERFACE;
             ns.b = b;
         } else if (sec != kNoLab && snap > 0.0f) {
-            const float4 gq = psi_grad_point(f, a, sec, q);
+            PsiCell pc;
+            psi_cell_init(pc);
+            const float4 gq = psi_grad_at(f, a, sec, q, pc);
             const float v = gq.w;
             const float gn = sqrtf(gq.x * gq.x + gq.y * gq.y + gq.z * gq.z);
             if (gn > 1e-9f && v / gn < snap * h) {
-                const P3ok pr = onto_interface(f, a, sec, q, 1.2f * snap * h);
+                const P3ok pr = onto_interface_at(f, a, sec, q, 1.2f * snap * h, pc);
                 if (pr.ok) {
                     q = pr.p;
                     ty = TN_INTERFACE;
@@ -649,9 +658,11 @@ __device__ __noinline__ float trapped_step(const Field& f, float h, P3 s, float 
             }
         }
     } else if (ty == TN_INTERFACE) {
-        s = drop_normal(f, a, b, p, s);
+        PsiCell pc;   // the glide and its projection probe the same few cells
+        psi_cell_init(pc);
+        s = drop_normal_at(f, a, b, p, s, pc);
         q = p3(p.x + s.x, p.y + s.y, p.z + s.z);
-        const P3ok pr = onto_interface(f, a, b, q, 0.5f * h);
+        const P3ok pr = onto_interface_at(f, a, b, q, 0.5f * h, pc);
         if (!pr.ok) {
             return 0.0f;
         }

@@ -3,12 +3,12 @@
 // Rebuild (src/v2m_particles.cpp:771-820): k_keys (cell key + count per cell),
 // k_scan_* (exclusive scan: cell starts), k_radix_* (stable LSD radix sort of
 // the node ids by cell key: ascending id within a cell, as the CPU's serial
-// counting sort), k_gather (positions in cell order), k_neighbors (K-nearest
-// lists), k_order_* (move order grouped by node type).
+// counting sort), k_gather (positions in cell order), k_truss (K-nearest lists,
+// one wave per 64 nodes), k_order_* (move order grouped by node type).
 // Iteration (src/v2m_particles.cpp:828-947): k_force, k_move, k_stats (histogram,
-// max, Verlet count, FIRE power partial sums), k_refresh (own-list refresh of
-// nodes past a full skin), k_control (one block: FIRE's update, p99, the stop
-// and rebuild decisions, all on the device).
+// max, Verlet count, refresh flags, FIRE power partial sums), k_refresh_list +
+// k_truss<true> (own-list refresh of the nodes past a full skin), k_control (one
+// block: FIRE's update, p99, the stop and rebuild decisions, all on the device).
 #pragma once
 
 #include "tn_device.hpp"
@@ -19,7 +19,6 @@ using namespace tnd;
 
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
-constexpr int kNbrBlock = 128;   // K-nearest search: 128 lanes x 24 x 8 B of LDS K-lists = 24 KB
 
 // The device control block: per-iteration accumulators, FIRE's state and the
 // loop decisions. The host reads its head after each iteration.
@@ -36,6 +35,7 @@ struct Ctl {
     int32_t power_fallbacks;   // iterations whose power sign needed the serial sum
     int32_t nrefresh;      // nodes queued for an own-list refresh this iteration
     int32_t nhalf;         // nodes past skin/2 (this iteration)
+    int32_t nmove;         // nodes queued for the full trapped step (this iteration)
     // FIRE (FireCtl, src/v2m_particles.h:47-76)
     float dt0, dt, dtmax, alpha;
     int32_t npos;
@@ -264,32 +264,396 @@ __global__ __launch_bounds__(kBlock) void k_gather(const int32_t* __restrict__ s
     }
 }
 
-// ---- K-nearest lists ----------------------------------------------------------------
+// ---- K-nearest lists, one wave per 64 nodes in cell order ----------------------------
+//
+// v2m_neighbors (src/opencl/v2m_particle_body.cl:73-151) for 64 query nodes at once.
+// The queries are consecutive in cell order, so their scan boxes overlap: per hash
+// level the wave scans the union of its lanes' boxes (rows no lane reaches are
+// skipped), loading each row's candidates once, coalesced, into LDS, and every
+// lane tests every candidate against its own query. Valid candidates (within
+// reach, level within Li +- 1, no interior bar across labels) are appended to the
+// lane's LDS buffer, which is cut back to its 24 best whenever it fills; at the
+// end each lane sorts its buffer by (distance, id) and keeps the first 24. The
+// list equals the reference's: its K smallest valid candidates in (distance, id)
+// order, and every valid candidate lies inside the lane's own box, which the
+// union contains. (That holds when every node's reach fits its level's cell,
+// H.prune; otherwise a lane also checks that the candidate's cell is in its own
+// box, the reference's scan region, before accepting it.)
 
-// nodes visited in cell order: a wave's lanes scan the same cells
-__global__ __launch_bounds__(kNbrBlock) void k_neighbors(Hash H, TrussIn in, int n, int32_t* __restrict__ nbrT,
-                                                         int32_t* __restrict__ nnb) {
-    __shared__ float ds[kK * kNbrBlock];
-    __shared__ int ids[kK * kNbrBlock];
-    const int s = blockIdx.x * kNbrBlock + threadIdx.x;
-    if (s >= n) {
-        return;
+constexpr int kTrussWave = 64;   // one wave per block
+#ifndef TN_TRUSS_COLLECT
+#define TN_TRUSS_COLLECT 32
+#endif
+#ifndef TN_TRUSS_SPAN
+#define TN_TRUSS_SPAN 16
+#endif
+constexpr int kCollect = TN_TRUSS_COLLECT;   // per-lane candidate buffer (entries)
+constexpr int kGroupSpan = TN_TRUSS_SPAN;    // cells along a row that one group of lanes may span
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        v = min(v, __shfl_xor(v, d, kWave));
     }
-    truss_of(H, in, in.sorted[s], n, ds + threadIdx.x, ids + threadIdx.x, kNbrBlock, nbrT, nnb);
+    return v;
 }
 
-// own-list refresh of the nodes queued by k_stats (v2m_particles.cpp:925-931):
-// stale cells and Ps, current positions and types; then P0 = P for them
-__global__ __launch_bounds__(kNbrBlock) void k_refresh(Hash H, TrussIn in, int n, const int32_t* __restrict__ queue,
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        v = max(v, __shfl_xor(v, d, kWave));
+    }
+    return v;
+}
+
+// (distance, id) order of the K-list (src/opencl/v2m_particle_body.cl:128-141)
+__device__ __forceinline__ bool before(float da, int ja, float db, int jb) {
+    return da < db || (da == db && ja < jb);
+}
+
+// insertion sort of a lane's buffer (entries e*64 + lane)
+__device__ __forceinline__ void sort_lane(float* cd, int* cj, int lane, int cnt) {
+    for (int e = 1; e < cnt; ++e) {
+        const float d = cd[e * kTrussWave + lane];
+        const int j = cj[e * kTrussWave + lane];
+        int k = e;
+        while (k > 0) {
+            const float dp = cd[(k - 1) * kTrussWave + lane];
+            const int jp = cj[(k - 1) * kTrussWave + lane];
+            if (!before(d, j, dp, jp)) {
+                break;
+            }
+            cd[k * kTrussWave + lane] = dp;
+            cj[k * kTrussWave + lane] = jp;
+            --k;
+        }
+        cd[k * kTrussWave + lane] = d;
+        cj[k * kTrussWave + lane] = j;
+    }
+}
+
+// The lists of all nodes at a rebuild: query q is the node at sorted position q.
+__global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n, int32_t* __restrict__ nbrT,
+                                                      int32_t* __restrict__ nnb) {
+    __shared__ float4 cpos[kTrussWave];
+    __shared__ int2 cmeta[kTrussWave];
+    __shared__ float cd[kCollect * kTrussWave];
+    __shared__ int cj[kCollect * kTrussWave];
+    const int lane = threadIdx.x;
+    const int m = n;
+    const float tskin = in.t + in.skin;
+    for (int qb = blockIdx.x * kTrussWave; qb < m; qb += gridDim.x * kTrussWave) {
+        const int q = qb + lane;
+        const bool active = q < m;
+        int i = 0;
+        float px = 0.0f, py = 0.0f, pz = 0.0f, hi = 0.0f, Ri = 0.0f;
+        int Li = 0, ti_int = 0, li = 0;
+        if (active) {
+            i = in.sorted[q];
+            const float4 xi = in.X[i];
+            px = xi.x;
+            py = xi.y;
+            pz = xi.z;
+            hi = xi.w;
+            Ri = tskin * hi;
+            Li = level_for(H, Ri);
+            ti_int = in.typ[i] == TN_INTERIOR;
+            li = in.lab[i];
+        }
+        // The lanes are served in groups that share a cell row of their own level and lie
+        // within kGroupSpan cells of the group's first lane along it, so that a group's
+        // union of scan boxes stays tight (a wave of queries can straddle rows, and a
+        // refresh wave holds scattered nodes).
+        int grow = -1, gx = 0;
+        if (active) {
+            const int cy = cell_coord(H, Li, py, H.oy, 1), cz = cell_coord(H, Li, pz, H.oz, 2);
+            grow = H.off[Li] + H.dim[Li][0] * (cy + H.dim[Li][1] * cz);
+            gx = cell_coord(H, Li, px, H.ox, 0);
+        }
+        bool pending = active;
+        int cnt = 0;
+        for (;;) {
+        const uint64_t pm = __ballot(pending);
+        if (!pm) {
+            break;
+        }
+        const int lead = __ffsll((long long)pm) - 1;
+        const int lrow = __shfl(grow, lead, kWave), lx = __shfl(gx, lead, kWave);
+        const bool ingroup = pending && grow == lrow && abs(gx - lx) <= kGroupSpan;
+        pending = pending && !ingroup;
+        const int lmin = wave_min(ingroup ? max(Li - 1, 0) : INT_MAX);
+        const int lmax = wave_max(ingroup ? min(Li + 1, H.nlev - 1) : -1);
+        for (int L = lmin; L <= lmax; ++L) {
+            const bool mine = ingroup && L >= Li - 1 && L <= Li + 1;
+            const float bL = H.b0 * (float)(1 << L);
+            const float r = 0.5f * (Ri + bL);
+            int x0 = INT_MAX, x1 = -1, y0 = INT_MAX, y1 = -1, z0 = INT_MAX, z1 = -1;
+            if (mine) {
+                x0 = cell_coord(H, L, px - r, H.ox, 0);
+                x1 = cell_coord(H, L, px + r, H.ox, 0);
+                y0 = cell_coord(H, L, py - r, H.oy, 1);
+                y1 = cell_coord(H, L, py + r, H.oy, 1);
+                z0 = cell_coord(H, L, pz - r, H.oz, 2);
+                z1 = cell_coord(H, L, pz + r, H.oz, 2);
+            }
+            const int Y0 = wave_min(y0), Y1 = wave_max(y1), Z0 = wave_min(z0), Z1 = wave_max(z1);
+            // rows beyond reach are skipped (a valid bar is shorter than r; margin for rounding)
+            const float rw = r * 1.0001f + 1e-3f * bL + 2e-6f * (fabsf(px) + fabsf(py) + fabsf(pz));
+            const float rr = rw * rw;
+            const int ny1 = H.dim[L][1] - 1, nz1 = H.dim[L][2] - 1;
+            for (int cz = Z0; cz <= Z1; ++cz) {
+                const float zlo = cz == 0 ? -__builtin_huge_valf() : H.oz + (float)cz * bL;
+                const float zhi = cz == nz1 ? __builtin_huge_valf() : H.oz + (float)(cz + 1) * bL;
+                const float dz = fmaxf(0.0f, fmaxf(zlo - pz, pz - zhi));
+                for (int cy = Y0; cy <= Y1; ++cy) {
+                    const float ylo = cy == 0 ? -__builtin_huge_valf() : H.oy + (float)cy * bL;
+                    const float yhi = cy == ny1 ? __builtin_huge_valf() : H.oy + (float)(cy + 1) * bL;
+                    const float dy = fmaxf(0.0f, fmaxf(ylo - py, py - yhi));
+                    int xa = INT_MAX, xb = -1;
+                    if (mine && cz >= z0 && cz <= z1 && cy >= y0 && cy <= y1) {
+                        if (!H.prune) {
+                            xa = x0;
+                            xb = x1;
+                        } else {
+                            const float rem = rr - dz * dz - dy * dy;
+                            if (rem >= 0.0f) {
+                                const float w = sqrtf(rem);
+                                xa = max(x0, cell_coord(H, L, px - w, H.ox, 0));
+                                xb = min(x1, cell_coord(H, L, px + w, H.ox, 0));
+                            }
+                        }
+                    }
+                    const int XA = wave_min(xa), XB = wave_max(xb);
+                    if (XA > XB) {
+                        continue;
+                    }
+                    const int krow = H.off[L] + H.dim[L][0] * (cy + H.dim[L][1] * cz);
+                    const int sa = in.cstart[krow + XA], sb = in.cstart[krow + XB + 1];
+                    for (int base = sa; base < sb; base += kTrussWave) {
+                        __syncthreads();   // the previous chunk is consumed
+                        const int s = base + lane;
+                        if (s < sb) {
+                            const int j = in.sorted[s];
+                            cpos[lane] = in.Ps[s];
+                            cmeta[lane] = make_int2(j, (int)in.typ[j] | ((int)in.lab[j] << 8));
+                        }
+                        __syncthreads();
+                        const int cn = min(kTrussWave, sb - base);
+                        if (!mine) {
+                            continue;
+                        }
+                        const bool row_own = cz >= z0 && cz <= z1 && cy >= y0 && cy <= y1;
+                        for (int k = 0; k < cn; ++k) {
+                            const float4 c = cpos[k];
+                            const float ex = c.x - px, ey = c.y - py, ez = c.z - pz;
+                            const float d2 = ex * ex + ey * ey + ez * ez;
+                            const float rj = tskin * 0.5f * (hi + c.w);
+                            if (d2 >= rj * rj) {
+                                continue;
+                            }
+                            if (!H.prune) {   // the reference's own scan region only
+                                const int cx = cell_coord(H, L, c.x, H.ox, 0);
+                                if (!row_own || cx < x0 || cx > x1) {
+                                    continue;
+                                }
+                            }
+                            const int2 mt = cmeta[k];
+                            if (mt.x == i || (ti_int && (mt.y & 0xFF) == TN_INTERIOR && (mt.y >> 8) != li)) {
+                                continue;
+                            }
+                            if (cnt == kCollect) {   // keep the 24 best so far
+                                sort_lane(cd, cj, lane, cnt);
+                                cnt = kK;
+                            }
+                            cd[cnt * kTrussWave + lane] = sqrtf(d2);
+                            cj[cnt * kTrussWave + lane] = mt.x;
+                            ++cnt;
+                        }
+                    }
+                }
+            }
+        }
+        }   // the groups
+        if (active) {
+            sort_lane(cd, cj, lane, cnt);
+            const int kk = cnt < kK ? cnt : kK;
+            nnb[i] = kk;
+            for (int k = 0; k < kk; ++k) {
+                nbrT[(size_t)k * n + i] = cj[k * kTrussWave + lane];
+            }
+        }
+        __syncthreads();   // the buffers are reused by the next block of queries
+    }
+}
+
+// ---- own-list refresh: one wave per node --------------------------------------------
+//
+// The nodes past a full skin refresh their own list every iteration
+// (src/v2m_particles.cpp:925-931: the current position and size against the cells
+// and candidate positions of the last rebuild; then P0 = P). They are few (about
+// 0.1% of the nodes) and scattered, so each gets a whole wave: the lanes test the
+// node's candidates 64 at a time with coalesced loads, valid ones are appended to
+// the wave's list in LDS, and the K best in (distance, id) order are picked by rank
+// (the order is total, so ranks are distinct). The scan region and the row pruning
+// are k_truss's for a single lane.
+
+constexpr int kRefreshCap = 256;   // LDS list entries (cut back to the best K when full)
+
+// Keep the K best of the cnt entries of (cd, cj), in order, at [0, min(cnt, K)).
+__device__ __forceinline__ int keep_best(float* cd, int* cj, int cnt, int lane) {
+    constexpr int kPer = kRefreshCap / kWave;
+    float d[kPer];
+    int j[kPer], rank[kPer];
+#pragma unroll
+    for (int t = 0; t < kPer; ++t) {
+        const int e = lane + t * kWave;
+        d[t] = e < cnt ? cd[e] : 0.0f;
+        j[t] = e < cnt ? cj[e] : 0;
+        rank[t] = 0;
+    }
+    for (int e = 0; e < cnt; ++e) {
+        const float de = cd[e];
+        const int je = cj[e];
+#pragma unroll
+        for (int t = 0; t < kPer; ++t) {
+            rank[t] += before(de, je, d[t], j[t]) ? 1 : 0;
+        }
+    }
+    __syncthreads();   // every entry is read before any is moved
+#pragma unroll
+    for (int t = 0; t < kPer; ++t) {
+        if (lane + t * kWave < cnt && rank[t] < kK) {
+            cd[rank[t]] = d[t];
+            cj[rank[t]] = j[t];
+        }
+    }
+    __syncthreads();
+    return cnt < kK ? cnt : kK;
+}
+
+__global__ __launch_bounds__(kWave) void k_truss_node(Hash H, TrussIn in, int n, const int32_t* __restrict__ list,
                                                        const Ctl* __restrict__ ctl, int32_t* __restrict__ nbrT,
                                                        int32_t* __restrict__ nnb, float4* __restrict__ P0) {
-    __shared__ float ds[kK * kNbrBlock];
-    __shared__ int ids[kK * kNbrBlock];
+    __shared__ float cd[kRefreshCap];
+    __shared__ int cj[kRefreshCap];
+    const int lane = threadIdx.x;
     const int m = ctl->nrefresh;
-    for (int q = blockIdx.x * kNbrBlock + threadIdx.x; q < m; q += gridDim.x * kNbrBlock) {
-        const int i = queue[q];
-        truss_of(H, in, i, n, ds + threadIdx.x, ids + threadIdx.x, kNbrBlock, nbrT, nnb);
-        P0[i] = in.X[i];
+    const float tskin = in.t + in.skin;
+    for (int q = blockIdx.x; q < m; q += gridDim.x) {
+        const int i = in.sorted[list[q]];
+        const float4 xi = in.X[i];
+        const float px = xi.x, py = xi.y, pz = xi.z, hi = xi.w;
+        const float Ri = tskin * hi;
+        const int Li = level_for(H, Ri);
+        const int ti_int = in.typ[i] == TN_INTERIOR;
+        const int li = in.lab[i];
+        int cnt = 0;   // the wave's list length (the same in every lane)
+        const int l0 = Li - 1 > 0 ? Li - 1 : 0, l1 = Li + 1 < H.nlev - 1 ? Li + 1 : H.nlev - 1;
+        for (int L = l0; L <= l1; ++L) {
+            const float bL = H.b0 * (float)(1 << L);
+            const float r = 0.5f * (Ri + bL);
+            const int x0 = cell_coord(H, L, px - r, H.ox, 0), x1 = cell_coord(H, L, px + r, H.ox, 0);
+            const int y0 = cell_coord(H, L, py - r, H.oy, 1), y1 = cell_coord(H, L, py + r, H.oy, 1);
+            const int z0 = cell_coord(H, L, pz - r, H.oz, 2), z1 = cell_coord(H, L, pz + r, H.oz, 2);
+            const float rw = r * 1.0001f + 1e-3f * bL + 2e-6f * (fabsf(px) + fabsf(py) + fabsf(pz));
+            const float rr = rw * rw;
+            const int ny1 = H.dim[L][1] - 1, nz1 = H.dim[L][2] - 1;
+            for (int cz = z0; cz <= z1; ++cz) {
+                const float zlo = cz == 0 ? -__builtin_huge_valf() : H.oz + (float)cz * bL;
+                const float zhi = cz == nz1 ? __builtin_huge_valf() : H.oz + (float)(cz + 1) * bL;
+                const float dz = fmaxf(0.0f, fmaxf(zlo - pz, pz - zhi));
+                for (int cy = y0; cy <= y1; ++cy) {
+                    const float ylo = cy == 0 ? -__builtin_huge_valf() : H.oy + (float)cy * bL;
+                    const float yhi = cy == ny1 ? __builtin_huge_valf() : H.oy + (float)(cy + 1) * bL;
+                    const float dy = fmaxf(0.0f, fmaxf(ylo - py, py - yhi));
+                    int xa = x0, xb = x1;
+                    if (H.prune) {
+                        const float rem = rr - dz * dz - dy * dy;
+                        if (rem < 0.0f) {
+                            continue;
+                        }
+                        const float wx = sqrtf(rem);
+                        xa = max(x0, cell_coord(H, L, px - wx, H.ox, 0));
+                        xb = min(x1, cell_coord(H, L, px + wx, H.ox, 0));
+                        if (xa > xb) {
+                            continue;
+                        }
+                    }
+                    const int krow = H.off[L] + H.dim[L][0] * (cy + H.dim[L][1] * cz);
+                    const int sa = in.cstart[krow + xa], sb = in.cstart[krow + xb + 1];
+                    for (int base = sa; base < sb; base += kWave) {
+                        const int s = base + lane;
+                        bool ok = false;
+                        float dist = 0.0f;
+                        int j = 0;
+                        if (s < sb) {
+                            const float4 c = in.Ps[s];
+                            const float ex = c.x - px, ey = c.y - py, ez = c.z - pz;
+                            const float d2 = ex * ex + ey * ey + ez * ez;
+                            const float rj = tskin * 0.5f * (hi + c.w);
+                            if (d2 < rj * rj) {
+                                j = in.sorted[s];
+                                ok = j != i && !(ti_int && in.typ[j] == TN_INTERIOR && in.lab[j] != li);
+                                dist = sqrtf(d2);
+                            }
+                        }
+                        const uint64_t vm = __ballot(ok);
+                        if (vm) {
+                            const int add = __popcll(vm);
+                            if (cnt + add > kRefreshCap) {
+                                cnt = keep_best(cd, cj, cnt, lane);
+                            }
+                            if (ok) {
+                                const int pos = cnt + __popcll(vm & lanemask_lt());
+                                cd[pos] = dist;
+                                cj[pos] = j;
+                            }
+                            cnt += add;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();   // the appends are visible to every lane
+        const int kk = keep_best(cd, cj, cnt, lane);
+        if (lane < kk) {
+            nbrT[(size_t)lane * n + i] = cj[lane];
+        }
+        if (lane == 0) {
+            nnb[i] = kk;
+            P0[i] = xi;
+        }
+        __syncthreads();   // the list is read before the next node reuses it
+    }
+}
+
+// the sorted positions of the nodes flagged for an own-list refresh, appended block
+// by block (in cell order within a block): spatially coherent query waves
+__global__ __launch_bounds__(kBlock) void k_refresh_list(const int32_t* __restrict__ sorted,
+                                                         const uint8_t* __restrict__ flag, int n,
+                                                         Ctl* __restrict__ ctl, int32_t* __restrict__ list) {
+    __shared__ int woff[kBlock / kWave];
+    __shared__ int base;
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
+    const bool f = s < n && flag[sorted[s]];
+    const uint64_t mask = __ballot(f);
+    if (lane == 0) {
+        woff[w] = __popcll(mask);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int k = 0; k < kBlock / kWave; ++k) {
+            const int c = woff[k];
+            woff[k] = tot;
+            tot += c;
+        }
+        base = tot ? atomicAdd(&ctl->nrefresh, tot) : 0;
+    }
+    __syncthreads();
+    if (f) {
+        list[base + woff[w] + __popcll(mask & lanemask_lt())] = s;
     }
 }
 
@@ -299,6 +663,14 @@ __device__ __forceinline__ int type_rank(int t) {
     return t == TN_INTERFACE ? 0 : (t == TN_JUNCTION ? 1 : (t == TN_INTERIOR ? 2 : 3));
 }
 
+// lanes of this wave whose node has type rank r (4 ballots)
+__device__ __forceinline__ void rank_masks(bool valid, int r, uint64_t* m) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        m[k] = __ballot(valid && r == k);
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_order_count(const uint8_t* __restrict__ typ, int n, Ctl* __restrict__ ctl) {
     __shared__ int c[4];
     if (threadIdx.x < 4) {
@@ -306,8 +678,16 @@ __global__ __launch_bounds__(kBlock) void k_order_count(const uint8_t* __restric
     }
     __syncthreads();
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) {
-        atomicAdd(&c[type_rank(typ[i] & 3)], 1);
+    const bool valid = i < n;
+    uint64_t m[4];
+    rank_masks(valid, valid ? type_rank(typ[i] & 3) : 0, m);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (m[k]) {
+                atomicAdd(&c[k], __popcll(m[k]));
+            }
+        }
     }
     __syncthreads();
     if (threadIdx.x < 4 && c[threadIdx.x]) {
@@ -320,15 +700,22 @@ __global__ __launch_bounds__(kBlock) void k_order_count(const uint8_t* __restric
 __global__ __launch_bounds__(kBlock) void k_order_scatter(const uint8_t* __restrict__ typ, int n,
                                                           Ctl* __restrict__ ctl, int32_t* __restrict__ order) {
     __shared__ int c[4], b[4];
+    __shared__ int woff[kBlock / kWave][4];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
     if (threadIdx.x < 4) {
         c[threadIdx.x] = 0;
     }
     __syncthreads();
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    int r = 0, loc = 0;
-    if (i < n) {
-        r = type_rank(typ[i] & 3);
-        loc = atomicAdd(&c[r], 1);
+    const bool valid = i < n;
+    const int r = valid ? type_rank(typ[i] & 3) : 0;
+    uint64_t m[4];
+    rank_masks(valid, r, m);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            woff[w][k] = m[k] ? atomicAdd(&c[k], __popcll(m[k])) : 0;
+        }
     }
     __syncthreads();
     if (threadIdx.x < 4) {
@@ -339,8 +726,13 @@ __global__ __launch_bounds__(kBlock) void k_order_scatter(const uint8_t* __restr
         b[threadIdx.x] = c[threadIdx.x] ? start + atomicAdd(&ctl->order_cur[threadIdx.x], c[threadIdx.x]) : 0;
     }
     __syncthreads();
-    if (i < n) {
-        order[b[r] + loc] = i;
+    if (valid) {
+        uint64_t mr = m[0];
+#pragma unroll
+        for (int k = 1; k < 4; ++k) {
+            mr = r == k ? m[k] : mr;
+        }
+        order[b[r] + woff[w][r] + __popcll(mr & lanemask_lt())] = i;
     }
 }
 
@@ -366,95 +758,215 @@ __device__ __forceinline__ uint32_t pack_part(int b, int c) {
     return (uint32_t)(b & 0xFFFF) | ((uint32_t)(c & 0xFFFF) << 16);
 }
 
-__global__ __launch_bounds__(kBlock) void k_move(Field f, MoveArgs ma, const Ctl* __restrict__ ctl,
-                                                 const int32_t* __restrict__ order, int n,
-                                                 const float4* __restrict__ F, float4* __restrict__ X,
-                                                 float4* __restrict__ V, uint8_t* __restrict__ typ,
-                                                 uint32_t* __restrict__ part, const uint16_t* __restrict__ lab,
-                                                 float* __restrict__ mv, float* __restrict__ pw) {
-    const int g = blockIdx.x * kBlock + threadIdx.x;
-    if (g >= n) {
-        return;
-    }
-    const int i = order[g];
-    const float fdt = ctl->dt, falpha = ctl->alpha;
-    const int fire = ma.fire;
-    const float4 x = X[i];
-    const float h = x.w;   // h at the node: v2m_h_at(P_i), kept current by every move
-    const int ty0 = typ[i];
-    float p_w = 0.0f;
-    float4 vout = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float m = 0.0f;
-    if (ty0 != TN_CORNER) {
-        const float4 Fi = F[i];
-        const float stiff = Fi.w > 1.0f ? Fi.w : 1.0f;
-        const float ac[3] = { Fi.x / stiff, Fi.y / stiff, Fi.z / stiff };
-        float s[3];
-        if (fire) {
-            float v[3] = { 0.0f, 0.0f, 0.0f };
-            if (!ctl->zero_v) {
-                const float4 v4 = V[i];
-                v[0] = v4.x;
-                v[1] = v4.y;
-                v[2] = v4.z;
-            }
-            const float pv = ac[0] * v[0] + ac[1] * v[1] + ac[2] * v[2];
-            p_w = pv;
-            if (pv < 0.0f) {
-                v[0] = v[1] = v[2] = 0.0f;
-            } else {
-                const float vn = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-                const float an = sqrtf(ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2]);
-                if (an > 0.0f) {
-                    for (int k = 0; k < 3; ++k) {
-                        v[k] = (1.0f - falpha) * v[k] + falpha * vn * ac[k] / an;
-                    }
-                }
-            }
-            for (int k = 0; k < 3; ++k) {
-                v[k] += fdt * ac[k];
-                s[k] = fdt * v[k];
-            }
+// The FIRE / Jacobi part of v2m_move (src/opencl/v2m_particle_body.cl:853-912) for a
+// node that is not a corner: the step s, capped at maxstep h, the power a . v, and
+// whether the node moves at all (a settled node, |s| < 5e-4 h, stays put).
+struct NodeStep {
+    float s[3];
+    float pw;
+    bool moves;
+};
+
+__device__ __forceinline__ NodeStep node_step(const MoveArgs& ma, const Ctl* __restrict__ ctl, float4 Fi,
+                                              const float4* __restrict__ V, int i, float h) {
+    NodeStep r;
+    r.pw = 0.0f;
+    const float stiff = Fi.w > 1.0f ? Fi.w : 1.0f;
+    const float ac[3] = { Fi.x / stiff, Fi.y / stiff, Fi.z / stiff };
+    if (ma.fire) {
+        const float fdt = ctl->dt, falpha = ctl->alpha;
+        float v[3] = { 0.0f, 0.0f, 0.0f };
+        if (!ctl->zero_v) {
+            const float4 v4 = V[i];
+            v[0] = v4.x;
+            v[1] = v4.y;
+            v[2] = v4.z;
+        }
+        const float pv = ac[0] * v[0] + ac[1] * v[1] + ac[2] * v[2];
+        r.pw = pv;
+        if (pv < 0.0f) {
+            v[0] = v[1] = v[2] = 0.0f;
         } else {
-            for (int k = 0; k < 3; ++k) {
-                s[k] = ma.dt * ac[k];
-            }
-        }
-        const float sl = sqrtf(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
-        if (!(sl < 5e-4f * h)) {
-            if (sl > ma.maxstep * h) {
+            const float vn = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            const float an = sqrtf(ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2]);
+            if (an > 0.0f) {
                 for (int k = 0; k < 3; ++k) {
-                    s[k] *= ma.maxstep * h / sl;
+                    v[k] = (1.0f - falpha) * v[k] + falpha * vn * ac[k] / an;
                 }
             }
-            const uint32_t pp = part[i];
-            NodeState ns;
-            ns.p = p3(x.x, x.y, x.z);
-            ns.hn = h;
-            ns.ty = ty0;
-            ns.b = (int)(pp & 0xFFFF);
-            ns.c = (int)(pp >> 16);
-            m = trapped_step(f, h, p3(s[0], s[1], s[2]), ma.maxstep, ma.snap, lab[i], ns);
-            if (m > 0.0f) {
-                X[i] = make_float4(ns.p.x, ns.p.y, ns.p.z, ns.hn);
-            }
-            if (ns.ty != ty0) {
-                typ[i] = (uint8_t)ns.ty;
-            }
-            const uint32_t np = pack_part(ns.b, ns.c);
-            if (np != pp) {
-                part[i] = np;
-            }
-            if (fire && m > 0.0f && ns.ty == ty0) {
-                vout = make_float4((ns.p.x - x.x) / fdt, (ns.p.y - x.y) / fdt, (ns.p.z - x.z) / fdt, 0.0f);
-            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            v[k] += fdt * ac[k];
+            r.s[k] = fdt * v[k];
+        }
+    } else {
+        for (int k = 0; k < 3; ++k) {
+            r.s[k] = ma.dt * ac[k];
         }
     }
-    if (fire) {
+    const float sl = sqrtf(r.s[0] * r.s[0] + r.s[1] * r.s[1] + r.s[2] * r.s[2]);
+    r.moves = !(sl < 5e-4f * h);
+    if (r.moves && sl > ma.maxstep * h) {
+        for (int k = 0; k < 3; ++k) {
+            r.s[k] *= ma.maxstep * h / sl;
+        }
+    }
+    return r;
+}
+
+// the writes that end v2m_move for node i (src/opencl/v2m_particle_body.cl:921-934)
+__device__ __forceinline__ void move_store(const MoveArgs& ma, const Ctl* __restrict__ ctl, int i, float4 x, int ty0,
+                                           uint32_t pp, const NodeState& ns, float m, float p_w,
+                                           float4* __restrict__ X, float4* __restrict__ V,
+                                           uint8_t* __restrict__ typ, uint32_t* __restrict__ part,
+                                           float* __restrict__ mv, float* __restrict__ pw) {
+    if (m > 0.0f) {
+        X[i] = make_float4(ns.p.x, ns.p.y, ns.p.z, ns.hn);
+    }
+    if (ns.ty != ty0) {
+        typ[i] = (uint8_t)ns.ty;
+    }
+    const uint32_t np = pack_part(ns.b, ns.c);
+    if (np != pp) {
+        part[i] = np;
+    }
+    if (ma.fire) {
+        const float fdt = ctl->dt;
+        float4 vout = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (m > 0.0f && ns.ty == ty0) {
+            vout = make_float4((ns.p.x - x.x) / fdt, (ns.p.y - x.y) / fdt, (ns.p.z - x.z) / fdt, 0.0f);
+        }
         V[i] = vout;
         pw[i] = p_w;
     }
     mv[i] = m;
+}
+
+// Pass 1 of the move, over every node in type order: corners, settled nodes and
+// interior nodes whose step stays in their own region with no interface within
+// snap range are finished here, exactly as v2m_move finishes them. Every other node
+// (interface and junction nodes, interior nodes that cross or may snap) is queued,
+// untouched, for k_move. Most nodes are interior, and this pass needs a fraction
+// of the registers of the full trapped step.
+__global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl* __restrict__ ctl,
+                                                      const int32_t* __restrict__ order, int n,
+                                                      const float4* __restrict__ F, float4* __restrict__ X,
+                                                      float4* __restrict__ V, uint8_t* __restrict__ typ,
+                                                      uint32_t* __restrict__ part, const uint16_t* __restrict__ lab,
+                                                      float* __restrict__ mv, float* __restrict__ pw,
+                                                      int32_t* __restrict__ queue) {
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    int i = 0;
+    bool hard = false;
+    if (g < n) {
+        i = order[g];
+        const int ty0 = typ[i];
+        if (ty0 == TN_INTERFACE || ty0 == TN_JUNCTION) {
+            hard = true;
+        } else if (ty0 == TN_CORNER) {
+            if (ma.fire) {
+                V[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                pw[i] = 0.0f;
+            }
+            mv[i] = 0.0f;
+        } else {
+            const float4 x = X[i];
+            const float h = x.w;
+            const NodeStep st = node_step(ma, ctl, F[i], V, i, h);
+            if (!st.moves) {
+                if (ma.fire) {
+                    V[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    pw[i] = st.pw;
+                }
+                mv[i] = 0.0f;
+            } else {
+                // trapped_step's interior branch up to its first decision
+                const int a = lab[i];
+                const P3 p = p3(x.x, x.y, x.z);
+                const P3 q = p3(p.x + st.s[0], p.y + st.s[1], p.z + st.s[2]);
+                const int2 own = owner_at(f, q);
+                if (own.x != a) {
+                    hard = true;
+                } else if (own.y != kNoLab && ma.snap > 0.0f) {
+                    PsiCell pc;
+                    psi_cell_init(pc);
+                    const float4 gq = psi_grad_at(f, a, own.y, q, pc);
+                    const float gn = sqrtf(gq.x * gq.x + gq.y * gq.y + gq.z * gq.z);
+                    hard = gn > 1e-9f && gq.w / gn < ma.snap * h;
+                }
+                if (!hard) {   // the rest of trapped_step for an interior node that stays interior
+                    const uint32_t pp = part[i];
+                    NodeState ns;
+                    ns.p = p;
+                    ns.hn = h;
+                    ns.ty = TN_INTERIOR;
+                    ns.b = (int)(pp & 0xFFFF);
+                    ns.c = (int)(pp >> 16);
+                    float m = 0.0f;
+                    const float lim = (ma.maxstep + fmaxf(ma.snap, 0.5f)) * h;
+                    const float m2 = dist2(q, p);
+                    if (!(m2 <= lim * lim)) {
+                        ns.b = kNoLab;
+                    } else {
+                        ns.p = q;
+                        ns.hn = size_at(f, q.x, q.y, q.z);
+                        m = sqrtf(m2) / h;
+                    }
+                    move_store(ma, ctl, i, x, TN_INTERIOR, pp, ns, m, st.pw, X, V, typ, part, mv, pw);
+                }
+            }
+        }
+    }
+    // queue the rest, one atomic per wave
+    const uint64_t hm = __ballot(hard);
+    if (hm) {
+        const int lead = __ffsll((long long)hm) - 1;
+        int base = 0;
+        if ((int)(threadIdx.x & (kWave - 1)) == lead) {
+            base = atomicAdd(&ctl->nmove, __popcll(hm));
+        }
+        base = __shfl(base, lead, kWave);
+        if (hard) {
+            queue[base + __popcll(hm & lanemask_lt())] = i;
+        }
+    }
+}
+
+// waves per SIMD the full move kernel is compiled for (its register budget: 512 / waves)
+#ifndef TN_MOVE_WAVES
+#define TN_MOVE_WAVES 1
+#endif
+
+// Pass 2: v2m_move (src/opencl/v2m_particle_body.cl:844-935) for the ctl->nmove
+// queued nodes, with the whole trapped step.
+__global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArgs ma, const Ctl* __restrict__ ctl,
+                                                                const int32_t* __restrict__ queue,
+                                                                const float4* __restrict__ F, float4* __restrict__ X,
+                                                                float4* __restrict__ V, uint8_t* __restrict__ typ,
+                                                                uint32_t* __restrict__ part,
+                                                                const uint16_t* __restrict__ lab,
+                                                                float* __restrict__ mv, float* __restrict__ pw) {
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= ctl->nmove) {
+        return;
+    }
+    const int i = queue[g];
+    const float4 x = X[i];
+    const float h = x.w;   // h at the node: v2m_h_at(P_i), kept current by every move
+    const int ty0 = typ[i];
+    const uint32_t pp = part[i];
+    NodeState ns;
+    ns.p = p3(x.x, x.y, x.z);
+    ns.hn = h;
+    ns.ty = ty0;
+    ns.b = (int)(pp & 0xFFFF);
+    ns.c = (int)(pp >> 16);
+    float m = 0.0f;
+    const NodeStep st = node_step(ma, ctl, F[i], V, i, h);   // queued nodes are never corners
+    if (st.moves) {
+        m = trapped_step(f, h, p3(st.s[0], st.s[1], st.s[2]), ma.maxstep, ma.snap, lab[i], ns);
+    }
+    move_store(ma, ctl, i, x, ty0, pp, ns, m, st.pw, X, V, typ, part, mv, pw);
 }
 
 // ---- per-iteration reductions + Verlet bookkeeping ----------------------------------
@@ -479,16 +991,20 @@ __device__ __forceinline__ int hist_bin(float m, const float* __restrict__ thr) 
 
 constexpr int kHistThr = 35;   // exponents -21..13
 
+// Per iteration: the histogram, the largest move, the count past skin/2, the
+// flags of the nodes past a full skin (their lists are refreshed), and FIRE's
+// power partial sums (src/v2m_particles.cpp:849-937). Counts are reduced per
+// wave with ballots before any atomic.
 __global__ __launch_bounds__(kBlock) void k_stats(const float4* __restrict__ X, const float4* __restrict__ P0,
                                                   const float* __restrict__ mv, const float* __restrict__ pw, int n,
                                                   float skin, int fire, const float* __restrict__ thr,
-                                                  Ctl* __restrict__ ctl, int32_t* __restrict__ queue,
+                                                  Ctl* __restrict__ ctl, uint8_t* __restrict__ refresh,
                                                   double2* __restrict__ partials) {
     __shared__ uint32_t lh[32];
     __shared__ uint32_t lmax;
     __shared__ int lhalf;
     __shared__ double lps[kBlock / kWave][2];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1);
     if (tid < 32) {
         lh[tid] = 0;
     }
@@ -499,22 +1015,44 @@ __global__ __launch_bounds__(kBlock) void k_stats(const float4* __restrict__ X, 
     __syncthreads();
     const int i = blockIdx.x * kBlock + tid;
     double ps = 0.0, pa = 0.0;
+    int bin = -1;
+    uint32_t mbits = 0;
+    bool half = false;
     if (i < n) {
         const float m = mv[i];
-        atomicAdd(&lh[hist_bin(m, thr)], 1u);
-        atomicMax(&lmax, __float_as_uint(m));
+        bin = hist_bin(m, thr);
+        mbits = __float_as_uint(m);
         const float4 x = X[i], x0 = P0[i];
         const float ex = x.x - x0.x, ey = x.y - x0.y, ez = x.z - x0.z;
         const float h = x.w;
         const float m2 = ex * ex + ey * ey + ez * ez, s2 = skin * skin * h * h;
-        if (m2 > s2) {
-            queue[atomicAdd(&ctl->nrefresh, 1)] = i;
-        } else if (m2 > 0.25f * s2) {
-            atomicAdd(&lhalf, 1);
-        }
+        const bool far = m2 > s2;
+        refresh[i] = far ? 1 : 0;
+        half = !far && m2 > 0.25f * s2;
         if (fire) {
             ps = (double)pw[i];
             pa = fabs(ps);
+        }
+    }
+    // histogram: one LDS add per (wave, occupied bin)
+    uint64_t left = __ballot(bin >= 0);
+    while (left) {
+        const int b = __shfl(bin, __ffsll((long long)left) - 1, kWave);
+        const uint64_t same = __ballot(bin == b);
+        if (lane == 0) {
+            atomicAdd(&lh[b], (uint32_t)__popcll(same));
+        }
+        left &= ~same;
+    }
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        mbits = max(mbits, (uint32_t)__shfl_xor((int)mbits, d, kWave));
+    }
+    const int nh = __popcll(__ballot(half));
+    if (lane == 0) {
+        atomicMax(&lmax, mbits);
+        if (nh) {
+            atomicAdd(&lhalf, nh);
         }
     }
     if (fire) {   // fixed-shape tree: the partial sums are reproducible
@@ -523,7 +1061,7 @@ __global__ __launch_bounds__(kBlock) void k_stats(const float4* __restrict__ X, 
             ps += __shfl_xor(ps, d, kWave);
             pa += __shfl_xor(pa, d, kWave);
         }
-        if ((tid & (kWave - 1)) == 0) {
+        if (lane == 0) {
             lps[tid >> 6][0] = ps;
             lps[tid >> 6][1] = pa;
         }
@@ -661,11 +1199,12 @@ __global__ __launch_bounds__(kBlock) void k_control(CtlArgs ca, Ctl* __restrict_
     ctl->mmax_bits = 0;
 }
 
-// clears the per-iteration counters that k_stats and k_refresh consume
+// clears the per-iteration counters that k_stats and k_refresh_list fill
 __global__ void k_iter_begin(Ctl* __restrict__ ctl) {
     if (threadIdx.x == 0) {
         ctl->nhalf = 0;
         ctl->nrefresh = 0;
+        ctl->nmove = 0;
     }
 }
 

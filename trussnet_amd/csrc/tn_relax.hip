@@ -90,6 +90,14 @@ tnd::Hash make_hash(const tn_grid_view* g, const tn_relax_params* prm, int64_t* 
     }
     H.nkeys = (int)nkeys;
     *nkeys_out = nkeys;
+    // the K-nearest search may skip cells beyond a node's reach only if every node's
+    // reach (t + skin) h fits the cell of its level, i.e. the largest h fits the top level
+    const size_t nv = (size_t)g->nx * g->ny * g->nz;
+    float hbig = 0.0f;
+    for (size_t v = 0; v < nv; ++v) {
+        hbig = std::max(hbig, g->h[v]);
+    }
+    H.prune = (prm->t + prm->skin) * hbig <= H.b0 * (float)(1 << (H.nlev - 1)) ? 1 : 0;
     return H;
 }
 
@@ -185,10 +193,10 @@ tnd::Field upload_grid(const tn_grid_view* g, DevMem& mem) {
 }
 
 // Kernel classes of the per-kernel device timings.
-enum KClass { KC_KEYS, KC_SCAN, KC_RADIX, KC_GATHER, KC_NEIGHBORS, KC_ORDER, KC_FORCE, KC_MOVE, KC_STATS, KC_REFRESH,
-              KC_CONTROL, KC_COUNT };
+enum KClass { KC_KEYS, KC_SCAN, KC_RADIX, KC_GATHER, KC_NEIGHBORS, KC_ORDER, KC_FORCE, KC_MOVE_EASY, KC_MOVE,
+              KC_STATS, KC_REFRESH, KC_CONTROL, KC_COUNT };
 const char* const kKernelNames[KC_COUNT] = { "keys", "scan", "radix_sort", "gather", "neighbors", "order",
-                                             "force", "move", "stats", "refresh", "control" };
+                                             "force", "move_easy", "move", "stats", "refresh", "control" };
 
 // Device time per kernel class from event pairs on one stream. Elapsed times are
 // read only after the stream has been synchronised (collect), so timing adds
@@ -277,13 +285,14 @@ struct tn_plan {
     uint16_t* dLab = nullptr;
     float *dMv = nullptr, *dPw = nullptr, *dThr = nullptr;
     int32_t *dKey = nullptr, *dCstart = nullptr, *dSorted = nullptr, *dNbrT = nullptr, *dNnb = nullptr;
-    int32_t *dOrder = nullptr, *dQueue = nullptr, *dBhist = nullptr, *dBsum = nullptr;
+    int32_t *dOrder = nullptr, *dQueue = nullptr, *dMoveQ = nullptr, *dBhist = nullptr, *dBsum = nullptr;
+    uint8_t* dFlag = nullptr;   // nodes past a full skin this iteration
     int32_t* dRk[2] = { nullptr, nullptr };
     int32_t* dRv[2] = { nullptr, nullptr };
     double2* dPartials = nullptr;
     tnk::Ctl* dCtl = nullptr;
     tnk::Ctl* hCtl = nullptr;   // pinned
-    int nparts = 0, radix_passes = 0, radix_nblk = 0;
+    int nparts = 0, radix_passes = 0, radix_nblk = 0, refresh_blocks = 1;
     // host-side loop state
     int rebuilds = 0;
     bool finished = false;
@@ -358,7 +367,7 @@ struct tn_plan {
         tnk::k_gather<<<nb, tnk::kBlock, 0, stream>>>(dSorted, dX, n, dPs, dP0);
         timer.end(KC_GATHER, t);
         t = timer.begin();
-        tnk::k_neighbors<<<blocks_for(n, tnk::kNbrBlock), tnk::kNbrBlock, 0, stream>>>(H, truss_in(), n, dNbrT, dNnb);
+        tnk::k_truss<<<blocks_for(n, tnk::kTrussWave), tnk::kTrussWave, 0, stream>>>(H, truss_in(), n, dNbrT, dNnb);
         timer.end(KC_NEIGHBORS, t);
         t = timer.begin();
         tnk::k_order_begin<<<1, 64, 0, stream>>>(dCtl);
@@ -402,14 +411,19 @@ struct tn_plan {
         timer.end(KC_FORCE, t);
         t = timer.begin();
         tnk::MoveArgs ma{ prm.dt, prm.maxstep, prm.snap, fire };
-        tnk::k_move<<<nb, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dOrder, n, dF, dX, dV, dTyp, dPart, dLab, dMv, dPw);
+        tnk::k_move_easy<<<nb, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dOrder, n, dF, dX, dV, dTyp, dPart, dLab, dMv,
+                                                         dPw, dMoveQ);
+        timer.end(KC_MOVE_EASY, t);
+        t = timer.begin();
+        tnk::k_move<<<nb, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQ, dF, dX, dV, dTyp, dPart, dLab, dMv, dPw);
         timer.end(KC_MOVE, t);
         t = timer.begin();
-        tnk::k_stats<<<nb, tnk::kBlock, 0, stream>>>(dX, dP0, dMv, dPw, n, prm.skin, fire, dThr, dCtl, dQueue,
+        tnk::k_stats<<<nb, tnk::kBlock, 0, stream>>>(dX, dP0, dMv, dPw, n, prm.skin, fire, dThr, dCtl, dFlag,
                                                      dPartials);
         timer.end(KC_STATS, t);
         t = timer.begin();
-        tnk::k_refresh<<<256, tnk::kNbrBlock, 0, stream>>>(H, truss_in(), n, dQueue, dCtl, dNbrT, dNnb, dP0);
+        tnk::k_refresh_list<<<nb, tnk::kBlock, 0, stream>>>(dSorted, dFlag, n, dCtl, dQueue);
+        tnk::k_truss_node<<<refresh_blocks, tnk::kWave, 0, stream>>>(H, truss_in(), n, dQueue, dCtl, dNbrT, dNnb, dP0);
         timer.end(KC_REFRESH, t);
         t = timer.begin();
         tnk::CtlArgs ca{ n, prm.max_iters, fire, rebuild_div, prm.dptol, nparts };
@@ -454,7 +468,7 @@ struct tn_plan {
         const double* ms = timer.ms;
         st->ms_hash = ms[KC_KEYS] + ms[KC_SCAN] + ms[KC_RADIX] + ms[KC_GATHER] + ms[KC_NEIGHBORS] + ms[KC_ORDER];
         st->ms_force = ms[KC_FORCE];
-        st->ms_move = ms[KC_MOVE] + ms[KC_STATS] + ms[KC_REFRESH] + ms[KC_CONTROL];
+        st->ms_move = ms[KC_MOVE_EASY] + ms[KC_MOVE] + ms[KC_STATS] + ms[KC_REFRESH] + ms[KC_CONTROL];
         if (n > 0) {   // type counts (src/v2m_particles.cpp:968-975), counted on the device
             tnk::k_order_begin<<<1, 64, 0, stream>>>(dCtl);
             tnk::k_order_count<<<blocks_for(n, tnk::kBlock), tnk::kBlock, 0, stream>>>(dTyp, n, dCtl);
@@ -679,6 +693,8 @@ int tn_plan_create(const tn_grid_view* g, const tn_relax_params* prm, const tn_n
         p->dNnb = p->alloc<int32_t>(n);
         p->dOrder = p->alloc<int32_t>(n);
         p->dQueue = p->alloc<int32_t>(n);
+        p->dMoveQ = p->alloc<int32_t>(n);
+        p->dFlag = p->alloc<uint8_t>(n);
         p->dCstart = p->alloc<int32_t>((size_t)p->nkeys + 1);
         for (int k = 0; k < 2; ++k) {
             p->dRk[k] = p->alloc<int32_t>(n);
@@ -692,6 +708,7 @@ int tn_plan_create(const tn_grid_view* g, const tn_relax_params* prm, const tn_n
         const int64_t scan_max = std::max<int64_t>(p->nkeys + 1, (int64_t)256 * p->radix_nblk);
         p->dBsum = p->alloc<int32_t>(blocks_for(scan_max, tnk::kScanTile));
         p->nparts = (int)blocks_for(n, tnk::kBlock);
+        p->refresh_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(n, 8192));   // one wave per listed node
         p->dPartials = p->alloc<double2>(std::max(p->nparts, 1));
         p->dThr = p->alloc<float>(tnk::kHistThr);
         p->dCtl = p->alloc<tnk::Ctl>(1);

@@ -21,7 +21,7 @@ from ._abi import (K, TN_ERR_NODEVICE, TN_ERR_UNSUPPORTED, TN_OK, Grid, GridView
                    NodesView, RelaxParams, RelaxParamsC, RelaxStatsC)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libtrussnet_amd.so")
+LIB_PATH = os.environ.get("TN_LIBRARY") or os.path.join(HERE, "lib", "libtrussnet_amd.so")
 
 # every entry point of include/trussnet_amd.h
 SYMBOLS = ["tn_relax_params_default", "tn_last_error", "tn_device_count", "tn_relax", "tn_seed_count", "tn_seed",
