@@ -44,19 +44,36 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-# algorithmic HBM bytes per node per launch of each kernel class (K = mean list length);
-# neighbour and field gathers count as reuse
-def kernel_bytes(kbar: float, radix_passes: int) -> dict:
+# Algorithmic HBM bytes per item (node, or cell for the scan) of each kernel class
+# (K = mean list length). Neighbour and field gathers count as reuse, not traffic.
+def item_bytes(kbar: float, radix_passes: int) -> dict:
     return {
-        "force": 37 + 4 * kbar,      # P,h 16 + typ 1 + nnb 4 + nbr 4K read; F 16 written
-        "move": 96,                  # F 16, P,h 16, V 12, typ 1, part 4, lab 2, order 4 read; P,h 16, V 12, typ 1, part 4, mv 4, pw 4 written
-        "stats": 36,                 # mv 4, pw 4, P,h 16, P0 12 read
-        "neighbors": 23 + 4 * kbar,  # P,h 16, typ 1, lab 2 read; nbr 4K + nnb 4 written
-        "keys": 24,                  # P 12 + h 4 read; key 4 written; count 4
+        "force": 37 + 4 * kbar,       # P,h 16 + typ 1 + nnb 4 + nbr 4K read; F 16 written
+        "move": 104,                  # queue 4, F 16, P,h 16, V 16, typ 1, part 4, lab 2 read; P,h 16, V 16, typ 1,
+                                      # part 4, mv 4, pw 4 written
+        "stats": 36,                  # mv 4, pw 4, P,h 16, P0 12 read
+        "refresh": 47 + 4 * kbar,     # list 4, sorted 4, P,h 16, typ 1, lab 2 read; nbr 4K, nnb 4, P0 16 written
+        "neighbors": 23 + 4 * kbar,   # P,h 16, typ 1, lab 2 read; nbr 4K + nnb 4 written
+        "keys": 24,                   # P 12 + h 4 read; key 4 written; count 4
+        "scan": 8,                    # per cell: read + write
         "radix_sort": 20 * radix_passes,
-        "gather": 68,                # sorted 4 + X 16 read, Ps 16 written; P0 16 + 16
+        "gather": 68,                 # sorted 4 + X 16 read, Ps 16 written; P0 16 + 16
         "order": 6,
     }
+
+
+def launch_bytes(name: str, timings: dict, per_item: dict) -> float | None:
+    """Algorithmic bytes of one average launch of a kernel class."""
+    launches, _, items = timings[name]
+    if not launches:
+        return None
+    if name == "move_easy":   # every node: order 4 + typ 1; finished nodes 94 more; queued ones 4 (the queue)
+        queued = timings["move"][2] / max(timings["move"][0], 1)
+        n = items / launches
+        return 5 * n + 94 * (n - queued) + 4 * queued
+    if name not in per_item:
+        return None
+    return per_item[name] * items / launches
 
 
 def build_workload(name: str):
@@ -147,20 +164,24 @@ def main() -> int:
     ms_step = 1e3 * dt / args.steps
     log(f"timed: {args.steps} steps in {dt:.3f} s -> {value:.4g} node-iterations/s; "
         f"{st['rebuilds']} rebuilds/step, mean list {kbar:.2f}")
-    for name, (cnt, ms) in sorted(timings.items(), key=lambda kv: -kv[1][1]):
+    for name, (cnt, ms, items) in sorted(timings.items(), key=lambda kv: -kv[1][1]):
         if cnt:
-            log(f"  {name:<11} {cnt:6d} launches {ms:9.2f} ms total {1e3 * ms / cnt:9.1f} us/launch")
+            log(f"  {name:<11} {cnt:6d} launches {ms:9.2f} ms total {1e3 * ms / cnt:9.1f} us/launch "
+                f"{items / cnt:11.0f} items/launch")
     # roofline of the dominant kernel
     passes = 0 if plan.hash.nkeys <= 1 else (int(plan.hash.nkeys - 1).bit_length() + 7) // 8
-    per_node = kernel_bytes(kbar, passes)
-    ranked = [k for k, _ in sorted(timings.items(), key=lambda kv: -kv[1][1]) if k in per_node and timings[k][0]]
+    per_item = item_bytes(kbar, passes)
+    ranked = [k for k, v in sorted(timings.items(), key=lambda kv: -kv[1][1])
+              if v[0] and launch_bytes(k, timings, per_item) is not None]
     dom = ranked[0]
-    cnt, ms = timings[dom]
+    cnt, ms, _ = timings[dom]
     avg_s = ms / cnt / 1e3
-    achieved = per_node[dom] * n / avg_s
+    nbytes = launch_bytes(dom, timings, per_item)
+    achieved = nbytes / avg_s
     roofline = {"bound": "hbm", "kernel": f"k_{dom}", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                "bytes_per_launch": per_node[dom] * n, "avg_launch_us": avg_s * 1e6}
+                "bytes_per_launch": nbytes, "avg_launch_us": avg_s * 1e6,
+                "items_per_launch": timings[dom][2] / cnt}
     loop_bytes = iters * n * (83 + 4 * kbar) + st["rebuilds"] * n * (179 + 4 * kbar)
     cpu = None if args.no_cpu else cpu_baseline(grid, seeds, args.cpu_iters)
     line = {

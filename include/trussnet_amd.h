@@ -193,13 +193,16 @@ int tn_plan_reset(tn_plan* plan);
  * src/v2m_gpu.cpp:250-258, without its extra synchronisations). Off by default
  * (on with prm->verbose or TN_KERNEL_TIMING in the environment); switching
  * clears the totals. Kernel classes: keys, scan, radix_sort, gather, neighbors,
- * order (the rebuild), force, move, stats, refresh, control (an iteration).
- * The tn_relax_stats ms_* fields are filled from these when timing is on.
+ * order (the rebuild), force, move_easy, move, stats, refresh, control (an
+ * iteration). items: the nodes the timed launches processed (cells for scan;
+ * for move, the nodes queued for the full trapped step; for refresh, the listed
+ * nodes). The tn_relax_stats ms_* fields are filled from these when timing is on.
  */
 typedef struct tn_kernel_time {
     char name[16];
     int64_t launches;
     double ms;
+    int64_t items;
 } tn_kernel_time;
 
 int tn_plan_set_timing(tn_plan* plan, int32_t on);

@@ -50,7 +50,8 @@ def test_struct_layouts_match_header(tmp_path):
     from trussnet_amd import _abi
 
     structs = {"tn_grid_view": _abi.GridView, "tn_relax_params": _abi.RelaxParamsC, "tn_nodes_view": _abi.NodesView,
-               "tn_relax_stats": _abi.RelaxStatsC, "tn_hash_info": _abi.HashInfoC}
+               "tn_relax_stats": _abi.RelaxStatsC, "tn_hash_info": _abi.HashInfoC,
+               "tn_kernel_time": _abi.KernelTimeC}
     src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for cname, cls in structs.items():
         src.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

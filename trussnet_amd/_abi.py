@@ -76,7 +76,7 @@ class HashInfoC(C.Structure):
 
 
 class KernelTimeC(C.Structure):
-    _fields_ = [("name", C.c_char * 16), ("launches", C.c_int64), ("ms", C.c_double)]
+    _fields_ = [("name", C.c_char * 16), ("launches", C.c_int64), ("ms", C.c_double), ("items", C.c_int64)]
 
 
 def _ptr(a: np.ndarray | None, ctype):

@@ -225,14 +225,9 @@ __device__ __forceinline__ float psi_at(const Field& f, int a, int b, P3 p, PsiC
     return trilerp(pc.d, u - (float)i0, v - (float)j0, w - (float)k0);
 }
 
-// v2m_psi_grad (gm == 0), src/opencl/v2m_seed_body.cl:211-247: (gx, gy, gz, psi),
-// through a cell cache
-__device__ __forceinline__ float4 psi_grad_at(const Field& f, int a, int b, P3 p, PsiCell& pc) {
-    const float u = p.x / f.vx, v = p.y / f.vy, w = p.z / f.vz;
-    const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
-    const float fx = u - (float)i0, fy = v - (float)j0, fz = w - (float)k0;
-    psi_cell_load(f, a, b, i0, j0, k0, pc);
-    const float* c = pc.d;
+// the trilinear value and gradient of v2m_psi_grad (src/opencl/v2m_seed_body.cl:
+// 233-246) from the corner differences c: (gx, gy, gz, psi)
+__device__ __forceinline__ float4 trilerp_grad(const Field& f, const float* c, float fx, float fy, float fz) {
     const float x00 = c[0] + fx * (c[1] - c[0]), x10 = c[2] + fx * (c[3] - c[2]);
     const float x01 = c[4] + fx * (c[5] - c[4]), x11 = c[6] + fx * (c[7] - c[6]);
     const float y0 = x00 + fy * (x10 - x00), y1 = x01 + fy * (x11 - x01);
@@ -244,6 +239,16 @@ __device__ __forceinline__ float4 psi_grad_at(const Field& f, int a, int b, P3 p
     g.z = (y1 - y0) / f.vz;
     g.w = y0 + fz * (y1 - y0);
     return g;
+}
+
+// v2m_psi_grad (gm == 0), src/opencl/v2m_seed_body.cl:211-247: (gx, gy, gz, psi),
+// through a cell cache
+__device__ __forceinline__ float4 psi_grad_at(const Field& f, int a, int b, P3 p, PsiCell& pc) {
+    const float u = p.x / f.vx, v = p.y / f.vy, w = p.z / f.vz;
+    const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
+    const float fx = u - (float)i0, fy = v - (float)j0, fz = w - (float)k0;
+    psi_cell_load(f, a, b, i0, j0, k0, pc);
+    return trilerp_grad(f, pc.d, fx, fy, fz);
 }
 
 // psi_ab and its gradient at one point (no cache to share)
@@ -278,6 +283,47 @@ __device__ __forceinline__ int brick_label(const Field& f, BrickLabels bl, int s
     return bl.b < 0 ? 0 : (int)f.bl_lab[(size_t)bl.b * kBL + s];
 }
 
+// The trilinear cell of a point, and whether it lies in one brick that is the brick
+// of the point's rounded voxel (whose labels v2m_labels_near lists): the slot of
+// that brick's s-th label is then bl_slot[b] + s, with no search (the labels of a
+// brick are distinct, so the search would stop at s).
+struct PointCell {
+    int i0, j0, k0;
+    float fx, fy, fz;
+    int s0;     // bl_slot of the shared brick (-1: uniform brick)
+    int same;   // the cell lies in the labels' brick
+};
+
+__device__ __forceinline__ PointCell point_cell(const Field& f, P3 p, BrickLabels bl) {
+    PointCell c;
+    const float u = p.x / f.vx, v = p.y / f.vy, w = p.z / f.vz;
+    c.i0 = (int)floorf(u);
+    c.j0 = (int)floorf(v);
+    c.k0 = (int)floorf(w);
+    c.fx = u - (float)c.i0;
+    c.fy = v - (float)c.j0;
+    c.fz = w - (float)c.k0;
+    c.same = bl.b >= 0 && cell_in_one_brick(f, c.i0, c.j0, c.k0) && brick_at(f, c.i0, c.j0, c.k0) == bl.b;
+    c.s0 = c.same ? f.bl_slot[bl.b] : -1;
+    return c;
+}
+
+// the corners of phi_l for the s-th label l of bl at the point's cell (v2m_phi_cell)
+__device__ __forceinline__ void label_corners(const Field& f, const PointCell& pc, int s, int l, float* c) {
+    if (pc.same) {
+        corners_in_brick(f, l, pc.s0 < 0 ? -1 : pc.s0 + s, 1, pc.i0, pc.j0, pc.k0, c);
+    } else {
+        phi_corners(f, l, pc.i0, pc.j0, pc.k0, c);
+    }
+}
+
+// phi of the s-th label l of bl at the point (v2m_phi_at)
+__device__ __forceinline__ float label_phi(const Field& f, const PointCell& pc, int s, int l) {
+    float c[8];
+    label_corners(f, pc, s, l, c);
+    return trilerp(c, pc.fx, pc.fy, pc.fz);
+}
+
 // v2m_label_of (gm == 0, voxmode 0), src/opencl/v2m_seed_body.cl:116-175: (best, second)
 __device__ __forceinline__ int2 owner_at(const Field& f, P3 p) {
     const BrickLabels bl = brick_labels_at(f, p);
@@ -286,9 +332,10 @@ __device__ __forceinline__ int2 owner_at(const Field& f, P3 p) {
     if (bl.n == 1) {
         return make_int2(best, kNoLab);
     }
+    const PointCell pc = point_cell(f, p, bl);
     for (int s = 0; s < bl.n; ++s) {
         const int l = brick_label(f, bl, s);
-        const float v = phi_point(f, l, p);
+        const float v = label_phi(f, pc, s, l);
         if (v > pb) {
             ps = pb;
             sec = best;
@@ -300,6 +347,67 @@ __device__ __forceinline__ int2 owner_at(const Field& f, P3 p) {
         }
     }
     return make_int2(best, sec);
+}
+
+// owner_at, and when the owner is label a with a runner-up, psi_{a,second} and its
+// gradient at p from the same corners (v2m_psi_grad, src/opencl/v2m_seed_body.cl:
+// 211-247): the interior branch of the trapped step needs exactly these.
+struct OwnerPsi {
+    int best, sec;
+    float4 g;   // (grad psi, psi) when best == a and sec != kNoLab
+};
+
+__device__ __forceinline__ OwnerPsi owner_psi_at(const Field& f, int a, P3 p) {
+    const BrickLabels bl = brick_labels_at(f, p);
+    OwnerPsi r;
+    r.best = brick_label(f, bl, 0);
+    r.sec = kNoLab;
+    r.g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (bl.n == 1) {
+        return r;
+    }
+    const PointCell pc = point_cell(f, p, bl);
+    float pb = -1.0f, ps = pb;
+    float cb[8], cs[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        cb[m] = cs[m] = 0.0f;
+    }
+    for (int s = 0; s < bl.n; ++s) {
+        const int l = brick_label(f, bl, s);
+        float c[8];
+        label_corners(f, pc, s, l, c);
+        const float v = trilerp(c, pc.fx, pc.fy, pc.fz);
+        if (v > pb) {
+            ps = pb;
+            r.sec = r.best;
+            pb = v;
+            r.best = l;
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                cs[m] = cb[m];
+                cb[m] = c[m];
+            }
+        } else if (v > ps) {
+            ps = v;
+            r.sec = l;
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                cs[m] = c[m];
+            }
+        }
+    }
+    if (r.best == a && r.sec != kNoLab) {
+        // from the second label on, cs holds the runner-up's corners (label 0 becomes
+        // best at s = 0; at s = 1 either it moves to second or label 1 takes second)
+        float d[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            d[m] = cb[m] - cs[m];
+        }
+        r.g = trilerp_grad(f, d, pc.fx, pc.fy, pc.fz);
+    }
+    return r;
 }
 
 __device__ __forceinline__ P3 along(P3 p, float sg, float t, P3 u) {
@@ -548,10 +656,11 @@ __device__ __noinline__ int competing_label_near(const Field& f, int a, int b, P
     const float pa = phi_point(f, a, q);
     int c = kNoLab;
     float pc = 0.5f * pa;
+    const PointCell cell = point_cell(f, q, bl);
     for (int s = 0; s < bl.n; ++s) {
         const int l = brick_label(f, bl, s);
         if (l != a && l != b) {
-            const float w = phi_point(f, l, q);
+            const float w = label_phi(f, cell, s, l);
             if (w > pc) {
                 pc = w;
                 c = l;
@@ -580,9 +689,10 @@ __device__ __noinline__ int labels_dominate_near(const Field& f, int a, int b, i
     if (c != kNoLab) {
         mine = fminf(mine, phi_point(f, c, q));
     }
+    const PointCell cell = point_cell(f, q, bl);
     for (int s = 0; s < bl.n; ++s) {
         const int l = brick_label(f, bl, s);
-        if (l != a && l != b && l != c && phi_point(f, l, q) > mine + 0.02f) {
+        if (l != a && l != b && l != c && label_phi(f, cell, s, l) > mine + 0.02f) {
             return 0;
         }
     }

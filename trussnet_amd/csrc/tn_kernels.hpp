@@ -884,13 +884,11 @@ __global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl*
                 const int a = lab[i];
                 const P3 p = p3(x.x, x.y, x.z);
                 const P3 q = p3(p.x + st.s[0], p.y + st.s[1], p.z + st.s[2]);
-                const int2 own = owner_at(f, q);
-                if (own.x != a) {
+                const OwnerPsi own = owner_psi_at(f, a, q);
+                if (own.best != a) {
                     hard = true;
-                } else if (own.y != kNoLab && ma.snap > 0.0f) {
-                    PsiCell pc;
-                    psi_cell_init(pc);
-                    const float4 gq = psi_grad_at(f, a, own.y, q, pc);
+                } else if (own.sec != kNoLab && ma.snap > 0.0f) {
+                    const float4 gq = own.g;
                     const float gn = sqrtf(gq.x * gq.x + gq.y * gq.y + gq.z * gq.z);
                     hard = gn > 1e-9f && gq.w / gn < ma.snap * h;
                 }

@@ -212,6 +212,7 @@ struct KernelTimer {
     std::vector<Rec> pending;
     double ms[KC_COUNT] = {};
     int64_t launches[KC_COUNT] = {};
+    int64_t items[KC_COUNT] = {};
 
     ~KernelTimer() {
         for (const Rec& r : pending) {
@@ -259,9 +260,15 @@ struct KernelTimer {
         }
         pending.clear();
     }
+    void count(int kc, int64_t m) {   // nodes (or cells) a timed launch processed
+        if (on) {
+            items[kc] += m;
+        }
+    }
     void clear() {
         std::fill(ms, ms + KC_COUNT, 0.0);
         std::fill(launches, launches + KC_COUNT, 0);
+        std::fill(items, items + KC_COUNT, 0);
     }
 };
 
@@ -375,6 +382,10 @@ struct tn_plan {
         tnk::k_order_scatter<<<nb, tnk::kBlock, 0, stream>>>(dTyp, n, dCtl, dOrder);
         timer.end(KC_ORDER, t);
         TN_HIP(hipGetLastError());
+        for (int kc : { KC_KEYS, KC_RADIX, KC_GATHER, KC_NEIGHBORS, KC_ORDER }) {
+            timer.count(kc, n);
+        }
+        timer.count(KC_SCAN, nkeys + 1);
         ++rebuilds;
     }
 
@@ -433,6 +444,12 @@ struct tn_plan {
         TN_HIP(hipMemcpyAsync(hCtl, dCtl, sizeof(tnk::Ctl), hipMemcpyDeviceToHost, stream));
         TN_HIP(hipStreamSynchronize(stream));
         timer.collect();
+        for (int kc : { KC_FORCE, KC_MOVE_EASY, KC_STATS }) {
+            timer.count(kc, n);
+        }
+        timer.count(KC_MOVE, hCtl->nmove);
+        timer.count(KC_REFRESH, hCtl->nrefresh);
+        timer.count(KC_CONTROL, 1);
         if (hCtl->done) {
             finished = true;
             return false;
@@ -756,6 +773,7 @@ int tn_plan_timings(const tn_plan* p, tn_kernel_time* out, int32_t cap, int32_t*
             std::strncpy(out[k].name, kKernelNames[k], sizeof out[k].name - 1);
             out[k].launches = p->timer.launches[k];
             out[k].ms = p->timer.ms[k];
+            out[k].items = p->timer.items[k];
         }
     });
 }

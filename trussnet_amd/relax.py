@@ -222,8 +222,10 @@ class Plan:
         _check(lib().tn_plan_set_timing(self._h, 1 if on else 0))
 
     def timings(self) -> dict:
-        """{kernel class: (launches, total ms)} since timing was switched on."""
+        """{kernel class: (launches, total ms, items)} since timing was switched on (items: the
+        nodes those launches processed)."""
         buf = (KernelTimeC * 32)()
         cnt = C.c_int32(0)
         _check(lib().tn_plan_timings(self._h, buf, 32, C.byref(cnt)))
-        return {buf[k].name.decode(): (int(buf[k].launches), float(buf[k].ms)) for k in range(min(cnt.value, 32))}
+        return {buf[k].name.decode(): (int(buf[k].launches), float(buf[k].ms), int(buf[k].items))
+                for k in range(min(cnt.value, 32))}
