@@ -15,8 +15,12 @@ ends with a device synchronisation (the plan's stream), so the host clock
 brackets exactly the device work.
 
 roofline: the kernel with the largest device time in the timed region (HIP
-events on the plan's stream), its algorithmic bytes per launch (per-node
-figures below, times the nodes) over its average launch time, against 8 TB/s.
+events on the plan's stream), its algorithmic bytes per launch (the per-node
+figures below, times the nodes the launches processed) over its average launch
+time, against 8 TB/s. traffic: its HBM bytes per launch from two rocprofv3 PMC
+passes (FETCH_SIZE, WRITE_SIZE) over a short run of this script made before the
+timed run, each scaled by what the counter reads for the plain device copy of the
+node positions (16 bytes per node) in the same pass.
 hbm_frac_loop: the whole loop's algorithmic bytes, iterations x n x (83 + 4 K)
 + rebuilds x n x (179 + 4 K), over the step time, against 8 TB/s.
 cpu_baseline: the reference's own relax_cpu (oracle/_ref, compiled from its
@@ -28,9 +32,14 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|sma
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -57,7 +66,8 @@ def item_bytes(kbar: float, radix_passes: int) -> dict:
         "keys": 24,                   # P 12 + h 4 read; key 4 written; count 4
         "scan": 8,                    # per cell: read + write
         "radix_sort": 20 * radix_passes,
-        "gather": 68,                 # sorted 4 + X 16 read, Ps 16 written; P0 16 + 16
+        "permute": 134,               # sorted 4, X 16, V 16, typ 1, part 4, lab 2, id 4 read; the same 47 written,
+                                      # Ps 16, P0 16, Ms 8, slot_of 4
         "order": 6,
     }
 
@@ -74,6 +84,55 @@ def launch_bytes(name: str, timings: dict, per_item: dict) -> float | None:
     if name not in per_item:
         return None
     return per_item[name] * items / launches
+
+
+# the kernels of each timed class (their function names in a rocprofv3 trace)
+CLASS_KERNELS = {
+    "force": ["k_force"], "move_easy": ["k_move_easy"], "move": ["k_move"], "stats": ["k_stats"],
+    "neighbors": ["k_truss"], "refresh": ["k_refresh_list", "k_truss_node"], "permute": ["k_permute"],
+    "keys": ["k_fill_i32", "k_keys"], "order": ["k_order_begin", "k_order_count", "k_order_scatter"],
+    "control": ["k_control"],
+}
+
+
+def pmc_traffic(args) -> dict | None:
+    """HBM bytes per launch of each kernel from two rocprofv3 PMC passes (FETCH_SIZE, then
+    WRITE_SIZE) over a short run of this script, run before this process touches the GPU.
+    Returns {kernel: (mean FETCH_SIZE KiB, mean WRITE_SIZE KiB)} and the same counters for
+    the largest plain device copy of the run (the positions restored at a reset, 16 bytes
+    per node), which calibrates them."""
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        log("rocprofv3 not found: no traffic measurement")
+        return None
+    out = tempfile.mkdtemp(prefix="tn_pmc_", dir="/tmp")
+    child = [sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu", "--no-traffic",
+             "--iters", "10", "--workload", args.workload]
+    vals: dict = {}
+    copy = {}
+    for k, counter in enumerate(("FETCH_SIZE", "WRITE_SIZE")):
+        t = time.perf_counter()
+        cmd = [exe, "--pmc", counter, "-f", "csv", "-d", out, "-o", counter.lower(), "--"] + child
+        try:
+            r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), capture_output=True, text=True,
+                               timeout=300)
+        except subprocess.TimeoutExpired:
+            log(f"rocprofv3 --pmc {counter}: timed out")
+            return None
+        files = glob.glob(os.path.join(out, "**", f"{counter.lower()}_counter_collection.csv"), recursive=True)
+        if r.returncode != 0 or not files:
+            log(f"rocprofv3 --pmc {counter}: rc {r.returncode}\n{r.stderr[-1500:]}")
+            return None
+        per: dict = {}
+        for row in csv.DictReader(open(files[0])):
+            name = row["Kernel_Name"].split("(")[0].split("::")[-1]
+            per.setdefault(name, []).append(float(row["Counter_Value"]))
+        for name, v in per.items():
+            vals.setdefault(name, [0.0, 0.0])[k] = sum(v) / len(v)
+        copy[counter] = max(per.get("__amd_rocclr_copyBuffer", [0.0]))
+        log(f"pmc {counter}: {len(per)} kernels ({time.perf_counter() - t:.0f} s)")
+    shutil.rmtree(out, ignore_errors=True)
+    return {"kernels": vals, "copy_kib": (copy["FETCH_SIZE"], copy["WRITE_SIZE"])}
 
 
 def build_workload(name: str):
@@ -131,12 +190,15 @@ def main() -> int:
     ap.add_argument("--iters", type=int, default=0, help="iterations per step (default: the workload's)")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 PMC passes")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py: this build runs the relaxation on one GPU; --gpus {args.gpus} "
               f"(world size {world}) is not supported", file=sys.stderr)
         return 2
+
+    pmc = None if args.no_traffic else pmc_traffic(args)   # before this process initialises the GPU
 
     from trussnet_amd import relax
     from trussnet_amd._abi import RelaxParams
@@ -178,8 +240,20 @@ def main() -> int:
     avg_s = ms / cnt / 1e3
     nbytes = launch_bytes(dom, timings, per_item)
     achieved = nbytes / avg_s
+    traffic = None
+    if pmc and pmc["copy_kib"][0] > 0 and pmc["copy_kib"][1] > 0:
+        # the copy moves 16 n bytes each way: bytes = counter KiB x 1024 x factor
+        fr = 16.0 * n / (pmc["copy_kib"][0] * 1024.0)
+        fw = 16.0 * n / (pmc["copy_kib"][1] * 1024.0)
+        ks = [pmc["kernels"].get(k) for k in CLASS_KERNELS.get(dom, [])]
+        if ks and all(ks):
+            rd = sum(v[0] for v in ks) * 1024.0 * fr
+            wr = sum(v[1] for v in ks) * 1024.0 * fw
+            traffic = rd + wr
+            log(f"traffic of {dom}: {rd / 1e6:.1f} MB read + {wr / 1e6:.1f} MB written per launch "
+                f"(calibration: read x{fr:.3f}, write x{fw:.3f})")
     roofline = {"bound": "hbm", "kernel": f"k_{dom}", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                 "bytes_per_launch": nbytes, "avg_launch_us": avg_s * 1e6,
                 "items_per_launch": timings[dom][2] / cnt}
     loop_bytes = iters * n * (83 + 4 * kbar) + st["rebuilds"] * n * (179 + 4 * kbar)

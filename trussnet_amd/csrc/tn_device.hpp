@@ -38,12 +38,18 @@ constexpr int kNoLab = TN_NOLAB;
 struct Field {
     int nx, ny, nz, nbx, nby, nbz;
     float vx, vy, vz;
+    float ivx, ivy, ivz;   // 1 / v, used only when every voxel size is a power of two
+    int pow2;
     const uint16_t* __restrict__ L;
     const int32_t* __restrict__ bl_cnt;
     const uint16_t* __restrict__ bl_lab;
     const int32_t* __restrict__ bl_slot;
     const float* __restrict__ phi;
     const float* __restrict__ h;
+    // this struct's copy in device memory. Out-of-line functions take it instead of a
+    // reference: a reference to the kernel argument would move the whole struct into
+    // scratch memory and turn the kernel's loads through its pointers into flat loads.
+    const struct Field* self;
 };
 
 // The multi-level hash geometry (V2mHash, src/opencl/v2m_particle_body.cl:21-27).
@@ -56,6 +62,19 @@ struct Hash {
     int nkeys;
     int prune;   // every node's reach fits its level's cell: the K-nearest search may skip far cells
 };
+
+// x / v for the voxel sizes: when they are powers of two, x * (1 / v) is the same
+// correctly rounded value (both round the exact x 2^-k), and a multiplication is a
+// fraction of an IEEE division's cost
+__device__ __forceinline__ float over_vx(const Field& f, float x) {
+    return f.pow2 ? x * f.ivx : x / f.vx;
+}
+__device__ __forceinline__ float over_vy(const Field& f, float x) {
+    return f.pow2 ? x * f.ivy : x / f.vy;
+}
+__device__ __forceinline__ float over_vz(const Field& f, float x) {
+    return f.pow2 ? x * f.ivz : x / f.vz;
+}
 
 struct P3 {
     float x, y, z;
@@ -100,7 +119,7 @@ __device__ __forceinline__ int slot_in(const Field& f, int b, int l, int* has) {
 }
 
 // v2m_phi_vox, src/opencl/v2m_grid_body.cl:137-153
-__device__ __noinline__ float phi_voxel(const Field& f, int l, int i, int j, int k) {
+__device__ __forceinline__ float phi_voxel(const Field& f, int l, int i, int j, int k) {
     if (i < 0 || j < 0 || k < 0 || i >= f.nx || j >= f.ny || k >= f.nz) {
         return l == 0 ? 1.0f : 0.0f;
     }
@@ -137,6 +156,22 @@ __device__ __forceinline__ void corners_in_brick(const Field& f, int l, int s, i
     }
 }
 
+// v2m_phi_cell's general case (src/opencl/v2m_grid_body.cl:182-185): a cell across
+// bricks or at the grid's edge, corner by corner
+struct Corners {
+    float v[8];
+};
+
+__device__ __noinline__ Corners phi_corners_split(const Field* fp, int l, int i0, int j0, int k0) {
+    const Field f = *fp;
+    Corners c;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        c.v[m] = phi_voxel(f, l, i0 + (m & 1), j0 + ((m >> 1) & 1), k0 + ((m >> 2) & 1));
+    }
+    return c;
+}
+
 // v2m_phi_cell, src/opencl/v2m_grid_body.cl:159-186 (corner bit 0 -> +i, 1 -> +j, 2 -> +k)
 __device__ __forceinline__ void phi_corners(const Field& f, int l, int i0, int j0, int k0, float* c) {
     if (cell_in_one_brick(f, i0, j0, k0)) {
@@ -145,9 +180,10 @@ __device__ __forceinline__ void phi_corners(const Field& f, int l, int i0, int j
         corners_in_brick(f, l, s, has, i0, j0, k0, c);
         return;
     }
+    const Corners cc = phi_corners_split(f.self, l, i0, j0, k0);
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-        c[m] = phi_voxel(f, l, i0 + (m & 1), j0 + ((m >> 1) & 1), k0 + ((m >> 2) & 1));
+        c[m] = cc.v[m];
     }
 }
 
@@ -188,10 +224,12 @@ __device__ __forceinline__ void psi_cell_load(const Field& f, int a, int b, int 
         corners_in_brick(f, a, pc.sa, pc.ha, i0, j0, k0, c);
         corners_in_brick(f, b, pc.sb, pc.hb, i0, j0, k0, e);
     } else {
+        const Corners ca = phi_corners_split(f.self, a, i0, j0, k0);
+        const Corners cb = phi_corners_split(f.self, b, i0, j0, k0);
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
-            c[m] = phi_voxel(f, a, i0 + (m & 1), j0 + ((m >> 1) & 1), k0 + ((m >> 2) & 1));
-            e[m] = phi_voxel(f, b, i0 + (m & 1), j0 + ((m >> 1) & 1), k0 + ((m >> 2) & 1));
+            c[m] = ca.v[m];
+            e[m] = cb.v[m];
         }
     }
 #pragma unroll
@@ -210,7 +248,7 @@ __device__ __forceinline__ float trilerp(const float* c, float fx, float fy, flo
 
 // v2m_phi_at (v2m_phi_f, gm == 0), src/opencl/v2m_grid_body.cl:189-200
 __device__ __forceinline__ float phi_point(const Field& f, int l, P3 p) {
-    const float u = p.x / f.vx, v = p.y / f.vy, w = p.z / f.vz;
+    const float u = over_vx(f, p.x), v = over_vy(f, p.y), w = over_vz(f, p.z);
     const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
     float c[8];
     phi_corners(f, l, i0, j0, k0, c);
@@ -219,7 +257,7 @@ __device__ __forceinline__ float phi_point(const Field& f, int l, P3 p) {
 
 // v2m_psi (gm == 0), src/opencl/v2m_seed_body.cl:179-205, through a cell cache
 __device__ __forceinline__ float psi_at(const Field& f, int a, int b, P3 p, PsiCell& pc) {
-    const float u = p.x / f.vx, v = p.y / f.vy, w = p.z / f.vz;
+    const float u = over_vx(f, p.x), v = over_vy(f, p.y), w = over_vz(f, p.z);
     const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
     psi_cell_load(f, a, b, i0, j0, k0, pc);
     return trilerp(pc.d, u - (float)i0, v - (float)j0, w - (float)k0);
@@ -234,9 +272,9 @@ __device__ __forceinline__ float4 trilerp_grad(const Field& f, const float* c, f
     const float dx0 = (c[1] - c[0]) + fy * ((c[3] - c[2]) - (c[1] - c[0]));
     const float dx1 = (c[5] - c[4]) + fy * ((c[7] - c[6]) - (c[5] - c[4]));
     float4 g;
-    g.x = (dx0 + fz * (dx1 - dx0)) / f.vx;
-    g.y = ((x10 - x00) + fz * ((x11 - x01) - (x10 - x00))) / f.vy;
-    g.z = (y1 - y0) / f.vz;
+    g.x = over_vx(f, dx0 + fz * (dx1 - dx0));
+    g.y = over_vy(f, (x10 - x00) + fz * ((x11 - x01) - (x10 - x00)));
+    g.z = over_vz(f, y1 - y0);
     g.w = y0 + fz * (y1 - y0);
     return g;
 }
@@ -244,7 +282,7 @@ __device__ __forceinline__ float4 trilerp_grad(const Field& f, const float* c, f
 // v2m_psi_grad (gm == 0), src/opencl/v2m_seed_body.cl:211-247: (gx, gy, gz, psi),
 // through a cell cache
 __device__ __forceinline__ float4 psi_grad_at(const Field& f, int a, int b, P3 p, PsiCell& pc) {
-    const float u = p.x / f.vx, v = p.y / f.vy, w = p.z / f.vz;
+    const float u = over_vx(f, p.x), v = over_vy(f, p.y), w = over_vz(f, p.z);
     const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
     const float fx = u - (float)i0, fy = v - (float)j0, fz = w - (float)k0;
     psi_cell_load(f, a, b, i0, j0, k0, pc);
@@ -252,7 +290,8 @@ __device__ __forceinline__ float4 psi_grad_at(const Field& f, int a, int b, P3 p
 }
 
 // psi_ab and its gradient at one point (no cache to share)
-__device__ __noinline__ float4 psi_grad_point(const Field& f, int a, int b, P3 p) {
+__device__ __noinline__ float4 psi_grad_point(const Field* fp, int a, int b, P3 p) {
+    const Field f = *fp;
     PsiCell pc;
     psi_cell_init(pc);
     return psi_grad_at(f, a, b, p, pc);
@@ -266,7 +305,8 @@ struct BrickLabels {
 };
 
 __device__ __forceinline__ BrickLabels brick_labels_at(const Field& f, P3 p) {
-    const int i = (int)floorf(p.x / f.vx + 0.5f), j = (int)floorf(p.y / f.vy + 0.5f), k = (int)floorf(p.z / f.vz + 0.5f);
+    const int i = (int)floorf(over_vx(f, p.x) + 0.5f), j = (int)floorf(over_vy(f, p.y) + 0.5f),
+              k = (int)floorf(over_vz(f, p.z) + 0.5f);
     BrickLabels r;
     if (i < 0 || j < 0 || k < 0 || i >= f.nx || j >= f.ny || k >= f.nz) {
         r.b = -1;
@@ -296,7 +336,7 @@ struct PointCell {
 
 __device__ __forceinline__ PointCell point_cell(const Field& f, P3 p, BrickLabels bl) {
     PointCell c;
-    const float u = p.x / f.vx, v = p.y / f.vy, w = p.z / f.vz;
+    const float u = over_vx(f, p.x), v = over_vy(f, p.y), w = over_vz(f, p.z);
     c.i0 = (int)floorf(u);
     c.j0 = (int)floorf(v);
     c.k0 = (int)floorf(w);
@@ -485,14 +525,16 @@ __device__ __forceinline__ P3ok onto_interface_at(const Field& f, int a, int b, 
     return bracket_along(f, a, b, p, u, g.w, rad, 1.5f * fabsf(g.w) / gn, pc);
 }
 
-__device__ __noinline__ P3ok onto_interface(const Field& f, int a, int b, P3 p, float rad) {
+__device__ __noinline__ P3ok onto_interface(const Field* fp, int a, int b, P3 p, float rad) {
+    const Field f = *fp;
     PsiCell pc;
     psi_cell_init(pc);
     return onto_interface_at(f, a, b, p, rad, pc);
 }
 
 // v2m_project2 (gm == 0), src/opencl/v2m_seed_body.cl:342-420 (ok = 0: p unchanged)
-__device__ __noinline__ P3ok onto_junction(const Field& f, int a, int b, int c, P3 p, float rad) {
+__device__ __noinline__ P3ok onto_junction(const Field* fp, int a, int b, int c, P3 p, float rad) {
+    const Field f = *fp;
     P3ok fail;
     fail.p = p;
     fail.ok = 0;
@@ -547,7 +589,8 @@ __device__ __noinline__ P3ok onto_junction(const Field& f, int a, int b, int c, 
 
 // v2m_h_at, src/opencl/v2m_particle_body.cl:30-36
 __device__ __forceinline__ float size_at(const Field& f, float px, float py, float pz) {
-    int i = (int)floorf(px / f.vx + 0.5f), j = (int)floorf(py / f.vy + 0.5f), k = (int)floorf(pz / f.vz + 0.5f);
+    int i = (int)floorf(over_vx(f, px) + 0.5f), j = (int)floorf(over_vy(f, py) + 0.5f),
+        k = (int)floorf(over_vz(f, pz) + 0.5f);
     i = i < 0 ? 0 : (i >= f.nx ? f.nx - 1 : i);
     j = j < 0 ? 0 : (j >= f.ny ? f.ny - 1 : j);
     k = k < 0 ? 0 : (k >= f.nz ? f.nz - 1 : k);
@@ -579,13 +622,15 @@ __device__ __forceinline__ int cell_key(const Hash& H, int L, float px, float py
 // ---- truss -----------------------------------------------------------------------------
 
 // The inputs of one K-nearest search (v2m_neighbors, src/opencl/v2m_particle_body.cl:73-151).
+// Nodes sit in slots renumbered in cell order at each rebuild.
 struct TrussIn {
-    const float4* __restrict__ X;      // current positions + h
-    const uint16_t* __restrict__ lab;
-    const uint8_t* __restrict__ typ;
-    const int32_t* __restrict__ cstart;
-    const int32_t* __restrict__ sorted;
-    const float4* __restrict__ Ps;     // positions + h in cell order, as of the last rebuild
+    const float4* __restrict__ X;        // current positions + h, by slot
+    const uint16_t* __restrict__ lab;    // by slot
+    const uint8_t* __restrict__ typ;     // current types, by slot
+    const int32_t* __restrict__ cstart;  // cell starts (slots) as of the last rebuild
+    const float4* __restrict__ Ps;       // positions + h as of the last rebuild, by slot
+    const int2* __restrict__ Ms;         // (id, typ | lab << 8) as of the last rebuild, by slot
+    const int32_t* __restrict__ slot_of; // id -> slot
     float t, skin;
 };
 
@@ -652,7 +697,8 @@ __device__ __forceinline__ P3 drop_normal_at(const Field& f, int a, int b, P3 p,
 
 // v2m_third_label, src/opencl/v2m_particle_body.cl:232-260. The brick test runs
 // inline; the field comparisons (a third label is near: rare) are a call.
-__device__ __noinline__ int competing_label_near(const Field& f, int a, int b, P3 q, BrickLabels bl) {
+__device__ __noinline__ int competing_label_near(const Field* fp, int a, int b, P3 q, BrickLabels bl) {
+    const Field f = *fp;
     const float pa = phi_point(f, a, q);
     int c = kNoLab;
     float pc = 0.5f * pa;
@@ -680,11 +726,12 @@ __device__ __forceinline__ int competing_label(const Field& f, int a, int b, P3 
     if (!other) {
         return kNoLab;
     }
-    return competing_label_near(f, a, b, q, bl);
+    return competing_label_near(f.self, a, b, q, bl);
 }
 
 // v2m_valid_on, src/opencl/v2m_particle_body.cl:266-291 (the same split)
-__device__ __noinline__ int labels_dominate_near(const Field& f, int a, int b, int c, P3 q, BrickLabels bl) {
+__device__ __noinline__ int labels_dominate_near(const Field* fp, int a, int b, int c, P3 q, BrickLabels bl) {
+    const Field f = *fp;
     float mine = fminf(phi_point(f, a, q), phi_point(f, b, q));
     if (c != kNoLab) {
         mine = fminf(mine, phi_point(f, c, q));
@@ -709,7 +756,7 @@ __device__ __forceinline__ int labels_dominate(const Field& f, int a, int b, int
     if (!other) {
         return 1;
     }
-    return labels_dominate_near(f, a, b, c, q, bl);
+    return labels_dominate_near(f.self, a, b, c, q, bl);
 }
 
 __device__ __forceinline__ float dist2(P3 q, P3 p) {
@@ -778,15 +825,15 @@ __device__ __forceinline__ float trapped_step(const Field& f, float h, P3 s, flo
         }
         q = pr.p;
     } else {
-        const float4 g1 = psi_grad_point(f, a, b, p);
-        const float4 g2 = psi_grad_point(f, a, c, p);
+        const float4 g1 = psi_grad_point(f.self, a, b, p);
+        const float4 g2 = psi_grad_point(f.self, a, c, p);
         const float t0 = g1.y * g2.z - g1.z * g2.y;
         const float t1 = g1.z * g2.x - g1.x * g2.z;
         const float t2 = g1.x * g2.y - g1.y * g2.x;
         const float tl2 = t0 * t0 + t1 * t1 + t2 * t2;
         const float st = tl2 > 1e-20f ? (s.x * t0 + s.y * t1 + s.z * t2) / tl2 : 0.0f;
         q = p3(p.x + st * t0, p.y + st * t1, p.z + st * t2);
-        const P3ok pr = onto_junction(f, a, b, c, q, 0.5f * h);
+        const P3ok pr = onto_junction(f.self, a, b, c, q, 0.5f * h);
         if (!pr.ok) {
             return 0.0f;
         }
@@ -796,7 +843,7 @@ __device__ __forceinline__ float trapped_step(const Field& f, float h, P3 s, flo
     if (ty == TN_INTERFACE) {
         const int cc = competing_label(f, a, b, q);
         if (cc != kNoLab) {
-            const P3ok pr = onto_junction(f, a, b, cc, q, 0.5f * h);
+            const P3ok pr = onto_junction(f.self, a, b, cc, q, 0.5f * h);
             if (pr.ok) {
                 q = pr.p;
                 ty = TN_JUNCTION;
@@ -821,7 +868,7 @@ __device__ __forceinline__ float trapped_step(const Field& f, float h, P3 s, flo
             const int cc = competing_label(f, a, ns.b, q);
             int joined = 0;
             if (cc != kNoLab) {
-                const P3ok pr = onto_junction(f, a, ns.b, cc, q, 0.5f * h);
+                const P3ok pr = onto_junction(f.self, a, ns.b, cc, q, 0.5f * h);
                 if (pr.ok && labels_dominate(f, a, ns.b, cc, pr.p)) {
                     q = pr.p;
                     joined = 1;

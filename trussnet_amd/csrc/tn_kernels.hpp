@@ -1,14 +1,19 @@
 // tn_kernels.hpp -- the relaxation's gfx950 kernels.
 //
 // Rebuild (src/v2m_particles.cpp:771-820): k_keys (cell key + count per cell),
-// k_scan_* (exclusive scan: cell starts), k_radix_* (stable LSD radix sort of
-// the node ids by cell key: ascending id within a cell, as the CPU's serial
-// counting sort), k_gather (positions in cell order), k_truss (K-nearest lists,
-// one wave per 64 nodes), k_order_* (move order grouped by node type).
-// Iteration (src/v2m_particles.cpp:828-947): k_force, k_move, k_stats (histogram,
-// max, Verlet count, refresh flags, FIRE power partial sums), k_refresh_list +
-// k_truss<true> (own-list refresh of the nodes past a full skin), k_control (one
-// block: FIRE's update, p99, the stop and rebuild decisions, all on the device).
+// k_scan_* (exclusive scan: cell starts), k_radix_* (stable LSD radix sort of the
+// nodes by cell key), k_permute (the nodes renumbered in cell order), k_truss
+// (K-nearest lists, one wave per 64 nodes), k_order_* (move order grouped by type).
+// Iteration (src/v2m_particles.cpp:828-947): k_force, k_move_easy + k_move,
+// k_stats (histogram, max, Verlet count, refresh flags, FIRE power partial sums),
+// k_refresh_list + k_truss_node (own-list refresh of the nodes past a full skin),
+// k_control (one block: FIRE's update, p99, the stop and rebuild decisions).
+//
+// Node slots: at every rebuild the nodes are renumbered in cell order (slot t holds
+// the node at old slot sorted[t]), so a wave's nodes, their list entries and their
+// neighbours lie close in memory. id[t] is the node's index in the caller's arrays:
+// it orders equal distances in the K-lists, as the reference's node index does, and
+// places the results; slot_of is its inverse. The lists hold slots.
 #pragma once
 
 #include "tn_device.hpp"
@@ -253,14 +258,40 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const int32_t* __restr
     }
 }
 
-// ---- gather (positions in cell order) + P0 = P ---------------------------------------
+// ---- renumbering in cell order ---------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_gather(const int32_t* __restrict__ sorted, const float4* __restrict__ X,
-                                                   int n, float4* __restrict__ Ps, float4* __restrict__ P0) {
-    const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s < n) {
-        Ps[s] = X[sorted[s]];
-        P0[s] = X[s];
+// Slot t takes the node of old slot sorted[t]; Ps and P0 (the positions as of this
+// rebuild) and Ms (id, typ | lab << 8) are written in the new order.
+struct NodeArrays {
+    float4* X;
+    float4* V;
+    uint8_t* typ;
+    uint32_t* part;
+    uint16_t* lab;
+    int32_t* id;
+};
+
+__global__ __launch_bounds__(kBlock) void k_permute(const int32_t* __restrict__ sorted, int n, NodeArrays in,
+                                                    NodeArrays out, float4* __restrict__ Ps,
+                                                    float4* __restrict__ P0, int2* __restrict__ Ms,
+                                                    int32_t* __restrict__ slot_of) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t < n) {
+        const int s = sorted[t];
+        const float4 x = in.X[s];
+        out.X[t] = x;
+        Ps[t] = x;
+        P0[t] = x;
+        out.V[t] = in.V[s];
+        const int ty = in.typ[s];
+        out.typ[t] = (uint8_t)ty;
+        out.part[t] = in.part[s];
+        const int l = in.lab[s];
+        out.lab[t] = (uint16_t)l;
+        const int o = in.id[s];
+        out.id[t] = o;
+        Ms[t] = make_int2(o, ty | (l << 8));
+        slot_of[o] = t;
     }
 }
 
@@ -332,7 +363,7 @@ __device__ __forceinline__ void sort_lane(float* cd, int* cj, int lane, int cnt)
     }
 }
 
-// The lists of all nodes at a rebuild: query q is the node at sorted position q.
+// The lists of all nodes at a rebuild: query q is the node in slot q.
 __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n, int32_t* __restrict__ nbrT,
                                                       int32_t* __restrict__ nnb) {
     __shared__ float4 cpos[kTrussWave];
@@ -340,30 +371,30 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
     __shared__ float cd[kCollect * kTrussWave];
     __shared__ int cj[kCollect * kTrussWave];
     const int lane = threadIdx.x;
-    const int m = n;
     const float tskin = in.t + in.skin;
-    for (int qb = blockIdx.x * kTrussWave; qb < m; qb += gridDim.x * kTrussWave) {
+    const float half = tskin * 0.5f;   // (t + skin) * 0.5f * (hi + hj), left to right
+    for (int qb = blockIdx.x * kTrussWave; qb < n; qb += gridDim.x * kTrussWave) {
         const int q = qb + lane;
-        const bool active = q < m;
+        const bool active = q < n;
         int i = 0;
         float px = 0.0f, py = 0.0f, pz = 0.0f, hi = 0.0f, Ri = 0.0f;
         int Li = 0, ti_int = 0, li = 0;
-        if (active) {
-            i = in.sorted[q];
-            const float4 xi = in.X[i];
+        if (active) {   // the node in slot q (nodes are in cell order); i: its id
+            const float4 xi = in.X[q];
             px = xi.x;
             py = xi.y;
             pz = xi.z;
             hi = xi.w;
             Ri = tskin * hi;
             Li = level_for(H, Ri);
-            ti_int = in.typ[i] == TN_INTERIOR;
-            li = in.lab[i];
+            const int2 mi = in.Ms[q];
+            i = mi.x;
+            ti_int = (mi.y & 0xFF) == TN_INTERIOR;
+            li = mi.y >> 8;
         }
         // The lanes are served in groups that share a cell row of their own level and lie
         // within kGroupSpan cells of the group's first lane along it, so that a group's
-        // union of scan boxes stays tight (a wave of queries can straddle rows, and a
-        // refresh wave holds scattered nodes).
+        // union of scan boxes stays tight (a wave of queries can straddle rows).
         int grow = -1, gx = 0;
         if (active) {
             const int cy = cell_coord(H, Li, py, H.oy, 1), cz = cell_coord(H, Li, pz, H.oz, 2);
@@ -373,113 +404,127 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
         bool pending = active;
         int cnt = 0;
         for (;;) {
-        const uint64_t pm = __ballot(pending);
-        if (!pm) {
-            break;
-        }
-        const int lead = __ffsll((long long)pm) - 1;
-        const int lrow = __shfl(grow, lead, kWave), lx = __shfl(gx, lead, kWave);
-        const bool ingroup = pending && grow == lrow && abs(gx - lx) <= kGroupSpan;
-        pending = pending && !ingroup;
-        const int lmin = wave_min(ingroup ? max(Li - 1, 0) : INT_MAX);
-        const int lmax = wave_max(ingroup ? min(Li + 1, H.nlev - 1) : -1);
-        for (int L = lmin; L <= lmax; ++L) {
-            const bool mine = ingroup && L >= Li - 1 && L <= Li + 1;
-            const float bL = H.b0 * (float)(1 << L);
-            const float r = 0.5f * (Ri + bL);
-            int x0 = INT_MAX, x1 = -1, y0 = INT_MAX, y1 = -1, z0 = INT_MAX, z1 = -1;
-            if (mine) {
-                x0 = cell_coord(H, L, px - r, H.ox, 0);
-                x1 = cell_coord(H, L, px + r, H.ox, 0);
-                y0 = cell_coord(H, L, py - r, H.oy, 1);
-                y1 = cell_coord(H, L, py + r, H.oy, 1);
-                z0 = cell_coord(H, L, pz - r, H.oz, 2);
-                z1 = cell_coord(H, L, pz + r, H.oz, 2);
+            const uint64_t pm = __ballot(pending);
+            if (!pm) {
+                break;
             }
-            const int Y0 = wave_min(y0), Y1 = wave_max(y1), Z0 = wave_min(z0), Z1 = wave_max(z1);
-            // rows beyond reach are skipped (a valid bar is shorter than r; margin for rounding)
-            const float rw = r * 1.0001f + 1e-3f * bL + 2e-6f * (fabsf(px) + fabsf(py) + fabsf(pz));
-            const float rr = rw * rw;
-            const int ny1 = H.dim[L][1] - 1, nz1 = H.dim[L][2] - 1;
-            for (int cz = Z0; cz <= Z1; ++cz) {
-                const float zlo = cz == 0 ? -__builtin_huge_valf() : H.oz + (float)cz * bL;
-                const float zhi = cz == nz1 ? __builtin_huge_valf() : H.oz + (float)(cz + 1) * bL;
-                const float dz = fmaxf(0.0f, fmaxf(zlo - pz, pz - zhi));
-                for (int cy = Y0; cy <= Y1; ++cy) {
-                    const float ylo = cy == 0 ? -__builtin_huge_valf() : H.oy + (float)cy * bL;
-                    const float yhi = cy == ny1 ? __builtin_huge_valf() : H.oy + (float)(cy + 1) * bL;
-                    const float dy = fmaxf(0.0f, fmaxf(ylo - py, py - yhi));
-                    int xa = INT_MAX, xb = -1;
-                    if (mine && cz >= z0 && cz <= z1 && cy >= y0 && cy <= y1) {
-                        if (!H.prune) {
-                            xa = x0;
-                            xb = x1;
-                        } else {
-                            const float rem = rr - dz * dz - dy * dy;
-                            if (rem >= 0.0f) {
-                                const float w = sqrtf(rem);
-                                xa = max(x0, cell_coord(H, L, px - w, H.ox, 0));
-                                xb = min(x1, cell_coord(H, L, px + w, H.ox, 0));
-                            }
-                        }
-                    }
-                    const int XA = wave_min(xa), XB = wave_max(xb);
-                    if (XA > XB) {
-                        continue;
-                    }
-                    const int krow = H.off[L] + H.dim[L][0] * (cy + H.dim[L][1] * cz);
-                    const int sa = in.cstart[krow + XA], sb = in.cstart[krow + XB + 1];
-                    for (int base = sa; base < sb; base += kTrussWave) {
-                        __syncthreads();   // the previous chunk is consumed
-                        const int s = base + lane;
-                        if (s < sb) {
-                            const int j = in.sorted[s];
-                            cpos[lane] = in.Ps[s];
-                            cmeta[lane] = make_int2(j, (int)in.typ[j] | ((int)in.lab[j] << 8));
-                        }
-                        __syncthreads();
-                        const int cn = min(kTrussWave, sb - base);
-                        if (!mine) {
-                            continue;
-                        }
-                        const bool row_own = cz >= z0 && cz <= z1 && cy >= y0 && cy <= y1;
-                        for (int k = 0; k < cn; ++k) {
-                            const float4 c = cpos[k];
-                            const float ex = c.x - px, ey = c.y - py, ez = c.z - pz;
-                            const float d2 = ex * ex + ey * ey + ez * ez;
-                            const float rj = tskin * 0.5f * (hi + c.w);
-                            if (d2 >= rj * rj) {
-                                continue;
-                            }
-                            if (!H.prune) {   // the reference's own scan region only
-                                const int cx = cell_coord(H, L, c.x, H.ox, 0);
-                                if (!row_own || cx < x0 || cx > x1) {
-                                    continue;
+            const int lead = __ffsll((long long)pm) - 1;
+            const int lrow = __shfl(grow, lead, kWave), lx = __shfl(gx, lead, kWave);
+            const bool ingroup = pending && grow == lrow && abs(gx - lx) <= kGroupSpan;
+            pending = pending && !ingroup;
+            const int lmin = wave_min(ingroup ? max(Li - 1, 0) : INT_MAX);
+            const int lmax = wave_max(ingroup ? min(Li + 1, H.nlev - 1) : -1);
+            for (int L = lmin; L <= lmax; ++L) {
+                const bool mine = ingroup && L >= Li - 1 && L <= Li + 1;
+                const float bL = H.b0 * (float)(1 << L);
+                const float r = 0.5f * (Ri + bL);
+                int x0 = INT_MAX, x1 = -1, y0 = INT_MAX, y1 = -1, z0 = INT_MAX, z1 = -1;
+                if (mine) {
+                    x0 = cell_coord(H, L, px - r, H.ox, 0);
+                    x1 = cell_coord(H, L, px + r, H.ox, 0);
+                    y0 = cell_coord(H, L, py - r, H.oy, 1);
+                    y1 = cell_coord(H, L, py + r, H.oy, 1);
+                    z0 = cell_coord(H, L, pz - r, H.oz, 2);
+                    z1 = cell_coord(H, L, pz + r, H.oz, 2);
+                }
+                const int Y0 = wave_min(y0), Y1 = wave_max(y1), Z0 = wave_min(z0), Z1 = wave_max(z1);
+                // rows beyond reach are skipped (a valid bar is shorter than r; margin for rounding)
+                const float rw = r * 1.0001f + 1e-3f * bL + 2e-6f * (fabsf(px) + fabsf(py) + fabsf(pz));
+                const float rr = rw * rw;
+                const int ny1 = H.dim[L][1] - 1, nz1 = H.dim[L][2] - 1;
+                for (int cz = Z0; cz <= Z1; ++cz) {
+                    const float zlo = cz == 0 ? -__builtin_huge_valf() : H.oz + (float)cz * bL;
+                    const float zhi = cz == nz1 ? __builtin_huge_valf() : H.oz + (float)(cz + 1) * bL;
+                    const float dz = fmaxf(0.0f, fmaxf(zlo - pz, pz - zhi));
+                    for (int cy = Y0; cy <= Y1; ++cy) {
+                        const float ylo = cy == 0 ? -__builtin_huge_valf() : H.oy + (float)cy * bL;
+                        const float yhi = cy == ny1 ? __builtin_huge_valf() : H.oy + (float)(cy + 1) * bL;
+                        const float dy = fmaxf(0.0f, fmaxf(ylo - py, py - yhi));
+                        const bool row_own = mine && cz >= z0 && cz <= z1 && cy >= y0 && cy <= y1;
+                        int xa = INT_MAX, xb = -1;
+                        if (row_own) {
+                            if (!H.prune) {
+                                xa = x0;
+                                xb = x1;
+                            } else {
+                                const float rem = rr - dz * dz - dy * dy;
+                                if (rem >= 0.0f) {
+                                    const float w = sqrtf(rem);
+                                    xa = max(x0, cell_coord(H, L, px - w, H.ox, 0));
+                                    xb = min(x1, cell_coord(H, L, px + w, H.ox, 0));
                                 }
                             }
-                            const int2 mt = cmeta[k];
-                            if (mt.x == i || (ti_int && (mt.y & 0xFF) == TN_INTERIOR && (mt.y >> 8) != li)) {
+                        }
+                        const int XA = wave_min(xa), XB = wave_max(xb);
+                        if (XA > XB) {
+                            continue;
+                        }
+                        const bool scan = xa <= xb;   // this lane has cells in the row
+                        const int krow = H.off[L] + H.dim[L][0] * (cy + H.dim[L][1] * cz);
+                        const int sa = in.cstart[krow + XA], sb = in.cstart[krow + XB + 1];
+                        for (int base = sa; base < sb; base += kTrussWave) {
+                            __syncthreads();   // the previous chunk is consumed
+                            const int s = base + lane;
+                            if (s < sb) {
+                                cpos[lane] = in.Ps[s];
+                                cmeta[lane] = in.Ms[s];
+                            } else {   // past the row's end: far away, it fails every test
+                                const float inf = __builtin_huge_valf();
+                                cpos[lane] = make_float4(inf, inf, inf, 0.0f);
+                                cmeta[lane] = make_int2(-1, 0);
+                            }
+                            __syncthreads();
+                            if (!scan) {
                                 continue;
                             }
-                            if (cnt == kCollect) {   // keep the 24 best so far
-                                sort_lane(cd, cj, lane, cnt);
-                                cnt = kK;
+                            const int cn = min(kTrussWave, sb - base);
+                            for (int k0 = 0; k0 < cn; k0 += 8) {
+                                // distance tests of 8 candidates, then the rare hits one by one
+                                uint32_t hit = 0;
+#pragma unroll
+                                for (int u = 0; u < 8; ++u) {
+                                    const float4 c = cpos[k0 + u];
+                                    const float ex = c.x - px, ey = c.y - py, ez = c.z - pz;
+                                    const float d2 = ex * ex + ey * ey + ez * ez;
+                                    const float rj = half * (hi + c.w);
+                                    hit |= (d2 < rj * rj) ? (1u << u) : 0u;
+                                }
+                                while (hit) {
+                                    const int k = k0 + __builtin_ctz(hit);
+                                    hit &= hit - 1;
+                                    const float4 c = cpos[k];
+                                    if (!H.prune) {   // the reference's own scan region only
+                                        const int cx = cell_coord(H, L, c.x, H.ox, 0);
+                                        if (cx < x0 || cx > x1) {
+                                            continue;
+                                        }
+                                    }
+                                    const int2 mt = cmeta[k];
+                                    if (mt.x == i || (ti_int && (mt.y & 0xFF) == TN_INTERIOR && (mt.y >> 8) != li)) {
+                                        continue;
+                                    }
+                                    const float ex = c.x - px, ey = c.y - py, ez = c.z - pz;
+                                    const float d2 = ex * ex + ey * ey + ez * ez;
+                                    if (cnt == kCollect) {   // keep the 24 best so far
+                                        sort_lane(cd, cj, lane, cnt);
+                                        cnt = kK;
+                                    }
+                                    cd[cnt * kTrussWave + lane] = sqrtf(d2);
+                                    cj[cnt * kTrussWave + lane] = mt.x;
+                                    ++cnt;
+                                }
                             }
-                            cd[cnt * kTrussWave + lane] = sqrtf(d2);
-                            cj[cnt * kTrussWave + lane] = mt.x;
-                            ++cnt;
                         }
                     }
                 }
             }
-        }
         }   // the groups
-        if (active) {
+        if (active) {   // (distance, id) order; the list holds slots
             sort_lane(cd, cj, lane, cnt);
             const int kk = cnt < kK ? cnt : kK;
-            nnb[i] = kk;
+            nnb[q] = kk;
             for (int k = 0; k < kk; ++k) {
-                nbrT[(size_t)k * n + i] = cj[k * kTrussWave + lane];
+                nbrT[(size_t)k * n + q] = in.slot_of[cj[k * kTrussWave + lane]];
             }
         }
         __syncthreads();   // the buffers are reused by the next block of queries
@@ -540,7 +585,7 @@ __global__ __launch_bounds__(kWave) void k_truss_node(Hash H, TrussIn in, int n,
     const int m = ctl->nrefresh;
     const float tskin = in.t + in.skin;
     for (int q = blockIdx.x; q < m; q += gridDim.x) {
-        const int i = in.sorted[list[q]];
+        const int i = list[q];   // the node's slot
         const float4 xi = in.X[i];
         const float px = xi.x, py = xi.y, pz = xi.z, hi = xi.w;
         const float Ri = tskin * hi;
@@ -591,9 +636,9 @@ __global__ __launch_bounds__(kWave) void k_truss_node(Hash H, TrussIn in, int n,
                             const float ex = c.x - px, ey = c.y - py, ez = c.z - pz;
                             const float d2 = ex * ex + ey * ey + ez * ez;
                             const float rj = tskin * 0.5f * (hi + c.w);
-                            if (d2 < rj * rj) {
-                                j = in.sorted[s];
-                                ok = j != i && !(ti_int && in.typ[j] == TN_INTERIOR && in.lab[j] != li);
+                            if (d2 < rj * rj) {   // slot s: the candidate's current type, its id
+                                ok = s != i && !(ti_int && in.typ[s] == TN_INTERIOR && in.lab[s] != li);
+                                j = in.Ms[s].x;
                                 dist = sqrtf(d2);
                             }
                         }
@@ -617,7 +662,7 @@ __global__ __launch_bounds__(kWave) void k_truss_node(Hash H, TrussIn in, int n,
         __syncthreads();   // the appends are visible to every lane
         const int kk = keep_best(cd, cj, cnt, lane);
         if (lane < kk) {
-            nbrT[(size_t)lane * n + i] = cj[lane];
+            nbrT[(size_t)lane * n + i] = in.slot_of[cj[lane]];
         }
         if (lane == 0) {
             nnb[i] = kk;
@@ -627,16 +672,15 @@ __global__ __launch_bounds__(kWave) void k_truss_node(Hash H, TrussIn in, int n,
     }
 }
 
-// the sorted positions of the nodes flagged for an own-list refresh, appended block
-// by block (in cell order within a block): spatially coherent query waves
-__global__ __launch_bounds__(kBlock) void k_refresh_list(const int32_t* __restrict__ sorted,
-                                                         const uint8_t* __restrict__ flag, int n,
+// the slots of the nodes flagged for an own-list refresh, appended block by block
+// (in slot order, which is cell order as of the last rebuild, within a block)
+__global__ __launch_bounds__(kBlock) void k_refresh_list(const uint8_t* __restrict__ flag, int n,
                                                          Ctl* __restrict__ ctl, int32_t* __restrict__ list) {
     __shared__ int woff[kBlock / kWave];
     __shared__ int base;
     const int s = blockIdx.x * kBlock + threadIdx.x;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
-    const bool f = s < n && flag[sorted[s]];
+    const bool f = s < n && flag[s];
     const uint64_t mask = __ballot(f);
     if (lane == 0) {
         woff[w] = __popcll(mask);
@@ -1098,7 +1142,8 @@ struct CtlArgs {
 // order, exactly as the CPU does.
 __global__ __launch_bounds__(kBlock) void k_control(CtlArgs ca, Ctl* __restrict__ ctl,
                                                     const double2* __restrict__ partials,
-                                                    const float* __restrict__ pw) {
+                                                    const float* __restrict__ pw,
+                                                    const int32_t* __restrict__ slot_of) {
     __shared__ double red[kBlock][2];
     __shared__ float chunk[kScanTile];
     __shared__ int use_serial;
@@ -1128,11 +1173,11 @@ __global__ __launch_bounds__(kBlock) void k_control(CtlArgs ca, Ctl* __restrict_
         use_serial = ca.fire && A > 0.0 && !(fabs(P) > bound);
     }
     __syncthreads();
-    if (use_serial) {   // the CPU's own summation, in index order
+    if (use_serial) {   // the CPU's own summation, in node index (id) order
         double pt = 0.0;
         for (int c0 = 0; c0 < ca.n; c0 += kScanTile) {
             for (int k = tid; k < kScanTile; k += kBlock) {
-                chunk[k] = c0 + k < ca.n ? pw[c0 + k] : 0.0f;
+                chunk[k] = c0 + k < ca.n ? pw[slot_of[c0 + k]] : 0.0f;
             }
             __syncthreads();
             if (tid == 0) {

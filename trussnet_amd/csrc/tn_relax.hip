@@ -183,19 +183,31 @@ tnd::Field upload_grid(const tn_grid_view* g, DevMem& mem) {
     f.vx = g->vs[0];
     f.vy = g->vs[1];
     f.vz = g->vs[2];
+    // a power of two whose reciprocal is a normal float: x / v == x * (1 / v) exactly
+    auto pow2 = [](float v) {
+        int e = 0;
+        return std::isfinite(v) && v > 0.0f && std::frexp(v, &e) == 0.5f && e > -120 && e < 120;
+    };
+    f.pow2 = pow2(f.vx) && pow2(f.vy) && pow2(f.vz);
+    f.ivx = 1.0f / f.vx;
+    f.ivy = 1.0f / f.vy;
+    f.ivz = 1.0f / f.vz;
     f.L = mem.upload(g->L, nv);
     f.bl_cnt = mem.upload(g->bl_cnt, nb);
     f.bl_lab = mem.upload(g->bl_lab, nb * TN_BL);
     f.bl_slot = mem.upload(g->bl_slot, nb);
     f.phi = mem.upload(g->phi, (size_t)g->nslot * TN_SLOT);
     f.h = mem.upload(g->h, nv);
+    tnd::Field* self = mem.alloc<tnd::Field>(1);
+    f.self = self;
+    TN_HIP(hipMemcpy(self, &f, sizeof f, hipMemcpyHostToDevice));
     return f;
 }
 
 // Kernel classes of the per-kernel device timings.
-enum KClass { KC_KEYS, KC_SCAN, KC_RADIX, KC_GATHER, KC_NEIGHBORS, KC_ORDER, KC_FORCE, KC_MOVE_EASY, KC_MOVE,
+enum KClass { KC_KEYS, KC_SCAN, KC_RADIX, KC_PERMUTE, KC_NEIGHBORS, KC_ORDER, KC_FORCE, KC_MOVE_EASY, KC_MOVE,
               KC_STATS, KC_REFRESH, KC_CONTROL, KC_COUNT };
-const char* const kKernelNames[KC_COUNT] = { "keys", "scan", "radix_sort", "gather", "neighbors", "order",
+const char* const kKernelNames[KC_COUNT] = { "keys", "scan", "radix_sort", "permute", "neighbors", "order",
                                              "force", "move_easy", "move", "stats", "refresh", "control" };
 
 // Device time per kernel class from event pairs on one stream. Elapsed times are
@@ -286,10 +298,16 @@ struct tn_plan {
     KernelTimer timer;
     // device buffers
     DevMem mem;
+    // node state by slot (renumbered in cell order at every rebuild; *b: the other
+    // buffer of each pair), and the seeds by id
     float4 *dX0 = nullptr, *dX = nullptr, *dV = nullptr, *dF = nullptr, *dP0 = nullptr, *dPs = nullptr;
-    uint8_t *dTyp0 = nullptr, *dTyp = nullptr;
-    uint32_t *dPart0 = nullptr, *dPart = nullptr;
-    uint16_t* dLab = nullptr;
+    float4 *dXb = nullptr, *dVb = nullptr;
+    int2* dMs = nullptr;
+    uint8_t *dTyp0 = nullptr, *dTyp = nullptr, *dTypb = nullptr;
+    uint32_t *dPart0 = nullptr, *dPart = nullptr, *dPartb = nullptr;
+    uint16_t *dLab0 = nullptr, *dLab = nullptr, *dLabb = nullptr;
+    int32_t *dId = nullptr, *dIdb = nullptr, *dSlotOf = nullptr;
+    const int32_t* dKeySorted = nullptr;   // the keys in slot order after the last rebuild
     float *dMv = nullptr, *dPw = nullptr, *dThr = nullptr;
     int32_t *dKey = nullptr, *dCstart = nullptr, *dSorted = nullptr, *dNbrT = nullptr, *dNnb = nullptr;
     int32_t *dOrder = nullptr, *dQueue = nullptr, *dMoveQ = nullptr, *dBhist = nullptr, *dBsum = nullptr;
@@ -337,8 +355,9 @@ struct tn_plan {
         in.lab = dLab;
         in.typ = dTyp;
         in.cstart = dCstart;
-        in.sorted = dSorted;
         in.Ps = dPs;
+        in.Ms = dMs;
+        in.slot_of = dSlotOf;
         in.t = prm.t;
         in.skin = prm.skin;
         return in;
@@ -371,8 +390,16 @@ struct tn_plan {
         }
         timer.end(KC_RADIX, t);
         t = timer.begin();
-        tnk::k_gather<<<nb, tnk::kBlock, 0, stream>>>(dSorted, dX, n, dPs, dP0);
-        timer.end(KC_GATHER, t);
+        dKeySorted = radix_passes == 0 ? dKey : dRk[(radix_passes - 1) & 1];
+        const tnk::NodeArrays cur{ dX, dV, dTyp, dPart, dLab, dId }, nxt{ dXb, dVb, dTypb, dPartb, dLabb, dIdb };
+        tnk::k_permute<<<nb, tnk::kBlock, 0, stream>>>(dSorted, n, cur, nxt, dPs, dP0, dMs, dSlotOf);
+        std::swap(dX, dXb);
+        std::swap(dV, dVb);
+        std::swap(dTyp, dTypb);
+        std::swap(dPart, dPartb);
+        std::swap(dLab, dLabb);
+        std::swap(dId, dIdb);
+        timer.end(KC_PERMUTE, t);
         t = timer.begin();
         tnk::k_truss<<<blocks_for(n, tnk::kTrussWave), tnk::kTrussWave, 0, stream>>>(H, truss_in(), n, dNbrT, dNnb);
         timer.end(KC_NEIGHBORS, t);
@@ -382,7 +409,7 @@ struct tn_plan {
         tnk::k_order_scatter<<<nb, tnk::kBlock, 0, stream>>>(dTyp, n, dCtl, dOrder);
         timer.end(KC_ORDER, t);
         TN_HIP(hipGetLastError());
-        for (int kc : { KC_KEYS, KC_RADIX, KC_GATHER, KC_NEIGHBORS, KC_ORDER }) {
+        for (int kc : { KC_KEYS, KC_RADIX, KC_PERMUTE, KC_NEIGHBORS, KC_ORDER }) {
             timer.count(kc, n);
         }
         timer.count(KC_SCAN, nkeys + 1);
@@ -395,6 +422,10 @@ struct tn_plan {
         TN_HIP(hipMemcpyAsync(dX, dX0, sizeof(float4) * n, hipMemcpyDeviceToDevice, stream));
         TN_HIP(hipMemcpyAsync(dTyp, dTyp0, n, hipMemcpyDeviceToDevice, stream));
         TN_HIP(hipMemcpyAsync(dPart, dPart0, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, stream));
+        TN_HIP(hipMemcpyAsync(dLab, dLab0, sizeof(uint16_t) * n, hipMemcpyDeviceToDevice, stream));
+        if (n > 0) {   // slot = id
+            k_iota<<<blocks_for(n, tnk::kBlock), tnk::kBlock, 0, stream>>>(dId, n);
+        }
         tnk::Ctl c;
         std::memset(&c, 0, sizeof c);
         c.dt0 = std::sqrt(prm.dt);   // FireCtl's constructor (src/v2m_particles.h:50-51)
@@ -433,12 +464,12 @@ struct tn_plan {
                                                      dPartials);
         timer.end(KC_STATS, t);
         t = timer.begin();
-        tnk::k_refresh_list<<<nb, tnk::kBlock, 0, stream>>>(dSorted, dFlag, n, dCtl, dQueue);
+        tnk::k_refresh_list<<<nb, tnk::kBlock, 0, stream>>>(dFlag, n, dCtl, dQueue);
         tnk::k_truss_node<<<refresh_blocks, tnk::kWave, 0, stream>>>(H, truss_in(), n, dQueue, dCtl, dNbrT, dNnb, dP0);
         timer.end(KC_REFRESH, t);
         t = timer.begin();
         tnk::CtlArgs ca{ n, prm.max_iters, fire, rebuild_div, prm.dptol, nparts };
-        tnk::k_control<<<1, tnk::kBlock, 0, stream>>>(ca, dCtl, dPartials, dPw);
+        tnk::k_control<<<1, tnk::kBlock, 0, stream>>>(ca, dCtl, dPartials, dPw, dSlotOf);
         timer.end(KC_CONTROL, t);
         TN_HIP(hipGetLastError());
         TN_HIP(hipMemcpyAsync(hCtl, dCtl, sizeof(tnk::Ctl), hipMemcpyDeviceToHost, stream));
@@ -483,7 +514,7 @@ struct tn_plan {
         st->fire_resets = c.fire_resets;
         st->fire_dt = c.fire_dt;
         const double* ms = timer.ms;
-        st->ms_hash = ms[KC_KEYS] + ms[KC_SCAN] + ms[KC_RADIX] + ms[KC_GATHER] + ms[KC_NEIGHBORS] + ms[KC_ORDER];
+        st->ms_hash = ms[KC_KEYS] + ms[KC_SCAN] + ms[KC_RADIX] + ms[KC_PERMUTE] + ms[KC_NEIGHBORS] + ms[KC_ORDER];
         st->ms_force = ms[KC_FORCE];
         st->ms_move = ms[KC_MOVE_EASY] + ms[KC_MOVE] + ms[KC_STATS] + ms[KC_REFRESH] + ms[KC_CONTROL];
         if (n > 0) {   // type counts (src/v2m_particles.cpp:968-975), counted on the device
@@ -693,15 +724,25 @@ int tn_plan_create(const tn_grid_view* g, const tn_relax_params* prm, const tn_n
         // node state
         p->dX0 = p->alloc<float4>(n);
         p->dX = p->alloc<float4>(n);
+        p->dXb = p->alloc<float4>(n);
         p->dV = p->alloc<float4>(n);
+        p->dVb = p->alloc<float4>(n);
         p->dF = p->alloc<float4>(n);
         p->dP0 = p->alloc<float4>(n);
         p->dPs = p->alloc<float4>(n);
+        p->dMs = p->alloc<int2>(n);
         p->dTyp0 = p->alloc<uint8_t>(n);
         p->dTyp = p->alloc<uint8_t>(n);
+        p->dTypb = p->alloc<uint8_t>(n);
         p->dPart0 = p->alloc<uint32_t>(n);
         p->dPart = p->alloc<uint32_t>(n);
+        p->dPartb = p->alloc<uint32_t>(n);
+        p->dLab0 = p->alloc<uint16_t>(n);
         p->dLab = p->alloc<uint16_t>(n);
+        p->dLabb = p->alloc<uint16_t>(n);
+        p->dId = p->alloc<int32_t>(n);
+        p->dIdb = p->alloc<int32_t>(n);
+        p->dSlotOf = p->alloc<int32_t>(n);
         p->dMv = p->alloc<float>(n);
         p->dPw = p->alloc<float>(n);
         p->dKey = p->alloc<int32_t>(n);
@@ -746,7 +787,7 @@ int tn_plan_create(const tn_grid_view* g, const tn_relax_params* prm, const tn_n
             }
             TN_HIP(hipMemcpy(p->dPart0, part.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
             TN_HIP(hipMemcpy(p->dTyp0, seeds->typ, n, hipMemcpyHostToDevice));
-            TN_HIP(hipMemcpy(p->dLab, seeds->lab, sizeof(uint16_t) * n, hipMemcpyHostToDevice));
+            TN_HIP(hipMemcpy(p->dLab0, seeds->lab, sizeof(uint16_t) * n, hipMemcpyHostToDevice));
         }
         p->reset();
         *out = p.release();
@@ -820,15 +861,20 @@ int tn_plan_download(tn_plan* p, tn_nodes_view* nd) {
         }
         std::vector<float4> X(n);
         std::vector<uint32_t> part(n);
+        std::vector<uint8_t> typ(n);
+        std::vector<int32_t> id(n);
         TN_HIP(hipMemcpy(X.data(), p->dX, sizeof(float4) * n, hipMemcpyDeviceToHost));
         TN_HIP(hipMemcpy(part.data(), p->dPart, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-        TN_HIP(hipMemcpy(nd->typ, p->dTyp, n, hipMemcpyDeviceToHost));
-        for (int i = 0; i < n; ++i) {
-            nd->P[3 * i] = X[i].x;
-            nd->P[3 * i + 1] = X[i].y;
-            nd->P[3 * i + 2] = X[i].z;
-            nd->part[2 * i] = (uint16_t)(part[i] & 0xFFFF);
-            nd->part[2 * i + 1] = (uint16_t)(part[i] >> 16);
+        TN_HIP(hipMemcpy(typ.data(), p->dTyp, n, hipMemcpyDeviceToHost));
+        TN_HIP(hipMemcpy(id.data(), p->dId, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        for (int t = 0; t < n; ++t) {   // slot t holds node id[t]
+            const int i = id[t];
+            nd->P[3 * i] = X[t].x;
+            nd->P[3 * i + 1] = X[t].y;
+            nd->P[3 * i + 2] = X[t].z;
+            nd->typ[i] = typ[t];
+            nd->part[2 * i] = (uint16_t)(part[t] & 0xFFFF);
+            nd->part[2 * i + 1] = (uint16_t)(part[t] >> 16);
         }
     });
 }
@@ -867,26 +913,41 @@ int tn_plan_rebuild(tn_plan* p, int32_t* key, int32_t* sorted, int32_t* cstart, 
             }
             return;
         }
-        if (key) {
-            TN_HIP(hipMemcpy(key, p->dKey, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        // by node id, as the reference's rebuild leaves them (src/v2m_particles.cpp:771-820)
+        std::vector<int32_t> id(n), ks(n), cnt(n);
+        TN_HIP(hipMemcpy(id.data(), p->dId, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        TN_HIP(hipMemcpy(ks.data(), p->dKeySorted, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        TN_HIP(hipMemcpy(cnt.data(), p->dNnb, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        std::vector<int32_t> kid(n);
+        for (int t = 0; t < n; ++t) {
+            kid[id[t]] = ks[t];
         }
-        if (sorted) {
-            TN_HIP(hipMemcpy(sorted, p->dSorted, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        if (key) {
+            std::memcpy(key, kid.data(), sizeof(int32_t) * n);
+        }
+        if (sorted) {   // the counting sort's order: by key, ascending id within a cell
+            std::vector<int32_t> o(n);
+            for (int i = 0; i < n; ++i) {
+                o[i] = i;
+            }
+            std::stable_sort(o.begin(), o.end(), [&](int32_t a, int32_t b) { return kid[a] < kid[b]; });
+            std::memcpy(sorted, o.data(), sizeof(int32_t) * n);
         }
         if (cstart) {
             TN_HIP(hipMemcpy(cstart, p->dCstart, sizeof(int32_t) * (p->nkeys + 1), hipMemcpyDeviceToHost));
         }
-        std::vector<int32_t> cnt(n);
-        TN_HIP(hipMemcpy(cnt.data(), p->dNnb, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
         if (nnb) {
-            std::memcpy(nnb, cnt.data(), sizeof(int32_t) * n);
+            for (int t = 0; t < n; ++t) {
+                nnb[id[t]] = cnt[t];
+            }
         }
-        if (nbr) {   // slot-major on the device, node-major (src/v2m_particles.cpp:764) here
-            std::vector<int32_t> t((size_t)n * TN_K);
-            TN_HIP(hipMemcpy(t.data(), p->dNbrT, sizeof(int32_t) * n * TN_K, hipMemcpyDeviceToHost));
-            for (int i = 0; i < n; ++i) {
+        if (nbr) {   // slot-major by slot on the device, node-major by id (src/v2m_particles.cpp:764) here
+            std::vector<int32_t> tr((size_t)n * TN_K);
+            TN_HIP(hipMemcpy(tr.data(), p->dNbrT, sizeof(int32_t) * n * TN_K, hipMemcpyDeviceToHost));
+            for (int t = 0; t < n; ++t) {
+                const size_t i = (size_t)id[t];
                 for (int k = 0; k < TN_K; ++k) {
-                    nbr[(size_t)i * TN_K + k] = k < cnt[i] ? t[(size_t)k * n + i] : -1;
+                    nbr[i * TN_K + k] = k < cnt[t] ? id[tr[(size_t)k * n + t]] : -1;
                 }
             }
         }
@@ -905,7 +966,13 @@ int tn_plan_force(tn_plan* p, float* F) {
                                                                                p->prm.fscale, p->prm.fsurf, n, p->dF);
         TN_HIP(hipGetLastError());
         TN_HIP(hipStreamSynchronize(p->stream));
-        TN_HIP(hipMemcpy(F, p->dF, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        std::vector<float4> Fs(n);
+        std::vector<int32_t> id(n);
+        TN_HIP(hipMemcpy(Fs.data(), p->dF, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        TN_HIP(hipMemcpy(id.data(), p->dId, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        for (int t = 0; t < n; ++t) {   // by node id
+            std::memcpy(F + 4 * (size_t)id[t], &Fs[t], sizeof(float4));
+        }
     });
 }
 

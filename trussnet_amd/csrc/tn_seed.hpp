@@ -133,12 +133,12 @@ __global__ __launch_bounds__(kBlock) void k_seed_classify(Field f, int n, float*
         return;
     }
     const float hs = size_at(f, p.x, p.y, p.z);
-    const float4 g = psi_grad_point(f, a, b, p);
+    const float4 g = psi_grad_point(f.self, a, b, p);
     const float gn = sqrtf(g.x * g.x + g.y * g.y + g.z * g.z);
     if (gn < 1e-9f || fabsf(g.w) / gn > 0.5f * hs) {
         return;
     }
-    const P3ok pr = onto_interface(f, a, b, p, 0.6f * hs);
+    const P3ok pr = onto_interface(f.self, a, b, p, 0.6f * hs);
     if (!pr.ok) {
         return;
     }
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(kBlock) void k_seed_classify(Field f, int n, float*
     }
     int pc_lab = kNoLab;
     if (c != kNoLab && pc > 0.5f * pa) {
-        const P3ok pj = onto_junction(f, a, b, c, q, 0.5f * hs);
+        const P3ok pj = onto_junction(f.self, a, b, c, q, 0.5f * hs);
         if (pj.ok) {
             q = pj.p;
             ty = TN_JUNCTION;
