@@ -60,6 +60,7 @@ def item_bytes(kbar: float, radix_passes: int) -> dict:
         "force": 37 + 4 * kbar,       # P,h 16 + typ 1 + nnb 4 + nbr 4K read; F 16 written
         "move": 104,                  # queue 4, F 16, P,h 16, V 16, typ 1, part 4, lab 2 read; P,h 16, V 16, typ 1,
                                       # part 4, mv 4, pw 4 written
+        "move_rest": 104,
         "stats": 36,                  # mv 4, pw 4, P,h 16, P0 12 read
         "refresh": 47 + 4 * kbar,     # list 4, sorted 4, P,h 16, typ 1, lab 2 read; nbr 4K, nnb 4, P0 16 written
         "neighbors": 23 + 4 * kbar,   # P,h 16, typ 1, lab 2 read; nbr 4K + nnb 4 written
@@ -68,7 +69,6 @@ def item_bytes(kbar: float, radix_passes: int) -> dict:
         "radix_sort": 20 * radix_passes,
         "permute": 134,               # sorted 4, X 16, V 16, typ 1, part 4, lab 2, id 4 read; the same 47 written,
                                       # Ps 16, P0 16, Ms 8, slot_of 4
-        "order": 6,
     }
 
 
@@ -77,10 +77,10 @@ def launch_bytes(name: str, timings: dict, per_item: dict) -> float | None:
     launches, _, items = timings[name]
     if not launches:
         return None
-    if name == "move_easy":   # every node: order 4 + typ 1; finished nodes 94 more; queued ones 4 (the queue)
-        queued = timings["move"][2] / max(timings["move"][0], 1)
+    if name == "move_easy":   # every node: typ 1; finished nodes 94 more; queued ones 4 (the queue)
+        queued = sum(timings[k][2] / max(timings[k][0], 1) for k in ("move", "move_rest") if k in timings)
         n = items / launches
-        return 5 * n + 94 * (n - queued) + 4 * queued
+        return 1 * n + 94 * (n - queued) + 4 * queued
     if name not in per_item:
         return None
     return per_item[name] * items / launches
@@ -88,9 +88,10 @@ def launch_bytes(name: str, timings: dict, per_item: dict) -> float | None:
 
 # the kernels of each timed class (their function names in a rocprofv3 trace)
 CLASS_KERNELS = {
-    "force": ["k_force"], "move_easy": ["k_move_easy"], "move": ["k_move"], "stats": ["k_stats"],
+    "force": ["k_force"], "move_easy": ["k_move_easy"], "move": ["k_move<true>"], "move_rest": ["k_move<false>"],
+    "stats": ["k_stats"],
     "neighbors": ["k_truss"], "refresh": ["k_refresh_list", "k_truss_node"], "permute": ["k_permute"],
-    "keys": ["k_fill_i32", "k_keys"], "order": ["k_order_begin", "k_order_count", "k_order_scatter"],
+    "keys": ["k_fill_i32", "k_keys"],
     "control": ["k_control"],
 }
 

@@ -192,11 +192,11 @@ int tn_plan_reset(tn_plan* plan);
  * plan's stream around each launch (the OpenCL host's V2M_CL_PROFILE timings,
  * src/v2m_gpu.cpp:250-258, without its extra synchronisations). Off by default
  * (on with prm->verbose or TN_KERNEL_TIMING in the environment); switching
- * clears the totals. Kernel classes: keys, scan, radix_sort, gather, neighbors,
- * order (the rebuild), force, move_easy, move, stats, refresh, control (an
+ * clears the totals. Kernel classes: keys, scan, radix_sort, permute, neighbors
+ * (the rebuild), force, move_easy, move, move_rest, stats, refresh, control (an
  * iteration). items: the nodes the timed launches processed (cells for scan;
- * for move, the nodes queued for the full trapped step; for refresh, the listed
- * nodes). The tn_relax_stats ms_* fields are filled from these when timing is on.
+ * for move, the interface nodes queued for the trapped step; for move_rest, the
+ * other queued nodes; for refresh, the listed nodes). The tn_relax_stats ms_* fields are filled from these when timing is on.
  */
 typedef struct tn_kernel_time {
     char name[16];

@@ -13,11 +13,12 @@
 // labels, cells across bricks), taking and returning points by value; the hot
 // paths are inline and share trilinear-cell caches (PsiCell).
 //
-// Layout in HBM (device buffers of a plan):
-//   X[i]    float4 (x, y, z, h at the node)  -- positions and sizes, one load per gather
-//   typ[i]  u8, lab[i] u16, part[i] u32 (partner b | partner c << 16)
-//   nbrT[k*n + i]  int32, slot-major K lists (coalesced per slot), nnb[i] int32
-//   Ps[s]   float4 positions and sizes in cell order (the neighbour search's candidates)
+// Layout in HBM (device buffers of a plan), by node slot t (nodes renumbered in cell
+// order at every rebuild; id[t] is the node's index in the caller's arrays):
+//   X[t]    float4 (x, y, z, h at the node)  -- positions and sizes, one load per gather
+//   typ[t]  u8, lab[t] u16, part[t] u32 (partner b | partner c << 16)
+//   nbrT[k*n + t]  int32 slots, list-position-major K lists (coalesced), nnb[t] int32
+//   Ps[t]   float4 positions and sizes as of the last rebuild (the search's candidates)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -103,19 +104,46 @@ __device__ __forceinline__ int brick_at(const Field& f, int i, int j, int k) {
     return (i >> 3) + f.nbx * ((j >> 3) + f.nby * (k >> 3));
 }
 
-// v2m_slot_of, src/opencl/v2m_grid_body.cl:122-134
-__device__ __forceinline__ int slot_in(const Field& f, int b, int l, int* has) {
-    const int cnt = f.bl_cnt[b];
-    *has = 0;
-    const uint16_t* lb = f.bl_lab + (size_t)b * kBL;
-    for (int s = 0; s < cnt && s < kBL; ++s) {
-        if (lb[s] == l) {
-            *has = 1;
-            const int s0 = f.bl_slot[b];
-            return s0 < 0 ? -1 : s0 + s;
-        }
+// A brick's label list (bl_lab[16 b .. 16 b + 16): ascending, padded), its count
+// and first slot, read with two 16-byte loads and two scalar ones at once rather
+// than label by label.
+struct BrickList {
+    uint4 lo, hi;
+    int cnt, s0;
+};
+
+__device__ __forceinline__ BrickList brick_list(const Field& f, int b) {
+    const uint4* w = reinterpret_cast<const uint4*>(f.bl_lab + (size_t)b * kBL);
+    BrickList r;
+    r.lo = w[0];
+    r.hi = w[1];
+    r.cnt = f.bl_cnt[b];
+    r.s0 = f.bl_slot[b];
+    return r;
+}
+
+// the first position s < min(cnt, 16) of the list holding label l, or -1
+__device__ __forceinline__ int list_find(const BrickList& bl, int l) {
+    const uint32_t w[8] = { bl.lo.x, bl.lo.y, bl.lo.z, bl.lo.w, bl.hi.x, bl.hi.y, bl.hi.z, bl.hi.w };
+    const int n = bl.cnt < kBL ? bl.cnt : kBL;
+    int pos = -1;
+#pragma unroll
+    for (int s = kBL - 1; s >= 0; --s) {
+        const int v = (int)((w[s >> 1] >> ((s & 1) * 16)) & 0xFFFFu);
+        pos = (s < n && v == l) ? s : pos;
     }
-    return -1;
+    return pos;
+}
+
+// v2m_slot_of, src/opencl/v2m_grid_body.cl:122-134, on a loaded list
+__device__ __forceinline__ int list_slot(const BrickList& bl, int l, int* has) {
+    const int pos = list_find(bl, l);
+    *has = pos >= 0;
+    return (pos < 0 || bl.s0 < 0) ? -1 : bl.s0 + pos;
+}
+
+__device__ __forceinline__ int slot_in(const Field& f, int b, int l, int* has) {
+    return list_slot(brick_list(f, b), l, has);
 }
 
 // v2m_phi_vox, src/opencl/v2m_grid_body.cl:137-153
@@ -218,8 +246,9 @@ __device__ __forceinline__ void psi_cell_load(const Field& f, int a, int b, int 
         const int bk = brick_at(f, i0, j0, k0);
         if (bk != pc.bk) {
             pc.bk = bk;
-            pc.sa = slot_in(f, bk, a, &pc.ha);
-            pc.sb = slot_in(f, bk, b, &pc.hb);
+            const BrickList bl = brick_list(f, bk);
+            pc.sa = list_slot(bl, a, &pc.ha);
+            pc.sb = list_slot(bl, b, &pc.hb);
         }
         corners_in_brick(f, a, pc.sa, pc.ha, i0, j0, k0, c);
         corners_in_brick(f, b, pc.sb, pc.hb, i0, j0, k0, e);
@@ -302,6 +331,7 @@ __device__ __noinline__ float4 psi_grad_point(const Field* fp, int a, int b, P3 
 // the one label 0).
 struct BrickLabels {
     int b, n;
+    BrickList list;   // the brick's list (b >= 0)
 };
 
 __device__ __forceinline__ BrickLabels brick_labels_at(const Field& f, P3 p) {
@@ -311,16 +341,27 @@ __device__ __forceinline__ BrickLabels brick_labels_at(const Field& f, P3 p) {
     if (i < 0 || j < 0 || k < 0 || i >= f.nx || j >= f.ny || k >= f.nz) {
         r.b = -1;
         r.n = 1;
+        r.list.lo = r.list.hi = make_uint4(0u, 0u, 0u, 0u);
+        r.list.cnt = 1;
+        r.list.s0 = -1;
         return r;
     }
     r.b = brick_at(f, i, j, k);
-    const int cnt = f.bl_cnt[r.b];
-    r.n = cnt < kBL ? cnt : kBL;
+    r.list = brick_list(f, r.b);
+    r.n = r.list.cnt < kBL ? r.list.cnt : kBL;
     return r;
 }
 
-__device__ __forceinline__ int brick_label(const Field& f, BrickLabels bl, int s) {
-    return bl.b < 0 ? 0 : (int)f.bl_lab[(size_t)bl.b * kBL + s];
+// entry s of a loaded list (a select over its eight words, no memory access)
+__device__ __forceinline__ int list_label(const BrickList& bl, int s) {
+    const int q = s >> 1;
+    const uint4 v = s < 8 ? bl.lo : bl.hi;
+    const uint32_t w = (q & 2) ? ((q & 1) ? v.w : v.z) : ((q & 1) ? v.y : v.x);
+    return (int)((w >> ((s & 1) * 16)) & 0xFFFFu);
+}
+
+__device__ __forceinline__ int brick_label(const Field& f, const BrickLabels& bl, int s) {
+    return bl.b < 0 ? 0 : list_label(bl.list, s);
 }
 
 // The trilinear cell of a point, and whether it lies in one brick that is the brick
@@ -344,7 +385,7 @@ __device__ __forceinline__ PointCell point_cell(const Field& f, P3 p, BrickLabel
     c.fy = v - (float)c.j0;
     c.fz = w - (float)c.k0;
     c.same = bl.b >= 0 && cell_in_one_brick(f, c.i0, c.j0, c.k0) && brick_at(f, c.i0, c.j0, c.k0) == bl.b;
-    c.s0 = c.same ? f.bl_slot[bl.b] : -1;
+    c.s0 = c.same ? bl.list.s0 : -1;
     return c;
 }
 
@@ -644,20 +685,38 @@ __device__ __forceinline__ float4 bar_force(const float4* __restrict__ X, const 
     const int ti = typ[i];
     float F0 = 0.0f, F1 = 0.0f, F2 = 0.0f, F3 = 0.0f;
     const int m = nnb[i];
-    for (int k = 0; k < m; ++k) {
-        const int j = nbrT[(size_t)k * n_nodes + i];
-        const float4 xj = X[j];
-        const float ex = px - xj.x, ey = py - xj.y, ez = pz - xj.z;
-        const float l = sqrtf(ex * ex + ey * ey + ez * ez);
-        const float l0 = ((ti != TN_INTERIOR && typ[j] != TN_INTERIOR) ? fsurf : fscale) * 0.5f * (hi + xj.w);
-        if (l >= l0 || l <= 0.0f) {
-            continue;
+    // eight bars at a time: their loads are issued together, the sum keeps list order
+    // (entries past the list read the node itself and are not summed)
+    constexpr int kB = 8;
+    static_assert(TN_K % kB == 0, "the list length is a multiple of the batch");
+    for (int k0 = 0; k0 < m; k0 += kB) {
+        int j[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            j[u] = k0 + u < m ? nbrT[(size_t)(k0 + u) * n_nodes + i] : i;
         }
-        const float fr = (l0 - l) / l;
-        F0 += fr * ex;
-        F1 += fr * ey;
-        F2 += fr * ez;
-        F3 += 1.0f;
+        float4 xj[kB];
+        int tj[kB];
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            xj[u] = X[j[u]];
+            tj[u] = typ[j[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < kB; ++u) {
+            if (k0 + u < m) {
+                const float ex = px - xj[u].x, ey = py - xj[u].y, ez = pz - xj[u].z;
+                const float l = sqrtf(ex * ex + ey * ey + ez * ez);
+                const float l0 = ((ti != TN_INTERIOR && tj[u] != TN_INTERIOR) ? fsurf : fscale) * 0.5f * (hi + xj[u].w);
+                if (!(l >= l0 || l <= 0.0f)) {
+                    const float fr = (l0 - l) / l;
+                    F0 += fr * ex;
+                    F1 += fr * ey;
+                    F2 += fr * ez;
+                    F3 += 1.0f;
+                }
+            }
+        }
     }
     return make_float4(F0, F1, F2, F3);
 }
@@ -759,6 +818,8 @@ __device__ __forceinline__ int labels_dominate(const Field& f, int a, int b, int
     return labels_dominate_near(f.self, a, b, c, q, bl);
 }
 
+constexpr float kDeferred = -1.0f;   // trapped_step<true>: the node needs the general path
+
 __device__ __forceinline__ float dist2(P3 q, P3 p) {
     return (q.x - p.x) * (q.x - p.x) + (q.y - p.y) * (q.y - p.y) + (q.z - p.z) * (q.z - p.z);
 }
@@ -774,16 +835,21 @@ struct NodeState {
 // v2m_move_step (smooth mode), src/opencl/v2m_particle_body.cl:659-833: apply the
 // capped step s with interface trapping. Returns |dp| / h (0: the node stayed;
 // its type and partners may still have changed, as in the reference).
+// kInterface: the node is known to be an interface node; the other branches are
+// compiled out, and so is the rare path (a third label strong enough for a junction
+// where the node lands): the step then returns kDeferred and leaves ns untouched,
+// and the caller hands the node to the general instance.
+template <bool kInterface>
 __device__ __forceinline__ float trapped_step(const Field& f, float h, P3 s, float maxstep, float snap, int a,
                                               NodeState& ns) {
     const P3 p = ns.p;
     P3 q;
-    const int typ0 = ns.ty;   // typ[i] as stored until the final write
-    int ty = ns.ty;
+    const int typ0 = kInterface ? TN_INTERFACE : ns.ty;   // typ[i] as stored until the final write
+    int ty = typ0;
     int b = ns.b;             // the locals b, c of the reference; ns.b / ns.c are the stored part[]
     const int c = ns.c;
 
-    if (ty == TN_INTERIOR) {
+    if (!kInterface && ty == TN_INTERIOR) {
         q = p3(p.x + s.x, p.y + s.y, p.z + s.z);
         const int2 own = owner_at(f, q);
         const int l = own.x, sec = own.y;
@@ -814,7 +880,7 @@ __device__ __forceinline__ float trapped_step(const Field& f, float h, P3 s, flo
                 }
             }
         }
-    } else if (ty == TN_INTERFACE) {
+    } else if (kInterface || ty == TN_INTERFACE) {
         PsiCell pc;   // the glide and its projection probe the same few cells
         psi_cell_init(pc);
         s = drop_normal_at(f, a, b, p, s, pc);
@@ -824,6 +890,52 @@ __device__ __forceinline__ float trapped_step(const Field& f, float h, P3 s, flo
             return 0.0f;
         }
         q = pr.p;
+        if (kInterface) {
+            // The general tail below for an interface node, inline: one pass over the
+            // labels of q's brick evaluates both v2m_third_label (competing_label_near)
+            // and v2m_valid_on (labels_dominate_near) at q, with the same values. A
+            // third label found (the node would try to join a junction) hands the node
+            // back for the general path. Otherwise the step fails if it is too long or
+            // another label dominates at q (the second third-label search then finds
+            // none either: same labels, same point), and ends here if not.
+            const BrickLabels bl = brick_labels_at(f, q);
+            int other = 0;
+            for (int s = 0; s < bl.n; ++s) {
+                const int l = brick_label(f, bl, s);
+                other |= l != a && l != b;
+            }
+            bool dominated = false;
+            if (other) {
+                const PointCell cell = point_cell(f, q, bl);
+                const float pa = phi_point(f, a, q), pb = phi_point(f, b, q);
+                const float mine = fminf(pa, pb);
+                float pc = 0.5f * pa;
+                int cc = kNoLab;
+                for (int s = 0; s < bl.n; ++s) {
+                    const int l = brick_label(f, bl, s);
+                    if (l != a && l != b) {
+                        const float w = label_phi(f, cell, s, l);
+                        if (w > pc) {
+                            pc = w;
+                            cc = l;
+                        }
+                        dominated = dominated || w > mine + 0.02f;
+                    }
+                }
+                if (cc != kNoLab) {
+                    return kDeferred;
+                }
+            }
+            const float lim = (maxstep + fmaxf(snap, 0.5f)) * h;
+            const float m2 = dist2(q, p);
+            if (!(m2 <= lim * lim) || dominated) {
+                return 0.0f;
+            }
+            ns.p = q;
+            ns.ty = TN_INTERFACE;
+            ns.hn = size_at(f, q.x, q.y, q.z);
+            return sqrtf(m2) / h;
+        }
     } else {
         const float4 g1 = psi_grad_point(f.self, a, b, p);
         const float4 g2 = psi_grad_point(f.self, a, c, p);

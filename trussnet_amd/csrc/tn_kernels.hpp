@@ -3,7 +3,7 @@
 // Rebuild (src/v2m_particles.cpp:771-820): k_keys (cell key + count per cell),
 // k_scan_* (exclusive scan: cell starts), k_radix_* (stable LSD radix sort of the
 // nodes by cell key), k_permute (the nodes renumbered in cell order), k_truss
-// (K-nearest lists, one wave per 64 nodes), k_order_* (move order grouped by type).
+// (K-nearest lists, one wave per 64 nodes).
 // Iteration (src/v2m_particles.cpp:828-947): k_force, k_move_easy + k_move,
 // k_stats (histogram, max, Verlet count, refresh flags, FIRE power partial sums),
 // k_refresh_list + k_truss_node (own-list refresh of the nodes past a full skin),
@@ -40,15 +40,15 @@ struct Ctl {
     int32_t power_fallbacks;   // iterations whose power sign needed the serial sum
     int32_t nrefresh;      // nodes queued for an own-list refresh this iteration
     int32_t nhalf;         // nodes past skin/2 (this iteration)
-    int32_t nmove;         // nodes queued for the full trapped step (this iteration)
+    int32_t nmove;         // interface nodes queued for the trapped step (this iteration)
+    int32_t nmove_rest;    // other nodes queued for the full trapped step (this iteration)
     // FIRE (FireCtl, src/v2m_particles.h:47-76)
     float dt0, dt, dtmax, alpha;
     int32_t npos;
     // accumulators, cleared by k_control
     uint32_t hist[32];
     uint32_t mmax_bits;
-    int32_t order_cnt[4];
-    int32_t order_cur[4];
+    int32_t order_cnt[4];  // node counts by type rank (interface, junction, interior, corner)
     double power;
 };
 
@@ -164,17 +164,29 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const int32_t* in, int32_t
 
 // ---- keys (v2m_h_at + v2m_level_of + v2m_bin_key; count per cell) ---------------
 
-__global__ __launch_bounds__(kBlock) void k_keys(Field f, Hash H, const float4* __restrict__ X, int n, float tskin,
+// X.w is v2m_h_at at the node (set with every move), the size the rebuild takes.
+// The nodes sit in cell order from the previous rebuild, so neighbouring lanes
+// mostly share a key: each run of equal keys in a wave adds its length at once.
+__global__ __launch_bounds__(kBlock) void k_keys(Hash H, const float4* __restrict__ X, int n, float tskin,
                                                  int32_t* __restrict__ key, int32_t* __restrict__ cnt) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) {
-        return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool valid = i < n;
+    int k = -1;
+    if (valid) {
+        const float4 x = X[i];
+        k = cell_key(H, level_for(H, tskin * x.w), x.x, x.y, x.z);
+        key[i] = k;
     }
-    const float4 x = X[i];
-    const float h = size_at(f, x.x, x.y, x.z);
-    const int k = cell_key(H, level_for(H, tskin * h), x.x, x.y, x.z);
-    key[i] = k;
-    atomicAdd(&cnt[k], 1);
+    const int prev = __shfl_up(k, 1, kWave);
+    const bool head = valid && (lane == 0 || prev != k);
+    const uint64_t heads = __ballot(head);
+    const int nvalid = __popcll(__ballot(valid));   // the valid lanes are a prefix
+    if (head) {
+        const uint64_t later = lane == kWave - 1 ? 0ull : heads & (~0ull << (lane + 1));
+        const int end = later ? __ffsll((long long)later) - 1 : nvalid;
+        atomicAdd(&cnt[k], end - lane);
+    }
 }
 
 // ---- stable LSD radix sort of (key, id) by key, 8-bit digits --------------------------
@@ -701,7 +713,7 @@ __global__ __launch_bounds__(kBlock) void k_refresh_list(const uint8_t* __restri
     }
 }
 
-// ---- move order grouped by type: interface, junction, interior, corner ---------------
+// ---- node counts by type: interface, junction, interior, corner ------------------------
 
 __device__ __forceinline__ int type_rank(int t) {
     return t == TN_INTERFACE ? 0 : (t == TN_JUNCTION ? 1 : (t == TN_INTERIOR ? 2 : 3));
@@ -736,47 +748,6 @@ __global__ __launch_bounds__(kBlock) void k_order_count(const uint8_t* __restric
     __syncthreads();
     if (threadIdx.x < 4 && c[threadIdx.x]) {
         atomicAdd(&ctl->order_cnt[threadIdx.x], c[threadIdx.x]);
-    }
-}
-
-// (the order within a type group is not deterministic; it only steers performance:
-// the move of node i touches node i alone)
-__global__ __launch_bounds__(kBlock) void k_order_scatter(const uint8_t* __restrict__ typ, int n,
-                                                          Ctl* __restrict__ ctl, int32_t* __restrict__ order) {
-    __shared__ int c[4], b[4];
-    __shared__ int woff[kBlock / kWave][4];
-    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
-    if (threadIdx.x < 4) {
-        c[threadIdx.x] = 0;
-    }
-    __syncthreads();
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    const bool valid = i < n;
-    const int r = valid ? type_rank(typ[i] & 3) : 0;
-    uint64_t m[4];
-    rank_masks(valid, r, m);
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            woff[w][k] = m[k] ? atomicAdd(&c[k], __popcll(m[k])) : 0;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        int start = 0;
-        for (int k = 0; k < (int)threadIdx.x; ++k) {
-            start += ctl->order_cnt[k];
-        }
-        b[threadIdx.x] = c[threadIdx.x] ? start + atomicAdd(&ctl->order_cur[threadIdx.x], c[threadIdx.x]) : 0;
-    }
-    __syncthreads();
-    if (valid) {
-        uint64_t mr = m[0];
-#pragma unroll
-        for (int k = 1; k < 4; ++k) {
-            mr = r == k ? m[k] : mr;
-        }
-        order[b[r] + woff[w][r] + __popcll(mr & lanemask_lt())] = i;
     }
 }
 
@@ -886,26 +857,43 @@ __device__ __forceinline__ void move_store(const MoveArgs& ma, const Ctl* __rest
     mv[i] = m;
 }
 
-// Pass 1 of the move, over every node in type order: corners, settled nodes and
+// Pass 1 of the move, over every node in slot order: corners, settled nodes and
 // interior nodes whose step stays in their own region with no interface within
 // snap range are finished here, exactly as v2m_move finishes them. Every other node
 // (interface and junction nodes, interior nodes that cross or may snap) is queued,
 // untouched, for k_move. Most nodes are interior, and this pass needs a fraction
 // of the registers of the full trapped step.
-__global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl* __restrict__ ctl,
-                                                      const int32_t* __restrict__ order, int n,
+// append i to queue for the lanes with flag set (one atomic on *count per wave)
+__device__ __forceinline__ void wave_append(bool flag, int i, int32_t* count, int32_t* __restrict__ queue) {
+    const uint64_t m = __ballot(flag);
+    if (m) {
+        const int lead = __ffsll((long long)m) - 1;
+        int base = 0;
+        if ((int)(threadIdx.x & (kWave - 1)) == lead) {
+            base = atomicAdd(count, __popcll(m));
+        }
+        base = __shfl(base, lead, kWave);
+        if (flag) {
+            queue[base + __popcll(m & lanemask_lt())] = i;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl* __restrict__ ctl, int n,
                                                       const float4* __restrict__ F, float4* __restrict__ X,
                                                       float4* __restrict__ V, uint8_t* __restrict__ typ,
                                                       uint32_t* __restrict__ part, const uint16_t* __restrict__ lab,
                                                       float* __restrict__ mv, float* __restrict__ pw,
-                                                      int32_t* __restrict__ queue) {
+                                                      int32_t* __restrict__ queue, int32_t* __restrict__ queue_rest) {
     const int g = blockIdx.x * kBlock + threadIdx.x;
     int i = 0;
-    bool hard = false;
+    bool surf = false, hard = false;
     if (g < n) {
-        i = order[g];
+        i = g;
         const int ty0 = typ[i];
-        if (ty0 == TN_INTERFACE || ty0 == TN_JUNCTION) {
+        if (ty0 == TN_INTERFACE) {
+            surf = true;
+        } else if (ty0 == TN_JUNCTION) {
             hard = true;
         } else if (ty0 == TN_CORNER) {
             if (ma.fire) {
@@ -959,19 +947,9 @@ __global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl*
             }
         }
     }
-    // queue the rest, one atomic per wave
-    const uint64_t hm = __ballot(hard);
-    if (hm) {
-        const int lead = __ffsll((long long)hm) - 1;
-        int base = 0;
-        if ((int)(threadIdx.x & (kWave - 1)) == lead) {
-            base = atomicAdd(&ctl->nmove, __popcll(hm));
-        }
-        base = __shfl(base, lead, kWave);
-        if (hard) {
-            queue[base + __popcll(hm & lanemask_lt())] = i;
-        }
-    }
+    // queue the rest, one atomic per wave and queue
+    wave_append(surf, i, &ctl->nmove, queue);
+    wave_append(hard, i, &ctl->nmove_rest, queue_rest);
 }
 
 // waves per SIMD the full move kernel is compiled for (its register budget: 512 / waves)
@@ -979,23 +957,16 @@ __global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl*
 #define TN_MOVE_WAVES 1
 #endif
 
-// Pass 2: v2m_move (src/opencl/v2m_particle_body.cl:844-935) for the ctl->nmove
-// queued nodes, with the whole trapped step.
-__global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArgs ma, const Ctl* __restrict__ ctl,
-                                                                const int32_t* __restrict__ queue,
-                                                                const float4* __restrict__ F, float4* __restrict__ X,
-                                                                float4* __restrict__ V, uint8_t* __restrict__ typ,
-                                                                uint32_t* __restrict__ part,
-                                                                const uint16_t* __restrict__ lab,
-                                                                float* __restrict__ mv, float* __restrict__ pw) {
-    const int g = blockIdx.x * kBlock + threadIdx.x;
-    if (g >= ctl->nmove) {
-        return;
-    }
-    const int i = queue[g];
+// v2m_move for queued node i; true: handed back (kInterface, a third label near)
+template <bool kInterface>
+__device__ __forceinline__ bool move_node(const Field& f, const MoveArgs& ma, const Ctl* __restrict__ ctl, int i,
+                                          const float4* __restrict__ F, float4* __restrict__ X,
+                                          float4* __restrict__ V, uint8_t* __restrict__ typ,
+                                          uint32_t* __restrict__ part, const uint16_t* __restrict__ lab,
+                                          float* __restrict__ mv, float* __restrict__ pw) {
     const float4 x = X[i];
     const float h = x.w;   // h at the node: v2m_h_at(P_i), kept current by every move
-    const int ty0 = typ[i];
+    const int ty0 = kInterface ? TN_INTERFACE : typ[i];
     const uint32_t pp = part[i];
     NodeState ns;
     ns.p = p3(x.x, x.y, x.z);
@@ -1006,10 +977,49 @@ __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArg
     float m = 0.0f;
     const NodeStep st = node_step(ma, ctl, F[i], V, i, h);   // queued nodes are never corners
     if (st.moves) {
-        m = trapped_step(f, h, p3(st.s[0], st.s[1], st.s[2]), ma.maxstep, ma.snap, lab[i], ns);
+        m = trapped_step<kInterface>(f, h, p3(st.s[0], st.s[1], st.s[2]), ma.maxstep, ma.snap, lab[i], ns);
+        if (kInterface && m == kDeferred) {
+            return true;
+        }
     }
     move_store(ma, ctl, i, x, ty0, pp, ns, m, st.pw, X, V, typ, part, mv, pw);
+    return false;
 }
+
+// Pass 2: v2m_move (src/opencl/v2m_particle_body.cl:844-935) for the queued nodes.
+// kInterface: the ctl->nmove interface nodes, with the interface branch of the
+// trapped step only (no out-of-line calls, whose register budgets would bound the
+// kernel's occupancy); a node with a third label near is handed to the rest queue.
+// Otherwise: the ctl->nmove_rest others, with the whole trapped step; this launch
+// runs after the interface one, so it sees the handed-back nodes.
+template <bool kInterface>
+__global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArgs ma, Ctl* __restrict__ ctl,
+                                                                const int32_t* __restrict__ queue,
+                                                                const float4* __restrict__ F, float4* __restrict__ X,
+                                                                float4* __restrict__ V, uint8_t* __restrict__ typ,
+                                                                uint32_t* __restrict__ part,
+                                                                const uint16_t* __restrict__ lab,
+                                                                float* __restrict__ mv, float* __restrict__ pw,
+                                                                int32_t* __restrict__ queue_rest) {
+    // a fixed grid strides over the queue, whose length only the device knows
+    const int count = kInterface ? ctl->nmove : ctl->nmove_rest;
+    for (int base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
+        const int g = base + threadIdx.x;
+        bool defer = false;
+        int i = 0;
+        if (g < count) {
+            i = queue[g];
+            defer = move_node<kInterface>(f, ma, ctl, i, F, X, V, typ, part, lab, mv, pw);
+        }
+        if (kInterface) {   // every lane of the wave is here
+            wave_append(defer, i, &ctl->nmove_rest, queue_rest);
+        }
+    }
+}
+
+constexpr int kMoveGrid = 2048;      // blocks of k_move<true> at most
+constexpr int kMoveRestGrid = 256;   // blocks of k_move<false> at most (a few thousand nodes)
+
 
 // ---- per-iteration reductions + Verlet bookkeeping ----------------------------------
 
@@ -1032,6 +1042,7 @@ __device__ __forceinline__ int hist_bin(float m, const float* __restrict__ thr) 
 }
 
 constexpr int kHistThr = 35;   // exponents -21..13
+constexpr int kStatsBlocks = 2048;   // k_stats' grid at most (8 blocks per CU)
 
 // Per iteration: the histogram, the largest move, the count past skin/2, the
 // flags of the nodes past a full skin (their lists are refreshed), and FIRE's
@@ -1054,58 +1065,68 @@ __global__ __launch_bounds__(kBlock) void k_stats(const float4* __restrict__ X, 
         lmax = 0;
         lhalf = 0;
     }
-    __syncthreads();
-    const int i = blockIdx.x * kBlock + tid;
-    double ps = 0.0, pa = 0.0;
-    int bin = -1;
-    uint32_t mbits = 0;
-    bool half = false;
-    if (i < n) {
-        const float m = mv[i];
-        bin = hist_bin(m, thr);
-        mbits = __float_as_uint(m);
-        const float4 x = X[i], x0 = P0[i];
-        const float ex = x.x - x0.x, ey = x.y - x0.y, ez = x.z - x0.z;
-        const float h = x.w;
-        const float m2 = ex * ex + ey * ey + ez * ez, s2 = skin * skin * h * h;
-        const bool far = m2 > s2;
-        refresh[i] = far ? 1 : 0;
-        half = !far && m2 > 0.25f * s2;
-        if (fire) {
-            ps = (double)pw[i];
-            pa = fabs(ps);
+    double bs = 0.0, ba = 0.0;   // this block's power partial sums (thread 0), chunk by chunk
+    // chunks of kBlock nodes, grid-strided: each block adds its counts to the control
+    // block once, and its partial sums have a fixed shape
+    for (int c = blockIdx.x; (int64_t)c * kBlock < n; c += gridDim.x) {
+        __syncthreads();   // lps of the previous chunk has been read
+        const int i = c * kBlock + tid;
+        double ps = 0.0, pa = 0.0;
+        int bin = -1;
+        uint32_t mbits = 0;
+        bool half = false;
+        if (i < n) {
+            const float m = mv[i];
+            bin = hist_bin(m, thr);
+            mbits = __float_as_uint(m);
+            const float4 x = X[i], x0 = P0[i];
+            const float ex = x.x - x0.x, ey = x.y - x0.y, ez = x.z - x0.z;
+            const float h = x.w;
+            const float m2 = ex * ex + ey * ey + ez * ez, s2 = skin * skin * h * h;
+            const bool far = m2 > s2;
+            refresh[i] = far ? 1 : 0;
+            half = !far && m2 > 0.25f * s2;
+            if (fire) {
+                ps = (double)pw[i];
+                pa = fabs(ps);
+            }
         }
-    }
-    // histogram: one LDS add per (wave, occupied bin)
-    uint64_t left = __ballot(bin >= 0);
-    while (left) {
-        const int b = __shfl(bin, __ffsll((long long)left) - 1, kWave);
-        const uint64_t same = __ballot(bin == b);
-        if (lane == 0) {
-            atomicAdd(&lh[b], (uint32_t)__popcll(same));
+        // histogram: one LDS add per (wave, occupied bin)
+        uint64_t left = __ballot(bin >= 0);
+        while (left) {
+            const int b = __shfl(bin, __ffsll((long long)left) - 1, kWave);
+            const uint64_t same = __ballot(bin == b);
+            if (lane == 0) {
+                atomicAdd(&lh[b], (uint32_t)__popcll(same));
+            }
+            left &= ~same;
         }
-        left &= ~same;
-    }
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) {
-        mbits = max(mbits, (uint32_t)__shfl_xor((int)mbits, d, kWave));
-    }
-    const int nh = __popcll(__ballot(half));
-    if (lane == 0) {
-        atomicMax(&lmax, mbits);
-        if (nh) {
-            atomicAdd(&lhalf, nh);
-        }
-    }
-    if (fire) {   // fixed-shape tree: the partial sums are reproducible
 #pragma unroll
         for (int d = kWave / 2; d > 0; d >>= 1) {
-            ps += __shfl_xor(ps, d, kWave);
-            pa += __shfl_xor(pa, d, kWave);
+            mbits = max(mbits, (uint32_t)__shfl_xor((int)mbits, d, kWave));
         }
+        const int nh = __popcll(__ballot(half));
         if (lane == 0) {
-            lps[tid >> 6][0] = ps;
-            lps[tid >> 6][1] = pa;
+            atomicMax(&lmax, mbits);
+            if (nh) {
+                atomicAdd(&lhalf, nh);
+            }
+        }
+        if (fire) {   // fixed-shape tree: the partial sums are reproducible
+#pragma unroll
+            for (int d = kWave / 2; d > 0; d >>= 1) {
+                ps += __shfl_xor(ps, d, kWave);
+                pa += __shfl_xor(pa, d, kWave);
+            }
+            if (lane == 0) {
+                lps[tid >> 6][0] = ps;
+                lps[tid >> 6][1] = pa;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                bs += (lps[0][0] + lps[1][0]) + (lps[2][0] + lps[3][0]);
+                ba += (lps[0][1] + lps[1][1]) + (lps[2][1] + lps[3][1]);
+            }
         }
     }
     __syncthreads();
@@ -1118,9 +1139,7 @@ __global__ __launch_bounds__(kBlock) void k_stats(const float4* __restrict__ X, 
             atomicAdd(&ctl->nhalf, lhalf);
         }
         if (fire) {
-            const double s = (lps[0][0] + lps[1][0]) + (lps[2][0] + lps[3][0]);
-            const double a = (lps[0][1] + lps[1][1]) + (lps[2][1] + lps[3][1]);
-            partials[blockIdx.x] = make_double2(s, a);
+            partials[blockIdx.x] = make_double2(bs, ba);
         }
     }
 }
@@ -1248,13 +1267,13 @@ __global__ void k_iter_begin(Ctl* __restrict__ ctl) {
         ctl->nhalf = 0;
         ctl->nrefresh = 0;
         ctl->nmove = 0;
+        ctl->nmove_rest = 0;
     }
 }
 
 __global__ void k_order_begin(Ctl* __restrict__ ctl) {
     if (threadIdx.x < 4) {
         ctl->order_cnt[threadIdx.x] = 0;
-        ctl->order_cur[threadIdx.x] = 0;
     }
 }
 

@@ -205,10 +205,10 @@ tnd::Field upload_grid(const tn_grid_view* g, DevMem& mem) {
 }
 
 // Kernel classes of the per-kernel device timings.
-enum KClass { KC_KEYS, KC_SCAN, KC_RADIX, KC_PERMUTE, KC_NEIGHBORS, KC_ORDER, KC_FORCE, KC_MOVE_EASY, KC_MOVE,
+enum KClass { KC_KEYS, KC_SCAN, KC_RADIX, KC_PERMUTE, KC_NEIGHBORS, KC_FORCE, KC_MOVE_EASY, KC_MOVE, KC_MOVE_REST,
               KC_STATS, KC_REFRESH, KC_CONTROL, KC_COUNT };
-const char* const kKernelNames[KC_COUNT] = { "keys", "scan", "radix_sort", "permute", "neighbors", "order",
-                                             "force", "move_easy", "move", "stats", "refresh", "control" };
+const char* const kKernelNames[KC_COUNT] = { "keys", "scan", "radix_sort", "permute", "neighbors",
+                                             "force", "move_easy", "move", "move_rest", "stats", "refresh", "control" };
 
 // Device time per kernel class from event pairs on one stream. Elapsed times are
 // read only after the stream has been synchronised (collect), so timing adds
@@ -310,7 +310,7 @@ struct tn_plan {
     const int32_t* dKeySorted = nullptr;   // the keys in slot order after the last rebuild
     float *dMv = nullptr, *dPw = nullptr, *dThr = nullptr;
     int32_t *dKey = nullptr, *dCstart = nullptr, *dSorted = nullptr, *dNbrT = nullptr, *dNnb = nullptr;
-    int32_t *dOrder = nullptr, *dQueue = nullptr, *dMoveQ = nullptr, *dBhist = nullptr, *dBsum = nullptr;
+    int32_t *dQueue = nullptr, *dMoveQ = nullptr, *dMoveQ2 = nullptr, *dBhist = nullptr, *dBsum = nullptr;
     uint8_t* dFlag = nullptr;   // nodes past a full skin this iteration
     int32_t* dRk[2] = { nullptr, nullptr };
     int32_t* dRv[2] = { nullptr, nullptr };
@@ -369,7 +369,7 @@ struct tn_plan {
         const unsigned nb = blocks_for(n, tnk::kBlock);
         hipEvent_t t = timer.begin();
         tnk::k_fill_i32<<<blocks_for(nkeys + 1, tnk::kBlock), tnk::kBlock, 0, stream>>>(dCstart, nkeys + 1, 0);
-        tnk::k_keys<<<nb, tnk::kBlock, 0, stream>>>(f, H, dX, n, tskin, dKey, dCstart);
+        tnk::k_keys<<<nb, tnk::kBlock, 0, stream>>>(H, dX, n, tskin, dKey, dCstart);
         timer.end(KC_KEYS, t);
         t = timer.begin();
         scan(dCstart, dCstart, nkeys + 1);
@@ -403,13 +403,8 @@ struct tn_plan {
         t = timer.begin();
         tnk::k_truss<<<blocks_for(n, tnk::kTrussWave), tnk::kTrussWave, 0, stream>>>(H, truss_in(), n, dNbrT, dNnb);
         timer.end(KC_NEIGHBORS, t);
-        t = timer.begin();
-        tnk::k_order_begin<<<1, 64, 0, stream>>>(dCtl);
-        tnk::k_order_count<<<nb, tnk::kBlock, 0, stream>>>(dTyp, n, dCtl);
-        tnk::k_order_scatter<<<nb, tnk::kBlock, 0, stream>>>(dTyp, n, dCtl, dOrder);
-        timer.end(KC_ORDER, t);
         TN_HIP(hipGetLastError());
-        for (int kc : { KC_KEYS, KC_RADIX, KC_PERMUTE, KC_NEIGHBORS, KC_ORDER }) {
+        for (int kc : { KC_KEYS, KC_RADIX, KC_PERMUTE, KC_NEIGHBORS }) {
             timer.count(kc, n);
         }
         timer.count(KC_SCAN, nkeys + 1);
@@ -453,15 +448,21 @@ struct tn_plan {
         timer.end(KC_FORCE, t);
         t = timer.begin();
         tnk::MoveArgs ma{ prm.dt, prm.maxstep, prm.snap, fire };
-        tnk::k_move_easy<<<nb, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dOrder, n, dF, dX, dV, dTyp, dPart, dLab, dMv,
-                                                         dPw, dMoveQ);
+        tnk::k_move_easy<<<nb, tnk::kBlock, 0, stream>>>(f, ma, dCtl, n, dF, dX, dV, dTyp, dPart, dLab, dMv,
+                                                         dPw, dMoveQ, dMoveQ2);
         timer.end(KC_MOVE_EASY, t);
         t = timer.begin();
-        tnk::k_move<<<nb, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQ, dF, dX, dV, dTyp, dPart, dLab, dMv, dPw);
+        const unsigned g1 = std::min<unsigned>(nb, tnk::kMoveGrid), g2 = std::min<unsigned>(nb, tnk::kMoveRestGrid);
+        tnk::k_move<true><<<g1, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQ, dF, dX, dV, dTyp, dPart, dLab, dMv, dPw,
+                                                          dMoveQ2);
         timer.end(KC_MOVE, t);
         t = timer.begin();
-        tnk::k_stats<<<nb, tnk::kBlock, 0, stream>>>(dX, dP0, dMv, dPw, n, prm.skin, fire, dThr, dCtl, dFlag,
-                                                     dPartials);
+        tnk::k_move<false><<<g2, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQ2, dF, dX, dV, dTyp, dPart, dLab, dMv,
+                                                           dPw, nullptr);
+        timer.end(KC_MOVE_REST, t);
+        t = timer.begin();
+        tnk::k_stats<<<nparts, tnk::kBlock, 0, stream>>>(dX, dP0, dMv, dPw, n, prm.skin, fire, dThr, dCtl, dFlag,
+                                                         dPartials);
         timer.end(KC_STATS, t);
         t = timer.begin();
         tnk::k_refresh_list<<<nb, tnk::kBlock, 0, stream>>>(dFlag, n, dCtl, dQueue);
@@ -479,6 +480,7 @@ struct tn_plan {
             timer.count(kc, n);
         }
         timer.count(KC_MOVE, hCtl->nmove);
+        timer.count(KC_MOVE_REST, hCtl->nmove_rest);
         timer.count(KC_REFRESH, hCtl->nrefresh);
         timer.count(KC_CONTROL, 1);
         if (hCtl->done) {
@@ -514,9 +516,9 @@ struct tn_plan {
         st->fire_resets = c.fire_resets;
         st->fire_dt = c.fire_dt;
         const double* ms = timer.ms;
-        st->ms_hash = ms[KC_KEYS] + ms[KC_SCAN] + ms[KC_RADIX] + ms[KC_PERMUTE] + ms[KC_NEIGHBORS] + ms[KC_ORDER];
+        st->ms_hash = ms[KC_KEYS] + ms[KC_SCAN] + ms[KC_RADIX] + ms[KC_PERMUTE] + ms[KC_NEIGHBORS];
         st->ms_force = ms[KC_FORCE];
-        st->ms_move = ms[KC_MOVE_EASY] + ms[KC_MOVE] + ms[KC_STATS] + ms[KC_REFRESH] + ms[KC_CONTROL];
+        st->ms_move = ms[KC_MOVE_EASY] + ms[KC_MOVE] + ms[KC_MOVE_REST] + ms[KC_STATS] + ms[KC_REFRESH] + ms[KC_CONTROL];
         if (n > 0) {   // type counts (src/v2m_particles.cpp:968-975), counted on the device
             tnk::k_order_begin<<<1, 64, 0, stream>>>(dCtl);
             tnk::k_order_count<<<blocks_for(n, tnk::kBlock), tnk::kBlock, 0, stream>>>(dTyp, n, dCtl);
@@ -749,9 +751,9 @@ int tn_plan_create(const tn_grid_view* g, const tn_relax_params* prm, const tn_n
         p->dSorted = p->alloc<int32_t>(n);
         p->dNbrT = p->alloc<int32_t>((size_t)n * TN_K);
         p->dNnb = p->alloc<int32_t>(n);
-        p->dOrder = p->alloc<int32_t>(n);
         p->dQueue = p->alloc<int32_t>(n);
         p->dMoveQ = p->alloc<int32_t>(n);
+        p->dMoveQ2 = p->alloc<int32_t>(n);
         p->dFlag = p->alloc<uint8_t>(n);
         p->dCstart = p->alloc<int32_t>((size_t)p->nkeys + 1);
         for (int k = 0; k < 2; ++k) {
@@ -765,8 +767,8 @@ int tn_plan_create(const tn_grid_view* g, const tn_relax_params* prm, const tn_n
         p->dBhist = p->alloc<int32_t>((size_t)256 * p->radix_nblk);
         const int64_t scan_max = std::max<int64_t>(p->nkeys + 1, (int64_t)256 * p->radix_nblk);
         p->dBsum = p->alloc<int32_t>(blocks_for(scan_max, tnk::kScanTile));
-        p->nparts = (int)blocks_for(n, tnk::kBlock);
-        p->refresh_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(n, 8192));   // one wave per listed node
+        p->nparts = (int)std::max<unsigned>(1, std::min<unsigned>(blocks_for(n, tnk::kBlock), tnk::kStatsBlocks));
+        p->refresh_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(n, 2048));   // one wave per listed node
         p->dPartials = p->alloc<double2>(std::max(p->nparts, 1));
         p->dThr = p->alloc<float>(tnk::kHistThr);
         p->dCtl = p->alloc<tnk::Ctl>(1);
