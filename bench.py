@@ -253,7 +253,8 @@ def main() -> int:
             traffic = rd + wr
             log(f"traffic of {dom}: {rd / 1e6:.1f} MB read + {wr / 1e6:.1f} MB written per launch "
                 f"(calibration: read x{fr:.3f}, write x{fw:.3f})")
-    roofline = {"bound": "hbm", "kernel": f"k_{dom}", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+    kname = "+".join(CLASS_KERNELS.get(dom, [f"k_{dom}"]))
+    roofline = {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                 "bytes_per_launch": nbytes, "avg_launch_us": avg_s * 1e6,
                 "items_per_launch": timings[dom][2] / cnt}

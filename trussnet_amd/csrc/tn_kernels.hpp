@@ -952,6 +952,11 @@ __global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl*
     wave_append(hard, i, &ctl->nmove_rest, queue_rest);
 }
 
+#ifndef TN_REST_LANES
+#define TN_REST_LANES 4
+#endif
+constexpr int kRestLanes = TN_REST_LANES;   // nodes per wave in k_move<false>
+
 // waves per SIMD the full move kernel is compiled for (its register budget: 512 / waves)
 #ifndef TN_MOVE_WAVES
 #define TN_MOVE_WAVES 1
@@ -1001,13 +1006,20 @@ __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArg
                                                                 const uint16_t* __restrict__ lab,
                                                                 float* __restrict__ mv, float* __restrict__ pw,
                                                                 int32_t* __restrict__ queue_rest) {
-    // a fixed grid strides over the queue, whose length only the device knows
+    // a fixed grid strides over the queue, whose length only the device knows.
+    // kInterface: kBlock nodes per block. Otherwise kRestLanes nodes per wave: these
+    // few thousand nodes follow long, divergent paths (junction projections, segment
+    // crossings), and a wave executes every path its lanes take, so spreading them
+    // thinly over many waves shortens the longest wave.
     const int count = kInterface ? ctl->nmove : ctl->nmove_rest;
-    for (int base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
-        const int g = base + threadIdx.x;
+    const int per_block = kInterface ? kBlock : kRestLanes * (kBlock / kWave);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int slot = kInterface ? (int)threadIdx.x : (lane < kRestLanes ? (threadIdx.x >> 6) * kRestLanes + lane : -1);
+    for (int base = blockIdx.x * per_block; base < count; base += gridDim.x * per_block) {
+        const int g = base + slot;
         bool defer = false;
         int i = 0;
-        if (g < count) {
+        if (slot >= 0 && g < count) {
             i = queue[g];
             defer = move_node<kInterface>(f, ma, ctl, i, F, X, V, typ, part, lab, mv, pw);
         }
@@ -1017,8 +1029,8 @@ __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArg
     }
 }
 
-constexpr int kMoveGrid = 2048;      // blocks of k_move<true> at most
-constexpr int kMoveRestGrid = 256;   // blocks of k_move<false> at most (a few thousand nodes)
+constexpr int kMoveGrid = 2048;       // blocks of k_move<true> at most
+constexpr int kMoveRestGrid = 1024;   // blocks of k_move<false> (4 waves of kRestLanes nodes each)
 
 
 // ---- per-iteration reductions + Verlet bookkeeping ----------------------------------
