@@ -27,7 +27,7 @@ cpu_baseline: the reference's own relax_cpu (oracle/_ref, compiled from its
 sources) on this host's cores, on the same grid and seeds, for a bounded
 number of iterations.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|small]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|small|batch]
 """
 from __future__ import annotations
 
@@ -60,7 +60,9 @@ def item_bytes(kbar: float, radix_passes: int) -> dict:
         "force": 37 + 4 * kbar,       # P,h 16 + typ 1 + nnb 4 + nbr 4K read; F 16 written
         "move": 104,                  # queue 4, F 16, P,h 16, V 16, typ 1, part 4, lab 2 read; P,h 16, V 16, typ 1,
                                       # part 4, mv 4, pw 4 written
+        "move_int": 104,
         "move_rest": 104,
+        "move_def": 104,
         "stats": 36,                  # mv 4, pw 4, P,h 16, P0 12 read
         "refresh": 47 + 4 * kbar,     # list 4, sorted 4, P,h 16, typ 1, lab 2 read; nbr 4K, nnb 4, P0 16 written
         "neighbors": 23 + 4 * kbar,   # P,h 16, typ 1, lab 2 read; nbr 4K + nnb 4 written
@@ -78,7 +80,7 @@ def launch_bytes(name: str, timings: dict, per_item: dict) -> float | None:
     if not launches:
         return None
     if name == "move_easy":   # every node: typ 1; finished nodes 94 more; queued ones 4 (the queue)
-        queued = sum(timings[k][2] / max(timings[k][0], 1) for k in ("move", "move_rest") if k in timings)
+        queued = sum(timings[k][2] / max(timings[k][0], 1) for k in ("move", "move_int", "move_rest") if k in timings)
         n = items / launches
         return 1 * n + 94 * (n - queued) + 4 * queued
     if name not in per_item:
@@ -88,7 +90,8 @@ def launch_bytes(name: str, timings: dict, per_item: dict) -> float | None:
 
 # the kernels of each timed class (their function names in a rocprofv3 trace)
 CLASS_KERNELS = {
-    "force": ["k_force"], "move_easy": ["k_move_easy"], "move": ["k_move<true>"], "move_rest": ["k_move<false>"],
+    "force": ["k_force"], "move_easy": ["k_move_easy"], "move": ["k_move<1>"], "move_int": ["k_move<2>"],
+    "move_rest": ["k_move<0>"], "move_def": ["k_move<0>"],
     "stats": ["k_stats"],
     "neighbors": ["k_truss"], "refresh": ["k_refresh_list", "k_truss_node"], "permute": ["k_permute"],
     "keys": ["k_fill_i32", "k_keys"],
@@ -182,12 +185,81 @@ def cpu_baseline(grid, seeds, iters: int) -> dict | None:
                       f"({st['rebuilds']} rebuilds), {dt:.1f} s"}
 
 
+def bench_batch(args) -> int:
+    """Many small volumes: tn_relax_batch (all volumes' plans concurrent on their own
+    streams) against a loop of tn_relax calls, one volume after another. A step is one
+    relaxation of every volume, uploads and downloads included."""
+    from trussnet_amd import relax, workload
+    from trussnet_amd._abi import RelaxParams
+
+    t = time.perf_counter()
+    grids = workload.batch_volumes(64, seed=0)
+    log(f"64 grids ({time.perf_counter() - t:.1f} s)")
+    seeds = [relax.seed(g, RelaxParams(jseed=0.0, corners=False), device=0) for g in grids]
+    ns = [sd.n for sd in seeds]
+    total = sum(ns)
+    log(f"seeds: {total} in 64 volumes ({min(ns)} .. {max(ns)})")
+    iters = args.iters or 100
+    prm = RelaxParams(max_iters=iters, dptol=0.0)
+    for _ in range(args.warmup):
+        relax.relax_batch(grids, seeds, prm, device=0)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        out = relax.relax_batch(grids, seeds, prm, device=0)
+    dt = time.perf_counter() - t
+    t = time.perf_counter()
+    loop = [relax.relax(g, sd, prm, device=0) for g, sd in zip(grids, seeds)]
+    dt_loop = time.perf_counter() - t
+    same = all(a[0].equal(b[0]) for a, b in zip(out, loop))
+    value = total * iters * args.steps / dt
+    log(f"batch {value:.4g} node-iterations/s; loop of tn_relax {total * iters / dt_loop:.4g}; identical {same}")
+    cpu = None
+    if not args.no_cpu:   # the reference on the first four volumes
+        cpu = cpu_baseline_many(grids[:4], seeds[:4], args.cpu_iters)
+    line = {
+        "metric": "node-iterations/sec, 64 volumes relaxed together (batched)", "value": value,
+        "unit": "node-iterations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batch_volumes(64, seed 0): edges U(32,160), 2-4 nested ellipsoid labels, "
+                               "uniform h U(1.5,4) voxels, device HCP seeding, 100 iterations, uploads included",
+                   "nodes": total, "volumes": 64, "iterations": iters, "parallelism": "single GPU"},
+        "loop_value": total * iters / dt_loop, "identical_to_loop": same, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def cpu_baseline_many(grids, seeds, iters: int) -> dict | None:
+    from trussnet_amd._abi import RelaxParams
+
+    try:
+        from oracle import ref
+
+        if not ref.available():
+            raise RuntimeError("reference library not built")
+    except Exception as e:
+        log(f"reference CPU path unavailable ({e})")
+        return None
+    prm = RelaxParams(max_iters=iters, dptol=0.0)
+    nit, secs = 0, 0.0
+    for g, sd in zip(grids, seeds):
+        rg = ref.RefGrid.from_grid(g)
+        t = time.perf_counter()
+        _, st = rg.relax(sd, prm)
+        secs += time.perf_counter() - t
+        nit += sd.n * st["iters"]
+    cores = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 64))
+    return {"value": nit / secs, "unit": "node-iterations/s", "cores": cores, "kind": "reference",
+            "sample": f"relax_cpu on {len(grids)} of the volumes, {iters} iterations each, {secs:.1f} s"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["large", "small"], default="large")
+    ap.add_argument("--workload", choices=["large", "small", "batch"], default="large")
     ap.add_argument("--iters", type=int, default=0, help="iterations per step (default: the workload's)")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
@@ -199,6 +271,8 @@ def main() -> int:
               f"(world size {world}) is not supported", file=sys.stderr)
         return 2
 
+    if args.workload == "batch":
+        return bench_batch(args)
     pmc = None if args.no_traffic else pmc_traffic(args)   # before this process initialises the GPU
 
     from trussnet_amd import relax

@@ -129,21 +129,37 @@ int tn_relax(const tn_grid_view* g, const tn_relax_params* prm, tn_nodes_view* n
              int32_t device);
 
 /*
- * Seeding on the device: the graded HCP lattice seeds of seed_cpu
- * (src/v2m_particles.h:98; its body at src/v2m_particles.cpp:637-669): each
- * voxel claims the points of its seed level's lattice inside its box
- * (v2m_seed_voxel, count -> scan -> fill, node order = voxel order) and every
- * seed is classified and projected onto a nearby interface or junction curve
- * (v2m_seed_classify). grade[nx*ny*nz] is Grid::grade (src/v2m_grid.h:77).
- * Junction-line seeds and corner nodes are not generated: prm->jseed must be 0
- * and prm->corners 0 (TN_ERR_UNSUPPORTED otherwise). tn_seed_count gives the
- * seed count; tn_seed writes exactly that many nodes into the arrays
- * (P[3n], lab[n], typ[n], part[2n]).
+ * Relax `count` independent volumes on one device: volume k is relaxed exactly as
+ * tn_relax(&grids[k], &prms[k], &nodes[k], &stats[k], device) would relax it, to
+ * the same bits. No counterpart in the reference, which relaxes one volume per
+ * relax_cl call (src/v2m_gpu.h:18); for callers with many volumes. Each volume's
+ * plan runs on its own HIP stream and one host thread advances them as their
+ * iterations complete, so small volumes overlap. stats may be null. On error no
+ * volume's nodes are written.
+ */
+int tn_relax_batch(const tn_grid_view* grids, const tn_relax_params* prms, tn_nodes_view* nodes,
+                   tn_relax_stats* stats, int32_t count, int32_t device);
+
+/*
+ * Seeding on the device: seed_cpu (src/v2m_particles.h:98; its body at
+ * src/v2m_particles.cpp:637-726), node for node. Each voxel claims the points of
+ * its seed level's HCP lattice inside its box (v2m_seed_voxel, count -> scan ->
+ * fill, node order = voxel order) and every seed is classified and projected onto
+ * a nearby interface or junction curve (v2m_seed_classify). With prm->jseed > 0
+ * the junction-line seeds follow (seed_junctions, src/v2m_particles.cpp:80-244):
+ * one seed per (seed level, cell of jseed x level spacing, label triple) at the
+ * 3-label grid vertices, the lattice nodes within 0.4 h of one dropped, the seeds
+ * appended in key order. With prm->corners the fixed CORNER nodes at the 4+-label
+ * vertices come last (v2m_corner_vertex). grade[nx*ny*nz] is Grid::grade
+ * (src/v2m_grid.h:77); the grid's feature curves (add_feature_nodes) are not part
+ * of a tn_grid_view. tn_seed_count gives the seed count; tn_seed writes exactly
+ * that many nodes into the arrays (P[3n], lab[n], typ[n], part[2n], and part3[n],
+ * the third partner of a corner node, unless part3 is null).
  */
 int tn_seed_count(const tn_grid_view* g, const uint8_t* grade, const tn_relax_params* prm, int32_t device,
                   int64_t* n);
 int tn_seed(const tn_grid_view* g, const uint8_t* grade, const tn_relax_params* prm, int32_t device, int64_t n,
-            float* P, uint16_t* lab, uint8_t* typ, uint16_t* part);
+            float* P, uint16_t* lab, uint8_t* typ, uint16_t* part, uint16_t* part3);
 
 /*
  * A relaxation plan: the grid and the seed nodes resident on one device, with
@@ -189,14 +205,18 @@ int tn_plan_reset(tn_plan* plan);
 
 /*
  * Per-kernel device times of a plan, from HIP event pairs recorded on the
- * plan's stream around each launch (the OpenCL host's V2M_CL_PROFILE timings,
+ * stream of each launch (the OpenCL host's V2M_CL_PROFILE timings,
  * src/v2m_gpu.cpp:250-258, without its extra synchronisations). Off by default
  * (on with prm->verbose or TN_KERNEL_TIMING in the environment); switching
  * clears the totals. Kernel classes: keys, scan, radix_sort, permute, neighbors
- * (the rebuild), force, move_easy, move, move_rest, stats, refresh, control (an
- * iteration). items: the nodes the timed launches processed (cells for scan;
- * for move, the interface nodes queued for the trapped step; for move_rest, the
- * other queued nodes; for refresh, the listed nodes). The tn_relax_stats ms_* fields are filled from these when timing is on.
+ * (the rebuild), force, move_easy, move, move_int, move_rest, move_def, stats,
+ * refresh, control (an iteration). move, move_int and move_rest run at the same
+ * time on three streams, so their times overlap. items: the nodes the timed
+ * launches processed (cells for scan; for move and move_int, the interface and
+ * interior nodes queued for the trapped step; for move_rest, the junction and
+ * other nodes queued for the general step; for move_def, the nodes the move and
+ * move_int passes handed to it; for refresh, the listed nodes). The
+ * tn_relax_stats ms_* fields are sums of these when timing is on.
  */
 typedef struct tn_kernel_time {
     char name[16];

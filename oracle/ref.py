@@ -58,6 +58,8 @@ def lib():
         L.ref_move.argtypes = [vp, C.POINTER(RelaxParamsC), C.POINTER(NodesView), C.c_void_p, C.c_void_p, C.c_void_p,
                                C.c_float, C.c_float, C.c_void_p, C.c_void_p]
         L.ref_field_points.argtypes = [vp, C.c_int, C.c_int, C.c_int64] + [C.c_void_p] * 6
+        L.ref_classify_points.restype = C.c_int64
+        L.ref_classify_points.argtypes = [vp, C.c_int64] + [C.c_void_p] * 5
         _lib = L
     return _lib
 
@@ -131,7 +133,7 @@ class RefGrid:
         part = np.zeros((n, 2), np.uint16)
         part3 = np.zeros(n, np.uint16)
         lib().ref_seed_copy(self.h, _p(P), _p(lab), _p(typ), _p(part), _p(part3))
-        return Nodes(P, lab, typ, part)
+        return Nodes(P, lab, typ, part, part3)
 
     def relax(self, nodes: Nodes, prm: RelaxParams) -> tuple[Nodes, dict]:
         """relax_cpu (src/v2m_particles.cpp:728) on a copy of the nodes."""
@@ -179,6 +181,18 @@ class RefGrid:
         pc = prm.to_c()
         lib().ref_move(self.h, C.byref(pc), C.byref(v), _p(hn), _p(F), _p(V), fdt, falpha, _p(pw), _p(mv))
         return out, hn, V, pw, mv
+
+    def classify_points(self, X: np.ndarray) -> Nodes:
+        """Seeds from points: labelled by v2m_label_of, exterior dropped, classified by
+        v2m_seed_classify (the reference's bodies)."""
+        X = np.ascontiguousarray(X, np.float32)
+        m = X.shape[0]
+        P = np.zeros((m, 3), np.float32)
+        lab = np.zeros(m, np.uint16)
+        typ = np.zeros(m, np.uint8)
+        part = np.zeros((m, 2), np.uint16)
+        n = lib().ref_classify_points(self.h, m, _p(X), _p(P), _p(lab), _p(typ), _p(part))
+        return Nodes(P[:n], lab[:n], typ[:n], part[:n], np.full(n, 0xFFFF, np.uint16))
 
     def field_points(self, a: int, b: int, X: np.ndarray):
         X = np.ascontiguousarray(X, np.float32)

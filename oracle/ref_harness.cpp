@@ -429,6 +429,43 @@ void ref_move(void* h, const tn_relax_params* prm, tn_nodes_view* nd, float* hn,
     }
 }
 
+// Seeds from arbitrary points: each labelled by v2m_label_of (voxel mode off), the
+// exterior ones (label 0) dropped, the rest classified by v2m_seed_classify
+// (src/opencl/v2m_seed_body.cl:116-175, 511-599). Writes the kept nodes in input
+// order (P, lab, typ, part) and returns their count.
+int64_t ref_classify_points(void* h, int64_t m, const float* X, float* P, uint16_t* lab, uint8_t* typ,
+                            uint16_t* part) {
+    const RefGrid* r = static_cast<RefGrid*>(h);
+    const tn::Grid& g = r->g;
+    const refh::V2mDims d = dims_of(g);
+    int64_t n = 0;
+
+    for (int64_t k = 0; k < m; ++k) {
+        const float* x = X + 3 * k;
+        int sec;
+        float mg;
+        const int l = refh::v2m_label_of(d, g.L->data(), g.bl_cnt.data(), g.bl_lab.data(), g.bl_slot.data(),
+                                         g.phi.data(), g.gI, g.gTW.data(), g.gm, 0, x[0], x[1], x[2], &sec, &mg);
+
+        if (l == 0 || l == V2M_NOLAB) {
+            continue;
+        }
+
+        P[3 * n] = x[0];
+        P[3 * n + 1] = x[1];
+        P[3 * n + 2] = x[2];
+        lab[n] = static_cast<uint16_t>(l);
+        ++n;
+    }
+
+    for (int64_t i = 0; i < n; ++i) {
+        refh::v2m_seed_classify(d, g.L->data(), g.bl_cnt.data(), g.bl_lab.data(), g.bl_slot.data(), g.phi.data(),
+                                g.gI, g.gTW.data(), g.gm, g.h.data(), static_cast<int>(i), P, lab, typ, part);
+    }
+
+    return n;
+}
+
 // point queries on the field, for unit fixtures: phi_l and psi_ab (+ gradient)
 void ref_field_points(void* h, int a, int b, int64_t m, const float* X, float* phi_a, float* psi, float* grad,
                       int32_t* label, int32_t* second) {

@@ -54,9 +54,14 @@ def graded_size(lab: np.ndarray, lo: float, hi: float, slope: float) -> np.ndarr
 
 
 CASES = {
-    # the small plain workload: 64^3, spheres r 24 / 12, uniform 3-voxel size
+    # the small plain workload: 64^3, spheres r 24 / 12, uniform 3-voxel size; HCP seeds
     "spheres64": dict(vol=lambda: volumes.two_spheres(64, 24.0, 12.0), grid=dict(hbase=3.0, hmin=3.0, hmax=3.0),
                       runs=[(1, 0.0, True), (5, 0.0, True), (50, 0.0, True), (300, 2e-3, True), (40, 0.0, False)]),
+    # the same volume seeded as the small plain benchmark states it: a 3 mm cubic grid
+    # jittered by +-0.25 x 3 per axis (SplitMix64, seed 0), labelled and classified
+    "jitter64": dict(vol=lambda: volumes.two_spheres(64, 24.0, 12.0), grid=dict(hbase=3.0, hmin=3.0, hmax=3.0),
+                     points=lambda: volumes.jittered_points(64, 3.0, 0.25, 0),
+                     runs=[(1, 0.0, True), (50, 0.0, True), (300, 2e-3, True), (50, 0.0, False)]),
     # a graded 4-label ball: junction lines, corners, a multi-level hash
     "sectors48": dict(vol=lambda: volumes.sectors(48), grid=dict(hbase=3.0),
                       runs=[(1, 0.0, True), (10, 0.0, True), (100, 0.0, True), (400, 2e-3, True), (40, 0.0, False)]),
@@ -91,15 +96,24 @@ def make(name: str, spec: dict) -> str:
     rg = ref.RefGrid.build(lab, vs=spec.get("vs", (1.0, 1.0, 1.0)), hvox=hvox, **spec["grid"])
     G = rg.grid()
     prm = RelaxParams()
-    seeds = rg.seed(prm)
+    hcp = rg.seed(prm)
+    # the relaxation's input: seed_cpu's nodes, or the case's own points classified
+    seeds = rg.classify_points(spec["points"]()) if "points" in spec else hcp
     out = {}
     for k, v in G.arrays().items():
         out["grid_" + k] = v
     out["seed_P"], out["seed_lab"], out["seed_typ"], out["seed_part"] = seeds.P, seeds.lab, seeds.typ, seeds.part
+    out["seed_part3"] = seeds.part3
+    if "points" in spec:   # seed_cpu's own output, for the seeding tests
+        out["hcp_P"], out["hcp_lab"], out["hcp_typ"], out["hcp_part"] = hcp.P, hcp.lab, hcp.typ, hcp.part
+        out["hcp_part3"] = hcp.part3
     # the lattice seeds alone: seed_cpu without junction-line seeds and corners
     out["grid_grade"] = G.grade
     lat = rg.seed(RelaxParams(jseed=0.0, corners=False))
     out["lat_P"], out["lat_lab"], out["lat_typ"], out["lat_part"] = lat.P, lat.lab, lat.typ, lat.part
+    # junction-line seeds on a finer key grid, no corners
+    js = rg.seed(RelaxParams(jseed=0.4, corners=False))
+    out["js_P"], out["js_lab"], out["js_typ"], out["js_part"] = js.P, js.lab, js.typ, js.part
     hi = rg.hash_info(prm)
     out["hash_nlev"] = np.int32(hi.nlev)
     out["hash_nkeys"] = np.int32(hi.nkeys)

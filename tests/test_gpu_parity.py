@@ -58,15 +58,25 @@ def test_relax_runs(tn, case):
             assert np.float32(st[k]) == np.float32(v), (case.name, prm.max_iters, k, st[k], v)
 
 
-def test_seeding(tn, case):
-    """Device seeding = the reference's seed_cpu without junction-line seeds and corners."""
+@pytest.mark.parametrize("variant", ["lattice", "junctions", "default"])
+def test_seeding(tn, case, variant):
+    """Device seeding = the reference's seed_cpu: the lattice alone; with junction-line
+    seeds on a 0.4-spacing key grid; with the defaults (junction seeds at 0.8, corners)."""
     from trussnet_amd._abi import Nodes
 
     d = case.d
-    got = tn.seed(case.grid, RelaxParams(jseed=0.0, corners=False), device=0)
-    want = Nodes(d["lat_P"], d["lat_lab"], d["lat_typ"], d["lat_part"])
-    assert got.n == want.n
-    assert got.equal(want), int((got.P != want.P).any(axis=1).sum())
+    hcp = "hcp_" if "hcp_P" in d else "seed_"   # seed_cpu's own nodes (a case may relax other seeds)
+    prm, pre = {"lattice": (RelaxParams(jseed=0.0, corners=False), "lat_"),
+                "junctions": (RelaxParams(jseed=0.4, corners=False), "js_"),
+                "default": (RelaxParams(), hcp)}[variant]
+    got = tn.seed(case.grid, prm, device=0)
+    want = Nodes(d[pre + "P"], d[pre + "lab"], d[pre + "typ"], d[pre + "part"])
+    assert got.n == want.n, (case.name, variant, got.n, want.n)
+    assert got.equal(want), (case.name, variant, int((got.P != want.P).any(axis=1).sum()))
+    if variant == "default":
+        assert np.array_equal(got.part3, d[pre + "part3"])
+    else:
+        assert (got.part3 == 0xFFFF).all()
 
 
 def test_generated_workload_vs_oracle(tn, oracle_lib):
@@ -107,3 +117,21 @@ def test_steps_compose(tn):
         for _ in range(12):
             plan.step(1)
         assert plan.download().equal(whole)
+
+
+def test_batch_equals_the_reference_runs(tn):
+    """tn_relax_batch: every golden case's runs relaxed together, each to the reference's bits."""
+    grids, seeds, prms, wants = [], [], [], []
+    for name in CASES:
+        fx = Fixture(name)
+        for prm, want, stats in fx.runs():
+            grids.append(fx.grid)
+            seeds.append(fx.seeds)
+            prms.append(prm)
+            wants.append((name, want, stats))
+    out = tn.relax_batch(grids, seeds, prms, device=0)
+    assert len(out) == len(wants)
+    for (got, st), (name, want, stats), prm in zip(out, wants, prms):
+        assert got.equal(want), (name, prm.max_iters, prm.dptol, prm.fire)
+        for k, v in stats.items():
+            assert np.float32(st[k]) == np.float32(v), (name, prm.max_iters, k, st[k], v)

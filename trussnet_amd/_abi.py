@@ -221,27 +221,34 @@ class Grid:
 
 @dataclass
 class Nodes:
-    """Nodes (src/v2m_particles.h:78-87): P (n,3) float32, lab, typ, part (n,2)."""
+    """Nodes (src/v2m_particles.h:78-87): P (n,3) float32, lab, typ, part (n,2), and
+    optionally part3 (n,), the third partner of a CORNER node (not read by the
+    relaxation, which keeps corners fixed)."""
     P: np.ndarray
     lab: np.ndarray
     typ: np.ndarray
     part: np.ndarray
+    part3: np.ndarray | None = None
 
     def __post_init__(self):
         self.P = np.ascontiguousarray(self.P, np.float32).reshape(-1, 3)
         self.lab = np.ascontiguousarray(self.lab, np.uint16).ravel()
         self.typ = np.ascontiguousarray(self.typ, np.uint8).ravel()
         self.part = np.ascontiguousarray(self.part, np.uint16).reshape(-1, 2)
+        if self.part3 is not None:
+            self.part3 = np.ascontiguousarray(self.part3, np.uint16).ravel()
         n = self.lab.size
-        if self.P.shape[0] != n or self.typ.size != n or self.part.shape[0] != n:
-            raise ValueError("nodes: P, lab, typ and part must describe the same node count")
+        if self.P.shape[0] != n or self.typ.size != n or self.part.shape[0] != n or (
+                self.part3 is not None and self.part3.size != n):
+            raise ValueError("nodes: P, lab, typ, part and part3 must describe the same node count")
 
     @property
     def n(self) -> int:
         return int(self.lab.size)
 
     def copy(self) -> "Nodes":
-        return Nodes(self.P.copy(), self.lab.copy(), self.typ.copy(), self.part.copy())
+        return Nodes(self.P.copy(), self.lab.copy(), self.typ.copy(), self.part.copy(),
+                     None if self.part3 is None else self.part3.copy())
 
     def view(self) -> NodesView:
         v = NodesView()

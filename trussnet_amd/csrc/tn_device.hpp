@@ -835,21 +835,85 @@ struct NodeState {
 // v2m_move_step (smooth mode), src/opencl/v2m_particle_body.cl:659-833: apply the
 // capped step s with interface trapping. Returns |dp| / h (0: the node stayed;
 // its type and partners may still have changed, as in the reference).
-// kInterface: the node is known to be an interface node; the other branches are
-// compiled out, and so is the rare path (a third label strong enough for a junction
-// where the node lands): the step then returns kDeferred and leaves ns untouched,
-// and the caller hands the node to the general instance.
-template <bool kInterface>
+// The general tail below for an interior or interface node, without its out-of-line
+// calls: one pass over the labels of q's brick evaluates both v2m_third_label
+// (competing_label_near) and v2m_valid_on (labels_dominate_near) at q, with the same
+// values. A third label found (the node would try to join a junction) returns
+// kDeferred. Otherwise the step fails if it is too long or another label dominates
+// at q (the second third-label search then finds none either: same labels, same
+// point), and ends as the general tail ends if not.
+__device__ __forceinline__ float lean_tail(const Field& f, float h, float maxstep, float snap, int a, int typ0,
+                                           int ty, int b, P3 p, P3 q, NodeState& ns) {
+    bool dominated = false;
+    if (ty == TN_INTERFACE) {
+        const BrickLabels bl = brick_labels_at(f, q);
+        int other = 0;
+        for (int s = 0; s < bl.n; ++s) {
+            const int l = brick_label(f, bl, s);
+            other |= l != a && l != b;
+        }
+        if (other) {
+            const PointCell cell = point_cell(f, q, bl);
+            const float pa = phi_point(f, a, q), pb = phi_point(f, b, q);
+            const float mine = fminf(pa, pb);
+            float pc = 0.5f * pa;
+            int cc = kNoLab;
+            for (int s = 0; s < bl.n; ++s) {
+                const int l = brick_label(f, bl, s);
+                if (l != a && l != b) {
+                    const float w = label_phi(f, cell, s, l);
+                    if (w > pc) {
+                        pc = w;
+                        cc = l;
+                    }
+                    dominated = dominated || w > mine + 0.02f;
+                }
+            }
+            if (cc != kNoLab) {
+                return kDeferred;
+            }
+        }
+    }
+    const float lim = (maxstep + fmaxf(snap, 0.5f)) * h;
+    const float m2 = dist2(q, p);
+    if (!(m2 <= lim * lim)) {
+        if (typ0 == TN_INTERIOR) {
+            ns.b = kNoLab;
+        }
+        return 0.0f;
+    }
+    if (ty != TN_INTERIOR && dominated) {
+        ns.ty = typ0 == TN_INTERIOR ? TN_INTERIOR : ty;
+        if (ns.ty == TN_INTERIOR) {
+            ns.b = kNoLab;
+        }
+        return 0.0f;
+    }
+    ns.p = q;
+    ns.ty = ty;
+    ns.hn = size_at(f, q.x, q.y, q.z);
+    return sqrtf(m2) / h;
+}
+
+// Step modes: the general path, or a lean one for nodes known to be interface
+// (kLeanInterface) or interior (kLeanInterior) nodes: the other branches are compiled
+// out, and so is the rare path (a third label strong enough for a junction where the
+// node lands), for which the step returns kDeferred and leaves the stored state
+// untouched; the caller hands such a node to the general path.
+constexpr int kGeneral = 0, kLeanInterface = 1, kLeanInterior = 2;
+
+template <int kMode>
 __device__ __forceinline__ float trapped_step(const Field& f, float h, P3 s, float maxstep, float snap, int a,
                                               NodeState& ns) {
     const P3 p = ns.p;
     P3 q;
-    const int typ0 = kInterface ? TN_INTERFACE : ns.ty;   // typ[i] as stored until the final write
+    // typ[i] as stored until the final write
+    const int typ0 = kMode == kLeanInterface ? TN_INTERFACE : (kMode == kLeanInterior ? TN_INTERIOR : ns.ty);
     int ty = typ0;
     int b = ns.b;             // the locals b, c of the reference; ns.b / ns.c are the stored part[]
     const int c = ns.c;
 
-    if (!kInterface && ty == TN_INTERIOR) {
+    if (kMode != kLeanInterface && ty == TN_INTERIOR) {
         q = p3(p.x + s.x, p.y + s.y, p.z + s.z);
         const int2 own = owner_at(f, q);
         const int l = own.x, sec = own.y;
@@ -880,7 +944,7 @@ __device__ __forceinline__ float trapped_step(const Field& f, float h, P3 s, flo
                 }
             }
         }
-    } else if (kInterface || ty == TN_INTERFACE) {
+    } else if (kMode == kLeanInterface || ty == TN_INTERFACE) {
         PsiCell pc;   // the glide and its projection probe the same few cells
         psi_cell_init(pc);
         s = drop_normal_at(f, a, b, p, s, pc);
@@ -890,52 +954,6 @@ __device__ __forceinline__ float trapped_step(const Field& f, float h, P3 s, flo
             return 0.0f;
         }
         q = pr.p;
-        if (kInterface) {
-            // The general tail below for an interface node, inline: one pass over the
-            // labels of q's brick evaluates both v2m_third_label (competing_label_near)
-            // and v2m_valid_on (labels_dominate_near) at q, with the same values. A
-            // third label found (the node would try to join a junction) hands the node
-            // back for the general path. Otherwise the step fails if it is too long or
-            // another label dominates at q (the second third-label search then finds
-            // none either: same labels, same point), and ends here if not.
-            const BrickLabels bl = brick_labels_at(f, q);
-            int other = 0;
-            for (int s = 0; s < bl.n; ++s) {
-                const int l = brick_label(f, bl, s);
-                other |= l != a && l != b;
-            }
-            bool dominated = false;
-            if (other) {
-                const PointCell cell = point_cell(f, q, bl);
-                const float pa = phi_point(f, a, q), pb = phi_point(f, b, q);
-                const float mine = fminf(pa, pb);
-                float pc = 0.5f * pa;
-                int cc = kNoLab;
-                for (int s = 0; s < bl.n; ++s) {
-                    const int l = brick_label(f, bl, s);
-                    if (l != a && l != b) {
-                        const float w = label_phi(f, cell, s, l);
-                        if (w > pc) {
-                            pc = w;
-                            cc = l;
-                        }
-                        dominated = dominated || w > mine + 0.02f;
-                    }
-                }
-                if (cc != kNoLab) {
-                    return kDeferred;
-                }
-            }
-            const float lim = (maxstep + fmaxf(snap, 0.5f)) * h;
-            const float m2 = dist2(q, p);
-            if (!(m2 <= lim * lim) || dominated) {
-                return 0.0f;
-            }
-            ns.p = q;
-            ns.ty = TN_INTERFACE;
-            ns.hn = size_at(f, q.x, q.y, q.z);
-            return sqrtf(m2) / h;
-        }
     } else {
         const float4 g1 = psi_grad_point(f.self, a, b, p);
         const float4 g2 = psi_grad_point(f.self, a, c, p);
@@ -950,6 +968,10 @@ __device__ __forceinline__ float trapped_step(const Field& f, float h, P3 s, flo
             return 0.0f;
         }
         q = pr.p;
+    }
+
+    if (kMode != kGeneral) {
+        return lean_tail(f, h, maxstep, snap, a, typ0, ty, b, p, q, ns);
     }
 
     if (ty == TN_INTERFACE) {

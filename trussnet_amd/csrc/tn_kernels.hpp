@@ -41,7 +41,9 @@ struct Ctl {
     int32_t nrefresh;      // nodes queued for an own-list refresh this iteration
     int32_t nhalf;         // nodes past skin/2 (this iteration)
     int32_t nmove;         // interface nodes queued for the trapped step (this iteration)
-    int32_t nmove_rest;    // other nodes queued for the full trapped step (this iteration)
+    int32_t nmove_int;     // interior nodes queued for the trapped step (crossing or in snap range)
+    int32_t nmove_rest;    // other nodes queued for the general trapped step (this iteration)
+    int32_t nmove_def;     // nodes a lean pass handed to the general step (this iteration)
     // FIRE (FireCtl, src/v2m_particles.h:47-76)
     float dt0, dt, dtmax, alpha;
     int32_t npos;
@@ -313,10 +315,11 @@ __global__ __launch_bounds__(kBlock) void k_permute(const int32_t* __restrict__ 
 // The queries are consecutive in cell order, so their scan boxes overlap: per hash
 // level the wave scans the union of its lanes' boxes (rows no lane reaches are
 // skipped), loading each row's candidates once, coalesced, into LDS, and every
-// lane tests every candidate against its own query. Valid candidates (within
-// reach, level within Li +- 1, no interior bar across labels) are appended to the
-// lane's LDS buffer, which is cut back to its 24 best whenever it fills; at the
-// end each lane sorts its buffer by (distance, id) and keeps the first 24. The
+// lane tests every candidate against its own query, eight at a time. Candidates
+// within reach are appended raw, as (d^2, slot), to the lane's LDS buffer; the
+// other checks (not the node itself, no interior bar across labels) and the
+// distance sqrtf(d^2) wait until the buffer fills, when it is cut back to its 24
+// best, and the end, when the first 24 in (distance, id) order are kept. The
 // list equals the reference's: its K smallest valid candidates in (distance, id)
 // order, and every valid candidate lies inside the lane's own box, which the
 // union contains. (That holds when every node's reach fits its level's cell,
@@ -354,9 +357,10 @@ __device__ __forceinline__ bool before(float da, int ja, float db, int jb) {
     return da < db || (da == db && ja < jb);
 }
 
-// insertion sort of a lane's buffer (entries e*64 + lane)
-__device__ __forceinline__ void sort_lane(float* cd, int* cj, int lane, int cnt) {
-    for (int e = 1; e < cnt; ++e) {
+// insertion sort of entries [lo, cnt) of a lane's buffer (entries e*64 + lane) into
+// its sorted prefix [0, lo)
+__device__ __forceinline__ void sort_lane(float* cd, int* cj, int lane, int lo, int cnt) {
+    for (int e = lo > 1 ? lo : 1; e < cnt; ++e) {
         const float d = cd[e * kTrussWave + lane];
         const int j = cj[e * kTrussWave + lane];
         int k = e;
@@ -375,11 +379,32 @@ __device__ __forceinline__ void sort_lane(float* cd, int* cj, int lane, int cnt)
     }
 }
 
+// Settle a lane's buffer: entries [0, nval) are valid (distance, id) pairs in order;
+// entries [nval, cnt) are raw hits (d^2, slot). The raw ones are checked (not the
+// node itself, no interior bar across labels, v2m_particle_body.cl:117-121), turned
+// into (sqrtf(d^2), id) and merged in; at most kK are kept. Returns the new count.
+__device__ __forceinline__ int settle_lane(float* cd, int* cj, int lane, int nval, int cnt, int q, int ti_int,
+                                           int li, const int2* __restrict__ Ms) {
+    int m = nval;
+    for (int e = nval; e < cnt; ++e) {
+        const float d2 = cd[e * kTrussWave + lane];
+        const int s = cj[e * kTrussWave + lane];
+        const int2 mt = Ms[s];
+        if (s == q || (ti_int && (mt.y & 0xFF) == TN_INTERIOR && (mt.y >> 8) != li)) {
+            continue;
+        }
+        cd[m * kTrussWave + lane] = sqrtf(d2);
+        cj[m * kTrussWave + lane] = mt.x;
+        ++m;
+    }
+    sort_lane(cd, cj, lane, nval, m);
+    return m < kK ? m : kK;
+}
+
 // The lists of all nodes at a rebuild: query q is the node in slot q.
 __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n, int32_t* __restrict__ nbrT,
                                                       int32_t* __restrict__ nnb) {
     __shared__ float4 cpos[kTrussWave];
-    __shared__ int2 cmeta[kTrussWave];
     __shared__ float cd[kCollect * kTrussWave];
     __shared__ int cj[kCollect * kTrussWave];
     const int lane = threadIdx.x;
@@ -388,10 +413,9 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
     for (int qb = blockIdx.x * kTrussWave; qb < n; qb += gridDim.x * kTrussWave) {
         const int q = qb + lane;
         const bool active = q < n;
-        int i = 0;
         float px = 0.0f, py = 0.0f, pz = 0.0f, hi = 0.0f, Ri = 0.0f;
         int Li = 0, ti_int = 0, li = 0;
-        if (active) {   // the node in slot q (nodes are in cell order); i: its id
+        if (active) {   // the node in slot q (nodes are in cell order)
             const float4 xi = in.X[q];
             px = xi.x;
             py = xi.y;
@@ -400,7 +424,6 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
             Ri = tskin * hi;
             Li = level_for(H, Ri);
             const int2 mi = in.Ms[q];
-            i = mi.x;
             ti_int = (mi.y & 0xFF) == TN_INTERIOR;
             li = mi.y >> 8;
         }
@@ -414,7 +437,7 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
             gx = cell_coord(H, Li, px, H.ox, 0);
         }
         bool pending = active;
-        int cnt = 0;
+        int cnt = 0, nval = 0;
         for (;;) {
             const uint64_t pm = __ballot(pending);
             if (!pm) {
@@ -479,11 +502,9 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                             const int s = base + lane;
                             if (s < sb) {
                                 cpos[lane] = in.Ps[s];
-                                cmeta[lane] = in.Ms[s];
                             } else {   // past the row's end: far away, it fails every test
                                 const float inf = __builtin_huge_valf();
                                 cpos[lane] = make_float4(inf, inf, inf, 0.0f);
-                                cmeta[lane] = make_int2(-1, 0);
                             }
                             __syncthreads();
                             if (!scan) {
@@ -501,7 +522,7 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                                     const float rj = half * (hi + c.w);
                                     hit |= (d2 < rj * rj) ? (1u << u) : 0u;
                                 }
-                                while (hit) {
+                                while (hit) {   // appended raw: (d^2, slot), checked when settled
                                     const int k = k0 + __builtin_ctz(hit);
                                     hit &= hit - 1;
                                     const float4 c = cpos[k];
@@ -511,18 +532,13 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                                             continue;
                                         }
                                     }
-                                    const int2 mt = cmeta[k];
-                                    if (mt.x == i || (ti_int && (mt.y & 0xFF) == TN_INTERIOR && (mt.y >> 8) != li)) {
-                                        continue;
-                                    }
                                     const float ex = c.x - px, ey = c.y - py, ez = c.z - pz;
                                     const float d2 = ex * ex + ey * ey + ez * ez;
                                     if (cnt == kCollect) {   // keep the 24 best so far
-                                        sort_lane(cd, cj, lane, cnt);
-                                        cnt = kK;
+                                        nval = cnt = settle_lane(cd, cj, lane, nval, cnt, q, ti_int, li, in.Ms);
                                     }
-                                    cd[cnt * kTrussWave + lane] = sqrtf(d2);
-                                    cj[cnt * kTrussWave + lane] = mt.x;
+                                    cd[cnt * kTrussWave + lane] = d2;
+                                    cj[cnt * kTrussWave + lane] = base + k;
                                     ++cnt;
                                 }
                             }
@@ -532,8 +548,7 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
             }
         }   // the groups
         if (active) {   // (distance, id) order; the list holds slots
-            sort_lane(cd, cj, lane, cnt);
-            const int kk = cnt < kK ? cnt : kK;
+            const int kk = settle_lane(cd, cj, lane, nval, cnt, q, ti_int, li, in.Ms);
             nnb[q] = kk;
             for (int k = 0; k < kk; ++k) {
                 nbrT[(size_t)k * n + q] = in.slot_of[cj[k * kTrussWave + lane]];
@@ -884,10 +899,11 @@ __global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl*
                                                       float4* __restrict__ V, uint8_t* __restrict__ typ,
                                                       uint32_t* __restrict__ part, const uint16_t* __restrict__ lab,
                                                       float* __restrict__ mv, float* __restrict__ pw,
-                                                      int32_t* __restrict__ queue, int32_t* __restrict__ queue_rest) {
+                                                      int32_t* __restrict__ queue, int32_t* __restrict__ queue_int,
+                                                      int32_t* __restrict__ queue_rest) {
     const int g = blockIdx.x * kBlock + threadIdx.x;
     int i = 0;
-    bool surf = false, hard = false;
+    bool surf = false, inner = false, hard = false;   // the three queues
     if (g < n) {
         i = g;
         const int ty0 = typ[i];
@@ -918,13 +934,13 @@ __global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl*
                 const P3 q = p3(p.x + st.s[0], p.y + st.s[1], p.z + st.s[2]);
                 const OwnerPsi own = owner_psi_at(f, a, q);
                 if (own.best != a) {
-                    hard = true;
+                    inner = true;
                 } else if (own.sec != kNoLab && ma.snap > 0.0f) {
                     const float4 gq = own.g;
                     const float gn = sqrtf(gq.x * gq.x + gq.y * gq.y + gq.z * gq.z);
-                    hard = gn > 1e-9f && gq.w / gn < ma.snap * h;
+                    inner = gn > 1e-9f && gq.w / gn < ma.snap * h;
                 }
-                if (!hard) {   // the rest of trapped_step for an interior node that stays interior
+                if (!inner) {   // the rest of trapped_step for an interior node that stays interior
                     const uint32_t pp = part[i];
                     NodeState ns;
                     ns.p = p;
@@ -949,21 +965,22 @@ __global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl*
     }
     // queue the rest, one atomic per wave and queue
     wave_append(surf, i, &ctl->nmove, queue);
+    wave_append(inner, i, &ctl->nmove_int, queue_int);
     wave_append(hard, i, &ctl->nmove_rest, queue_rest);
 }
 
 #ifndef TN_REST_LANES
 #define TN_REST_LANES 4
 #endif
-constexpr int kRestLanes = TN_REST_LANES;   // nodes per wave in k_move<false>
+constexpr int kRestLanes = TN_REST_LANES;   // nodes per wave in k_move<kGeneral>
 
 // waves per SIMD the full move kernel is compiled for (its register budget: 512 / waves)
 #ifndef TN_MOVE_WAVES
 #define TN_MOVE_WAVES 1
 #endif
 
-// v2m_move for queued node i; true: handed back (kInterface, a third label near)
-template <bool kInterface>
+// v2m_move for queued node i; true: handed back (a lean mode, a third label near)
+template <int kMode>
 __device__ __forceinline__ bool move_node(const Field& f, const MoveArgs& ma, const Ctl* __restrict__ ctl, int i,
                                           const float4* __restrict__ F, float4* __restrict__ X,
                                           float4* __restrict__ V, uint8_t* __restrict__ typ,
@@ -971,7 +988,7 @@ __device__ __forceinline__ bool move_node(const Field& f, const MoveArgs& ma, co
                                           float* __restrict__ mv, float* __restrict__ pw) {
     const float4 x = X[i];
     const float h = x.w;   // h at the node: v2m_h_at(P_i), kept current by every move
-    const int ty0 = kInterface ? TN_INTERFACE : typ[i];
+    const int ty0 = kMode == kLeanInterface ? TN_INTERFACE : (kMode == kLeanInterior ? TN_INTERIOR : typ[i]);
     const uint32_t pp = part[i];
     NodeState ns;
     ns.p = p3(x.x, x.y, x.z);
@@ -982,8 +999,8 @@ __device__ __forceinline__ bool move_node(const Field& f, const MoveArgs& ma, co
     float m = 0.0f;
     const NodeStep st = node_step(ma, ctl, F[i], V, i, h);   // queued nodes are never corners
     if (st.moves) {
-        m = trapped_step<kInterface>(f, h, p3(st.s[0], st.s[1], st.s[2]), ma.maxstep, ma.snap, lab[i], ns);
-        if (kInterface && m == kDeferred) {
+        m = trapped_step<kMode>(f, h, p3(st.s[0], st.s[1], st.s[2]), ma.maxstep, ma.snap, lab[i], ns);
+        if (kMode != kGeneral && m == kDeferred) {
             return true;
         }
     }
@@ -991,13 +1008,15 @@ __device__ __forceinline__ bool move_node(const Field& f, const MoveArgs& ma, co
     return false;
 }
 
-// Pass 2: v2m_move (src/opencl/v2m_particle_body.cl:844-935) for the queued nodes.
-// kInterface: the ctl->nmove interface nodes, with the interface branch of the
-// trapped step only (no out-of-line calls, whose register budgets would bound the
-// kernel's occupancy); a node with a third label near is handed to the rest queue.
-// Otherwise: the ctl->nmove_rest others, with the whole trapped step; this launch
-// runs after the interface one, so it sees the handed-back nodes.
-template <bool kInterface>
+// Pass 2: v2m_move (src/opencl/v2m_particle_body.cl:844-935) for the queued nodes,
+// *count of them. kLeanInterface / kLeanInterior: interface or interior nodes, with
+// their branch of the trapped step only and no out-of-line calls (whose register
+// budgets would bound the kernel's occupancy); a node with a third label near is
+// appended to (defer_count, defer_queue) for a later kGeneral pass. kGeneral: the
+// whole trapped step (junction nodes and the handed-back ones). Each node's move
+// reads and writes only that node's state, so the passes over disjoint queues may
+// run concurrently.
+template <int kMode>
 __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArgs ma, Ctl* __restrict__ ctl,
                                                                 const int32_t* __restrict__ queue,
                                                                 const float4* __restrict__ F, float4* __restrict__ X,
@@ -1005,32 +1024,35 @@ __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArg
                                                                 uint32_t* __restrict__ part,
                                                                 const uint16_t* __restrict__ lab,
                                                                 float* __restrict__ mv, float* __restrict__ pw,
-                                                                int32_t* __restrict__ queue_rest) {
-    // a fixed grid strides over the queue, whose length only the device knows.
-    // kInterface: kBlock nodes per block. Otherwise kRestLanes nodes per wave: these
-    // few thousand nodes follow long, divergent paths (junction projections, segment
-    // crossings), and a wave executes every path its lanes take, so spreading them
-    // thinly over many waves shortens the longest wave.
-    const int count = kInterface ? ctl->nmove : ctl->nmove_rest;
-    const int per_block = kInterface ? kBlock : kRestLanes * (kBlock / kWave);
+                                                                const int32_t* __restrict__ qcount,
+                                                                int32_t* __restrict__ defer_count,
+                                                                int32_t* __restrict__ defer_queue) {
+    // a fixed grid strides over the queue, whose length only the device knows. The
+    // lean modes: kBlock nodes per block. kGeneral: kRestLanes nodes per wave: these
+    // few thousand nodes follow long, divergent paths (junction projections), and a
+    // wave executes every path its lanes take, so spreading them thinly over many
+    // waves shortens the longest wave.
+    constexpr bool kLean = kMode != kGeneral;
+    const int count = *qcount;
+    const int per_block = kLean ? kBlock : kRestLanes * (kBlock / kWave);
     const int lane = threadIdx.x & (kWave - 1);
-    const int slot = kInterface ? (int)threadIdx.x : (lane < kRestLanes ? (threadIdx.x >> 6) * kRestLanes + lane : -1);
+    const int slot = kLean ? (int)threadIdx.x : (lane < kRestLanes ? (threadIdx.x >> 6) * kRestLanes + lane : -1);
     for (int base = blockIdx.x * per_block; base < count; base += gridDim.x * per_block) {
         const int g = base + slot;
         bool defer = false;
         int i = 0;
         if (slot >= 0 && g < count) {
             i = queue[g];
-            defer = move_node<kInterface>(f, ma, ctl, i, F, X, V, typ, part, lab, mv, pw);
+            defer = move_node<kMode>(f, ma, ctl, i, F, X, V, typ, part, lab, mv, pw);
         }
-        if (kInterface) {   // every lane of the wave is here
-            wave_append(defer, i, &ctl->nmove_rest, queue_rest);
+        if (kLean) {   // every lane of the wave is here
+            wave_append(defer, i, defer_count, defer_queue);
         }
     }
 }
 
-constexpr int kMoveGrid = 2048;       // blocks of k_move<true> at most
-constexpr int kMoveRestGrid = 1024;   // blocks of k_move<false> (4 waves of kRestLanes nodes each)
+constexpr int kMoveGrid = 2048;       // blocks of a lean k_move at most
+constexpr int kMoveRestGrid = 1024;   // blocks of k_move<kGeneral> (4 waves of kRestLanes nodes each)
 
 
 // ---- per-iteration reductions + Verlet bookkeeping ----------------------------------
@@ -1279,7 +1301,9 @@ __global__ void k_iter_begin(Ctl* __restrict__ ctl) {
         ctl->nhalf = 0;
         ctl->nrefresh = 0;
         ctl->nmove = 0;
+        ctl->nmove_int = 0;
         ctl->nmove_rest = 0;
+        ctl->nmove_def = 0;
     }
 }
 

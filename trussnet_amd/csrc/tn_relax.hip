@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -205,14 +206,16 @@ tnd::Field upload_grid(const tn_grid_view* g, DevMem& mem) {
 }
 
 // Kernel classes of the per-kernel device timings.
-enum KClass { KC_KEYS, KC_SCAN, KC_RADIX, KC_PERMUTE, KC_NEIGHBORS, KC_FORCE, KC_MOVE_EASY, KC_MOVE, KC_MOVE_REST,
+enum KClass { KC_KEYS, KC_SCAN, KC_RADIX, KC_PERMUTE, KC_NEIGHBORS, KC_FORCE, KC_MOVE_EASY, KC_MOVE, KC_MOVE_INT,
+              KC_MOVE_REST, KC_MOVE_DEF,
               KC_STATS, KC_REFRESH, KC_CONTROL, KC_COUNT };
 const char* const kKernelNames[KC_COUNT] = { "keys", "scan", "radix_sort", "permute", "neighbors",
-                                             "force", "move_easy", "move", "move_rest", "stats", "refresh", "control" };
+                                             "force", "move_easy", "move", "move_int", "move_rest", "move_def",
+                                             "stats", "refresh", "control" };
 
-// Device time per kernel class from event pairs on one stream. Elapsed times are
-// read only after the stream has been synchronised (collect), so timing adds
-// event records to the stream but no host waits.
+// Device time per kernel class from event pairs on the stream of each launch.
+// Elapsed times are read only after the streams have been synchronised (collect),
+// so timing adds event records to the streams but no host waits.
 struct KernelTimer {
     bool on = false;
     hipStream_t stream = nullptr;
@@ -245,20 +248,20 @@ struct KernelTimer {
         pool.pop_back();
         return e;
     }
-    hipEvent_t begin() {
+    hipEvent_t begin(hipStream_t s = nullptr) {
         if (!on) {
             return nullptr;
         }
         hipEvent_t e = take();
-        TN_HIP(hipEventRecord(e, stream));
+        TN_HIP(hipEventRecord(e, s ? s : stream));
         return e;
     }
-    void end(int kc, hipEvent_t a) {
+    void end(int kc, hipEvent_t a, hipStream_t s = nullptr) {
         if (!on || !a) {
             return;
         }
         hipEvent_t b = take();
-        TN_HIP(hipEventRecord(b, stream));
+        TN_HIP(hipEventRecord(b, s ? s : stream));
         pending.push_back({ kc, a, b });
     }
     void collect() {   // the stream has been synchronised
@@ -310,13 +313,20 @@ struct tn_plan {
     const int32_t* dKeySorted = nullptr;   // the keys in slot order after the last rebuild
     float *dMv = nullptr, *dPw = nullptr, *dThr = nullptr;
     int32_t *dKey = nullptr, *dCstart = nullptr, *dSorted = nullptr, *dNbrT = nullptr, *dNnb = nullptr;
-    int32_t *dQueue = nullptr, *dMoveQ = nullptr, *dMoveQ2 = nullptr, *dBhist = nullptr, *dBsum = nullptr;
+    int32_t *dQueue = nullptr, *dBhist = nullptr, *dBsum = nullptr;
+    // the move queues: interface (lean), interior (lean), general, handed back
+    int32_t *dMoveQ = nullptr, *dMoveQi = nullptr, *dMoveQ2 = nullptr, *dMoveQd = nullptr;
     uint8_t* dFlag = nullptr;   // nodes past a full skin this iteration
     int32_t* dRk[2] = { nullptr, nullptr };
     int32_t* dRv[2] = { nullptr, nullptr };
     double2* dPartials = nullptr;
     tnk::Ctl* dCtl = nullptr;
     tnk::Ctl* hCtl = nullptr;   // pinned
+    hipEvent_t ev_iter = nullptr;   // behind each iteration's control-block copy
+    // the general and the lean interior passes run on side streams, beside the lean
+    // interface pass: ev_easy starts them, ev_rest and ev_int join them back
+    hipStream_t side[2] = { nullptr, nullptr };
+    hipEvent_t ev_easy = nullptr, ev_rest = nullptr, ev_int = nullptr;
     int nparts = 0, radix_passes = 0, radix_nblk = 0, refresh_blocks = 1;
     // host-side loop state
     int rebuilds = 0;
@@ -336,6 +346,21 @@ struct tn_plan {
         }
         if (hCtl) {
             (void)hipHostFree(hCtl);
+        }
+        for (hipStream_t s : side) {
+            if (s) {
+                (void)hipStreamSynchronize(s);
+            }
+        }
+        for (hipEvent_t e : { ev_iter, ev_easy, ev_rest, ev_int }) {
+            if (e) {
+                (void)hipEventDestroy(e);
+            }
+        }
+        for (hipStream_t s : side) {
+            if (s) {
+                (void)hipStreamDestroy(s);
+            }
         }
         if (stream) {
             (void)hipStreamDestroy(stream);
@@ -440,7 +465,9 @@ struct tn_plan {
     }
 
     // one iteration (src/v2m_particles.cpp:828-946); returns false once converged
-    bool iterate() {
+    // one iteration (src/v2m_particles.cpp:828-946), enqueued: the kernels, then the
+    // control block's copy to the host and ev_iter behind them
+    void enqueue_iteration() {
         const unsigned nb = blocks_for(n, tnk::kBlock);
         tnk::k_iter_begin<<<1, 64, 0, stream>>>(dCtl);
         hipEvent_t t = timer.begin();
@@ -449,17 +476,36 @@ struct tn_plan {
         t = timer.begin();
         tnk::MoveArgs ma{ prm.dt, prm.maxstep, prm.snap, fire };
         tnk::k_move_easy<<<nb, tnk::kBlock, 0, stream>>>(f, ma, dCtl, n, dF, dX, dV, dTyp, dPart, dLab, dMv,
-                                                         dPw, dMoveQ, dMoveQ2);
+                                                         dPw, dMoveQ, dMoveQi, dMoveQ2);
         timer.end(KC_MOVE_EASY, t);
-        t = timer.begin();
+        // the queued nodes: three passes over disjoint queues at once, each bounded by
+        // its slowest node, then the general pass over the nodes handed back
+        TN_HIP(hipEventRecord(ev_easy, stream));
         const unsigned g1 = std::min<unsigned>(nb, tnk::kMoveGrid), g2 = std::min<unsigned>(nb, tnk::kMoveRestGrid);
-        tnk::k_move<true><<<g1, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQ, dF, dX, dV, dTyp, dPart, dLab, dMv, dPw,
-                                                          dMoveQ2);
-        timer.end(KC_MOVE, t);
+        TN_HIP(hipStreamWaitEvent(side[0], ev_easy, 0));
+        t = timer.begin(side[0]);
+        tnk::k_move<tnd::kGeneral><<<g2, tnk::kBlock, 0, side[0]>>>(f, ma, dCtl, dMoveQ2, dF, dX, dV, dTyp, dPart, dLab,
+                                                                    dMv, dPw, &dCtl->nmove_rest, nullptr, nullptr);
+        timer.end(KC_MOVE_REST, t, side[0]);
+        TN_HIP(hipEventRecord(ev_rest, side[0]));
+        TN_HIP(hipStreamWaitEvent(side[1], ev_easy, 0));
+        t = timer.begin(side[1]);
+        tnk::k_move<tnd::kLeanInterior><<<g1, tnk::kBlock, 0, side[1]>>>(f, ma, dCtl, dMoveQi, dF, dX, dV, dTyp, dPart,
+                                                                         dLab, dMv, dPw, &dCtl->nmove_int,
+                                                                         &dCtl->nmove_def, dMoveQd);
+        timer.end(KC_MOVE_INT, t, side[1]);
+        TN_HIP(hipEventRecord(ev_int, side[1]));
         t = timer.begin();
-        tnk::k_move<false><<<g2, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQ2, dF, dX, dV, dTyp, dPart, dLab, dMv,
-                                                           dPw, nullptr);
-        timer.end(KC_MOVE_REST, t);
+        tnk::k_move<tnd::kLeanInterface><<<g1, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQ, dF, dX, dV, dTyp, dPart,
+                                                                         dLab, dMv, dPw, &dCtl->nmove, &dCtl->nmove_def,
+                                                                         dMoveQd);
+        timer.end(KC_MOVE, t);
+        TN_HIP(hipStreamWaitEvent(stream, ev_int, 0));
+        t = timer.begin();
+        tnk::k_move<tnd::kGeneral><<<g2, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQd, dF, dX, dV, dTyp, dPart, dLab,
+                                                                   dMv, dPw, &dCtl->nmove_def, nullptr, nullptr);
+        timer.end(KC_MOVE_DEF, t);
+        TN_HIP(hipStreamWaitEvent(stream, ev_rest, 0));
         t = timer.begin();
         tnk::k_stats<<<nparts, tnk::kBlock, 0, stream>>>(dX, dP0, dMv, dPw, n, prm.skin, fire, dThr, dCtl, dFlag,
                                                          dPartials);
@@ -474,13 +520,21 @@ struct tn_plan {
         timer.end(KC_CONTROL, t);
         TN_HIP(hipGetLastError());
         TN_HIP(hipMemcpyAsync(hCtl, dCtl, sizeof(tnk::Ctl), hipMemcpyDeviceToHost, stream));
-        TN_HIP(hipStreamSynchronize(stream));
+        TN_HIP(hipEventRecord(ev_iter, stream));
+    }
+
+    // after ev_iter: the host's part of the iteration (the decisions are the device's,
+    // in the control block). Enqueues the rebuild when one is due; false once the
+    // relaxation has converged.
+    bool finish_iteration() {
         timer.collect();
         for (int kc : { KC_FORCE, KC_MOVE_EASY, KC_STATS }) {
             timer.count(kc, n);
         }
         timer.count(KC_MOVE, hCtl->nmove);
+        timer.count(KC_MOVE_INT, hCtl->nmove_int);
         timer.count(KC_MOVE_REST, hCtl->nmove_rest);
+        timer.count(KC_MOVE_DEF, hCtl->nmove_def);
         timer.count(KC_REFRESH, hCtl->nrefresh);
         timer.count(KC_CONTROL, 1);
         if (hCtl->done) {
@@ -491,6 +545,16 @@ struct tn_plan {
             rebuild();
         }
         return true;
+    }
+
+    bool iterate() {
+        enqueue_iteration();
+        TN_HIP(hipEventSynchronize(ev_iter));
+        return finish_iteration();
+    }
+
+    bool more() const {   // iterations remain
+        return n > 0 && !finished && hCtl->it < prm.max_iters;
     }
 
     void run_iters(int iters) {
@@ -518,7 +582,7 @@ struct tn_plan {
         const double* ms = timer.ms;
         st->ms_hash = ms[KC_KEYS] + ms[KC_SCAN] + ms[KC_RADIX] + ms[KC_PERMUTE] + ms[KC_NEIGHBORS];
         st->ms_force = ms[KC_FORCE];
-        st->ms_move = ms[KC_MOVE_EASY] + ms[KC_MOVE] + ms[KC_MOVE_REST] + ms[KC_STATS] + ms[KC_REFRESH] + ms[KC_CONTROL];
+        st->ms_move = ms[KC_MOVE_EASY] + ms[KC_MOVE] + ms[KC_MOVE_INT] + ms[KC_MOVE_REST] + ms[KC_MOVE_DEF] + ms[KC_STATS] + ms[KC_REFRESH] + ms[KC_CONTROL];
         if (n > 0) {   // type counts (src/v2m_particles.cpp:968-975), counted on the device
             tnk::k_order_begin<<<1, 64, 0, stream>>>(dCtl);
             tnk::k_order_count<<<blocks_for(n, tnk::kBlock), tnk::kBlock, 0, stream>>>(dTyp, n, dCtl);
@@ -589,28 +653,205 @@ void check_nodes(const tn_nodes_view* nd) {
     }
 }
 
-// The seeding of seed_cpu_body (src/v2m_particles.cpp:637-669): per-voxel counts,
-// their scan, the fill in voxel order and the classification, on one device.
-// The seeds stay in `mem`; returns their count.
+// The seeding of seed_cpu_body (src/v2m_particles.cpp:637-726) on one device: the
+// lattice (per-voxel counts, their scan, the fill in voxel order, the
+// classification), then the junction-line seeds (seed_junctions,
+// src/v2m_particles.cpp:80-244) and the corner nodes. The seeds stay in `mem`.
 struct Seeds {
     int64_t n = 0;
     float* P = nullptr;
     uint16_t* lab = nullptr;
     uint8_t* typ = nullptr;
     uint16_t* part = nullptr;
+    uint16_t* part3 = nullptr;
 };
 
-Seeds run_seeding(const tn_grid_view* g, const uint8_t* grade, const tn_relax_params* prm, int dev, DevMem& mem,
-                  bool fill) {
+// Device scans and sorts of the seeding, on the null stream.
+struct SeedOps {
+    DevMem& mem;
+    // offs[0..m) holds counts; afterwards offs[0..m] their exclusive scan. Returns the total.
+    int32_t scan(int32_t* offs, int64_t m) {
+        const unsigned nb = blocks_for(m + 1, tnk::kScanTile);
+        int32_t* bsum = mem.alloc<int32_t>(nb);
+        tnk::k_scan_reduce<<<nb, tnk::kBlock>>>(offs, m + 1, bsum);
+        tnk::k_scan_top<<<1, tnk::kBlock>>>(bsum, nb);
+        tnk::k_scan_down<<<nb, tnk::kBlock>>>(offs, offs, m + 1, bsum);
+        TN_HIP(hipGetLastError());
+        int32_t total = 0;
+        TN_HIP(hipMemcpy(&total, offs + m, sizeof total, hipMemcpyDeviceToHost));
+        return total;
+    }
+    // a zeroed count array of m + 1 entries
+    int32_t* counts(int64_t m) {
+        int32_t* c = mem.alloc<int32_t>((size_t)m + 1);
+        tnk::k_fill_i32<<<blocks_for(m + 1, tnk::kBlock), tnk::kBlock>>>(c, m + 1, 0);
+        return c;
+    }
+    // Stable sort of (keys, vals) by keys in [0, 2^bits): LSD passes of 8 bits.
+    // vals == nullptr: the values are the element indices. The results are left in
+    // the returned pair (keys, vals); the inputs are overwritten.
+    std::pair<int32_t*, int32_t*> sort(int32_t* keys, int32_t* vals, int n, int bits) {
+        const int passes = (bits + 7) / 8;
+        const int nblk = (int)std::max<unsigned>(1, blocks_for(n, tnk::kRadixTile));
+        int32_t* bhist = mem.alloc<int32_t>((size_t)256 * nblk);
+        int32_t* k2 = mem.alloc<int32_t>(n);
+        int32_t* v2 = mem.alloc<int32_t>(n);
+        if (vals == nullptr) {
+            vals = mem.alloc<int32_t>(n);
+            k_iota<<<blocks_for(n, tnk::kBlock), tnk::kBlock>>>(vals, n);
+        }
+        for (int p = 0; p < passes; ++p) {
+            tnk::k_radix_hist<<<nblk, tnk::kBlock>>>(keys, n, 8 * p, nblk, bhist);
+            scan_exclusive(bhist, (int64_t)256 * nblk);
+            tnk::k_radix_scatter<<<nblk, tnk::kBlock>>>(keys, vals, n, 8 * p, nblk, bhist, k2, v2);
+            std::swap(keys, k2);
+            std::swap(vals, v2);
+        }
+        TN_HIP(hipGetLastError());
+        return { keys, vals };
+    }
+    void scan_exclusive(int32_t* a, int64_t m) {   // in place, no total
+        const unsigned nb = blocks_for(m, tnk::kScanTile);
+        int32_t* bsum = mem.alloc<int32_t>(nb);
+        tnk::k_scan_reduce<<<nb, tnk::kBlock>>>(a, m, bsum);
+        tnk::k_scan_top<<<1, tnk::kBlock>>>(bsum, nb);
+        tnk::k_scan_down<<<nb, tnk::kBlock>>>(a, a, m, bsum);
+    }
+};
+
+int bit_length(uint32_t x) {
+    int b = 0;
+    while (x) {
+        ++b;
+        x >>= 1;
+    }
+    return b;
+}
+
+// seed_junctions (src/v2m_particles.cpp:80-244): the candidates at the 3-label
+// vertices, one kept per (level, cell, label triple) -- the smallest vertex -- in
+// key order, and the lattice nodes within 0.4 h of a kept seed dropped. Writes the
+// result into `out` (capacity: the lattice count + the candidate count + extra)
+// and returns its node count.
+int64_t junction_pass(const tn_grid_view* g, const tnd::Field& f, const tnk::SeedArgs& sa, float cellfac,
+                      const tnk::VertexGrid& vg, const Seeds& lat, int64_t extra, SeedOps& ops, Seeds& out) {
+    DevMem& mem = ops.mem;
+    const int n = (int)lat.n;
+    int32_t* flag = ops.counts(vg.nvert);
+    tnk::k_vertex_mark<0><<<blocks_for(vg.nvert, tnk::kBlock), tnk::kBlock>>>(f, vg, flag);
+    const int m3 = ops.scan(flag, vg.nvert);
+    tnk::JunctionCands jc{};
+    jc.m = m3;
+    int nk = 0;
+    tnk::KeptSeeds ks{};
+    int32_t *sorted_bkey = nullptr, *sidx = nullptr;
+    if (m3 > 0) {
+        int32_t* list = mem.alloc<int32_t>(m3);
+        tnk::k_compact_index<<<blocks_for(vg.nvert, tnk::kBlock), tnk::kBlock>>>(flag, vg.nvert, list);
+        jc.x = mem.alloc<float>((size_t)3 * m3);
+        jc.lab3 = mem.alloc<int32_t>((size_t)3 * m3);
+        jc.key = mem.alloc<int32_t>((size_t)7 * m3);
+        jc.ok = mem.alloc<int32_t>((size_t)m3 + 1);
+        tnk::k_fill_i32<<<blocks_for(m3 + 1, tnk::kBlock), tnk::kBlock>>>(jc.ok, m3 + 1, 0);
+        tnk::k_junction_cand<<<blocks_for(m3, tnk::kBlock), tnk::kBlock>>>(f, sa, cellfac, vg, list, jc);
+        const int nc = ops.scan(jc.ok, m3);
+        if (nc > 0) {
+            // the candidates (in vertex order), compacted
+            int32_t* cl = mem.alloc<int32_t>(nc);
+            tnk::k_compact_index<<<blocks_for(m3, tnk::kBlock), tnk::kBlock>>>(jc.ok, m3, cl);
+            tnk::JunctionCands cc{};
+            cc.m = nc;
+            cc.x = mem.alloc<float>((size_t)3 * nc);
+            cc.lab3 = mem.alloc<int32_t>((size_t)3 * nc);
+            cc.key = mem.alloc<int32_t>((size_t)7 * nc);
+            tnk::k_gather_cands<<<blocks_for(nc, tnk::kBlock), tnk::kBlock>>>(jc, cl, cc);
+            // each key field's range, for the LSD passes
+            int32_t* range = mem.alloc<int32_t>(14);
+            tnk::k_key_range_init<<<1, 64>>>(range);
+            tnk::k_key_range<<<blocks_for(nc, tnk::kBlock), tnk::kBlock>>>(cc.key, nc, range);
+            int32_t hr[14];
+            TN_HIP(hipMemcpy(hr, range, sizeof hr, hipMemcpyDeviceToHost));
+            // stable passes from the least significant field up: ties stay in vertex order
+            int32_t* perm = mem.alloc<int32_t>(nc);
+            k_iota<<<blocks_for(nc, tnk::kBlock), tnk::kBlock>>>(perm, nc);
+            int32_t* fk = mem.alloc<int32_t>(nc);
+            for (int e = 6; e >= 0; --e) {
+                const uint32_t span = (uint32_t)((int64_t)hr[2 * e + 1] - hr[2 * e]);
+                if (span == 0) {
+                    continue;
+                }
+                tnk::k_gather_field<<<blocks_for(nc, tnk::kBlock), tnk::kBlock>>>(cc.key + (size_t)e * nc, perm, nc,
+                                                                                 -hr[2 * e], fk);
+                const auto r = ops.sort(fk, perm, nc, bit_length(span));
+                fk = r.first;
+                perm = r.second;
+            }
+            int32_t* head = ops.counts(nc);
+            tnk::k_key_heads<<<blocks_for(nc, tnk::kBlock), tnk::kBlock>>>(cc.key, perm, nc, head);
+            nk = ops.scan(head, nc);
+            // the drop hash of the kept seeds (src/v2m_particles.cpp:160-181)
+            ks.nk = nk;
+            ks.cell = 0.4f * g->hmax;
+            ks.ox = g->vs[0];
+            ks.oy = g->vs[1];
+            ks.oz = g->vs[2];
+            const float ext[3] = { (g->nx + 1) * g->vs[0], (g->ny + 1) * g->vs[1], (g->nz + 1) * g->vs[2] };
+            int64_t cells = 1;
+            for (int e = 0; e < 3; ++e) {
+                ks.hd[e] = (int)(ext[e] / ks.cell) + 2;
+                cells *= ks.hd[e];
+            }
+            if (cells >= ((int64_t)1 << 31)) {
+                throw TnError(TN_ERR_UNSUPPORTED, "seeding: the junction-seed hash needs more than 2^31 cells "
+                                                  "(hmax too small for the grid)");
+            }
+            ks.x = mem.alloc<float>((size_t)3 * nk);
+            ks.lab3 = mem.alloc<int32_t>((size_t)3 * nk);
+            ks.bkey = mem.alloc<int32_t>(nk);
+            tnk::k_keep_seeds<<<blocks_for(nc, tnk::kBlock), tnk::kBlock>>>(cc, perm, head, ks);
+            int32_t* bk = mem.alloc<int32_t>(nk);
+            TN_HIP(hipMemcpy(bk, ks.bkey, sizeof(int32_t) * nk, hipMemcpyDeviceToDevice));
+            const auto r = ops.sort(bk, nullptr, nk, bit_length((uint32_t)(cells - 1)));
+            sorted_bkey = r.first;
+            sidx = r.second;
+        }
+    }
+    int32_t* keep = ops.counts(n);
+    int n1 = n;
+    if (n > 0) {
+        if (nk > 0) {
+            tnk::k_drop_near_seeds<<<blocks_for(n, tnk::kBlock), tnk::kBlock>>>(f, n, lat.P, lat.typ, ks, sorted_bkey,
+                                                                              sidx, keep);
+        } else {
+            tnk::k_fill_i32<<<blocks_for(n, tnk::kBlock), tnk::kBlock>>>(keep, n, 1);
+        }
+        n1 = ops.scan(keep, n);
+    }
+    const int64_t total = (int64_t)n1 + nk;
+    out.n = total;
+    const size_t cap = (size_t)std::max<int64_t>(total + extra, 1);
+    out.P = mem.alloc<float>(3 * cap);
+    out.lab = mem.alloc<uint16_t>(cap);
+    out.typ = mem.alloc<uint8_t>(cap);
+    out.part = mem.alloc<uint16_t>(2 * cap);
+    out.part3 = mem.alloc<uint16_t>(cap);
+    if (n + nk > 0) {
+        tnk::SeedNodes in{ lat.P, lat.lab, lat.typ, lat.part, lat.part3 };
+        tnk::SeedNodes o{ out.P, out.lab, out.typ, out.part, out.part3 };
+        tnk::k_merge_junctions<<<blocks_for((int64_t)n + nk, tnk::kBlock), tnk::kBlock>>>(in, n, keep, ks, o);
+        TN_HIP(hipGetLastError());
+    }
+    return total;
+}
+
+Seeds run_seeding(const tn_grid_view* g, const uint8_t* grade, const tn_relax_params* prm, int dev, DevMem& mem) {
     check_grid(g);
     arg_check(prm != nullptr && grade != nullptr, "seeding: null argument");
-    if (prm->jseed > 0.0f || prm->corners) {
-        throw TnError(TN_ERR_UNSUPPORTED, "seeding: junction-line seeds (jseed > 0) and corner nodes are not "
-                                          "generated on the device; set jseed = 0 and corners = 0");
-    }
     arg_check(prm->nseed >= 1 && prm->nseed <= 64, "seeding: nseed out of range");
+    arg_check(prm->jseed >= 0.0f && std::isfinite(prm->jseed), "seeding: jseed must be finite and >= 0");
     TN_HIP(hipSetDevice(dev));
     mem.device = dev;
+    SeedOps ops{ mem };
     const tnd::Field f = upload_grid(g, mem);
     const int64_t nv = (int64_t)g->nx * g->ny * g->nz;
     // v2m_seed_spacing (src/opencl/v2m_seed_body.cl:445-447), with the host's powf
@@ -624,29 +865,51 @@ Seeds run_seeding(const tn_grid_view* g, const uint8_t* grade, const tn_relax_pa
     sa.grade = mem.upload(grade, (size_t)nv);
     sa.spacing = mem.upload(spacing.data(), spacing.size());
     sa.nseed = prm->nseed;
-    int32_t* offs = mem.alloc<int32_t>((size_t)nv + 1);
-    int32_t* bsum = mem.alloc<int32_t>(blocks_for(nv + 1, tnk::kScanTile));
-    tnk::k_fill_i32<<<blocks_for(nv + 1, tnk::kBlock), tnk::kBlock>>>(offs, nv + 1, 0);
-    tnk::k_seed_count<<<blocks_for(nv, tnk::kBlock), tnk::kBlock>>>(f, sa, offs);
-    const unsigned nb = blocks_for(nv + 1, tnk::kScanTile);
-    tnk::k_scan_reduce<<<nb, tnk::kBlock>>>(offs, nv + 1, bsum);
-    tnk::k_scan_top<<<1, tnk::kBlock>>>(bsum, nb);
-    tnk::k_scan_down<<<nb, tnk::kBlock>>>(offs, offs, nv + 1, bsum);
-    TN_HIP(hipGetLastError());
-    int32_t total = 0;
-    TN_HIP(hipMemcpy(&total, offs + nv, sizeof total, hipMemcpyDeviceToHost));
-    Seeds s;
-    s.n = total;
-    if (!fill || total == 0) {
-        return s;
+    tnk::VertexGrid vg;
+    vg.vx1 = g->nx + 1;
+    vg.vy1 = g->ny + 1;
+    vg.nvert = (int64_t)vg.vx1 * vg.vy1 * (g->nz + 1);
+    // the corner vertices first: their count sizes the arrays the lattice goes into
+    int32_t* clist = nullptr;
+    int nc = 0;
+    if (prm->corners) {
+        int32_t* cflag = ops.counts(vg.nvert);
+        tnk::k_vertex_mark<1><<<blocks_for(vg.nvert, tnk::kBlock), tnk::kBlock>>>(f, vg, cflag);
+        nc = ops.scan(cflag, vg.nvert);
+        if (nc > 0) {
+            clist = mem.alloc<int32_t>(nc);
+            tnk::k_compact_index<<<blocks_for(vg.nvert, tnk::kBlock), tnk::kBlock>>>(cflag, vg.nvert, clist);
+        }
     }
-    s.P = mem.alloc<float>((size_t)3 * total);
-    s.lab = mem.alloc<uint16_t>(total);
-    s.typ = mem.alloc<uint8_t>(total);
-    s.part = mem.alloc<uint16_t>((size_t)2 * total);
-    tnk::k_seed_fill<<<blocks_for(nv, tnk::kBlock), tnk::kBlock>>>(f, sa, offs, s.P, s.lab);
-    tnk::k_seed_classify<<<blocks_for(total, tnk::kBlock), tnk::kBlock>>>(f, total, s.P, s.lab, s.typ, s.part);
-    TN_HIP(hipGetLastError());
+    int32_t* offs = ops.counts(nv);
+    tnk::k_seed_count<<<blocks_for(nv, tnk::kBlock), tnk::kBlock>>>(f, sa, offs);
+    const int32_t total = ops.scan(offs, nv);
+    const bool junctions = prm->jseed > 0.0f;
+    Seeds lat;
+    lat.n = total;
+    const size_t cap = (size_t)std::max<int64_t>(junctions ? total : (int64_t)total + nc, 1);
+    lat.P = mem.alloc<float>(3 * cap);
+    lat.lab = mem.alloc<uint16_t>(cap);
+    lat.typ = mem.alloc<uint8_t>(cap);
+    lat.part = mem.alloc<uint16_t>(2 * cap);
+    lat.part3 = mem.alloc<uint16_t>(cap);
+    if (total > 0) {
+        tnk::k_seed_fill<<<blocks_for(nv, tnk::kBlock), tnk::kBlock>>>(f, sa, offs, lat.P, lat.lab);
+        tnk::k_seed_classify<<<blocks_for(total, tnk::kBlock), tnk::kBlock>>>(f, total, lat.P, lat.lab, lat.typ,
+                                                                              lat.part);
+        TN_HIP(hipMemset(lat.part3, 0xFF, sizeof(uint16_t) * total));   // kNoLab
+        TN_HIP(hipGetLastError());
+    }
+    Seeds s = lat;
+    if (junctions) {
+        junction_pass(g, f, sa, prm->jseed, vg, lat, nc, ops, s);
+    }
+    if (nc > 0) {
+        tnk::SeedNodes o{ s.P, s.lab, s.typ, s.part, s.part3 };
+        tnk::k_corner_fill<<<blocks_for(nc, tnk::kBlock), tnk::kBlock>>>(f, vg, clist, nc, o, (int)s.n);
+        TN_HIP(hipGetLastError());
+        s.n += nc;
+    }
     TN_HIP(hipDeviceSynchronize());
     return s;
 }
@@ -717,6 +980,13 @@ int tn_plan_create(const tn_grid_view* g, const tn_relax_params* prm, const tn_n
         p->tskin = prm->t + prm->skin;
         p->H = make_hash(g, prm, &p->nkeys);
         TN_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+        TN_HIP(hipEventCreateWithFlags(&p->ev_iter, hipEventDisableTiming));
+        for (hipStream_t& s : p->side) {
+            TN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        }
+        for (hipEvent_t* e : { &p->ev_easy, &p->ev_rest, &p->ev_int }) {
+            TN_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        }
         p->timer.stream = p->stream;
         p->timer.on = prm->verbose || std::getenv("TN_KERNEL_TIMING") != nullptr;
         const int n = p->n;
@@ -753,7 +1023,9 @@ int tn_plan_create(const tn_grid_view* g, const tn_relax_params* prm, const tn_n
         p->dNnb = p->alloc<int32_t>(n);
         p->dQueue = p->alloc<int32_t>(n);
         p->dMoveQ = p->alloc<int32_t>(n);
+        p->dMoveQi = p->alloc<int32_t>(n);
         p->dMoveQ2 = p->alloc<int32_t>(n);
+        p->dMoveQd = p->alloc<int32_t>(n);
         p->dFlag = p->alloc<uint8_t>(n);
         p->dCstart = p->alloc<int32_t>((size_t)p->nkeys + 1);
         for (int k = 0; k < 2; ++k) {
@@ -983,16 +1255,16 @@ int tn_seed_count(const tn_grid_view* g, const uint8_t* grade, const tn_relax_pa
     return guarded([&] {
         arg_check(n != nullptr, "null count");
         DevMem mem;
-        *n = run_seeding(g, grade, prm, pick_device(device), mem, false).n;
+        *n = run_seeding(g, grade, prm, pick_device(device), mem).n;
     });
 }
 
 int tn_seed(const tn_grid_view* g, const uint8_t* grade, const tn_relax_params* prm, int32_t device, int64_t n,
-            float* P, uint16_t* lab, uint8_t* typ, uint16_t* part) {
+            float* P, uint16_t* lab, uint8_t* typ, uint16_t* part, uint16_t* part3) {
     return guarded([&] {
         arg_check(n >= 0 && (n == 0 || (P && lab && typ && part)), "seeding: null output array");
         DevMem mem;
-        const Seeds s = run_seeding(g, grade, prm, pick_device(device), mem, true);
+        const Seeds s = run_seeding(g, grade, prm, pick_device(device), mem);
         arg_check(s.n == n, "seeding: the output holds " + std::to_string(n) + " nodes, the seeding made " +
                                 std::to_string(s.n) + " (tn_seed_count)");
         if (s.n == 0) {
@@ -1002,6 +1274,9 @@ int tn_seed(const tn_grid_view* g, const uint8_t* grade, const tn_relax_params* 
         TN_HIP(hipMemcpy(lab, s.lab, sizeof(uint16_t) * s.n, hipMemcpyDeviceToHost));
         TN_HIP(hipMemcpy(typ, s.typ, s.n, hipMemcpyDeviceToHost));
         TN_HIP(hipMemcpy(part, s.part, sizeof(uint16_t) * 2 * s.n, hipMemcpyDeviceToHost));
+        if (part3) {
+            TN_HIP(hipMemcpy(part3, s.part3, sizeof(uint16_t) * s.n, hipMemcpyDeviceToHost));
+        }
     });
 }
 
@@ -1020,6 +1295,71 @@ int tn_relax(const tn_grid_view* g, const tn_relax_params* prm, tn_nodes_view* n
     tn_plan_destroy(p);
     g_err = err;
     return rc;
+}
+
+int tn_relax_batch(const tn_grid_view* grids, const tn_relax_params* prms, tn_nodes_view* nodes,
+                   tn_relax_stats* stats, int32_t count, int32_t device) {
+    return guarded([&] {
+        arg_check(count >= 0 && (count == 0 || (grids && prms && nodes)), "batch: null arrays");
+        struct Destroy {
+            void operator()(tn_plan* p) const {
+                tn_plan_destroy(p);
+            }
+        };
+        std::vector<std::unique_ptr<tn_plan, Destroy>> plans;
+        plans.reserve(count);
+        for (int k = 0; k < count; ++k) {   // each plan: grid and seeds uploaded, first rebuild done
+            tn_plan* p = nullptr;
+            const int rc = tn_plan_create(&grids[k], &prms[k], &nodes[k], device, &p);
+            if (rc != TN_OK) {
+                throw TnError(rc, "volume " + std::to_string(k) + ": " + g_err);
+            }
+            plans.emplace_back(p);
+        }
+        // one host thread advances every volume as its last iteration completes; the
+        // volumes' streams keep the device busy meanwhile
+        std::vector<tn_plan*> active;
+        for (auto& p : plans) {
+            if (p->more()) {
+                p->enqueue_iteration();
+                active.push_back(p.get());
+            }
+        }
+        while (!active.empty()) {
+            bool progress = false;
+            for (size_t k = 0; k < active.size();) {
+                tn_plan* p = active[k];
+                const hipError_t q = hipEventQuery(p->ev_iter);
+                if (q == hipErrorNotReady) {
+                    ++k;
+                    continue;
+                }
+                TN_HIP(q);
+                progress = true;
+                if (p->finish_iteration() && p->more()) {
+                    p->enqueue_iteration();
+                    ++k;
+                } else {
+                    TN_HIP(hipStreamSynchronize(p->stream));
+                    p->timer.collect();
+                    active[k] = active.back();
+                    active.pop_back();
+                }
+            }
+            if (!progress) {
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
+            }
+        }
+        for (int k = 0; k < count; ++k) {
+            plans[k]->fill_stats(stats ? &stats[k] : nullptr);
+        }
+        for (int k = 0; k < count; ++k) {
+            const int rc = tn_plan_download(plans[k].get(), &nodes[k]);
+            if (rc != TN_OK) {
+                throw TnError(rc, "volume " + std::to_string(k) + ": " + g_err);
+            }
+        }
+    });
 }
 
 }  // extern "C"

@@ -24,7 +24,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("TN_LIBRARY") or os.path.join(HERE, "lib", "libtrussnet_amd.so")
 
 # every entry point of include/trussnet_amd.h
-SYMBOLS = ["tn_relax_params_default", "tn_last_error", "tn_device_count", "tn_relax", "tn_seed_count", "tn_seed",
+SYMBOLS = ["tn_relax_params_default", "tn_last_error", "tn_device_count", "tn_relax", "tn_relax_batch",
+           "tn_seed_count", "tn_seed",
            "tn_plan_create", "tn_plan_run", "tn_plan_download", "tn_plan_destroy", "tn_plan_hash_info",
            "tn_plan_reset", "tn_plan_rebuild", "tn_plan_force", "tn_plan_step", "tn_plan_set_timing",
            "tn_plan_timings"]
@@ -61,6 +62,8 @@ def lib():
         L.tn_device_count.argtypes = [C.POINTER(C.c_int32)]
         L.tn_relax.argtypes = [C.POINTER(GridView), C.POINTER(RelaxParamsC), C.POINTER(NodesView),
                                C.POINTER(RelaxStatsC), C.c_int32]
+        L.tn_relax_batch.argtypes = [C.POINTER(GridView), C.POINTER(RelaxParamsC), C.POINTER(NodesView),
+                                     C.POINTER(RelaxStatsC), C.c_int32, C.c_int32]
         L.tn_plan_create.argtypes = [C.POINTER(GridView), C.POINTER(RelaxParamsC), C.POINTER(NodesView), C.c_int32,
                                      C.POINTER(vp)]
         L.tn_plan_run.argtypes = [vp, C.POINTER(RelaxStatsC)]
@@ -76,7 +79,7 @@ def lib():
         L.tn_seed_count.argtypes = [C.POINTER(GridView), vp, C.POINTER(RelaxParamsC), C.c_int32,
                                     C.POINTER(C.c_int64)]
         L.tn_seed.argtypes = [C.POINTER(GridView), vp, C.POINTER(RelaxParamsC), C.c_int32, C.c_int64, vp, vp, vp,
-                              vp]
+                              vp, vp]
         for name in SYMBOLS:
             if name != "tn_last_error" and name != "tn_relax_params_default":
                 getattr(L, name).restype = C.c_int
@@ -125,13 +128,28 @@ def relax(grid: Grid, nodes: Nodes, prm: RelaxParams | None = None, device: int 
     return out, st.as_dict()
 
 
-def seed(grid: Grid, prm: RelaxParams | None = None, device: int = -1) -> Nodes:
-    """The graded HCP lattice seeds of seed_cpu, classified, made on the device.
+def relax_batch(grids: list, nodes: list, prms=None, device: int = -1) -> list:
+    """Relax many volumes concurrently on one device (tn_relax_batch): [(nodes, stats)]
+    per volume, each equal to relax(grids[k], nodes[k], prms[k], device)."""
+    count = len(grids)
+    if len(nodes) != count:
+        raise ValueError("one node set per grid")
+    if prms is None or isinstance(prms, RelaxParams):
+        prms = [prms or RelaxParams()] * count
+    outs = [nd.copy() for nd in nodes]
+    gv = (GridView * max(count, 1))(*[g.view() for g in grids])
+    pc = (RelaxParamsC * max(count, 1))(*[p.to_c() for p in prms])
+    nv = (NodesView * max(count, 1))(*[o.view() for o in outs])
+    st = (RelaxStatsC * max(count, 1))()
+    _check(lib().tn_relax_batch(gv, pc, nv, st, count, device))
+    return [(outs[k], st[k].as_dict()) for k in range(count)]
 
-    Needs ``grid.grade``; junction-line seeds and corner nodes are not made, so
-    ``prm.jseed`` must be 0 and ``prm.corners`` False.
-    """
-    prm = prm or RelaxParams(jseed=0.0, corners=False)
+
+def seed(grid: Grid, prm: RelaxParams | None = None, device: int = -1) -> Nodes:
+    """seed_cpu on the device (src/v2m_particles.cpp:625-726): the graded HCP lattice
+    seeds, classified, then the junction-line seeds (``prm.jseed`` > 0) and the
+    corner nodes (``prm.corners``), with part3 filled. Needs ``grid.grade``."""
+    prm = prm or RelaxParams()
     if grid.grade is None:
         raise ValueError("seeding needs the grid's grades")
     g = grid.normalized() if grid.grade.dtype != np.uint8 else grid
@@ -144,8 +162,10 @@ def seed(grid: Grid, prm: RelaxParams | None = None, device: int = -1) -> Nodes:
     lab = np.zeros(n, np.uint16)
     typ = np.zeros(n, np.uint8)
     part = np.zeros((n, 2), np.uint16)
-    _check(lib().tn_seed(C.byref(gv), _p(grade), C.byref(pc), device, n, _p(P), _p(lab), _p(typ), _p(part)))
-    return Nodes(P, lab, typ, part)
+    part3 = np.zeros(n, np.uint16)
+    _check(lib().tn_seed(C.byref(gv), _p(grade), C.byref(pc), device, n, _p(P), _p(lab), _p(typ), _p(part),
+                         _p(part3)))
+    return Nodes(P, lab, typ, part, part3)
 
 
 class Plan:

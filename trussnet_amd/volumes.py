@@ -96,3 +96,35 @@ def nested_ellipsoids(n: int = 256, nlab: int = 6, seed: int = 0, axes=(20.0, 12
             sl = lab[z0:z1]
             sl[s < 0] = k + 1
     return lab
+
+
+_M64 = (1 << 64) - 1
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    """SplitMix64 (Steele, Lea and Flood, OOPSLA 2014) of each 64-bit state: the
+    generator's output for counter x (state = x * golden gamma)."""
+    with np.errstate(over="ignore"):
+        z = (x.astype(np.uint64) + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def jittered_points(n: int = 64, spacing: float = 3.0, jitter: float = 0.25, seed: int = 0,
+                    vs=(1.0, 1.0, 1.0)) -> np.ndarray:
+    """A cubic grid of points at `spacing` (mm) over the volume's voxel centres
+    [0, (n-1) v], each coordinate moved by U(-jitter, jitter) * spacing from a
+    counter-based generator: point i (x fastest), axis a draws counter
+    seed * 2^32 + 3 i + a of SplitMix64, whose top 24 bits give u in [0, 1).
+    Returns float32 (m, 3) positions in mm (the small plain workload's seeds before
+    labelling and classification)."""
+    ext = (n - 1) * np.asarray(vs, np.float64)
+    axes = [np.arange(0.0, e + 1e-9, spacing) for e in ext]
+    z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    base = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    m = base.shape[0]
+    ctr = (np.uint64(seed) << np.uint64(32)) + np.arange(3 * m, dtype=np.uint64)
+    u = (splitmix64(ctr) >> np.uint64(40)).astype(np.float64) * 2.0 ** -24
+    off = (2.0 * u - 1.0).reshape(m, 3) * jitter * spacing
+    return (base + off).astype(np.float32)
