@@ -42,6 +42,10 @@ def lib():
         L.ref_grid_build.restype = vp
         L.ref_grid_build.argtypes = [C.POINTER(C.c_uint16), C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float),
                                      C.POINTER(RefGridParams), C.POINTER(C.c_float)]
+        L.ref_grid_build_prob.restype = vp
+        L.ref_grid_build_prob.argtypes = [C.POINTER(C.c_uint16), C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int,
+                                          C.c_int, C.POINTER(C.c_float), C.POINTER(RefGridParams),
+                                          C.POINTER(C.c_float)]
         L.ref_grid_from_view.restype = vp
         L.ref_grid_from_view.argtypes = [C.POINTER(GridView)]
         L.ref_grid_free.argtypes = [vp]
@@ -79,17 +83,21 @@ class RefGrid:
 
     @staticmethod
     def build(labels: np.ndarray, vs=(1.0, 1.0, 1.0), *, sigma=1.0, hbase=0.0, hmin=0.0, hmax=0.0, K_=3.0, g=0.3,
-              hvox: np.ndarray | None = None) -> "RefGrid":
-        """build_grid_cpu (src/v2m_grid.cpp:47) on a (nz, ny, nx) label array."""
+              hvox: np.ndarray | None = None, prob: np.ndarray | None = None) -> "RefGrid":
+        """build_grid_cpu (src/v2m_grid.cpp:47) on a (nz, ny, nx) label array; with
+        prob (C, nz, ny, nx), a tissue-probability input (label l's field from prob[l])."""
         nz, ny, nx = labels.shape
         lab = np.ascontiguousarray(labels, np.uint16)
         vsa = np.array(vs, np.float32)
         gp = RefGridParams(sigma, hbase, hmin, hmax, K_, g)
         hv = None if hvox is None else np.ascontiguousarray(hvox, np.float32)
-        h = lib().ref_grid_build(lab.ctypes.data_as(C.POINTER(C.c_uint16)), nx, ny, nz,
-                                 vsa.ctypes.data_as(C.POINTER(C.c_float)), C.byref(gp),
-                                 None if hv is None else hv.ctypes.data_as(C.POINTER(C.c_float)))
-        return RefGrid(h)
+        pr = None if prob is None else np.ascontiguousarray(prob, np.float32)
+        h = lib().ref_grid_build_prob(lab.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                      None if pr is None else pr.ctypes.data_as(C.POINTER(C.c_float)),
+                                      0 if pr is None else pr.shape[0], nx, ny, nz,
+                                      vsa.ctypes.data_as(C.POINTER(C.c_float)), C.byref(gp),
+                                      None if hv is None else hv.ctypes.data_as(C.POINTER(C.c_float)))
+        return RefGrid(h, keep=pr)
 
     @staticmethod
     def from_grid(grid: Grid) -> "RefGrid":

@@ -138,13 +138,30 @@ const char* ref_last_error(void) {
     return g_err.c_str();
 }
 
+void* ref_grid_build_prob(const uint16_t* labels, const float* prob, int nprob, int nx, int ny, int nz,
+                          const float* vs, const ref_grid_params* gp, const float* hvox);
+
 // build_grid_cpu over a label volume (x fastest); hvox may be null
 void* ref_grid_build(const uint16_t* labels, int nx, int ny, int nz, const float* vs, const ref_grid_params* gp,
                      const float* hvox) {
+    return ref_grid_build_prob(labels, nullptr, 0, nx, ny, nz, vs, gp, hvox);
+}
+
+// the same with per-label probabilities (a tissue-probability input, label-major
+// prob[l * nv + v], label 0 the exterior): the interface fields are then the
+// smoothed probabilities (src/v2m_grid.cpp:154-168, 467)
+void* ref_grid_build_prob(const uint16_t* labels, const float* prob, int nprob, int nx, int ny, int nz,
+                          const float* vs, const ref_grid_params* gp, const float* hvox) {
     try {
         RefGrid* r = new RefGrid;
         const size_t nv = static_cast<size_t>(nx) * ny * nz;
         r->lv.data.assign(labels, labels + nv);
+
+        if (prob && nprob > 0) {
+            r->lv.prob.assign(prob, prob + nv * nprob);
+            r->lv.nprob = nprob;
+        }
+
         r->lv.nx = nx;
         r->lv.ny = ny;
         r->lv.nz = nz;

@@ -70,6 +70,11 @@ CASES = {
                                                                grf_amp=1.0, centre_jitter=2.0),
                          grid=dict(hbase=4.0, hmin=1.5, hmax=4.0), hvox=(1.5, 4.0, 0.25),
                          runs=[(1, 0.0, True), (30, 0.0, True), (200, 2e-3, True)]),
+    # a tissue-probability input: the softmax of six Gaussian random fields (channel 0
+    # the exterior), labels by argmax, each label's field its smoothed probability
+    "tpm32": dict(vol=lambda: volumes.tpm_softmax((32, 34, 30), 6, 3.0, 1)[1],
+                  prob=lambda: volumes.tpm_softmax((32, 34, 30), 6, 3.0, 1)[0], grid=dict(hbase=3.0, hmin=2.0),
+                  runs=[(1, 0.0, True), (30, 0.0, True), (200, 2e-3, True)]),
     # anisotropic voxels (0.8 x 0.9 x 1.3 mm): every coordinate -> voxel division is inexact
     "aniso40": dict(vol=lambda: volumes.sectors(40), vs=(0.8, 0.9, 1.3), grid=dict(hbase=2.5),
                     runs=[(1, 0.0, True), (20, 0.0, True), (150, 2e-3, True), (30, 0.0, False)]),
@@ -93,7 +98,8 @@ def make(name: str, spec: dict) -> str:
     if "hvox" in spec:
         lo, hi, slope = spec["hvox"]
         hvox = graded_size(lab, lo, hi, slope).ravel()
-    rg = ref.RefGrid.build(lab, vs=spec.get("vs", (1.0, 1.0, 1.0)), hvox=hvox, **spec["grid"])
+    prob = spec["prob"]() if "prob" in spec else None
+    rg = ref.RefGrid.build(lab, vs=spec.get("vs", (1.0, 1.0, 1.0)), hvox=hvox, prob=prob, **spec["grid"])
     G = rg.grid()
     prm = RelaxParams()
     hcp = rg.seed(prm)

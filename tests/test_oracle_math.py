@@ -154,7 +154,7 @@ def test_move_from_rest_follows_the_force(golden):
     wl = np.linalg.norm(want, axis=1)
     cap = prm.maxstep * h
     want = np.where((wl > cap)[:, None], want * (cap / np.maximum(wl, 1e-30))[:, None], want)
-    assert free.sum() > 0.3 * n
+    assert free.sum() > 0.2 * n   # enough free interior nodes to test
     # positions are float32: the displacement is known to a few units of their last place
     ulp = float(np.spacing(np.float32(np.abs(P0).max())))
     assert np.allclose(s[free], want[free], rtol=2e-5, atol=2 * ulp)

@@ -1027,16 +1027,22 @@ __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArg
                                                                 const int32_t* __restrict__ qcount,
                                                                 int32_t* __restrict__ defer_count,
                                                                 int32_t* __restrict__ defer_queue) {
-    // a fixed grid strides over the queue, whose length only the device knows. The
-    // lean modes: kBlock nodes per block. kGeneral: kRestLanes nodes per wave: these
-    // few thousand nodes follow long, divergent paths (junction projections), and a
-    // wave executes every path its lanes take, so spreading them thinly over many
-    // waves shortens the longest wave.
+    // a fixed grid strides over the queue, whose length only the device knows, with
+    // as few nodes per wave as the grid allows (at most 64, or kRestLanes for the
+    // general step): a short queue's nodes follow long, divergent paths (projections
+    // onto interfaces and junction curves), and a wave executes every path its lanes
+    // take, so spreading them thinly over many waves shortens the longest wave.
     constexpr bool kLean = kMode != kGeneral;
+    constexpr int kWaves = kBlock / kWave;
+    constexpr int kMaxLanes = kLean ? kWave : kRestLanes;
     const int count = *qcount;
-    const int per_block = kLean ? kBlock : kRestLanes * (kBlock / kWave);
+    const int waves = (int)gridDim.x * kWaves;
+    const int spread = (count + waves - 1) / waves;
+    // (a long queue fills whole waves: its lanes then share the waves' issue slots)
+    const int lpw = spread < 1 ? 1 : (spread > kRestLanes ? kMaxLanes : spread);
+    const int per_block = lpw * kWaves;
     const int lane = threadIdx.x & (kWave - 1);
-    const int slot = kLean ? (int)threadIdx.x : (lane < kRestLanes ? (threadIdx.x >> 6) * kRestLanes + lane : -1);
+    const int slot = lane < lpw ? (threadIdx.x >> 6) * lpw + lane : -1;
     for (int base = blockIdx.x * per_block; base < count; base += gridDim.x * per_block) {
         const int g = base + slot;
         bool defer = false;
@@ -1052,7 +1058,7 @@ __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArg
 }
 
 constexpr int kMoveGrid = 2048;       // blocks of a lean k_move at most
-constexpr int kMoveRestGrid = 1024;   // blocks of k_move<kGeneral> (4 waves of kRestLanes nodes each)
+constexpr int kMoveRestGrid = 1024;   // blocks of k_move<kGeneral> (4 waves of up to kRestLanes nodes each)
 
 
 // ---- per-iteration reductions + Verlet bookkeeping ----------------------------------

@@ -128,3 +128,18 @@ def jittered_points(n: int = 64, spacing: float = 3.0, jitter: float = 0.25, see
     u = (splitmix64(ctr) >> np.uint64(40)).astype(np.float64) * 2.0 ** -24
     off = (2.0 * u - 1.0).reshape(m, 3) * jitter * spacing
     return (base + off).astype(np.float32)
+
+
+def tpm_softmax(shape=(182, 218, 182), channels: int = 6, sigma: float = 6.0, seed: int = 1,
+                gain: float = 1.0) -> tuple[np.ndarray, np.ndarray]:
+    """A tissue-probability map: the softmax over `channels` Gaussian random fields
+    (white noise smoothed by `sigma` voxels, unit std, times `gain`), channel 0 the
+    exterior. `shape` is (nx, ny, nz). Returns (prob float32 (C, nz, ny, nx),
+    labels uint16 (nz, ny, nx) = the argmax channel)."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = shape
+    f = np.stack([gaussian_field((nz, ny, nx), sigma, rng) * np.float32(gain) for _ in range(channels)])
+    f -= f.max(axis=0, keepdims=True)
+    np.exp(f, out=f)
+    f /= f.sum(axis=0, keepdims=True)
+    return f.astype(np.float32), np.argmax(f, axis=0).astype(np.uint16)

@@ -62,11 +62,14 @@ def grade_of(h: np.ndarray, hmin: float, hmax: float) -> np.ndarray:
 
 
 def build_grid(labels: np.ndarray, h, vs=(1.0, 1.0, 1.0), sigma: float = 1.0, R: int = 6,
-               hmin: float | None = None, hmax: float | None = None) -> Grid:
+               hmin: float | None = None, hmax: float | None = None, prob: np.ndarray | None = None) -> Grid:
     """A relaxation grid from a (nz, ny, nx) label volume and a sizing field (array or scalar, mm).
 
     R: the brick label radius (Grid::R: the reference's default is 6, from its
     wider curvature smoothing); phi itself is smoothed with radius ceil(3 sigma).
+    prob: per-label probabilities (C, nz, ny, nx) of a tissue-probability input;
+    label l's field is then its smoothed probability instead of its smoothed
+    indicator (build_grid_cpu's TPM source, src/v2m_grid.cpp:154-168).
     """
     from scipy.ndimage import correlate1d
 
@@ -109,7 +112,10 @@ def build_grid(labels: np.ndarray, h, vs=(1.0, 1.0, 1.0), sigma: float = 1.0, R:
         rows = mixed_idx[near[k, mixed_idx]]
         if rows.size == 0:
             continue
-        f = (L == l).astype(np.float32)
+        if prob is None:
+            f = (L == l).astype(np.float32)
+        else:
+            f = np.array(prob[l], np.float32) if l < prob.shape[0] else np.zeros(L.shape, np.float32)
         for ax in range(3):
             f = correlate1d(f, taps, axis=ax, mode="constant", cval=1.0 if l == 0 else 0.0)
         blocks = _bricks_of(_pad_to_bricks(f, np.float32(1.0 if l == 0 else 0.0))).reshape(nb, SLOT)
@@ -167,10 +173,19 @@ def plain_large(seed: int = 0, n: int = 256) -> Grid:
     return build_grid(lab, graded_size(lab, 1.5, 6.0, LARGE_SLOPE), hmin=1.5, hmax=6.0)
 
 
+def prob_large(seed: int = 1, shape=(182, 218, 182), h: float = 1.8) -> Grid:
+    """The probability-input workload: a 182x218x182x6 softmax of six Gaussian random
+    fields (sigma 6 voxels, channel 0 the exterior), labels by argmax, each label's
+    interface field its smoothed probability; uniform size h voxels."""
+    prob, lab = volumes.tpm_softmax(shape, 6, 6.0, seed)
+    return build_grid(lab, np.float32(h), hmin=h, hmax=h, prob=prob)
+
+
 def batch_volumes(count: int = 64, seed: int = 0) -> list:
     """Many independent volumes: edge U(32, 160) voxels, 2-4 nested perturbed ellipsoid
     labels (semi-axes U(0.25, 0.5) x edge), a uniform node size U(1.5, 4) voxels per
-    volume. The default 64 volumes hold about 3.3 M seeds, 200 to 350 k per volume."""
+    volume. The default 64 volumes hold about 3.3 M seeds, from a few hundred to
+    350 k per volume."""
     rng = np.random.default_rng(seed)
     grids = []
     for k in range(count):
