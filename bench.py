@@ -147,13 +147,17 @@ def build_workload(name: str):
     if name == "large":
         grid = workload.plain_large()
         desc = ("plain_large: 256^3 six-label nested perturbed ellipsoids, h graded 1.5-6 voxels, "
-                "device HCP seeding, 100 iterations")
+                "device seeding (seed_cpu defaults), 100 iterations")
+    elif name == "prob":
+        grid = workload.prob_large()
+        desc = ("prob_large: 182x218x182x6 softmax of six Gaussian random fields (sigma 6, seed 1), labels by "
+                "argmax, fields = smoothed probabilities, uniform h 1.8, device seeding, 100 iterations")
     else:
         grid = workload.plain_small()
-        desc = "plain_small: 64^3 two spheres r 24/12, uniform h 3, device HCP seeding, 50 iterations"
+        desc = "plain_small: 64^3 two spheres r 24/12, uniform h 3, device seeding, 50 iterations"
     log(f"grid {grid.nx}x{grid.ny}x{grid.nz}: {grid.nslot} slots ({time.perf_counter() - t:.1f} s)")
     t = time.perf_counter()
-    seeds = relax.seed(grid, RelaxParams(jseed=0.0, corners=False), device=0)
+    seeds = relax.seed(grid, RelaxParams(), device=0)
     log(f"seeds: {seeds.n} ({time.perf_counter() - t:.1f} s)")
     return grid, seeds, desc
 
@@ -195,7 +199,7 @@ def bench_batch(args) -> int:
     t = time.perf_counter()
     grids = workload.batch_volumes(64, seed=0)
     log(f"64 grids ({time.perf_counter() - t:.1f} s)")
-    seeds = [relax.seed(g, RelaxParams(jseed=0.0, corners=False), device=0) for g in grids]
+    seeds = [relax.seed(g, RelaxParams(), device=0) for g in grids]
     ns = [sd.n for sd in seeds]
     total = sum(ns)
     log(f"seeds: {total} in 64 volumes ({min(ns)} .. {max(ns)})")
@@ -222,11 +226,89 @@ def bench_batch(args) -> int:
         "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "batch_volumes(64, seed 0): edges U(32,160), 2-4 nested ellipsoid labels, "
-                               "uniform h U(1.5,4) voxels, device HCP seeding, 100 iterations, uploads included",
+                               "uniform h U(2,5) voxels, device seeding (seed_cpu defaults), 100 iterations, uploads included",
                    "nodes": total, "volumes": 64, "iterations": iters, "parallelism": "single GPU"},
         "loop_value": total * iters / dt_loop, "identical_to_loop": same, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_adjoint(args) -> int:
+    """The differentiable relaxation: forward (iters Jacobi steps of the smoothed
+    truss, then the energy + quality loss) and the hand-written adjoint (gradients
+    with respect to the start positions and a 32^3 sizing grid, checkpoints every 8
+    steps), on the 128^3 three-label volume. A step is one forward and one backward."""
+    import numpy as np
+    import torch
+
+    from oracle import orc
+    from trussnet_amd import adjoint, relax, workload
+    from trussnet_amd._abi import RelaxParams
+
+    t = time.perf_counter()
+    grid = workload.adjoint_volume()
+    seeds = relax.seed(grid, RelaxParams(), device=0)
+    iters = args.iters or 64
+    prm = adjoint.AdjParams(iters=iters, ckpt_every=8)
+    pb, x0, theta = adjoint.build_problem(grid, seeds, prm, grid_pts=32, device=0)
+    n = seeds.n
+    log(f"adjoint problem: {n} nodes, {pb.tets.shape[0]} tets, theta {theta.shape} ({time.perf_counter() - t:.1f} s)")
+    xd = torch.from_numpy(x0).cuda()
+    td = torch.from_numpy(theta).cuda()
+    gx = torch.empty_like(xd)
+    gt = torch.empty_like(td)
+    for _ in range(args.warmup):
+        pb.forward_ptr(xd.data_ptr(), td.data_ptr())
+        pb.backward_ptr(td.data_ptr(), gx.data_ptr(), gt.data_ptr())
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        loss = pb.forward_ptr(xd.data_ptr(), td.data_ptr())
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t) / args.steps
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        pb.backward_ptr(td.data_ptr(), gx.data_ptr(), gt.data_ptr())
+    torch.cuda.synchronize()
+    t_bwd = (time.perf_counter() - t) / args.steps
+    log(f"forward {1e3 * t_fwd:.2f} ms, backward {1e3 * t_bwd:.2f} ms (x{t_bwd / t_fwd:.2f}), loss {loss:.6f}")
+    check = None
+    if not args.no_cpu:
+        # the FP64 CPU gradient and central differences (eps 1e-3 h) of the FP64 forward
+        pc, keep = adjoint.problem_struct(pb.nbr, pb.nnb, pb.typ, pb.tets, theta.shape, pb.origin, pb.spacing, prm)
+        t = time.perf_counter()
+        L64, gx64, gt64 = orc.adj_grad(pc, x0, theta)
+        t_cpu = time.perf_counter() - t
+        g1, g2 = gx.cpu().numpy().astype(np.float64), gt.cpu().numpy().astype(np.float64)
+        rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        rng = np.random.default_rng(0)
+        fd = []
+        for eps_h in (1e-3, 1e-5):
+            dx, dt = rng.standard_normal(x0.shape), rng.standard_normal(theta.shape)
+            eps = eps_h * float(np.mean(theta))
+            lp = orc.adj_loss(pc, x0 + eps * dx, theta + eps * dt)[0]
+            lm = orc.adj_loss(pc, x0 - eps * dx, theta - eps * dt)[0]
+            want = (lp - lm) / (2 * eps)
+            got = float((g1 * dx).sum() + (g2 * dt).sum())
+            fd.append({"eps_over_h": eps_h, "fd": want, "adjoint": got, "rel": abs(got - want) / abs(want)})
+        check = {"loss_fp64": L64, "loss_rel": abs(loss - L64) / abs(L64), "grad_x_rel": rel(g1, gx64),
+                 "grad_theta_rel": rel(g2, gt64), "fd": fd, "cpu_grad_s": t_cpu}
+        log(f"check: {check}")
+    line = {
+        "metric": "node-iterations/sec of the differentiable relaxation forward; adjoint cost ratio",
+        "value": n * iters / t_fwd, "unit": "node-iterations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * (t_fwd + t_bwd), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "adjoint_volume: 128^3 three nested perturbed ellipsoid labels, h 1.5, device seeding; "
+                               "64 smoothed Jacobi steps, checkpoints every 8, 32^3 trilinear sizing grid, "
+                               "loss = mean truss energy + mean (1 - smoothed Joe-Liu) over a Delaunay of the seeds",
+                   "nodes": n, "tets": int(pb.tets.shape[0]), "iterations": iters, "parallelism": "single GPU"},
+        "forward_ms": 1e3 * t_fwd, "backward_ms": 1e3 * t_bwd, "adjoint_cost_ratio": t_bwd / t_fwd,
+        "loss": loss, "check": check,
+    }
+    print(json.dumps(line), flush=True)
+    pb.close()
     return 0
 
 
@@ -259,7 +341,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["large", "small", "batch"], default="large")
+    ap.add_argument("--workload", choices=["large", "small", "prob", "batch", "adjoint"], default="large")
     ap.add_argument("--iters", type=int, default=0, help="iterations per step (default: the workload's)")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
@@ -273,13 +355,15 @@ def main() -> int:
 
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
     pmc = None if args.no_traffic else pmc_traffic(args)   # before this process initialises the GPU
 
     from trussnet_amd import relax
     from trussnet_amd._abi import RelaxParams
 
     grid, seeds, desc = build_workload(args.workload)
-    iters = args.iters or (100 if args.workload == "large" else 50)
+    iters = args.iters or (50 if args.workload == "small" else 100)
     prm = RelaxParams(max_iters=iters, dptol=0.0)
     plan = relax.Plan(grid, seeds, prm, device=0)
     log(f"plan: {seeds.n} nodes, hash {plan.hash.nlev} levels / {plan.hash.nkeys} cells")

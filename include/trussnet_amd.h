@@ -231,6 +231,45 @@ int tn_plan_rebuild(tn_plan* plan, int32_t* key, int32_t* sorted, int32_t* cstar
 int tn_plan_force(tn_plan* plan, float* F);
 int tn_plan_step(tn_plan* plan, int32_t iters, tn_relax_stats* st);
 
+/*
+ * A differentiable relaxation and its adjoint. No counterpart in the reference
+ * (its relaxation is not differentiable: hinge bars, nearest-voxel sizes,
+ * projections and FIRE resets); for callers that fit a sizing field or seed
+ * positions by gradient descent. The surrogate keeps the reference's bar force and
+ * Jacobi step (src/opencl/v2m_particle_body.cl:161-184, 844-870) with a softplus
+ * bar law, a smooth active-bar count and a trilinear sizing grid; interior nodes
+ * move, the others stay; lists and tets are fixed. After `iters` steps the loss is
+ * the mean truss energy plus wq x the mean (1 - smoothed Joe-Liu quality) of the
+ * tets (src/v2m_tetra.cpp:100-120). The adjoint recomputes the steps from
+ * checkpoints every `ckpt_every` steps. x0, theta, xT and the gradients are DEVICE
+ * pointers (x as 3n floats, AoS; theta[(z gy + y) gx + x]); stream may be null (the
+ * null stream). tn_adj_forward returns the loss on the host; tn_adj_backward needs
+ * the forward of the same inputs before it and gives dL/dx0 and dL/dtheta.
+ */
+typedef struct tn_adj_problem {
+    int64_t n;
+    const int32_t* nbr;     /* [n * TN_K] host: node-major lists (entries past nnb ignored) */
+    const int32_t* nnb;     /* [n] host */
+    const uint8_t* typ;     /* [n] host: TN_INTERIOR nodes move */
+    int64_t ntet;
+    const int32_t* tet;     /* [4 ntet] host: node indices */
+    int32_t gx, gy, gz;     /* the sizing grid */
+    float origin[3];        /* position of theta[0] (mm) */
+    float spacing[3];       /* grid spacing (mm) */
+    float dt, fscale, fsurf;
+    float beta;             /* softplus sharpness (1/mm) */
+    float wq;               /* weight of the quality term */
+    float eps_v;            /* volume smoothing (mm^3) */
+    int32_t iters, ckpt_every;
+} tn_adj_problem;
+
+typedef struct tn_adj tn_adj;
+int tn_adj_create(const tn_adj_problem* p, int32_t device, tn_adj** out);
+int tn_adj_forward(tn_adj* a, const float* x0, const float* theta, float* xT, double* loss, void* stream);
+int tn_adj_backward(tn_adj* a, const float* theta, float* grad_x0, float* grad_theta, void* stream);
+int tn_adj_loss_terms(const tn_adj* a, double* terms);   /* loss, energy, quality term of the last forward */
+int tn_adj_destroy(tn_adj* a);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,13 @@ void orc_field_points(const tn_grid_view* g, int32_t a, int32_t b, int64_t m, co
 /* the histogram bin of a move (src/v2m_particles.cpp:881) */
 int32_t orc_hist_bin(float mv);
 
+/* The differentiable relaxation of tn_adj_* in double precision (oracle/adjoint.c):
+ * the loss after p->iters steps (xT, terms may be null), and the loss with its
+ * gradients with respect to x0 [3n] and theta [gx gy gz]. */
+double orc_adj_loss(const tn_adj_problem* p, const double* x0, const double* theta, double* xT, double* terms);
+double orc_adj_grad(const tn_adj_problem* p, const double* x0, const double* theta, double* gx0, double* gtheta,
+                    double* terms);
+
 const char* orc_last_error(void);
 
 #ifdef __cplusplus

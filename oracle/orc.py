@@ -122,3 +122,38 @@ def field_points(grid: Grid, a: int, b: int, X: np.ndarray):
 
 def hist_bin(m: float) -> int:
     return int(lib().orc_hist_bin(m))
+
+
+def _adj_bind():
+    L = lib()
+    if not getattr(L, "_adj_bound", False):
+        from trussnet_amd.adjoint import AdjProblemC
+
+        L.orc_adj_loss.restype = C.c_double
+        L.orc_adj_loss.argtypes = [C.POINTER(AdjProblemC), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_adj_grad.restype = C.c_double
+        L.orc_adj_grad.argtypes = [C.POINTER(AdjProblemC)] + [C.c_void_p] * 5
+        L._adj_bound = True
+    return L
+
+
+def adj_loss(pc, x0: np.ndarray, theta: np.ndarray) -> tuple[float, np.ndarray, np.ndarray]:
+    """The differentiable relaxation's loss in double precision (oracle/adjoint.c):
+    (loss, x_T, [loss, energy, quality term]). pc: an AdjProblemC."""
+    x0 = np.ascontiguousarray(x0, np.float64)
+    th = np.ascontiguousarray(theta, np.float64)
+    xT = np.zeros_like(x0)
+    terms = np.zeros(3)
+    L = _adj_bind().orc_adj_loss(C.byref(pc), _p(x0), _p(th), _p(xT), _p(terms))
+    return float(L), xT, terms
+
+
+def adj_grad(pc, x0: np.ndarray, theta: np.ndarray) -> tuple[float, np.ndarray, np.ndarray]:
+    """(loss, dL/dx0, dL/dtheta) in double precision (reverse mode, every state kept)."""
+    x0 = np.ascontiguousarray(x0, np.float64)
+    th = np.ascontiguousarray(theta, np.float64)
+    gx = np.zeros_like(x0)
+    gt = np.zeros_like(th)
+    terms = np.zeros(3)
+    L = _adj_bind().orc_adj_grad(C.byref(pc), _p(x0), _p(th), _p(gx), _p(gt), _p(terms))
+    return float(L), gx, gt

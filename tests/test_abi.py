@@ -48,10 +48,11 @@ def test_struct_layouts_match_header(tmp_path):
     import ctypes as C
 
     from trussnet_amd import _abi
+    from trussnet_amd.adjoint import AdjProblemC
 
     structs = {"tn_grid_view": _abi.GridView, "tn_relax_params": _abi.RelaxParamsC, "tn_nodes_view": _abi.NodesView,
                "tn_relax_stats": _abi.RelaxStatsC, "tn_hash_info": _abi.HashInfoC,
-               "tn_kernel_time": _abi.KernelTimeC}
+               "tn_kernel_time": _abi.KernelTimeC, "tn_adj_problem": AdjProblemC}
     src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for cname, cls in structs.items():
         src.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

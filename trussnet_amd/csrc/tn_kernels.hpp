@@ -1027,19 +1027,23 @@ __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArg
                                                                 const int32_t* __restrict__ qcount,
                                                                 int32_t* __restrict__ defer_count,
                                                                 int32_t* __restrict__ defer_queue) {
-    // a fixed grid strides over the queue, whose length only the device knows, with
-    // as few nodes per wave as the grid allows (at most 64, or kRestLanes for the
-    // general step): a short queue's nodes follow long, divergent paths (projections
-    // onto interfaces and junction curves), and a wave executes every path its lanes
-    // take, so spreading them thinly over many waves shortens the longest wave.
+    // A fixed grid strides over the queue, whose length only the device knows. A
+    // short queue's nodes follow long, divergent paths (projections onto interfaces
+    // and junction curves), and a wave executes every path its lanes take, so they
+    // are spread thinly over the grid's waves (lean passes: from one node per wave;
+    // the general pass: from kRestLanes, measured best); a long queue fills whole
+    // waves, whose lanes then share the issue slots.
     constexpr bool kLean = kMode != kGeneral;
     constexpr int kWaves = kBlock / kWave;
-    constexpr int kMaxLanes = kLean ? kWave : kRestLanes;
     const int count = *qcount;
     const int waves = (int)gridDim.x * kWaves;
     const int spread = (count + waves - 1) / waves;
-    // (a long queue fills whole waves: its lanes then share the waves' issue slots)
-    const int lpw = spread < 1 ? 1 : (spread > kRestLanes ? kMaxLanes : spread);
+    int lpw;
+    if (kLean) {
+        lpw = spread < 1 ? 1 : (spread > kRestLanes ? kWave : spread);
+    } else {
+        lpw = spread < kRestLanes ? kRestLanes : (spread > kWave ? kWave : spread);
+    }
     const int per_block = lpw * kWaves;
     const int lane = threadIdx.x & (kWave - 1);
     const int slot = lane < lpw ? (threadIdx.x >> 6) * lpw + lane : -1;
@@ -1058,7 +1062,7 @@ __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArg
 }
 
 constexpr int kMoveGrid = 2048;       // blocks of a lean k_move at most
-constexpr int kMoveRestGrid = 1024;   // blocks of k_move<kGeneral> (4 waves of up to kRestLanes nodes each)
+constexpr int kMoveRestGrid = 512;    // blocks of k_move<kGeneral>: the waves resident at its register budget
 
 
 // ---- per-iteration reductions + Verlet bookkeeping ----------------------------------

@@ -12,6 +12,7 @@
 #include <thread>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -327,6 +328,10 @@ struct tn_plan {
     // interface pass: ev_easy starts them, ev_rest and ev_int join them back
     hipStream_t side[2] = { nullptr, nullptr };
     hipEvent_t ev_easy = nullptr, ev_rest = nullptr, ev_int = nullptr;
+    // false: the three passes run one after another on the plan's stream (many plans
+    // at once keep the device busy by themselves; cross-stream waits of many plans
+    // would queue behind each other on the few hardware queues)
+    bool overlap_moves = true;
     int nparts = 0, radix_passes = 0, radix_nblk = 0, refresh_blocks = 1;
     // host-side loop state
     int rebuilds = 0;
@@ -480,32 +485,43 @@ struct tn_plan {
         timer.end(KC_MOVE_EASY, t);
         // the queued nodes: three passes over disjoint queues at once, each bounded by
         // its slowest node, then the general pass over the nodes handed back
-        TN_HIP(hipEventRecord(ev_easy, stream));
         const unsigned g1 = std::min<unsigned>(nb, tnk::kMoveGrid), g2 = std::min<unsigned>(nb, tnk::kMoveRestGrid);
-        TN_HIP(hipStreamWaitEvent(side[0], ev_easy, 0));
-        t = timer.begin(side[0]);
-        tnk::k_move<tnd::kGeneral><<<g2, tnk::kBlock, 0, side[0]>>>(f, ma, dCtl, dMoveQ2, dF, dX, dV, dTyp, dPart, dLab,
-                                                                    dMv, dPw, &dCtl->nmove_rest, nullptr, nullptr);
-        timer.end(KC_MOVE_REST, t, side[0]);
-        TN_HIP(hipEventRecord(ev_rest, side[0]));
-        TN_HIP(hipStreamWaitEvent(side[1], ev_easy, 0));
-        t = timer.begin(side[1]);
-        tnk::k_move<tnd::kLeanInterior><<<g1, tnk::kBlock, 0, side[1]>>>(f, ma, dCtl, dMoveQi, dF, dX, dV, dTyp, dPart,
-                                                                         dLab, dMv, dPw, &dCtl->nmove_int,
-                                                                         &dCtl->nmove_def, dMoveQd);
-        timer.end(KC_MOVE_INT, t, side[1]);
-        TN_HIP(hipEventRecord(ev_int, side[1]));
+        const hipStream_t s0 = overlap_moves ? side[0] : stream, s1 = overlap_moves ? side[1] : stream;
+        if (overlap_moves) {
+            TN_HIP(hipEventRecord(ev_easy, stream));
+            TN_HIP(hipStreamWaitEvent(s0, ev_easy, 0));
+        }
+        t = timer.begin(s0);
+        tnk::k_move<tnd::kGeneral><<<g2, tnk::kBlock, 0, s0>>>(f, ma, dCtl, dMoveQ2, dF, dX, dV, dTyp, dPart, dLab, dMv,
+                                                               dPw, &dCtl->nmove_rest, nullptr, nullptr);
+        timer.end(KC_MOVE_REST, t, s0);
+        if (overlap_moves) {
+            TN_HIP(hipEventRecord(ev_rest, s0));
+            TN_HIP(hipStreamWaitEvent(s1, ev_easy, 0));
+        }
+        t = timer.begin(s1);
+        tnk::k_move<tnd::kLeanInterior><<<g1, tnk::kBlock, 0, s1>>>(f, ma, dCtl, dMoveQi, dF, dX, dV, dTyp, dPart, dLab,
+                                                                    dMv, dPw, &dCtl->nmove_int, &dCtl->nmove_def,
+                                                                    dMoveQd);
+        timer.end(KC_MOVE_INT, t, s1);
+        if (overlap_moves) {
+            TN_HIP(hipEventRecord(ev_int, s1));
+        }
         t = timer.begin();
         tnk::k_move<tnd::kLeanInterface><<<g1, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQ, dF, dX, dV, dTyp, dPart,
                                                                          dLab, dMv, dPw, &dCtl->nmove, &dCtl->nmove_def,
                                                                          dMoveQd);
         timer.end(KC_MOVE, t);
-        TN_HIP(hipStreamWaitEvent(stream, ev_int, 0));
+        if (overlap_moves) {
+            TN_HIP(hipStreamWaitEvent(stream, ev_int, 0));
+        }
         t = timer.begin();
         tnk::k_move<tnd::kGeneral><<<g2, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQd, dF, dX, dV, dTyp, dPart, dLab,
                                                                    dMv, dPw, &dCtl->nmove_def, nullptr, nullptr);
         timer.end(KC_MOVE_DEF, t);
-        TN_HIP(hipStreamWaitEvent(stream, ev_rest, 0));
+        if (overlap_moves) {
+            TN_HIP(hipStreamWaitEvent(stream, ev_rest, 0));
+        }
         t = timer.begin();
         tnk::k_stats<<<nparts, tnk::kBlock, 0, stream>>>(dX, dP0, dMv, dPw, n, prm.skin, fire, dThr, dCtl, dFlag,
                                                          dPartials);
@@ -1301,6 +1317,7 @@ int tn_relax_batch(const tn_grid_view* grids, const tn_relax_params* prms, tn_no
                    tn_relax_stats* stats, int32_t count, int32_t device) {
     return guarded([&] {
         arg_check(count >= 0 && (count == 0 || (grids && prms && nodes)), "batch: null arrays");
+        const auto t_start = std::chrono::steady_clock::now();
         struct Destroy {
             void operator()(tn_plan* p) const {
                 tn_plan_destroy(p);
@@ -1316,9 +1333,14 @@ int tn_relax_batch(const tn_grid_view* grids, const tn_relax_params* prms, tn_no
             }
             plans.emplace_back(p);
         }
+        const auto t_created = std::chrono::steady_clock::now();
         // one host thread advances every volume as its last iteration completes; the
         // volumes' streams keep the device busy meanwhile
         std::vector<tn_plan*> active;
+        for (auto& p : plans) {
+            p->overlap_moves = count == 1;
+        }
+        int64_t polls = 0, steps = 0;
         for (auto& p : plans) {
             if (p->more()) {
                 p->enqueue_iteration();
@@ -1330,12 +1352,14 @@ int tn_relax_batch(const tn_grid_view* grids, const tn_relax_params* prms, tn_no
             for (size_t k = 0; k < active.size();) {
                 tn_plan* p = active[k];
                 const hipError_t q = hipEventQuery(p->ev_iter);
+                ++polls;
                 if (q == hipErrorNotReady) {
                     ++k;
                     continue;
                 }
                 TN_HIP(q);
                 progress = true;
+                ++steps;
                 if (p->finish_iteration() && p->more()) {
                     p->enqueue_iteration();
                     ++k;
@@ -1350,6 +1374,7 @@ int tn_relax_batch(const tn_grid_view* grids, const tn_relax_params* prms, tn_no
                 std::this_thread::sleep_for(std::chrono::microseconds(20));
             }
         }
+        const auto t_looped = std::chrono::steady_clock::now();
         for (int k = 0; k < count; ++k) {
             plans[k]->fill_stats(stats ? &stats[k] : nullptr);
         }
@@ -1359,7 +1384,15 @@ int tn_relax_batch(const tn_grid_view* grids, const tn_relax_params* prms, tn_no
                 throw TnError(rc, "volume " + std::to_string(k) + ": " + g_err);
             }
         }
+        if (std::getenv("TN_BATCH_VERBOSE")) {
+            const auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+            std::fprintf(stderr, "[tn_relax_batch] %d volumes: plans %.1f ms, loop %.1f ms (%lld iterations, %lld polls), "
+                         "download %.1f ms\n", count, ms(t_start, t_created), ms(t_created, t_looped),
+                         (long long)steps, (long long)polls, ms(t_looped, std::chrono::steady_clock::now()));
+        }
     });
 }
 
 }  // extern "C"
+
+#include "tn_adjoint.hpp"

@@ -28,7 +28,8 @@ SYMBOLS = ["tn_relax_params_default", "tn_last_error", "tn_device_count", "tn_re
            "tn_seed_count", "tn_seed",
            "tn_plan_create", "tn_plan_run", "tn_plan_download", "tn_plan_destroy", "tn_plan_hash_info",
            "tn_plan_reset", "tn_plan_rebuild", "tn_plan_force", "tn_plan_step", "tn_plan_set_timing",
-           "tn_plan_timings"]
+           "tn_plan_timings", "tn_adj_create", "tn_adj_forward", "tn_adj_backward", "tn_adj_loss_terms",
+           "tn_adj_destroy"]
 
 
 class DeviceError(RuntimeError):

@@ -181,17 +181,24 @@ def prob_large(seed: int = 1, shape=(182, 218, 182), h: float = 1.8) -> Grid:
     return build_grid(lab, np.float32(h), hmin=h, hmax=h, prob=prob)
 
 
+def adjoint_volume(n: int = 128, seed: int = 5, h: float = 1.5) -> Grid:
+    """The differentiation workload: 128^3, three nested perturbed ellipsoid labels
+    (semi-axes U(20, 60) voxels), uniform size h voxels (about 200 k seeds)."""
+    lab = volumes.nested_ellipsoids(n, nlab=3, seed=seed, axes=(20.0, 60.0), grf_sigma=3.0, grf_amp=2.0,
+                                    centre_jitter=3.0)
+    return build_grid(lab, np.float32(h), hmin=h, hmax=h)
+
+
 def batch_volumes(count: int = 64, seed: int = 0) -> list:
     """Many independent volumes: edge U(32, 160) voxels, 2-4 nested perturbed ellipsoid
-    labels (semi-axes U(0.25, 0.5) x edge), a uniform node size U(1.5, 4) voxels per
-    volume. The default 64 volumes hold about 3.3 M seeds, from a few hundred to
-    350 k per volume."""
+    labels (semi-axes U(0.25, 0.5) x edge), a uniform node size U(2, 5) voxels per
+    volume."""
     rng = np.random.default_rng(seed)
     grids = []
     for k in range(count):
         n = int(rng.integers(32, 161))
         nlab = int(rng.integers(2, 5))
-        h = float(rng.uniform(1.5, 4.0))
+        h = float(rng.uniform(2.0, 5.0))
         lab = volumes.nested_ellipsoids(n, nlab=nlab, seed=int(rng.integers(1 << 30)),
                                         axes=(0.25 * n, 0.5 * n), grf_sigma=3.0, grf_amp=2.0,
                                         centre_jitter=0.03 * n)
