@@ -37,7 +37,7 @@ using tnk::kWave;
 
 struct AdjArgs {
     int n;
-    const int32_t* __restrict__ nbr;     // [n * K] node-major
+    const int32_t* __restrict__ nbr;     // [K * n] list position-major: entry k of node i at k * n + i
     const int32_t* __restrict__ nnb;     // [n]
     const uint8_t* __restrict__ typ;     // [n]
     const int32_t* __restrict__ rstart;  // [n + 1] reverse lists: the bars pointing at each node
@@ -47,6 +47,7 @@ struct AdjArgs {
     const int32_t* __restrict__ vtet;    // entries 4 t + corner
     int ntet;
     const float* __restrict__ theta;
+    const float4* __restrict__ B;        // [n] the start positions; states hold displacements from them
     int gx, gy, gz;
     float ox, oy, oz, dx, dy, dz;
     float dt, fscale, fsurf, beta, wq, eps_v;
@@ -172,11 +173,14 @@ struct Bar {
     bool ok;
 };
 
-__device__ __forceinline__ Bar bar_at(const AdjArgs& A, float4 xi, float4 xj, float c) {
+// bar i -> j from the start positions b and the states x (displacement, h): the
+// start positions' difference is exact for neighbours (Sterbenz), so e keeps the
+// displacements' precision however far the volume is from the origin
+__device__ __forceinline__ Bar bar_at(const AdjArgs& A, float4 bi, float4 xi, float4 bj, float4 xj, float c) {
     Bar b;
-    b.ex = xi.x - xj.x;
-    b.ey = xi.y - xj.y;
-    b.ez = xi.z - xj.z;
+    b.ex = (bi.x - bj.x) + (xi.x - xj.x);
+    b.ey = (bi.y - bj.y) + (xi.y - xj.y);
+    b.ez = (bi.z - bj.z) + (xi.z - xj.z);
     b.l = sqrtf(b.ex * b.ex + b.ey * b.ey + b.ez * b.ez);
     b.ok = b.l > 0.0f;
     const float u = c * 0.5f * (xi.w + xj.w) - b.l;
@@ -185,32 +189,36 @@ __device__ __forceinline__ Bar bar_at(const AdjArgs& A, float4 xi, float4 xj, fl
     return b;
 }
 
-// F_i and D_i of node i at the state X (xyz, h)
+// F_i and D_i of node i at the state X (xyz, h). The bar forces of a relaxed node
+// nearly cancel, and the adjoint weighs F_i itself (kappa = mu . F / D^2): the sum
+// runs in double so F_i keeps its relative precision.
 __device__ __forceinline__ float4 force_and_stiffness(const AdjArgs& A, const float4* __restrict__ X, int i) {
-    const float4 xi = X[i];
+    const float4 xi = X[i], bi = A.B[i];
     const int ti = A.typ[i], m = A.nnb[i];
-    float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f, a = 0.0f;
+    double Fx = 0.0, Fy = 0.0, Fz = 0.0, a = 0.0;
     for (int k = 0; k < m; ++k) {
-        const int j = A.nbr[(size_t)i * TN_K + k];
-        const Bar b = bar_at(A, xi, X[j], bar_c(A, ti, A.typ[j]));
+        const int j = A.nbr[(size_t)k * A.n + i];
+        const Bar b = bar_at(A, bi, xi, A.B[j], X[j], bar_c(A, ti, A.typ[j]));
         if (b.ok) {
-            const float w = b.sp / b.l;
+            const double w = (double)b.sp / (double)b.l;
             Fx += w * b.ex;
             Fy += w * b.ey;
             Fz += w * b.ez;
             a += b.sg;
         }
     }
-    return make_float4(Fx, Fy, Fz, 1.0f + a);
+    return make_float4((float)Fx, (float)Fy, (float)Fz, (float)(1.0 + a));
 }
 
 // ---- forward ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_adj_load(AdjArgs A, const float* __restrict__ x0, float4* __restrict__ X) {
+__global__ __launch_bounds__(kBlock) void k_adj_load(AdjArgs A, const float* __restrict__ x0, float4* __restrict__ B,
+                                                     float4* __restrict__ X) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < A.n) {
         const float x = x0[3 * (size_t)i], y = x0[3 * (size_t)i + 1], z = x0[3 * (size_t)i + 2];
-        X[i] = make_float4(x, y, z, size_of(A, x, y, z));
+        B[i] = make_float4(x, y, z, 0.0f);
+        X[i] = make_float4(0.0f, 0.0f, 0.0f, size_of(A, x, y, z));
     }
 }
 
@@ -231,15 +239,24 @@ __global__ __launch_bounds__(kBlock) void k_adj_step(AdjArgs A, const float4* __
         x.x += s * fd.x;
         x.y += s * fd.y;
         x.z += s * fd.z;
-        x.w = size_of(A, x.x, x.y, x.z);
+        const float4 b = A.B[i];
+        x.w = size_of(A, b.x + x.x, b.y + x.y, b.z + x.z);
     }
     Xn[i] = x;
 }
 
-__global__ __launch_bounds__(kBlock) void k_adj_unload(int n, const float4* __restrict__ X, float* __restrict__ out) {
+// out = B + X (positions), or X alone (B null: an adjoint), 3 floats per node
+__global__ __launch_bounds__(kBlock) void k_adj_unload(int n, const float4* __restrict__ B,
+                                                       const float4* __restrict__ X, float* __restrict__ out) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) {
-        const float4 x = X[i];
+        float4 x = X[i];
+        if (B) {
+            const float4 b = B[i];
+            x.x += b.x;
+            x.y += b.y;
+            x.z += b.z;
+        }
         out[3 * (size_t)i] = x.x;
         out[3 * (size_t)i + 1] = x.y;
         out[3 * (size_t)i + 2] = x.z;
@@ -254,8 +271,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_unload(int n, const float4* __re
 struct D3 {
     double x, y, z;
 };
-__device__ __forceinline__ D3 d3(float4 v) {
-    return D3{ (double)v.x, (double)v.y, (double)v.z };
+__device__ __forceinline__ D3 d3(float4 b, float4 x) {
+    return D3{ (double)b.x + (double)x.x, (double)b.y + (double)x.y, (double)b.z + (double)x.z };
 }
 __device__ __forceinline__ D3 sub3(D3 a, D3 b) {
     return D3{ a.x - b.x, a.y - b.y, a.z - b.z };
@@ -274,7 +291,8 @@ struct TetQ {
 
 __device__ __forceinline__ TetQ tet_quality(const AdjArgs& A, const float4* __restrict__ X, int t) {
     const int4 v = *reinterpret_cast<const int4*>(A.tet + 4 * (size_t)t);
-    const D3 p0 = d3(X[v.x]), p1 = d3(X[v.y]), p2 = d3(X[v.z]), p3 = d3(X[v.w]);
+    const D3 p0 = d3(A.B[v.x], X[v.x]), p1 = d3(A.B[v.y], X[v.y]), p2 = d3(A.B[v.z], X[v.z]),
+             p3 = d3(A.B[v.w], X[v.w]);
     TetQ r;
     r.a = sub3(p1, p0);
     r.b = sub3(p2, p0);
@@ -293,12 +311,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_loss_parts(AdjArgs A, const floa
     __shared__ double se[kBlock / kWave], sq[kBlock / kWave];
     double e = 0.0, q = 0.0;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < A.n; i += gridDim.x * kBlock) {
-        const float4 xi = X[i];
+        const float4 xi = X[i], bi = A.B[i];
         const int ti = A.typ[i], m = A.nnb[i];
         float s = 0.0f;
         for (int k = 0; k < m; ++k) {
-            const int j = A.nbr[(size_t)i * TN_K + k];
-            const Bar b = bar_at(A, xi, X[j], bar_c(A, ti, A.typ[j]));
+            const int j = A.nbr[(size_t)k * A.n + i];
+            const Bar b = bar_at(A, bi, xi, A.B[j], X[j], bar_c(A, ti, A.typ[j]));
             if (b.ok) {
                 s += 0.5f * b.sp * b.sp;
             }
@@ -328,15 +346,27 @@ __global__ __launch_bounds__(kBlock) void k_adj_loss_parts(AdjArgs A, const floa
     }
 }
 
-// one block: the loss from the partials, in block order
-__global__ void k_adj_loss_sum(AdjArgs A, const double2* __restrict__ parts, int m, double* __restrict__ out) {
-    if (threadIdx.x == 0) {
-        double e = 0.0, q = 0.0;
-        for (int k = 0; k < m; ++k) {
-            e += parts[k].x;
-            q += parts[k].y;
+// one block: the loss from the partials (a fixed reduction tree)
+__global__ __launch_bounds__(kBlock) void k_adj_loss_sum(AdjArgs A, const double2* __restrict__ parts, int m,
+                                                         double* __restrict__ out) {
+    __shared__ double se[kBlock], sq[kBlock];
+    double e = 0.0, q = 0.0;
+    for (int k = threadIdx.x; k < m; k += kBlock) {
+        e += parts[k].x;
+        q += parts[k].y;
+    }
+    se[threadIdx.x] = e;
+    sq[threadIdx.x] = q;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            se[threadIdx.x] += se[threadIdx.x + w];
+            sq[threadIdx.x] += sq[threadIdx.x + w];
         }
-        const double E = A.n > 0 ? e / A.n : 0.0, Q = A.ntet > 0 ? q / A.ntet : 0.0;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double E = A.n > 0 ? se[0] / A.n : 0.0, Q = A.ntet > 0 ? sq[0] / A.ntet : 0.0;
         out[0] = E + (double)A.wq * Q;
         out[1] = E;
         out[2] = Q;
@@ -353,16 +383,16 @@ __global__ __launch_bounds__(kBlock) void k_adj_loss_grad(AdjArgs A, const float
         p = A.n - 1;
     }
     const float en = 1.0f / (float)A.n;
-    const float4 xp = X[p];
+    const float4 xp = X[p], bp = A.B[p];
     const int tp = A.typ[p];
     float3 g = make_float3(0.0f, 0.0f, 0.0f);
     float hd = 0.0f;
     // own bars p -> j: dE/de = -(1/n) sp s n, dE/dl0 = (1/n) sp s
     const int m = A.nnb[p];
     for (int k = 0; k < m; ++k) {
-        const int j = A.nbr[(size_t)p * TN_K + k];
+        const int j = A.nbr[(size_t)k * A.n + p];
         const float c = bar_c(A, tp, A.typ[j]);
-        const Bar b = bar_at(A, xp, X[j], c);
+        const Bar b = bar_at(A, bp, xp, A.B[j], X[j], c);
         if (b.ok) {
             const float w = en * b.sp * b.sg;
             g.x -= w * b.ex / b.l;
@@ -375,7 +405,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_loss_grad(AdjArgs A, const float
     for (int r = A.rstart[p]; r < A.rstart[p + 1]; ++r) {
         const int i = A.rsrc[r] / TN_K;
         const float c = bar_c(A, A.typ[i], tp);
-        const Bar b = bar_at(A, X[i], xp, c);
+        const Bar b = bar_at(A, A.B[i], X[i], bp, xp, c);
         if (b.ok) {
             const float w = en * b.sp * b.sg;
             g.x += w * b.ex / b.l;
@@ -387,7 +417,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_loss_grad(AdjArgs A, const float
     // tets: d(-wq/ntet q)/dx_p, q = 12 (3 Vs)^(2/3) / S (in double)
     const double wt = A.ntet > 0 ? -(double)A.wq / (double)A.ntet : 0.0;
     double tx = 0.0, ty = 0.0, tz = 0.0;
-    const D3 pp = d3(xp);
+    const D3 pp = d3(bp, xp);
     for (int r = A.vstart[p]; r < A.vstart[p + 1]; ++r) {
         const int t = A.vtet[r] >> 2, corner = A.vtet[r] & 3;
         const TetQ Q = tet_quality(A, X, t);
@@ -409,7 +439,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_loss_grad(AdjArgs A, const float
         D3 ds{ 0.0, 0.0, 0.0 };
         for (int o = 0; o < 4; ++o) {
             if (o != corner) {
-                const D3 xo = d3(X[vv[o]]);
+                const D3 xo = d3(A.B[vv[o]], X[vv[o]]);
                 ds.x += 2.0 * (pp.x - xo.x);
                 ds.y += 2.0 * (pp.y - xo.y);
                 ds.z += 2.0 * (pp.z - xo.z);
@@ -424,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_loss_grad(AdjArgs A, const float
     g.x += (float)tx;
     g.y += (float)ty;
     g.z += (float)tz;
-    size_adjoint(A, xp.x, xp.y, xp.z, valid ? hd : 0.0f, g, gtheta);
+    size_adjoint(A, bp.x + xp.x, bp.y + xp.y, bp.z + xp.z, valid ? hd : 0.0f, g, gtheta);
     if (valid) {
         lam[p] = make_float4(g.x, g.y, g.z, 0.0f);
     }
@@ -453,28 +483,36 @@ __device__ __forceinline__ float4 nu_kappa(const AdjArgs& A, int ti, float4 lamn
     return make_float4(nu.x, nu.y, nu.z, (nu.x * fd.x + nu.y * fd.y + nu.z * fd.z) / fd.w);
 }
 
-// lambda_k from lambda_{k+1} at the state X_k (with F, D of every node at X_k)
+__global__ __launch_bounds__(kBlock) void k_adj_nu(AdjArgs A, const float4* __restrict__ FD,
+                                                   const float4* __restrict__ lamn, float4* __restrict__ nk) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < A.n) {
+        nk[i] = nu_kappa(A, A.typ[i], lamn[i], FD[i]);
+    }
+}
+
+// lambda_k from lambda_{k+1} at the state X_k (nk: each node's (nu, kappa) at X_k)
 __global__ __launch_bounds__(kBlock) void k_adj_back(AdjArgs A, const float4* __restrict__ X,
-                                                     const float4* __restrict__ FD, const float4* __restrict__ lamn,
+                                                     const float4* __restrict__ nk, const float4* __restrict__ lamn,
                                                      float4* __restrict__ lam, double* __restrict__ gtheta) {
     int p = blockIdx.x * kBlock + threadIdx.x;
     const bool valid = p < A.n;   // the others stand in for node n - 1: every lane joins the scatter
     if (!valid) {
         p = A.n - 1;
     }
-    const float4 xp = X[p];
+    const float4 xp = X[p], bp = A.B[p];
     const int tp = A.typ[p];
     const float4 lp = lamn[p];
     float3 g = make_float3(lp.x, lp.y, lp.z);
     float hd = 0.0f;
-    const float4 nkp = nu_kappa(A, tp, lp, FD[p]);
     if (tp == TN_INTERIOR) {   // own bars p -> j
+        const float4 nkp = nk[p];
         const float3 nu = make_float3(nkp.x, nkp.y, nkp.z);
         const int m = A.nnb[p];
         for (int k = 0; k < m; ++k) {
-            const int j = A.nbr[(size_t)p * TN_K + k];
+            const int j = A.nbr[(size_t)k * A.n + p];
             const float c = bar_c(A, tp, A.typ[j]);
-            const Bar b = bar_at(A, xp, X[j], c);
+            const Bar b = bar_at(A, bp, xp, A.B[j], X[j], c);
             if (b.ok) {
                 const float4 ge = bar_adjoint(b, nu, nkp.w, A.beta);
                 g.x += ge.x;
@@ -490,18 +528,18 @@ __global__ __launch_bounds__(kBlock) void k_adj_back(AdjArgs A, const float4* __
         if (ti != TN_INTERIOR) {
             continue;
         }
-        const float4 nk = nu_kappa(A, ti, lamn[i], FD[i]);
+        const float4 nki = nk[i];
         const float c = bar_c(A, ti, tp);
-        const Bar b = bar_at(A, X[i], xp, c);
+        const Bar b = bar_at(A, A.B[i], X[i], bp, xp, c);
         if (b.ok) {
-            const float4 ge = bar_adjoint(b, make_float3(nk.x, nk.y, nk.z), nk.w, A.beta);
+            const float4 ge = bar_adjoint(b, make_float3(nki.x, nki.y, nki.z), nki.w, A.beta);
             g.x -= ge.x;
             g.y -= ge.y;
             g.z -= ge.z;
             hd += 0.5f * c * ge.w;
         }
     }
-    size_adjoint(A, xp.x, xp.y, xp.z, valid ? hd : 0.0f, g, gtheta);
+    size_adjoint(A, bp.x + xp.x, bp.y + xp.y, bp.z + xp.z, valid ? hd : 0.0f, g, gtheta);
     if (valid) {
         lam[p] = make_float4(g.x, g.y, g.z, 0.0f);
     }
@@ -531,7 +569,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_targets(int n, const int32_t* __
     }
     const int i = (int)(e / TN_K), k = (int)(e % TN_K);
     if (k < nnb[i]) {
-        const int j = nbr[e];
+        const int j = nbr[(size_t)k * n + i];
         key[e] = j;
         atomicAdd(&cnt[j], 1);
     } else {
@@ -560,7 +598,7 @@ struct tn_adj {
     float4* ckpts = nullptr;   // [nseg * n]: the state at the start of each segment
     float4* seg = nullptr;     // [ckpt * n]: a segment's states, recomputed
     float4* fd = nullptr;      // [ckpt * n]: F and D of those states
-    float4 *Xa = nullptr, *Xb = nullptr, *lamA = nullptr, *lamB = nullptr;
+    float4 *Xa = nullptr, *Xb = nullptr, *lamA = nullptr, *lamB = nullptr, *nk = nullptr, *base = nullptr;
     double* gtheta = nullptr;
     double2* parts = nullptr;
     double* dloss = nullptr;
@@ -630,7 +668,15 @@ int tn_adj_create(const tn_adj_problem* p, int32_t device, tn_adj** out) {
         SeedOps ops{ mem };
         tnadj::AdjArgs& A = a->A;
         A.n = n;
-        A.nbr = mem.upload(p->nbr, (size_t)n * TN_K);
+        {   // the lists, list position-major (coalesced per position across nodes)
+            std::vector<int32_t> t((size_t)n * TN_K, 0);
+            for (int i = 0; i < n; ++i) {
+                for (int k = 0; k < p->nnb[i]; ++k) {
+                    t[(size_t)k * n + i] = p->nbr[(size_t)i * TN_K + k];
+                }
+            }
+            A.nbr = mem.upload(t.data(), t.size());
+        }
         A.nnb = mem.upload(p->nnb, (size_t)n);
         A.typ = mem.upload(p->typ, (size_t)n);
         A.ntet = a->ntet;
@@ -682,6 +728,9 @@ int tn_adj_create(const tn_adj_problem* p, int32_t device, tn_adj** out) {
         a->Xb = mem.alloc<float4>(n);
         a->lamA = mem.alloc<float4>(n);
         a->lamB = mem.alloc<float4>(n);
+        a->nk = mem.alloc<float4>(n);
+        a->base = mem.alloc<float4>(n);
+        A.B = a->base;
         a->gtheta = mem.alloc<double>(a->gcells);
         a->nparts = (int)std::min<unsigned>(2048, blocks_for(std::max(n, a->ntet), tnk::kBlock));
         a->parts = mem.alloc<double2>(a->nparts);
@@ -703,7 +752,7 @@ int tn_adj_forward(tn_adj* a, const float* x0, const float* theta, float* xT, do
         A.theta = theta;
         // state k lives in ckpts[k / ckpt] when k is a segment start, else in Xa / Xb
         float4* cur = a->ckpts;
-        tnadj::k_adj_load<<<nb, tnk::kBlock, 0, s>>>(A, x0, cur);
+        tnadj::k_adj_load<<<nb, tnk::kBlock, 0, s>>>(A, x0, a->base, cur);
         for (int k = 0; k < a->iters; ++k) {
             const int k1 = k + 1;
             float4* nxt = (k1 % a->ckpt == 0 && k1 < a->iters) ? a->ckpts + (size_t)(k1 / a->ckpt) * n
@@ -712,9 +761,9 @@ int tn_adj_forward(tn_adj* a, const float* x0, const float* theta, float* xT, do
             cur = nxt;
         }
         tnadj::k_adj_loss_parts<<<a->nparts, tnk::kBlock, 0, s>>>(A, cur, a->parts);
-        tnadj::k_adj_loss_sum<<<1, 64, 0, s>>>(A, a->parts, a->nparts, a->dloss);
+        tnadj::k_adj_loss_sum<<<1, tnk::kBlock, 0, s>>>(A, a->parts, a->nparts, a->dloss);
         if (xT) {
-            tnadj::k_adj_unload<<<nb, tnk::kBlock, 0, s>>>(n, cur, xT);
+            tnadj::k_adj_unload<<<nb, tnk::kBlock, 0, s>>>(n, a->base, cur, xT);
         }
         TN_HIP(hipGetLastError());
         TN_HIP(hipMemcpyAsync(a->loss, a->dloss, sizeof a->loss, hipMemcpyDeviceToHost, s));
@@ -758,13 +807,13 @@ int tn_adj_backward(tn_adj* a, const float* theta, float* grad_x0, float* grad_t
             }
             for (int k = k1 - 1; k >= k0; --k) {
                 const int r = k - k0;
-                tnadj::k_adj_back<<<nb, tnk::kBlock, 0, s>>>(A, a->seg + (size_t)r * n, a->fd + (size_t)r * n, lam,
-                                                             lamo, a->gtheta);
+                tnadj::k_adj_nu<<<nb, tnk::kBlock, 0, s>>>(A, a->fd + (size_t)r * n, lam, a->nk);
+                tnadj::k_adj_back<<<nb, tnk::kBlock, 0, s>>>(A, a->seg + (size_t)r * n, a->nk, lam, lamo, a->gtheta);
                 std::swap(lam, lamo);
             }
         }
         if (grad_x0) {
-            tnadj::k_adj_unload<<<nb, tnk::kBlock, 0, s>>>(n, lam, grad_x0);
+            tnadj::k_adj_unload<<<nb, tnk::kBlock, 0, s>>>(n, nullptr, lam, grad_x0);
         }
         if (grad_theta) {
             tnadj::k_adj_theta_out<<<blocks_for(a->gcells, tnk::kBlock), tnk::kBlock, 0, s>>>(a->gcells, a->gtheta,

@@ -603,15 +603,13 @@ __device__ __forceinline__ int keep_best(float* cd, int* cj, int cnt, int lane) 
     return cnt < kK ? cnt : kK;
 }
 
-__global__ __launch_bounds__(kWave) void k_truss_node(Hash H, TrussIn in, int n, const int32_t* __restrict__ list,
-                                                       const Ctl* __restrict__ ctl, int32_t* __restrict__ nbrT,
-                                                       int32_t* __restrict__ nnb, float4* __restrict__ P0) {
+__device__ __forceinline__ void k_truss_node_body(Hash H, TrussIn in, int n, const int32_t* __restrict__ list, const Ctl* __restrict__ ctl, int32_t* __restrict__ nbrT, int32_t* __restrict__ nnb, float4* __restrict__ P0, int bid, int nblk) {
     __shared__ float cd[kRefreshCap];
     __shared__ int cj[kRefreshCap];
     const int lane = threadIdx.x;
     const int m = ctl->nrefresh;
     const float tskin = in.t + in.skin;
-    for (int q = blockIdx.x; q < m; q += gridDim.x) {
+    for (int q = bid; q < m; q += nblk) {
         const int i = list[q];   // the node's slot
         const float4 xi = in.X[i];
         const float px = xi.x, py = xi.y, pz = xi.z, hi = xi.w;
@@ -699,13 +697,16 @@ __global__ __launch_bounds__(kWave) void k_truss_node(Hash H, TrussIn in, int n,
     }
 }
 
+__global__ __launch_bounds__(kWave) void k_truss_node(Hash H, TrussIn in, int n, const int32_t* __restrict__ list, const Ctl* __restrict__ ctl, int32_t* __restrict__ nbrT, int32_t* __restrict__ nnb, float4* __restrict__ P0) {
+    k_truss_node_body(H, in, n, list, ctl, nbrT, nnb, P0, blockIdx.x, gridDim.x);
+}
+
 // the slots of the nodes flagged for an own-list refresh, appended block by block
 // (in slot order, which is cell order as of the last rebuild, within a block)
-__global__ __launch_bounds__(kBlock) void k_refresh_list(const uint8_t* __restrict__ flag, int n,
-                                                         Ctl* __restrict__ ctl, int32_t* __restrict__ list) {
+__device__ __forceinline__ void k_refresh_list_body(const uint8_t* __restrict__ flag, int n, Ctl* __restrict__ ctl, int32_t* __restrict__ list, int bid, int nblk) {
     __shared__ int woff[kBlock / kWave];
     __shared__ int base;
-    const int s = blockIdx.x * kBlock + threadIdx.x;
+    const int s = bid * kBlock + threadIdx.x;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
     const bool f = s < n && flag[s];
     const uint64_t mask = __ballot(f);
@@ -726,6 +727,10 @@ __global__ __launch_bounds__(kBlock) void k_refresh_list(const uint8_t* __restri
     if (f) {
         list[base + woff[w] + __popcll(mask & lanemask_lt())] = s;
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_refresh_list(const uint8_t* __restrict__ flag, int n, Ctl* __restrict__ ctl, int32_t* __restrict__ list) {
+    k_refresh_list_body(flag, n, ctl, list, blockIdx.x, gridDim.x);
 }
 
 // ---- node counts by type: interface, junction, interior, corner ------------------------
@@ -768,13 +773,15 @@ __global__ __launch_bounds__(kBlock) void k_order_count(const uint8_t* __restric
 
 // ---- force --------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ X, const uint8_t* __restrict__ typ,
-                                                  const int32_t* __restrict__ nbrT, const int32_t* __restrict__ nnb,
-                                                  float fscale, float fsurf, int n, float4* __restrict__ F) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void k_force_body(const float4* __restrict__ X, const uint8_t* __restrict__ typ, const int32_t* __restrict__ nbrT, const int32_t* __restrict__ nnb, float fscale, float fsurf, int n, float4* __restrict__ F, int bid, int nblk) {
+    const int i = bid * kBlock + threadIdx.x;
     if (i < n) {
         F[i] = bar_force(X, typ, nbrT, nnb, fscale, fsurf, i, n);
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ X, const uint8_t* __restrict__ typ, const int32_t* __restrict__ nbrT, const int32_t* __restrict__ nnb, float fscale, float fsurf, int n, float4* __restrict__ F) {
+    k_force_body(X, typ, nbrT, nnb, fscale, fsurf, n, F, blockIdx.x, gridDim.x);
 }
 
 // ---- move (v2m_move, src/opencl/v2m_particle_body.cl:844-935) ----------------------
@@ -894,14 +901,8 @@ __device__ __forceinline__ void wave_append(bool flag, int i, int32_t* count, in
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl* __restrict__ ctl, int n,
-                                                      const float4* __restrict__ F, float4* __restrict__ X,
-                                                      float4* __restrict__ V, uint8_t* __restrict__ typ,
-                                                      uint32_t* __restrict__ part, const uint16_t* __restrict__ lab,
-                                                      float* __restrict__ mv, float* __restrict__ pw,
-                                                      int32_t* __restrict__ queue, int32_t* __restrict__ queue_int,
-                                                      int32_t* __restrict__ queue_rest) {
-    const int g = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void k_move_easy_body(Field f, MoveArgs ma, Ctl* __restrict__ ctl, int n, const float4* __restrict__ F, float4* __restrict__ X, float4* __restrict__ V, uint8_t* __restrict__ typ, uint32_t* __restrict__ part, const uint16_t* __restrict__ lab, float* __restrict__ mv, float* __restrict__ pw, int32_t* __restrict__ queue, int32_t* __restrict__ queue_int, int32_t* __restrict__ queue_rest, int bid, int nblk) {
+    const int g = bid * kBlock + threadIdx.x;
     int i = 0;
     bool surf = false, inner = false, hard = false;   // the three queues
     if (g < n) {
@@ -969,6 +970,10 @@ __global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl*
     wave_append(hard, i, &ctl->nmove_rest, queue_rest);
 }
 
+__global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl* __restrict__ ctl, int n, const float4* __restrict__ F, float4* __restrict__ X, float4* __restrict__ V, uint8_t* __restrict__ typ, uint32_t* __restrict__ part, const uint16_t* __restrict__ lab, float* __restrict__ mv, float* __restrict__ pw, int32_t* __restrict__ queue, int32_t* __restrict__ queue_int, int32_t* __restrict__ queue_rest) {
+    k_move_easy_body(f, ma, ctl, n, F, X, V, typ, part, lab, mv, pw, queue, queue_int, queue_rest, blockIdx.x, gridDim.x);
+}
+
 #ifndef TN_REST_LANES
 #define TN_REST_LANES 4
 #endif
@@ -1017,16 +1022,7 @@ __device__ __forceinline__ bool move_node(const Field& f, const MoveArgs& ma, co
 // reads and writes only that node's state, so the passes over disjoint queues may
 // run concurrently.
 template <int kMode>
-__global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArgs ma, Ctl* __restrict__ ctl,
-                                                                const int32_t* __restrict__ queue,
-                                                                const float4* __restrict__ F, float4* __restrict__ X,
-                                                                float4* __restrict__ V, uint8_t* __restrict__ typ,
-                                                                uint32_t* __restrict__ part,
-                                                                const uint16_t* __restrict__ lab,
-                                                                float* __restrict__ mv, float* __restrict__ pw,
-                                                                const int32_t* __restrict__ qcount,
-                                                                int32_t* __restrict__ defer_count,
-                                                                int32_t* __restrict__ defer_queue) {
+__device__ __forceinline__ void k_move_body(Field f, MoveArgs ma, Ctl* __restrict__ ctl, const int32_t* __restrict__ queue, const float4* __restrict__ F, float4* __restrict__ X, float4* __restrict__ V, uint8_t* __restrict__ typ, uint32_t* __restrict__ part, const uint16_t* __restrict__ lab, float* __restrict__ mv, float* __restrict__ pw, const int32_t* __restrict__ qcount, int32_t* __restrict__ defer_count, int32_t* __restrict__ defer_queue, int bid, int nblk) {
     // A fixed grid strides over the queue, whose length only the device knows. A
     // short queue's nodes follow long, divergent paths (projections onto interfaces
     // and junction curves), and a wave executes every path its lanes take, so they
@@ -1036,7 +1032,7 @@ __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArg
     constexpr bool kLean = kMode != kGeneral;
     constexpr int kWaves = kBlock / kWave;
     const int count = *qcount;
-    const int waves = (int)gridDim.x * kWaves;
+    const int waves = (int)nblk * kWaves;
     const int spread = (count + waves - 1) / waves;
     int lpw;
     if (kLean) {
@@ -1047,7 +1043,7 @@ __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArg
     const int per_block = lpw * kWaves;
     const int lane = threadIdx.x & (kWave - 1);
     const int slot = lane < lpw ? (threadIdx.x >> 6) * lpw + lane : -1;
-    for (int base = blockIdx.x * per_block; base < count; base += gridDim.x * per_block) {
+    for (int base = bid * per_block; base < count; base += nblk * per_block) {
         const int g = base + slot;
         bool defer = false;
         int i = 0;
@@ -1059,6 +1055,11 @@ __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArg
             wave_append(defer, i, defer_count, defer_queue);
         }
     }
+}
+
+template <int kMode>
+__global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArgs ma, Ctl* __restrict__ ctl, const int32_t* __restrict__ queue, const float4* __restrict__ F, float4* __restrict__ X, float4* __restrict__ V, uint8_t* __restrict__ typ, uint32_t* __restrict__ part, const uint16_t* __restrict__ lab, float* __restrict__ mv, float* __restrict__ pw, const int32_t* __restrict__ qcount, int32_t* __restrict__ defer_count, int32_t* __restrict__ defer_queue) {
+    k_move_body<kMode>(f, ma, ctl, queue, F, X, V, typ, part, lab, mv, pw, qcount, defer_count, defer_queue, blockIdx.x, gridDim.x);
 }
 
 constexpr int kMoveGrid = 2048;       // blocks of a lean k_move at most
@@ -1092,11 +1093,7 @@ constexpr int kStatsBlocks = 2048;   // k_stats' grid at most (8 blocks per CU)
 // flags of the nodes past a full skin (their lists are refreshed), and FIRE's
 // power partial sums (src/v2m_particles.cpp:849-937). Counts are reduced per
 // wave with ballots before any atomic.
-__global__ __launch_bounds__(kBlock) void k_stats(const float4* __restrict__ X, const float4* __restrict__ P0,
-                                                  const float* __restrict__ mv, const float* __restrict__ pw, int n,
-                                                  float skin, int fire, const float* __restrict__ thr,
-                                                  Ctl* __restrict__ ctl, uint8_t* __restrict__ refresh,
-                                                  double2* __restrict__ partials) {
+__device__ __forceinline__ void k_stats_body(const float4* __restrict__ X, const float4* __restrict__ P0, const float* __restrict__ mv, const float* __restrict__ pw, int n, float skin, int fire, const float* __restrict__ thr, Ctl* __restrict__ ctl, uint8_t* __restrict__ refresh, double2* __restrict__ partials, int bid, int nblk) {
     __shared__ uint32_t lh[32];
     __shared__ uint32_t lmax;
     __shared__ int lhalf;
@@ -1112,7 +1109,7 @@ __global__ __launch_bounds__(kBlock) void k_stats(const float4* __restrict__ X, 
     double bs = 0.0, ba = 0.0;   // this block's power partial sums (thread 0), chunk by chunk
     // chunks of kBlock nodes, grid-strided: each block adds its counts to the control
     // block once, and its partial sums have a fixed shape
-    for (int c = blockIdx.x; (int64_t)c * kBlock < n; c += gridDim.x) {
+    for (int c = bid; (int64_t)c * kBlock < n; c += nblk) {
         __syncthreads();   // lps of the previous chunk has been read
         const int i = c * kBlock + tid;
         double ps = 0.0, pa = 0.0;
@@ -1183,9 +1180,13 @@ __global__ __launch_bounds__(kBlock) void k_stats(const float4* __restrict__ X, 
             atomicAdd(&ctl->nhalf, lhalf);
         }
         if (fire) {
-            partials[blockIdx.x] = make_double2(bs, ba);
+            partials[bid] = make_double2(bs, ba);
         }
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_stats(const float4* __restrict__ X, const float4* __restrict__ P0, const float* __restrict__ mv, const float* __restrict__ pw, int n, float skin, int fire, const float* __restrict__ thr, Ctl* __restrict__ ctl, uint8_t* __restrict__ refresh, double2* __restrict__ partials) {
+    k_stats_body(X, P0, mv, pw, n, skin, fire, thr, ctl, refresh, partials, blockIdx.x, gridDim.x);
 }
 
 // ---- control (one block) ---------------------------------------------------------------
@@ -1203,10 +1204,7 @@ struct CtlArgs {
 // (n + 64) * 2^-53 * A of the exact sum, so when |S| exceeds twice that bound both
 // have the exact sum's sign. Otherwise the serial sum is recomputed here, in index
 // order, exactly as the CPU does.
-__global__ __launch_bounds__(kBlock) void k_control(CtlArgs ca, Ctl* __restrict__ ctl,
-                                                    const double2* __restrict__ partials,
-                                                    const float* __restrict__ pw,
-                                                    const int32_t* __restrict__ slot_of) {
+__device__ __forceinline__ void k_control_body(CtlArgs ca, Ctl* __restrict__ ctl, const double2* __restrict__ partials, const float* __restrict__ pw, const int32_t* __restrict__ slot_of, int bid, int nblk) {
     __shared__ double red[kBlock][2];
     __shared__ float chunk[kScanTile];
     __shared__ int use_serial;
@@ -1305,8 +1303,12 @@ __global__ __launch_bounds__(kBlock) void k_control(CtlArgs ca, Ctl* __restrict_
     ctl->mmax_bits = 0;
 }
 
+__global__ __launch_bounds__(kBlock) void k_control(CtlArgs ca, Ctl* __restrict__ ctl, const double2* __restrict__ partials, const float* __restrict__ pw, const int32_t* __restrict__ slot_of) {
+    k_control_body(ca, ctl, partials, pw, slot_of, blockIdx.x, gridDim.x);
+}
+
 // clears the per-iteration counters that k_stats and k_refresh_list fill
-__global__ void k_iter_begin(Ctl* __restrict__ ctl) {
+__device__ __forceinline__ void k_iter_begin_body(Ctl* __restrict__ ctl, int bid, int nblk) {
     if (threadIdx.x == 0) {
         ctl->nhalf = 0;
         ctl->nrefresh = 0;
@@ -1316,6 +1318,129 @@ __global__ void k_iter_begin(Ctl* __restrict__ ctl) {
         ctl->nmove_def = 0;
     }
 }
+
+__global__ void k_iter_begin(Ctl* __restrict__ ctl) {
+    k_iter_begin_body(ctl, blockIdx.x, gridDim.x);
+}
+
+// ---- grouped launches: one launch runs an iteration's kernel for many plans -------
+//
+// Each plan's arguments sit in a device table; a block finds its plan and its block
+// index within that plan's grid (bplan / bfirst, per grid shape) and runs the same
+// body a single-plan launch of that grid runs, so every plan computes exactly what
+// its own launches would.
+
+enum GridShape { kShNodes, kShLean, kShGen, kShParts, kShRefresh, kShOne, kShapes };
+
+struct PlanIter {
+    Field f;
+    Hash H;
+    TrussIn tin;
+    MoveArgs ma;
+    CtlArgs ca;
+    int n, fire, nparts;
+    float fscale, fsurf, skin;
+    Ctl* ctl;
+    int32_t* nbrT;
+    int32_t* nnb;
+    float4 *X, *V, *F, *P0;
+    uint8_t* typ;
+    uint32_t* part;
+    const uint16_t* lab;
+    float *mv, *pw;
+    const float* thr;
+    uint8_t* flag;
+    double2* partials;
+    int32_t *rqueue, *mq, *mqi, *mq2, *mqd;
+    const int32_t* slot_of;
+    int nblk[kShapes];
+};
+
+#define TN_PLAN_BLOCK(shape)                              \
+    const int v_ = bplan[blockIdx.x];                     \
+    const PlanIter& P = tab[v_];                          \
+    const int bid_ = (int)blockIdx.x - bfirst[v_];        \
+    const int nb_ = P.nblk[shape]
+
+__global__ void kg_iter_begin(const PlanIter* __restrict__ tab, const int32_t* __restrict__ bplan,
+                              const int32_t* __restrict__ bfirst) {
+    TN_PLAN_BLOCK(kShOne);
+    k_iter_begin_body(P.ctl, bid_, nb_);
+}
+
+__global__ __launch_bounds__(kBlock) void kg_force(const PlanIter* __restrict__ tab, const int32_t* __restrict__ bplan,
+                                                   const int32_t* __restrict__ bfirst) {
+    TN_PLAN_BLOCK(kShNodes);
+    k_force_body(P.X, P.typ, P.nbrT, P.nnb, P.fscale, P.fsurf, P.n, P.F, bid_, nb_);
+}
+
+__global__ __launch_bounds__(kBlock) void kg_move_easy(const PlanIter* __restrict__ tab,
+                                                       const int32_t* __restrict__ bplan,
+                                                       const int32_t* __restrict__ bfirst) {
+    TN_PLAN_BLOCK(kShNodes);
+    k_move_easy_body(P.f, P.ma, P.ctl, P.n, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv, P.pw, P.mq, P.mqi, P.mq2, bid_,
+                     nb_);
+}
+
+// the four move passes: kQueue 0 = the general queue, 1 = interior (lean), 2 =
+// interface (lean), 3 = the handed-back nodes (general)
+template <int kQueue>
+__global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void kg_move(const PlanIter* __restrict__ tab,
+                                                                 const int32_t* __restrict__ bplan,
+                                                                 const int32_t* __restrict__ bfirst) {
+    constexpr int kMode = kQueue == 1 ? kLeanInterior : (kQueue == 2 ? kLeanInterface : kGeneral);
+    TN_PLAN_BLOCK(kMode == kGeneral ? kShGen : kShLean);
+    const int32_t* queue = kQueue == 0 ? P.mq2 : (kQueue == 1 ? P.mqi : (kQueue == 2 ? P.mq : P.mqd));
+    const int32_t* count = kQueue == 0 ? &P.ctl->nmove_rest
+                                       : (kQueue == 1 ? &P.ctl->nmove_int
+                                                      : (kQueue == 2 ? &P.ctl->nmove : &P.ctl->nmove_def));
+    const bool lean = kMode != kGeneral;
+    k_move_body<kMode>(P.f, P.ma, P.ctl, queue, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv, P.pw, count,
+                       lean ? &P.ctl->nmove_def : nullptr, lean ? P.mqd : nullptr, bid_, nb_);
+}
+
+__global__ __launch_bounds__(kBlock) void kg_stats(const PlanIter* __restrict__ tab, const int32_t* __restrict__ bplan,
+                                                   const int32_t* __restrict__ bfirst) {
+    TN_PLAN_BLOCK(kShParts);
+    k_stats_body(P.X, P.P0, P.mv, P.pw, P.n, P.skin, P.fire, P.thr, P.ctl, P.flag, P.partials, bid_, nb_);
+}
+
+__global__ __launch_bounds__(kBlock) void kg_refresh_list(const PlanIter* __restrict__ tab,
+                                                          const int32_t* __restrict__ bplan,
+                                                          const int32_t* __restrict__ bfirst) {
+    TN_PLAN_BLOCK(kShNodes);
+    k_refresh_list_body(P.flag, P.n, P.ctl, P.rqueue, bid_, nb_);
+}
+
+__global__ __launch_bounds__(kWave) void kg_truss_node(const PlanIter* __restrict__ tab,
+                                                       const int32_t* __restrict__ bplan,
+                                                       const int32_t* __restrict__ bfirst) {
+    TN_PLAN_BLOCK(kShRefresh);
+    k_truss_node_body(P.H, P.tin, P.n, P.rqueue, P.ctl, P.nbrT, P.nnb, P.P0, bid_, nb_);
+}
+
+__global__ __launch_bounds__(kBlock) void kg_control(const PlanIter* __restrict__ tab,
+                                                     const int32_t* __restrict__ bplan,
+                                                     const int32_t* __restrict__ bfirst) {
+    TN_PLAN_BLOCK(kShOne);
+    k_control_body(P.ca, P.ctl, P.partials, P.pw, P.slot_of, bid_, nb_);
+}
+
+// every plan's control block into one array (one copy to the host)
+__global__ void kg_ctl_gather(const PlanIter* __restrict__ tab, const int32_t* __restrict__ bplan,
+                              const int32_t* __restrict__ bfirst, Ctl* __restrict__ out) {
+    TN_PLAN_BLOCK(kShOne);
+    (void)bid_;
+    (void)nb_;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(P.ctl);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(out + v_);
+    static_assert(sizeof(Ctl) % 4 == 0, "Ctl is copied in words");
+    for (int k = threadIdx.x; k < (int)(sizeof(Ctl) / 4); k += blockDim.x) {
+        dst[k] = src[k];
+    }
+}
+
+#undef TN_PLAN_BLOCK
 
 __global__ void k_order_begin(Ctl* __restrict__ ctl) {
     if (threadIdx.x < 4) {

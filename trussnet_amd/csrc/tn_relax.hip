@@ -563,6 +563,52 @@ struct tn_plan {
         return true;
     }
 
+    // the arguments of enqueue_iteration's launches, for the grouped launches of many
+    // plans (they change at every rebuild, which swaps the node buffers)
+    tnk::PlanIter iter_args() const {
+        tnk::PlanIter a{};
+        a.f = f;
+        a.H = H;
+        a.tin = truss_in();
+        a.ma = tnk::MoveArgs{ prm.dt, prm.maxstep, prm.snap, fire };
+        a.ca = tnk::CtlArgs{ n, prm.max_iters, fire, rebuild_div, prm.dptol, nparts };
+        a.n = n;
+        a.fire = fire;
+        a.nparts = nparts;
+        a.fscale = prm.fscale;
+        a.fsurf = prm.fsurf;
+        a.skin = prm.skin;
+        a.ctl = dCtl;
+        a.nbrT = dNbrT;
+        a.nnb = dNnb;
+        a.X = dX;
+        a.V = dV;
+        a.F = dF;
+        a.P0 = dP0;
+        a.typ = dTyp;
+        a.part = dPart;
+        a.lab = dLab;
+        a.mv = dMv;
+        a.pw = dPw;
+        a.thr = dThr;
+        a.flag = dFlag;
+        a.partials = dPartials;
+        a.rqueue = dQueue;
+        a.mq = dMoveQ;
+        a.mqi = dMoveQi;
+        a.mq2 = dMoveQ2;
+        a.mqd = dMoveQd;
+        a.slot_of = dSlotOf;
+        const int nb = (int)blocks_for(n, tnk::kBlock);
+        a.nblk[tnk::kShNodes] = nb;
+        a.nblk[tnk::kShLean] = std::min(nb, tnk::kMoveGrid);
+        a.nblk[tnk::kShGen] = std::min(nb, tnk::kMoveRestGrid);
+        a.nblk[tnk::kShParts] = nparts;
+        a.nblk[tnk::kShRefresh] = refresh_blocks;
+        a.nblk[tnk::kShOne] = 1;
+        return a;
+    }
+
     bool iterate() {
         enqueue_iteration();
         TN_HIP(hipEventSynchronize(ev_iter));
@@ -928,6 +974,143 @@ Seeds run_seeding(const tn_grid_view* g, const uint8_t* grade, const tn_relax_pa
     }
     TN_HIP(hipDeviceSynchronize());
     return s;
+}
+
+// Every plan's iterations as grouped launches on one stream: each of an iteration's
+// kernels is launched once for all the plans still running (a device table holds
+// their arguments), one copy brings all their control blocks to the host, and the
+// plans that asked for a rebuild get it on the same stream before the next
+// iteration. Each plan computes exactly what its own launches would.
+template <class Plans>
+void run_grouped(Plans& plans, int64_t& launches, int64_t& steps) {
+    const int count = (int)plans.size();
+    hipStream_t bs = plans[0]->stream;
+    std::vector<hipStream_t> own(count);
+    for (int k = 0; k < count; ++k) {
+        own[k] = plans[k]->stream;
+    }
+    struct RestoreStreams {
+        Plans& p;
+        std::vector<hipStream_t>& s;
+        ~RestoreStreams() {
+            for (size_t k = 0; k < s.size(); ++k) {
+                p[k]->stream = s[k];
+            }
+        }
+    } restore{ plans, own };
+    TN_HIP(hipDeviceSynchronize());   // every plan's set-up work is done
+    for (int k = 0; k < count; ++k) {
+        plans[k]->stream = bs;
+        plans[k]->overlap_moves = false;
+    }
+    DevMem gm;
+    gm.device = plans[0]->device;
+    tnk::PlanIter* dtab = gm.alloc<tnk::PlanIter>(count);
+    tnk::Ctl* dctl = gm.alloc<tnk::Ctl>(count);
+    // pinned host copies: the table rows, the control blocks, the block maps
+    struct Pinned {
+        void* p = nullptr;
+        ~Pinned() {
+            if (p) {
+                (void)hipHostFree(p);
+            }
+        }
+    };
+    Pinned ptab, pctl, pmap;
+    TN_HIP(hipHostMalloc(&ptab.p, sizeof(tnk::PlanIter) * count, hipHostMallocDefault));
+    TN_HIP(hipHostMalloc(&pctl.p, sizeof(tnk::Ctl) * count, hipHostMallocDefault));
+    auto* htab = static_cast<tnk::PlanIter*>(ptab.p);
+    auto* hctl = static_cast<tnk::Ctl*>(pctl.p);
+    for (int k = 0; k < count; ++k) {
+        htab[k] = plans[k]->iter_args();
+    }
+    TN_HIP(hipMemcpyAsync(dtab, htab, sizeof(tnk::PlanIter) * count, hipMemcpyHostToDevice, bs));
+    // block maps: for each grid shape, the plan of every block and each plan's first block
+    int64_t map_total = 0;
+    for (int k = 0; k < count; ++k) {
+        for (int sh = 0; sh < tnk::kShapes; ++sh) {
+            map_total += htab[k].nblk[sh];
+        }
+    }
+    map_total += (int64_t)tnk::kShapes * count;
+    TN_HIP(hipHostMalloc(&pmap.p, sizeof(int32_t) * map_total, hipHostMallocDefault));
+    int32_t* hmap = static_cast<int32_t*>(pmap.p);
+    int32_t* dmap = gm.alloc<int32_t>((size_t)map_total);
+    const int32_t* bplan[tnk::kShapes];
+    const int32_t* bfirst[tnk::kShapes];
+    unsigned nblocks[tnk::kShapes];
+    std::vector<int> active;
+    for (int k = 0; k < count; ++k) {
+        if (plans[k]->more()) {
+            active.push_back(k);
+        }
+    }
+    auto build_maps = [&]() {
+        int64_t o = 0;
+        for (int sh = 0; sh < tnk::kShapes; ++sh) {
+            int32_t* first = hmap + o;
+            o += count;
+            int32_t* plan = hmap + o;
+            int nb = 0;
+            for (int k : active) {
+                first[k] = nb;
+                for (int b = 0; b < htab[k].nblk[sh]; ++b) {
+                    plan[nb++] = k;
+                }
+            }
+            bfirst[sh] = dmap + (first - hmap);
+            bplan[sh] = dmap + (plan - hmap);
+            nblocks[sh] = (unsigned)nb;
+            o += nb;
+        }
+        TN_HIP(hipMemcpyAsync(dmap, hmap, sizeof(int32_t) * o, hipMemcpyHostToDevice, bs));
+    };
+    build_maps();
+    while (!active.empty()) {
+        using namespace tnk;
+        kg_iter_begin<<<nblocks[kShOne], 64, 0, bs>>>(dtab, bplan[kShOne], bfirst[kShOne]);
+        kg_force<<<nblocks[kShNodes], kBlock, 0, bs>>>(dtab, bplan[kShNodes], bfirst[kShNodes]);
+        kg_move_easy<<<nblocks[kShNodes], kBlock, 0, bs>>>(dtab, bplan[kShNodes], bfirst[kShNodes]);
+        kg_move<0><<<nblocks[kShGen], kBlock, 0, bs>>>(dtab, bplan[kShGen], bfirst[kShGen]);
+        kg_move<1><<<nblocks[kShLean], kBlock, 0, bs>>>(dtab, bplan[kShLean], bfirst[kShLean]);
+        kg_move<2><<<nblocks[kShLean], kBlock, 0, bs>>>(dtab, bplan[kShLean], bfirst[kShLean]);
+        kg_move<3><<<nblocks[kShGen], kBlock, 0, bs>>>(dtab, bplan[kShGen], bfirst[kShGen]);
+        kg_stats<<<nblocks[kShParts], kBlock, 0, bs>>>(dtab, bplan[kShParts], bfirst[kShParts]);
+        kg_refresh_list<<<nblocks[kShNodes], kBlock, 0, bs>>>(dtab, bplan[kShNodes], bfirst[kShNodes]);
+        kg_truss_node<<<nblocks[kShRefresh], kWave, 0, bs>>>(dtab, bplan[kShRefresh], bfirst[kShRefresh]);
+        kg_control<<<nblocks[kShOne], kBlock, 0, bs>>>(dtab, bplan[kShOne], bfirst[kShOne]);
+        kg_ctl_gather<<<nblocks[kShOne], 64, 0, bs>>>(dtab, bplan[kShOne], bfirst[kShOne], dctl);
+        TN_HIP(hipGetLastError());
+        TN_HIP(hipMemcpyAsync(hctl, dctl, sizeof(tnk::Ctl) * count, hipMemcpyDeviceToHost, bs));
+        TN_HIP(hipStreamSynchronize(bs));
+        ++launches;
+        bool changed = false;
+        for (size_t a = 0; a < active.size();) {
+            const int k = active[a];
+            tn_plan* p = &*plans[k];
+            *p->hCtl = hctl[k];
+            ++steps;
+            const int before = p->rebuilds;
+            const bool go = p->finish_iteration() && p->more();   // enqueues a rebuild on bs when due
+            if (p->rebuilds != before) {   // the rebuild swapped the node buffers
+                htab[k] = p->iter_args();
+                TN_HIP(hipMemcpyAsync(dtab + k, htab + k, sizeof(tnk::PlanIter), hipMemcpyHostToDevice, bs));
+            }
+            if (go) {
+                ++a;
+            } else {
+                active[a] = active.back();
+                active.pop_back();
+                changed = true;
+            }
+        }
+        if (changed && !active.empty()) {
+            TN_HIP(hipStreamSynchronize(bs));   // the maps' host copy is about to change
+            std::sort(active.begin(), active.end());
+            build_maps();
+        }
+    }
+    TN_HIP(hipStreamSynchronize(bs));
 }
 
 }  // namespace
@@ -1334,45 +1517,9 @@ int tn_relax_batch(const tn_grid_view* grids, const tn_relax_params* prms, tn_no
             plans.emplace_back(p);
         }
         const auto t_created = std::chrono::steady_clock::now();
-        // one host thread advances every volume as its last iteration completes; the
-        // volumes' streams keep the device busy meanwhile
-        std::vector<tn_plan*> active;
-        for (auto& p : plans) {
-            p->overlap_moves = count == 1;
-        }
-        int64_t polls = 0, steps = 0;
-        for (auto& p : plans) {
-            if (p->more()) {
-                p->enqueue_iteration();
-                active.push_back(p.get());
-            }
-        }
-        while (!active.empty()) {
-            bool progress = false;
-            for (size_t k = 0; k < active.size();) {
-                tn_plan* p = active[k];
-                const hipError_t q = hipEventQuery(p->ev_iter);
-                ++polls;
-                if (q == hipErrorNotReady) {
-                    ++k;
-                    continue;
-                }
-                TN_HIP(q);
-                progress = true;
-                ++steps;
-                if (p->finish_iteration() && p->more()) {
-                    p->enqueue_iteration();
-                    ++k;
-                } else {
-                    TN_HIP(hipStreamSynchronize(p->stream));
-                    p->timer.collect();
-                    active[k] = active.back();
-                    active.pop_back();
-                }
-            }
-            if (!progress) {
-                std::this_thread::sleep_for(std::chrono::microseconds(20));
-            }
+        int64_t launches = 0, steps = 0;
+        if (count > 0) {
+            run_grouped(plans, launches, steps);
         }
         const auto t_looped = std::chrono::steady_clock::now();
         for (int k = 0; k < count; ++k) {
@@ -1386,9 +1533,10 @@ int tn_relax_batch(const tn_grid_view* grids, const tn_relax_params* prms, tn_no
         }
         if (std::getenv("TN_BATCH_VERBOSE")) {
             const auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
-            std::fprintf(stderr, "[tn_relax_batch] %d volumes: plans %.1f ms, loop %.1f ms (%lld iterations, %lld polls), "
-                         "download %.1f ms\n", count, ms(t_start, t_created), ms(t_created, t_looped),
-                         (long long)steps, (long long)polls, ms(t_looped, std::chrono::steady_clock::now()));
+            std::fprintf(stderr, "[tn_relax_batch] %d volumes: plans %.1f ms, loop %.1f ms (%lld grouped iterations, "
+                         "%lld volume-iterations), download %.1f ms\n", count, ms(t_start, t_created),
+                         ms(t_created, t_looped), (long long)launches, (long long)steps,
+                         ms(t_looped, std::chrono::steady_clock::now()));
         }
     });
 }
