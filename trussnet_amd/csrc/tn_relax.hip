@@ -140,8 +140,14 @@ __global__ void k_iota(int32_t* __restrict__ a, int n) {
 }
 
 // Device buffers freed together (a plan's, or one seeding call's).
+// Device memory owned by one object and freed with it. Small buffers are carved
+// from shared 16 MB chunks (256-byte aligned): a plan holds a few dozen buffers,
+// and every hipMalloc / hipFree costs tens of microseconds and a device-wide sync.
 struct DevMem {
+    static constexpr size_t kChunk = (size_t)16 << 20, kAlign = 256;
     std::vector<void*> owned;
+    char* chunk = nullptr;
+    size_t chunk_left = 0;
     int device = -1;
     DevMem() = default;
     DevMem(const DevMem&) = delete;
@@ -154,12 +160,29 @@ struct DevMem {
             (void)hipFree(p);
         }
     }
+    void* raw(size_t bytes) {
+        bytes = (std::max<size_t>(bytes, 1) + kAlign - 1) / kAlign * kAlign;
+        if (bytes > kChunk / 4) {   // a buffer of its own
+            void* p = nullptr;
+            TN_HIP(hipMalloc(&p, bytes));
+            owned.push_back(p);
+            return p;
+        }
+        if (bytes > chunk_left) {
+            void* p = nullptr;
+            TN_HIP(hipMalloc(&p, kChunk));
+            owned.push_back(p);
+            chunk = static_cast<char*>(p);
+            chunk_left = kChunk;
+        }
+        void* p = chunk;
+        chunk += bytes;
+        chunk_left -= bytes;
+        return p;
+    }
     template <class T>
     T* alloc(size_t count) {
-        void* p = nullptr;
-        TN_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
-        owned.push_back(p);
-        return static_cast<T*>(p);
+        return static_cast<T*>(raw(count * sizeof(T)));
     }
     template <class T>
     T* upload(const T* host, size_t count) {
@@ -1091,8 +1114,14 @@ void run_grouped(Plans& plans, int64_t& launches, int64_t& steps) {
             *p->hCtl = hctl[k];
             ++steps;
             const int before = p->rebuilds;
-            const bool go = p->finish_iteration() && p->more();   // enqueues a rebuild on bs when due
+            // a rebuild runs on the plan's own stream (bs is idle here), so the plans'
+            // rebuilds overlap; the next grouped launch waits for it
+            p->stream = own[k];
+            const bool go = p->finish_iteration() && p->more();
+            p->stream = bs;
             if (p->rebuilds != before) {   // the rebuild swapped the node buffers
+                TN_HIP(hipEventRecord(p->ev_iter, own[k]));
+                TN_HIP(hipStreamWaitEvent(bs, p->ev_iter, 0));
                 htab[k] = p->iter_args();
                 TN_HIP(hipMemcpyAsync(dtab + k, htab + k, sizeof(tnk::PlanIter), hipMemcpyHostToDevice, bs));
             }
