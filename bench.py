@@ -273,6 +273,29 @@ def bench_adjoint(args) -> int:
     torch.cuda.synchronize()
     t_bwd = (time.perf_counter() - t) / args.steps
     log(f"forward {1e3 * t_fwd:.2f} ms, backward {1e3 * t_bwd:.2f} ms (x{t_bwd / t_fwd:.2f}), loss {loss:.6f}")
+    # the same with every state kept (checkpoint every step: nothing recomputed)
+    prm1 = adjoint.AdjParams(iters=iters, ckpt_every=1)
+    pb1 = adjoint.Problem(pb.nbr, pb.nnb, pb.typ, pb.tets, theta.shape, pb.origin, pb.spacing, prm1, device=0)
+    gx1, gt1 = torch.empty_like(xd), torch.empty_like(td)
+    pb1.forward_ptr(xd.data_ptr(), td.data_ptr())
+    pb1.backward_ptr(td.data_ptr(), gx1.data_ptr(), gt1.data_ptr())
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        pb1.forward_ptr(xd.data_ptr(), td.data_ptr())
+    torch.cuda.synchronize()
+    t_fwd1 = (time.perf_counter() - t) / args.steps
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        pb1.backward_ptr(td.data_ptr(), gx1.data_ptr(), gt1.data_ptr())
+    torch.cuda.synchronize()
+    t_bwd1 = (time.perf_counter() - t) / args.steps
+    # the position gradient has no atomics (identical); the grid gradient's double
+    # atomics add in any order
+    same = bool(torch.equal(gx1, gx)) and float((gt1 - gt).abs().max()) <= 1e-6 * float(gt.abs().max())
+    log(f"every state kept: forward {1e3 * t_fwd1:.2f} ms, backward {1e3 * t_bwd1:.2f} ms (x{t_bwd1 / t_fwd1:.2f}), "
+        f"same gradient as the checkpointed run: {same}")
+    pb1.close()
     check = None
     if not args.no_cpu:
         # the FP64 CPU gradient and central differences (eps 1e-3 h) of the FP64 forward
@@ -284,14 +307,18 @@ def bench_adjoint(args) -> int:
         rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
         rng = np.random.default_rng(0)
         fd = []
-        for eps_h in (1e-3, 1e-5):
-            dx, dt = rng.standard_normal(x0.shape), rng.standard_normal(theta.shape)
+        dx, dt = rng.standard_normal(x0.shape), rng.standard_normal(theta.shape)
+        for eps_h in (1e-3, 1e-6):
+            # central differences of the FP64 forward along one random direction; at
+            # eps = 1e-3 h the smoothed quality's curvature dominates their error
             eps = eps_h * float(np.mean(theta))
             lp = orc.adj_loss(pc, x0 + eps * dx, theta + eps * dt)[0]
             lm = orc.adj_loss(pc, x0 - eps * dx, theta - eps * dt)[0]
             want = (lp - lm) / (2 * eps)
             got = float((g1 * dx).sum() + (g2 * dt).sum())
-            fd.append({"eps_over_h": eps_h, "fd": want, "adjoint": got, "rel": abs(got - want) / abs(want)})
+            got64 = float((gx64 * dx).sum() + (gt64 * dt).sum())
+            fd.append({"eps_over_h": eps_h, "fd": want, "adjoint": got, "rel": abs(got - want) / abs(want),
+                       "fp64_adjoint_rel": abs(got64 - want) / abs(want)})
         check = {"loss_fp64": L64, "loss_rel": abs(loss - L64) / abs(L64), "grad_x_rel": rel(g1, gx64),
                  "grad_theta_rel": rel(g2, gt64), "fd": fd, "cpu_grad_s": t_cpu}
         log(f"check: {check}")
@@ -305,6 +332,8 @@ def bench_adjoint(args) -> int:
                                "loss = mean truss energy + mean (1 - smoothed Joe-Liu) over a Delaunay of the seeds",
                    "nodes": n, "tets": int(pb.tets.shape[0]), "iterations": iters, "parallelism": "single GPU"},
         "forward_ms": 1e3 * t_fwd, "backward_ms": 1e3 * t_bwd, "adjoint_cost_ratio": t_bwd / t_fwd,
+        "every_state_kept": {"forward_ms": 1e3 * t_fwd1, "backward_ms": 1e3 * t_bwd1,
+                             "adjoint_cost_ratio": t_bwd1 / t_fwd1, "identical_gradient": same},
         "loss": loss, "check": check,
     }
     print(json.dumps(line), flush=True)

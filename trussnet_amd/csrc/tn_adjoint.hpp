@@ -598,6 +598,10 @@ struct tn_adj {
     float4* ckpts = nullptr;   // [nseg * n]: the state at the start of each segment
     float4* seg = nullptr;     // [ckpt * n]: a segment's states, recomputed
     float4* fd = nullptr;      // [ckpt * n]: F and D of those states
+    // ckpt_every == 1: every state is a checkpoint and the forward keeps F, D of each
+    // ([iters * n]), so the backward recomputes nothing
+    float4* fd_all = nullptr;
+    float4* last = nullptr;    // the final state of the last forward
     float4 *Xa = nullptr, *Xb = nullptr, *lamA = nullptr, *lamB = nullptr, *nk = nullptr, *base = nullptr;
     double* gtheta = nullptr;
     double2* parts = nullptr;
@@ -724,6 +728,9 @@ int tn_adj_create(const tn_adj_problem* p, int32_t device, tn_adj** out) {
         a->ckpts = mem.alloc<float4>((size_t)std::max(a->nseg, 1) * n);
         a->seg = mem.alloc<float4>((size_t)a->ckpt * n);
         a->fd = mem.alloc<float4>((size_t)a->ckpt * n);
+        if (a->ckpt == 1 && a->iters > 0) {
+            a->fd_all = mem.alloc<float4>((size_t)a->iters * n);
+        }
         a->Xa = mem.alloc<float4>(n);
         a->Xb = mem.alloc<float4>(n);
         a->lamA = mem.alloc<float4>(n);
@@ -757,9 +764,10 @@ int tn_adj_forward(tn_adj* a, const float* x0, const float* theta, float* xT, do
             const int k1 = k + 1;
             float4* nxt = (k1 % a->ckpt == 0 && k1 < a->iters) ? a->ckpts + (size_t)(k1 / a->ckpt) * n
                                                                : (cur == a->Xa ? a->Xb : a->Xa);
-            tnadj::k_adj_step<<<nb, tnk::kBlock, 0, s>>>(A, cur, nxt, nullptr);
+            tnadj::k_adj_step<<<nb, tnk::kBlock, 0, s>>>(A, cur, nxt, a->fd_all ? a->fd_all + (size_t)k * n : nullptr);
             cur = nxt;
         }
+        a->last = cur;
         tnadj::k_adj_loss_parts<<<a->nparts, tnk::kBlock, 0, s>>>(A, cur, a->parts);
         tnadj::k_adj_loss_sum<<<1, tnk::kBlock, 0, s>>>(A, a->parts, a->nparts, a->dloss);
         if (xT) {
@@ -793,7 +801,16 @@ int tn_adj_backward(tn_adj* a, const float* theta, float* grad_x0, float* grad_t
         if (T == 0) {
             tnadj::k_adj_loss_grad<<<nb, tnk::kBlock, 0, s>>>(A, a->ckpts, lam, a->gtheta);
         }
-        for (int sg = a->nseg - 1; sg >= 0; --sg) {
+        if (a->fd_all) {   // every state and its F, D kept by the forward
+            tnadj::k_adj_loss_grad<<<nb, tnk::kBlock, 0, s>>>(A, a->last, lam, a->gtheta);
+            for (int k = T - 1; k >= 0; --k) {
+                tnadj::k_adj_nu<<<nb, tnk::kBlock, 0, s>>>(A, a->fd_all + (size_t)k * n, lam, a->nk);
+                tnadj::k_adj_back<<<nb, tnk::kBlock, 0, s>>>(A, a->ckpts + (size_t)k * n, a->nk, lam, lamo,
+                                                             a->gtheta);
+                std::swap(lam, lamo);
+            }
+        }
+        for (int sg = a->fd_all ? -1 : a->nseg - 1; sg >= 0; --sg) {
             const int k0 = sg * a->ckpt, k1 = std::min(T, k0 + a->ckpt);   // states k0 .. k1-1 here
             // recompute the segment: seg[k - k0] = X_k, fd[k - k0] = F, D at X_k
             TN_HIP(hipMemcpyAsync(a->seg, a->ckpts + (size_t)sg * n, sizeof(float4) * n, hipMemcpyDeviceToDevice, s));

@@ -401,6 +401,107 @@ __device__ __forceinline__ int settle_lane(float* cd, int* cj, int lane, int nva
     return m < kK ? m : kK;
 }
 
+#ifndef TN_TRUSS_MFMA
+#define TN_TRUSS_MFMA 1
+#endif
+
+// ---- the distance prefilter on the matrix cores ------------------------------------------
+//
+// For query p (this lane's node) and candidate c, with both taken relative to a
+// wave-uniform origin and k = (t + skin) / 2, the exact test is d^2 < (k (h_p + h_c))^2.
+// Its difference d^2 - k^2 (h_p + h_c)^2 is a dot product of two 6-vectors,
+//   b_c = (-2c, -2 k^2 h_c, 1, |c|^2 - k^2 h_c^2),  a_p = (p, h_p, |p|^2 - k^2 h_p^2, 1),
+// so the 64 x 64 differences of a chunk of candidates and a wave's queries come from
+// four 32 x 32 tiles of v_mfma_f32_32x32x2f32 (three K-steps each), candidates along
+// the rows and queries along the columns. A candidate whose difference is below a
+// rounding margin is a possible hit; every possible hit is then tested exactly as
+// before, so the prefilter only has to keep every true hit.
+// (v_mfma_f32_32x32x2f32: lane l supplies A[l % 32][l / 32] and B[l / 32][l % 32];
+// its output register e holds D[8 (e / 4) + 4 (l / 32) + e % 4][l % 32]. So in the
+// tile of its own query's columns a lane holds half of the tile's candidate rows
+// for its query, and lane l ^ 32 holds the other half.)
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float wave_maxf(float v) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        v = fmaxf(v, __shfl_xor(v, d, kWave));
+    }
+    return v;
+}
+
+// this lane's B operands: queries 32 c + l % 32 (c = the column tile), three K-steps
+struct QueryOps {
+    float b[2][3];
+};
+
+__device__ __forceinline__ QueryOps query_ops(float qx, float qy, float qz, float h, float Aq, int lane) {
+    const bool up = lane >= 32;
+    QueryOps o;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int src = 32 * c + (lane & 31);
+        const float x = __shfl(qx, src, kWave), y = __shfl(qy, src, kWave), z = __shfl(qz, src, kWave);
+        const float hh = __shfl(h, src, kWave), A = __shfl(Aq, src, kWave);
+        // K-index 2 t + l / 32: (x, y), (z, h), (|p|^2 - k^2 h^2, 1)
+        o.b[c][0] = up ? y : x;
+        o.b[c][1] = up ? hh : z;
+        o.b[c][2] = up ? 1.0f : A;
+    }
+    return o;
+}
+
+// the bits of one output tile: register e's test at bit 8 (e / 4) + e % 4
+__device__ __forceinline__ uint32_t tile_bits(const f32x16& acc, float margin) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        m |= acc[e] < margin ? (1u << (8 * (e / 4) + (e % 4))) : 0u;
+    }
+    return m;
+}
+
+// the 64-bit mask of this lane's query over the chunk's candidates (cpos) whose
+// difference, relative to the origin o, is below `margin`
+__device__ __forceinline__ uint64_t chunk_hits(const QueryOps& qo, const float4* cpos, float ox, float oy, float oz,
+                                               float k2, float margin, int lane) {
+    const bool up = lane >= 32;
+    const int h = lane >> 5;
+    uint32_t out[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {   // candidates 32 r .. 32 r + 31 (the row tile)
+        const float4 v = cpos[32 * r + (lane & 31)];
+        float a0, a1, a2;
+        if (v.x < __builtin_huge_valf()) {
+            const float cx = v.x - ox, cy = v.y - oy, cz = v.z - oz;
+            const float C = cx * cx + cy * cy + cz * cz - k2 * v.w * v.w;
+            // K-index 2 t + l / 32: (-2 cx, -2 cy), (-2 cz, -2 k^2 h), (1, |c|^2 - k^2 h^2)
+            a0 = up ? -2.0f * cy : -2.0f * cx;
+            a1 = up ? -2.0f * k2 * v.w : -2.0f * cz;
+            a2 = up ? C : 1.0f;
+        } else {   // past the row's end: a difference far above any margin
+            a0 = 0.0f;
+            a1 = 0.0f;
+            a2 = up ? 1e30f : 0.0f;
+        }
+        uint32_t bits[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {   // queries 32 c .. 32 c + 31 (the column tile)
+            f32x16 acc = {};
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, qo.b[c][0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, qo.b[c][1], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, qo.b[c][2], acc, 0, 0, 0);
+            bits[c] = tile_bits(acc, margin);
+        }
+        // my query's column tile is h: my half of its rows is mine, lane l ^ 32 has the rest
+        const uint32_t mine = h ? bits[1] : bits[0], give = h ? bits[0] : bits[1];
+        const uint32_t got = (uint32_t)__shfl_xor((int)give, 32, kWave);
+        out[r] = (mine << (4 * h)) | (got << (4 * (1 - h)));
+    }
+    return ((uint64_t)out[1] << 32) | out[0];
+}
+
 // The lists of all nodes at a rebuild: query q is the node in slot q.
 __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n, int32_t* __restrict__ nbrT,
                                                       int32_t* __restrict__ nnb) {
@@ -447,6 +548,15 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
             const int lrow = __shfl(grow, lead, kWave), lx = __shfl(gx, lead, kWave);
             const bool ingroup = pending && grow == lrow && abs(gx - lx) <= kGroupSpan;
             pending = pending && !ingroup;
+#if TN_TRUSS_MFMA
+            // the prefilter's origin: the group's first query; its query operands
+            const float gox = __shfl(px, lead, kWave), goy = __shfl(py, lead, kWave), goz = __shfl(pz, lead, kWave);
+            const float k2 = half * half;
+            const float qx = px - gox, qy = py - goy, qz = pz - goz;
+            const float q2 = qx * qx + qy * qy + qz * qz;
+            const QueryOps qops = query_ops(qx, qy, qz, hi, q2 - k2 * hi * hi, lane);
+            const float P2 = wave_maxf(ingroup ? q2 : 0.0f), HP = wave_maxf(ingroup ? hi : 0.0f);
+#endif
             const int lmin = wave_min(ingroup ? max(Li - 1, 0) : INT_MAX);
             const int lmax = wave_max(ingroup ? min(Li + 1, H.nlev - 1) : -1);
             for (int L = lmin; L <= lmax; ++L) {
@@ -507,6 +617,51 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                                 cpos[lane] = make_float4(inf, inf, inf, 0.0f);
                             }
                             __syncthreads();
+#if TN_TRUSS_MFMA
+                            if (!__ballot(scan)) {
+                                continue;
+                            }
+                            {
+                                const int cn = min(kTrussWave, sb - base);
+                                // the chunk's own bound for the rounding margin
+                                const float4 mine_c = cpos[lane];
+                                const bool cval = mine_c.x < __builtin_huge_valf();
+                                const float ccx = mine_c.x - gox, ccy = mine_c.y - goy, ccz = mine_c.z - goz;
+                                const float C2 = wave_maxf(cval ? ccx * ccx + ccy * ccy + ccz * ccz : 0.0f);
+                                const float HC = wave_maxf(cval ? mine_c.w : 0.0f);
+                                const float hs = HP + HC;
+                                const float margin = (2.0f * (P2 + C2) + 2.0f * k2 * hs * hs) * (1.0f / 65536.0f);
+                                uint64_t hit = chunk_hits(qops, cpos, gox, goy, goz, k2, margin, lane);
+                                if (!scan) {
+                                    continue;
+                                }
+                                hit &= cn == kTrussWave ? ~0ull : ((1ull << cn) - 1);
+                                while (hit) {   // possible hits: the exact test, then appended raw
+                                    const int k = __builtin_ctzll(hit);
+                                    hit &= hit - 1;
+                                    const float4 c = cpos[k];
+                                    const float ex = c.x - px, ey = c.y - py, ez = c.z - pz;
+                                    const float d2 = ex * ex + ey * ey + ez * ez;
+                                    const float rj = half * (hi + c.w);
+                                    if (!(d2 < rj * rj)) {
+                                        continue;
+                                    }
+                                    if (!H.prune) {   // the reference's own scan region only
+                                        const int cx = cell_coord(H, L, c.x, H.ox, 0);
+                                        if (cx < x0 || cx > x1) {
+                                            continue;
+                                        }
+                                    }
+                                    if (cnt == kCollect) {   // keep the 24 best so far
+                                        nval = cnt = settle_lane(cd, cj, lane, nval, cnt, q, ti_int, li, in.Ms);
+                                    }
+                                    cd[cnt * kTrussWave + lane] = d2;
+                                    cj[cnt * kTrussWave + lane] = base + k;
+                                    ++cnt;
+                                }
+                                continue;
+                            }
+#endif
                             if (!scan) {
                                 continue;
                             }
