@@ -48,7 +48,7 @@ enum {
     TN_ERR_ARG = 1,         /* a null pointer, a size mismatch, an out-of-range value */
     TN_ERR_NODEVICE = 2,    /* no HIP device, or the device index is out of range */
     TN_ERR_HIP = 3,         /* a HIP runtime call failed */
-    TN_ERR_UNSUPPORTED = 4, /* a mode this build does not run on the device (gray / shape fields, voxel trap) */
+    TN_ERR_UNSUPPORTED = 4, /* a mode this build does not run on the device (gray-scale / shape fields) */
     TN_ERR_INTERNAL = 5
 };
 

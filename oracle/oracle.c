@@ -7,7 +7,7 @@
  * the reference's association and operand order exactly (the results must be
  * bit-identical), compiled with -ffp-contract=off like the reference
  * (CMakeLists.txt:69-72). Only label volumes (gm == 0) with interface trapping
- * are covered; gray-scale and shape fields and the voxel trap are refused.
+ * are covered, the voxel trap included; gray-scale and shape fields are refused.
  */
 #include "oracle.h"
 
@@ -733,10 +733,254 @@ static float trapped_step(const Fld* f, float h, float* s, float maxstep, float 
     return mv / h;
 }
 
+/* ---- the voxel trap: the exact staircase of the label volume ----------------------- */
+
+/* v2m_label_at, src/opencl/v2m_grid_body.cl:72-78 (0 outside the grid) */
+static inline int label_at(const Fld* f, int i, int j, int k) {
+    if (i < 0 || j < 0 || k < 0 || i >= f->nx || j >= f->ny || k >= f->nz) {
+        return 0;
+    }
+    return f->L[vox(f, i, j, k)];
+}
+
+/* v2m_dda_hit, src/opencl/v2m_particle_body.cl:373-431: walk p -> p + s through the
+ * voxels of label a; at the first face into another label, the face point into x,
+ * that label into *b, and the ray fraction is returned; -1 if the step stays in a */
+static float dda_hit(const Fld* f, int a, const float* p, const float* s, float* x, int* b) {
+    const float vs[3] = { f->vx, f->vy, f->vz };
+    float u[3], w[3];
+    int id[3];
+    for (int k = 0; k < 3; ++k) {
+        u[k] = p[k] / vs[k] + 0.5f;
+        w[k] = s[k] / vs[k];
+        id[k] = (int)floorf(u[k]);
+    }
+    int l = label_at(f, id[0], id[1], id[2]);
+    if (l != a) {
+        x[0] = p[0];
+        x[1] = p[1];
+        x[2] = p[2];
+        *b = l;
+        return 0.0f;
+    }
+    float t = 0.0f;
+    for (int it = 0; it < 64; ++it) {
+        float ht[3];
+        for (int k = 0; k < 3; ++k) {
+            ht[k] = w[k] != 0.0f ? fabsf(((float)id[k] + (w[k] > 0.0f ? 1.0f : 0.0f) - u[k]) / w[k]) : 1e30f;
+        }
+        const float dt = fminf(fminf(ht[0], ht[1]), ht[2]);
+        const int ax = dt == ht[0] ? 0 : (dt == ht[1] ? 1 : 2);
+        if (t + dt >= 1.0f) {
+            return -1.0f;
+        }
+        t += dt;
+        for (int k = 0; k < 3; ++k) {
+            u[k] += dt * w[k];
+        }
+        u[ax] = (float)id[ax] + (w[ax] > 0.0f ? 1.0f : 0.0f);
+        id[ax] += w[ax] > 0.0f ? 1 : -1;
+        l = label_at(f, id[0], id[1], id[2]);
+        if (l != a) {
+            for (int k = 0; k < 3; ++k) {
+                x[k] = (u[k] - 0.5f) * vs[k];
+            }
+            *b = l;
+            return t;
+        }
+    }
+    return -1.0f;
+}
+
+/* v2m_vox_nearest, src/opencl/v2m_particle_body.cl:437-529: the nearest point to q on
+ * the a|b voxel faces (c == NOLAB) or the a|b|c voxel edges within R voxels; its
+ * squared distance, or -1 */
+static float vox_nearest(const Fld* f, int a, int b, int c, const float* q, int R, float* out) {
+    const float vs[3] = { f->vx, f->vy, f->vz };
+    float u[3];
+    int ci[3];
+    for (int k = 0; k < 3; ++k) {
+        u[k] = q[k] / vs[k] + 0.5f;
+        ci[k] = (int)floorf(u[k]);
+    }
+    float best = -1.0f;
+    for (int vk = ci[2] - R; vk <= ci[2] + R; ++vk) {
+        for (int vj = ci[1] - R; vj <= ci[1] + R; ++vj) {
+            for (int vi = ci[0] - R; vi <= ci[0] + R; ++vi) {
+                if (label_at(f, vi, vj, vk) != a) {
+                    continue;
+                }
+                const int v[3] = { vi, vj, vk };
+                for (int ax = 0; ax < 3; ++ax) {
+                    const int o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
+                    if (c == TN_NOLAB) {
+                        for (int sd = 0; sd < 2; ++sd) {
+                            int nn[3] = { v[0], v[1], v[2] };
+                            nn[ax] += sd ? 1 : -1;
+                            if (label_at(f, nn[0], nn[1], nn[2]) != b) {
+                                continue;
+                            }
+                            float y[3];
+                            y[ax] = (float)(v[ax] + sd);
+                            y[o1] = fminf(fmaxf(u[o1], (float)v[o1]), (float)(v[o1] + 1));
+                            y[o2] = fminf(fmaxf(u[o2], (float)v[o2]), (float)(v[o2] + 1));
+                            float d2 = 0.0f;
+                            for (int k = 0; k < 3; ++k) {
+                                y[k] = (y[k] - 0.5f) * vs[k];
+                                d2 += (y[k] - q[k]) * (y[k] - q[k]);
+                            }
+                            if (best < 0.0f || d2 < best) {
+                                best = d2;
+                                out[0] = y[0];
+                                out[1] = y[1];
+                                out[2] = y[2];
+                            }
+                        }
+                    } else {
+                        for (int e = 0; e < 4; ++e) {
+                            const int e1 = e & 1, e2 = e >> 1;
+                            int hb = 0, hc = 0;
+                            for (int m = 0; m < 4; ++m) {
+                                int nn[3] = { v[0], v[1], v[2] };
+                                nn[o1] += e1 - 1 + (m & 1);
+                                nn[o2] += e2 - 1 + (m >> 1);
+                                const int ln = label_at(f, nn[0], nn[1], nn[2]);
+                                hb |= ln == b;
+                                hc |= ln == c;
+                            }
+                            if (!hb || !hc) {
+                                continue;
+                            }
+                            float y[3];
+                            y[ax] = fminf(fmaxf(u[ax], (float)v[ax]), (float)(v[ax] + 1));
+                            y[o1] = (float)(v[o1] + e1);
+                            y[o2] = (float)(v[o2] + e2);
+                            float d2 = 0.0f;
+                            for (int k = 0; k < 3; ++k) {
+                                y[k] = (y[k] - 0.5f) * vs[k];
+                                d2 += (y[k] - q[k]) * (y[k] - q[k]);
+                            }
+                            if (best < 0.0f || d2 < best) {
+                                best = d2;
+                                out[0] = y[0];
+                                out[1] = y[1];
+                                out[2] = y[2];
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    return best;
+}
+
+/* v2m_vox_third, src/opencl/v2m_particle_body.cl:533-545 */
+static int vox_third(const Fld* f, int a, int b, const float* p) {
+    const int i0 = (int)floorf(p[0] / f->vx), j0 = (int)floorf(p[1] / f->vy), k0 = (int)floorf(p[2] / f->vz);
+    for (int m = 0; m < 8; ++m) {
+        const int l = label_at(f, i0 + (m & 1), j0 + ((m >> 1) & 1), k0 + (m >> 2));
+        if (l != a && l != b) {
+            return l;
+        }
+    }
+    return TN_NOLAB;
+}
+
+/* v2m_move_voxel, src/opencl/v2m_particle_body.cl:549-650: node i's step s (capped) in
+ * voxel mode; returns |dp| / h */
+static float voxel_step(const Fld* f, float h, float snap, int i, const float* s, NodeIO* io) {
+    const float vmin = fminf(fminf(f->vx, f->vy), f->vz);
+    float p[3], q[3], x[3];
+    p[0] = io->P[3 * i];
+    p[1] = io->P[3 * i + 1];
+    p[2] = io->P[3 * i + 2];
+    const int a = io->lab[i];
+    int ty = io->typ[i], b = io->part[2 * i], c = io->part[2 * i + 1];
+    const float sl = sqrtf(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    const int R = 1 + (int)ceilf(sl / vmin);
+    for (int k = 0; k < 3; ++k) {
+        q[k] = p[k] + s[k];
+    }
+    if (ty == TN_INTERIOR) {
+        int l;
+        const float t = dda_hit(f, a, p, s, x, &l);
+        if (t >= 0.0f) {
+            b = l;
+            if (vox_nearest(f, a, b, TN_NOLAB, q, R, q) < 0.0f) {
+                q[0] = x[0];
+                q[1] = x[1];
+                q[2] = x[2];
+            }
+            ty = TN_INTERFACE;
+        } else if (snap > 0.0f) {
+            const int Rs = 1 + (int)ceilf(snap * h / vmin);
+            float best = -1.0f, y[3];
+            for (int dz = -1; dz <= 1; ++dz) {
+                for (int dy = -1; dy <= 1; ++dy) {
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int l2 = label_at(f, (int)floorf(q[0] / f->vx + 0.5f) + dx * Rs,
+                                                (int)floorf(q[1] / f->vy + 0.5f) + dy * Rs,
+                                                (int)floorf(q[2] / f->vz + 0.5f) + dz * Rs);
+                        if (l2 == a || l2 == b) {
+                            continue;
+                        }
+                        const float d2 = vox_nearest(f, a, l2, TN_NOLAB, q, Rs, y);
+                        if (d2 >= 0.0f && d2 < snap * snap * h * h && (best < 0.0f || d2 < best)) {
+                            best = d2;
+                            b = l2;
+                            x[0] = y[0];
+                            x[1] = y[1];
+                            x[2] = y[2];
+                        }
+                    }
+                }
+            }
+            if (best >= 0.0f) {
+                q[0] = x[0];
+                q[1] = x[1];
+                q[2] = x[2];
+                ty = TN_INTERFACE;
+            }
+        }
+    } else if (ty == TN_INTERFACE) {
+        if (vox_nearest(f, a, b, TN_NOLAB, q, R, q) < 0.0f) {
+            return 0.0f;
+        }
+    } else {
+        if (vox_nearest(f, a, b, c, q, R, q) < 0.0f) {
+            return 0.0f;
+        }
+    }
+    if (ty == TN_INTERFACE) {
+        const int cc = vox_third(f, a, b, q);
+        if (cc != TN_NOLAB) {
+            float r[3];
+            const float d2 = vox_nearest(f, a, b, cc, q, 1, r);
+            if (d2 >= 0.0f && d2 < 0.0625f * h * h) {
+                q[0] = r[0];
+                q[1] = r[1];
+                q[2] = r[2];
+                ty = TN_JUNCTION;
+                c = cc;
+            }
+        }
+    }
+    const float mv = sqrtf(dist2(q, p));
+    io->P[3 * i] = q[0];
+    io->P[3 * i + 1] = q[1];
+    io->P[3 * i + 2] = q[2];
+    io->typ[i] = (uint8_t)ty;
+    io->part[2 * i] = (uint16_t)(ty == TN_INTERIOR ? TN_NOLAB : b);
+    io->part[2 * i + 1] = (uint16_t)(ty >= TN_JUNCTION ? c : TN_NOLAB);
+    io->hn[i] = size_at(f, q[0], q[1], q[2]);
+    return mv / h;
+}
+
 /* v2m_move, src/opencl/v2m_particle_body.cl:844-935: the step from the force (FIRE or
  * Jacobi), the settle and cap tests, the trapped step, and FIRE's velocity */
-static float node_move(const Fld* f, const float* F, float dt, float maxstep, float snap, int i, NodeIO* io,
-                       int fire, float* V, float fdt, float falpha, float* pw) {
+static float node_move(const Fld* f, const float* F, float dt, float maxstep, float snap, int voxmode, int i,
+                       NodeIO* io, int fire, float* V, float fdt, float falpha, float* pw) {
     float s[3];
     const float p0 = io->P[3 * i], p1 = io->P[3 * i + 1], p2 = io->P[3 * i + 2];
     const float h = size_at(f, p0, p1, p2);
@@ -791,7 +1035,7 @@ static float node_move(const Fld* f, const float* F, float dt, float maxstep, fl
             s[k] *= maxstep * h / sl;
         }
     }
-    const float m = trapped_step(f, h, s, maxstep, snap, i, io);
+    const float m = voxmode ? voxel_step(f, h, snap, i, s, io) : trapped_step(f, h, s, maxstep, snap, i, io);
     if (fire) {
         if (m > 0.0f && io->typ[i] == ty0) {
             V[3 * i] = (io->P[3 * i] - p0) / fdt;
@@ -859,9 +1103,6 @@ static int supported(const tn_grid_view* g, const tn_relax_params* prm) {
     }
     if (g->gm != 0) {
         return fail("oracle: only label volumes (gm == 0)");
-    }
-    if (prm->voxel_trap) {
-        return fail("oracle: the voxel trap is not restated");
     }
     return 0;
 }
@@ -991,7 +1232,8 @@ int orc_move(const tn_grid_view* g, const tn_relax_params* prm, tn_nodes_view* n
     NodeIO io = { nd->P, nd->lab, nd->typ, nd->part, hn };
 #pragma omp parallel for schedule(dynamic, 1024)
     for (int i = 0; i < n; ++i) {
-        mv[i] = node_move(&f, F, prm->dt, prm->maxstep, prm->snap, i, &io, fire, V, fdt, falpha, pw);
+        mv[i] = node_move(&f, F, prm->dt, prm->maxstep, prm->snap, prm->voxel_trap != 0, i, &io, fire, V, fdt, falpha,
+                          pw);
     }
     return 0;
 }
@@ -1055,7 +1297,8 @@ int orc_relax(const tn_grid_view* g, const tn_relax_params* prm, tn_nodes_view* 
         float mmax = 0.0f;
 #pragma omp parallel for schedule(dynamic, 1024) reduction(max : mmax)
         for (int i = 0; i < n; ++i) {
-            mv[i] = node_move(&f, F, prm->dt, prm->maxstep, prm->snap, i, &io, fire, V, fc.dt, fc.alpha, pw);
+            mv[i] = node_move(&f, F, prm->dt, prm->maxstep, prm->snap, prm->voxel_trap != 0, i, &io, fire, V, fc.dt,
+                              fc.alpha, pw);
             mmax = mmax > mv[i] ? mmax : mv[i];
         }
         double pt = 0.0;

@@ -34,8 +34,10 @@ class Fixture:
         from trussnet_amd._abi import Nodes, RelaxParams
 
         out = []
-        for r, (iters, dptol, fire) in enumerate(self.d["runs"]):
-            prm = RelaxParams(max_iters=int(iters), dptol=float(dptol), fire=bool(fire))
+        for r, run in enumerate(self.d["runs"]):
+            iters, dptol, fire = run[:3]
+            vox = len(run) > 3 and bool(run[3])
+            prm = RelaxParams(max_iters=int(iters), dptol=float(dptol), fire=bool(fire), voxel_trap=vox)
             res = Nodes(self.d[f"run{r}_P"], self.seeds.lab, self.d[f"run{r}_typ"], self.d[f"run{r}_part"])
             stats = dict(zip(self.stat_keys, self.d[f"run{r}_stats"]))
             out.append((prm, res, stats))

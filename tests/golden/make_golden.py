@@ -64,7 +64,8 @@ CASES = {
                      runs=[(1, 0.0, True), (50, 0.0, True), (300, 2e-3, True), (50, 0.0, False)]),
     # a graded 4-label ball: junction lines, corners, a multi-level hash
     "sectors48": dict(vol=lambda: volumes.sectors(48), grid=dict(hbase=3.0),
-                      runs=[(1, 0.0, True), (10, 0.0, True), (100, 0.0, True), (400, 2e-3, True), (40, 0.0, False)]),
+                      runs=[(1, 0.0, True), (10, 0.0, True), (100, 0.0, True), (400, 2e-3, True), (40, 0.0, False),
+                            (60, 0.0, False, True), (300, 2e-3, False, True)]),
     # nested perturbed ellipsoids with a user sizing field graded off the interfaces
     "ellipsoids56": dict(vol=lambda: volumes.nested_ellipsoids(56, nlab=4, seed=3, axes=(6.0, 24.0), grf_sigma=2.0,
                                                                grf_amp=1.0, centre_jitter=2.0),
@@ -77,7 +78,7 @@ CASES = {
                   runs=[(1, 0.0, True), (30, 0.0, True), (200, 2e-3, True)]),
     # anisotropic voxels (0.8 x 0.9 x 1.3 mm): every coordinate -> voxel division is inexact
     "aniso40": dict(vol=lambda: volumes.sectors(40), vs=(0.8, 0.9, 1.3), grid=dict(hbase=2.5),
-                    runs=[(1, 0.0, True), (20, 0.0, True), (150, 2e-3, True), (30, 0.0, False)]),
+                    runs=[(1, 0.0, True), (20, 0.0, True), (150, 2e-3, True), (30, 0.0, False), (40, 0.0, False, True)]),
 }
 
 
@@ -117,6 +118,9 @@ def make(name: str, spec: dict) -> str:
     out["grid_grade"] = G.grade
     lat = rg.seed(RelaxParams(jseed=0.0, corners=False))
     out["lat_P"], out["lat_lab"], out["lat_typ"], out["lat_part"] = lat.P, lat.lab, lat.typ, lat.part
+    # the lattice seeds in voxel mode (each point takes its voxel's label)
+    vx = rg.seed(RelaxParams(jseed=0.0, corners=False, voxel_trap=True))
+    out["vox_P"], out["vox_lab"], out["vox_typ"], out["vox_part"] = vx.P, vx.lab, vx.typ, vx.part
     # junction-line seeds on a finer key grid, no corners
     js = rg.seed(RelaxParams(jseed=0.4, corners=False))
     out["js_P"], out["js_lab"], out["js_typ"], out["js_part"] = js.P, js.lab, js.typ, js.part
@@ -147,12 +151,14 @@ def make(name: str, spec: dict) -> str:
     out["fq_X"], out["fq_ab"] = X, np.array([a, b], np.int32)
     out["fq_phi"], out["fq_psi"], out["fq_grad"], out["fq_label"], out["fq_second"] = phi, psi, grad, labx, sec
     runs = []
-    for r, (iters, dptol, fire) in enumerate(spec["runs"]):
-        p = RelaxParams(max_iters=iters, dptol=dptol, fire=fire)
+    for r, run in enumerate(spec["runs"]):
+        iters, dptol, fire = run[:3]
+        vox = len(run) > 3 and run[3]
+        p = RelaxParams(max_iters=iters, dptol=dptol, fire=fire, voxel_trap=vox)
         res, st = rg.relax(seeds, p)
         out[f"run{r}_P"], out[f"run{r}_typ"], out[f"run{r}_part"] = res.P, res.typ, res.part
         out[f"run{r}_stats"] = np.array([st[k] for k in STAT_KEYS], np.float64)
-        runs.append((iters, dptol, int(fire)))
+        runs.append((iters, dptol, int(fire), int(vox)))
         print(f"  {name} run {r}: iters {st['iters']} rebuilds {st['rebuilds']} resets {st['fire_resets']} "
               f"types {[st['n_interior'], st['n_interface'], st['n_junction'], st['n_corner']]}")
     out["runs"] = np.array(runs, np.float64)

@@ -103,7 +103,7 @@ def test_bad_arguments_rejected(tnlib):
     with pytest.raises(tnlib.DeviceError):
         tnlib.relax(bad, fx.seeds)
     with pytest.raises(tnlib.DeviceError):
-        tnlib.relax(g, fx.seeds, RelaxParams(voxel_trap=True))
+        tnlib.relax(g, fx.seeds, RelaxParams(max_iters=-1))
 
 
 def test_cpp_wrapper_builds_and_links(tnlib, tmp_path):

@@ -58,17 +58,19 @@ def test_relax_runs(tn, case):
             assert np.float32(st[k]) == np.float32(v), (case.name, prm.max_iters, k, st[k], v)
 
 
-@pytest.mark.parametrize("variant", ["lattice", "junctions", "default"])
+@pytest.mark.parametrize("variant", ["lattice", "junctions", "default", "voxel"])
 def test_seeding(tn, case, variant):
     """Device seeding = the reference's seed_cpu: the lattice alone; with junction-line
-    seeds on a 0.4-spacing key grid; with the defaults (junction seeds at 0.8, corners)."""
+    seeds on a 0.4-spacing key grid; with the defaults (junction seeds at 0.8, corners);
+    the lattice in voxel-trap mode (each point takes its voxel's label)."""
     from trussnet_amd._abi import Nodes
 
     d = case.d
     hcp = "hcp_" if "hcp_P" in d else "seed_"   # seed_cpu's own nodes (a case may relax other seeds)
     prm, pre = {"lattice": (RelaxParams(jseed=0.0, corners=False), "lat_"),
                 "junctions": (RelaxParams(jseed=0.4, corners=False), "js_"),
-                "default": (RelaxParams(), hcp)}[variant]
+                "default": (RelaxParams(), hcp),
+                "voxel": (RelaxParams(jseed=0.0, corners=False, voxel_trap=True), "vox_")}[variant]
     got = tn.seed(case.grid, prm, device=0)
     want = Nodes(d[pre + "P"], d[pre + "lab"], d[pre + "typ"], d[pre + "part"])
     assert got.n == want.n, (case.name, variant, got.n, want.n)

@@ -1030,4 +1030,251 @@ __device__ __forceinline__ float trapped_step(const Field& f, float h, P3 s, flo
     return mv / h;
 }
 
+// ---- the voxel trap: the exact staircase of the label volume ---------------------------
+// (v2m_move_voxel and its helpers, src/opencl/v2m_particle_body.cl:373-650: interior
+// nodes walk their step voxel by voxel and stop on the first face into another label;
+// interface and junction nodes take the nearest point on their a|b faces or a|b|c
+// edges. Integer label reads only; no phi.)
+
+// v2m_label_at, src/opencl/v2m_grid_body.cl:72-78 (0 outside the grid)
+__device__ __forceinline__ int label_or_zero(const Field& f, int i, int j, int k) {
+    if (i < 0 || j < 0 || k < 0 || i >= f.nx || j >= f.ny || k >= f.nz) {
+        return 0;
+    }
+    return f.L[vox(f, i, j, k)];
+}
+
+__device__ __forceinline__ float over_v(const Field& f, float x, int ax) {
+    return ax == 0 ? over_vx(f, x) : (ax == 1 ? over_vy(f, x) : over_vz(f, x));
+}
+
+// v2m_dda_hit, src/opencl/v2m_particle_body.cl:373-431
+__device__ float dda_hit(const Field& f, int a, const float* p, const float* s, float* x, int* b) {
+    const float vs[3] = { f.vx, f.vy, f.vz };
+    float u[3], w[3];
+    int id[3];
+    for (int k = 0; k < 3; ++k) {
+        u[k] = over_v(f, p[k], k) + 0.5f;
+        w[k] = over_v(f, s[k], k);
+        id[k] = (int)floorf(u[k]);
+    }
+    int l = label_or_zero(f, id[0], id[1], id[2]);
+    if (l != a) {
+        x[0] = p[0];
+        x[1] = p[1];
+        x[2] = p[2];
+        *b = l;
+        return 0.0f;
+    }
+    float t = 0.0f;
+    for (int it = 0; it < 64; ++it) {
+        float ht[3];
+        for (int k = 0; k < 3; ++k) {
+            ht[k] = w[k] != 0.0f ? fabsf(((float)id[k] + (w[k] > 0.0f ? 1.0f : 0.0f) - u[k]) / w[k]) : 1e30f;
+        }
+        const float dt = fminf(fminf(ht[0], ht[1]), ht[2]);
+        const int ax = dt == ht[0] ? 0 : (dt == ht[1] ? 1 : 2);
+        if (t + dt >= 1.0f) {
+            return -1.0f;
+        }
+        t += dt;
+        for (int k = 0; k < 3; ++k) {
+            u[k] += dt * w[k];
+        }
+        u[ax] = (float)id[ax] + (w[ax] > 0.0f ? 1.0f : 0.0f);
+        id[ax] += w[ax] > 0.0f ? 1 : -1;
+        l = label_or_zero(f, id[0], id[1], id[2]);
+        if (l != a) {
+            for (int k = 0; k < 3; ++k) {
+                x[k] = (u[k] - 0.5f) * vs[k];
+            }
+            *b = l;
+            return t;
+        }
+    }
+    return -1.0f;
+}
+
+// v2m_vox_nearest, src/opencl/v2m_particle_body.cl:437-529
+__device__ float vox_nearest(const Field& f, int a, int b, int c, const float* q, int R, float* out) {
+    const float vs[3] = { f.vx, f.vy, f.vz };
+    float u[3];
+    int ci[3];
+    for (int k = 0; k < 3; ++k) {
+        u[k] = over_v(f, q[k], k) + 0.5f;
+        ci[k] = (int)floorf(u[k]);
+    }
+    float best = -1.0f;
+    for (int vk = ci[2] - R; vk <= ci[2] + R; ++vk) {
+        for (int vj = ci[1] - R; vj <= ci[1] + R; ++vj) {
+            for (int vi = ci[0] - R; vi <= ci[0] + R; ++vi) {
+                if (label_or_zero(f, vi, vj, vk) != a) {
+                    continue;
+                }
+                const int v[3] = { vi, vj, vk };
+                for (int ax = 0; ax < 3; ++ax) {
+                    const int o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
+                    if (c == kNoLab) {   // the two faces normal to ax
+                        for (int sd = 0; sd < 2; ++sd) {
+                            int nn[3] = { v[0], v[1], v[2] };
+                            nn[ax] += sd ? 1 : -1;
+                            if (label_or_zero(f, nn[0], nn[1], nn[2]) != b) {
+                                continue;
+                            }
+                            float y[3];
+                            y[ax] = (float)(v[ax] + sd);
+                            y[o1] = fminf(fmaxf(u[o1], (float)v[o1]), (float)(v[o1] + 1));
+                            y[o2] = fminf(fmaxf(u[o2], (float)v[o2]), (float)(v[o2] + 1));
+                            float d2 = 0.0f;
+                            for (int k = 0; k < 3; ++k) {
+                                y[k] = (y[k] - 0.5f) * vs[k];
+                                d2 += (y[k] - q[k]) * (y[k] - q[k]);
+                            }
+                            if (best < 0.0f || d2 < best) {
+                                best = d2;
+                                out[0] = y[0];
+                                out[1] = y[1];
+                                out[2] = y[2];
+                            }
+                        }
+                    } else {   // the four edges parallel to ax
+                        for (int e = 0; e < 4; ++e) {
+                            const int e1 = e & 1, e2 = e >> 1;
+                            int hb = 0, hc = 0;
+                            for (int m = 0; m < 4; ++m) {
+                                int nn[3] = { v[0], v[1], v[2] };
+                                nn[o1] += e1 - 1 + (m & 1);
+                                nn[o2] += e2 - 1 + (m >> 1);
+                                const int ln = label_or_zero(f, nn[0], nn[1], nn[2]);
+                                hb |= ln == b;
+                                hc |= ln == c;
+                            }
+                            if (!hb || !hc) {
+                                continue;
+                            }
+                            float y[3];
+                            y[ax] = fminf(fmaxf(u[ax], (float)v[ax]), (float)(v[ax] + 1));
+                            y[o1] = (float)(v[o1] + e1);
+                            y[o2] = (float)(v[o2] + e2);
+                            float d2 = 0.0f;
+                            for (int k = 0; k < 3; ++k) {
+                                y[k] = (y[k] - 0.5f) * vs[k];
+                                d2 += (y[k] - q[k]) * (y[k] - q[k]);
+                            }
+                            if (best < 0.0f || d2 < best) {
+                                best = d2;
+                                out[0] = y[0];
+                                out[1] = y[1];
+                                out[2] = y[2];
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    return best;
+}
+
+// v2m_vox_third, src/opencl/v2m_particle_body.cl:533-545
+__device__ __forceinline__ int vox_third(const Field& f, int a, int b, const float* p) {
+    const int i0 = (int)floorf(over_vx(f, p[0])), j0 = (int)floorf(over_vy(f, p[1])),
+              k0 = (int)floorf(over_vz(f, p[2]));
+    for (int m = 0; m < 8; ++m) {
+        const int l = label_or_zero(f, i0 + (m & 1), j0 + ((m >> 1) & 1), k0 + (m >> 2));
+        if (l != a && l != b) {
+            return l;
+        }
+    }
+    return kNoLab;
+}
+
+// v2m_move_voxel, src/opencl/v2m_particle_body.cl:549-650: the capped step s of a node
+// of label a in state ns. Returns |dp| / h (0 and ns untouched when the node's face
+// patch or edge vanished within reach).
+__device__ float voxel_step(const Field& f, float h, float snap, int a, const float* s, NodeState& ns) {
+    const float vmin = fminf(fminf(f.vx, f.vy), f.vz);
+    const float p[3] = { ns.p.x, ns.p.y, ns.p.z };
+    float q[3], x[3];
+    int ty = ns.ty, b = ns.b, c = ns.c;
+    const float sl = sqrtf(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    const int R = 1 + (int)ceilf(sl / vmin);
+    for (int k = 0; k < 3; ++k) {
+        q[k] = p[k] + s[k];
+    }
+    if (ty == TN_INTERIOR) {
+        int l;
+        const float t = dda_hit(f, a, p, s, x, &l);
+        if (t >= 0.0f) {   // hit a wall: trap on it, glide with the rest of the step
+            b = l;
+            if (vox_nearest(f, a, b, kNoLab, q, R, q) < 0.0f) {
+                q[0] = x[0];
+                q[1] = x[1];
+                q[2] = x[2];
+            }
+            ty = TN_INTERFACE;
+        } else if (snap > 0.0f) {   // close to any wall: snap onto it
+            const int Rs = 1 + (int)ceilf(snap * h / vmin);
+            float best = -1.0f, y[3];
+            for (int dz = -1; dz <= 1; ++dz) {
+                for (int dy = -1; dy <= 1; ++dy) {
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int l2 = label_or_zero(f, (int)floorf(over_vx(f, q[0]) + 0.5f) + dx * Rs,
+                                                     (int)floorf(over_vy(f, q[1]) + 0.5f) + dy * Rs,
+                                                     (int)floorf(over_vz(f, q[2]) + 0.5f) + dz * Rs);
+                        if (l2 == a || l2 == b) {
+                            continue;
+                        }
+                        const float d2 = vox_nearest(f, a, l2, kNoLab, q, Rs, y);
+                        if (d2 >= 0.0f && d2 < snap * snap * h * h && (best < 0.0f || d2 < best)) {
+                            best = d2;
+                            b = l2;
+                            x[0] = y[0];
+                            x[1] = y[1];
+                            x[2] = y[2];
+                        }
+                    }
+                }
+            }
+            if (best >= 0.0f) {
+                q[0] = x[0];
+                q[1] = x[1];
+                q[2] = x[2];
+                ty = TN_INTERFACE;
+            }
+        }
+    } else if (ty == TN_INTERFACE) {
+        if (vox_nearest(f, a, b, kNoLab, q, R, q) < 0.0f) {
+            return 0.0f;
+        }
+    } else {   // junction: the nearest a|b|c voxel edge
+        if (vox_nearest(f, a, b, c, q, R, q) < 0.0f) {
+            return 0.0f;
+        }
+    }
+    // an interface node at a staircase corner where a third label meets: pinned on the
+    // nearest a|b|c edge when that is close
+    if (ty == TN_INTERFACE) {
+        const int cc = vox_third(f, a, b, q);
+        if (cc != kNoLab) {
+            float r[3];
+            const float d2 = vox_nearest(f, a, b, cc, q, 1, r);
+            if (d2 >= 0.0f && d2 < 0.0625f * h * h) {
+                q[0] = r[0];
+                q[1] = r[1];
+                q[2] = r[2];
+                ty = TN_JUNCTION;
+                c = cc;
+            }
+        }
+    }
+    const float mv = sqrtf((q[0] - p[0]) * (q[0] - p[0]) + (q[1] - p[1]) * (q[1] - p[1]) + (q[2] - p[2]) * (q[2] - p[2]));
+    ns.p = p3(q[0], q[1], q[2]);
+    ns.ty = ty;
+    ns.b = ty == TN_INTERIOR ? kNoLab : b;
+    ns.c = ty >= TN_JUNCTION ? c : kNoLab;
+    ns.hn = size_at(f, q[0], q[1], q[2]);
+    return mv / h;
+}
+
 }  // namespace tnd

@@ -1168,6 +1168,46 @@ __device__ __forceinline__ bool move_node(const Field& f, const MoveArgs& ma, co
     return false;
 }
 
+// The whole move in voxel-trap mode (v2m_move with voxmode, src/opencl/v2m_particle_body.cl:
+// 844-935 and 549-650): every node in one pass; corners and settled nodes stay.
+__device__ __forceinline__ void k_move_voxel_body(const Field& f, const MoveArgs& ma, const Ctl* __restrict__ ctl,
+                                                  int n, const float4* __restrict__ F, float4* __restrict__ X,
+                                                  float4* __restrict__ V, uint8_t* __restrict__ typ,
+                                                  uint32_t* __restrict__ part, const uint16_t* __restrict__ lab,
+                                                  float* __restrict__ mv, float* __restrict__ pw, int bid) {
+    const int i = bid * kBlock + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const float4 x = X[i];
+    const float h = x.w;
+    const int ty0 = typ[i];
+    const uint32_t pp = part[i];
+    NodeState ns;
+    ns.p = p3(x.x, x.y, x.z);
+    ns.hn = h;
+    ns.ty = ty0;
+    ns.b = (int)(pp & 0xFFFF);
+    ns.c = (int)(pp >> 16);
+    float m = 0.0f, p_w = 0.0f;
+    if (ty0 != TN_CORNER) {
+        const NodeStep st = node_step(ma, ctl, F[i], V, i, h);
+        p_w = st.pw;
+        if (st.moves) {
+            m = voxel_step(f, h, ma.snap, lab[i], st.s, ns);
+        }
+    }
+    move_store(ma, ctl, i, x, ty0, pp, ns, m, p_w, X, V, typ, part, mv, pw);
+}
+
+__global__ __launch_bounds__(kBlock) void k_move_voxel(Field f, MoveArgs ma, const Ctl* __restrict__ ctl, int n,
+                                                       const float4* __restrict__ F, float4* __restrict__ X,
+                                                       float4* __restrict__ V, uint8_t* __restrict__ typ,
+                                                       uint32_t* __restrict__ part, const uint16_t* __restrict__ lab,
+                                                       float* __restrict__ mv, float* __restrict__ pw) {
+    k_move_voxel_body(f, ma, ctl, n, F, X, V, typ, part, lab, mv, pw, blockIdx.x);
+}
+
 // Pass 2: v2m_move (src/opencl/v2m_particle_body.cl:844-935) for the queued nodes,
 // *count of them. kLeanInterface / kLeanInterior: interface or interior nodes, with
 // their branch of the trapped step only and no out-of-line calls (whose register
@@ -1493,7 +1533,7 @@ struct PlanIter {
     TrussIn tin;
     MoveArgs ma;
     CtlArgs ca;
-    int n, fire, nparts;
+    int n, fire, nparts, voxel;   // voxel: the voxel-trap move (no queues)
     float fscale, fsurf, skin;
     Ctl* ctl;
     int32_t* nbrT;
@@ -1533,6 +1573,10 @@ __global__ __launch_bounds__(kBlock) void kg_move_easy(const PlanIter* __restric
                                                        const int32_t* __restrict__ bplan,
                                                        const int32_t* __restrict__ bfirst) {
     TN_PLAN_BLOCK(kShNodes);
+    if (P.voxel) {   // the whole move; its queues stay empty
+        k_move_voxel_body(P.f, P.ma, P.ctl, P.n, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv, P.pw, bid_);
+        return;
+    }
     k_move_easy_body(P.f, P.ma, P.ctl, P.n, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv, P.pw, P.mq, P.mqi, P.mq2, bid_,
                      nb_);
 }

@@ -503,6 +503,13 @@ struct tn_plan {
         timer.end(KC_FORCE, t);
         t = timer.begin();
         tnk::MoveArgs ma{ prm.dt, prm.maxstep, prm.snap, fire };
+        if (prm.voxel_trap) {   // the staircase move, every node in one pass
+            tnk::k_move_voxel<<<nb, tnk::kBlock, 0, stream>>>(f, ma, dCtl, n, dF, dX, dV, dTyp, dPart, dLab, dMv,
+                                                              dPw);
+            timer.end(KC_MOVE_EASY, t);
+            enqueue_iteration_tail(nb);
+            return;
+        }
         tnk::k_move_easy<<<nb, tnk::kBlock, 0, stream>>>(f, ma, dCtl, n, dF, dX, dV, dTyp, dPart, dLab, dMv,
                                                          dPw, dMoveQ, dMoveQi, dMoveQ2);
         timer.end(KC_MOVE_EASY, t);
@@ -545,7 +552,12 @@ struct tn_plan {
         if (overlap_moves) {
             TN_HIP(hipStreamWaitEvent(stream, ev_rest, 0));
         }
-        t = timer.begin();
+        enqueue_iteration_tail(nb);
+    }
+
+    // stats, own-list refreshes, control, and the control block's copy to the host
+    void enqueue_iteration_tail(unsigned nb) {
+        hipEvent_t t = timer.begin();
         tnk::k_stats<<<nparts, tnk::kBlock, 0, stream>>>(dX, dP0, dMv, dPw, n, prm.skin, fire, dThr, dCtl, dFlag,
                                                          dPartials);
         timer.end(KC_STATS, t);
@@ -598,6 +610,7 @@ struct tn_plan {
         a.n = n;
         a.fire = fire;
         a.nparts = nparts;
+        a.voxel = prm.voxel_trap ? 1 : 0;
         a.fscale = prm.fscale;
         a.fsurf = prm.fsurf;
         a.skin = prm.skin;
@@ -723,9 +736,6 @@ void check_grid(const tn_grid_view* g) {
 
 void check_params(const tn_relax_params* prm) {
     arg_check(prm != nullptr, "null params");
-    if (prm->voxel_trap) {
-        throw TnError(TN_ERR_UNSUPPORTED, "the voxel trap mode is not run on the device");
-    }
     arg_check(prm->t + prm->skin > 0, "params: t + skin must be positive");
     arg_check(prm->max_iters >= 0, "params: max_iters must be >= 0");
 }
@@ -950,6 +960,7 @@ Seeds run_seeding(const tn_grid_view* g, const uint8_t* grade, const tn_relax_pa
     sa.grade = mem.upload(grade, (size_t)nv);
     sa.spacing = mem.upload(spacing.data(), spacing.size());
     sa.nseed = prm->nseed;
+    sa.voxmode = prm->voxel_trap ? 1 : 0;
     tnk::VertexGrid vg;
     vg.vx1 = g->nx + 1;
     vg.vy1 = g->ny + 1;

@@ -35,7 +35,19 @@ struct SeedArgs {
     const uint8_t* __restrict__ grade;
     const float* __restrict__ spacing;   // per seed level (v2m_seed_spacing, computed on the host)
     int nseed;
+    int voxmode;   // the voxel trap: a point's label is its voxel's
 };
+
+// v2m_label_of with voxmode 1 (src/opencl/v2m_seed_body.cl:116-175): a brick with one
+// label gives that label; otherwise the label of the voxel nearest the point
+__device__ __forceinline__ int voxel_owner(const Field& f, P3 p) {
+    const BrickLabels bl = brick_labels_at(f, p);
+    if (bl.n == 1) {
+        return brick_label(f, bl, 0);
+    }
+    return label_or_zero(f, (int)floorf(over_vx(f, p.x) + 0.5f), (int)floorf(over_vy(f, p.y) + 0.5f),
+                         (int)floorf(over_vz(f, p.z) + 0.5f));
+}
 
 // v2m_seed_voxel, src/opencl/v2m_seed_body.cl:449-495. Write: store the kept
 // points from slot `out` on. Returns the number kept.
@@ -61,7 +73,7 @@ __device__ int seed_voxel(const Field& f, const SeedArgs& sa, int i, int j, int 
             const int i0 = (int)ceilf((x0 - ox) / s), i1 = (int)ceilf((x1 - ox) / s);
             for (int ii = i0; ii < i1; ++ii) {
                 const P3 p = hcp_point(s, ii, jj, kk);
-                const int l = owner_at(f, p).x;
+                const int l = sa.voxmode ? voxel_owner(f, p) : owner_at(f, p).x;
                 if (l == 0 || l == kNoLab) {
                     continue;
                 }
@@ -198,21 +210,13 @@ __device__ __forceinline__ void vertex_ijk(const VertexGrid& vg, int64_t v, int&
     k = (int)(v / ((int64_t)vg.vx1 * vg.vy1)) - 1;
 }
 
-// v2m_label_at, src/opencl/v2m_grid_body.cl:72-78 (0 outside the grid)
-__device__ __forceinline__ int label_or_exterior(const Field& f, int i, int j, int k) {
-    if (i < 0 || j < 0 || k < 0 || i >= f.nx || j >= f.ny || k >= f.nz) {
-        return 0;
-    }
-    return f.L[vox(f, i, j, k)];
-}
-
 // v2m_vertex_nlab, src/opencl/v2m_seed_body.cl:613-630: the distinct labels of the
 // 8 voxels at vertex (i,j,k) in order of first appearance; returns their count
 __device__ __forceinline__ int vertex_labels(const Field& f, int i, int j, int k, int* labs) {
     int n = 0;
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-        const int l = label_or_exterior(f, i + (m & 1), j + ((m >> 1) & 1), k + (m >> 2));
+        const int l = label_or_zero(f, i + (m & 1), j + ((m >> 1) & 1), k + (m >> 2));
         bool seen = false;
         for (int x = 0; x < n; ++x) {
             seen |= labs[x] == l;
@@ -229,7 +233,7 @@ __device__ __forceinline__ int vertex_label_count(const Field& f, int i, int j, 
     int l[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-        l[m] = label_or_exterior(f, i + (m & 1), j + ((m >> 1) & 1), k + (m >> 2));
+        l[m] = label_or_zero(f, i + (m & 1), j + ((m >> 1) & 1), k + (m >> 2));
     }
     int n = 0;
 #pragma unroll
