@@ -48,7 +48,7 @@ enum {
     TN_ERR_ARG = 1,         /* a null pointer, a size mismatch, an out-of-range value */
     TN_ERR_NODEVICE = 2,    /* no HIP device, or the device index is out of range */
     TN_ERR_HIP = 3,         /* a HIP runtime call failed */
-    TN_ERR_UNSUPPORTED = 4, /* a mode this build does not run on the device (gray-scale / shape fields) */
+    TN_ERR_UNSUPPORTED = 4, /* a mode this build does not run on the device (shape fields, gm < 0) */
     TN_ERR_INTERNAL = 5
 };
 
@@ -69,10 +69,12 @@ typedef struct tn_grid_view {
     int64_t nslot;
     const float* h;           /* [nx*ny*nz] sizing field (mm) */
     float hmin, hmax;         /* sizing range (Grid::hmin/hmax): the hash geometry */
-    const float* gI;          /* gray-scale intensity (gm > 0), else null */
-    const float* gTW;         /* gray thresholds / shape programs (gm != 0), else null */
-    int64_t ngTW;
-    int32_t gm;               /* 0: label volume (the device path); != 0: unsupported here */
+    const float* gI;          /* gray-scale intensity per voxel (gm > 0), else null */
+    const float* gTW;         /* gm > 0: [t_0..t_{m-1}, W_0..W_{m-1}], the ascending thresholds and
+                                 their membership widths (Grid::gTW); gm < 0: shape programs */
+    int64_t ngTW;             /* values in gTW (>= 2 gm for a gray-scale grid) */
+    int32_t gm;               /* 0: label volume; m > 0: gray-scale volume with m thresholds;
+                                 < 0: shape fields (TN_ERR_UNSUPPORTED on the device) */
 } tn_grid_view;
 
 /* RelaxParams (src/v2m_particles.h:21-41) plus the rebuild divisor that the

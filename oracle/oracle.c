@@ -6,8 +6,9 @@
  * Each function names the reference lines it restates. The arithmetic keeps
  * the reference's association and operand order exactly (the results must be
  * bit-identical), compiled with -ffp-contract=off like the reference
- * (CMakeLists.txt:69-72). Only label volumes (gm == 0) with interface trapping
- * are covered, the voxel trap included; gray-scale and shape fields are refused.
+ * (CMakeLists.txt:69-72). Label volumes (gm == 0) and gray-scale volumes
+ * (gm = m > 0 thresholds) are covered, the voxel trap included; shape fields
+ * (gm < 0) are refused.
  */
 #include "oracle.h"
 
@@ -42,6 +43,9 @@ typedef struct {
     const int32_t* bl_slot;
     const float* phi;
     const float* h;
+    const float* gI;    /* gray-scale mode: intensity per voxel */
+    const float* gTW;   /* [t_0..t_{m-1}, W_0..W_{m-1}] */
+    int gm;             /* m thresholds (0: label volume) */
 } Fld;
 
 static Fld fld_of(const tn_grid_view* g) {
@@ -60,6 +64,9 @@ static Fld fld_of(const tn_grid_view* g) {
     f.bl_slot = g->bl_slot;
     f.phi = g->phi;
     f.h = g->h;
+    f.gI = g->gI;
+    f.gTW = g->gTW;
+    f.gm = g->gm;
     return f;
 }
 
@@ -142,8 +149,57 @@ static float phi_point(const Fld* f, int l, float px, float py, float pz) {
     return trilerp(c, u - (float)i0, v - (float)j0, w - (float)k0);
 }
 
-/* v2m_psi (gm == 0), src/opencl/v2m_seed_body.cl:179-205: phi_a - phi_b, trilinear */
+/* v2m_gray_at, src/opencl/v2m_seed_body.cl:65-93: the trilinear intensity at p and
+ * (g non-null) its gradient; a corner outside the grid reads t_0 - W_0 */
+static float gray_at(const Fld* f, float px, float py, float pz, float* g) {
+    const float u = px / f->vx, v = py / f->vy, w = pz / f->vz;
+    const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
+    const float fx = u - (float)i0, fy = v - (float)j0, fz = w - (float)k0;
+    const float out = f->gTW[0] - f->gTW[f->gm];
+    float c[8];
+    for (int m = 0; m < 8; ++m) {
+        const int i = i0 + (m & 1), j = j0 + ((m >> 1) & 1), k = k0 + (m >> 2);
+        c[m] = (i < 0 || j < 0 || k < 0 || i >= f->nx || j >= f->ny || k >= f->nz) ? out : f->gI[vox(f, i, j, k)];
+    }
+    const float x00 = c[0] + fx * (c[1] - c[0]), x10 = c[2] + fx * (c[3] - c[2]);
+    const float x01 = c[4] + fx * (c[5] - c[4]), x11 = c[6] + fx * (c[7] - c[6]);
+    const float y0 = x00 + fy * (x10 - x00), y1 = x01 + fy * (x11 - x01);
+    if (g) {
+        const float dx0 = (c[1] - c[0]) + fy * ((c[3] - c[2]) - (c[1] - c[0]));
+        const float dx1 = (c[5] - c[4]) + fy * ((c[7] - c[6]) - (c[5] - c[4]));
+        g[0] = (dx0 + fz * (dx1 - dx0)) / f->vx;
+        g[1] = ((x10 - x00) + fz * ((x11 - x01) - (x10 - x00))) / f->vy;
+        g[2] = (y1 - y0) / f->vz;
+    }
+    return y0 + fz * (y1 - y0);
+}
+
+/* v2m_gray_psi, src/opencl/v2m_seed_body.cl:97-113: for adjacent labels (|a - b| = 1)
+ * of a gray-scale volume, psi_ab = +-2 (I - t_k) / W_k with k = min(a, b), and its
+ * gradient (g non-null). Returns 0 where it does not apply (also for a pair above
+ * the last threshold, which a gray-scale volume does not hold: gTW has no entry). */
+static int gray_psi(const Fld* f, int a, int b, float px, float py, float pz, float* v, float* g) {
+    const int k = a < b ? a : b;
+    if (f->gm <= 0 || (a - b != 1 && b - a != 1) || k >= f->gm) {
+        return 0;
+    }
+    const float sg = (a > b ? 2.0f : -2.0f) / f->gTW[f->gm + k];
+    *v = sg * (gray_at(f, px, py, pz, g) - f->gTW[k]);
+    if (g) {
+        g[0] *= sg;
+        g[1] *= sg;
+        g[2] *= sg;
+    }
+    return 1;
+}
+
+/* v2m_psi, src/opencl/v2m_seed_body.cl:179-205: phi_a - phi_b, trilinear (the
+ * gray-scale form for adjacent labels) */
 static float psi_point(const Fld* f, int a, int b, float px, float py, float pz) {
+    float gv;
+    if (gray_psi(f, a, b, px, py, pz, &gv, NULL)) {
+        return gv;
+    }
     const float u = px / f->vx, v = py / f->vy, w = pz / f->vz;
     const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
     float c[8], e[8];
@@ -155,9 +211,13 @@ static float psi_point(const Fld* f, int a, int b, float px, float py, float pz)
     return trilerp(c, u - (float)i0, v - (float)j0, w - (float)k0);
 }
 
-/* v2m_psi_grad (gm == 0), src/opencl/v2m_seed_body.cl:211-247: psi and its analytic
- * gradient from one trilinear cell */
+/* v2m_psi_grad, src/opencl/v2m_seed_body.cl:211-247: psi and its analytic gradient
+ * from one trilinear cell (the gray-scale form for adjacent labels) */
 static float psi_grad_point(const Fld* f, int a, int b, float px, float py, float pz, float* g) {
+    float gv;
+    if (gray_psi(f, a, b, px, py, pz, &gv, g)) {
+        return gv;
+    }
     const float u = px / f->vx, v = py / f->vy, w = pz / f->vz;
     const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
     const float fx = u - (float)i0, fy = v - (float)j0, fz = w - (float)k0;
@@ -193,9 +253,24 @@ static int brick_labels_at(const Fld* f, float px, float py, float pz, int* lab)
     return n;
 }
 
-/* v2m_label_of (gm == 0, voxmode 0), src/opencl/v2m_seed_body.cl:116-175: argmax of
- * phi over the brick's labels, plus the runner-up and the margin */
+/* v2m_label_of (voxmode 0), src/opencl/v2m_seed_body.cl:116-175: argmax of phi over
+ * the brick's labels, plus the runner-up and the margin. A gray-scale volume labels p
+ * by the number of thresholds <= I(p); the runner-up lies across the nearer threshold
+ * in membership units (none above the last). */
 static int owner_at(const Fld* f, float px, float py, float pz, int* second, float* margin) {
+    if (f->gm > 0) {
+        const float I = gray_at(f, px, py, pz, NULL);
+        const int m = f->gm;
+        int l = 0;
+        while (l < m && I >= f->gTW[l]) {
+            ++l;
+        }
+        const float dl = l > 0 ? (I - f->gTW[l - 1]) / f->gTW[m + l - 1] : 1e30f;
+        const float du = l < m ? (f->gTW[l] - I) / f->gTW[m + l] : 1e30f;
+        *second = dl < du ? l - 1 : (l < m ? l + 1 : TN_NOLAB);
+        *margin = 2.0f * fminf(dl, du);
+        return l;
+    }
     int lab[TN_BL];
     const int n = brick_labels_at(f, px, py, pz, lab);
     int best = lab[0], sec = TN_NOLAB;
@@ -1101,8 +1176,11 @@ static int supported(const tn_grid_view* g, const tn_relax_params* prm) {
     if (!g || !prm) {
         return fail("oracle: null argument");
     }
-    if (g->gm != 0) {
-        return fail("oracle: only label volumes (gm == 0)");
+    if (g->gm < 0) {
+        return fail("oracle: shape fields (gm < 0) are not covered");
+    }
+    if (g->gm > 0 && (!g->gI || !g->gTW || g->ngTW < 2 * (int64_t)g->gm)) {
+        return fail("oracle: a gray-scale grid needs gI and 2 gm values in gTW");
     }
     return 0;
 }

@@ -46,6 +46,10 @@ def lib():
         L.ref_grid_build_prob.argtypes = [C.POINTER(C.c_uint16), C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int,
                                           C.c_int, C.POINTER(C.c_float), C.POINTER(RefGridParams),
                                           C.POINTER(C.c_float)]
+        L.ref_grid_build_gray.restype = vp
+        L.ref_grid_build_gray.argtypes = [C.POINTER(C.c_uint16), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int,
+                                          C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(RefGridParams),
+                                          C.POINTER(C.c_float)]
         L.ref_grid_from_view.restype = vp
         L.ref_grid_from_view.argtypes = [C.POINTER(GridView)]
         L.ref_grid_free.argtypes = [vp]
@@ -83,9 +87,15 @@ class RefGrid:
 
     @staticmethod
     def build(labels: np.ndarray, vs=(1.0, 1.0, 1.0), *, sigma=1.0, hbase=0.0, hmin=0.0, hmax=0.0, K_=3.0, g=0.3,
-              hvox: np.ndarray | None = None, prob: np.ndarray | None = None) -> "RefGrid":
+              hvox: np.ndarray | None = None, prob: np.ndarray | None = None,
+              thresholds=None) -> "RefGrid":
         """build_grid_cpu (src/v2m_grid.cpp:47) on a (nz, ny, nx) label array; with
-        prob (C, nz, ny, nx), a tissue-probability input (label l's field from prob[l])."""
+        prob (C, nz, ny, nx), a tissue-probability input (label l's field from prob[l]);
+        with thresholds, `labels` is a gray-scale intensity (float) labelled by the
+        number of thresholds <= it (apply_thresholds, src/v2m_volume.cpp:93-152)."""
+        if thresholds is not None:
+            return RefGrid._build_gray(labels, np.sort(np.asarray(thresholds, np.float32)), vs,
+                                       RefGridParams(sigma, hbase, hmin, hmax, K_, g), hvox)
         nz, ny, nx = labels.shape
         lab = np.ascontiguousarray(labels, np.uint16)
         vsa = np.array(vs, np.float32)
@@ -98,6 +108,18 @@ class RefGrid:
                                       vsa.ctypes.data_as(C.POINTER(C.c_float)), C.byref(gp),
                                       None if hv is None else hv.ctypes.data_as(C.POINTER(C.c_float)))
         return RefGrid(h, keep=pr)
+
+    @staticmethod
+    def _build_gray(gray, thr, vs, gp, hvox) -> "RefGrid":
+        gray = np.ascontiguousarray(gray, np.float32)
+        nz, ny, nx = gray.shape
+        lab = np.ascontiguousarray((gray[..., None] >= thr).sum(-1), np.uint16)
+        vsa = np.array(vs, np.float32)
+        hv = None if hvox is None else np.ascontiguousarray(hvox, np.float32)
+        fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        h = lib().ref_grid_build_gray(lab.ctypes.data_as(C.POINTER(C.c_uint16)), fp(gray), fp(thr), thr.size,
+                                      nx, ny, nz, fp(vsa), C.byref(gp), None if hv is None else fp(hv))
+        return RefGrid(h, keep=(gray, thr, lab))
 
     @staticmethod
     def from_grid(grid: Grid) -> "RefGrid":
@@ -124,10 +146,11 @@ class RefGrid:
         nb = v.nbx * v.nby * v.nbz
         cp = lambda ptr, cnt, dt: np.ctypeslib.as_array(ptr, shape=(cnt,)).astype(dt, copy=True)
         gr = cp(grade, nv, np.uint8) if grade else None
+        gray = (cp(v.gI, nv, np.float32), cp(v.gTW, 2 * v.gm, np.float32), int(v.gm)) if v.gm > 0 else (None, None, 0)
         return Grid(v.nx, v.ny, v.nz, tuple(v.vs), cp(v.L, nv, np.uint16), cp(v.bl_cnt, nb, np.int32),
                     cp(v.bl_lab, nb * 16, np.uint16), cp(v.bl_slot, nb, np.int32),
                     cp(v.phi, v.nslot * 512, np.float32) if v.nslot else np.zeros(0, np.float32),
-                    cp(v.h, nv, np.float32), float(v.hmin), float(v.hmax), gr)
+                    cp(v.h, nv, np.float32), float(v.hmin), float(v.hmax), gr, *gray)
 
     def seed(self, prm: RelaxParams) -> Nodes:
         """seed_cpu (src/v2m_particles.cpp:625)."""

@@ -140,6 +140,11 @@ const char* ref_last_error(void) {
 
 void* ref_grid_build_prob(const uint16_t* labels, const float* prob, int nprob, int nx, int ny, int nz,
                           const float* vs, const ref_grid_params* gp, const float* hvox);
+void* ref_grid_build_gray(const uint16_t* labels, const float* gray, const float* thr, int m, int nx, int ny, int nz,
+                          const float* vs, const ref_grid_params* gp, const float* hvox);
+void* ref_grid_build_gray_prob(const uint16_t* labels, const float* prob, int nprob, const float* gray,
+                               const float* thr, int m_thr, int nx, int ny, int nz, const float* vs,
+                               const ref_grid_params* gp, const float* hvox);
 
 // build_grid_cpu over a label volume (x fastest); hvox may be null
 void* ref_grid_build(const uint16_t* labels, int nx, int ny, int nz, const float* vs, const ref_grid_params* gp,
@@ -152,6 +157,21 @@ void* ref_grid_build(const uint16_t* labels, int nx, int ny, int nz, const float
 // smoothed probabilities (src/v2m_grid.cpp:154-168, 467)
 void* ref_grid_build_prob(const uint16_t* labels, const float* prob, int nprob, int nx, int ny, int nz,
                           const float* vs, const ref_grid_params* gp, const float* hvox) {
+    return ref_grid_build_gray_prob(labels, prob, nprob, nullptr, nullptr, 0, nx, ny, nz, vs, gp, hvox);
+}
+
+// the same over a gray-scale volume: gray[v] the intensity, thr the m ascending
+// thresholds, labels[v] the number of thresholds <= gray[v] (apply_thresholds with
+// no smoothing, src/v2m_volume.cpp:93-152); the fields are then the memberships of
+// the intensity (src/v2m_grid.cpp:374-433)
+void* ref_grid_build_gray(const uint16_t* labels, const float* gray, const float* thr, int m, int nx, int ny, int nz,
+                          const float* vs, const ref_grid_params* gp, const float* hvox) {
+    return ref_grid_build_gray_prob(labels, nullptr, 0, gray, thr, m, nx, ny, nz, vs, gp, hvox);
+}
+
+void* ref_grid_build_gray_prob(const uint16_t* labels, const float* prob, int nprob, const float* gray,
+                               const float* thr, int m_thr, int nx, int ny, int nz, const float* vs,
+                               const ref_grid_params* gp, const float* hvox) {
     try {
         RefGrid* r = new RefGrid;
         const size_t nv = static_cast<size_t>(nx) * ny * nz;
@@ -160,6 +180,11 @@ void* ref_grid_build_prob(const uint16_t* labels, const float* prob, int nprob, 
         if (prob && nprob > 0) {
             r->lv.prob.assign(prob, prob + nv * nprob);
             r->lv.nprob = nprob;
+        }
+
+        if (gray && thr && m_thr > 0) {
+            r->lv.gray.assign(gray, gray + nv);
+            r->lv.thresholds.assign(thr, thr + m_thr);
         }
 
         r->lv.nx = nx;
@@ -227,6 +252,16 @@ void* ref_grid_from_view(const tn_grid_view* v) {
     g.hmin = v->hmin;
     g.hmax = v->hmax;
     g.gm = 0;
+    g.gTW.assign(1, 0.0f);
+
+    if (v->gm > 0 && v->gI && v->gTW && v->ngTW >= 2 * static_cast<int64_t>(v->gm)) {
+        r->lv.gray.assign(v->gI, v->gI + nv);
+        g.gI = r->lv.gray.data();
+        g.gm = v->gm;
+        g.gTW.assign(v->gTW, v->gTW + 2 * v->gm);
+        g.gray_w.assign(v->gTW + v->gm, v->gTW + 2 * v->gm);
+    }
+
     return r;
 }
 
@@ -267,6 +302,12 @@ void ref_grid_view(void* h, tn_grid_view* v, const uint8_t** grade, float* hbase
     v->hmin = g.hmin;
     v->hmax = g.hmax;
     v->gm = g.gm;
+
+    if (g.gm > 0) {
+        v->gI = g.gI;
+        v->gTW = g.gTW.data();
+        v->ngTW = static_cast<int64_t>(g.gTW.size());
+    }
 
     if (grade) {
         *grade = g.grade.empty() ? nullptr : g.grade.data();

@@ -76,6 +76,11 @@ CASES = {
     "tpm32": dict(vol=lambda: volumes.tpm_softmax((32, 34, 30), 6, 3.0, 1)[1],
                   prob=lambda: volumes.tpm_softmax((32, 34, 30), 6, 3.0, 1)[0], grid=dict(hbase=3.0, hmin=2.0),
                   runs=[(1, 0.0, True), (30, 0.0, True), (200, 2e-3, True)]),
+    # a gray-scale input: three thresholds cut nested perturbed shells out of a smooth
+    # intensity; the fields are the intensity's memberships, psi of adjacent labels the
+    # intensity itself
+    "gray40": dict(vol=lambda: volumes.gray_ball(40), thr=volumes.GRAY_THRESHOLDS, grid=dict(hbase=2.5),
+                   runs=[(1, 0.0, True), (30, 0.0, True), (200, 2e-3, True), (30, 0.0, False), (40, 0.0, False, True)]),
     # anisotropic voxels (0.8 x 0.9 x 1.3 mm): every coordinate -> voxel division is inexact
     "aniso40": dict(vol=lambda: volumes.sectors(40), vs=(0.8, 0.9, 1.3), grid=dict(hbase=2.5),
                     runs=[(1, 0.0, True), (20, 0.0, True), (150, 2e-3, True), (30, 0.0, False), (40, 0.0, False, True)]),
@@ -100,7 +105,8 @@ def make(name: str, spec: dict) -> str:
         lo, hi, slope = spec["hvox"]
         hvox = graded_size(lab, lo, hi, slope).ravel()
     prob = spec["prob"]() if "prob" in spec else None
-    rg = ref.RefGrid.build(lab, vs=spec.get("vs", (1.0, 1.0, 1.0)), hvox=hvox, prob=prob, **spec["grid"])
+    rg = ref.RefGrid.build(lab, vs=spec.get("vs", (1.0, 1.0, 1.0)), hvox=hvox, prob=prob, thresholds=spec.get("thr"),
+                           **spec["grid"])
     G = rg.grid()
     prm = RelaxParams()
     hcp = rg.seed(prm)

@@ -249,11 +249,25 @@ def _dense_phi(grid, l):
     return np.pad(out, 1, constant_values=np.float32(1.0 if l == 0 else 0.0))
 
 
+def _gray_psi_field(g, a, b):
+    """sg (I - t_k) on every voxel, padded by one voxel of t_0 - W_0 (the intensity
+    outside the grid), for adjacent labels a, b of a gray-scale grid: k = min(a, b),
+    sg = +-2 / W_k, positive when a is the upper label."""
+    m = g.gm
+    I = g.gI.reshape(g.nz, g.ny, g.nx).transpose(2, 1, 0).astype(np.float64)
+    I = np.pad(I, 1, constant_values=float(np.float32(g.gTW[0] - g.gTW[m])))
+    k = min(a, b)
+    sg = (2.0 if a > b else -2.0) / float(g.gTW[m + k])
+    return sg * (I - float(g.gTW[k])), I
+
+
 def test_field_queries_are_trilinear(golden):
     """phi_a and psi_ab = phi_a - phi_b are trilinear in the voxel values
     (src/opencl/v2m_grid_body.cl:137-202, v2m_seed_body.cl:179-253), their
     gradient is the interpolant's, and the owner is the argmax of phi over the
-    brick's labels (v2m_seed_body.cl:116-177)."""
+    brick's labels (v2m_seed_body.cl:116-177). On a gray-scale grid, psi of
+    adjacent labels is +-2 (I - t_k) / W_k of the trilinear intensity and the owner
+    the number of thresholds <= I (v2m_seed_body.cl:65-131)."""
     d = golden.d
     g = golden.grid
     a, b = (int(x) for x in d["fq_ab"])
@@ -261,10 +275,12 @@ def test_field_queries_are_trilinear(golden):
     vs = np.asarray(g.vs, np.float64)
     u = (X / vs).T + 1.0
     pa, pb = _dense_phi(g, a), _dense_phi(g, b)
+    gray = g.gm > 0 and abs(a - b) == 1
+    pd, Ipad = _gray_psi_field(g, a, b) if gray else ((pa - pb).astype(np.float64), None)
     phi = ndimage.map_coordinates(pa.astype(np.float64), u, order=1, mode="nearest")
-    psi = ndimage.map_coordinates((pa - pb).astype(np.float64), u, order=1, mode="nearest")
+    psi = ndimage.map_coordinates(pd, u, order=1, mode="nearest")
     assert np.abs(phi - d["fq_phi"]).max() < 3e-6
-    assert np.abs(psi - d["fq_psi"]).max() < 6e-6
+    assert np.abs(psi - d["fq_psi"]).max() < 6e-6 * max(1.0, np.abs(psi).max())
     # the interpolant's gradient: central differences that stay inside the cell
     frac = u - np.floor(u)
     inside = np.all((frac > 1e-3) & (frac < 1 - 1e-3), axis=0)
@@ -273,12 +289,27 @@ def test_field_queries_are_trilinear(golden):
     for ax in range(3):
         du = np.zeros((3, 1))
         du[ax] = eps
-        G[ax] = (ndimage.map_coordinates((pa - pb).astype(np.float64), u + du, order=1, mode="nearest")
-                 - ndimage.map_coordinates((pa - pb).astype(np.float64), u - du, order=1, mode="nearest")) / (2 * eps)
+        G[ax] = (ndimage.map_coordinates(pd, u + du, order=1, mode="nearest")
+                 - ndimage.map_coordinates(pd, u - du, order=1, mode="nearest")) / (2 * eps)
         G[ax] /= vs[ax]
     got = d["fq_grad"].astype(np.float64).T
     assert inside.sum() > 0.5 * X.shape[0]
     assert np.abs(G[:, inside] - got[:, inside]).max() < 1e-4 * max(1.0, np.abs(got).max())
+    if g.gm > 0:
+        # owner: the number of thresholds <= I, the runner-up across the nearer one
+        m = g.gm
+        t, w = g.gTW[:m].astype(np.float64), g.gTW[m:2 * m].astype(np.float64)
+        I = ndimage.map_coordinates(Ipad if Ipad is not None else _gray_psi_field(g, 1, 0)[1], u, order=1,
+                                    mode="nearest")
+        lab = (I[:, None] >= t[None, :]).sum(1)
+        dl = np.where(lab > 0, (I - t[np.maximum(lab - 1, 0)]) / w[np.maximum(lab - 1, 0)], 1e30)
+        du_ = np.where(lab < m, (t[np.minimum(lab, m - 1)] - I) / w[np.minimum(lab, m - 1)], 1e30)
+        sec = np.where(dl < du_, lab - 1, np.where(lab < m, lab + 1, 0xFFFF))
+        clear = (np.abs(I[:, None] - t[None, :]).min(1) > 1e-5) & (np.abs(dl - du_) > 1e-5)
+        assert clear.sum() > 0.9 * X.shape[0]
+        assert np.array_equal(d["fq_label"][clear], lab[clear])
+        assert np.array_equal(d["fq_second"][clear], sec[clear])
+        return
     # owner: the argmax over the brick's labels where the margin is clear
     labels = np.unique(g.L)
     vals = np.stack([ndimage.map_coordinates(_dense_phi(g, int(l)).astype(np.float64), u, order=1, mode="nearest")

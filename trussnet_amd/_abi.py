@@ -121,6 +121,8 @@ class Grid:
     """The field the nodes move in (v2mesh Grid, src/v2m_grid.h:45-81), x fastest.
 
     ``L`` and ``h`` are flat arrays of nx*ny*nz; ``phi`` holds nslot*512 values.
+    Gray-scale input (gm = m > 0): ``gI`` is the intensity per voxel and ``gTW`` the m
+    ascending thresholds followed by their m membership widths (Grid::gI, gray_w).
     """
     nx: int
     ny: int
@@ -135,6 +137,9 @@ class Grid:
     hmin: float
     hmax: float
     grade: np.ndarray | None = None
+    gI: np.ndarray | None = None
+    gTW: np.ndarray | None = None
+    gm: int = 0
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -163,7 +168,9 @@ class Grid:
                     np.ascontiguousarray(self.phi, np.float32).ravel(),
                     np.ascontiguousarray(self.h, np.float32).ravel(),
                     float(self.hmin), float(self.hmax),
-                    None if self.grade is None else np.ascontiguousarray(self.grade, np.uint8).ravel())
+                    None if self.grade is None else np.ascontiguousarray(self.grade, np.uint8).ravel(),
+                    None if self.gI is None else np.ascontiguousarray(self.gI, np.float32).ravel(),
+                    None if self.gTW is None else np.ascontiguousarray(self.gTW, np.float32).ravel(), int(self.gm))
 
     def check(self) -> None:
         nv = self.nx * self.ny * self.nz
@@ -190,10 +197,10 @@ class Grid:
         v.nslot = g.nslot
         v.h = _ptr(g.h, C.c_float)
         v.hmin, v.hmax = g.hmin, g.hmax
-        v.gI = _ptr(None, C.c_float)
-        v.gTW = _ptr(None, C.c_float)
-        v.ngTW = 0
-        v.gm = 0
+        v.gI = _ptr(g.gI, C.c_float)
+        v.gTW = _ptr(g.gTW, C.c_float)
+        v.ngTW = 0 if g.gTW is None else g.gTW.size
+        v.gm = int(g.gm)
         if g is not self:
             self._keep.append(g)
         return v
@@ -201,12 +208,15 @@ class Grid:
     def _is_normal(self) -> bool:
         want = [(self.L, np.uint16), (self.bl_cnt, np.int32), (self.bl_lab, np.uint16), (self.bl_slot, np.int32),
                 (self.phi, np.float32), (self.h, np.float32)]
+        want += [(a, np.float32) for a in (self.gI, self.gTW) if a is not None]
         return all(a.dtype == t and a.flags["C_CONTIGUOUS"] for a, t in want)
 
     def arrays(self) -> dict:
         d = {"dims": np.array([self.nx, self.ny, self.nz], np.int32), "vs": np.array(self.vs, np.float32),
              "hminmax": np.array([self.hmin, self.hmax], np.float32), "L": self.L, "bl_cnt": self.bl_cnt,
              "bl_lab": self.bl_lab, "bl_slot": self.bl_slot, "phi": self.phi, "h": self.h}
+        if self.gm:
+            d.update(gI=self.gI, gTW=self.gTW, gm=np.int32(self.gm))
         return d
 
     @staticmethod
@@ -215,8 +225,10 @@ class Grid:
         nx, ny, nz = (int(x) for x in g("dims"))
         hmin, hmax = (float(x) for x in g("hminmax"))
         grade = d[prefix + "grade"] if (prefix + "grade") in d else None
+        gray = (prefix + "gm") in d and int(d[prefix + "gm"]) != 0
         return Grid(nx, ny, nz, tuple(float(x) for x in g("vs")), g("L"), g("bl_cnt"), g("bl_lab"), g("bl_slot"),
-                    g("phi"), g("h"), hmin, hmax, grade).normalized()
+                    g("phi"), g("h"), hmin, hmax, grade, g("gI") if gray else None, g("gTW") if gray else None,
+                    int(d[prefix + "gm"]) if gray else 0).normalized()
 
 
 @dataclass

@@ -47,6 +47,9 @@ struct Field {
     const int32_t* __restrict__ bl_slot;
     const float* __restrict__ phi;
     const float* __restrict__ h;
+    const float* __restrict__ gI;    // gray-scale mode: the intensity per voxel
+    const float* __restrict__ gTW;   // gray-scale mode: [t_0..t_{m-1}, W_0..W_{m-1}]
+    int gm;                          // gray-scale mode: m > 0 thresholds (0: a label volume)
     // this struct's copy in device memory. Out-of-line functions take it instead of a
     // reference: a reference to the kernel argument would move the whole struct into
     // scratch memory and turn the kernel's loads through its pointers into flat loads.
@@ -284,14 +287,6 @@ __device__ __forceinline__ float phi_point(const Field& f, int l, P3 p) {
     return trilerp(c, u - (float)i0, v - (float)j0, w - (float)k0);
 }
 
-// v2m_psi (gm == 0), src/opencl/v2m_seed_body.cl:179-205, through a cell cache
-__device__ __forceinline__ float psi_at(const Field& f, int a, int b, P3 p, PsiCell& pc) {
-    const float u = over_vx(f, p.x), v = over_vy(f, p.y), w = over_vz(f, p.z);
-    const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
-    psi_cell_load(f, a, b, i0, j0, k0, pc);
-    return trilerp(pc.d, u - (float)i0, v - (float)j0, w - (float)k0);
-}
-
 // the trilinear value and gradient of v2m_psi_grad (src/opencl/v2m_seed_body.cl:
 // 233-246) from the corner differences c: (gx, gy, gz, psi)
 __device__ __forceinline__ float4 trilerp_grad(const Field& f, const float* c, float fx, float fy, float fz) {
@@ -308,9 +303,82 @@ __device__ __forceinline__ float4 trilerp_grad(const Field& f, const float* c, f
     return g;
 }
 
-// v2m_psi_grad (gm == 0), src/opencl/v2m_seed_body.cl:211-247: (gx, gy, gz, psi),
-// through a cell cache
+// v2m_gray_at, src/opencl/v2m_seed_body.cl:65-93: the trilinear intensity at p,
+// (gx, gy, gz, I); a corner outside the grid reads t_0 - W_0. The gradient costs a
+// few subtractions more than the value, so both are always formed.
+__device__ __forceinline__ float4 gray_at(const Field& f, P3 p) {
+    const float u = over_vx(f, p.x), v = over_vy(f, p.y), w = over_vz(f, p.z);
+    const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
+    const float fx = u - (float)i0, fy = v - (float)j0, fz = w - (float)k0;
+    const float out = f.gTW[0] - f.gTW[f.gm];
+    float c[8];
+    if (i0 >= 0 && j0 >= 0 && k0 >= 0 && i0 + 1 < f.nx && j0 + 1 < f.ny && k0 + 1 < f.nz) {
+        const float* b0 = f.gI + vox(f, i0, j0, k0);
+        const size_t sy = (size_t)f.nx, sz = (size_t)f.nx * (size_t)f.ny;
+        c[0] = b0[0];
+        c[1] = b0[1];
+        c[2] = b0[sy];
+        c[3] = b0[sy + 1];
+        c[4] = b0[sz];
+        c[5] = b0[sz + 1];
+        c[6] = b0[sz + sy];
+        c[7] = b0[sz + sy + 1];
+    } else {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int i = i0 + (m & 1), j = j0 + ((m >> 1) & 1), k = k0 + (m >> 2);
+            c[m] = (i < 0 || j < 0 || k < 0 || i >= f.nx || j >= f.ny || k >= f.nz) ? out : f.gI[vox(f, i, j, k)];
+        }
+    }
+    return trilerp_grad(f, c, fx, fy, fz);
+}
+
+// v2m_gray_psi, src/opencl/v2m_seed_body.cl:97-113: for labels a, b of a gray-scale
+// volume with |a - b| = 1, psi_ab = +-2 (I - t_k) / W_k (k = min(a, b)) and its
+// gradient, (gx, gy, gz, psi). A pair above the last threshold (labels > m, which a
+// gray-scale volume does not hold) takes the phi form, so t_k and W_k stay in gTW.
+__device__ __forceinline__ bool gray_pair(const Field& f, int a, int b) {
+    return f.gm > 0 && (a - b == 1 || b - a == 1) && (a < b ? a : b) < f.gm;
+}
+
+__device__ __forceinline__ float4 gray_psi(const Field& f, int a, int b, float4 ig) {
+    const int k = a < b ? a : b;
+    const float sg = (a > b ? 2.0f : -2.0f) / f.gTW[f.gm + k];
+    return make_float4(ig.x * sg, ig.y * sg, ig.z * sg, sg * (ig.w - f.gTW[k]));
+}
+
+// v2m_label_of in gray-scale mode (voxmode 0), src/opencl/v2m_seed_body.cl:119-131:
+// (the number of thresholds <= I, the runner-up across the nearer threshold in
+// membership units)
+__device__ __forceinline__ int2 gray_owner(const Field& f, float I) {
+    const int m = f.gm;
+    int l = 0;
+    while (l < m && I >= f.gTW[l]) {
+        ++l;
+    }
+    const float dl = l > 0 ? (I - f.gTW[l - 1]) / f.gTW[m + l - 1] : 1e30f;
+    const float du = l < m ? (f.gTW[l] - I) / f.gTW[m + l] : 1e30f;
+    return make_int2(l, dl < du ? l - 1 : (l < m ? l + 1 : kNoLab));
+}
+
+// v2m_psi, src/opencl/v2m_seed_body.cl:179-205, through a cell cache (the gray-scale
+// form for adjacent labels)
+__device__ __forceinline__ float psi_at(const Field& f, int a, int b, P3 p, PsiCell& pc) {
+    if (gray_pair(f, a, b)) {
+        return gray_psi(f, a, b, gray_at(f, p)).w;
+    }
+    const float u = over_vx(f, p.x), v = over_vy(f, p.y), w = over_vz(f, p.z);
+    const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
+    psi_cell_load(f, a, b, i0, j0, k0, pc);
+    return trilerp(pc.d, u - (float)i0, v - (float)j0, w - (float)k0);
+}
+
+// v2m_psi_grad, src/opencl/v2m_seed_body.cl:211-247: (gx, gy, gz, psi), through a
+// cell cache (the gray-scale form for adjacent labels)
 __device__ __forceinline__ float4 psi_grad_at(const Field& f, int a, int b, P3 p, PsiCell& pc) {
+    if (gray_pair(f, a, b)) {
+        return gray_psi(f, a, b, gray_at(f, p));
+    }
     const float u = over_vx(f, p.x), v = over_vy(f, p.y), w = over_vz(f, p.z);
     const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
     const float fx = u - (float)i0, fy = v - (float)j0, fz = w - (float)k0;
@@ -405,8 +473,11 @@ __device__ __forceinline__ float label_phi(const Field& f, const PointCell& pc, 
     return trilerp(c, pc.fx, pc.fy, pc.fz);
 }
 
-// v2m_label_of (gm == 0, voxmode 0), src/opencl/v2m_seed_body.cl:116-175: (best, second)
+// v2m_label_of (voxmode 0), src/opencl/v2m_seed_body.cl:116-175: (best, second)
 __device__ __forceinline__ int2 owner_at(const Field& f, P3 p) {
+    if (f.gm > 0) {
+        return gray_owner(f, gray_at(f, p).w);
+    }
     const BrickLabels bl = brick_labels_at(f, p);
     int best = brick_label(f, bl, 0), sec = kNoLab;
     float pb = -1.0f, ps = pb;
@@ -439,8 +510,16 @@ struct OwnerPsi {
 };
 
 __device__ __forceinline__ OwnerPsi owner_psi_at(const Field& f, int a, P3 p) {
-    const BrickLabels bl = brick_labels_at(f, p);
     OwnerPsi r;
+    if (f.gm > 0) {   // the runner-up is adjacent: psi from the same intensity
+        const float4 ig = gray_at(f, p);
+        const int2 o = gray_owner(f, ig.w);
+        r.best = o.x;
+        r.sec = o.y;
+        r.g = (o.x == a && o.y != kNoLab) ? gray_psi(f, a, o.y, ig) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return r;
+    }
+    const BrickLabels bl = brick_labels_at(f, p);
     r.best = brick_label(f, bl, 0);
     r.sec = kNoLab;
     r.g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);

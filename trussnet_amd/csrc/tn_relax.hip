@@ -223,6 +223,9 @@ tnd::Field upload_grid(const tn_grid_view* g, DevMem& mem) {
     f.bl_slot = mem.upload(g->bl_slot, nb);
     f.phi = mem.upload(g->phi, (size_t)g->nslot * TN_SLOT);
     f.h = mem.upload(g->h, nv);
+    f.gm = g->gm > 0 ? g->gm : 0;
+    f.gI = f.gm ? mem.upload(g->gI, nv) : nullptr;
+    f.gTW = f.gm ? mem.upload(g->gTW, 2 * (size_t)f.gm) : nullptr;
     tnd::Field* self = mem.alloc<tnd::Field>(1);
     f.self = self;
     TN_HIP(hipMemcpy(self, &f, sizeof f, hipMemcpyHostToDevice));
@@ -714,9 +717,11 @@ int pick_device(int32_t device) {
 
 void check_grid(const tn_grid_view* g) {
     arg_check(g != nullptr, "null grid");
-    if (g->gm != 0) {
-        throw TnError(TN_ERR_UNSUPPORTED, "gray-scale and shape fields (gm != 0) are not run on the device");
+    if (g->gm < 0) {
+        throw TnError(TN_ERR_UNSUPPORTED, "shape fields (gm < 0) are not run on the device");
     }
+    arg_check(g->gm == 0 || (g->gI && g->gTW && g->ngTW >= 2 * (int64_t)g->gm),
+              "grid: a gray-scale grid (gm > 0) needs gI and 2 gm values in gTW");
     arg_check(g->nx > 0 && g->ny > 0 && g->nz > 0, "grid: empty");
     arg_check(g->nbx == (g->nx + 7) / 8 && g->nby == (g->ny + 7) / 8 && g->nbz == (g->nz + 7) / 8,
               "grid: brick counts must be ceil(n / 8) per axis");

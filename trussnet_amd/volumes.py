@@ -143,3 +143,27 @@ def tpm_softmax(shape=(182, 218, 182), channels: int = 6, sigma: float = 6.0, se
     np.exp(f, out=f)
     f /= f.sum(axis=0, keepdims=True)
     return f.astype(np.float32), np.argmax(f, axis=0).astype(np.uint16)
+
+
+GRAY_THRESHOLDS = (0.2, 0.5, 0.75)
+
+
+def gray_ball(n: int = 40, radius: float | None = None, seed: int = 2, amp: float = 0.06,
+              sigma: float = 3.0) -> np.ndarray:
+    """A gray-scale intensity (float32, (n, n, n)): 1 - r / radius about the volume
+    centre plus `amp` times a Gaussian random field (sigma voxels), so that the
+    thresholds GRAY_THRESHOLDS cut three nested, perturbed shells (labels 1..3, the
+    number of thresholds <= the intensity)."""
+    radius = 0.45 * n if radius is None else radius
+    c = (n - 1) / 2.0
+    z, y, x = np.meshgrid(_centres(n), _centres(n), _centres(n), indexing="ij")
+    r = np.sqrt((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2)
+    g = gaussian_field((n, n, n), sigma, np.random.default_rng(seed))
+    return (1.0 - r / radius + amp * g).astype(np.float32)
+
+
+def gray_labels(gray: np.ndarray, thresholds) -> np.ndarray:
+    """The label of each voxel: the number of thresholds <= its intensity
+    (apply_thresholds without smoothing, src/v2m_volume.cpp:140-149)."""
+    t = np.sort(np.asarray(thresholds, np.float32))
+    return (np.asarray(gray, np.float32)[..., None] >= t).sum(-1).astype(np.uint16)
