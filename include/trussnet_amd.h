@@ -164,6 +164,64 @@ int tn_seed(const tn_grid_view* g, const uint8_t* grade, const tn_relax_params* 
             float* P, uint16_t* lab, uint8_t* typ, uint16_t* part, uint16_t* part3);
 
 /*
+ * The grid build on the device: build_grid_cpu (src/v2m_grid.cpp:47-722, the
+ * Grid of src/v2m_grid.h:45-81 from a LabelVolume and GridParams) for a label
+ * volume, a tissue-probability input (prob) or a gray-scale input (gray and its
+ * ascending thresholds; L then holds the number of thresholds <= each intensity,
+ * apply_thresholds). It makes the labels near each 8^3 brick, the slots of the
+ * mixed bricks, the interface fields (smoothed indicators or probabilities and
+ * their thickness-adaptive blend, or the gray-scale memberships), the sizing field
+ * (curvature, per-label and interface sizes, thin layers, the user's field), its
+ * gradient limit and the grades, each array identical to the reference's. Shape
+ * inputs (analytic fields) are not built here. The grid stays on the device until
+ * tn_grid_download; tn_grid_get_info gives the sizes (nslot, the phi slots) and
+ * the scalars of the Grid. Arrays of tn_grid_download may be null to skip them:
+ * bl_cnt[nb], bl_lab[16 nb], bl_slot[nb], phi[512 nslot], h[nx*ny*nz],
+ * grade[nx*ny*nz], gTW[2 gm] (gray-scale: thresholds, then widths).
+ */
+typedef struct tn_grid_params {   /* GridParams, src/v2m_grid.h:21-43 */
+    float sigma, sigma_thin, thin_lo, thin_hi, thick, thin_floor, preserve, sigma_curv;
+    float hbase, hmin, hmax, K, g;
+    const float* hlab;      /* per-label size (mm), index = label, 0 = hbase */
+    int32_t nhlab;
+    float iglob;            /* interface sizes (InterfaceSizes, src/v2m_isize.h:25-28): every interface */
+    const float* ilab;      /* per label */
+    int32_t nilab;
+    const float* ipair;     /* (a, b, h) triples, a < b */
+    int32_t npair;
+} tn_grid_params;
+
+typedef struct tn_volume_view {   /* LabelVolume, src/v2m_volume.h */
+    int32_t nx, ny, nz;
+    float vs[3];                  /* voxel size (mm) */
+    const uint16_t* L;            /* [nx*ny*nz] labels, x fastest */
+    const float* prob;            /* optional [nprob][nx*ny*nz]: label l's probability (label 0 the exterior) */
+    int32_t nprob;
+    const float* gray;            /* optional [nx*ny*nz] gray-scale intensity */
+    const float* thresholds;      /* [nthr] ascending */
+    int32_t nthr;
+    const float* hvox;            /* optional [nx*ny*nz] user sizing field (mm; > 0 replaces the automatic size) */
+} tn_volume_view;
+
+typedef struct tn_grid_info {
+    int32_t nx, ny, nz, nbx, nby, nbz;
+    int64_t nslot;
+    float hbase, hmin, hmax, hcurv;
+    int32_t R;                    /* the brick label radius */
+    int32_t limit_sweeps;         /* gradient-limiting sweeps run */
+    int32_t overflow_bricks;      /* bricks with more than 16 labels nearby */
+    int32_t gm;                   /* gray-scale thresholds (0: none) */
+} tn_grid_info;
+
+typedef struct tn_grid tn_grid;
+void tn_grid_params_default(tn_grid_params* p);
+int tn_grid_build(const tn_volume_view* vol, const tn_grid_params* prm, int32_t device, tn_grid** out);
+int tn_grid_get_info(const tn_grid* g, tn_grid_info* info);
+int tn_grid_download(const tn_grid* g, int32_t* bl_cnt, uint16_t* bl_lab, int32_t* bl_slot, float* phi, float* h,
+                     uint8_t* grade, float* gTW);
+int tn_grid_destroy(tn_grid* g);
+
+/*
  * A relaxation plan: the grid and the seed nodes resident on one device, with
  * every buffer the loop needs allocated once. tn_plan_run restores the seeds on
  * the device and relaxes them, so repeated runs do the same work with no host

@@ -50,6 +50,9 @@ def lib():
         L.ref_grid_build_gray.argtypes = [C.POINTER(C.c_uint16), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int,
                                           C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(RefGridParams),
                                           C.POINTER(C.c_float)]
+        L.ref_grid_build_full.restype = vp
+        L.ref_grid_build_full.argtypes = [vp, vp]
+        L.ref_grid_scalars.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_float)]
         L.ref_grid_from_view.restype = vp
         L.ref_grid_from_view.argtypes = [C.POINTER(GridView)]
         L.ref_grid_free.argtypes = [vp]
@@ -120,6 +123,27 @@ class RefGrid:
         h = lib().ref_grid_build_gray(lab.ctypes.data_as(C.POINTER(C.c_uint16)), fp(gray), fp(thr), thr.size,
                                       nx, ny, nz, fp(vsa), C.byref(gp), None if hv is None else fp(hv))
         return RefGrid(h, keep=(gray, thr, lab))
+
+    @staticmethod
+    def build_full(labels, vs=(1.0, 1.0, 1.0), prm=None, *, prob=None, gray=None, thresholds=None,
+                   hvox=None) -> "RefGrid":
+        """build_grid_cpu with every GridParams field (trussnet_amd._abi.GridParams),
+        from the same volume view the device build takes."""
+        from trussnet_amd._abi import GridParams
+        from trussnet_amd.gridbuild import volume_view
+
+        pc, pkeep = (prm or GridParams()).to_c()
+        v, vkeep = volume_view(labels, vs, prob, gray, thresholds, hvox)
+        h = lib().ref_grid_build_full(C.addressof(v), C.addressof(pc))
+        return RefGrid(h, keep=(v, vkeep, pc, pkeep))
+
+    def scalars(self) -> dict:
+        """R, limit_sweeps, overflow_bricks, nslot, hbase, hmin, hmax, hcurv of the Grid."""
+        info = (C.c_int64 * 4)()
+        sc = (C.c_float * 4)()
+        lib().ref_grid_scalars(self.h, info, sc)
+        return {"R": info[0], "limit_sweeps": info[1], "overflow_bricks": info[2], "nslot": info[3],
+                "hbase": sc[0], "hmin": sc[1], "hmax": sc[2], "hcurv": sc[3]}
 
     @staticmethod
     def from_grid(grid: Grid) -> "RefGrid":

@@ -222,6 +222,93 @@ void* ref_grid_build_gray_prob(const uint16_t* labels, const float* prob, int np
     }
 }
 
+// build_grid_cpu with every GridParams field, from the device build's own views
+// (a label volume, a probability input or a gray-scale input)
+void* ref_grid_build_full(const tn_volume_view* v, const tn_grid_params* gp) {
+    try {
+        RefGrid* r = new RefGrid;
+        const size_t nv = static_cast<size_t>(v->nx) * v->ny * v->nz;
+        r->lv.data.assign(v->L, v->L + nv);
+        r->lv.nx = v->nx;
+        r->lv.ny = v->ny;
+        r->lv.nz = v->nz;
+
+        for (int a = 0; a < 3; ++a) {
+            r->lv.voxelsize[a] = v->vs[a];
+        }
+
+        if (v->prob && v->nprob > 0) {
+            r->lv.prob.assign(v->prob, v->prob + nv * v->nprob);
+            r->lv.nprob = v->nprob;
+        }
+
+        if (v->gray && v->thresholds && v->nthr > 0) {
+            r->lv.gray.assign(v->gray, v->gray + nv);
+            r->lv.thresholds.assign(v->thresholds, v->thresholds + v->nthr);
+        }
+
+        int m = 0;
+
+        for (uint16_t l : r->lv.data) {
+            m = std::max<int>(m, l);
+        }
+
+        r->lv.maxlabel = m;
+        tn::GridParams p;
+        p.sigma = gp->sigma;
+        p.sigma_thin = gp->sigma_thin;
+        p.thin_lo = gp->thin_lo;
+        p.thin_hi = gp->thin_hi;
+        p.thick = gp->thick;
+        p.thin_floor = gp->thin_floor;
+        p.preserve = gp->preserve;
+        p.sigma_curv = gp->sigma_curv;
+        p.hbase = gp->hbase;
+        p.hmin = gp->hmin;
+        p.hmax = gp->hmax;
+        p.K = gp->K;
+        p.g = gp->g;
+
+        if (gp->nhlab > 0) {
+            p.hlab.assign(gp->hlab, gp->hlab + gp->nhlab);
+        }
+
+        p.isize.glob = gp->iglob;
+
+        if (gp->nilab > 0) {
+            p.isize.lab.assign(gp->ilab, gp->ilab + gp->nilab);
+        }
+
+        if (gp->npair > 0) {
+            p.isize.pair.assign(gp->ipair, gp->ipair + 3 * static_cast<size_t>(gp->npair));
+        }
+
+        if (v->hvox) {
+            p.hvox.assign(v->hvox, v->hvox + nv);
+        }
+
+        tn::build_grid_cpu(r->lv, p, r->g);
+        return r;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+// the Grid's scalars: info[0..3] = R, limit_sweeps, overflow_bricks, nslot;
+// sc[0..3] = hbase, hmin, hmax, hcurv
+void ref_grid_scalars(void* h, int64_t* info, float* sc) {
+    const tn::Grid& g = static_cast<RefGrid*>(h)->g;
+    info[0] = g.R;
+    info[1] = g.limit_sweeps;
+    info[2] = g.overflow_bricks;
+    info[3] = static_cast<int64_t>(g.phi.size() / V2M_SLOT);
+    sc[0] = g.hbase;
+    sc[1] = g.hmin;
+    sc[2] = g.hmax;
+    sc[3] = g.hcurv;
+}
+
 // a grid from raw arrays (a fixture), for relaxing it with the reference
 void* ref_grid_from_view(const tn_grid_view* v) {
     RefGrid* r = new RefGrid;

@@ -52,7 +52,9 @@ def test_struct_layouts_match_header(tmp_path):
 
     structs = {"tn_grid_view": _abi.GridView, "tn_relax_params": _abi.RelaxParamsC, "tn_nodes_view": _abi.NodesView,
                "tn_relax_stats": _abi.RelaxStatsC, "tn_hash_info": _abi.HashInfoC,
-               "tn_kernel_time": _abi.KernelTimeC, "tn_adj_problem": AdjProblemC}
+               "tn_kernel_time": _abi.KernelTimeC, "tn_adj_problem": AdjProblemC,
+               "tn_grid_params": _abi.GridParamsC, "tn_volume_view": _abi.VolumeViewC,
+               "tn_grid_info": _abi.GridInfoC}
     src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for cname, cls in structs.items():
         src.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
@@ -79,6 +81,45 @@ def test_default_params_match_reference(tnlib):
     got, want = tnlib.default_params(), RelaxParams()
     for f in fields(RelaxParams):
         assert np.float32(getattr(got, f.name)) == np.float32(getattr(want, f.name)), f.name
+
+
+def test_default_grid_params_match_reference(tnlib):
+    """tn_grid_params_default is GridParams' defaults (src/v2m_grid.h:21-43)."""
+    import ctypes as C
+
+    from trussnet_amd import gridbuild
+    from trussnet_amd._abi import GridParams, GridParamsC
+
+    L = gridbuild._bind()
+    got = GridParamsC()
+    L.tn_grid_params_default(C.cast(C.byref(got), C.c_void_p))
+    want, _ = GridParams().to_c()
+    for name, _t in GridParamsC._fields_:
+        if name in ("hlab", "ilab", "ipair"):
+            assert not getattr(got, name), name
+        else:
+            assert getattr(got, name) == getattr(want, name), name
+
+
+def test_grid_build_rejects_bad_volumes(tnlib):
+    """Malformed volumes fail on the host before any device work."""
+    from trussnet_amd import gridbuild, volumes
+    from trussnet_amd._abi import GridParams
+
+    lab = volumes.two_spheres(16, 6.0, 3.0)
+    gray = np.zeros(lab.shape, np.float32)
+    with pytest.raises(tnlib.DeviceError):   # descending thresholds
+        gridbuild.build(lab, gray=gray, thresholds=None, prm=GridParams(sigma_curv=0.0))
+    v, keep = gridbuild.volume_view(lab, gray=gray, thresholds=[0.2, 0.5])
+    v.thresholds[0], v.thresholds[1] = 0.5, 0.2
+    import ctypes as C
+
+    pc, pk = GridParams().to_c()
+    h = C.c_void_p()
+    rc = gridbuild._bind().tn_grid_build(C.byref(v), C.cast(C.byref(pc), C.c_void_p), -1, C.byref(h))
+    from trussnet_amd._abi import TN_ERR_ARG
+
+    assert rc == TN_ERR_ARG
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is visible here")

@@ -86,6 +86,64 @@ def _ptr(a: np.ndarray | None, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
 
 
+class GridParamsC(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("sigma", "sigma_thin", "thin_lo", "thin_hi", "thick", "thin_floor", "preserve",
+                                          "sigma_curv", "hbase", "hmin", "hmax", "K", "g")] + [
+        ("hlab", _f32p), ("nhlab", C.c_int32), ("iglob", C.c_float), ("ilab", _f32p), ("nilab", C.c_int32),
+        ("ipair", _f32p), ("npair", C.c_int32)]
+
+
+class VolumeViewC(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("vs", C.c_float * 3), ("L", _u16p),
+                ("prob", _f32p), ("nprob", C.c_int32), ("gray", _f32p), ("thresholds", _f32p), ("nthr", C.c_int32),
+                ("hvox", _f32p)]
+
+
+class GridInfoC(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("nbx", C.c_int32), ("nby", C.c_int32),
+                ("nbz", C.c_int32), ("nslot", C.c_int64), ("hbase", C.c_float), ("hmin", C.c_float),
+                ("hmax", C.c_float), ("hcurv", C.c_float), ("R", C.c_int32), ("limit_sweeps", C.c_int32),
+                ("overflow_bricks", C.c_int32), ("gm", C.c_int32)]
+
+
+@dataclass
+class GridParams:
+    """GridParams (src/v2m_grid.h:21-43): smoothing, thin-layer blend, sizing and
+    gradient limit of the grid build; hlab per-label sizes, iglob / ilab / ipair the
+    interface sizes (InterfaceSizes, src/v2m_isize.h:25-28, ipair as (a, b, h) rows)."""
+    sigma: float = 1.0
+    sigma_thin: float = 0.35
+    thin_lo: float = 2.0
+    thin_hi: float = 4.0
+    thick: float = 0.0
+    thin_floor: float = 0.5
+    preserve: float = 0.0
+    sigma_curv: float = 2.0
+    hbase: float = 0.0
+    hmin: float = 0.0
+    hmax: float = 0.0
+    K: float = 3.0
+    g: float = 0.3
+    hlab: tuple = ()
+    iglob: float = 0.0
+    ilab: tuple = ()
+    ipair: tuple = ()
+
+    def to_c(self) -> tuple[GridParamsC, list]:
+        """The C struct and the arrays it points into (keep them alive with it)."""
+        c = GridParamsC()
+        for n in ("sigma", "sigma_thin", "thin_lo", "thin_hi", "thick", "thin_floor", "preserve", "sigma_curv",
+                  "hbase", "hmin", "hmax", "K", "g", "iglob"):
+            setattr(c, n, float(getattr(self, n)))
+        keep = []
+        for n, cnt in (("hlab", "nhlab"), ("ilab", "nilab"), ("ipair", "npair")):
+            a = np.ascontiguousarray(np.asarray(getattr(self, n), np.float32).ravel())
+            keep.append(a)
+            setattr(c, n, a.ctypes.data_as(_f32p) if a.size else None)
+            setattr(c, cnt, a.size // 3 if n == "ipair" else a.size)
+        return c, keep
+
+
 @dataclass
 class RelaxParams:
     """RelaxParams with the reference defaults (src/v2m_particles.h:22-39)."""

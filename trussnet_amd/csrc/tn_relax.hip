@@ -1589,3 +1589,5 @@ int tn_relax_batch(const tn_grid_view* grids, const tn_relax_params* prms, tn_no
 }  // extern "C"
 
 #include "tn_adjoint.hpp"
+
+#include "tn_grid.hpp"

@@ -29,7 +29,8 @@ SYMBOLS = ["tn_relax_params_default", "tn_last_error", "tn_device_count", "tn_re
            "tn_plan_create", "tn_plan_run", "tn_plan_download", "tn_plan_destroy", "tn_plan_hash_info",
            "tn_plan_reset", "tn_plan_rebuild", "tn_plan_force", "tn_plan_step", "tn_plan_set_timing",
            "tn_plan_timings", "tn_adj_create", "tn_adj_forward", "tn_adj_backward", "tn_adj_loss_terms",
-           "tn_adj_destroy"]
+           "tn_adj_destroy", "tn_grid_params_default", "tn_grid_build", "tn_grid_get_info", "tn_grid_download",
+           "tn_grid_destroy"]
 
 
 class DeviceError(RuntimeError):
@@ -82,7 +83,7 @@ def lib():
         L.tn_seed.argtypes = [C.POINTER(GridView), vp, C.POINTER(RelaxParamsC), C.c_int32, C.c_int64, vp, vp, vp,
                               vp, vp]
         for name in SYMBOLS:
-            if name != "tn_last_error" and name != "tn_relax_params_default":
+            if name not in ("tn_last_error", "tn_relax_params_default", "tn_grid_params_default"):
                 getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
