@@ -27,7 +27,7 @@ cpu_baseline: the reference's own relax_cpu (oracle/_ref, compiled from its
 sources) on this host's cores, on the same grid and seeds, for a bounded
 number of iterations.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|small|batch]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|small|prob|batch|adjoint|grid]
 """
 from __future__ import annotations
 
@@ -341,6 +341,61 @@ def bench_adjoint(args) -> int:
     return 0
 
 
+def bench_grid(args) -> int:
+    """The grid build on the device (tn_grid_build) of the large workload's 256^3
+    label volume with the reference's default GridParams (hbase 4.5, hmin 1.5, hmax
+    6 voxels): bricks, slots, curvature and interface fields with the thin-layer
+    blend, sizing, gradient limit, grades. A step is one build from the host labels
+    (their 32 MB upload included; the grid stays on the device). Against the
+    reference's build_grid_cpu on this host's cores."""
+    from trussnet_amd import gridbuild, volumes
+    from trussnet_amd._abi import GridParams
+
+    lab = volumes.nested_ellipsoids(256, nlab=6, seed=0, axes=(20.0, 120.0), grf_sigma=4.0, grf_amp=3.0)
+    prm = GridParams(hbase=4.5, hmin=1.5, hmax=6.0)
+    nv = lab.size
+    info = {}
+    for _ in range(args.warmup):
+        gridbuild.build(lab, prm=prm, device=0, info=info)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        G = gridbuild.build_resident(lab, prm=prm, device=0)
+        G.close()
+    dt = time.perf_counter() - t
+    value = nv * args.steps / dt
+    log(f"grid build: {1e3 * dt / args.steps:.1f} ms per build, {info.get('nslot')} slots, "
+        f"{info.get('limit_sweeps')} limit sweeps")
+    cpu = None
+    if not args.no_cpu:
+        try:
+            from oracle import ref
+
+            if not ref.available():
+                raise RuntimeError("reference library not built")
+            t = time.perf_counter()
+            r = ref.RefGrid.build_full(lab, prm=prm)
+            dtc = time.perf_counter() - t
+            sc = r.scalars()
+            cores = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 64))
+            cpu = {"value": nv / dtc, "unit": "voxels/s", "cores": cores, "kind": "reference",
+                   "sample": f"build_grid_cpu on the same volume and parameters, one build, {dtc:.2f} s "
+                             f"({sc['nslot']} slots, {sc['limit_sweeps']} limit sweeps)"}
+        except Exception as e:
+            log(f"reference CPU path unavailable ({e})")
+    line = {
+        "metric": "voxels/s of the grid build (build_grid_cpu on the device)", "value": value, "unit": "voxels/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plain_large's volume: 256^3 six nested perturbed ellipsoid labels (seed 0), "
+                               "GridParams hbase 4.5, hmin 1.5, hmax 6 voxels, defaults otherwise; label upload included",
+                   "voxels": nv, "slots": info.get("nslot"), "limit_sweeps": info.get("limit_sweeps"),
+                   "parallelism": "single GPU"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def cpu_baseline_many(grids, seeds, iters: int) -> dict | None:
     from trussnet_amd._abi import RelaxParams
 
@@ -370,7 +425,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["large", "small", "prob", "batch", "adjoint"], default="large")
+    ap.add_argument("--workload", choices=["large", "small", "prob", "batch", "adjoint", "grid"], default="large")
     ap.add_argument("--iters", type=int, default=0, help="iterations per step (default: the workload's)")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
@@ -386,6 +441,8 @@ def main() -> int:
         return bench_batch(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "grid":
+        return bench_grid(args)
     pmc = None if args.no_traffic else pmc_traffic(args)   # before this process initialises the GPU
 
     from trussnet_amd import relax

@@ -437,51 +437,9 @@ __global__ void k_size(Field f, SizeArgs z, float* __restrict__ h) {
 }
 
 // ---- thickness (v2m_thick_voxel, :241-276) ----
-// Pass 1: m = the largest phi of the voxel's own label over its label's voxels
-// within +-r; where the thickness is finite, (voxel, m, (1 - m)(1 + m)) is listed
-// for the logarithm. Elsewhere tvox = 1e30.
-__global__ void k_thick_m(Field f, int r, float* __restrict__ tvox, int32_t* __restrict__ count,
-                          int32_t* __restrict__ idx, float* __restrict__ mval, float* __restrict__ larg) {
-    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nv = (int64_t)f.nx * f.ny * f.nz;
-    bool want = false;
-    float m = 0.0f;
-    if (v < nv) {
-        const int i = (int)(v % f.nx), j = (int)((v / f.nx) % f.ny), k = (int)(v / ((int64_t)f.nx * f.ny));
-        if (f.bl_slot[tnd::brick_at(f, i, j, k)] >= 0) {
-            const int a = f.L[v];
-            for (int dz = -r; dz <= r; ++dz) {
-                for (int dy = -r; dy <= r; ++dy) {
-                    for (int dx = -r; dx <= r; ++dx) {
-                        if (label_at(f, i + dx, j + dy, k + dz) != a) {
-                            continue;
-                        }
-                        const float q = tnd::phi_voxel(f, a, i + dx, j + dy, k + dz);
-                        m = q > m ? q : m;
-                    }
-                }
-            }
-            want = !(m >= 0.9999f);
-        }
-        tvox[v] = 1e30f;
-    }
-    const uint64_t bal = __ballot(want);
-    if (bal) {
-        const int lead = __ffsll((long long)bal) - 1;
-        int base = 0;
-        if ((int)(threadIdx.x & 63) == lead) {
-            base = atomicAdd(count, __popcll(bal));
-        }
-        base = __shfl(base, lead, 64);
-        if (want) {
-            const int at = base + __popcll(bal & tnk::lanemask_lt());
-            idx[at] = (int32_t)v;
-            mval[at] = m;
-            larg[at] = (1.0f - m) * (1.0f + m);
-        }
-    }
-}
-
+// Pass 1 (k_own_phi, k_thick_window below): m = the largest phi of the voxel's own
+// label over its label's voxels within +-2; where the thickness is finite, (voxel,
+// m, (1 - m)(1 + m)) is listed for the logarithm. Elsewhere tvox = 1e30.
 // Pass 2: erfinv (M. Giles, "Approximating the erfinv function", GPU Computing Gems
 // Jade, 2011; single-precision coefficients) from the host's logarithm, and
 // t = 2 sqrt(2) sigma erfinv(m)
@@ -612,30 +570,172 @@ __global__ void k_preserve(Field f, float mg, float* __restrict__ phi) {
     }
 }
 
-// ---- 5. gradient limiting: one Jacobi sweep (v2m_limit_voxel, :477-496) ----
-__global__ void k_limit(Field f, float gl, const float* __restrict__ h, float* __restrict__ hn,
-                        int32_t* __restrict__ changed) {
+// ---- 5. gradient limiting: Jacobi sweeps (v2m_limit_voxel, :477-496) ----
+// h <= h(n) + g |x - x_n| over the 26 neighbours, every voxel from the previous
+// sweep's values, until no value drops.
+// One sweep over listed bricks, one workgroup of 512 per brick: the brick and a
+// one-voxel halo in LDS (+inf outside the grid, which no minimum takes), so each
+// voxel reads its 26 neighbours from LDS; the neighbour distances g |dx| are formed
+// once per workgroup. A minimum is exact in any order, so the result is the
+// reference's sweep. A brick is listed while a brick within one of it changed in the
+// sweep before (the others cannot change), which keeps the reference's count.
+__global__ __launch_bounds__(512) void k_limit_bricks(Field f, float gl, const int32_t* __restrict__ list,
+                                                      const float* __restrict__ h, float* __restrict__ hn,
+                                                      int32_t* __restrict__ bchanged) {
+    __shared__ float tile[10 * 10 * 10];
+    __shared__ float dist[27];
+    const int b = list[blockIdx.x], tid = threadIdx.x;
+    const int bx = b % f.nbx, by = (b / f.nbx) % f.nby, bz = b / (f.nbx * f.nby);
+    const int x0 = bx * kBS - 1, y0 = by * kBS - 1, z0 = bz * kBS - 1;
+    for (int t = tid; t < 1000; t += 512) {
+        const int x = x0 + t % 10, y = y0 + (t / 10) % 10, z = z0 + t / 100;
+        tile[t] = (x < 0 || y < 0 || z < 0 || x >= f.nx || y >= f.ny || z >= f.nz) ? INFINITY
+                                                                                    : h[tnd::vox(f, x, y, z)];
+    }
+    if (tid < 27) {
+        const float ex = (float)(tid % 3 - 1) * f.vx, ey = (float)((tid / 3) % 3 - 1) * f.vy,
+                    ez = (float)(tid / 9 - 1) * f.vz;
+        dist[tid] = gl * sqrtf(ex * ex + ey * ey + ez * ez);
+    }
+    __syncthreads();
+    const int lx = tid % kBS, ly = (tid / kBS) % kBS, lz = tid / (kBS * kBS);
+    const int i = bx * kBS + lx, j = by * kBS + ly, k = bz * kBS + lz;
+    if (i >= f.nx || j >= f.ny || k >= f.nz) {
+        return;
+    }
+    const float* c = tile + (lx + 1) + 10 * ((ly + 1) + 10 * (lz + 1));
+    const float h0 = c[0];
+    float m = h0;
+#pragma unroll
+    for (int n = 0; n < 27; ++n) {
+        if (n != 13) {
+            m = fminf(m, c[(n % 3 - 1) + 10 * ((n / 3) % 3 - 1) + 100 * (n / 9 - 1)] + dist[n]);
+        }
+    }
+    const size_t v = tnd::vox(f, i, j, k);
+    hn[v] = m;
+    if (m < h0) {
+        bchanged[b] = 1;
+    }
+}
+
+// h = hn on the listed bricks
+__global__ __launch_bounds__(512) void k_commit_bricks(Field f, const int32_t* __restrict__ list,
+                                                       const float* __restrict__ hn, float* __restrict__ h) {
+    const int b = list[blockIdx.x], tid = threadIdx.x;
+    const int i = (b % f.nbx) * kBS + tid % kBS, j = ((b / f.nbx) % f.nby) * kBS + (tid / kBS) % kBS,
+              k = (b / (f.nbx * f.nby)) * kBS + tid / (kBS * kBS);
+    if (i < f.nx && j < f.ny && k < f.nz) {
+        const size_t v = tnd::vox(f, i, j, k);
+        h[v] = hn[v];
+    }
+}
+
+// the bricks of the next sweep: those with a changed brick within one; the flags
+// are read from bchanged and cleared in bnext's twin by the caller
+__global__ void k_next_bricks(Field f, const int32_t* __restrict__ bchanged, int32_t* __restrict__ count,
+                              int32_t* __restrict__ list) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nb = f.nbx * f.nby * f.nbz;
+    bool on = false;
+    if (b < nb) {
+        const int bx = b % f.nbx, by = (b / f.nbx) % f.nby, bz = b / (f.nbx * f.nby);
+        for (int dz = -1; dz <= 1 && !on; ++dz) {
+            for (int dy = -1; dy <= 1 && !on; ++dy) {
+                for (int dx = -1; dx <= 1 && !on; ++dx) {
+                    const int x = bx + dx, y = by + dy, z = bz + dz;
+                    on = x >= 0 && y >= 0 && z >= 0 && x < f.nbx && y < f.nby && z < f.nbz &&
+                         bchanged[x + f.nbx * (y + f.nby * z)] != 0;
+                }
+            }
+        }
+    }
+    const uint64_t bal = __ballot(on);
+    if (bal) {
+        const int lead = __ffsll((long long)bal) - 1;
+        int base = 0;
+        if ((int)(threadIdx.x & 63) == lead) {
+            base = atomicAdd(count, __popcll(bal));
+        }
+        base = __shfl(base, lead, 64);
+        if (on) {
+            list[base + __popcll(bal & tnk::lanemask_lt())] = b;
+        }
+    }
+}
+
+// Only voxels of label a enter voxel v's maximum, each with phi_a at itself: so m
+// is the largest "own phi" (phi of a voxel's own label at the voxel, 1 for the
+// exterior outside the grid) among the voxels of label a within +-2. k_own_phi forms
+// the own phi per voxel; k_thick_window takes the window maximum from an LDS tile of the
+// brick and a two-voxel halo.
+__global__ void k_own_phi(Field f, float* __restrict__ own) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t nv = (int64_t)f.nx * f.ny * f.nz;
-    bool ch = false;
     if (v < nv) {
         const int i = (int)(v % f.nx), j = (int)((v / f.nx) % f.ny), k = (int)(v / ((int64_t)f.nx * f.ny));
-        const float h0 = h[v];
-        float m = h0;
-        for (int n = 0; n < 27; ++n) {
-            const int dx = n % 3 - 1, dy = (n / 3) % 3 - 1, dz = n / 9 - 1;
-            const int x = i + dx, y = j + dy, z = k + dz;
-            if (n == 13 || x < 0 || y < 0 || z < 0 || x >= f.nx || y >= f.ny || z >= f.nz) {
-                continue;
-            }
-            const float ex = (float)dx * f.vx, ey = (float)dy * f.vy, ez = (float)dz * f.vz;
-            m = fminf(m, h[tnd::vox(f, x, y, z)] + gl * sqrtf(ex * ex + ey * ey + ez * ez));
-        }
-        hn[v] = m;
-        ch = m < h0;
+        own[v] = tnd::phi_voxel(f, f.L[v], i, j, k);
     }
-    if (__ballot(ch) && (threadIdx.x & 63) == 0) {
-        atomicOr(changed, 1);
+}
+
+__global__ __launch_bounds__(512) void k_thick_window(Field f, const float* __restrict__ own, float* __restrict__ tvox,
+                                                      int32_t* __restrict__ count, int32_t* __restrict__ idx,
+                                                      float* __restrict__ mval, float* __restrict__ larg) {
+    constexpr int W = kBS + 4;
+    __shared__ float tp[W * W * W];
+    __shared__ uint16_t tl[W * W * W];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int bx = b % f.nbx, by = (b / f.nbx) % f.nby, bz = b / (f.nbx * f.nby);
+    const int lx = tid % kBS, ly = (tid / kBS) % kBS, lz = tid / (kBS * kBS);
+    const int i = bx * kBS + lx, j = by * kBS + ly, k = bz * kBS + lz;
+    const bool in = i < f.nx && j < f.ny && k < f.nz;
+    if (f.bl_slot[b] < 0) {   // a uniform brick: no finite thickness
+        if (in) {
+            tvox[tnd::vox(f, i, j, k)] = 1e30f;
+        }
+        return;
+    }
+    const int x0 = bx * kBS - 2, y0 = by * kBS - 2, z0 = bz * kBS - 2;
+    for (int t = tid; t < W * W * W; t += 512) {
+        const int x = x0 + t % W, y = y0 + (t / W) % W, z = z0 + t / (W * W);
+        const bool ok = x >= 0 && y >= 0 && z >= 0 && x < f.nx && y < f.ny && z < f.nz;
+        const size_t v = ok ? tnd::vox(f, x, y, z) : 0;
+        tl[t] = ok ? f.L[v] : (uint16_t)0;
+        tp[t] = ok ? own[v] : 1.0f;
+    }
+    __syncthreads();
+    bool want = false;
+    float m = 0.0f;
+    if (in) {
+        const int c = (lx + 2) + W * ((ly + 2) + W * (lz + 2));
+        const int a = tl[c];
+        for (int dz = -2; dz <= 2; ++dz) {
+            for (int dy = -2; dy <= 2; ++dy) {
+                for (int dx = -2; dx <= 2; ++dx) {
+                    const int q = c + dx + W * (dy + W * dz);
+                    if (tl[q] == a) {
+                        m = tp[q] > m ? tp[q] : m;
+                    }
+                }
+            }
+        }
+        want = !(m >= 0.9999f);
+        tvox[tnd::vox(f, i, j, k)] = 1e30f;
+    }
+    const uint64_t bal = __ballot(want);
+    if (bal) {
+        const int lead = __ffsll((long long)bal) - 1;
+        int base = 0;
+        if ((int)(threadIdx.x & 63) == lead) {
+            base = atomicAdd(count, __popcll(bal));
+        }
+        base = __shfl(base, lead, 64);
+        if (want) {
+            const int at = base + __popcll(bal & tnk::lanemask_lt());
+            idx[at] = (int32_t)tnd::vox(f, i, j, k);
+            mval[at] = m;
+            larg[at] = (1.0f - m) * (1.0f + m);
+        }
     }
 }
 
@@ -674,6 +774,23 @@ __global__ void k_grade(int64_t nv, const float* __restrict__ h, float hmin, flo
         if (open) {
             doubt[base + __popcll(bal & tnk::lanemask_lt())] = (int32_t)v;
         }
+    }
+}
+
+// the sizes of the listed voxels, and their grades put back
+__global__ void k_gather_f32(int32_t n, const int32_t* __restrict__ idx, const float* __restrict__ a,
+                             float* __restrict__ out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) {
+        out[q] = a[idx[q]];
+    }
+}
+
+__global__ void k_scatter_u8(int32_t n, const int32_t* __restrict__ idx, const uint8_t* __restrict__ val,
+                             uint8_t* __restrict__ a) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) {
+        a[idx[q]] = val[q];
     }
 }
 
@@ -796,6 +913,17 @@ int tn_grid_build(const tn_volume_view* vol, const tn_grid_params* prm, int32_t 
         DevMem tmp;   // scratch freed at the end of the build
         tmp.device = dev;
         SeedOps ops{ tmp };
+        // TN_GRID_VERBOSE: device time per stage (events on the null stream)
+        const bool verbose = std::getenv("TN_GRID_VERBOSE") != nullptr;
+        std::vector<std::pair<const char*, hipEvent_t>> marks;
+        auto mark = [&](const char* name) {
+            if (verbose) {
+                hipEvent_t e;
+                TN_HIP(hipEventCreate(&e));
+                TN_HIP(hipEventRecord(e, nullptr));
+                marks.emplace_back(name, e);
+            }
+        };
         const int nx = vol->nx, ny = vol->ny, nz = vol->nz;
         const int64_t nv = (int64_t)nx * ny * nz;
         tn_grid_info& I = G->info;
@@ -863,12 +991,14 @@ int tn_grid_build(const tn_volume_view* vol, const tn_grid_params* prm, int32_t 
         f.vy = vol->vs[1];
         f.vz = vol->vs[2];
         f.L = mem.upload(vol->L, (size_t)nv);
+        mark("upload");
         // 1. bricks
         int32_t* bl_cnt = mem.alloc<int32_t>(nb);
         uint16_t* bl_lab = mem.alloc<uint16_t>((size_t)nb * TN_BL);
         int32_t* ovf = ops.counts(0);
         tng::k_brick_labels<<<nb, 256>>>(f, I.R, bl_cnt, bl_lab, ovf);
         TN_HIP(hipGetLastError());
+        mark("bricks");
         // 2. slots
         int32_t* offs = ops.counts(nb);
         tng::k_slot_counts<<<blocks_for(nb, 256), 256>>>(bl_cnt, nb, offs);
@@ -899,8 +1029,10 @@ int tn_grid_build(const tn_volume_view* vol, const tn_grid_params* prm, int32_t 
                                                                         slot_label, dst);
             TN_HIP(hipGetLastError());
         };
+        mark("slots");
         // 3. the curvature fields, from the labels
         smooth(labels_src, prm->sigma_curv, Rc, phi);
+        mark("smooth_curv");
         // 4. sizing
         float* h = mem.alloc<float>((size_t)nv);
         tng::SizeArgs za;
@@ -920,6 +1052,7 @@ int tn_grid_build(const tn_volume_view* vol, const tn_grid_params* prm, int32_t 
         za.hvox = d_hvox;
         tng::k_size<<<nbv, 256>>>(f, za, h);
         TN_HIP(hipGetLastError());
+        mark("size");
         // thickness from the curvature fields (src/v2m_grid.cpp:292-305)
         float* tvox = nullptr;
         if (prm->thick > 0.0f || prm->sigma_thin > 0.0f) {
@@ -928,7 +1061,9 @@ int tn_grid_build(const tn_volume_view* vol, const tn_grid_params* prm, int32_t 
             int32_t* idx = tmp.alloc<int32_t>((size_t)nv);
             float* mval = tmp.alloc<float>((size_t)nv);
             float* larg = tmp.alloc<float>((size_t)nv);
-            tng::k_thick_m<<<nbv, 256>>>(f, 2, tvox, cnt, idx, mval, larg);
+            float* own = tmp.alloc<float>((size_t)nv);
+            tng::k_own_phi<<<nbv, 256>>>(f, own);
+            tng::k_thick_window<<<nb, 512>>>(f, own, tvox, cnt, idx, mval, larg);
             TN_HIP(hipGetLastError());
             int32_t n = 0;
             TN_HIP(hipMemcpy(&n, cnt, sizeof n, hipMemcpyDeviceToHost));
@@ -946,6 +1081,7 @@ int tn_grid_build(const tn_volume_view* vol, const tn_grid_params* prm, int32_t 
             tng::k_thin_size<<<nbv, 256>>>(nv, tvox, vmin, prm->thick, floor_mm, h);
             hmin = std::min(hmin, floor_mm);
         }
+        mark("thickness");
         // the interface fields (src/v2m_grid.cpp:343-468)
         const bool tpm_raw = tpm && prm->sigma <= 0.0f;
         if (gray) {
@@ -987,6 +1123,7 @@ int tn_grid_build(const tn_volume_view* vol, const tn_grid_params* prm, int32_t 
             smooth(src, prm->sigma, Ri, phi);
         }
         TN_HIP(hipGetLastError());
+        mark("interface");
         if (!gray && !tpm_raw && prm->sigma_thin > 0.0f) {   // the thickness-adaptive blend
             float* w = tmp.alloc<float>((size_t)nv);
             float* t2 = tmp.alloc<float>((size_t)nv);
@@ -1016,28 +1153,31 @@ int tn_grid_build(const tn_volume_view* vol, const tn_grid_params* prm, int32_t 
         if (!gray && !tpm && prm->preserve > 0.0f) {
             tng::k_preserve<<<nbv, 256>>>(f, prm->preserve, phi);
         }
+        mark("blend");
         // 5. gradient limiting: Jacobi sweeps until nothing changes
-        float* const h_kept = h;   // in the handle's memory
+        // (over the bricks that can still change: all at first, then those within
+        // one brick of a change; the commit keeps h whole for the next sweep's reads)
         float* hn = tmp.alloc<float>((size_t)nv);
-        int32_t* changed = ops.counts(0);
+        int32_t* bchanged = tmp.alloc<int32_t>(nb);
+        int32_t* list = tmp.alloc<int32_t>(nb);
+        int32_t* list2 = tmp.alloc<int32_t>(nb);
+        int32_t* cnt = ops.counts(0);
+        k_iota<<<blocks_for(nb, 256), 256>>>(list, nb);
+        int32_t nlist = nb;
         I.limit_sweeps = 0;
-        for (;;) {
-            TN_HIP(hipMemset(changed, 0, sizeof(int32_t)));
-            tng::k_limit<<<nbv, 256>>>(f, prm->g, h, hn, changed);
-            TN_HIP(hipGetLastError());
+        while (nlist > 0) {
+            TN_HIP(hipMemset(bchanged, 0, sizeof(int32_t) * nb));
+            tng::k_limit_bricks<<<nlist, 512>>>(f, prm->g, list, h, hn, bchanged);
+            tng::k_commit_bricks<<<nlist, 512>>>(f, list, hn, h);
             ++I.limit_sweeps;
-            std::swap(h, hn);
-            int32_t c = 0;
-            TN_HIP(hipMemcpy(&c, changed, sizeof c, hipMemcpyDeviceToHost));
-            if (!c) {
-                break;
-            }
-        }
-        if (h != h_kept) {   // the last sweep wrote the scratch buffer
-            TN_HIP(hipMemcpy(h_kept, h, sizeof(float) * nv, hipMemcpyDeviceToDevice));
-            h = h_kept;
+            TN_HIP(hipMemset(cnt, 0, sizeof(int32_t)));
+            tng::k_next_bricks<<<blocks_for(nb, 256), 256>>>(f, bchanged, cnt, list2);
+            TN_HIP(hipGetLastError());
+            TN_HIP(hipMemcpy(&nlist, cnt, sizeof nlist, hipMemcpyDeviceToHost));
+            std::swap(list, list2);
         }
         f.h = h;
+        mark("limit");
         // 6. grades
         G->grade = mem.alloc<uint8_t>((size_t)nv);
         if (hmax <= hmin * 1.0001f) {
@@ -1051,19 +1191,36 @@ int tn_grid_build(const tn_volume_view* vol, const tn_grid_params* prm, int32_t 
             int32_t n = 0;
             TN_HIP(hipMemcpy(&n, cnt, sizeof n, hipMemcpyDeviceToHost));
             if (n > 0) {   // the host's logf for the voxels whose bin was in doubt
-                std::vector<int32_t> vi((size_t)n);
-                TN_HIP(hipMemcpy(vi.data(), doubt, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-                for (int32_t v : vi) {
-                    float hv;
-                    TN_HIP(hipMemcpy(&hv, h + v, sizeof hv, hipMemcpyDeviceToHost));
-                    float t = std::log(hv / hmin) / lden;
+                float* hv = tmp.alloc<float>((size_t)n);
+                uint8_t* gv = tmp.alloc<uint8_t>((size_t)n);
+                tng::k_gather_f32<<<blocks_for(n, 256), 256>>>(n, doubt, h, hv);
+                std::vector<float> hh((size_t)n);
+                TN_HIP(hipMemcpy(hh.data(), hv, sizeof(float) * n, hipMemcpyDeviceToHost));
+                std::vector<uint8_t> gg((size_t)n);
+                for (int32_t q = 0; q < n; ++q) {
+                    float t = std::log(hh[q] / hmin) / lden;
                     t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
-                    const uint8_t gb = (uint8_t)(t * 255.0f + 0.5f);
-                    TN_HIP(hipMemcpy(G->grade + v, &gb, 1, hipMemcpyHostToDevice));
+                    gg[q] = (uint8_t)(t * 255.0f + 0.5f);
                 }
+                TN_HIP(hipMemcpy(gv, gg.data(), (size_t)n, hipMemcpyHostToDevice));
+                tng::k_scatter_u8<<<blocks_for(n, 256), 256>>>(n, doubt, gv, G->grade);
+                TN_HIP(hipGetLastError());
             }
         }
+        mark("grade");
         TN_HIP(hipDeviceSynchronize());
+        if (verbose && !marks.empty()) {
+            std::fprintf(stderr, "[tn_grid_build] %d x %d x %d, %d slots, %d sweeps:", nx, ny, nz, nslot, I.limit_sweeps);
+            for (size_t k = 1; k < marks.size(); ++k) {
+                float ms = 0.0f;
+                TN_HIP(hipEventElapsedTime(&ms, marks[k - 1].second, marks[k].second));
+                std::fprintf(stderr, " %s %.2f", marks[k].first, ms);
+            }
+            std::fprintf(stderr, " (ms)\n");
+            for (auto& m : marks) {
+                (void)hipEventDestroy(m.second);
+            }
+        }
         I.hbase = hbase;
         I.hmin = hmin;
         I.hmax = hmax;

@@ -93,3 +93,37 @@ def build(labels: np.ndarray, vs=(1.0, 1.0, 1.0), prm: GridParams | None = None,
     gI = np.ascontiguousarray(gray, np.float32).ravel() if gi.gm else None
     return Grid(gi.nx, gi.ny, gi.nz, tuple(float(x) for x in vs), lab, bl_cnt, bl_lab, bl_slot, phi, hh,
                 float(gi.hmin), float(gi.hmax), grade, gI, gTW if gi.gm else None, int(gi.gm))
+
+
+class Resident:
+    """A grid built on the device and kept there (a tn_grid handle)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self.h = handle
+
+    def info(self) -> dict:
+        gi = GridInfoC()
+        _check(_bind().tn_grid_get_info(self.h, C.byref(gi)))
+        return {k: getattr(gi, k) for k, _ in GridInfoC._fields_}
+
+    def close(self):
+        if self.h:
+            _bind().tn_grid_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def build_resident(labels: np.ndarray, vs=(1.0, 1.0, 1.0), prm: GridParams | None = None, *, prob=None, gray=None,
+                   thresholds=None, hvox=None, device: int = -1) -> Resident:
+    """Build the grid on the device and keep it there (no download)."""
+    L = _bind()
+    pc, pkeep = (prm or GridParams()).to_c()
+    v, vkeep = volume_view(labels, vs, prob, gray, thresholds, hvox)
+    h = C.c_void_p()
+    _check(L.tn_grid_build(C.byref(v), C.cast(C.byref(pc), C.c_void_p), device, C.byref(h)))
+    return Resident(h)
