@@ -6,9 +6,9 @@
  * Each function names the reference lines it restates. The arithmetic keeps
  * the reference's association and operand order exactly (the results must be
  * bit-identical), compiled with -ffp-contract=off like the reference
- * (CMakeLists.txt:69-72). Label volumes (gm == 0) and gray-scale volumes
- * (gm = m > 0 thresholds) are covered, the voxel trap included; shape fields
- * (gm < 0) are refused.
+ * (CMakeLists.txt:69-72). Label volumes (gm == 0), gray-scale volumes
+ * (gm = m > 0 thresholds) and shape fields (gm < 0, the labels' analytic-field
+ * program) are covered, the voxel trap included.
  */
 #include "oracle.h"
 
@@ -139,9 +139,315 @@ static inline float trilerp(const float* c, float fx, float fy, float fz) {
     return y0 + fz * (y1 - y0);
 }
 
-/* v2m_phi_at, src/opencl/v2m_grid_body.cl:189-200 (v2m_phi_f with gm == 0,
- * src/opencl/v2m_seed_body.cl:33-39) */
+/* ---- shape fields (gm < 0): the labels' analytic-field program ------------------ */
+/* The program layout and op codes are v2mesh's (src/opencl/v2m_sdf_body.cl:1-45):
+ * [0] labels N, [1] field width w, [2..4] origin, [5..5+N-1] code offsets, [5+N]
+ * crease blend radius, [6+N] gap-closing radius, [7+N] cull margin, [8+N] brick
+ * table offset (0: none); code: an op, its operands; op 0 ends a label's code. */
+enum { SDF_END = 0, SDF_PRIM = 1, SDF_MAX = 2, SDF_MIN = 3, SDF_NEG = 4, SDF_CONST = 5, SDF_BBOX = 6,
+       SDF_MCONST = 7, SDF_ADD = 8, SDF_SCALE = 9, SDF_CMAX = 10, SDF_CMIN = 11 };
+enum { SDF_SPHERE = 1, SDF_BOX = 2, SDF_CYL = 3, SDF_PLANE = 4, SDF_SLAB = 5, SDF_TORUS = 6, SDF_ELLIPSOID = 7,
+       SDF_CONE = 8, SDF_INFCYL = 9 };
+#define SDF_STACK 12
+
+typedef struct {
+    float v, x, y, z;   /* a value (> 0 inside) and its gradient */
+} SdfV;
+
+/* parameters per primitive type (v2m_sdf_nparam, :47-74) */
+static int sdf_nparam(int type) {
+    switch (type) {
+    case SDF_SPHERE: return 4;
+    case SDF_BOX: case SDF_PLANE: return 6;
+    case SDF_CYL: case SDF_INFCYL: return 7;
+    case SDF_SLAB: case SDF_TORUS: case SDF_CONE: return 8;
+    case SDF_ELLIPSOID: return 15;
+    default: return 0;
+    }
+}
+
+/* one primitive's signed distance at (x, y, z), > 0 inside (v2m_sdf_prim, :76-161) */
+static float sdf_prim(int type, const float* q, float x, float y, float z) {
+    if (type == SDF_SPHERE) {
+        const float dx = x - q[0], dy = y - q[1], dz = z - q[2];
+        return q[3] - sqrtf(dx * dx + dy * dy + dz * dz);
+    }
+    if (type == SDF_BOX) {
+        const float cx = 0.5f * (q[0] + q[3]), cy = 0.5f * (q[1] + q[4]), cz = 0.5f * (q[2] + q[5]);
+        const float ux = fabsf(x - cx) - 0.5f * fabsf(q[3] - q[0]), uy = fabsf(y - cy) - 0.5f * fabsf(q[4] - q[1]),
+                    uz = fabsf(z - cz) - 0.5f * fabsf(q[5] - q[2]);
+        const float ox = fmaxf(ux, 0.0f), oy = fmaxf(uy, 0.0f), oz = fmaxf(uz, 0.0f);
+        const float out = sqrtf(ox * ox + oy * oy + oz * oz), in = fminf(fmaxf(ux, fmaxf(uy, uz)), 0.0f);
+        return -(out + in);
+    }
+    if (type == SDF_PLANE) {
+        return -((x - q[0]) * q[3] + (y - q[1]) * q[4] + (z - q[2]) * q[5]);
+    }
+    if (type == SDF_SLAB) {
+        const float t = (x - q[0]) * q[3] + (y - q[1]) * q[4] + (z - q[2]) * q[5];
+        return fminf(t - q[6], q[7] - t);
+    }
+    if (type == SDF_INFCYL) {
+        const float dx = x - q[0], dy = y - q[1], dz = z - q[2];
+        const float t = dx * q[3] + dy * q[4] + dz * q[5];
+        const float rx = dx - t * q[3], ry = dy - t * q[4], rz = dz - t * q[5];
+        return q[6] - sqrtf(rx * rx + ry * ry + rz * rz);
+    }
+    if (type == SDF_TORUS) {
+        const float dx = x - q[0], dy = y - q[1], dz = z - q[2];
+        const float hh = dx * q[3] + dy * q[4] + dz * q[5];
+        const float rx = dx - hh * q[3], ry = dy - hh * q[4], rz = dz - hh * q[5];
+        const float a = sqrtf(rx * rx + ry * ry + rz * rz) - q[6];
+        return q[7] - sqrtf(a * a + hh * hh);
+    }
+    if (type == SDF_CYL) {   /* capped cylinder, I. Quilez's exact form */
+        const float bx = q[3] - q[0], by = q[4] - q[1], bz = q[5] - q[2];
+        const float px = x - q[0], py = y - q[1], pz = z - q[2];
+        const float baba = bx * bx + by * by + bz * bz, paba = px * bx + py * by + pz * bz;
+        const float ex = px * baba - bx * paba, ey = py * baba - by * paba, ez = pz * baba - bz * paba;
+        const float u = sqrtf(ex * ex + ey * ey + ez * ez) - q[6] * baba;
+        const float v = fabsf(paba - baba * 0.5f) - baba * 0.5f;
+        const float u2 = u * u, v2 = v * v * baba;
+        const float dd = fmaxf(u, v) < 0.0f ? -fminf(u2, v2) : ((u > 0.0f ? u2 : 0.0f) + (v > 0.0f ? v2 : 0.0f));
+        return -(dd < 0.0f ? -1.0f : 1.0f) * sqrtf(fabsf(dd)) / baba;
+    }
+    if (type == SDF_CONE) {   /* capped frustum, I. Quilez's exact form */
+        const float bx = q[3] - q[0], by = q[4] - q[1], bz = q[5] - q[2];
+        const float px = x - q[0], py = y - q[1], pz = z - q[2];
+        const float ra = q[6], rb = q[7];
+        const float rba = rb - ra, baba = bx * bx + by * by + bz * bz;
+        const float papa = px * px + py * py + pz * pz, paba = (px * bx + py * by + pz * bz) / baba;
+        const float xr = sqrtf(fmaxf(papa - paba * paba * baba, 0.0f));
+        const float cax = fmaxf(0.0f, xr - ((paba < 0.5f) ? ra : rb)), cay = fabsf(paba - 0.5f) - 0.5f;
+        const float k = rba * rba + baba;
+        const float fr = fminf(fmaxf((rba * (xr - ra) + paba * baba) / k, 0.0f), 1.0f);
+        const float cbx = xr - ra - fr * rba, cby = paba - fr;
+        const float sg = (cbx < 0.0f && cay < 0.0f) ? -1.0f : 1.0f;
+        const float d2 = fminf(cax * cax + cay * cay * baba, cbx * cbx + cby * cby * baba);
+        return -sg * sqrtf(d2);
+    }
+    if (type == SDF_ELLIPSOID) {   /* the bound k0 (k0 - 1) / k1, exact on the surface */
+        const float dx = x - q[0], dy = y - q[1], dz = z - q[2];
+        const float lx = q[6] * dx + q[7] * dy + q[8] * dz, ly = q[9] * dx + q[10] * dy + q[11] * dz,
+                    lz = q[12] * dx + q[13] * dy + q[14] * dz;
+        const float ax = lx / q[3], ay = ly / q[4], az = lz / q[5];
+        const float bx = ax / q[3], by = ay / q[4], bz = az / q[5];
+        const float k0 = sqrtf(ax * ax + ay * ay + az * az), k1 = sqrtf(bx * bx + by * by + bz * bz);
+        if (k1 < 1e-20f) {
+            return fminf(q[3], fminf(q[4], q[5]));
+        }
+        return -k0 * (k0 - 1.0f) / k1;
+    }
+    return -1e30f;
+}
+
+/* min (sgn 1) or max (sgn -1) of a and b with the crease blend of radius k and the
+ * gap closing of radius kgap (v2m_sdf_minmax, :191-273) */
+static SdfV sdf_minmax(SdfV a, SdfV b, float sgn, float k, float kgap, int grad, int* near) {
+    const float va = sgn * a.v, vb = sgn * b.v;
+    const int sliver = kgap > 0.0f && va > -kgap && vb > -kgap && va + vb < kgap;
+    if (sliver && !grad) {
+        *near = 1;
+    }
+    float pen = 0.0f, dpen = 0.0f;
+    if (sliver && grad) {
+        const float ga = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z), gb = sqrtf(b.x * b.x + b.y * b.y + b.z * b.z);
+        if (ga > 0.0f && gb > 0.0f) {
+            const float c = (a.x * b.x + a.y * b.y + a.z * b.z) / (ga * gb);
+            const float wg = fminf(1.0f, fmaxf(0.0f, (-c - 0.7f) / 0.25f));
+            const float u = fmaxf(0.0f, 1.0f - (va + vb) / kgap);
+            pen = kgap * wg * u * u;
+            dpen = 2.0f * wg * u;
+        }
+    }
+    SdfV r = a;
+    if (k > 0.0f && fabsf(va - vb) < k) {
+        if (!grad) {
+            *near = 1;
+        } else {
+            const float ga = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z), gb = sqrtf(b.x * b.x + b.y * b.y + b.z * b.z);
+            const float c = ga > 0.0f && gb > 0.0f ? (a.x * b.x + a.y * b.y + a.z * b.z) / (ga * gb) : 0.0f;
+            k *= fminf(1.0f, fmaxf(0.0f, (0.995f - c) / 0.095f));
+        }
+    }
+    if (k > 0.0f && fabsf(va - vb) < k) {
+        const float h = 0.5f + 0.5f * (vb - va) / k;
+        r.v = sgn * (vb * (1.0f - h) + va * h - k * h * (1.0f - h));
+        r.x = h * a.x + (1.0f - h) * b.x;
+        r.y = h * a.y + (1.0f - h) * b.y;
+        r.z = h * a.z + (1.0f - h) * b.z;
+    } else if (vb < va) {
+        r = b;
+    }
+    r.v -= sgn * pen;
+    if (dpen > 0.0f) {
+        r.x += dpen * (a.x + b.x);
+        r.y += dpen * (a.y + b.y);
+        r.z += dpen * (a.z + b.z);
+    }
+    return r;
+}
+
+static int sdf_is_minmax(int op) {
+    return op == SDF_MAX || op == SDF_MIN || op == SDF_CMAX || op == SDF_CMIN;
+}
+
+/* label l's field at grid-mm point p and (wantg) its gradient: the postfix program
+ * on a stack of SDF_STACK entries, values first and the gradients only where a gap
+ * or a crease needs them (v2m_sdf_evalv, :275-478) */
+static SdfV sdf_evalv(const float* prog, int l, float px, float py, float pz, int wantg) {
+    const int N = (int)prog[0];
+    const SdfV zero = { 0.0f, 0.0f, 0.0f, 0.0f };
+    SdfV none = zero;
+    none.v = -1e30f;
+    if (l < 0 || l >= N) {
+        return none;
+    }
+    const float x = px + prog[2], y = py + prog[3], z = pz + prog[4];
+    int start = (int)prog[5 + l];
+    const int T = (int)prog[8 + N];
+    if (T > 0) {   /* the point's brick's own program */
+        const int nbx = (int)prog[T], nby = (int)prog[T + 1], nbz = (int)prog[T + 2];
+        const float vx = prog[T + 3];
+        int bi = (int)floorf(px / vx + 0.5f) >> 3, bj = (int)floorf(py / vx + 0.5f) >> 3,
+            bk = (int)floorf(pz / vx + 0.5f) >> 3;
+        bi = bi < 0 ? 0 : (bi >= nbx ? nbx - 1 : bi);
+        bj = bj < 0 ? 0 : (bj >= nby ? nby - 1 : bj);
+        bk = bk < 0 ? 0 : (bk >= nbz ? nbz - 1 : bk);
+        start = (int)prog[(int)prog[T + 4 + bi + nbx * (bj + nby * bk)] + l];
+    }
+    SdfV st[SDF_STACK];   /* st[sp - 1] is the top */
+    int sp = 0;
+    const float kgap = prog[6 + N], kblend = prog[5 + N];
+    int grad = wantg != 0, near = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        sp = 0;
+        int pc = start;
+        near = 0;
+        for (int guard = 0; guard < 100000; ++guard) {
+            const int op = (int)prog[pc];
+            if (op == SDF_END) {
+                break;
+            }
+            if (sdf_is_minmax(op)) {   /* second op top -> top, pop */
+                if (sp > 1) {
+                    st[sp - 2] = sdf_minmax(st[sp - 2], st[sp - 1], (op == SDF_MAX || op == SDF_CMAX) ? -1.0f : 1.0f,
+                                            kblend, op >= SDF_CMAX ? 0.0f : kgap, grad, &near);
+                    --sp;
+                }
+                pc += 1;
+                continue;
+            }
+            if (op == SDF_NEG) {
+                if (sp > 0) {
+                    st[sp - 1].v = -st[sp - 1].v;
+                    st[sp - 1].x = -st[sp - 1].x;
+                    st[sp - 1].y = -st[sp - 1].y;
+                    st[sp - 1].z = -st[sp - 1].z;
+                }
+                pc += 1;
+                continue;
+            }
+            if (op == SDF_ADD) {   /* top += second, pop */
+                if (sp > 1) {
+                    SdfV t = st[sp - 1];
+                    t.v += st[sp - 2].v;
+                    t.x += st[sp - 2].x;
+                    t.y += st[sp - 2].y;
+                    t.z += st[sp - 2].z;
+                    st[sp - 2] = t;
+                    --sp;
+                }
+                pc += 1;
+                continue;
+            }
+            if (op == SDF_SCALE) {
+                const float fs = prog[pc + 1];
+                if (sp > 0) {
+                    st[sp - 1].v *= fs;
+                    st[sp - 1].x *= fs;
+                    st[sp - 1].y *= fs;
+                    st[sp - 1].z *= fs;
+                }
+                pc += 2;
+                continue;
+            }
+            SdfV nv = zero;
+            if (op == SDF_PRIM) {   /* the value and (grad) this primitive's central differences */
+                const int type = (int)prog[pc + 1];
+                const float* q = prog + pc + 2;
+                const float e = fmaxf(1e-4f * (fabsf(q[0]) + fabsf(q[1]) + fabsf(q[2]) + 1.0f), 1e-5f);
+                const int ne = grad ? 7 : 1;
+                float fv[7];
+                for (int k = 0; k < ne; ++k) {
+                    const float dd = k == 0 ? 0.0f : ((k & 1) ? e : -e);
+                    const int ax = (k - 1) >> 1;
+                    fv[k] = sdf_prim(type, q, x + (ax == 0 ? dd : 0.0f), y + (ax == 1 ? dd : 0.0f),
+                                     z + (ax == 2 ? dd : 0.0f));
+                }
+                nv.v = fv[0];
+                if (grad) {
+                    nv.x = (fv[1] - fv[2]) / (2.0f * e);
+                    nv.y = (fv[3] - fv[4]) / (2.0f * e);
+                    nv.z = (fv[5] - fv[6]) / (2.0f * e);
+                }
+                pc += 2 + sdf_nparam(type);
+            } else if (op == SDF_BBOX) {   /* outside the box (+ margin): the bound, code skipped */
+                const float m = prog[7 + N];
+                const float* q = prog + pc + 1;
+                if (!(x < q[0] - m || y < q[1] - m || z < q[2] - m || x > q[3] + m || y > q[4] + m || z > q[5] + m)) {
+                    pc += 9;
+                    continue;
+                }
+                nv.v = q[7] * m;
+                pc += 9 + (int)q[6];
+            } else if (op == SDF_MCONST) {
+                nv.v = prog[pc + 1] * prog[7 + N];
+                pc += 2;
+            } else if (op == SDF_CONST) {
+                nv.v = prog[pc + 1];
+                pc += 2;
+            } else {
+                break;   /* a malformed program */
+            }
+            {   /* an operand followed by a min / max (after an optional NEG) combines at once */
+                const int op1 = (int)prog[pc], neg = op1 == SDF_NEG;
+                const int op2 = neg ? (int)prog[pc + 1] : op1;
+                if (sp > 0 && sdf_is_minmax(op2)) {
+                    if (neg) {
+                        nv.v = -nv.v;
+                        nv.x = -nv.x;
+                        nv.y = -nv.y;
+                        nv.z = -nv.z;
+                    }
+                    st[sp - 1] = sdf_minmax(st[sp - 1], nv, (op2 == SDF_MAX || op2 == SDF_CMAX) ? -1.0f : 1.0f, kblend,
+                                            op2 >= SDF_CMAX ? 0.0f : kgap, grad, &near);
+                    pc += 1 + neg;
+                    continue;
+                }
+            }
+            if (sp < SDF_STACK) {
+                st[sp++] = nv;
+            }
+        }
+        if (grad || !near) {
+            break;
+        }
+        grad = 1;
+    }
+    return sp > 0 ? st[sp - 1] : none;
+}
+
+static float sdf_value(const float* prog, int l, float px, float py, float pz) {
+    return sdf_evalv(prog, l, px, py, pz, 0).v;
+}
+
+/* v2m_phi_f, src/opencl/v2m_seed_body.cl:33-39: v2m_phi_at (src/opencl/v2m_grid_body.cl:
+ * 189-200), or for shape fields clamp(0.5 + s_l / w) */
 static float phi_point(const Fld* f, int l, float px, float py, float pz) {
+    if (f->gm < 0) {
+        return fminf(1.0f, fmaxf(0.0f, 0.5f + sdf_value(f->gTW, l, px, py, pz) / f->gTW[1]));
+    }
     const float u = px / f->vx, v = py / f->vy, w = pz / f->vz;
     const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
     float c[8];
@@ -197,6 +503,9 @@ static int gray_psi(const Fld* f, int a, int b, float px, float py, float pz, fl
  * gray-scale form for adjacent labels) */
 static float psi_point(const Fld* f, int a, int b, float px, float py, float pz) {
     float gv;
+    if (f->gm < 0) {   /* shape fields: (s_a - s_b) / w */
+        return (sdf_value(f->gTW, a, px, py, pz) - sdf_value(f->gTW, b, px, py, pz)) / f->gTW[1];
+    }
     if (gray_psi(f, a, b, px, py, pz, &gv, NULL)) {
         return gv;
     }
@@ -215,6 +524,14 @@ static float psi_point(const Fld* f, int a, int b, float px, float py, float pz)
  * from one trilinear cell (the gray-scale form for adjacent labels) */
 static float psi_grad_point(const Fld* f, int a, int b, float px, float py, float pz, float* g) {
     float gv;
+    if (f->gm < 0) {   /* shape fields: the analytic gradients */
+        const float w = f->gTW[1];
+        const SdfV sa = sdf_evalv(f->gTW, a, px, py, pz, 1), sb = sdf_evalv(f->gTW, b, px, py, pz, 1);
+        g[0] = (sa.x - sb.x) / w;
+        g[1] = (sa.y - sb.y) / w;
+        g[2] = (sa.z - sb.z) / w;
+        return (sa.v - sb.v) / w;
+    }
     if (gray_psi(f, a, b, px, py, pz, &gv, g)) {
         return gv;
     }
@@ -274,14 +591,14 @@ static int owner_at(const Fld* f, float px, float py, float pz, int* second, flo
     int lab[TN_BL];
     const int n = brick_labels_at(f, px, py, pz, lab);
     int best = lab[0], sec = TN_NOLAB;
-    float pb = -1.0f, ps = pb;
+    float pb = f->gm < 0 ? -1e30f : -1.0f, ps = pb;
     if (n == 1) {
         *second = TN_NOLAB;
         *margin = 1.0f;
         return best;
     }
-    for (int s = 0; s < n; ++s) {
-        const float v = phi_point(f, lab[s], px, py, pz);
+    for (int s = 0; s < n; ++s) {   /* (shape fields: ranked by s itself) */
+        const float v = f->gm < 0 ? sdf_value(f->gTW, lab[s], px, py, pz) : phi_point(f, lab[s], px, py, pz);
         if (v > pb) {
             ps = pb;
             sec = best;
@@ -291,6 +608,10 @@ static int owner_at(const Fld* f, float px, float py, float pz, int* second, flo
             ps = v;
             sec = lab[s];
         }
+    }
+    if (f->gm < 0) {
+        pb = fminf(1.0f, fmaxf(0.0f, 0.5f + pb / f->gTW[1]));
+        ps = sec == TN_NOLAB ? 0.0f : fminf(1.0f, fmaxf(0.0f, 0.5f + ps / f->gTW[1]));
     }
     *second = sec;
     *margin = pb - ps;
@@ -364,6 +685,32 @@ static int onto_interface(const Fld* f, int a, int b, float* p, float rad) {
  * onto a|b along n_ab and onto a|c along n_ac's part orthogonal to n_ab */
 static int onto_junction(const Fld* f, int a, int b, int c, float* p, float rad) {
     float q[3] = { p[0], p[1], p[2] };
+    if (f->gm < 0) {
+        /* shape fields: of the three pairs' constraints, solve the two with the most
+         * independent gradients, their shared label first (src/opencl/v2m_seed_body.cl:345-375) */
+        float gab[3], gac[3], gbc[3];
+        psi_grad_point(f, a, b, q[0], q[1], q[2], gab);
+        psi_grad_point(f, a, c, q[0], q[1], q[2], gac);
+        psi_grad_point(f, b, c, q[0], q[1], q[2], gbc);
+        const float xa[3] = { gab[1] * gac[2] - gab[2] * gac[1], gab[2] * gac[0] - gab[0] * gac[2],
+                              gab[0] * gac[1] - gab[1] * gac[0] };
+        const float xb[3] = { gab[1] * gbc[2] - gab[2] * gbc[1], gab[2] * gbc[0] - gab[0] * gbc[2],
+                              gab[0] * gbc[1] - gab[1] * gbc[0] };
+        const float xc[3] = { gac[1] * gbc[2] - gac[2] * gbc[1], gac[2] * gbc[0] - gac[0] * gbc[2],
+                              gac[0] * gbc[1] - gac[1] * gbc[0] };
+        const float na = xa[0] * xa[0] + xa[1] * xa[1] + xa[2] * xa[2], nb = xb[0] * xb[0] + xb[1] * xb[1] + xb[2] * xb[2],
+                    nc = xc[0] * xc[0] + xc[1] * xc[1] + xc[2] * xc[2];
+        if (nb > na && nb >= nc) {   /* pivot b */
+            const int t = a;
+            a = b;
+            b = t;
+        } else if (nc > na && nc > nb) {   /* pivot c */
+            const int t = a;
+            a = c;
+            c = b;
+            b = t;
+        }
+    }
     for (int it = 0; it < 3; ++it) {
         float g1[3], g2[3];
         if (!onto_interface(f, a, b, q, rad)) {
@@ -1176,8 +1523,8 @@ static int supported(const tn_grid_view* g, const tn_relax_params* prm) {
     if (!g || !prm) {
         return fail("oracle: null argument");
     }
-    if (g->gm < 0) {
-        return fail("oracle: shape fields (gm < 0) are not covered");
+    if (g->gm < 0 && (!g->gTW || g->ngTW < 9)) {
+        return fail("oracle: a shape grid (gm < 0) needs its program in gTW");
     }
     if (g->gm > 0 && (!g->gI || !g->gTW || g->ngTW < 2 * (int64_t)g->gm)) {
         return fail("oracle: a gray-scale grid needs gI and 2 gm values in gTW");
@@ -1483,5 +1830,17 @@ void orc_field_points(const tn_grid_view* g, int32_t a, int32_t b, int64_t m, co
         float mg;
         label[k] = owner_at(&f, x[0], x[1], x[2], &sec, &mg);
         second[k] = sec;
+    }
+}
+
+void orc_sdf_points(const float* prog, int32_t l, int64_t m, const float* X, int32_t grad, float* v, float* g) {
+    for (int64_t k = 0; k < m; ++k) {
+        const SdfV r = sdf_evalv(prog, l, X[3 * k], X[3 * k + 1], X[3 * k + 2], grad);
+        v[k] = r.v;
+        if (g) {
+            g[3 * k] = r.x;
+            g[3 * k + 1] = r.y;
+            g[3 * k + 2] = r.z;
+        }
     }
 }

@@ -58,6 +58,8 @@ int orc_relax(const tn_grid_view* g, const tn_relax_params* prm, tn_nodes_view* 
               orc_iter_trace* trace, int32_t nthreads);
 
 /* point queries: phi_a, psi_ab and its gradient, label_of and the runner-up */
+/* label l's analytic field (and gradient, grad != 0) of a shape program at m points */
+void orc_sdf_points(const float* prog, int32_t l, int64_t m, const float* X, int32_t grad, float* v, float* g);
 void orc_field_points(const tn_grid_view* g, int32_t a, int32_t b, int64_t m, const float* X, float* phi_a,
                       float* psi, float* grad, int32_t* label, int32_t* second);
 

@@ -40,6 +40,7 @@ def lib():
         L.orc_relax.argtypes = [C.POINTER(GridView), C.POINTER(RelaxParamsC), C.POINTER(NodesView),
                                 C.POINTER(RelaxStatsC), C.POINTER(IterTrace), C.c_int32]
         L.orc_field_points.argtypes = [C.POINTER(GridView), C.c_int32, C.c_int32, C.c_int64] + [C.c_void_p] * 6
+        L.orc_sdf_points.argtypes = [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
         L.orc_hist_bin.argtypes = [C.c_float]
         L.orc_hist_bin.restype = C.c_int32
         _lib = L
@@ -157,3 +158,14 @@ def adj_grad(pc, x0: np.ndarray, theta: np.ndarray) -> tuple[float, np.ndarray, 
     terms = np.zeros(3)
     L = _adj_bind().orc_adj_grad(C.byref(pc), _p(x0), _p(th), _p(gx), _p(gt), _p(terms))
     return float(L), gx, gt
+
+
+def sdf_points(prog: np.ndarray, l: int, X: np.ndarray, grad: bool = False):
+    """Label l's analytic field of a shape program at points X (n, 3): (values, gradients or None)."""
+    prog = np.ascontiguousarray(prog, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    m = X.shape[0]
+    v = np.zeros(m, np.float32)
+    g = np.zeros((m, 3), np.float32) if grad else None
+    lib().orc_sdf_points(_p(prog), l, m, _p(X), int(grad), _p(v), _p(g) if grad else None)
+    return v, g

@@ -53,6 +53,8 @@ def lib():
         L.ref_grid_build_full.restype = vp
         L.ref_grid_build_full.argtypes = [vp, vp]
         L.ref_grid_scalars.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_float)]
+        L.ref_grid_build_shapes.restype = vp
+        L.ref_grid_build_shapes.argtypes = [C.c_char_p, C.c_double, vp]
         L.ref_grid_from_view.restype = vp
         L.ref_grid_from_view.argtypes = [C.POINTER(GridView)]
         L.ref_grid_free.argtypes = [vp]
@@ -137,6 +139,17 @@ class RefGrid:
         h = lib().ref_grid_build_full(C.addressof(v), C.addressof(pc))
         return RefGrid(h, keep=(v, vkeep, pc, pkeep))
 
+    @staticmethod
+    def build_shapes(json_text: str, voxel: float = 0.0, prm=None) -> "RefGrid":
+        """A grid from shape constructs (JSON), as shapes_volume and build_grid_cpu
+        make it (src/v2m_modes.cpp:508-545, src/v2m_grid.cpp:47): the labels'
+        analytic fields are its point queries (gm = -1, gTW the program)."""
+        from trussnet_amd._abi import GridParams
+
+        pc, pkeep = (prm or GridParams()).to_c()
+        h = lib().ref_grid_build_shapes(json_text.encode(), float(voxel), C.addressof(pc))
+        return RefGrid(h, keep=(pc, pkeep))
+
     def scalars(self) -> dict:
         """R, limit_sweeps, overflow_bricks, nslot, hbase, hmin, hmax, hcurv of the Grid."""
         info = (C.c_int64 * 4)()
@@ -170,7 +183,12 @@ class RefGrid:
         nb = v.nbx * v.nby * v.nbz
         cp = lambda ptr, cnt, dt: np.ctypeslib.as_array(ptr, shape=(cnt,)).astype(dt, copy=True)
         gr = cp(grade, nv, np.uint8) if grade else None
-        gray = (cp(v.gI, nv, np.float32), cp(v.gTW, 2 * v.gm, np.float32), int(v.gm)) if v.gm > 0 else (None, None, 0)
+        if v.gm > 0:
+            gray = (cp(v.gI, nv, np.float32), cp(v.gTW, 2 * v.gm, np.float32), int(v.gm))
+        elif v.gm < 0:
+            gray = (None, cp(v.gTW, v.ngTW, np.float32), int(v.gm))
+        else:
+            gray = (None, None, 0)
         return Grid(v.nx, v.ny, v.nz, tuple(v.vs), cp(v.L, nv, np.uint16), cp(v.bl_cnt, nb, np.int32),
                     cp(v.bl_lab, nb * 16, np.uint16), cp(v.bl_slot, nb, np.int32),
                     cp(v.phi, v.nslot * 512, np.float32) if v.nslot else np.zeros(0, np.float32),

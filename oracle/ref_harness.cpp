@@ -24,6 +24,8 @@
 
 #include "v2m_grid.h"
 #include "v2m_particles.h"
+#include "v2m_sdfshape.h"
+#include "v2m_tpm.h"
 #include "v2m_volume.h"
 
 #include "../include/trussnet_amd.h"
@@ -295,6 +297,58 @@ void* ref_grid_build_full(const tn_volume_view* v, const tn_grid_params* gp) {
     }
 }
 
+// A grid from shape constructs (JSON text): the steps of shapes_volume
+// (src/v2m_modes.cpp:508-545) -- load_shapes, the gap-closing radius and cull
+// margin written into the program, the per-brick programs, the raster's fields
+// through apply_tpm -- then build_grid_cpu with the analytic fields (gm = -1).
+// voxel <= 0: hbase / 3 (or the scene's extent / 160).
+void* ref_grid_build_shapes(const char* json_text, double voxel, const tn_grid_params* gp) {
+    try {
+        RefGrid* r = new RefGrid;
+        tn::ShapeScene sc = tn::load_shapes(json_text, true, "overwrite");
+
+        if (!(voxel > 0)) {
+            const double ext = std::max(sc.hi[0] - sc.lo[0], std::max(sc.hi[1] - sc.lo[1], sc.hi[2] - sc.lo[2]));
+            voxel = gp->hbase > 0 ? gp->hbase / 3.0 : ext / 160.0;
+        }
+
+        const double hb = gp->hbase > 0 ? gp->hbase : 3.0 * voxel;
+        const double gap = 0.5;   // PipelineOptions::shape_gap
+        sc.prog[6 + static_cast<size_t>(sc.nlab)] = static_cast<float>(gap * hb);
+        sc.prog[7 + static_cast<size_t>(sc.nlab)] = static_cast<float>(std::max(1.25 * hb, std::max(gap * hb, 3.0 * voxel)));
+        tn::build_brick_programs(sc, voxel);
+        const tn::Tpm tpm = tn::rasterize_scene(sc, voxel);
+        tn::TpmOptions to;
+        to.fields = true;
+        to.fill_holes = false;
+        to.exterior.assign(1, 0);
+        to.map.clear();
+        to.sigma = 0.0f;
+        tn::apply_tpm(tpm, to, r->lv);
+        r->lv.sdf = sc.prog;
+        r->lv.sdf_feat = sc.feat;
+        tn::GridParams p;
+        p.sigma = gp->sigma;
+        p.sigma_thin = gp->sigma_thin;
+        p.thin_lo = gp->thin_lo;
+        p.thin_hi = gp->thin_hi;
+        p.thick = gp->thick;
+        p.thin_floor = gp->thin_floor;
+        p.preserve = gp->preserve;
+        p.sigma_curv = gp->sigma_curv;
+        p.hbase = gp->hbase;
+        p.hmin = gp->hmin;
+        p.hmax = gp->hmax;
+        p.K = gp->K;
+        p.g = gp->g;
+        tn::build_grid_cpu(r->lv, p, r->g);
+        return r;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
 // the Grid's scalars: info[0..3] = R, limit_sweeps, overflow_bricks, nslot;
 // sc[0..3] = hbase, hmin, hmax, hcurv
 void ref_grid_scalars(void* h, int64_t* info, float* sc) {
@@ -347,6 +401,9 @@ void* ref_grid_from_view(const tn_grid_view* v) {
         g.gm = v->gm;
         g.gTW.assign(v->gTW, v->gTW + 2 * v->gm);
         g.gray_w.assign(v->gTW + v->gm, v->gTW + 2 * v->gm);
+    } else if (v->gm < 0 && v->gTW && v->ngTW > 0) {   // shape fields: the compiled program
+        g.gm = v->gm;
+        g.gTW.assign(v->gTW, v->gTW + v->ngTW);
     }
 
     return r;
@@ -390,8 +447,8 @@ void ref_grid_view(void* h, tn_grid_view* v, const uint8_t** grade, float* hbase
     v->hmax = g.hmax;
     v->gm = g.gm;
 
-    if (g.gm > 0) {
-        v->gI = g.gI;
+    if (g.gm != 0) {
+        v->gI = g.gm > 0 ? g.gI : nullptr;
         v->gTW = g.gTW.data();
         v->ngTW = static_cast<int64_t>(g.gTW.size());
     }

@@ -9,7 +9,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-CASES = ["spheres64", "jitter64", "sectors48", "ellipsoids56", "aniso40", "tpm32", "gray40"]
+CASES = ["spheres64", "jitter64", "sectors48", "ellipsoids56", "aniso40", "tpm32", "gray40", "shapes40"]
 
 
 def pytest_configure(config):

@@ -53,6 +53,13 @@ def graded_size(lab: np.ndarray, lo: float, hi: float, slope: float) -> np.ndarr
     return np.minimum(hi, lo + slope * dist).astype(np.float32)
 
 
+SHAPES40 = """{"Shapes": [
+  {"Grid":     {"Tag": 0, "Size": [40, 40, 40]}},
+  {"Sphere":   {"O": [20, 20, 20], "R": 15, "Tag": 1}},
+  {"Box":      {"O": [12, 12, 12], "Size": [16, 16, 10], "Tag": 2}},
+  {"Cylinder": {"C0": [0, 20, 26], "C1": [40, 20, 26], "R": 5, "Tag": 3}}
+]}"""
+
 CASES = {
     # the small plain workload: 64^3, spheres r 24 / 12, uniform 3-voxel size; HCP seeds
     "spheres64": dict(vol=lambda: volumes.two_spheres(64, 24.0, 12.0), grid=dict(hbase=3.0, hmin=3.0, hmax=3.0),
@@ -81,6 +88,10 @@ CASES = {
     # intensity itself
     "gray40": dict(vol=lambda: volumes.gray_ball(40), thr=volumes.GRAY_THRESHOLDS, grid=dict(hbase=2.5),
                    runs=[(1, 0.0, True), (30, 0.0, True), (200, 2e-3, True), (30, 0.0, False), (40, 0.0, False, True)]),
+    # shape constructs: the labels' analytic fields (a compiled program, gm = -1) with a
+    # box's edges and corners and a cylinder's rims pinned by the reference's seeding
+    "shapes40": dict(shapes=SHAPES40, grid=dict(hbase=3.0),
+                     runs=[(1, 0.0, True), (30, 0.0, True), (200, 2e-3, True), (30, 0.0, False)]),
     # anisotropic voxels (0.8 x 0.9 x 1.3 mm): every coordinate -> voxel division is inexact
     "aniso40": dict(vol=lambda: volumes.sectors(40), vs=(0.8, 0.9, 1.3), grid=dict(hbase=2.5),
                     runs=[(1, 0.0, True), (20, 0.0, True), (150, 2e-3, True), (30, 0.0, False), (40, 0.0, False, True)]),
@@ -99,14 +110,19 @@ def field_samples(G, nodes, rng, m=1500):
 
 
 def make(name: str, spec: dict) -> str:
-    lab = spec["vol"]()
+    lab = spec["vol"]() if "vol" in spec else None
     hvox = None
     if "hvox" in spec:
         lo, hi, slope = spec["hvox"]
         hvox = graded_size(lab, lo, hi, slope).ravel()
     prob = spec["prob"]() if "prob" in spec else None
-    rg = ref.RefGrid.build(lab, vs=spec.get("vs", (1.0, 1.0, 1.0)), hvox=hvox, prob=prob, thresholds=spec.get("thr"),
-                           **spec["grid"])
+    if "shapes" in spec:
+        from trussnet_amd._abi import GridParams
+
+        rg = ref.RefGrid.build_shapes(spec["shapes"], 0.0, GridParams(**spec["grid"]))
+    else:
+        rg = ref.RefGrid.build(lab, vs=spec.get("vs", (1.0, 1.0, 1.0)), hvox=hvox, prob=prob,
+                               thresholds=spec.get("thr"), **spec["grid"])
     G = rg.grid()
     prm = RelaxParams()
     hcp = rg.seed(prm)

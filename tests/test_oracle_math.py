@@ -270,6 +270,8 @@ def test_field_queries_are_trilinear(golden):
     the number of thresholds <= I (v2m_seed_body.cl:65-131)."""
     d = golden.d
     g = golden.grid
+    if g.gm < 0:
+        pytest.skip("analytic shape fields: pinned by test_sdf_primitives_match_closed_forms and the fixture")
     a, b = (int(x) for x in d["fq_ab"])
     X = d["fq_X"].astype(np.float64)
     vs = np.asarray(g.vs, np.float64)
@@ -341,3 +343,107 @@ def test_oracle_matches_fixture_lists(golden, oracle_lib):
     nbr[np.arange(K)[None, :] >= rb["nnb"][:, None]] = -1
     assert np.array_equal(nbr.ravel(), golden.d["rb_nbr"])
     assert np.array_equal(bits(rb["hn"]), bits(golden.d["rb_hn"]))
+
+
+def _program(codes, w=1.5, origin=(0.0, 0.0, 0.0), kblend=0.0, kgap=0.0, margin=50.0):
+    """A shape program (the layout of src/opencl/v2m_sdf_body.cl:1-45) with one code
+    list per label and no brick table."""
+    n = len(codes)
+    head = [float(n), w, *origin] + [0.0] * n + [kblend, kgap, margin, 0.0]
+    prog = list(head)
+    for l, c in enumerate(codes):
+        prog[5 + l] = float(len(prog))
+        prog += [float(x) for x in c] + [0.0]
+    return np.array(prog, np.float32)
+
+
+def _seg_dist(P, A, B):
+    """Distance from 2-D points P (m, 2) to the segment A-B."""
+    d = B - A
+    t = np.clip(((P - A) @ d) / (d @ d), 0.0, 1.0)
+    return np.linalg.norm(P - (A + t[:, None] * d), axis=1)
+
+
+def test_sdf_primitives_match_closed_forms():
+    """The shape-field interpreter's primitives (v2m_sdf_prim, src/opencl/v2m_sdf_body.cl:
+    76-161) against their distance functions in float64, and the CSG ops (max, min,
+    complement) against their definitions; the gradients are central differences of
+    the exact field."""
+    from oracle import orc
+
+    rng = np.random.default_rng(11)
+    X = rng.uniform(-12, 12, size=(4000, 3)).astype(np.float32)
+    x = X.astype(np.float64)
+    c = np.array([1.0, -2.0, 0.5])
+    cases = []
+    # sphere
+    cases.append(([1, 1, *c, 7.0], 7.0 - np.linalg.norm(x - c, axis=1)))
+    # box lo..hi: minus the exact box distance
+    lo, hi = np.array([-5.0, -3.0, -4.0]), np.array([6.0, 2.0, 3.0])
+    u = np.abs(x - (lo + hi) / 2) - (hi - lo) / 2
+    box = np.linalg.norm(np.maximum(u, 0), axis=1) + np.minimum(u.max(axis=1), 0)
+    cases.append(([1, 2, *lo, *hi], -box))
+    # plane through o with unit normal n: the half-space behind n
+    o, nn = np.array([0.5, 0.0, -1.0]), np.array([2.0, -1.0, 2.0]) / 3.0
+    cases.append(([1, 4, *o, *nn], -((x - o) @ nn)))
+    # slab lo < (p - o).n < hi
+    t = (x - o) @ nn
+    cases.append(([1, 5, *o, *nn, -2.0, 3.5], np.minimum(t + 2.0, 3.5 - t)))
+    # infinite cylinder through o along n, radius 4
+    rp = (x - o) - np.outer(t, nn)
+    cases.append(([1, 9, *o, *nn, 4.0], 4.0 - np.linalg.norm(rp, axis=1)))
+    # torus: centre o, axis n, radii 6 and 2
+    cases.append(([1, 6, *o, *nn, 6.0, 2.0], 2.0 - np.sqrt((np.linalg.norm(rp, axis=1) - 6.0) ** 2 + t ** 2)))
+    # capped cylinder a-b, radius 3: the signed distance to the (t, rho) rectangle
+    a, b = np.array([-6.0, 1.0, 0.0]), np.array([5.0, -2.0, 3.0])
+    L = np.linalg.norm(b - a)
+    ax = (b - a) / L
+    tt = (x - a) @ ax
+    rho = np.linalg.norm((x - a) - np.outer(tt, ax), axis=1)
+    dr, dt = rho - 3.0, np.maximum(-tt, tt - L)
+    cyl = np.where(np.maximum(dr, dt) < 0, np.maximum(dr, dt), np.hypot(np.maximum(dr, 0), np.maximum(dt, 0)))
+    cases.append(([1, 3, *a, *b, 3.0], -cyl))
+    # cone frustum a (radius 4) .. b (radius 1.5): the signed distance to its half cross-section
+    Q = np.stack([rho, tt], axis=1)
+    poly = [np.array([0.0, 0.0]), np.array([4.0, 0.0]), np.array([1.5, L]), np.array([0.0, L])]
+    d = np.min([_seg_dist(Q, poly[k], poly[k + 1]) for k in range(3)], axis=0)   # caps and side, not the axis
+    r_at = 4.0 + (1.5 - 4.0) * np.clip(tt / L, 0, 1)
+    inside = (tt > 0) & (tt < L) & (rho < r_at)
+    cases.append(([1, 8, *a, *b, 4.0, 1.5], np.where(inside, d, -d)))
+    for code, want in cases:
+        prog = _program([code])
+        v, _ = orc.sdf_points(prog, 0, X)
+        assert np.abs(v - want).max() < 2e-5 * 12, code[:2]
+        vg, g = orc.sdf_points(prog, 0, X, grad=True)
+        assert np.array_equal(vg, v)
+        # the gradient: central differences of the closed form where it is smooth
+        e = 1e-3
+        fd = np.zeros((X.shape[0], 3))
+        for k in range(3):
+            dx = np.zeros(3)
+            dx[k] = e
+            p1, _ = orc.sdf_points(prog, 0, (x + dx).astype(np.float32))
+            p0, _ = orc.sdf_points(prog, 0, (x - dx).astype(np.float32))
+            fd[:, k] = (p1.astype(np.float64) - p0) / (2 * e)
+        smooth = np.linalg.norm(fd, axis=1) > 0.5
+        assert np.median(np.abs(g[smooth] - fd[smooth])) < 2e-2, code[:2]
+    # an ellipsoid (bound k0 (k0 - 1) / k1): the sign and the zero set are exact
+    R = np.array([6.0, 3.0, 2.0])
+    prog = _program([[1, 7, 0.0, 0.0, 0.0, *R, 1, 0, 0, 0, 1, 0, 0, 0, 1]])
+    v, _ = orc.sdf_points(prog, 0, X)
+    inside = ((x / R) ** 2).sum(axis=1) < 1.0
+    k0 = np.sqrt(((x / R) ** 2).sum(axis=1))
+    far = np.abs(k0 - 1.0) > 1e-4
+    assert np.array_equal(v[far] > 0, inside[far])
+    dirs = rng.normal(size=(500, 3))
+    S = (dirs / np.sqrt(((dirs / R) ** 2).sum(axis=1))[:, None]).astype(np.float32)
+    vs, _ = orc.sdf_points(prog, 0, S)
+    assert np.abs(vs).max() < 1e-5 * 6
+    # CSG: union = max, intersection = min, subtraction = min(a, -b) (no blend, no gap)
+    s1, s2 = [1, 1, 0.0, 0.0, 0.0, 6.0], [1, 1, 4.0, 0.0, 0.0, 5.0]
+    f1 = 6.0 - np.linalg.norm(x, axis=1)
+    f2 = 5.0 - np.linalg.norm(x - [4.0, 0.0, 0.0], axis=1)
+    for code, want in (([*s1, *s2, 2], np.maximum(f1, f2)), ([*s1, *s2, 3], np.minimum(f1, f2)),
+                       ([*s1, *s2, 4, 3], np.minimum(f1, -f2))):
+        v, _ = orc.sdf_points(_program([code]), 0, X)
+        assert np.abs(v - want).max() < 2e-5 * 12, code

@@ -181,6 +181,7 @@ class Grid:
     ``L`` and ``h`` are flat arrays of nx*ny*nz; ``phi`` holds nslot*512 values.
     Gray-scale input (gm = m > 0): ``gI`` is the intensity per voxel and ``gTW`` the m
     ascending thresholds followed by their m membership widths (Grid::gI, gray_w).
+    Shape input (gm < 0): ``gTW`` is the labels' compiled analytic-field program.
     """
     nx: int
     ny: int
@@ -274,7 +275,9 @@ class Grid:
              "hminmax": np.array([self.hmin, self.hmax], np.float32), "L": self.L, "bl_cnt": self.bl_cnt,
              "bl_lab": self.bl_lab, "bl_slot": self.bl_slot, "phi": self.phi, "h": self.h}
         if self.gm:
-            d.update(gI=self.gI, gTW=self.gTW, gm=np.int32(self.gm))
+            d.update(gTW=self.gTW, gm=np.int32(self.gm))
+        if self.gI is not None:
+            d["gI"] = self.gI
         return d
 
     @staticmethod
@@ -283,10 +286,10 @@ class Grid:
         nx, ny, nz = (int(x) for x in g("dims"))
         hmin, hmax = (float(x) for x in g("hminmax"))
         grade = d[prefix + "grade"] if (prefix + "grade") in d else None
-        gray = (prefix + "gm") in d and int(d[prefix + "gm"]) != 0
+        gm = int(d[prefix + "gm"]) if (prefix + "gm") in d else 0
+        gI = d[prefix + "gI"] if gm > 0 else None
         return Grid(nx, ny, nz, tuple(float(x) for x in g("vs")), g("L"), g("bl_cnt"), g("bl_lab"), g("bl_slot"),
-                    g("phi"), g("h"), hmin, hmax, grade, g("gI") if gray else None, g("gTW") if gray else None,
-                    int(d[prefix + "gm"]) if gray else 0).normalized()
+                    g("phi"), g("h"), hmin, hmax, grade, gI, g("gTW") if gm else None, gm).normalized()
 
 
 @dataclass
