@@ -122,6 +122,37 @@ def test_grid_build_rejects_bad_volumes(tnlib):
     assert rc == TN_ERR_ARG
 
 
+def test_shape_program_verified(tnlib):
+    """A shape grid's program is walked on the host before any device work: every
+    read the interpreter can make must lie inside it (a corrupt one fails with
+    TN_ERR_ARG; a sound one gets as far as the device)."""
+    from trussnet_amd._abi import TN_ERR_ARG, Grid
+
+    fx = Fixture("shapes40")
+    g = fx.grid
+    assert g.gm < 0 and g.gTW.size > 100
+    bad_cases = []
+    p = g.gTW.copy()
+    n = int(p[0])
+    p[5] = p.size + 10          # label 0's code beyond the program
+    bad_cases.append(p)
+    p = g.gTW.copy()
+    p[int(p[5])] = 77           # an unknown op
+    bad_cases.append(p)
+    p = g.gTW.copy()
+    p[8 + n] = p.size - 2       # a brick table cut off
+    bad_cases.append(p)
+    for prog in bad_cases:
+        bad = Grid(g.nx, g.ny, g.nz, g.vs, g.L, g.bl_cnt, g.bl_lab, g.bl_slot, g.phi, g.h, g.hmin, g.hmax, None,
+                   None, prog, g.gm)
+        with pytest.raises(tnlib.DeviceError) as e:
+            tnlib.relax(bad, fx.seeds)
+        assert f"[{TN_ERR_ARG}]" in str(e.value), str(e.value)
+    if not os.path.exists("/dev/kfd"):
+        with pytest.raises(tnlib.NoDevice):
+            tnlib.relax(g, fx.seeds)
+
+
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is visible here")
 def test_no_device_raises(tnlib):
     """No HIP device: tn_relax fails with TN_ERR_NODEVICE and the nodes stay as they were."""

@@ -278,8 +278,321 @@ __device__ __forceinline__ float trilerp(const float* c, float fx, float fy, flo
     return y0 + fz * (y1 - y0);
 }
 
-// v2m_phi_at (v2m_phi_f, gm == 0), src/opencl/v2m_grid_body.cl:189-200
+// ---- shape fields (gm < 0): the labels' analytic-field program ----------------------
+// The program layout and op codes are v2mesh's (src/opencl/v2m_sdf_body.cl:1-45):
+// [0] labels N, [1] field width w, [2..4] origin, [5..5+N-1] code offsets, [5+N] crease
+// blend radius, [6+N] gap-closing radius, [7+N] cull margin, [8+N] brick table
+// offset (0: none); code: an op, its operands; op 0 ends a label's code.
+namespace sdf {
+constexpr int kEnd = 0, kPrim = 1, kMax = 2, kMin = 3, kNeg = 4, kConst = 5, kBBox = 6, kMConst = 7, kAdd = 8,
+              kScale = 9, kCMax = 10, kCMin = 11;
+constexpr int kSphere = 1, kBox = 2, kCyl = 3, kPlane = 4, kSlab = 5, kTorus = 6, kEllipsoid = 7, kCone = 8,
+              kInfCyl = 9;
+constexpr int kStack = 12;
+
+__host__ __device__ __forceinline__ int nparam(int type) {
+    return type == kSphere ? 4
+           : (type == kBox || type == kPlane) ? 6
+           : (type == kCyl || type == kInfCyl) ? 7
+           : (type == kSlab || type == kTorus || type == kCone) ? 8
+           : type == kEllipsoid ? 15 : 0;
+}
+
+// one primitive's signed distance, > 0 inside (v2m_sdf_prim, :76-161)
+__device__ float prim(int type, const float* __restrict__ q, float x, float y, float z) {
+    if (type == kSphere) {
+        const float dx = x - q[0], dy = y - q[1], dz = z - q[2];
+        return q[3] - sqrtf(dx * dx + dy * dy + dz * dz);
+    }
+    if (type == kBox) {
+        const float cx = 0.5f * (q[0] + q[3]), cy = 0.5f * (q[1] + q[4]), cz = 0.5f * (q[2] + q[5]);
+        const float ux = fabsf(x - cx) - 0.5f * fabsf(q[3] - q[0]), uy = fabsf(y - cy) - 0.5f * fabsf(q[4] - q[1]),
+                    uz = fabsf(z - cz) - 0.5f * fabsf(q[5] - q[2]);
+        const float ox = fmaxf(ux, 0.0f), oy = fmaxf(uy, 0.0f), oz = fmaxf(uz, 0.0f);
+        const float out = sqrtf(ox * ox + oy * oy + oz * oz), in = fminf(fmaxf(ux, fmaxf(uy, uz)), 0.0f);
+        return -(out + in);
+    }
+    if (type == kPlane) {
+        return -((x - q[0]) * q[3] + (y - q[1]) * q[4] + (z - q[2]) * q[5]);
+    }
+    if (type == kSlab) {
+        const float t = (x - q[0]) * q[3] + (y - q[1]) * q[4] + (z - q[2]) * q[5];
+        return fminf(t - q[6], q[7] - t);
+    }
+    if (type == kInfCyl) {
+        const float dx = x - q[0], dy = y - q[1], dz = z - q[2];
+        const float t = dx * q[3] + dy * q[4] + dz * q[5];
+        const float rx = dx - t * q[3], ry = dy - t * q[4], rz = dz - t * q[5];
+        return q[6] - sqrtf(rx * rx + ry * ry + rz * rz);
+    }
+    if (type == kTorus) {
+        const float dx = x - q[0], dy = y - q[1], dz = z - q[2];
+        const float hh = dx * q[3] + dy * q[4] + dz * q[5];
+        const float rx = dx - hh * q[3], ry = dy - hh * q[4], rz = dz - hh * q[5];
+        const float a = sqrtf(rx * rx + ry * ry + rz * rz) - q[6];
+        return q[7] - sqrtf(a * a + hh * hh);
+    }
+    if (type == kCyl) {   // capped cylinder, I. Quilez's exact form
+        const float bx = q[3] - q[0], by = q[4] - q[1], bz = q[5] - q[2];
+        const float px = x - q[0], py = y - q[1], pz = z - q[2];
+        const float baba = bx * bx + by * by + bz * bz, paba = px * bx + py * by + pz * bz;
+        const float ex = px * baba - bx * paba, ey = py * baba - by * paba, ez = pz * baba - bz * paba;
+        const float u = sqrtf(ex * ex + ey * ey + ez * ez) - q[6] * baba;
+        const float v = fabsf(paba - baba * 0.5f) - baba * 0.5f;
+        const float u2 = u * u, v2 = v * v * baba;
+        const float dd = fmaxf(u, v) < 0.0f ? -fminf(u2, v2) : ((u > 0.0f ? u2 : 0.0f) + (v > 0.0f ? v2 : 0.0f));
+        return -(dd < 0.0f ? -1.0f : 1.0f) * sqrtf(fabsf(dd)) / baba;
+    }
+    if (type == kCone) {   // capped frustum, I. Quilez's exact form
+        const float bx = q[3] - q[0], by = q[4] - q[1], bz = q[5] - q[2];
+        const float px = x - q[0], py = y - q[1], pz = z - q[2];
+        const float ra = q[6], rb = q[7];
+        const float rba = rb - ra, baba = bx * bx + by * by + bz * bz;
+        const float papa = px * px + py * py + pz * pz, paba = (px * bx + py * by + pz * bz) / baba;
+        const float xr = sqrtf(fmaxf(papa - paba * paba * baba, 0.0f));
+        const float cax = fmaxf(0.0f, xr - ((paba < 0.5f) ? ra : rb)), cay = fabsf(paba - 0.5f) - 0.5f;
+        const float k = rba * rba + baba;
+        const float fr = fminf(fmaxf((rba * (xr - ra) + paba * baba) / k, 0.0f), 1.0f);
+        const float cbx = xr - ra - fr * rba, cby = paba - fr;
+        const float sg = (cbx < 0.0f && cay < 0.0f) ? -1.0f : 1.0f;
+        const float d2 = fminf(cax * cax + cay * cay * baba, cbx * cbx + cby * cby * baba);
+        return -sg * sqrtf(d2);
+    }
+    if (type == kEllipsoid) {   // the bound k0 (k0 - 1) / k1, exact on the surface
+        const float dx = x - q[0], dy = y - q[1], dz = z - q[2];
+        const float lx = q[6] * dx + q[7] * dy + q[8] * dz, ly = q[9] * dx + q[10] * dy + q[11] * dz,
+                    lz = q[12] * dx + q[13] * dy + q[14] * dz;
+        const float ax = lx / q[3], ay = ly / q[4], az = lz / q[5];
+        const float bx = ax / q[3], by = ay / q[4], bz = az / q[5];
+        const float k0 = sqrtf(ax * ax + ay * ay + az * az), k1 = sqrtf(bx * bx + by * by + bz * bz);
+        if (k1 < 1e-20f) {
+            return fminf(q[3], fminf(q[4], q[5]));
+        }
+        return -k0 * (k0 - 1.0f) / k1;
+    }
+    return -1e30f;
+}
+
+// (value, gradient): .w the value
+__device__ __forceinline__ float4 sv(float v) {
+    return make_float4(0.0f, 0.0f, 0.0f, v);
+}
+
+// min (sgn 1) or max (sgn -1) of a and b with the crease blend of radius k and the
+// gap closing of radius kgap (v2m_sdf_minmax, :191-273)
+__device__ float4 minmax(float4 a, float4 b, float sgn, float k, float kgap, int grad, int* near) {
+    const float va = sgn * a.w, vb = sgn * b.w;
+    const bool sliver = kgap > 0.0f && va > -kgap && vb > -kgap && va + vb < kgap;
+    if (sliver && !grad) {
+        *near = 1;
+    }
+    float pen = 0.0f, dpen = 0.0f;
+    if (sliver && grad) {
+        const float ga = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z), gb = sqrtf(b.x * b.x + b.y * b.y + b.z * b.z);
+        if (ga > 0.0f && gb > 0.0f) {
+            const float c = (a.x * b.x + a.y * b.y + a.z * b.z) / (ga * gb);
+            const float wg = fminf(1.0f, fmaxf(0.0f, (-c - 0.7f) / 0.25f));
+            const float u = fmaxf(0.0f, 1.0f - (va + vb) / kgap);
+            pen = kgap * wg * u * u;
+            dpen = 2.0f * wg * u;
+        }
+    }
+    float4 r = a;
+    if (k > 0.0f && fabsf(va - vb) < k) {
+        if (!grad) {
+            *near = 1;
+        } else {
+            const float ga = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z), gb = sqrtf(b.x * b.x + b.y * b.y + b.z * b.z);
+            const float c = ga > 0.0f && gb > 0.0f ? (a.x * b.x + a.y * b.y + a.z * b.z) / (ga * gb) : 0.0f;
+            k *= fminf(1.0f, fmaxf(0.0f, (0.995f - c) / 0.095f));
+        }
+    }
+    if (k > 0.0f && fabsf(va - vb) < k) {
+        const float h = 0.5f + 0.5f * (vb - va) / k;
+        r.w = sgn * (vb * (1.0f - h) + va * h - k * h * (1.0f - h));
+        r.x = h * a.x + (1.0f - h) * b.x;
+        r.y = h * a.y + (1.0f - h) * b.y;
+        r.z = h * a.z + (1.0f - h) * b.z;
+    } else if (vb < va) {
+        r = b;
+    }
+    r.w -= sgn * pen;
+    if (dpen > 0.0f) {
+        r.x += dpen * (a.x + b.x);
+        r.y += dpen * (a.y + b.y);
+        r.z += dpen * (a.z + b.z);
+    }
+    return r;
+}
+
+__host__ __device__ __forceinline__ bool is_minmax(int op) {
+    return op == kMax || op == kMin || op == kCMax || op == kCMin;
+}
+
+// Label l's field at grid-mm point p and (wantg) its gradient, (gx, gy, gz, s)
+// (v2m_sdf_evalv, :275-478): the postfix code of the point's brick on a stack whose
+// top four entries live in registers (t0 the top) and the rest in a small spill
+// array, values first and the gradients only where a gap or crease needs them.
+__device__ __noinline__ float4 evalv(const float* __restrict__ prog, int l, float px, float py, float pz, int wantg) {
+    const int N = (int)prog[0];
+    if (l < 0 || l >= N) {
+        return sv(-1e30f);
+    }
+    const float x = px + prog[2], y = py + prog[3], z = pz + prog[4];
+    int start = (int)prog[5 + l];
+    const int T = (int)prog[8 + N];
+    if (T > 0) {
+        const int nbx = (int)prog[T], nby = (int)prog[T + 1], nbz = (int)prog[T + 2];
+        const float vx = prog[T + 3];
+        int bi = (int)floorf(px / vx + 0.5f) >> 3, bj = (int)floorf(py / vx + 0.5f) >> 3,
+            bk = (int)floorf(pz / vx + 0.5f) >> 3;
+        bi = bi < 0 ? 0 : (bi >= nbx ? nbx - 1 : bi);
+        bj = bj < 0 ? 0 : (bj >= nby ? nby - 1 : bj);
+        bk = bk < 0 ? 0 : (bk >= nbz ? nbz - 1 : bk);
+        start = (int)prog[(int)prog[T + 4 + bi + nbx * (bj + nby * bk)] + l];
+    }
+    const float kgap = prog[6 + N], kblend = prog[5 + N], margin = prog[7 + N];
+    float4 t0 = sv(0.0f), t1 = t0, t2 = t0, t3 = t0, spill[kStack - 4];
+    int sp = 0;
+    int grad = wantg != 0, near = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        sp = 0;
+        int pc = start;
+        near = 0;
+        for (int guard = 0; guard < 100000; ++guard) {
+            const int op = (int)prog[pc];
+            if (op == kEnd) {
+                break;
+            }
+            if (is_minmax(op)) {   // t1 op t0 -> t0, pop
+                if (sp > 1) {
+                    t0 = minmax(t1, t0, (op == kMax || op == kCMax) ? -1.0f : 1.0f, kblend, op >= kCMax ? 0.0f : kgap, grad,
+                                &near);
+                    t1 = t2;
+                    t2 = t3;
+                    if (sp > 4) {
+                        t3 = spill[sp - 5];
+                    }
+                    --sp;
+                }
+                pc += 1;
+                continue;
+            }
+            if (op == kNeg) {
+                t0 = make_float4(-t0.x, -t0.y, -t0.z, -t0.w);
+                pc += 1;
+                continue;
+            }
+            if (op == kAdd) {   // t1 + t0 -> t0, pop
+                if (sp > 1) {
+                    t0.w += t1.w;
+                    t0.x += t1.x;
+                    t0.y += t1.y;
+                    t0.z += t1.z;
+                    t1 = t2;
+                    t2 = t3;
+                    if (sp > 4) {
+                        t3 = spill[sp - 5];
+                    }
+                    --sp;
+                }
+                pc += 1;
+                continue;
+            }
+            if (op == kScale) {
+                const float fs = prog[pc + 1];
+                t0 = make_float4(t0.x * fs, t0.y * fs, t0.z * fs, t0.w * fs);
+                pc += 2;
+                continue;
+            }
+            float4 nv = sv(0.0f);
+            if (op == kPrim) {   // the value and (grad) this primitive's central differences
+                const int type = (int)prog[pc + 1];
+                const float* q = prog + pc + 2;
+                const float e = fmaxf(1e-4f * (fabsf(q[0]) + fabsf(q[1]) + fabsf(q[2]) + 1.0f), 1e-5f);
+                const int ne = grad ? 7 : 1;
+                float fv[7];
+                for (int k = 0; k < ne; ++k) {
+                    const float dd = k == 0 ? 0.0f : ((k & 1) ? e : -e);
+                    const int ax = (k - 1) >> 1;
+                    fv[k] = prim(type, q, x + (ax == 0 ? dd : 0.0f), y + (ax == 1 ? dd : 0.0f), z + (ax == 2 ? dd : 0.0f));
+                }
+                nv.w = fv[0];
+                if (grad) {
+                    nv.x = (fv[1] - fv[2]) / (2.0f * e);
+                    nv.y = (fv[3] - fv[4]) / (2.0f * e);
+                    nv.z = (fv[5] - fv[6]) / (2.0f * e);
+                }
+                pc += 2 + nparam(type);
+            } else if (op == kBBox) {   // outside the box (+ margin): the bound, its code skipped
+                const float* q = prog + pc + 1;
+                if (!(x < q[0] - margin || y < q[1] - margin || z < q[2] - margin || x > q[3] + margin ||
+                      y > q[4] + margin || z > q[5] + margin)) {
+                    pc += 9;
+                    continue;
+                }
+                nv.w = q[7] * margin;
+                pc += 9 + (int)q[6];
+            } else if (op == kMConst) {
+                nv.w = prog[pc + 1] * margin;
+                pc += 2;
+            } else if (op == kConst) {
+                nv.w = prog[pc + 1];
+                pc += 2;
+            } else {
+                break;   // a malformed program
+            }
+            const int op1 = (int)prog[pc], neg = op1 == kNeg;
+            const int op2 = neg ? (int)prog[pc + 1] : op1;
+            if (sp > 0 && is_minmax(op2)) {   // combined at once: "x MIN", "x NEG MIN"
+                if (neg) {
+                    nv = make_float4(-nv.x, -nv.y, -nv.z, -nv.w);
+                }
+                t0 = minmax(t0, nv, (op2 == kMax || op2 == kCMax) ? -1.0f : 1.0f, kblend, op2 >= kCMax ? 0.0f : kgap, grad,
+                            &near);
+                pc += 1 + neg;
+                continue;
+            }
+            if (sp < kStack) {   // push
+                if (sp >= 4) {
+                    spill[sp - 4] = t3;
+                }
+                t3 = t2;
+                t2 = t1;
+                t1 = t0;
+                t0 = nv;
+                ++sp;
+            }
+        }
+        if (grad || !near) {
+            break;
+        }
+        grad = 1;
+    }
+    return sp > 0 ? t0 : sv(-1e30f);
+}
+}  // namespace sdf
+
+// shape fields: clamp(0.5 + s_l / w) (v2m_phi_f, src/opencl/v2m_seed_body.cl:33-36)
+__device__ __noinline__ float shape_phi(const Field* fp, int l, P3 p) {
+    const float* prog = fp->gTW;
+    return fminf(1.0f, fmaxf(0.0f, 0.5f + sdf::evalv(prog, l, p.x, p.y, p.z, 0).w / prog[1]));
+}
+
+// shape fields: (grad psi, psi) of psi_ab = (s_a - s_b) / w (src/opencl/v2m_seed_body.cl:182-221)
+__device__ __noinline__ float4 shape_psi(const Field* fp, int a, int b, P3 p, int grad) {
+    const float* prog = fp->gTW;
+    const float w = prog[1];
+    const float4 sa = sdf::evalv(prog, a, p.x, p.y, p.z, grad), sb = sdf::evalv(prog, b, p.x, p.y, p.z, grad);
+    return make_float4((sa.x - sb.x) / w, (sa.y - sb.y) / w, (sa.z - sb.z) / w, (sa.w - sb.w) / w);
+}
+
+// v2m_phi_f, src/opencl/v2m_seed_body.cl:33-39: v2m_phi_at (src/opencl/v2m_grid_body.cl:
+// 189-200), or the shape fields' clamp
 __device__ __forceinline__ float phi_point(const Field& f, int l, P3 p) {
+    if (f.gm < 0) {
+        return shape_phi(f.self, l, p);
+    }
     const float u = over_vx(f, p.x), v = over_vy(f, p.y), w = over_vz(f, p.z);
     const int i0 = (int)floorf(u), j0 = (int)floorf(v), k0 = (int)floorf(w);
     float c[8];
@@ -364,6 +677,9 @@ __device__ __forceinline__ int2 gray_owner(const Field& f, float I) {
 // v2m_psi, src/opencl/v2m_seed_body.cl:179-205, through a cell cache (the gray-scale
 // form for adjacent labels)
 __device__ __forceinline__ float psi_at(const Field& f, int a, int b, P3 p, PsiCell& pc) {
+    if (f.gm < 0) {
+        return shape_psi(f.self, a, b, p, 0).w;
+    }
     if (gray_pair(f, a, b)) {
         return gray_psi(f, a, b, gray_at(f, p)).w;
     }
@@ -376,6 +692,9 @@ __device__ __forceinline__ float psi_at(const Field& f, int a, int b, P3 p, PsiC
 // v2m_psi_grad, src/opencl/v2m_seed_body.cl:211-247: (gx, gy, gz, psi), through a
 // cell cache (the gray-scale form for adjacent labels)
 __device__ __forceinline__ float4 psi_grad_at(const Field& f, int a, int b, P3 p, PsiCell& pc) {
+    if (f.gm < 0) {
+        return shape_psi(f.self, a, b, p, 1);
+    }
     if (gray_pair(f, a, b)) {
         return gray_psi(f, a, b, gray_at(f, p));
     }
@@ -474,9 +793,37 @@ __device__ __forceinline__ float label_phi(const Field& f, const PointCell& pc, 
 }
 
 // v2m_label_of (voxmode 0), src/opencl/v2m_seed_body.cl:116-175: (best, second)
+// shape fields: the brick's labels ranked by s itself (src/opencl/v2m_seed_body.cl:137-160)
+__device__ __noinline__ int2 shape_owner(const Field* fp, P3 p) {
+    const Field f = *fp;
+    const BrickLabels bl = brick_labels_at(f, p);
+    int best = brick_label(f, bl, 0), sec = kNoLab;
+    if (bl.n == 1) {
+        return make_int2(best, kNoLab);
+    }
+    float pb = -1e30f, ps = pb;
+    for (int s = 0; s < bl.n; ++s) {
+        const int l = brick_label(f, bl, s);
+        const float v = sdf::evalv(f.gTW, l, p.x, p.y, p.z, 0).w;
+        if (v > pb) {
+            ps = pb;
+            sec = best;
+            pb = v;
+            best = l;
+        } else if (v > ps) {
+            ps = v;
+            sec = l;
+        }
+    }
+    return make_int2(best, sec);
+}
+
 __device__ __forceinline__ int2 owner_at(const Field& f, P3 p) {
     if (f.gm > 0) {
         return gray_owner(f, gray_at(f, p).w);
+    }
+    if (f.gm < 0) {
+        return shape_owner(f.self, p);
     }
     const BrickLabels bl = brick_labels_at(f, p);
     int best = brick_label(f, bl, 0), sec = kNoLab;
@@ -517,6 +864,13 @@ __device__ __forceinline__ OwnerPsi owner_psi_at(const Field& f, int a, P3 p) {
         r.best = o.x;
         r.sec = o.y;
         r.g = (o.x == a && o.y != kNoLab) ? gray_psi(f, a, o.y, ig) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return r;
+    }
+    if (f.gm < 0) {   // the owner by the values, then psi of (a, second) with the gradients
+        const int2 o = shape_owner(f.self, p);
+        r.best = o.x;
+        r.sec = o.y;
+        r.g = (o.x == a && o.y != kNoLab) ? shape_psi(f.self, a, o.y, p, 1) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return r;
     }
     const BrickLabels bl = brick_labels_at(f, p);
@@ -662,6 +1016,26 @@ __device__ __noinline__ P3ok onto_junction(const Field* fp, int a, int b, int c,
     PsiCell pab, pac;   // the a|b and a|c cells: each projection ends where its cache points
     psi_cell_init(pab);
     psi_cell_init(pac);
+    if (f.gm < 0) {
+        // shape fields: of the three pairs' constraints, the two with the most
+        // independent gradients, their shared label first (src/opencl/v2m_seed_body.cl:345-375)
+        const float4 gab = shape_psi(fp, a, b, q, 1), gac = shape_psi(fp, a, c, q, 1), gbc = shape_psi(fp, b, c, q, 1);
+        const float xa[3] = { gab.y * gac.z - gab.z * gac.y, gab.z * gac.x - gab.x * gac.z, gab.x * gac.y - gab.y * gac.x };
+        const float xb[3] = { gab.y * gbc.z - gab.z * gbc.y, gab.z * gbc.x - gab.x * gbc.z, gab.x * gbc.y - gab.y * gbc.x };
+        const float xc[3] = { gac.y * gbc.z - gac.z * gbc.y, gac.z * gbc.x - gac.x * gbc.z, gac.x * gbc.y - gac.y * gbc.x };
+        const float na = xa[0] * xa[0] + xa[1] * xa[1] + xa[2] * xa[2], nb = xb[0] * xb[0] + xb[1] * xb[1] + xb[2] * xb[2],
+                    nc = xc[0] * xc[0] + xc[1] * xc[1] + xc[2] * xc[2];
+        if (nb > na && nb >= nc) {   // pivot b
+            const int t = a;
+            a = b;
+            b = t;
+        } else if (nc > na && nc > nb) {   // pivot c
+            const int t = a;
+            a = c;
+            c = b;
+            b = t;
+        }
+    }
     for (int it = 0; it < 3; ++it) {
         const P3ok r1 = onto_interface_at(f, a, b, q, rad, pab);
         if (!r1.ok) {
@@ -844,7 +1218,7 @@ __device__ __noinline__ int competing_label_near(const Field* fp, int a, int b, 
     for (int s = 0; s < bl.n; ++s) {
         const int l = brick_label(f, bl, s);
         if (l != a && l != b) {
-            const float w = label_phi(f, cell, s, l);
+            const float w = f.gm < 0 ? phi_point(f, l, q) : label_phi(f, cell, s, l);
             if (w > pc) {
                 pc = w;
                 c = l;
@@ -877,7 +1251,7 @@ __device__ __noinline__ int labels_dominate_near(const Field* fp, int a, int b, 
     const PointCell cell = point_cell(f, q, bl);
     for (int s = 0; s < bl.n; ++s) {
         const int l = brick_label(f, bl, s);
-        if (l != a && l != b && l != c && label_phi(f, cell, s, l) > mine + 0.02f) {
+        if (l != a && l != b && l != c && (f.gm < 0 ? phi_point(f, l, q) : label_phi(f, cell, s, l)) > mine + 0.02f) {
             return 0;
         }
     }
@@ -940,7 +1314,7 @@ __device__ __forceinline__ float lean_tail(const Field& f, float h, float maxste
             for (int s = 0; s < bl.n; ++s) {
                 const int l = brick_label(f, bl, s);
                 if (l != a && l != b) {
-                    const float w = label_phi(f, cell, s, l);
+                    const float w = f.gm < 0 ? phi_point(f, l, q) : label_phi(f, cell, s, l);
                     if (w > pc) {
                         pc = w;
                         cc = l;

@@ -1056,7 +1056,14 @@ __device__ __forceinline__ void wave_append(bool flag, int i, int32_t* count, in
     }
 }
 
+// kPlain: every grid of the launch is a label volume (gm == 0). The compiler then
+// drops the gray-scale and shape-field branches of the field queries, and the hot
+// kernels keep their register budget; other grids take the kPlain = false variants.
+template <bool kPlain>
 __device__ __forceinline__ void k_move_easy_body(Field f, MoveArgs ma, Ctl* __restrict__ ctl, int n, const float4* __restrict__ F, float4* __restrict__ X, float4* __restrict__ V, uint8_t* __restrict__ typ, uint32_t* __restrict__ part, const uint16_t* __restrict__ lab, float* __restrict__ mv, float* __restrict__ pw, int32_t* __restrict__ queue, int32_t* __restrict__ queue_int, int32_t* __restrict__ queue_rest, int bid, int nblk) {
+    if (kPlain) {
+        __builtin_assume(f.gm == 0);
+    }
     const int g = bid * kBlock + threadIdx.x;
     int i = 0;
     bool surf = false, inner = false, hard = false;   // the three queues
@@ -1125,8 +1132,10 @@ __device__ __forceinline__ void k_move_easy_body(Field f, MoveArgs ma, Ctl* __re
     wave_append(hard, i, &ctl->nmove_rest, queue_rest);
 }
 
+template <bool kPlain>
 __global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl* __restrict__ ctl, int n, const float4* __restrict__ F, float4* __restrict__ X, float4* __restrict__ V, uint8_t* __restrict__ typ, uint32_t* __restrict__ part, const uint16_t* __restrict__ lab, float* __restrict__ mv, float* __restrict__ pw, int32_t* __restrict__ queue, int32_t* __restrict__ queue_int, int32_t* __restrict__ queue_rest) {
-    k_move_easy_body(f, ma, ctl, n, F, X, V, typ, part, lab, mv, pw, queue, queue_int, queue_rest, blockIdx.x, gridDim.x);
+    k_move_easy_body<kPlain>(f, ma, ctl, n, F, X, V, typ, part, lab, mv, pw, queue, queue_int, queue_rest, blockIdx.x,
+                             gridDim.x);
 }
 
 #ifndef TN_REST_LANES
@@ -1216,8 +1225,11 @@ __global__ __launch_bounds__(kBlock) void k_move_voxel(Field f, MoveArgs ma, con
 // whole trapped step (junction nodes and the handed-back ones). Each node's move
 // reads and writes only that node's state, so the passes over disjoint queues may
 // run concurrently.
-template <int kMode>
+template <int kMode, bool kPlain>
 __device__ __forceinline__ void k_move_body(Field f, MoveArgs ma, Ctl* __restrict__ ctl, const int32_t* __restrict__ queue, const float4* __restrict__ F, float4* __restrict__ X, float4* __restrict__ V, uint8_t* __restrict__ typ, uint32_t* __restrict__ part, const uint16_t* __restrict__ lab, float* __restrict__ mv, float* __restrict__ pw, const int32_t* __restrict__ qcount, int32_t* __restrict__ defer_count, int32_t* __restrict__ defer_queue, int bid, int nblk) {
+    if (kPlain) {
+        __builtin_assume(f.gm == 0);
+    }
     // A fixed grid strides over the queue, whose length only the device knows. A
     // short queue's nodes follow long, divergent paths (projections onto interfaces
     // and junction curves), and a wave executes every path its lanes take, so they
@@ -1252,9 +1264,10 @@ __device__ __forceinline__ void k_move_body(Field f, MoveArgs ma, Ctl* __restric
     }
 }
 
-template <int kMode>
+template <int kMode, bool kPlain>
 __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArgs ma, Ctl* __restrict__ ctl, const int32_t* __restrict__ queue, const float4* __restrict__ F, float4* __restrict__ X, float4* __restrict__ V, uint8_t* __restrict__ typ, uint32_t* __restrict__ part, const uint16_t* __restrict__ lab, float* __restrict__ mv, float* __restrict__ pw, const int32_t* __restrict__ qcount, int32_t* __restrict__ defer_count, int32_t* __restrict__ defer_queue) {
-    k_move_body<kMode>(f, ma, ctl, queue, F, X, V, typ, part, lab, mv, pw, qcount, defer_count, defer_queue, blockIdx.x, gridDim.x);
+    k_move_body<kMode, kPlain>(f, ma, ctl, queue, F, X, V, typ, part, lab, mv, pw, qcount, defer_count, defer_queue,
+                               blockIdx.x, gridDim.x);
 }
 
 constexpr int kMoveGrid = 2048;       // blocks of a lean k_move at most
@@ -1569,6 +1582,7 @@ __global__ __launch_bounds__(kBlock) void kg_force(const PlanIter* __restrict__ 
     k_force_body(P.X, P.typ, P.nbrT, P.nnb, P.fscale, P.fsurf, P.n, P.F, bid_, nb_);
 }
 
+template <bool kPlain>
 __global__ __launch_bounds__(kBlock) void kg_move_easy(const PlanIter* __restrict__ tab,
                                                        const int32_t* __restrict__ bplan,
                                                        const int32_t* __restrict__ bfirst) {
@@ -1577,13 +1591,13 @@ __global__ __launch_bounds__(kBlock) void kg_move_easy(const PlanIter* __restric
         k_move_voxel_body(P.f, P.ma, P.ctl, P.n, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv, P.pw, bid_);
         return;
     }
-    k_move_easy_body(P.f, P.ma, P.ctl, P.n, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv, P.pw, P.mq, P.mqi, P.mq2, bid_,
-                     nb_);
+    k_move_easy_body<kPlain>(P.f, P.ma, P.ctl, P.n, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv, P.pw, P.mq, P.mqi, P.mq2,
+                             bid_, nb_);
 }
 
 // the four move passes: kQueue 0 = the general queue, 1 = interior (lean), 2 =
 // interface (lean), 3 = the handed-back nodes (general)
-template <int kQueue>
+template <int kQueue, bool kPlain>
 __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void kg_move(const PlanIter* __restrict__ tab,
                                                                  const int32_t* __restrict__ bplan,
                                                                  const int32_t* __restrict__ bfirst) {
@@ -1594,7 +1608,7 @@ __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void kg_move(const PlanIter*
                                        : (kQueue == 1 ? &P.ctl->nmove_int
                                                       : (kQueue == 2 ? &P.ctl->nmove : &P.ctl->nmove_def));
     const bool lean = kMode != kGeneral;
-    k_move_body<kMode>(P.f, P.ma, P.ctl, queue, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv, P.pw, count,
+    k_move_body<kMode, kPlain>(P.f, P.ma, P.ctl, queue, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv, P.pw, count,
                        lean ? &P.ctl->nmove_def : nullptr, lean ? P.mqd : nullptr, bid_, nb_);
 }
 

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <thread>
+#include <type_traits>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -223,9 +224,9 @@ tnd::Field upload_grid(const tn_grid_view* g, DevMem& mem) {
     f.bl_slot = mem.upload(g->bl_slot, nb);
     f.phi = mem.upload(g->phi, (size_t)g->nslot * TN_SLOT);
     f.h = mem.upload(g->h, nv);
-    f.gm = g->gm > 0 ? g->gm : 0;
-    f.gI = f.gm ? mem.upload(g->gI, nv) : nullptr;
-    f.gTW = f.gm ? mem.upload(g->gTW, 2 * (size_t)f.gm) : nullptr;
+    f.gm = g->gm;
+    f.gI = f.gm > 0 ? mem.upload(g->gI, nv) : nullptr;
+    f.gTW = f.gm > 0 ? mem.upload(g->gTW, 2 * (size_t)f.gm) : (f.gm < 0 ? mem.upload(g->gTW, (size_t)g->ngTW) : nullptr);
     tnd::Field* self = mem.alloc<tnd::Field>(1);
     f.self = self;
     TN_HIP(hipMemcpy(self, &f, sizeof f, hipMemcpyHostToDevice));
@@ -513,47 +514,56 @@ struct tn_plan {
             enqueue_iteration_tail(nb);
             return;
         }
-        tnk::k_move_easy<<<nb, tnk::kBlock, 0, stream>>>(f, ma, dCtl, n, dF, dX, dV, dTyp, dPart, dLab, dMv,
-                                                         dPw, dMoveQ, dMoveQi, dMoveQ2);
-        timer.end(KC_MOVE_EASY, t);
-        // the queued nodes: three passes over disjoint queues at once, each bounded by
-        // its slowest node, then the general pass over the nodes handed back
-        const unsigned g1 = std::min<unsigned>(nb, tnk::kMoveGrid), g2 = std::min<unsigned>(nb, tnk::kMoveRestGrid);
-        const hipStream_t s0 = overlap_moves ? side[0] : stream, s1 = overlap_moves ? side[1] : stream;
-        if (overlap_moves) {
-            TN_HIP(hipEventRecord(ev_easy, stream));
-            TN_HIP(hipStreamWaitEvent(s0, ev_easy, 0));
-        }
-        t = timer.begin(s0);
-        tnk::k_move<tnd::kGeneral><<<g2, tnk::kBlock, 0, s0>>>(f, ma, dCtl, dMoveQ2, dF, dX, dV, dTyp, dPart, dLab, dMv,
-                                                               dPw, &dCtl->nmove_rest, nullptr, nullptr);
-        timer.end(KC_MOVE_REST, t, s0);
-        if (overlap_moves) {
-            TN_HIP(hipEventRecord(ev_rest, s0));
-            TN_HIP(hipStreamWaitEvent(s1, ev_easy, 0));
-        }
-        t = timer.begin(s1);
-        tnk::k_move<tnd::kLeanInterior><<<g1, tnk::kBlock, 0, s1>>>(f, ma, dCtl, dMoveQi, dF, dX, dV, dTyp, dPart, dLab,
-                                                                    dMv, dPw, &dCtl->nmove_int, &dCtl->nmove_def,
-                                                                    dMoveQd);
-        timer.end(KC_MOVE_INT, t, s1);
-        if (overlap_moves) {
-            TN_HIP(hipEventRecord(ev_int, s1));
-        }
-        t = timer.begin();
-        tnk::k_move<tnd::kLeanInterface><<<g1, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQ, dF, dX, dV, dTyp, dPart,
-                                                                         dLab, dMv, dPw, &dCtl->nmove, &dCtl->nmove_def,
-                                                                         dMoveQd);
-        timer.end(KC_MOVE, t);
-        if (overlap_moves) {
-            TN_HIP(hipStreamWaitEvent(stream, ev_int, 0));
-        }
-        t = timer.begin();
-        tnk::k_move<tnd::kGeneral><<<g2, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQd, dF, dX, dV, dTyp, dPart, dLab,
-                                                                   dMv, dPw, &dCtl->nmove_def, nullptr, nullptr);
-        timer.end(KC_MOVE_DEF, t);
-        if (overlap_moves) {
-            TN_HIP(hipStreamWaitEvent(stream, ev_rest, 0));
+        // (label volumes: the move kernels compiled without the other field modes)
+        auto moves = [&](auto plain) {
+            constexpr bool kP = decltype(plain)::value;
+            tnk::k_move_easy<kP><<<nb, tnk::kBlock, 0, stream>>>(f, ma, dCtl, n, dF, dX, dV, dTyp, dPart, dLab, dMv,
+                                                             dPw, dMoveQ, dMoveQi, dMoveQ2);
+            timer.end(KC_MOVE_EASY, t);
+            // the queued nodes: three passes over disjoint queues at once, each bounded by
+            // its slowest node, then the general pass over the nodes handed back
+            const unsigned g1 = std::min<unsigned>(nb, tnk::kMoveGrid), g2 = std::min<unsigned>(nb, tnk::kMoveRestGrid);
+            const hipStream_t s0 = overlap_moves ? side[0] : stream, s1 = overlap_moves ? side[1] : stream;
+            if (overlap_moves) {
+                TN_HIP(hipEventRecord(ev_easy, stream));
+                TN_HIP(hipStreamWaitEvent(s0, ev_easy, 0));
+            }
+            t = timer.begin(s0);
+            tnk::k_move<tnd::kGeneral, kP><<<g2, tnk::kBlock, 0, s0>>>(f, ma, dCtl, dMoveQ2, dF, dX, dV, dTyp, dPart, dLab, dMv,
+                                                                   dPw, &dCtl->nmove_rest, nullptr, nullptr);
+            timer.end(KC_MOVE_REST, t, s0);
+            if (overlap_moves) {
+                TN_HIP(hipEventRecord(ev_rest, s0));
+                TN_HIP(hipStreamWaitEvent(s1, ev_easy, 0));
+            }
+            t = timer.begin(s1);
+            tnk::k_move<tnd::kLeanInterior, kP><<<g1, tnk::kBlock, 0, s1>>>(f, ma, dCtl, dMoveQi, dF, dX, dV, dTyp, dPart, dLab,
+                                                                        dMv, dPw, &dCtl->nmove_int, &dCtl->nmove_def,
+                                                                        dMoveQd);
+            timer.end(KC_MOVE_INT, t, s1);
+            if (overlap_moves) {
+                TN_HIP(hipEventRecord(ev_int, s1));
+            }
+            t = timer.begin();
+            tnk::k_move<tnd::kLeanInterface, kP><<<g1, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQ, dF, dX, dV, dTyp, dPart,
+                                                                             dLab, dMv, dPw, &dCtl->nmove, &dCtl->nmove_def,
+                                                                             dMoveQd);
+            timer.end(KC_MOVE, t);
+            if (overlap_moves) {
+                TN_HIP(hipStreamWaitEvent(stream, ev_int, 0));
+            }
+            t = timer.begin();
+            tnk::k_move<tnd::kGeneral, kP><<<g2, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQd, dF, dX, dV, dTyp, dPart, dLab,
+                                                                       dMv, dPw, &dCtl->nmove_def, nullptr, nullptr);
+            timer.end(KC_MOVE_DEF, t);
+            if (overlap_moves) {
+                TN_HIP(hipStreamWaitEvent(stream, ev_rest, 0));
+            }
+        };
+        if (f.gm == 0) {
+            moves(std::true_type{});
+        } else {
+            moves(std::false_type{});
         }
         enqueue_iteration_tail(nb);
     }
@@ -715,13 +725,84 @@ int pick_device(int32_t device) {
     return d;
 }
 
+// A shape program (the layout of src/opencl/v2m_sdf_body.cl:1-45) whose every read
+// the interpreter can make lies inside its n values: the header, the brick table,
+// and each label's code walked op by op to its end, with every bounding-box skip
+// landing on an op of the same code.
+void check_program(const float* prog, int64_t n) {
+    auto at = [&](int64_t i) -> double {
+        arg_check(i >= 0 && i < n, "shape program: a read beyond its " + std::to_string(n) + " values");
+        return (double)prog[i];
+    };
+    auto idx = [&](int64_t i) -> int64_t {
+        const double v = at(i);
+        arg_check(v >= 0.0 && v < (double)n && v == std::floor(v), "shape program: a bad offset");
+        return (int64_t)v;
+    };
+    arg_check(n >= 9, "shape program: too short");
+    const double dN = at(0);
+    arg_check(dN >= 1.0 && dN <= 65535.0 && dN == std::floor(dN), "shape program: bad label count");
+    const int N = (int)dN;
+    at(8 + N);
+    auto code = [&](int64_t start) {
+        std::vector<int64_t> ops, skips;
+        int64_t pc = start;
+        for (int guard = 0;; ++guard) {
+            arg_check(guard < 100000, "shape program: a code without its end");
+            const int op = (int)at(pc);
+            ops.push_back(pc);
+            if (op == tnd::sdf::kEnd) {
+                break;
+            }
+            if (op == tnd::sdf::kPrim) {
+                const int type = (int)at(pc + 1);
+                const int np = tnd::sdf::nparam(type);
+                at(pc + 1 + std::max(np, 1));
+                pc += 2 + np;
+            } else if (op == tnd::sdf::kBBox) {
+                const double len = at(pc + 7);
+                at(pc + 8);
+                arg_check(len >= 0.0 && len == std::floor(len), "shape program: a bad skip");
+                skips.push_back(pc + 9 + (int64_t)len);
+                pc += 9;
+            } else if (op == tnd::sdf::kMConst || op == tnd::sdf::kConst || op == tnd::sdf::kScale) {
+                at(pc + 1);
+                pc += 2;
+            } else {
+                arg_check(tnd::sdf::is_minmax(op) || op == tnd::sdf::kNeg || op == tnd::sdf::kAdd,
+                          "shape program: an unknown op");
+                pc += 1;
+            }
+        }
+        for (int64_t t : skips) {
+            arg_check(std::binary_search(ops.begin(), ops.end(), t), "shape program: a skip off the code");
+        }
+    };
+    for (int l = 0; l < N; ++l) {
+        code(idx(5 + l));
+    }
+    const int64_t T = (int64_t)at(8 + N);
+    if (T > 0) {
+        const int64_t nbx = (int64_t)at(T), nby = (int64_t)at(T + 1), nbz = (int64_t)at(T + 2);
+        arg_check(nbx >= 1 && nby >= 1 && nbz >= 1 && at(T + 3) > 0.0, "shape program: a bad brick table");
+        at(T + 3 + nbx * nby * nbz);
+        for (int64_t b = 0; b < nbx * nby * nbz; ++b) {
+            const int64_t e = idx(T + 4 + b);
+            for (int l = 0; l < N; ++l) {
+                code(idx(e + l));
+            }
+        }
+    }
+}
+
 void check_grid(const tn_grid_view* g) {
     arg_check(g != nullptr, "null grid");
-    if (g->gm < 0) {
-        throw TnError(TN_ERR_UNSUPPORTED, "shape fields (gm < 0) are not run on the device");
-    }
-    arg_check(g->gm == 0 || (g->gI && g->gTW && g->ngTW >= 2 * (int64_t)g->gm),
+    arg_check(g->gm <= 0 || (g->gI && g->gTW && g->ngTW >= 2 * (int64_t)g->gm),
               "grid: a gray-scale grid (gm > 0) needs gI and 2 gm values in gTW");
+    if (g->gm < 0) {
+        arg_check(g->gTW && g->ngTW >= 9, "grid: a shape grid (gm < 0) needs its program in gTW");
+        check_program(g->gTW, g->ngTW);
+    }
     arg_check(g->nx > 0 && g->ny > 0 && g->nz > 0, "grid: empty");
     arg_check(g->nbx == (g->nx + 7) / 8 && g->nby == (g->ny + 7) / 8 && g->nbz == (g->nz + 7) / 8,
               "grid: brick counts must be ceil(n / 8) per axis");
@@ -946,6 +1027,10 @@ int64_t junction_pass(const tn_grid_view* g, const tnd::Field& f, const tnk::See
 
 Seeds run_seeding(const tn_grid_view* g, const uint8_t* grade, const tn_relax_params* prm, int dev, DevMem& mem) {
     check_grid(g);
+    if (g->gm < 0) {
+        throw TnError(TN_ERR_UNSUPPORTED, "seeding a shape grid (gm < 0): its seeds start with the pinned feature "
+                                          "nodes (add_feature_nodes), which are made on the host");
+    }
     arg_check(prm != nullptr && grade != nullptr, "seeding: null argument");
     arg_check(prm->nseed >= 1 && prm->nseed <= 64, "seeding: nseed out of range");
     arg_check(prm->jseed >= 0.0f && std::isfinite(prm->jseed), "seeding: jseed must be finite and >= 0");
@@ -1024,6 +1109,10 @@ template <class Plans>
 void run_grouped(Plans& plans, int64_t& launches, int64_t& steps) {
     const int count = (int)plans.size();
     hipStream_t bs = plans[0]->stream;
+    bool all_plain = true;   // every grid a label volume
+    for (int k = 0; k < count; ++k) {
+        all_plain = all_plain && plans[k]->f.gm == 0;
+    }
     std::vector<hipStream_t> own(count);
     for (int k = 0; k < count; ++k) {
         own[k] = plans[k]->stream;
@@ -1109,11 +1198,19 @@ void run_grouped(Plans& plans, int64_t& launches, int64_t& steps) {
         using namespace tnk;
         kg_iter_begin<<<nblocks[kShOne], 64, 0, bs>>>(dtab, bplan[kShOne], bfirst[kShOne]);
         kg_force<<<nblocks[kShNodes], kBlock, 0, bs>>>(dtab, bplan[kShNodes], bfirst[kShNodes]);
-        kg_move_easy<<<nblocks[kShNodes], kBlock, 0, bs>>>(dtab, bplan[kShNodes], bfirst[kShNodes]);
-        kg_move<0><<<nblocks[kShGen], kBlock, 0, bs>>>(dtab, bplan[kShGen], bfirst[kShGen]);
-        kg_move<1><<<nblocks[kShLean], kBlock, 0, bs>>>(dtab, bplan[kShLean], bfirst[kShLean]);
-        kg_move<2><<<nblocks[kShLean], kBlock, 0, bs>>>(dtab, bplan[kShLean], bfirst[kShLean]);
-        kg_move<3><<<nblocks[kShGen], kBlock, 0, bs>>>(dtab, bplan[kShGen], bfirst[kShGen]);
+        auto moves = [&](auto plain) {
+            constexpr bool kP = decltype(plain)::value;
+            kg_move_easy<kP><<<nblocks[kShNodes], kBlock, 0, bs>>>(dtab, bplan[kShNodes], bfirst[kShNodes]);
+            kg_move<0, kP><<<nblocks[kShGen], kBlock, 0, bs>>>(dtab, bplan[kShGen], bfirst[kShGen]);
+            kg_move<1, kP><<<nblocks[kShLean], kBlock, 0, bs>>>(dtab, bplan[kShLean], bfirst[kShLean]);
+            kg_move<2, kP><<<nblocks[kShLean], kBlock, 0, bs>>>(dtab, bplan[kShLean], bfirst[kShLean]);
+            kg_move<3, kP><<<nblocks[kShGen], kBlock, 0, bs>>>(dtab, bplan[kShGen], bfirst[kShGen]);
+        };
+        if (all_plain) {
+            moves(std::true_type{});
+        } else {
+            moves(std::false_type{});
+        }
         kg_stats<<<nblocks[kShParts], kBlock, 0, bs>>>(dtab, bplan[kShParts], bfirst[kShParts]);
         kg_refresh_list<<<nblocks[kShNodes], kBlock, 0, bs>>>(dtab, bplan[kShNodes], bfirst[kShNodes]);
         kg_truss_node<<<nblocks[kShRefresh], kWave, 0, bs>>>(dtab, bplan[kShRefresh], bfirst[kShRefresh]);
