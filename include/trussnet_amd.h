@@ -48,7 +48,7 @@ enum {
     TN_ERR_ARG = 1,         /* a null pointer, a size mismatch, an out-of-range value */
     TN_ERR_NODEVICE = 2,    /* no HIP device, or the device index is out of range */
     TN_ERR_HIP = 3,         /* a HIP runtime call failed */
-    TN_ERR_UNSUPPORTED = 4, /* a mode this build does not run on the device (shape fields, gm < 0) */
+    TN_ERR_UNSUPPORTED = 4, /* a mode this build does not run on the device (seeding a shape grid, gm < 0) */
     TN_ERR_INTERNAL = 5
 };
 
@@ -71,10 +71,13 @@ typedef struct tn_grid_view {
     float hmin, hmax;         /* sizing range (Grid::hmin/hmax): the hash geometry */
     const float* gI;          /* gray-scale intensity per voxel (gm > 0), else null */
     const float* gTW;         /* gm > 0: [t_0..t_{m-1}, W_0..W_{m-1}], the ascending thresholds and
-                                 their membership widths (Grid::gTW); gm < 0: shape programs */
+                                 their membership widths (Grid::gTW); gm < 0: the labels' compiled
+                                 analytic-field program (src/opencl/v2m_sdf_body.cl layout), checked
+                                 on the host before use */
     int64_t ngTW;             /* values in gTW (>= 2 gm for a gray-scale grid) */
     int32_t gm;               /* 0: label volume; m > 0: gray-scale volume with m thresholds;
-                                 < 0: shape fields (TN_ERR_UNSUPPORTED on the device) */
+                                 < 0: shape fields (relaxation only: their seeds start with the
+                                 host's feature nodes, so tn_seed returns TN_ERR_UNSUPPORTED) */
 } tn_grid_view;
 
 /* RelaxParams (src/v2m_particles.h:21-41) plus the rebuild divisor that the
