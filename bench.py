@@ -27,7 +27,8 @@ cpu_baseline: the reference's own relax_cpu (oracle/_ref, compiled from its
 sources) on this host's cores, on the same grid and seeds, for a bounded
 number of iterations.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|small|prob|batch|adjoint|grid]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+                       [--workload large|small|prob|voxel|gray|batch|adjoint|grid]
 """
 from __future__ import annotations
 
@@ -152,20 +153,29 @@ def build_workload(name: str):
         grid = workload.prob_large()
         desc = ("prob_large: 182x218x182x6 softmax of six Gaussian random fields (sigma 6, seed 1), labels by "
                 "argmax, fields = smoothed probabilities, uniform h 1.8, device seeding, 100 iterations")
+    elif name == "voxel":
+        grid = workload.plain_large()
+        desc = ("plain_large in voxel-trap mode (the staircase move, voxel-labelled seeds, FIRE off): 256^3 "
+                "six-label nested perturbed ellipsoids, h graded 1.5-6 voxels, 100 iterations")
+    elif name == "gray":
+        grid = workload.gray_large()
+        desc = ("gray_large: 256^3 gray-scale ball (1 - r/115 + 0.06 x a Gaussian random field, sigma 3), "
+                "thresholds 0.2/0.5/0.75, grid built on the device (hbase 4.5, hmin 1.5, hmax 6 voxels), "
+                "device seeding, 100 iterations")
     else:
         grid = workload.plain_small()
         desc = "plain_small: 64^3 two spheres r 24/12, uniform h 3, device seeding, 50 iterations"
     log(f"grid {grid.nx}x{grid.ny}x{grid.nz}: {grid.nslot} slots ({time.perf_counter() - t:.1f} s)")
     t = time.perf_counter()
-    seeds = relax.seed(grid, RelaxParams(), device=0)
+    seeds = relax.seed(grid, RelaxParams(voxel_trap=name == "voxel"), device=0)
     log(f"seeds: {seeds.n} ({time.perf_counter() - t:.1f} s)")
     return grid, seeds, desc
 
 
-def cpu_baseline(grid, seeds, iters: int) -> dict | None:
+def cpu_baseline(grid, seeds, iters: int, voxel: bool = False) -> dict | None:
     from trussnet_amd._abi import RelaxParams
 
-    prm = RelaxParams(max_iters=iters, dptol=0.0)
+    prm = RelaxParams(max_iters=iters, dptol=0.0, voxel_trap=voxel)
     try:
         from oracle import ref
 
@@ -425,7 +435,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["large", "small", "prob", "batch", "adjoint", "grid"], default="large")
+    ap.add_argument("--workload", choices=["large", "small", "prob", "voxel", "gray", "batch", "adjoint", "grid"],
+                    default="large")
     ap.add_argument("--iters", type=int, default=0, help="iterations per step (default: the workload's)")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
@@ -450,7 +461,8 @@ def main() -> int:
 
     grid, seeds, desc = build_workload(args.workload)
     iters = args.iters or (50 if args.workload == "small" else 100)
-    prm = RelaxParams(max_iters=iters, dptol=0.0)
+    voxel = args.workload == "voxel"
+    prm = RelaxParams(max_iters=iters, dptol=0.0, voxel_trap=voxel)
     plan = relax.Plan(grid, seeds, prm, device=0)
     log(f"plan: {seeds.n} nodes, hash {plan.hash.nlev} levels / {plan.hash.nkeys} cells")
     for w in range(args.warmup):
@@ -503,7 +515,7 @@ def main() -> int:
                 "bytes_per_launch": nbytes, "avg_launch_us": avg_s * 1e6,
                 "items_per_launch": timings[dom][2] / cnt}
     loop_bytes = iters * n * (83 + 4 * kbar) + st["rebuilds"] * n * (179 + 4 * kbar)
-    cpu = None if args.no_cpu else cpu_baseline(grid, seeds, args.cpu_iters)
+    cpu = None if args.no_cpu else cpu_baseline(grid, seeds, args.cpu_iters, voxel)
     line = {
         "metric": METRIC, "value": value, "unit": "node-iterations/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",

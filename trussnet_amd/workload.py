@@ -181,6 +181,20 @@ def prob_large(seed: int = 1, shape=(182, 218, 182), h: float = 1.8) -> Grid:
     return build_grid(lab, np.float32(h), hmin=h, hmax=h, prob=prob)
 
 
+def gray_large(n: int = 256, seed: int = 2, device: int = 0) -> Grid:
+    """The gray-scale workload: an n^3 intensity 1 - r / (0.45 n) plus 0.06 x a
+    Gaussian random field (sigma 3 voxels), cut by the thresholds 0.2 / 0.5 / 0.75
+    into three nested perturbed shells; the grid built on the device with the
+    reference's defaults and hbase 4.5, hmin 1.5, hmax 6 voxels."""
+    from . import gridbuild
+    from ._abi import GridParams
+
+    gray = volumes.gray_ball(n, seed=seed)
+    lab = volumes.gray_labels(gray, volumes.GRAY_THRESHOLDS)
+    return gridbuild.build(lab, prm=GridParams(hbase=4.5, hmin=1.5, hmax=6.0), gray=gray,
+                           thresholds=volumes.GRAY_THRESHOLDS, device=device)
+
+
 def adjoint_volume(n: int = 128, seed: int = 5, h: float = 1.5) -> Grid:
     """The differentiation workload: 128^3, three nested perturbed ellipsoid labels
     (semi-axes U(20, 60) voxels), uniform size h voxels (about 200 k seeds)."""
