@@ -125,11 +125,16 @@ def test_steps_compose(tn):
         assert plan.download().equal(whole)
 
 
-def test_batch_equals_the_reference_runs(tn):
-    """tn_relax_batch: every golden case's runs relaxed together, each to the reference's bits."""
+@pytest.mark.parametrize("which", ["all", "labels"])
+def test_batch_equals_the_reference_runs(tn, which):
+    """tn_relax_batch: every golden case's runs relaxed together, each to the reference's
+    bits -- all cases (gray-scale and shape grids among them: the general move kernels),
+    and the label volumes alone (the move kernels compiled for label volumes)."""
     grids, seeds, prms, wants = [], [], [], []
     for name in CASES:
         fx = Fixture(name)
+        if which == "labels" and fx.grid.gm != 0:
+            continue
         for prm, want, stats in fx.runs():
             grids.append(fx.grid)
             seeds.append(fx.seeds)
