@@ -224,6 +224,7 @@ int tn_grid_download(const tn_grid* g, int32_t* bl_cnt, uint16_t* bl_lab, int32_
                      uint8_t* grade, float* gTW);
 int tn_grid_destroy(tn_grid* g);
 
+
 /*
  * A relaxation plan: the grid and the seed nodes resident on one device, with
  * every buffer the loop needs allocated once. tn_plan_run restores the seeds on
@@ -239,6 +240,20 @@ int tn_plan_create(const tn_grid_view* g, const tn_relax_params* prm, const tn_n
 int tn_plan_run(tn_plan* plan, tn_relax_stats* st);
 int tn_plan_download(tn_plan* plan, tn_nodes_view* nd);
 int tn_plan_destroy(tn_plan* plan);
+
+/*
+ * The steps after the grid build without leaving the device: seed_cpu on a built
+ * grid (as tn_seed_count / tn_seed, its grades included), and a relaxation plan
+ * over it (as tn_plan_create; the plan copies the grid device to device, so the
+ * grid may be destroyed afterwards). No counterpart in the reference, whose
+ * pipeline hands the Grid from build_grid_cpu to seed_cpu and relax_cl on the host
+ * (src/v2m_pipeline.cpp:425-500).
+ */
+int tn_seed_count_from_grid(const tn_grid* g, const tn_relax_params* prm, int64_t* n);
+int tn_seed_from_grid(const tn_grid* g, const tn_relax_params* prm, int64_t n, float* P, uint16_t* lab, uint8_t* typ,
+                      uint16_t* part, uint16_t* part3);
+int tn_plan_create_from_grid(const tn_grid* g, const tn_relax_params* prm, const tn_nodes_view* seeds,
+                             tn_plan** out);
 
 /* The hash geometry of a plan (V2mHash, src/opencl/v2m_particle_body.cl:21-27;
  * built at src/v2m_particles.cpp:738-761): b0, origin, levels, dims, offsets. */

@@ -73,3 +73,32 @@ def test_grid_build_large_properties():
     assert seeds.n > 100000
     out, st = relax.relax(G, seeds, RelaxParams(max_iters=20, dptol=0.0), device=0)
     assert st["iters"] == 20 and np.all(np.isfinite(out.P))
+
+
+@pytest.mark.parametrize("name", ["sectors_opts", "gray40"])
+def test_device_pipeline_equals_host_round_trip(name):
+    """Seeding and a relaxation plan straight from the device-built grid
+    (tn_seed_from_grid, tn_plan_create_from_grid) give the bits of the same steps
+    on the downloaded grid."""
+    from gridbuild_cases import inputs_of
+
+    from trussnet_amd import gridbuild, relax
+    from trussnet_amd._abi import RelaxParams
+
+    print(f"[device pipeline] case {name}", flush=True)
+    kw = inputs_of(GB, name)
+    G = gridbuild.build(**kw, device=0)
+    res = gridbuild.build_resident(**kw, device=0)
+    try:
+        s_dev = res.seed(RelaxParams())
+        s_host = relax.seed(G, RelaxParams(), device=0)
+        assert s_dev.equal(s_host) and np.array_equal(s_dev.part3, s_host.part3)
+        prm = RelaxParams(max_iters=40, dptol=0.0)
+        with res.plan(s_dev, prm) as plan:
+            st = plan.run()
+            got = plan.download()
+        want, st_h = relax.relax(G, s_host, prm, device=0)
+        assert got.equal(want)
+        assert st["iters"] == st_h["iters"] == 40 and st["rebuilds"] == st_h["rebuilds"]
+    finally:
+        res.close()

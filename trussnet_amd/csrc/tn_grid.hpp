@@ -820,6 +820,8 @@ struct tn_grid {
     uint8_t* grade = nullptr;       // [nx*ny*nz]
     std::vector<float> gTW;         // gray-scale: the thresholds, then their widths
     const float* gI = nullptr;      // gray-scale: the intensity on the device
+    const float* dTW = nullptr;     // gray-scale: gTW on the device
+    float hbig = 0.0f;              // the largest h
 };
 
 namespace {
@@ -1108,6 +1110,7 @@ int tn_grid_build(const tn_volume_view* vol, const tn_grid_params* prm, int32_t 
                 }
             }
             float* dTW = mem.upload(G->gTW.data(), G->gTW.size());
+            G->dTW = dTW;
             G->gI = mem.upload(vol->gray, (size_t)nv);
             if (nslot > 0) {
                 tng::k_gray_phi<<<blocks_for((int64_t)nslot * TN_SLOT, 256), 256>>>(f, dI, dTW, m, slot_brick, slot_label,
@@ -1208,6 +1211,22 @@ int tn_grid_build(const tn_volume_view* vol, const tn_grid_params* prm, int32_t 
             }
         }
         mark("grade");
+        {   // the largest h (the hash's pruning test), and the Field completed for seeding
+            int32_t* mm = tmp.alloc<int32_t>(2);
+            const int32_t init[2] = { 0x7F800000, 0 };
+            TN_HIP(hipMemcpy(mm, init, sizeof init, hipMemcpyHostToDevice));
+            tng::k_pos_minmax<<<nbv, 256>>>(h, nv, mm);
+            int32_t got[2];
+            TN_HIP(hipMemcpy(got, mm, sizeof got, hipMemcpyDeviceToHost));
+            std::memcpy(&G->hbig, &got[1], 4);
+            f.gm = I.gm;
+            f.gI = G->gI;
+            f.gTW = G->dTW;
+            set_reciprocals(f);
+            tnd::Field* self = mem.alloc<tnd::Field>(1);
+            f.self = self;
+            TN_HIP(hipMemcpy(self, &f, sizeof f, hipMemcpyHostToDevice));
+        }
         TN_HIP(hipDeviceSynchronize());
         if (verbose && !marks.empty()) {
             std::fprintf(stderr, "[tn_grid_build] %d x %d x %d, %d slots, %d sweeps:", nx, ny, nz, nslot, I.limit_sweeps);
@@ -1262,6 +1281,79 @@ int tn_grid_download(const tn_grid* g, int32_t* bl_cnt, uint16_t* bl_lab, int32_
 
 int tn_grid_destroy(tn_grid* g) {
     return guarded([&] { delete g; });
+}
+
+int tn_seed_count_from_grid(const tn_grid* g, const tn_relax_params* prm, int64_t* n) {
+    return guarded([&] {
+        arg_check(g != nullptr && n != nullptr, "null argument");
+        TN_HIP(hipSetDevice(g->mem.device));
+        DevMem mem;
+        mem.device = g->mem.device;
+        *n = seeding_core(g->f, g->grade, g->info.hmin, g->info.hmax, prm, mem).n;
+    });
+}
+
+int tn_seed_from_grid(const tn_grid* g, const tn_relax_params* prm, int64_t n, float* P, uint16_t* lab, uint8_t* typ,
+                      uint16_t* part, uint16_t* part3) {
+    return guarded([&] {
+        arg_check(g != nullptr, "null grid");
+        arg_check(n >= 0 && (n == 0 || (P && lab && typ && part)), "seeding: null output array");
+        TN_HIP(hipSetDevice(g->mem.device));
+        DevMem mem;
+        mem.device = g->mem.device;
+        const Seeds s = seeding_core(g->f, g->grade, g->info.hmin, g->info.hmax, prm, mem);
+        arg_check(s.n == n, "seeding: the output holds " + std::to_string(n) + " nodes, the seeding made " +
+                                std::to_string(s.n) + " (tn_seed_count_from_grid)");
+        if (s.n == 0) {
+            return;
+        }
+        TN_HIP(hipMemcpy(P, s.P, sizeof(float) * 3 * s.n, hipMemcpyDeviceToHost));
+        TN_HIP(hipMemcpy(lab, s.lab, sizeof(uint16_t) * s.n, hipMemcpyDeviceToHost));
+        TN_HIP(hipMemcpy(typ, s.typ, s.n, hipMemcpyDeviceToHost));
+        TN_HIP(hipMemcpy(part, s.part, sizeof(uint16_t) * 2 * s.n, hipMemcpyDeviceToHost));
+        if (part3) {
+            TN_HIP(hipMemcpy(part3, s.part3, sizeof(uint16_t) * s.n, hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+int tn_plan_create_from_grid(const tn_grid* g, const tn_relax_params* prm, const tn_nodes_view* seeds,
+                             tn_plan** out) {
+    return guarded([&] {
+        arg_check(g != nullptr && out != nullptr, "null argument");
+        *out = nullptr;
+        check_params(prm);
+        check_nodes(seeds);
+        const tn_grid_info& I = g->info;
+        const float vs[3] = { g->f.vx, g->f.vy, g->f.vz };
+        int64_t nkeys = 0;
+        const tnd::Hash H = make_hash_geom(I.nx, I.ny, I.nz, vs, I.hmin, I.hmax, g->hbig, prm, &nkeys);
+        // the plan keeps its own copy of the grid (device to device)
+        *out = plan_build(prm, seeds, g->mem.device, H, nkeys, [&](DevMem& mem) {
+            const size_t nv = (size_t)I.nx * I.ny * I.nz, nb = (size_t)I.nbx * I.nby * I.nbz;
+            auto dup = [&](auto* src, size_t count) {
+                using T = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
+                T* d = mem.alloc<T>(std::max<size_t>(count, 1));
+                if (count) {
+                    TN_HIP(hipMemcpy(d, src, count * sizeof(T), hipMemcpyDeviceToDevice));
+                }
+                return d;
+            };
+            tnd::Field f = g->f;
+            f.L = dup(g->f.L, nv);
+            f.bl_cnt = dup(g->f.bl_cnt, nb);
+            f.bl_lab = dup(g->f.bl_lab, nb * TN_BL);
+            f.bl_slot = dup(g->f.bl_slot, nb);
+            f.phi = dup(g->f.phi, (size_t)I.nslot * TN_SLOT);
+            f.h = dup(g->f.h, nv);
+            f.gI = I.gm > 0 ? dup(g->gI, nv) : nullptr;
+            f.gTW = I.gm > 0 ? dup(g->dTW, 2 * (size_t)I.gm) : nullptr;
+            tnd::Field* self = mem.alloc<tnd::Field>(1);
+            f.self = self;
+            TN_HIP(hipMemcpy(self, &f, sizeof f, hipMemcpyHostToDevice));
+            return f;
+        });
+    });
 }
 
 }  // extern "C"

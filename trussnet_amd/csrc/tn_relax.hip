@@ -69,16 +69,18 @@ void arg_check(bool ok, const std::string& what) {
 }
 
 // the hash geometry of relax_cpu (src/v2m_particles.cpp:738-761)
-tnd::Hash make_hash(const tn_grid_view* g, const tn_relax_params* prm, int64_t* nkeys_out) {
+// (hbig: the largest h of the grid)
+tnd::Hash make_hash_geom(int nx, int ny, int nz, const float* vs, float hmin, float hmax, float hbig,
+                         const tn_relax_params* prm, int64_t* nkeys_out) {
     tnd::Hash H;
     std::memset(&H, 0, sizeof H);
-    H.b0 = (prm->t + prm->skin) * g->hmin;
-    H.ox = -0.5f * g->vs[0];
-    H.oy = -0.5f * g->vs[1];
-    H.oz = -0.5f * g->vs[2];
-    const float ext[3] = { (float)g->nx * g->vs[0], (float)g->ny * g->vs[1], (float)g->nz * g->vs[2] };
+    H.b0 = (prm->t + prm->skin) * hmin;
+    H.ox = -0.5f * vs[0];
+    H.oy = -0.5f * vs[1];
+    H.oz = -0.5f * vs[2];
+    const float ext[3] = { (float)nx * vs[0], (float)ny * vs[1], (float)nz * vs[2] };
     H.nlev = 1;
-    while (H.nlev < TN_MAXLEV && H.b0 * (float)(1 << (H.nlev - 1)) < (prm->t + prm->skin) * g->hmax) {
+    while (H.nlev < TN_MAXLEV && H.b0 * (float)(1 << (H.nlev - 1)) < (prm->t + prm->skin) * hmax) {
         ++H.nlev;
     }
     int64_t nkeys = 0;
@@ -95,13 +97,17 @@ tnd::Hash make_hash(const tn_grid_view* g, const tn_relax_params* prm, int64_t* 
     *nkeys_out = nkeys;
     // the K-nearest search may skip cells beyond a node's reach only if every node's
     // reach (t + skin) h fits the cell of its level, i.e. the largest h fits the top level
+    H.prune = (prm->t + prm->skin) * hbig <= H.b0 * (float)(1 << (H.nlev - 1)) ? 1 : 0;
+    return H;
+}
+
+tnd::Hash make_hash(const tn_grid_view* g, const tn_relax_params* prm, int64_t* nkeys_out) {
     const size_t nv = (size_t)g->nx * g->ny * g->nz;
     float hbig = 0.0f;
     for (size_t v = 0; v < nv; ++v) {
         hbig = std::max(hbig, g->h[v]);
     }
-    H.prune = (prm->t + prm->skin) * hbig <= H.b0 * (float)(1 << (H.nlev - 1)) ? 1 : 0;
-    return H;
+    return make_hash_geom(g->nx, g->ny, g->nz, g->vs, g->hmin, g->hmax, hbig, prm, nkeys_out);
 }
 
 // thr[e + 21]: the smallest float m below 2^e with floor(log2f(m)) == e (the host's
@@ -195,6 +201,19 @@ struct DevMem {
     }
 };
 
+// the exact voxel-size reciprocals of a Field: a power of two whose reciprocal is a
+// normal float has x / v == x * (1 / v) exactly
+void set_reciprocals(tnd::Field& f) {
+    auto pow2 = [](float v) {
+        int e = 0;
+        return std::isfinite(v) && v > 0.0f && std::frexp(v, &e) == 0.5f && e > -120 && e < 120;
+    };
+    f.pow2 = pow2(f.vx) && pow2(f.vy) && pow2(f.vz);
+    f.ivx = 1.0f / f.vx;
+    f.ivy = 1.0f / f.vy;
+    f.ivz = 1.0f / f.vz;
+}
+
 // the grid's arrays on the device (Grid, src/v2m_grid.h:45-81)
 tnd::Field upload_grid(const tn_grid_view* g, DevMem& mem) {
     const size_t nv = (size_t)g->nx * g->ny * g->nz;
@@ -209,15 +228,7 @@ tnd::Field upload_grid(const tn_grid_view* g, DevMem& mem) {
     f.vx = g->vs[0];
     f.vy = g->vs[1];
     f.vz = g->vs[2];
-    // a power of two whose reciprocal is a normal float: x / v == x * (1 / v) exactly
-    auto pow2 = [](float v) {
-        int e = 0;
-        return std::isfinite(v) && v > 0.0f && std::frexp(v, &e) == 0.5f && e > -120 && e < 120;
-    };
-    f.pow2 = pow2(f.vx) && pow2(f.vy) && pow2(f.vz);
-    f.ivx = 1.0f / f.vx;
-    f.ivy = 1.0f / f.vy;
-    f.ivz = 1.0f / f.vz;
+    set_reciprocals(f);
     f.L = mem.upload(g->L, nv);
     f.bl_cnt = mem.upload(g->bl_cnt, nb);
     f.bl_lab = mem.upload(g->bl_lab, nb * TN_BL);
@@ -712,6 +723,109 @@ struct tn_plan {
 
 namespace {
 
+// A plan on device dev: make_grid(mem) puts the grid into the plan's memory and
+// returns its Field; H and nkeys are the hash geometry.
+template <class MakeGrid>
+tn_plan* plan_build(const tn_relax_params* prm, const tn_nodes_view* seeds, int dev, const tnd::Hash& H, int64_t nkeys,
+                    MakeGrid make_grid) {
+        TN_HIP(hipSetDevice(dev));
+        std::unique_ptr<tn_plan> p(new tn_plan);
+        p->device = dev;
+        p->prm = *prm;
+        p->n = (int)seeds->n;
+        p->fire = prm->fire && !prm->voxel_trap ? 1 : 0;
+        p->rebuild_div = prm->rebuild_div > 0 ? prm->rebuild_div : 300;
+        p->tskin = prm->t + prm->skin;
+        p->H = H;
+        p->nkeys = nkeys;
+        TN_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+        TN_HIP(hipEventCreateWithFlags(&p->ev_iter, hipEventDisableTiming));
+        for (hipStream_t& s : p->side) {
+            TN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        }
+        for (hipEvent_t* e : { &p->ev_easy, &p->ev_rest, &p->ev_int }) {
+            TN_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        }
+        p->timer.stream = p->stream;
+        p->timer.on = prm->verbose || std::getenv("TN_KERNEL_TIMING") != nullptr;
+        const int n = p->n;
+        p->mem.device = dev;
+        p->f = make_grid(p->mem);
+        const tnd::Field& f = p->f;
+        // node state
+        p->dX0 = p->alloc<float4>(n);
+        p->dX = p->alloc<float4>(n);
+        p->dXb = p->alloc<float4>(n);
+        p->dV = p->alloc<float4>(n);
+        p->dVb = p->alloc<float4>(n);
+        p->dF = p->alloc<float4>(n);
+        p->dP0 = p->alloc<float4>(n);
+        p->dPs = p->alloc<float4>(n);
+        p->dMs = p->alloc<int2>(n);
+        p->dTyp0 = p->alloc<uint8_t>(n);
+        p->dTyp = p->alloc<uint8_t>(n);
+        p->dTypb = p->alloc<uint8_t>(n);
+        p->dPart0 = p->alloc<uint32_t>(n);
+        p->dPart = p->alloc<uint32_t>(n);
+        p->dPartb = p->alloc<uint32_t>(n);
+        p->dLab0 = p->alloc<uint16_t>(n);
+        p->dLab = p->alloc<uint16_t>(n);
+        p->dLabb = p->alloc<uint16_t>(n);
+        p->dId = p->alloc<int32_t>(n);
+        p->dIdb = p->alloc<int32_t>(n);
+        p->dSlotOf = p->alloc<int32_t>(n);
+        p->dMv = p->alloc<float>(n);
+        p->dPw = p->alloc<float>(n);
+        p->dKey = p->alloc<int32_t>(n);
+        p->dSorted = p->alloc<int32_t>(n);
+        p->dNbrT = p->alloc<int32_t>((size_t)n * TN_K);
+        p->dNnb = p->alloc<int32_t>(n);
+        p->dQueue = p->alloc<int32_t>(n);
+        p->dMoveQ = p->alloc<int32_t>(n);
+        p->dMoveQi = p->alloc<int32_t>(n);
+        p->dMoveQ2 = p->alloc<int32_t>(n);
+        p->dMoveQd = p->alloc<int32_t>(n);
+        p->dFlag = p->alloc<uint8_t>(n);
+        p->dCstart = p->alloc<int32_t>((size_t)p->nkeys + 1);
+        for (int k = 0; k < 2; ++k) {
+            p->dRk[k] = p->alloc<int32_t>(n);
+            p->dRv[k] = p->alloc<int32_t>(n);
+        }
+        // radix sort geometry: 8-bit digits over the key range [0, nkeys)
+        const int bits = p->nkeys <= 1 ? 0 : 32 - __builtin_clz((unsigned)(p->nkeys - 1));
+        p->radix_passes = (bits + 7) / 8;
+        p->radix_nblk = (int)std::max<unsigned>(1, blocks_for(n, tnk::kRadixTile));
+        p->dBhist = p->alloc<int32_t>((size_t)256 * p->radix_nblk);
+        const int64_t scan_max = std::max<int64_t>(p->nkeys + 1, (int64_t)256 * p->radix_nblk);
+        p->dBsum = p->alloc<int32_t>(blocks_for(scan_max, tnk::kScanTile));
+        p->nparts = (int)std::max<unsigned>(1, std::min<unsigned>(blocks_for(n, tnk::kBlock), tnk::kStatsBlocks));
+        p->refresh_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(n, 2048));   // one wave per listed node
+        p->dPartials = p->alloc<double2>(std::max(p->nparts, 1));
+        p->dThr = p->alloc<float>(tnk::kHistThr);
+        p->dCtl = p->alloc<tnk::Ctl>(1);
+        TN_HIP(hipHostMalloc((void**)&p->hCtl, sizeof(tnk::Ctl), hipHostMallocDefault));
+        std::memset(p->hCtl, 0, sizeof(tnk::Ctl));
+        float thr[tnk::kHistThr];
+        hist_thresholds(thr);
+        TN_HIP(hipMemcpy(p->dThr, thr, sizeof thr, hipMemcpyHostToDevice));
+        // the seeds
+        if (n > 0) {
+            float* dP3 = p->alloc<float>((size_t)3 * n);
+            TN_HIP(hipMemcpy(dP3, seeds->P, sizeof(float) * 3 * n, hipMemcpyHostToDevice));
+            k_init_nodes<<<blocks_for(n, 256), 256, 0, p->stream>>>(f, dP3, n, p->dX0);
+            TN_HIP(hipGetLastError());
+            std::vector<uint32_t> part(n);
+            for (int i = 0; i < n; ++i) {
+                part[i] = (uint32_t)seeds->part[2 * i] | ((uint32_t)seeds->part[2 * i + 1] << 16);
+            }
+            TN_HIP(hipMemcpy(p->dPart0, part.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+            TN_HIP(hipMemcpy(p->dTyp0, seeds->typ, n, hipMemcpyHostToDevice));
+            TN_HIP(hipMemcpy(p->dLab0, seeds->lab, sizeof(uint16_t) * n, hipMemcpyHostToDevice));
+        }
+        p->reset();
+        return p.release();
+}
+
 int pick_device(int32_t device) {
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
@@ -914,7 +1028,7 @@ int bit_length(uint32_t x) {
 // key order, and the lattice nodes within 0.4 h of a kept seed dropped. Writes the
 // result into `out` (capacity: the lattice count + the candidate count + extra)
 // and returns its node count.
-int64_t junction_pass(const tn_grid_view* g, const tnd::Field& f, const tnk::SeedArgs& sa, float cellfac,
+int64_t junction_pass(float ghmax, const tnd::Field& f, const tnk::SeedArgs& sa, float cellfac,
                       const tnk::VertexGrid& vg, const Seeds& lat, int64_t extra, SeedOps& ops, Seeds& out) {
     DevMem& mem = ops.mem;
     const int n = (int)lat.n;
@@ -972,11 +1086,11 @@ int64_t junction_pass(const tn_grid_view* g, const tnd::Field& f, const tnk::See
             nk = ops.scan(head, nc);
             // the drop hash of the kept seeds (src/v2m_particles.cpp:160-181)
             ks.nk = nk;
-            ks.cell = 0.4f * g->hmax;
-            ks.ox = g->vs[0];
-            ks.oy = g->vs[1];
-            ks.oz = g->vs[2];
-            const float ext[3] = { (g->nx + 1) * g->vs[0], (g->ny + 1) * g->vs[1], (g->nz + 1) * g->vs[2] };
+            ks.cell = 0.4f * ghmax;
+            ks.ox = f.vx;
+            ks.oy = f.vy;
+            ks.oz = f.vz;
+            const float ext[3] = { (f.nx + 1) * f.vx, (f.ny + 1) * f.vy, (f.nz + 1) * f.vz };
             int64_t cells = 1;
             for (int e = 0; e < 3; ++e) {
                 ks.hd[e] = (int)(ext[e] / ks.cell) + 2;
@@ -1025,6 +1139,9 @@ int64_t junction_pass(const tn_grid_view* g, const tnd::Field& f, const tnk::See
     return total;
 }
 
+Seeds seeding_core(const tnd::Field& f, const uint8_t* d_grade, float hmin, float hmax, const tn_relax_params* prm,
+                   DevMem& mem);
+
 Seeds run_seeding(const tn_grid_view* g, const uint8_t* grade, const tn_relax_params* prm, int dev, DevMem& mem) {
     check_grid(g);
     if (g->gm < 0) {
@@ -1032,22 +1149,31 @@ Seeds run_seeding(const tn_grid_view* g, const uint8_t* grade, const tn_relax_pa
                                           "nodes (add_feature_nodes), which are made on the host");
     }
     arg_check(prm != nullptr && grade != nullptr, "seeding: null argument");
-    arg_check(prm->nseed >= 1 && prm->nseed <= 64, "seeding: nseed out of range");
-    arg_check(prm->jseed >= 0.0f && std::isfinite(prm->jseed), "seeding: jseed must be finite and >= 0");
     TN_HIP(hipSetDevice(dev));
     mem.device = dev;
-    SeedOps ops{ mem };
     const tnd::Field f = upload_grid(g, mem);
+    const uint8_t* d_grade = mem.upload(grade, (size_t)g->nx * g->ny * g->nz);
+    return seeding_core(f, d_grade, g->hmin, g->hmax, prm, mem);
+}
+
+// seed_cpu on a grid already on the device (f, its grades d_grade), scratch in mem
+Seeds seeding_core(const tnd::Field& f, const uint8_t* d_grade, float hmin, float hmax, const tn_relax_params* prm,
+                   DevMem& mem) {
+    arg_check(prm != nullptr, "seeding: null argument");
+    arg_check(prm->nseed >= 1 && prm->nseed <= 64, "seeding: nseed out of range");
+    arg_check(prm->jseed >= 0.0f && std::isfinite(prm->jseed), "seeding: jseed must be finite and >= 0");
+    SeedOps ops{ mem };
+    const tnd::Field* g = &f;   // (its dims)
     const int64_t nv = (int64_t)g->nx * g->ny * g->nz;
     // v2m_seed_spacing (src/opencl/v2m_seed_body.cl:445-447), with the host's powf
     std::vector<float> spacing(prm->nseed);
     for (int L = 0; L < prm->nseed; ++L) {
-        spacing[L] = prm->nseed <= 1 ? g->hmin
-                                     : g->hmin * std::pow(g->hmax / g->hmin, (float)L / (float)(prm->nseed - 1));
+        spacing[L] = prm->nseed <= 1 ? hmin : hmin * std::pow(hmax / hmin, (float)L / (float)(prm->nseed - 1));
         arg_check(spacing[L] > 0.0f && std::isfinite(spacing[L]), "seeding: bad seed spacing");
     }
+    (void)nv;
     tnk::SeedArgs sa;
-    sa.grade = mem.upload(grade, (size_t)nv);
+    sa.grade = d_grade;
     sa.spacing = mem.upload(spacing.data(), spacing.size());
     sa.nseed = prm->nseed;
     sa.voxmode = prm->voxel_trap ? 1 : 0;
@@ -1088,7 +1214,7 @@ Seeds run_seeding(const tn_grid_view* g, const uint8_t* grade, const tn_relax_pa
     }
     Seeds s = lat;
     if (junctions) {
-        junction_pass(g, f, sa, prm->jseed, vg, lat, nc, ops, s);
+        junction_pass(hmax, f, sa, prm->jseed, vg, lat, nc, ops, s);
     }
     if (nc > 0) {
         tnk::SeedNodes o{ s.P, s.lab, s.typ, s.part, s.part3 };
@@ -1311,101 +1437,9 @@ int tn_plan_create(const tn_grid_view* g, const tn_relax_params* prm, const tn_n
         check_params(prm);
         check_nodes(seeds);
         const int dev = pick_device(device);
-        TN_HIP(hipSetDevice(dev));
-        std::unique_ptr<tn_plan> p(new tn_plan);
-        p->device = dev;
-        p->prm = *prm;
-        p->n = (int)seeds->n;
-        p->fire = prm->fire && !prm->voxel_trap ? 1 : 0;
-        p->rebuild_div = prm->rebuild_div > 0 ? prm->rebuild_div : 300;
-        p->tskin = prm->t + prm->skin;
-        p->H = make_hash(g, prm, &p->nkeys);
-        TN_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
-        TN_HIP(hipEventCreateWithFlags(&p->ev_iter, hipEventDisableTiming));
-        for (hipStream_t& s : p->side) {
-            TN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        }
-        for (hipEvent_t* e : { &p->ev_easy, &p->ev_rest, &p->ev_int }) {
-            TN_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
-        }
-        p->timer.stream = p->stream;
-        p->timer.on = prm->verbose || std::getenv("TN_KERNEL_TIMING") != nullptr;
-        const int n = p->n;
-        p->mem.device = dev;
-        p->f = upload_grid(g, p->mem);
-        const tnd::Field& f = p->f;
-        // node state
-        p->dX0 = p->alloc<float4>(n);
-        p->dX = p->alloc<float4>(n);
-        p->dXb = p->alloc<float4>(n);
-        p->dV = p->alloc<float4>(n);
-        p->dVb = p->alloc<float4>(n);
-        p->dF = p->alloc<float4>(n);
-        p->dP0 = p->alloc<float4>(n);
-        p->dPs = p->alloc<float4>(n);
-        p->dMs = p->alloc<int2>(n);
-        p->dTyp0 = p->alloc<uint8_t>(n);
-        p->dTyp = p->alloc<uint8_t>(n);
-        p->dTypb = p->alloc<uint8_t>(n);
-        p->dPart0 = p->alloc<uint32_t>(n);
-        p->dPart = p->alloc<uint32_t>(n);
-        p->dPartb = p->alloc<uint32_t>(n);
-        p->dLab0 = p->alloc<uint16_t>(n);
-        p->dLab = p->alloc<uint16_t>(n);
-        p->dLabb = p->alloc<uint16_t>(n);
-        p->dId = p->alloc<int32_t>(n);
-        p->dIdb = p->alloc<int32_t>(n);
-        p->dSlotOf = p->alloc<int32_t>(n);
-        p->dMv = p->alloc<float>(n);
-        p->dPw = p->alloc<float>(n);
-        p->dKey = p->alloc<int32_t>(n);
-        p->dSorted = p->alloc<int32_t>(n);
-        p->dNbrT = p->alloc<int32_t>((size_t)n * TN_K);
-        p->dNnb = p->alloc<int32_t>(n);
-        p->dQueue = p->alloc<int32_t>(n);
-        p->dMoveQ = p->alloc<int32_t>(n);
-        p->dMoveQi = p->alloc<int32_t>(n);
-        p->dMoveQ2 = p->alloc<int32_t>(n);
-        p->dMoveQd = p->alloc<int32_t>(n);
-        p->dFlag = p->alloc<uint8_t>(n);
-        p->dCstart = p->alloc<int32_t>((size_t)p->nkeys + 1);
-        for (int k = 0; k < 2; ++k) {
-            p->dRk[k] = p->alloc<int32_t>(n);
-            p->dRv[k] = p->alloc<int32_t>(n);
-        }
-        // radix sort geometry: 8-bit digits over the key range [0, nkeys)
-        const int bits = p->nkeys <= 1 ? 0 : 32 - __builtin_clz((unsigned)(p->nkeys - 1));
-        p->radix_passes = (bits + 7) / 8;
-        p->radix_nblk = (int)std::max<unsigned>(1, blocks_for(n, tnk::kRadixTile));
-        p->dBhist = p->alloc<int32_t>((size_t)256 * p->radix_nblk);
-        const int64_t scan_max = std::max<int64_t>(p->nkeys + 1, (int64_t)256 * p->radix_nblk);
-        p->dBsum = p->alloc<int32_t>(blocks_for(scan_max, tnk::kScanTile));
-        p->nparts = (int)std::max<unsigned>(1, std::min<unsigned>(blocks_for(n, tnk::kBlock), tnk::kStatsBlocks));
-        p->refresh_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(n, 2048));   // one wave per listed node
-        p->dPartials = p->alloc<double2>(std::max(p->nparts, 1));
-        p->dThr = p->alloc<float>(tnk::kHistThr);
-        p->dCtl = p->alloc<tnk::Ctl>(1);
-        TN_HIP(hipHostMalloc((void**)&p->hCtl, sizeof(tnk::Ctl), hipHostMallocDefault));
-        std::memset(p->hCtl, 0, sizeof(tnk::Ctl));
-        float thr[tnk::kHistThr];
-        hist_thresholds(thr);
-        TN_HIP(hipMemcpy(p->dThr, thr, sizeof thr, hipMemcpyHostToDevice));
-        // the seeds
-        if (n > 0) {
-            float* dP3 = p->alloc<float>((size_t)3 * n);
-            TN_HIP(hipMemcpy(dP3, seeds->P, sizeof(float) * 3 * n, hipMemcpyHostToDevice));
-            k_init_nodes<<<blocks_for(n, 256), 256, 0, p->stream>>>(f, dP3, n, p->dX0);
-            TN_HIP(hipGetLastError());
-            std::vector<uint32_t> part(n);
-            for (int i = 0; i < n; ++i) {
-                part[i] = (uint32_t)seeds->part[2 * i] | ((uint32_t)seeds->part[2 * i + 1] << 16);
-            }
-            TN_HIP(hipMemcpy(p->dPart0, part.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
-            TN_HIP(hipMemcpy(p->dTyp0, seeds->typ, n, hipMemcpyHostToDevice));
-            TN_HIP(hipMemcpy(p->dLab0, seeds->lab, sizeof(uint16_t) * n, hipMemcpyHostToDevice));
-        }
-        p->reset();
-        *out = p.release();
+        int64_t nkeys = 0;
+        const tnd::Hash H = make_hash(g, prm, &nkeys);
+        *out = plan_build(prm, seeds, dev, H, nkeys, [&](DevMem& mem) { return upload_grid(g, mem); });
     });
 }
 

@@ -106,6 +106,33 @@ class Resident:
         _check(_bind().tn_grid_get_info(self.h, C.byref(gi)))
         return {k: getattr(gi, k) for k, _ in GridInfoC._fields_}
 
+    def seed(self, prm=None):
+        """seed_cpu on this grid without leaving the device (tn_seed_from_grid)."""
+        from ._abi import Nodes, RelaxParams, RelaxParamsC
+
+        L = _bind()
+        pc = (prm or RelaxParams()).to_c()
+        vp = C.c_void_p
+        L.tn_seed_count_from_grid.argtypes = [vp, C.POINTER(RelaxParamsC), C.POINTER(C.c_int64)]
+        L.tn_seed_from_grid.argtypes = [vp, C.POINTER(RelaxParamsC), C.c_int64] + [vp] * 5
+        n = C.c_int64(0)
+        _check(L.tn_seed_count_from_grid(self.h, C.byref(pc), C.byref(n)))
+        n = int(n.value)
+        P = np.zeros((n, 3), np.float32)
+        lab = np.zeros(n, np.uint16)
+        typ = np.zeros(n, np.uint8)
+        part = np.zeros((n, 2), np.uint16)
+        part3 = np.zeros(n, np.uint16)
+        ptr = lambda a: a.ctypes.data if a.size else None
+        _check(L.tn_seed_from_grid(self.h, C.byref(pc), n, ptr(P), ptr(lab), ptr(typ), ptr(part), ptr(part3)))
+        return Nodes(P, lab, typ, part, part3)
+
+    def plan(self, seeds, prm=None):
+        """A relaxation plan over this grid (tn_plan_create_from_grid)."""
+        from .relax import Plan
+
+        return Plan(None, seeds, prm, resident=self)
+
     def close(self):
         if self.h:
             _bind().tn_grid_destroy(self.h)

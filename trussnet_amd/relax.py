@@ -30,7 +30,7 @@ SYMBOLS = ["tn_relax_params_default", "tn_last_error", "tn_device_count", "tn_re
            "tn_plan_reset", "tn_plan_rebuild", "tn_plan_force", "tn_plan_step", "tn_plan_set_timing",
            "tn_plan_timings", "tn_adj_create", "tn_adj_forward", "tn_adj_backward", "tn_adj_loss_terms",
            "tn_adj_destroy", "tn_grid_params_default", "tn_grid_build", "tn_grid_get_info", "tn_grid_download",
-           "tn_grid_destroy"]
+           "tn_grid_destroy", "tn_seed_count_from_grid", "tn_seed_from_grid", "tn_plan_create_from_grid"]
 
 
 class DeviceError(RuntimeError):
@@ -173,14 +173,24 @@ def seed(grid: Grid, prm: RelaxParams | None = None, device: int = -1) -> Nodes:
 class Plan:
     """A grid and its seed nodes resident on one device (tn_plan_* of the C ABI)."""
 
-    def __init__(self, grid: Grid, seeds: Nodes, prm: RelaxParams | None = None, device: int = -1):
+    def __init__(self, grid: Grid | None, seeds: Nodes, prm: RelaxParams | None = None, device: int = -1,
+                 resident=None):
+        """resident: a grid built on the device (trussnet_amd.gridbuild.Resident) to
+        plan over instead of a host grid (tn_plan_create_from_grid)."""
         self.prm = prm or RelaxParams()
         self.grid = grid
         self.seeds = seeds
         self.n = seeds.n
         self._h = C.c_void_p()
-        gv, pc, nv = grid.view(), self.prm.to_c(), seeds.view()
-        _check(lib().tn_plan_create(C.byref(gv), C.byref(pc), C.byref(nv), device, C.byref(self._h)))
+        pc, nv = self.prm.to_c(), seeds.view()
+        if resident is not None:
+            L = lib()
+            L.tn_plan_create_from_grid.argtypes = [C.c_void_p, C.POINTER(RelaxParamsC), C.POINTER(NodesView),
+                                                   C.POINTER(C.c_void_p)]
+            _check(L.tn_plan_create_from_grid(resident.h, C.byref(pc), C.byref(nv), C.byref(self._h)))
+        else:
+            gv = grid.view()
+            _check(lib().tn_plan_create(C.byref(gv), C.byref(pc), C.byref(nv), device, C.byref(self._h)))
         self.hash = self.hash_info()
 
     def close(self):
