@@ -28,7 +28,7 @@ sources) on this host's cores, on the same grid and seeds, for a bounded
 number of iterations.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
-                       [--workload large|small|prob|voxel|gray|batch|adjoint|grid]
+                       [--workload large|small|prob|voxel|gray|batch|adjoint|grid|pipeline]
 """
 from __future__ import annotations
 
@@ -406,6 +406,76 @@ def bench_grid(args) -> int:
     return 0
 
 
+def bench_pipeline(args) -> int:
+    """Labels to relaxed nodes on the device: tn_grid_build of the large workload's
+    256^3 volume (hbase 4.5, hmin 1.5, hmax 6 voxels, the reference's defaults
+    otherwise), tn_seed_from_grid, tn_plan_create_from_grid, 100 iterations, the
+    nodes downloaded. A step is the whole chain; the reference's build_grid_cpu,
+    seed_cpu and relax_cpu (20 iterations, scaled to 100) time the same chain."""
+    from trussnet_amd import gridbuild, volumes
+    from trussnet_amd._abi import GridParams, RelaxParams
+
+    lab = volumes.nested_ellipsoids(256, nlab=6, seed=0, axes=(20.0, 120.0), grf_sigma=4.0, grf_amp=3.0)
+    gprm = GridParams(hbase=4.5, hmin=1.5, hmax=6.0)
+    iters = args.iters or 100
+    rprm = RelaxParams(max_iters=iters, dptol=0.0)
+
+    def chain():
+        t0 = time.perf_counter()
+        res = gridbuild.build_resident(lab, prm=gprm, device=0)
+        t1 = time.perf_counter()
+        seeds = res.seed(RelaxParams())
+        t2 = time.perf_counter()
+        with res.plan(seeds, rprm) as plan:
+            res.close()
+            st = plan.run()
+            out = plan.download()
+        t3 = time.perf_counter()
+        return seeds.n, st, (t1 - t0, t2 - t1, t3 - t2)
+
+    for _ in range(args.warmup):
+        chain()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        n, st, parts = chain()
+    dt = (time.perf_counter() - t) / args.steps
+    log(f"pipeline: {n} nodes; grid {1e3 * parts[0]:.1f} ms, seeds {1e3 * parts[1]:.1f} ms, "
+        f"plan + {iters} iterations + download {1e3 * parts[2]:.1f} ms")
+    cpu = None
+    if not args.no_cpu:
+        try:
+            from oracle import ref
+
+            if not ref.available():
+                raise RuntimeError("reference library not built")
+            t0 = time.perf_counter()
+            r = ref.RefGrid.build_full(lab, prm=gprm)
+            t1 = time.perf_counter()
+            sd = r.seed(RelaxParams())
+            t2 = time.perf_counter()
+            _, stc = r.relax(sd, RelaxParams(max_iters=args.cpu_iters, dptol=0.0))
+            t3 = time.perf_counter()
+            total = (t1 - t0) + (t2 - t1) + (t3 - t2) * iters / max(1, stc["iters"])
+            cores = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 64))
+            cpu = {"value": total, "unit": "s", "cores": cores, "kind": "reference",
+                   "sample": f"build_grid_cpu {t1 - t0:.2f} s + seed_cpu {t2 - t1:.2f} s ({sd.n} nodes) + "
+                             f"relax_cpu {stc['iters']} iterations {t3 - t2:.2f} s scaled to {iters}"}
+        except Exception as e:
+            log(f"reference CPU path unavailable ({e})")
+    line = {
+        "metric": "seconds from a 256^3 label volume to relaxed nodes (grid build, seeding, relaxation)",
+        "value": dt, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plain_large's volume with the reference's curvature sizing (GridParams hbase 4.5, "
+                               f"hmin 1.5, hmax 6), seed_cpu defaults, {iters} iterations, labels up and nodes down "
+                               "over PCIe included", "nodes": n, "iterations": iters,
+                   "stage_ms": [round(1e3 * x, 2) for x in parts], "parallelism": "single GPU"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def cpu_baseline_many(grids, seeds, iters: int) -> dict | None:
     from trussnet_amd._abi import RelaxParams
 
@@ -435,8 +505,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["large", "small", "prob", "voxel", "gray", "batch", "adjoint", "grid"],
-                    default="large")
+    ap.add_argument("--workload", choices=["large", "small", "prob", "voxel", "gray", "batch", "adjoint", "grid",
+                                           "pipeline"], default="large")
     ap.add_argument("--iters", type=int, default=0, help="iterations per step (default: the workload's)")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
@@ -454,6 +524,8 @@ def main() -> int:
         return bench_adjoint(args)
     if args.workload == "grid":
         return bench_grid(args)
+    if args.workload == "pipeline":
+        return bench_pipeline(args)
     pmc = None if args.no_traffic else pmc_traffic(args)   # before this process initialises the GPU
 
     from trussnet_amd import relax
