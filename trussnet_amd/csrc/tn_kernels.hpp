@@ -1139,7 +1139,7 @@ __global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl*
 }
 
 #ifndef TN_REST_LANES
-#define TN_REST_LANES 4
+#define TN_REST_LANES 2
 #endif
 constexpr int kRestLanes = TN_REST_LANES;   // nodes per wave in k_move<kGeneral>
 
