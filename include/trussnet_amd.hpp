@@ -70,9 +70,55 @@ public:
         return p_;
     }
 
+    // a plan over a device-built grid (tn_plan_create_from_grid)
+    static Plan from_grid(const tn_grid* g, const tn_relax_params& prm, const tn_nodes_view& seeds) {
+        Plan p;
+        check(tn_plan_create_from_grid(g, &prm, &seeds, &p.p_), "tn_plan_create_from_grid");
+        return p;
+    }
+
 private:
+    Plan() = default;
     tn_plan* p_ = nullptr;
 };
+
+// A grid built on the device (tn_grid_build), released on destruction.
+class DeviceGrid {
+public:
+    DeviceGrid(const tn_volume_view& vol, const tn_grid_params& prm, int device = -1) {
+        check(tn_grid_build(&vol, &prm, device, &g_), "tn_grid_build");
+    }
+    DeviceGrid(const DeviceGrid&) = delete;
+    DeviceGrid& operator=(const DeviceGrid&) = delete;
+    DeviceGrid(DeviceGrid&& o) noexcept : g_(std::exchange(o.g_, nullptr)) {}
+    ~DeviceGrid() {
+        if (g_) {
+            tn_grid_destroy(g_);
+        }
+    }
+    tn_grid_info info() const {
+        tn_grid_info in;
+        check(tn_grid_get_info(g_, &in), "tn_grid_get_info");
+        return in;
+    }
+    int64_t seed_count(const tn_relax_params& prm) const {
+        int64_t n = 0;
+        check(tn_seed_count_from_grid(g_, &prm, &n), "tn_seed_count_from_grid");
+        return n;
+    }
+    const tn_grid* get() const noexcept {
+        return g_;
+    }
+
+private:
+    tn_grid* g_ = nullptr;
+};
+
+inline tn_grid_params default_grid_params() {
+    tn_grid_params p;
+    tn_grid_params_default(&p);
+    return p;
+}
 
 }  // namespace tnamd
 

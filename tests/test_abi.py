@@ -185,12 +185,16 @@ def test_cpp_wrapper_builds_and_links(tnlib, tmp_path):
         '#include "trussnet_amd.hpp"\n#include <cstdio>\n'
         "int main() {\n"
         "  tn_relax_params p = tnamd::default_params();\n"
+        "  tn_grid_params gp = tnamd::default_grid_params();\n"
+        "  tn_volume_view v{};\n"
+        "  try { tnamd::DeviceGrid g(v, gp); } catch (const tnamd::Error& e) { std::printf(\"%d \", e.code()); }\n"
         "  try { tnamd::check(TN_ERR_ARG, \"probe\"); } catch (const std::runtime_error& e) {\n"
-        "    std::printf(\"%.3f %d %d\\n\", p.t, p.max_iters, static_cast<const tnamd::Error&>(e).code());\n"
+        "    std::printf(\"%.3f %d %d %.1f\\n\", p.t, p.max_iters, static_cast<const tnamd::Error&>(e).code(),\n"
+        "                gp.sigma_curv);\n"
         "  }\n  return 0;\n}\n")
     exe = tmp_path / "use"
     libdir = os.path.dirname(tnlib.LIB_PATH)
     subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",
                     str(exe), f"-L{libdir}", "-ltrussnet_amd", f"-Wl,-rpath,{libdir}"], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
-    assert out == ["1.300", "500", str(1)]
+    assert out == [str(1), "1.300", "500", str(1), "2.0"]   # an empty volume is refused on the host
