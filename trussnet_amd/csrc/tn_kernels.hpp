@@ -944,6 +944,7 @@ __global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ X, 
 struct MoveArgs {
     float dt, maxstep, snap;
     int fire;
+    int lanes;   // nodes per wave of the general pass at least (a launch shape; results do not depend on it)
 };
 
 __device__ __forceinline__ uint32_t pack_part(int b, int c) {
@@ -1245,7 +1246,8 @@ __device__ __forceinline__ void k_move_body(Field f, MoveArgs ma, Ctl* __restric
     if (kLean) {
         lpw = spread < 1 ? 1 : (spread > kRestLanes ? kWave : spread);
     } else {
-        lpw = spread < kRestLanes ? kRestLanes : (spread > kWave ? kWave : spread);
+        const int rl = ma.lanes > 0 ? ma.lanes : kRestLanes;
+        lpw = spread < rl ? rl : (spread > kWave ? kWave : spread);
     }
     const int per_block = lpw * kWaves;
     const int lane = threadIdx.x & (kWave - 1);

@@ -333,6 +333,7 @@ struct tn_plan {
     hipStream_t stream = nullptr;
     tn_relax_params prm{};
     int n = 0, fire = 0, rebuild_div = 300;
+    int rest_lanes = tnk::kRestLanes;   // the general move pass's nodes per wave (a launch shape)
     int64_t nkeys = 0;
     tnd::Field f{};
     tnd::Hash H{};
@@ -517,7 +518,7 @@ struct tn_plan {
         tnk::k_force<<<nb, tnk::kBlock, 0, stream>>>(dX, dTyp, dNbrT, dNnb, prm.fscale, prm.fsurf, n, dF);
         timer.end(KC_FORCE, t);
         t = timer.begin();
-        tnk::MoveArgs ma{ prm.dt, prm.maxstep, prm.snap, fire };
+        tnk::MoveArgs ma{ prm.dt, prm.maxstep, prm.snap, fire, rest_lanes };
         if (prm.voxel_trap) {   // the staircase move, every node in one pass
             tnk::k_move_voxel<<<nb, tnk::kBlock, 0, stream>>>(f, ma, dCtl, n, dF, dX, dV, dTyp, dPart, dLab, dMv,
                                                               dPw);
@@ -629,7 +630,7 @@ struct tn_plan {
         a.f = f;
         a.H = H;
         a.tin = truss_in();
-        a.ma = tnk::MoveArgs{ prm.dt, prm.maxstep, prm.snap, fire };
+        a.ma = tnk::MoveArgs{ prm.dt, prm.maxstep, prm.snap, fire, rest_lanes };
         a.ca = tnk::CtlArgs{ n, prm.max_iters, fire, rebuild_div, prm.dptol, nparts };
         a.n = n;
         a.fire = fire;
@@ -1320,17 +1321,60 @@ void run_grouped(Plans& plans, int64_t& launches, int64_t& steps) {
         TN_HIP(hipMemcpyAsync(dmap, hmap, sizeof(int32_t) * o, hipMemcpyHostToDevice, bs));
     };
     build_maps();
+    // Launch-shape tuning: the general move passes' nodes per wave, tried in turn on
+    // the first grouped iterations (kTuneIters each, timed by events around the two
+    // general passes), the fastest kept. The nodes' results do not depend on it.
+    static constexpr int kTuneLanes[] = { 1, 2, 4, 8, 16 };
+    constexpr int kTuneIters = 2;
+    const int ncand = (int)(sizeof kTuneLanes / sizeof kTuneLanes[0]);
+    std::vector<double> tune_ms(ncand, 0.0);
+    hipEvent_t tune_ev[4];
+    for (hipEvent_t& e : tune_ev) {
+        TN_HIP(hipEventCreate(&e));
+    }
+    struct EventsGone {
+        hipEvent_t* e;
+        ~EventsGone() {
+            for (int k = 0; k < 4; ++k) {
+                (void)hipEventDestroy(e[k]);
+            }
+        }
+    } events_gone{ tune_ev };
+    int tune_step = std::getenv("TN_NO_TUNE") ? ncand * kTuneIters : 0;
+    auto set_lanes = [&](int lanes) {
+        for (int k = 0; k < count; ++k) {
+            plans[k]->rest_lanes = lanes;
+            htab[k].ma.lanes = lanes;
+        }
+        TN_HIP(hipMemcpyAsync(dtab, htab, sizeof(tnk::PlanIter) * count, hipMemcpyHostToDevice, bs));
+    };
+    if (tune_step < ncand * kTuneIters) {
+        set_lanes(kTuneLanes[0]);
+    }
     while (!active.empty()) {
         using namespace tnk;
+        const bool tuning = tune_step < ncand * kTuneIters;
         kg_iter_begin<<<nblocks[kShOne], 64, 0, bs>>>(dtab, bplan[kShOne], bfirst[kShOne]);
         kg_force<<<nblocks[kShNodes], kBlock, 0, bs>>>(dtab, bplan[kShNodes], bfirst[kShNodes]);
         auto moves = [&](auto plain) {
             constexpr bool kP = decltype(plain)::value;
             kg_move_easy<kP><<<nblocks[kShNodes], kBlock, 0, bs>>>(dtab, bplan[kShNodes], bfirst[kShNodes]);
+            if (tuning) {
+                TN_HIP(hipEventRecord(tune_ev[0], bs));
+            }
             kg_move<0, kP><<<nblocks[kShGen], kBlock, 0, bs>>>(dtab, bplan[kShGen], bfirst[kShGen]);
+            if (tuning) {
+                TN_HIP(hipEventRecord(tune_ev[1], bs));
+            }
             kg_move<1, kP><<<nblocks[kShLean], kBlock, 0, bs>>>(dtab, bplan[kShLean], bfirst[kShLean]);
             kg_move<2, kP><<<nblocks[kShLean], kBlock, 0, bs>>>(dtab, bplan[kShLean], bfirst[kShLean]);
+            if (tuning) {
+                TN_HIP(hipEventRecord(tune_ev[2], bs));
+            }
             kg_move<3, kP><<<nblocks[kShGen], kBlock, 0, bs>>>(dtab, bplan[kShGen], bfirst[kShGen]);
+            if (tuning) {
+                TN_HIP(hipEventRecord(tune_ev[3], bs));
+            }
         };
         if (all_plain) {
             moves(std::true_type{});
@@ -1346,6 +1390,29 @@ void run_grouped(Plans& plans, int64_t& launches, int64_t& steps) {
         TN_HIP(hipMemcpyAsync(hctl, dctl, sizeof(tnk::Ctl) * count, hipMemcpyDeviceToHost, bs));
         TN_HIP(hipStreamSynchronize(bs));
         ++launches;
+        if (tuning) {
+            float a = 0.0f, b = 0.0f;
+            TN_HIP(hipEventElapsedTime(&a, tune_ev[0], tune_ev[1]));
+            TN_HIP(hipEventElapsedTime(&b, tune_ev[2], tune_ev[3]));
+            tune_ms[tune_step / kTuneIters] += (double)a + (double)b;
+            ++tune_step;
+            if (tune_step == ncand * kTuneIters) {   // keep the fastest
+                int best = 0;
+                for (int c = 1; c < ncand; ++c) {
+                    best = tune_ms[c] < tune_ms[best] ? c : best;
+                }
+                set_lanes(kTuneLanes[best]);
+                if (std::getenv("TN_BATCH_VERBOSE")) {
+                    std::fprintf(stderr, "[tn_relax_batch] general pass lanes:");
+                    for (int c = 0; c < ncand; ++c) {
+                        std::fprintf(stderr, " %d: %.3f ms", kTuneLanes[c], tune_ms[c] / kTuneIters);
+                    }
+                    std::fprintf(stderr, " -> %d\n", kTuneLanes[best]);
+                }
+            } else if (tune_step % kTuneIters == 0) {
+                set_lanes(kTuneLanes[tune_step / kTuneIters]);
+            }
+        }
         bool changed = false;
         for (size_t a = 0; a < active.size();) {
             const int k = active[a];
