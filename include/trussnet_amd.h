@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define TN_ABI_VERSION 1
+#define TN_ABI_VERSION 2
 
 /* Sizes fixed by the node and field formats. */
 #define TN_K 24          /* truss neighbours per node: V2M_K, src/opencl/v2m_particle_body.cl:18 */
@@ -105,7 +105,11 @@ typedef struct tn_nodes_view {
 } tn_nodes_view;
 
 /* RelaxStats (src/v2m_particles.h:89-96). The ms_* fields are device times while
- * kernel timing is on (tn_plan_set_timing), else 0. */
+ * kernel timing is on (tn_plan_set_timing), else 0. power_fallbacks (no counterpart
+ * in RelaxStats) counts the FIRE iterations whose power sign the device could not
+ * certify from its tree sum and took from the serial sum in node order, the CPU's
+ * own summation (src/v2m_particles.cpp:849-855). Setting TN_SERIAL_POWER=1 in the
+ * environment before a plan is created makes every FIRE iteration take that path. */
 typedef struct tn_relax_stats {
     int32_t iters, rebuilds;
     float last_move, last_p99;
@@ -113,6 +117,7 @@ typedef struct tn_relax_stats {
     float fire_dt;
     int64_t n_interior, n_interface, n_junction, n_corner;
     double ms_seed, ms_hash, ms_force, ms_move;
+    int32_t power_fallbacks;
 } tn_relax_stats;
 
 /* The RelaxParams defaults (src/v2m_particles.h:22-39), rebuild_div 300. */
@@ -214,6 +219,8 @@ typedef struct tn_grid_info {
     int32_t limit_sweeps;         /* gradient-limiting sweeps run */
     int32_t overflow_bricks;      /* bricks with more than 16 labels nearby */
     int32_t gm;                   /* gray-scale thresholds (0: none) */
+    int32_t prob_fields;          /* Grid::prob_fields: the fields are probabilities (a tissue-probability
+                                     input, src/v2m_grid.cpp:343-345); tessellation reads it */
 } tn_grid_info;
 
 typedef struct tn_grid tn_grid;
@@ -308,6 +315,17 @@ int tn_plan_timings(const tn_plan* plan, tn_kernel_time* out, int32_t cap, int32
 int tn_plan_rebuild(tn_plan* plan, int32_t* key, int32_t* sorted, int32_t* cstart, int32_t* nbr, int32_t* nnb);
 int tn_plan_force(tn_plan* plan, float* F);
 int tn_plan_step(tn_plan* plan, int32_t iters, tn_relax_stats* st);
+
+/*
+ * The field queries of the plan's grid at m points X[3m] (host arrays), as the
+ * move evaluates them on the device: phi[k] = phi_a (v2m_phi_f,
+ * src/opencl/v2m_seed_body.cl:33-39), psi[k] and grad[3k..3k+2] = psi_ab and its
+ * gradient (v2m_psi_grad, :211-247), label[k] and second[k] = the owner and the
+ * runner-up (v2m_label_of, :116-160, over v2m_labels_near, :42-59; TN_NOLAB when
+ * there is none). Any output may be null.
+ */
+int tn_plan_field_points(tn_plan* plan, int32_t a, int32_t b, int64_t m, const float* X, float* phi, float* psi,
+                         float* grad, int32_t* label, int32_t* second);
 
 /*
  * A differentiable relaxation and its adjoint. No counterpart in the reference

@@ -106,6 +106,11 @@ public:
         check(tn_seed_count_from_grid(g_, &prm, &n), "tn_seed_count_from_grid");
         return n;
     }
+    // the Grid's arrays to the host (any pointer may be null; sizes in tn_grid_download)
+    void download(int32_t* bl_cnt, uint16_t* bl_lab, int32_t* bl_slot, float* phi, float* h, uint8_t* grade,
+                  float* gTW) const {
+        check(tn_grid_download(g_, bl_cnt, bl_lab, bl_slot, phi, h, grade, gTW), "tn_grid_download");
+    }
     const tn_grid* get() const noexcept {
         return g_;
     }

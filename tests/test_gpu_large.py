@@ -3,8 +3,8 @@
 The large graded six-label volume (256^3, about 2 M nodes) and the
 probability-input volume (182x218x182x6, about 1.4 M nodes) are built here,
 seeded on the device (the reference's seed_cpu, defaults: junction-line seeds
-and corners included), relaxed for 30 iterations on the device and by the CPU
-restatement, and compared bit for bit, with the statistics.
+and corners included), relaxed for the benchmark's 100 iterations on the device
+and by the CPU restatement, and compared bit for bit, with the statistics.
 """
 import numpy as np
 import pytest
@@ -26,6 +26,7 @@ def tn():
     return relax
 
 
+@pytest.mark.timeout(600)
 @pytest.mark.parametrize("name", ["plain_large", "prob_large"])
 def test_full_size_vs_oracle(tn, oracle_lib, name):
     from trussnet_amd import workload
@@ -33,11 +34,11 @@ def test_full_size_vs_oracle(tn, oracle_lib, name):
     grid = getattr(workload, name)()
     seeds = tn.seed(grid, RelaxParams(), device=0)
     assert seeds.n > 1_000_000
-    prm = RelaxParams(max_iters=30, dptol=0.0)
+    prm = RelaxParams(max_iters=100, dptol=0.0)
     print(f"[large] {name}: {seeds.n} nodes, {prm.max_iters} iterations", flush=True)
     got, st = tn.relax(grid, seeds, prm, device=0)
     want, sw = oracle_lib.relax(grid, seeds, prm)
     assert got.equal(want), (name, int((got.P != want.P).any(axis=1).sum()))
     for k in STAT_KEYS:
         assert np.float32(st[k]) == np.float32(sw[k]), (name, k, st[k], sw[k])
-    assert st["rebuilds"] >= 5 and st["n_junction"] > 0 and st["n_corner"] >= 0
+    assert st["rebuilds"] >= 10 and st["n_junction"] > 0 and st["n_corner"] >= 0

@@ -61,6 +61,7 @@ class RelaxStatsC(C.Structure):
         ("fire_resets", C.c_int32), ("fire_dt", C.c_float),
         ("n_interior", C.c_int64), ("n_interface", C.c_int64), ("n_junction", C.c_int64), ("n_corner", C.c_int64),
         ("ms_seed", C.c_double), ("ms_hash", C.c_double), ("ms_force", C.c_double), ("ms_move", C.c_double),
+        ("power_fallbacks", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -103,7 +104,8 @@ class GridInfoC(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("nbx", C.c_int32), ("nby", C.c_int32),
                 ("nbz", C.c_int32), ("nslot", C.c_int64), ("hbase", C.c_float), ("hmin", C.c_float),
                 ("hmax", C.c_float), ("hcurv", C.c_float), ("R", C.c_int32), ("limit_sweeps", C.c_int32),
-                ("overflow_bricks", C.c_int32), ("gm", C.c_int32)]
+                ("overflow_bricks", C.c_int32), ("gm", C.c_int32),
+                ("prob_fields", C.c_int32)]
 
 
 @dataclass

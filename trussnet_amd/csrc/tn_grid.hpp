@@ -1020,6 +1020,7 @@ int tn_grid_build(const tn_volume_view* vol, const tn_grid_params* prm, int32_t 
         const tng::Source labels_src{ nullptr, 0, (size_t)nv };
         const bool tpm = vol->prob && vol->nprob > 0;
         const bool gray = !tpm && vol->nthr > 0;
+        I.prob_fields = tpm ? 1 : 0;   // Grid::prob_fields (src/v2m_grid.cpp:343-345)
         const tng::Source src{ tpm ? tmp.upload(vol->prob, (size_t)nv * vol->nprob) : nullptr, tpm ? vol->nprob : 0,
                                (size_t)nv };
         auto smooth = [&](const tng::Source& s, float sigma, int R, float* dst) {

@@ -1399,12 +1399,51 @@ __global__ __launch_bounds__(kBlock) void k_stats(const float4* __restrict__ X, 
     k_stats_body(X, P0, mv, pw, n, skin, fire, thr, ctl, refresh, partials, blockIdx.x, gridDim.x);
 }
 
+// ---- field queries at given points (the queries the move makes, for checking) ---------
+
+// phi_a (v2m_phi_f), psi_ab and its gradient (v2m_psi_grad), the owner and the
+// runner-up (v2m_label_of) at X[3k..3k+2]; any output may be null.
+__global__ __launch_bounds__(kBlock) void k_field_points(Field f, int a, int b, int m, const float* __restrict__ X,
+                                                         float* __restrict__ phi, float* __restrict__ psi,
+                                                         float* __restrict__ grad, int32_t* __restrict__ label,
+                                                         int32_t* __restrict__ second) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= m) {
+        return;
+    }
+    const P3 p = p3(X[3 * k], X[3 * k + 1], X[3 * k + 2]);
+    if (phi) {
+        phi[k] = phi_point(f, a, p);
+    }
+    if (psi || grad) {
+        const float4 g = psi_grad_point(f.self, a, b, p);
+        if (psi) {
+            psi[k] = g.w;
+        }
+        if (grad) {
+            grad[3 * k] = g.x;
+            grad[3 * k + 1] = g.y;
+            grad[3 * k + 2] = g.z;
+        }
+    }
+    if (label || second) {
+        const int2 o = owner_at(f, p);
+        if (label) {
+            label[k] = o.x;
+        }
+        if (second) {
+            second[k] = o.y;
+        }
+    }
+}
+
 // ---- control (one block) ---------------------------------------------------------------
 
 struct CtlArgs {
     int n, max_iters, fire, rebuild_div;
     float dptol;
     int nparts;
+    int force_serial;   // take the serial power sum every FIRE iteration (the exact path, for its tests)
 };
 
 // FIRE's power P = sum pw, whose sign (P >= 0) drives FireCtl::update
@@ -1441,7 +1480,7 @@ __device__ __forceinline__ void k_control_body(CtlArgs ca, Ctl* __restrict__ ctl
     if (tid == 0) {
         const double A = red[0][1];
         const double bound = 2.0 * ((double)ca.n + 64.0) * 0x1p-53 * A;
-        use_serial = ca.fire && A > 0.0 && !(fabs(P) > bound);
+        use_serial = ca.fire && (ca.force_serial || (A > 0.0 && !(fabs(P) > bound)));
     }
     __syncthreads();
     if (use_serial) {   // the CPU's own summation, in node index (id) order

@@ -333,6 +333,7 @@ struct tn_plan {
     hipStream_t stream = nullptr;
     tn_relax_params prm{};
     int n = 0, fire = 0, rebuild_div = 300;
+    int force_serial = 0;   // TN_SERIAL_POWER: the serial power sum every FIRE iteration
     int rest_lanes = tnk::kRestLanes;   // the general move pass's nodes per wave (a launch shape)
     int64_t nkeys = 0;
     tnd::Field f{};
@@ -591,7 +592,7 @@ struct tn_plan {
         tnk::k_truss_node<<<refresh_blocks, tnk::kWave, 0, stream>>>(H, truss_in(), n, dQueue, dCtl, dNbrT, dNnb, dP0);
         timer.end(KC_REFRESH, t);
         t = timer.begin();
-        tnk::CtlArgs ca{ n, prm.max_iters, fire, rebuild_div, prm.dptol, nparts };
+        tnk::CtlArgs ca{ n, prm.max_iters, fire, rebuild_div, prm.dptol, nparts, force_serial };
         tnk::k_control<<<1, tnk::kBlock, 0, stream>>>(ca, dCtl, dPartials, dPw, dSlotOf);
         timer.end(KC_CONTROL, t);
         TN_HIP(hipGetLastError());
@@ -631,7 +632,7 @@ struct tn_plan {
         a.H = H;
         a.tin = truss_in();
         a.ma = tnk::MoveArgs{ prm.dt, prm.maxstep, prm.snap, fire, rest_lanes };
-        a.ca = tnk::CtlArgs{ n, prm.max_iters, fire, rebuild_div, prm.dptol, nparts };
+        a.ca = tnk::CtlArgs{ n, prm.max_iters, fire, rebuild_div, prm.dptol, nparts, force_serial };
         a.n = n;
         a.fire = fire;
         a.nparts = nparts;
@@ -702,6 +703,7 @@ struct tn_plan {
         st->last_p99 = c.last_p99;
         st->fire_resets = c.fire_resets;
         st->fire_dt = c.fire_dt;
+        st->power_fallbacks = c.power_fallbacks;
         const double* ms = timer.ms;
         st->ms_hash = ms[KC_KEYS] + ms[KC_SCAN] + ms[KC_RADIX] + ms[KC_PERMUTE] + ms[KC_NEIGHBORS];
         st->ms_force = ms[KC_FORCE];
@@ -749,6 +751,10 @@ tn_plan* plan_build(const tn_relax_params* prm, const tn_nodes_view* seeds, int 
         }
         p->timer.stream = p->stream;
         p->timer.on = prm->verbose || std::getenv("TN_KERNEL_TIMING") != nullptr;
+        {
+            const char* e = std::getenv("TN_SERIAL_POWER");
+            p->force_serial = e && e[0] && e[0] != '0' ? 1 : 0;
+        }
         const int n = p->n;
         p->mem.device = dev;
         p->f = make_grid(p->mem);
@@ -1689,6 +1695,40 @@ int tn_plan_force(tn_plan* p, float* F) {
         for (int t = 0; t < n; ++t) {   // by node id
             std::memcpy(F + 4 * (size_t)id[t], &Fs[t], sizeof(float4));
         }
+    });
+}
+
+int tn_plan_field_points(tn_plan* p, int32_t a, int32_t b, int64_t m, const float* X, float* phi, float* psi,
+                         float* grad, int32_t* label, int32_t* second) {
+    return guarded([&] {
+        arg_check(p != nullptr && m >= 0 && m < (1ll << 28) && (m == 0 || X != nullptr), "null argument or bad count");
+        arg_check(a >= 0 && a < TN_NOLAB && b >= 0 && b < TN_NOLAB, "label out of range");
+        if (m == 0) {
+            return;
+        }
+        TN_HIP(hipSetDevice(p->device));
+        DevMem mem;
+        mem.device = p->device;
+        float* dX = mem.upload(X, (size_t)3 * m);
+        float* dphi = phi ? mem.alloc<float>(m) : nullptr;
+        float* dpsi = psi ? mem.alloc<float>(m) : nullptr;
+        float* dgrad = grad ? mem.alloc<float>((size_t)3 * m) : nullptr;
+        int32_t* dlab = label ? mem.alloc<int32_t>(m) : nullptr;
+        int32_t* dsec = second ? mem.alloc<int32_t>(m) : nullptr;
+        tnk::k_field_points<<<blocks_for(m, tnk::kBlock), tnk::kBlock, 0, p->stream>>>(p->f, a, b, (int)m, dX, dphi,
+                                                                                       dpsi, dgrad, dlab, dsec);
+        TN_HIP(hipGetLastError());
+        TN_HIP(hipStreamSynchronize(p->stream));
+        auto down = [&](void* h, const void* d, size_t bytes) {
+            if (h) {
+                TN_HIP(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost));
+            }
+        };
+        down(phi, dphi, sizeof(float) * m);
+        down(psi, dpsi, sizeof(float) * m);
+        down(grad, dgrad, sizeof(float) * 3 * m);
+        down(label, dlab, sizeof(int32_t) * m);
+        down(second, dsec, sizeof(int32_t) * m);
     });
 }
 

@@ -30,7 +30,8 @@ SYMBOLS = ["tn_relax_params_default", "tn_last_error", "tn_device_count", "tn_re
            "tn_plan_reset", "tn_plan_rebuild", "tn_plan_force", "tn_plan_step", "tn_plan_set_timing",
            "tn_plan_timings", "tn_adj_create", "tn_adj_forward", "tn_adj_backward", "tn_adj_loss_terms",
            "tn_adj_destroy", "tn_grid_params_default", "tn_grid_build", "tn_grid_get_info", "tn_grid_download",
-           "tn_grid_destroy", "tn_seed_count_from_grid", "tn_seed_from_grid", "tn_plan_create_from_grid"]
+           "tn_grid_destroy", "tn_seed_count_from_grid", "tn_seed_from_grid", "tn_plan_create_from_grid",
+           "tn_plan_field_points"]
 
 
 class DeviceError(RuntimeError):
@@ -75,6 +76,7 @@ def lib():
         L.tn_plan_reset.argtypes = [vp]
         L.tn_plan_rebuild.argtypes = [vp] + [vp] * 5
         L.tn_plan_force.argtypes = [vp, vp]
+        L.tn_plan_field_points.argtypes = [vp, C.c_int32, C.c_int32, C.c_int64, vp, vp, vp, vp, vp, vp]
         L.tn_plan_step.argtypes = [vp, C.c_int32, C.POINTER(RelaxStatsC)]
         L.tn_plan_set_timing.argtypes = [vp, C.c_int32]
         L.tn_plan_timings.argtypes = [vp, C.POINTER(KernelTimeC), C.c_int32, C.POINTER(C.c_int32)]
@@ -248,6 +250,18 @@ class Plan:
         F = np.zeros(self.n * 4, np.float32)
         _check(lib().tn_plan_force(self._h, _p(F)))
         return F
+
+    def field_points(self, a: int, b: int, X: np.ndarray):
+        """The device's field queries at points X (m x 3): phi_a, psi_ab, its gradient,
+        the owner and the runner-up (tn_plan_field_points)."""
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, 3)
+        m = X.shape[0]
+        phi, psi = np.zeros(m, np.float32), np.zeros(m, np.float32)
+        grad = np.zeros((m, 3), np.float32)
+        lab, sec = np.zeros(m, np.int32), np.zeros(m, np.int32)
+        _check(lib().tn_plan_field_points(self._h, int(a), int(b), m, _p(X), _p(phi), _p(psi), _p(grad), _p(lab),
+                                          _p(sec)))
+        return phi, psi, grad, lab, sec
 
     def set_timing(self, on: bool = True):
         """Per-kernel device timing (HIP events on the plan's stream); clears the totals."""
