@@ -139,6 +139,24 @@ int tn_relax(const tn_grid_view* g, const tn_relax_params* prm, tn_nodes_view* n
              int32_t device);
 
 /*
+ * tn_relax without allocation: no counterpart in the reference. For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments, results (the same bits
+ * as tn_relax) and errors, but every device buffer (the grid, the nodes, lists,
+ * hash, queues, control block) is carved from one static device array of
+ * TN_NOALLOC_SCRATCH_BYTES per GPU, held from the library's load until the process
+ * ends; no hipMalloc, hipFree, pinned host memory, stream or event is created. The
+ * work runs on the null stream with the move passes one after another. A
+ * relaxation that needs more than the array returns TN_ERR_ARG with the size in
+ * tn_last_error(). All calls on one GPU share its array, which keeps the last
+ * call's contents (each call zeroes it first); calls that overlap on one GPU would
+ * race on it, so the library serialises them with a per-device lock.
+ */
+#define TN_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int tn_relax_noalloc(const tn_grid_view* g, const tn_relax_params* prm, tn_nodes_view* nd, tn_relax_stats* st,
+                     int32_t device);
+
+/*
  * Relax `count` independent volumes on one device: volume k is relaxed exactly as
  * tn_relax(&grids[k], &prms[k], &nodes[k], &stats[k], device) would relax it, to
  * the same bits. No counterpart in the reference, which relaxes one volume per

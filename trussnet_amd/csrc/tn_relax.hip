@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -150,12 +151,16 @@ __global__ void k_iota(int32_t* __restrict__ a, int n) {
 // Device memory owned by one object and freed with it. Small buffers are carved
 // from shared 16 MB chunks (256-byte aligned): a plan holds a few dozen buffers,
 // and every hipMalloc / hipFree costs tens of microseconds and a device-wide sync.
+// With an arena (set_arena), every buffer is carved from that caller-given device
+// memory instead and nothing is allocated or freed; a request beyond its end throws.
 struct DevMem {
     static constexpr size_t kChunk = (size_t)16 << 20, kAlign = 256;
     std::vector<void*> owned;
     char* chunk = nullptr;
     size_t chunk_left = 0;
     int device = -1;
+    char* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
     DevMem() = default;
     DevMem(const DevMem&) = delete;
     DevMem& operator=(const DevMem&) = delete;
@@ -167,8 +172,24 @@ struct DevMem {
             (void)hipFree(p);
         }
     }
+    void set_arena(void* base, size_t cap) {
+        arena = static_cast<char*>(base);
+        arena_cap = cap;
+        arena_used = 0;
+    }
     void* raw(size_t bytes) {
         bytes = (std::max<size_t>(bytes, 1) + kAlign - 1) / kAlign * kAlign;
+        if (arena) {
+            if (bytes > arena_cap - arena_used) {
+                throw TnError(TN_ERR_ARG, "scratch memory: this relaxation needs more than the " +
+                                              std::to_string(arena_cap >> 20) + " MiB the no-allocation entry "
+                                              "point holds (" + std::to_string(arena_used + bytes) +
+                                              " bytes requested so far)");
+            }
+            void* p = arena + arena_used;
+            arena_used += bytes;
+            return p;
+        }
         if (bytes > kChunk / 4) {   // a buffer of its own
             void* p = nullptr;
             TN_HIP(hipMalloc(&p, bytes));
@@ -362,7 +383,11 @@ struct tn_plan {
     int32_t* dRv[2] = { nullptr, nullptr };
     double2* dPartials = nullptr;
     tnk::Ctl* dCtl = nullptr;
-    tnk::Ctl* hCtl = nullptr;   // pinned
+    tnk::Ctl* hCtl = nullptr;   // pinned (pageable: hctl_local, in a plan that allocates nothing)
+    tnk::Ctl hctl_local{};
+    // a plan that allocates nothing: its buffers are carved from a caller-given arena,
+    // it runs on the null stream with no side streams, events or pinned memory
+    bool noalloc = false;
     hipEvent_t ev_iter = nullptr;   // behind each iteration's control-block copy
     // the general and the lean interior passes run on side streams, beside the lean
     // interface pass: ev_easy starts them, ev_rest and ev_int join them back
@@ -386,10 +411,10 @@ struct tn_plan {
         if (device >= 0) {
             (void)hipSetDevice(device);
         }
-        if (stream) {
+        if (stream || noalloc) {
             (void)hipStreamSynchronize(stream);
         }
-        if (hCtl) {
+        if (hCtl && !noalloc) {
             (void)hipHostFree(hCtl);
         }
         for (hipStream_t s : side) {
@@ -597,7 +622,9 @@ struct tn_plan {
         timer.end(KC_CONTROL, t);
         TN_HIP(hipGetLastError());
         TN_HIP(hipMemcpyAsync(hCtl, dCtl, sizeof(tnk::Ctl), hipMemcpyDeviceToHost, stream));
-        TN_HIP(hipEventRecord(ev_iter, stream));
+        if (ev_iter) {
+            TN_HIP(hipEventRecord(ev_iter, stream));
+        }
     }
 
     // after ev_iter: the host's part of the iteration (the decisions are the device's,
@@ -673,7 +700,11 @@ struct tn_plan {
 
     bool iterate() {
         enqueue_iteration();
-        TN_HIP(hipEventSynchronize(ev_iter));
+        if (ev_iter) {
+            TN_HIP(hipEventSynchronize(ev_iter));
+        } else {
+            TN_HIP(hipStreamSynchronize(stream));
+        }
         return finish_iteration();
     }
 
@@ -727,10 +758,11 @@ struct tn_plan {
 namespace {
 
 // A plan on device dev: make_grid(mem) puts the grid into the plan's memory and
-// returns its Field; H and nkeys are the hash geometry.
+// returns its Field; H and nkeys are the hash geometry. With an arena, the plan
+// allocates nothing (DevMem::set_arena; tn_plan::noalloc).
 template <class MakeGrid>
 tn_plan* plan_build(const tn_relax_params* prm, const tn_nodes_view* seeds, int dev, const tnd::Hash& H, int64_t nkeys,
-                    MakeGrid make_grid) {
+                    MakeGrid make_grid, void* arena = nullptr, size_t arena_cap = 0) {
         TN_HIP(hipSetDevice(dev));
         std::unique_ptr<tn_plan> p(new tn_plan);
         p->device = dev;
@@ -741,16 +773,22 @@ tn_plan* plan_build(const tn_relax_params* prm, const tn_nodes_view* seeds, int 
         p->tskin = prm->t + prm->skin;
         p->H = H;
         p->nkeys = nkeys;
-        TN_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
-        TN_HIP(hipEventCreateWithFlags(&p->ev_iter, hipEventDisableTiming));
-        for (hipStream_t& s : p->side) {
-            TN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        }
-        for (hipEvent_t* e : { &p->ev_easy, &p->ev_rest, &p->ev_int }) {
-            TN_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        p->noalloc = arena != nullptr;
+        if (p->noalloc) {   // the null stream, the passes one after another, no timing
+            p->overlap_moves = false;
+            p->mem.set_arena(arena, arena_cap);
+        } else {
+            TN_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+            TN_HIP(hipEventCreateWithFlags(&p->ev_iter, hipEventDisableTiming));
+            for (hipStream_t& s : p->side) {
+                TN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            }
+            for (hipEvent_t* e : { &p->ev_easy, &p->ev_rest, &p->ev_int }) {
+                TN_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+            }
         }
         p->timer.stream = p->stream;
-        p->timer.on = prm->verbose || std::getenv("TN_KERNEL_TIMING") != nullptr;
+        p->timer.on = !p->noalloc && (prm->verbose || std::getenv("TN_KERNEL_TIMING") != nullptr);
         {
             const char* e = std::getenv("TN_SERIAL_POWER");
             p->force_serial = e && e[0] && e[0] != '0' ? 1 : 0;
@@ -810,7 +848,11 @@ tn_plan* plan_build(const tn_relax_params* prm, const tn_nodes_view* seeds, int 
         p->dPartials = p->alloc<double2>(std::max(p->nparts, 1));
         p->dThr = p->alloc<float>(tnk::kHistThr);
         p->dCtl = p->alloc<tnk::Ctl>(1);
-        TN_HIP(hipHostMalloc((void**)&p->hCtl, sizeof(tnk::Ctl), hipHostMallocDefault));
+        if (p->noalloc) {
+            p->hCtl = &p->hctl_local;
+        } else {
+            TN_HIP(hipHostMalloc((void**)&p->hCtl, sizeof(tnk::Ctl), hipHostMallocDefault));
+        }
         std::memset(p->hCtl, 0, sizeof(tnk::Ctl));
         float thr[tnk::kHistThr];
         hist_thresholds(thr);
@@ -1456,6 +1498,49 @@ void run_grouped(Plans& plans, int64_t& launches, int64_t& steps) {
 
 }  // namespace
 
+// The device memory of tn_relax_noalloc: one array per GPU, held from the library's
+// load until the process ends. Host code never passes `scratch` itself to a kernel
+// or a copy (on the host it only names the symbol); it uses the address that
+// hipGetSymbolAddress gives on each device (scratch_base).
+static __device__ __align__(256) unsigned char scratch[TN_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+// Per device: the array's address there (looked up once, with the device current)
+// and the lock that serialises tn_relax_noalloc calls on it (they share the array).
+struct ScratchSlot {
+    std::mutex lock;
+    void* base = nullptr;
+};
+
+ScratchSlot& scratch_slot(int dev) {
+    static std::mutex slots_lock;
+    static std::vector<std::unique_ptr<ScratchSlot>> slots;
+    std::lock_guard<std::mutex> g(slots_lock);
+    if ((int)slots.size() <= dev) {
+        slots.resize(dev + 1);
+    }
+    if (!slots[dev]) {
+        slots[dev].reset(new ScratchSlot);
+    }
+    return *slots[dev];
+}
+
+// the array's address on device dev (current); the caller holds the slot's lock
+void* scratch_base(ScratchSlot& s) {
+    if (!s.base) {
+        void* p = nullptr;
+        TN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            throw TnError(TN_ERR_HIP, "hipGetSymbolAddress(scratch): null address");
+        }
+        s.base = p;
+    }
+    return s.base;
+}
+
+}  // namespace
+
 extern "C" {
 
 void tn_relax_params_default(tn_relax_params* p) {
@@ -1777,6 +1862,38 @@ int tn_relax(const tn_grid_view* g, const tn_relax_params* prm, tn_nodes_view* n
     tn_plan_destroy(p);
     g_err = err;
     return rc;
+}
+
+int tn_relax_noalloc(const tn_grid_view* g, const tn_relax_params* prm, tn_nodes_view* nd, tn_relax_stats* st,
+                     int32_t device) {
+    return guarded([&] {
+        check_grid(g);
+        check_params(prm);
+        check_nodes(nd);
+        const int dev = pick_device(device);
+        TN_HIP(hipSetDevice(dev));
+        ScratchSlot& slot = scratch_slot(dev);
+        std::lock_guard<std::mutex> hold(slot.lock);
+        void* base = scratch_base(slot);
+        // the array keeps the last call's contents: zero it (a fill kernel on the null
+        // stream, ahead of every copy and kernel of this call)
+        const int64_t words = (int64_t)(TN_NOALLOC_SCRATCH_BYTES / sizeof(int32_t));
+        tnk::k_fill_i32<<<blocks_for(words, tnk::kBlock), tnk::kBlock, 0, nullptr>>>(static_cast<int32_t*>(base),
+                                                                                      words, 0);
+        TN_HIP(hipGetLastError());
+        int64_t nkeys = 0;
+        const tnd::Hash H = make_hash(g, prm, &nkeys);
+        std::unique_ptr<tn_plan> p(plan_build(prm, nd, dev, H, nkeys, [&](DevMem& mem) { return upload_grid(g, mem); },
+                                              base, TN_NOALLOC_SCRATCH_BYTES));
+        if (p->n > 0) {   // plan_build left it reset: seeds in place, first rebuild done
+            p->run_iters(p->prm.max_iters);
+        }
+        p->fill_stats(st);
+        const int rc = tn_plan_download(p.get(), nd);
+        if (rc != TN_OK) {
+            throw TnError(rc, g_err);
+        }
+    });
 }
 
 int tn_relax_batch(const tn_grid_view* grids, const tn_relax_params* prms, tn_nodes_view* nodes,

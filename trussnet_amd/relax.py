@@ -24,7 +24,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("TN_LIBRARY") or os.path.join(HERE, "lib", "libtrussnet_amd.so")
 
 # every entry point of include/trussnet_amd.h
-SYMBOLS = ["tn_relax_params_default", "tn_last_error", "tn_device_count", "tn_relax", "tn_relax_batch",
+SYMBOLS = ["tn_relax_params_default", "tn_last_error", "tn_device_count", "tn_relax", "tn_relax_noalloc",
+           "tn_relax_batch",
            "tn_seed_count", "tn_seed",
            "tn_plan_create", "tn_plan_run", "tn_plan_download", "tn_plan_destroy", "tn_plan_hash_info",
            "tn_plan_reset", "tn_plan_rebuild", "tn_plan_force", "tn_plan_step", "tn_plan_set_timing",
@@ -65,6 +66,7 @@ def lib():
         L.tn_device_count.argtypes = [C.POINTER(C.c_int32)]
         L.tn_relax.argtypes = [C.POINTER(GridView), C.POINTER(RelaxParamsC), C.POINTER(NodesView),
                                C.POINTER(RelaxStatsC), C.c_int32]
+        L.tn_relax_noalloc.argtypes = L.tn_relax.argtypes
         L.tn_relax_batch.argtypes = [C.POINTER(GridView), C.POINTER(RelaxParamsC), C.POINTER(NodesView),
                                      C.POINTER(RelaxStatsC), C.c_int32, C.c_int32]
         L.tn_plan_create.argtypes = [C.POINTER(GridView), C.POINTER(RelaxParamsC), C.POINTER(NodesView), C.c_int32,
@@ -129,6 +131,17 @@ def relax(grid: Grid, nodes: Nodes, prm: RelaxParams | None = None, device: int 
     gv, pc, nv = grid.view(), prm.to_c(), out.view()
     st = RelaxStatsC()
     _check(lib().tn_relax(C.byref(gv), C.byref(pc), C.byref(nv), C.byref(st), device))
+    return out, st.as_dict()
+
+
+def relax_noalloc(grid: Grid, nodes: Nodes, prm: RelaxParams | None = None, device: int = -1) -> tuple[Nodes, dict]:
+    """relax() through tn_relax_noalloc: the same result, with every device buffer carved
+    from the library's static scratch array (no device allocation)."""
+    prm = prm or RelaxParams()
+    out = nodes.copy()
+    gv, pc, nv = grid.view(), prm.to_c(), out.view()
+    st = RelaxStatsC()
+    _check(lib().tn_relax_noalloc(C.byref(gv), C.byref(pc), C.byref(nv), C.byref(st), device))
     return out, st.as_dict()
 
 
