@@ -24,7 +24,7 @@ HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-gpu-flush-denormals-to-zero",
     "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function",
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1", "-Wall", "-Wno-unused-function",
 ]
 SOURCES = ["tn_relax.hip"]
 DEPS = ["tn_relax.hip", "tn_kernels.hpp", "tn_device.hpp", "tn_seed.hpp", "tn_adjoint.hpp", "tn_grid.hpp", os.path.join("..", "..", "include", "trussnet_amd.h")]

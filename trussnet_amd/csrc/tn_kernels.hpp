@@ -312,19 +312,21 @@ __global__ __launch_bounds__(kBlock) void k_permute(const int32_t* __restrict__ 
 // ---- K-nearest lists, one wave per 64 nodes in cell order ----------------------------
 //
 // v2m_neighbors (src/opencl/v2m_particle_body.cl:73-151) for 64 query nodes at once.
-// The queries are consecutive in cell order, so their scan boxes overlap: per hash
-// level the wave scans the union of its lanes' boxes (rows no lane reaches are
-// skipped), loading each row's candidates once, coalesced, into LDS, and every
-// lane tests every candidate against its own query, eight at a time. Candidates
-// within reach are appended raw, as (d^2, slot), to the lane's LDS buffer; the
-// other checks (not the node itself, no interior bar across labels) and the
-// distance sqrtf(d^2) wait until the buffer fills, when it is cut back to its 24
-// best, and the end, when the first 24 in (distance, id) order are kept. The
-// list equals the reference's: its K smallest valid candidates in (distance, id)
-// order, and every valid candidate lies inside the lane's own box, which the
-// union contains. (That holds when every node's reach fits its level's cell,
-// H.prune; otherwise a lane also checks that the candidate's cell is in its own
-// box, the reference's scan region, before accepting it.)
+// The queries are consecutive in cell order, so their scan boxes overlap. Per group
+// of lanes (one cell row, a bounded span along it) the wave lists the rows of the
+// union of its lanes' boxes over their levels (rows no lane reaches are skipped),
+// reads all their cell starts at once, and tests the concatenation of the rows'
+// candidates in full chunks of 64 staged in LDS: a distance prefilter on the
+// matrix cores, then for every possible hit of a lane that scans the candidate's
+// row the reference's exact test and its other checks (not the node itself, no
+// interior bar across labels). Valid candidates are appended as (distance, id) to
+// the lane's LDS buffer, which is cut back to its 24 best when it fills; at the end
+// the first 24 in (distance, id) order are kept. The list equals the reference's:
+// its K smallest valid candidates in (distance, id) order, and every valid
+// candidate lies inside the lane's own box, which the union contains. (That holds
+// when every node's reach fits its level's cell, H.prune; otherwise a lane also
+// checks that the candidate's cell is in its own box, the reference's scan region,
+// before accepting it.)
 
 constexpr int kTrussWave = 64;   // one wave per block
 #ifndef TN_TRUSS_COLLECT
@@ -357,53 +359,35 @@ __device__ __forceinline__ bool before(float da, int ja, float db, int jb) {
     return da < db || (da == db && ja < jb);
 }
 
-// insertion sort of entries [lo, cnt) of a lane's buffer (entries e*64 + lane) into
-// its sorted prefix [0, lo)
-__device__ __forceinline__ void sort_lane(float* cd, int* cj, int lane, int lo, int cnt) {
+// A list entry as one 64-bit key: the distance's bits above the id. Distances are
+// >= +0, whose bits order as the floats do, so the keys order as (distance, id), the
+// K-list's order (src/opencl/v2m_particle_body.cl:128-141). While a hit waits in a
+// lane's buffer unsettled, the high word holds d^2 instead.
+__device__ __forceinline__ uint64_t list_key(float d, int id) {
+    return ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)id;
+}
+
+// Settle entries [lo, cnt) of a lane's buffer (entries e*64 + lane): d^2 -> sqrtf(d^2),
+// then insertion into the ordered prefix [0, lo).
+__device__ __forceinline__ void settle_keys(uint64_t* cb, int lane, int lo, int cnt) {
+    for (int e = lo; e < cnt; ++e) {
+        const uint64_t v = cb[e * kTrussWave + lane];
+        cb[e * kTrussWave + lane] = list_key(sqrtf(__uint_as_float((uint32_t)(v >> 32))), (int)(uint32_t)v);
+    }
     for (int e = lo > 1 ? lo : 1; e < cnt; ++e) {
-        const float d = cd[e * kTrussWave + lane];
-        const int j = cj[e * kTrussWave + lane];
+        const uint64_t v = cb[e * kTrussWave + lane];
         int k = e;
         while (k > 0) {
-            const float dp = cd[(k - 1) * kTrussWave + lane];
-            const int jp = cj[(k - 1) * kTrussWave + lane];
-            if (!before(d, j, dp, jp)) {
+            const uint64_t w = cb[(k - 1) * kTrussWave + lane];
+            if (!(v < w)) {
                 break;
             }
-            cd[k * kTrussWave + lane] = dp;
-            cj[k * kTrussWave + lane] = jp;
+            cb[k * kTrussWave + lane] = w;
             --k;
         }
-        cd[k * kTrussWave + lane] = d;
-        cj[k * kTrussWave + lane] = j;
+        cb[k * kTrussWave + lane] = v;
     }
 }
-
-// Settle a lane's buffer: entries [0, nval) are valid (distance, id) pairs in order;
-// entries [nval, cnt) are raw hits (d^2, slot). The raw ones are checked (not the
-// node itself, no interior bar across labels, v2m_particle_body.cl:117-121), turned
-// into (sqrtf(d^2), id) and merged in; at most kK are kept. Returns the new count.
-__device__ __forceinline__ int settle_lane(float* cd, int* cj, int lane, int nval, int cnt, int q, int ti_int,
-                                           int li, const int2* __restrict__ Ms) {
-    int m = nval;
-    for (int e = nval; e < cnt; ++e) {
-        const float d2 = cd[e * kTrussWave + lane];
-        const int s = cj[e * kTrussWave + lane];
-        const int2 mt = Ms[s];
-        if (s == q || (ti_int && (mt.y & 0xFF) == TN_INTERIOR && (mt.y >> 8) != li)) {
-            continue;
-        }
-        cd[m * kTrussWave + lane] = sqrtf(d2);
-        cj[m * kTrussWave + lane] = mt.x;
-        ++m;
-    }
-    sort_lane(cd, cj, lane, nval, m);
-    return m < kK ? m : kK;
-}
-
-#ifndef TN_TRUSS_MFMA
-#define TN_TRUSS_MFMA 1
-#endif
 
 // ---- the distance prefilter on the matrix cores ------------------------------------------
 //
@@ -452,14 +436,26 @@ __device__ __forceinline__ QueryOps query_ops(float qx, float qy, float qz, floa
     return o;
 }
 
-// the bits of one output tile: register e's test at bit 8 (e / 4) + e % 4
+// The tests of one output tile, dense: bit e for register e (its difference below the
+// margin). Each register costs one compare into VCC and one add-with-carry that
+// doubles the mask and shifts VCC in (the registers are taken from e = 15 down).
 __device__ __forceinline__ uint32_t tile_bits(const f32x16& acc, float margin) {
     uint32_t m = 0;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-        m |= acc[e] < margin ? (1u << (8 * (e / 4) + (e % 4))) : 0u;
+    for (int e = 15; e >= 0; --e) {
+        asm("v_cmp_lt_f32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc"
+            : "+v"(m)
+            : "v"(acc[e]), "v"(margin)
+            : "vcc");
     }
     return m;
+}
+
+// the 16 bits of a dense tile mask spread to nibbles at bits 8 k .. 8 k + 3
+__device__ __forceinline__ uint32_t spread_nibbles(uint32_t d) {
+    uint32_t x = d & 0xFFFFu;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    return (x | (x << 4)) & 0x0F0F0F0Fu;
 }
 
 // the 64-bit mask of this lane's query over the chunk's candidates (cpos) whose
@@ -494,28 +490,57 @@ __device__ __forceinline__ uint64_t chunk_hits(const QueryOps& qo, const float4*
             acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, qo.b[c][2], acc, 0, 0, 0);
             bits[c] = tile_bits(acc, margin);
         }
-        // my query's column tile is h: my half of its rows is mine, lane l ^ 32 has the rest
+        // my query's column tile is h: register e of lane l holds candidate
+        // 8 (e / 4) + 4 (l / 32) + e % 4 of the row tile; lane l ^ 32 has the other half
         const uint32_t mine = h ? bits[1] : bits[0], give = h ? bits[0] : bits[1];
         const uint32_t got = (uint32_t)__shfl_xor((int)give, 32, kWave);
-        out[r] = (mine << (4 * h)) | (got << (4 * (1 - h)));
+        out[r] = (spread_nibbles(mine) << (4 * h)) | (spread_nibbles(got) << (4 * (1 - h)));
     }
     return ((uint64_t)out[1] << 32) | out[0];
+}
+
+// The rows of one group, collected before any of their candidates is loaded: the
+// cell-start reads of all rows are then issued together, and the candidates of all
+// rows (and levels) are concatenated into full chunks.
+constexpr int kMaxRanges = 64;
+
+// exclusive prefix sum over the wave; *total: the sum of all lanes
+__device__ __forceinline__ int wave_excl_scan(int v, int lane, int* total) {
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int y = __shfl_up(x, d, kWave);
+        if (lane >= d) {
+            x += y;
+        }
+    }
+    *total = __shfl(x, kWave - 1, kWave);
+    return x - v;
 }
 
 // The lists of all nodes at a rebuild: query q is the node in slot q.
 __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n, int32_t* __restrict__ nbrT,
                                                       int32_t* __restrict__ nnb) {
-    __shared__ float4 cpos[kTrussWave];
-    __shared__ float cd[kCollect * kTrussWave];
-    __shared__ int cj[kCollect * kTrussWave];
+    __shared__ float4 cpos[kTrussWave];   // the chunk's candidates: position, h
+    __shared__ int2 cms[kTrussWave];      // their (id, typ | lab << 8)
+    __shared__ int crow[kTrussWave];      // their row (index into the row table)
+    // the row table of the current group: cell-start indices of the row's first cell and
+    // one past its last, the lanes that scan the row, its level; after the cell starts
+    // are read: the row's first slot and its position in the concatenation
+    __shared__ int r_key0[kMaxRanges], r_key1[kMaxRanges], r_lev[kMaxRanges], r_sa[kMaxRanges], r_beg[kMaxRanges],
+        r_len[kMaxRanges];
+    __shared__ uint64_t r_mask[kMaxRanges];
+    __shared__ uint64_t cb[kCollect * kTrussWave];   // the lanes' lists (list_key entries)
     const int lane = threadIdx.x;
     const float tskin = in.t + in.skin;
     const float half = tskin * 0.5f;   // (t + skin) * 0.5f * (hi + hj), left to right
+    const float k2 = half * half;
+    const float inf = __builtin_huge_valf();
     for (int qb = blockIdx.x * kTrussWave; qb < n; qb += gridDim.x * kTrussWave) {
         const int q = qb + lane;
         const bool active = q < n;
         float px = 0.0f, py = 0.0f, pz = 0.0f, hi = 0.0f, Ri = 0.0f;
-        int Li = 0, ti_int = 0, li = 0;
+        int Li = 0, ti_int = 0, li = 0, myid = -1;
         if (active) {   // the node in slot q (nodes are in cell order)
             const float4 xi = in.X[q];
             px = xi.x;
@@ -525,6 +550,7 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
             Ri = tskin * hi;
             Li = level_for(H, Ri);
             const int2 mi = in.Ms[q];
+            myid = mi.x;
             ti_int = (mi.y & 0xFF) == TN_INTERIOR;
             li = mi.y >> 8;
         }
@@ -538,7 +564,7 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
             gx = cell_coord(H, Li, px, H.ox, 0);
         }
         bool pending = active;
-        int cnt = 0, nval = 0;
+        int cnt = 0, nval = 0;   // the lane's list: [0, nval) settled, in order; [nval, cnt) appended
         for (;;) {
             const uint64_t pm = __ballot(pending);
             if (!pm) {
@@ -548,15 +574,113 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
             const int lrow = __shfl(grow, lead, kWave), lx = __shfl(gx, lead, kWave);
             const bool ingroup = pending && grow == lrow && abs(gx - lx) <= kGroupSpan;
             pending = pending && !ingroup;
-#if TN_TRUSS_MFMA
             // the prefilter's origin: the group's first query; its query operands
             const float gox = __shfl(px, lead, kWave), goy = __shfl(py, lead, kWave), goz = __shfl(pz, lead, kWave);
-            const float k2 = half * half;
             const float qx = px - gox, qy = py - goy, qz = pz - goz;
             const float q2 = qx * qx + qy * qy + qz * qz;
             const QueryOps qops = query_ops(qx, qy, qz, hi, q2 - k2 * hi * hi, lane);
             const float P2 = wave_maxf(ingroup ? q2 : 0.0f), HP = wave_maxf(ingroup ? hi : 0.0f);
-#endif
+            int nr = 0;   // rows in the table (wave-uniform)
+            // Read the cell starts of the table's rows, then test the concatenation of their
+            // candidates chunk by chunk: the distance prefilter on the matrix cores, then the
+            // exact test and the reference's other checks for each possible hit of a lane
+            // that scans the candidate's row; the valid ones are appended as (distance, id).
+            auto flush = [&]() {
+                __syncthreads();   // the table's rows are written
+                int sa = 0, len = 0;
+                if (lane < nr) {
+                    sa = in.cstart[r_key0[lane]];
+                    len = in.cstart[r_key1[lane]] - sa;
+                }
+                int total = 0;
+                const int beg = wave_excl_scan(len, lane, &total);
+                if (lane < nr) {
+                    r_sa[lane] = sa;
+                    r_beg[lane] = beg;
+                    r_len[lane] = len;
+                }
+                const int rows = nr;
+                nr = 0;
+                for (int base = 0; base < total; base += kTrussWave) {
+                    __syncthreads();   // the previous chunk is consumed; the row starts are written
+                    const int p = base + lane;
+                    int lo = 0;   // the candidate's row
+                    if (p < total) {   // the last row that starts at or before p holds it
+                        int up = rows;
+                        while (up - lo > 1) {
+                            const int mid = (lo + up) >> 1;
+                            if (r_beg[mid] <= p) {
+                                lo = mid;
+                            } else {
+                                up = mid;
+                            }
+                        }
+                        const int s = r_sa[lo] + (p - r_beg[lo]);
+                        cpos[lane] = in.Ps[s];
+                        cms[lane] = in.Ms[s];
+                        crow[lane] = lo;
+                    } else {   // past the end: far away, it fails every test
+                        cpos[lane] = make_float4(inf, inf, inf, 0.0f);
+                        cms[lane] = make_int2(-1, 0);
+                        crow[lane] = 0;
+                    }
+                    __syncthreads();
+                    const int cn = min(kTrussWave, total - base);
+                    // the chunk's own bound for the rounding margin
+                    const float4 mine_c = cpos[lane];
+                    const bool cval = mine_c.x < inf;
+                    const float ccx = mine_c.x - gox, ccy = mine_c.y - goy, ccz = mine_c.z - goz;
+                    const float C2 = wave_maxf(cval ? ccx * ccx + ccy * ccy + ccz * ccz : 0.0f);
+                    const float HC = wave_maxf(cval ? mine_c.w : 0.0f);
+                    const float hs = HP + HC;
+                    const float margin = (2.0f * (P2 + C2) + 2.0f * k2 * hs * hs) * (1.0f / 65536.0f);
+                    uint64_t hit = chunk_hits(qops, cpos, gox, goy, goz, k2, margin, lane);
+                    // the candidates of the rows this lane scans (its level range, its own box)
+                    uint64_t elig = 0;
+                    {
+                        const int r0 = __shfl(lo, 0, kWave), r1 = __shfl(lo, cn - 1, kWave);
+                        for (int r = r0; r <= r1; ++r) {
+                            const int a = max(r_beg[r] - base, 0), b = min(r_beg[r] + r_len[r] - base, cn);
+                            if (b > a && ((r_mask[r] >> lane) & 1ull)) {
+                                elig |= (b - a == kTrussWave ? ~0ull : ((1ull << (b - a)) - 1)) << a;
+                            }
+                        }
+                    }
+                    if (!ingroup) {
+                        continue;
+                    }
+                    hit &= elig;
+                    while (hit) {   // possible hits: the exact test and the checks, then appended
+                        const int k = __builtin_ctzll(hit);
+                        hit &= hit - 1;
+                        const float4 c = cpos[k];
+                        const float ex = c.x - px, ey = c.y - py, ez = c.z - pz;
+                        const float d2 = ex * ex + ey * ey + ez * ez;
+                        const float rj = half * (hi + c.w);
+                        if (!(d2 < rj * rj)) {
+                            continue;
+                        }
+                        if (!H.prune) {   // the reference's own scan region only
+                            const int L = r_lev[crow[k]];
+                            const float rr = 0.5f * (Ri + H.b0 * (float)(1 << L));
+                            const int cx = cell_coord(H, L, c.x, H.ox, 0);
+                            if (cx < cell_coord(H, L, px - rr, H.ox, 0) || cx > cell_coord(H, L, px + rr, H.ox, 0)) {
+                                continue;
+                            }
+                        }
+                        const int2 mt = cms[k];   // not the node itself; no interior bar across labels
+                        if (mt.x == myid || (ti_int && (mt.y & 0xFF) == TN_INTERIOR && (mt.y >> 8) != li)) {
+                            continue;
+                        }
+                        if (cnt == kCollect) {   // keep the 24 best so far
+                            settle_keys(cb, lane, nval, cnt);
+                            nval = cnt = kK;
+                        }
+                        cb[cnt * kTrussWave + lane] = ((uint64_t)__float_as_uint(d2) << 32) | (uint32_t)mt.x;
+                        ++cnt;
+                    }
+                }
+            };
             const int lmin = wave_min(ingroup ? max(Li - 1, 0) : INT_MAX);
             const int lmax = wave_max(ingroup ? min(Li + 1, H.nlev - 1) : -1);
             for (int L = lmin; L <= lmax; ++L) {
@@ -578,12 +702,12 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                 const float rr = rw * rw;
                 const int ny1 = H.dim[L][1] - 1, nz1 = H.dim[L][2] - 1;
                 for (int cz = Z0; cz <= Z1; ++cz) {
-                    const float zlo = cz == 0 ? -__builtin_huge_valf() : H.oz + (float)cz * bL;
-                    const float zhi = cz == nz1 ? __builtin_huge_valf() : H.oz + (float)(cz + 1) * bL;
+                    const float zlo = cz == 0 ? -inf : H.oz + (float)cz * bL;
+                    const float zhi = cz == nz1 ? inf : H.oz + (float)(cz + 1) * bL;
                     const float dz = fmaxf(0.0f, fmaxf(zlo - pz, pz - zhi));
                     for (int cy = Y0; cy <= Y1; ++cy) {
-                        const float ylo = cy == 0 ? -__builtin_huge_valf() : H.oy + (float)cy * bL;
-                        const float yhi = cy == ny1 ? __builtin_huge_valf() : H.oy + (float)(cy + 1) * bL;
+                        const float ylo = cy == 0 ? -inf : H.oy + (float)cy * bL;
+                        const float yhi = cy == ny1 ? inf : H.oy + (float)(cy + 1) * bL;
                         const float dy = fmaxf(0.0f, fmaxf(ylo - py, py - yhi));
                         const bool row_own = mine && cz >= z0 && cz <= z1 && cy >= y0 && cy <= y1;
                         int xa = INT_MAX, xb = -1;
@@ -604,109 +728,30 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                         if (XA > XB) {
                             continue;
                         }
-                        const bool scan = xa <= xb;   // this lane has cells in the row
-                        const int krow = H.off[L] + H.dim[L][0] * (cy + H.dim[L][1] * cz);
-                        const int sa = in.cstart[krow + XA], sb = in.cstart[krow + XB + 1];
-                        for (int base = sa; base < sb; base += kTrussWave) {
-                            __syncthreads();   // the previous chunk is consumed
-                            const int s = base + lane;
-                            if (s < sb) {
-                                cpos[lane] = in.Ps[s];
-                            } else {   // past the row's end: far away, it fails every test
-                                const float inf = __builtin_huge_valf();
-                                cpos[lane] = make_float4(inf, inf, inf, 0.0f);
-                            }
-                            __syncthreads();
-#if TN_TRUSS_MFMA
-                            if (!__ballot(scan)) {
-                                continue;
-                            }
-                            {
-                                const int cn = min(kTrussWave, sb - base);
-                                // the chunk's own bound for the rounding margin
-                                const float4 mine_c = cpos[lane];
-                                const bool cval = mine_c.x < __builtin_huge_valf();
-                                const float ccx = mine_c.x - gox, ccy = mine_c.y - goy, ccz = mine_c.z - goz;
-                                const float C2 = wave_maxf(cval ? ccx * ccx + ccy * ccy + ccz * ccz : 0.0f);
-                                const float HC = wave_maxf(cval ? mine_c.w : 0.0f);
-                                const float hs = HP + HC;
-                                const float margin = (2.0f * (P2 + C2) + 2.0f * k2 * hs * hs) * (1.0f / 65536.0f);
-                                uint64_t hit = chunk_hits(qops, cpos, gox, goy, goz, k2, margin, lane);
-                                if (!scan) {
-                                    continue;
-                                }
-                                hit &= cn == kTrussWave ? ~0ull : ((1ull << cn) - 1);
-                                while (hit) {   // possible hits: the exact test, then appended raw
-                                    const int k = __builtin_ctzll(hit);
-                                    hit &= hit - 1;
-                                    const float4 c = cpos[k];
-                                    const float ex = c.x - px, ey = c.y - py, ez = c.z - pz;
-                                    const float d2 = ex * ex + ey * ey + ez * ez;
-                                    const float rj = half * (hi + c.w);
-                                    if (!(d2 < rj * rj)) {
-                                        continue;
-                                    }
-                                    if (!H.prune) {   // the reference's own scan region only
-                                        const int cx = cell_coord(H, L, c.x, H.ox, 0);
-                                        if (cx < x0 || cx > x1) {
-                                            continue;
-                                        }
-                                    }
-                                    if (cnt == kCollect) {   // keep the 24 best so far
-                                        nval = cnt = settle_lane(cd, cj, lane, nval, cnt, q, ti_int, li, in.Ms);
-                                    }
-                                    cd[cnt * kTrussWave + lane] = d2;
-                                    cj[cnt * kTrussWave + lane] = base + k;
-                                    ++cnt;
-                                }
-                                continue;
-                            }
-#endif
-                            if (!scan) {
-                                continue;
-                            }
-                            const int cn = min(kTrussWave, sb - base);
-                            for (int k0 = 0; k0 < cn; k0 += 8) {
-                                // distance tests of 8 candidates, then the rare hits one by one
-                                uint32_t hit = 0;
-#pragma unroll
-                                for (int u = 0; u < 8; ++u) {
-                                    const float4 c = cpos[k0 + u];
-                                    const float ex = c.x - px, ey = c.y - py, ez = c.z - pz;
-                                    const float d2 = ex * ex + ey * ey + ez * ez;
-                                    const float rj = half * (hi + c.w);
-                                    hit |= (d2 < rj * rj) ? (1u << u) : 0u;
-                                }
-                                while (hit) {   // appended raw: (d^2, slot), checked when settled
-                                    const int k = k0 + __builtin_ctz(hit);
-                                    hit &= hit - 1;
-                                    const float4 c = cpos[k];
-                                    if (!H.prune) {   // the reference's own scan region only
-                                        const int cx = cell_coord(H, L, c.x, H.ox, 0);
-                                        if (cx < x0 || cx > x1) {
-                                            continue;
-                                        }
-                                    }
-                                    const float ex = c.x - px, ey = c.y - py, ez = c.z - pz;
-                                    const float d2 = ex * ex + ey * ey + ez * ez;
-                                    if (cnt == kCollect) {   // keep the 24 best so far
-                                        nval = cnt = settle_lane(cd, cj, lane, nval, cnt, q, ti_int, li, in.Ms);
-                                    }
-                                    cd[cnt * kTrussWave + lane] = d2;
-                                    cj[cnt * kTrussWave + lane] = base + k;
-                                    ++cnt;
-                                }
-                            }
+                        const uint64_t scan = __ballot(xa <= xb);   // the lanes with cells in the row
+                        if (lane == 0) {
+                            const int krow = H.off[L] + H.dim[L][0] * (cy + H.dim[L][1] * cz);
+                            r_key0[nr] = krow + XA;
+                            r_key1[nr] = krow + XB + 1;
+                            r_mask[nr] = scan;
+                            r_lev[nr] = L;
+                        }
+                        if (++nr == kMaxRanges) {
+                            flush();
                         }
                     }
                 }
             }
+            if (nr > 0) {
+                flush();
+            }
         }   // the groups
         if (active) {   // (distance, id) order; the list holds slots
-            const int kk = settle_lane(cd, cj, lane, nval, cnt, q, ti_int, li, in.Ms);
+            settle_keys(cb, lane, nval, cnt);
+            const int kk = cnt < kK ? cnt : kK;
             nnb[q] = kk;
             for (int k = 0; k < kk; ++k) {
-                nbrT[(size_t)k * n + q] = in.slot_of[cj[k * kTrussWave + lane]];
+                nbrT[(size_t)k * n + q] = in.slot_of[(int)(uint32_t)cb[k * kTrussWave + lane]];
             }
         }
         __syncthreads();   // the buffers are reused by the next block of queries
