@@ -190,6 +190,34 @@ int tn_seed(const tn_grid_view* g, const uint8_t* grade, const tn_relax_params* 
             float* P, uint16_t* lab, uint8_t* typ, uint16_t* part, uint16_t* part3);
 
 /*
+ * Node thinning on the device: thin_nodes (src/v2m_particles.h:101; body
+ * src/v2m_particles.cpp:1223-1357), which run_pipeline applies to the seeds
+ * before the relaxation when prm->thin > 0 (src/v2m_pipeline.cpp:467-476).
+ * Junction and corner nodes are kept; interface nodes, then interior nodes, in
+ * the order of h and then index, are each dropped when a node kept before them
+ * lies within thin * h on the same interface (or, for an interior node, carries
+ * its label). The n nodes (P[3n], lab, typ, part[2n], and part3[n], the corners'
+ * third partners; null: none) are compacted in place, in order, exactly as the
+ * reference compacts them; *n_kept is their new count. prm->thin <= 0 leaves them.
+ * Only the grid's sizes (h, dims, voxel sizes, hmin, hmax) are read.
+ */
+int tn_thin_nodes(const tn_grid_view* g, const tn_relax_params* prm, int64_t n, float* P, uint16_t* lab, uint8_t* typ,
+                  uint16_t* part, uint16_t* part3, int64_t* n_kept, int32_t device);
+
+/*
+ * Feature protection on the device: protect_features (src/v2m_particles.h:105;
+ * body src/v2m_particles.cpp:1002-1221), run on the relaxed nodes of a shape
+ * input (src/v2m_pipeline.cpp:549-555). feat[nfeat] is the grid's Grid::feat (the
+ * shapes' feature curves: points, segments, circles, polylines). Between
+ * neighbouring pinned (corner) nodes on each curve no node may sit inside the
+ * pair's diametral ball; those nodes are dropped and the rest compacted in place
+ * as above (*n_kept). The balls are built on the host as the reference builds
+ * them; the test of every node and the compaction run on the device.
+ */
+int tn_protect_features(const tn_grid_view* g, const float* feat, int64_t nfeat, int64_t n, float* P, uint16_t* lab,
+                        uint8_t* typ, uint16_t* part, uint16_t* part3, int64_t* n_kept, int32_t device);
+
+/*
  * The grid build on the device: build_grid_cpu (src/v2m_grid.cpp:47-722, the
  * Grid of src/v2m_grid.h:45-81 from a LabelVolume and GridParams) for a label
  * volume, a tissue-probability input (prob) or a gray-scale input (gray and its

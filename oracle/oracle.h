@@ -73,6 +73,16 @@ double orc_adj_loss(const tn_adj_problem* p, const double* x0, const double* the
 double orc_adj_grad(const tn_adj_problem* p, const double* x0, const double* theta, double* gx0, double* gtheta,
                     double* terms);
 
+/* oracle/thin.c: thin_nodes (src/v2m_particles.cpp:1223-1357) with thin factor B, and
+ * protect_features (src/v2m_particles.cpp:1002-1221) with the grid's feature curves
+ * (Grid::feat, nfeat floats): keep[i] = 1 for the nodes kept (the reference compacts
+ * them in index order); the count removed is returned. part3 may be null (no third
+ * partners). */
+int64_t orc_thin_nodes(const tn_grid_view* g, float B, int64_t n, const float* P, const uint16_t* lab,
+                       const uint8_t* typ, const uint16_t* part, const uint16_t* part3, uint8_t* keep);
+int64_t orc_protect_features(const tn_grid_view* g, const float* feat, int64_t nfeat, int64_t n, const float* P,
+                             const uint8_t* typ, uint8_t* keep);
+
 const char* orc_last_error(void);
 
 #ifdef __cplusplus

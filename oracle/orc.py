@@ -43,6 +43,11 @@ def lib():
         L.orc_sdf_points.argtypes = [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
         L.orc_hist_bin.argtypes = [C.c_float]
         L.orc_hist_bin.restype = C.c_int32
+        L.orc_thin_nodes.restype = C.c_int64
+        L.orc_thin_nodes.argtypes = [C.POINTER(GridView), C.c_float, C.c_int64] + [C.c_void_p] * 6
+        L.orc_protect_features.restype = C.c_int64
+        L.orc_protect_features.argtypes = [C.POINTER(GridView), C.c_void_p, C.c_int64, C.c_int64, C.c_void_p,
+                                           C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -119,6 +124,25 @@ def field_points(grid: Grid, a: int, b: int, X: np.ndarray):
     gv = grid.view()
     lib().orc_field_points(C.byref(gv), a, b, m, _p(X), _p(phi), _p(psi), _p(grad), _p(lab), _p(sec))
     return phi, psi, grad, lab, sec
+
+
+def thin_keep(grid: Grid, nodes: Nodes, thin: float) -> np.ndarray:
+    """thin_nodes (src/v2m_particles.cpp:1223-1357): the keep flags of the nodes."""
+    keep = np.zeros(nodes.n, np.uint8)
+    gv = grid.view()
+    p3 = None if nodes.part3 is None else np.ascontiguousarray(nodes.part3, np.uint16)
+    lib().orc_thin_nodes(C.byref(gv), thin, nodes.n, _p(nodes.P), _p(nodes.lab), _p(nodes.typ), _p(nodes.part),
+                         _p(p3) if p3 is not None else None, _p(keep))
+    return keep.astype(bool)
+
+
+def protect_keep(grid: Grid, feat: np.ndarray, nodes: Nodes) -> np.ndarray:
+    """protect_features (src/v2m_particles.cpp:1002-1221) with feature curves `feat`: keep flags."""
+    keep = np.zeros(nodes.n, np.uint8)
+    gv = grid.view()
+    feat = np.ascontiguousarray(feat, np.float32)
+    lib().orc_protect_features(C.byref(gv), _p(feat), feat.size, nodes.n, _p(nodes.P), _p(nodes.typ), _p(keep))
+    return keep.astype(bool)
 
 
 def hist_bin(m: float) -> int:

@@ -71,6 +71,12 @@ def lib():
         L.ref_move.argtypes = [vp, C.POINTER(RelaxParamsC), C.POINTER(NodesView), C.c_void_p, C.c_void_p, C.c_void_p,
                                C.c_float, C.c_float, C.c_void_p, C.c_void_p]
         L.ref_field_points.argtypes = [vp, C.c_int, C.c_int, C.c_int64] + [C.c_void_p] * 6
+        L.ref_thin_nodes.restype = C.c_int64
+        L.ref_thin_nodes.argtypes = [vp, C.POINTER(RelaxParamsC), C.c_int64] + [C.c_void_p] * 5
+        L.ref_protect_features.restype = C.c_int64
+        L.ref_protect_features.argtypes = [vp, C.c_int64] + [C.c_void_p] * 5
+        L.ref_grid_feat.restype = C.c_int64
+        L.ref_grid_feat.argtypes = [vp, C.c_void_p]
         L.ref_classify_points.restype = C.c_int64
         L.ref_classify_points.argtypes = [vp, C.c_int64] + [C.c_void_p] * 5
         _lib = L
@@ -207,6 +213,31 @@ class RefGrid:
         part3 = np.zeros(n, np.uint16)
         lib().ref_seed_copy(self.h, _p(P), _p(lab), _p(typ), _p(part), _p(part3))
         return Nodes(P, lab, typ, part, part3)
+
+    def _compacting(self, fn, nodes: Nodes) -> Nodes:
+        x = nodes.copy()
+        part3 = (x.part3.copy() if x.part3 is not None else np.full(x.n, 0xFFFF, np.uint16))
+        m = fn(x.n, _p(x.P), _p(x.lab), _p(x.typ), _p(x.part), _p(part3))
+        if m < 0:
+            raise RuntimeError("reference: " + lib().ref_last_error().decode())
+        return Nodes(x.P[:m], x.lab[:m], x.typ[:m], x.part[:m], part3[:m])
+
+    def thin(self, nodes: Nodes, thin: float) -> Nodes:
+        """thin_nodes (src/v2m_particles.cpp:1223) with factor `thin`: the kept nodes."""
+        pc = RelaxParams(thin=thin).to_c()
+        return self._compacting(lambda *a: lib().ref_thin_nodes(self.h, C.byref(pc), *a), nodes)
+
+    def protect_features(self, nodes: Nodes) -> Nodes:
+        """protect_features (src/v2m_particles.cpp:1002) with this grid's feature curves."""
+        return self._compacting(lambda *a: lib().ref_protect_features(self.h, *a), nodes)
+
+    def feat(self) -> np.ndarray:
+        """Grid::feat (the shape input's feature curves)."""
+        m = lib().ref_grid_feat(self.h, None)
+        out = np.zeros(m, np.float32)
+        if m:
+            lib().ref_grid_feat(self.h, _p(out))
+        return out
 
     def relax(self, nodes: Nodes, prm: RelaxParams) -> tuple[Nodes, dict]:
         """relax_cpu (src/v2m_particles.cpp:728) on a copy of the nodes."""

@@ -529,6 +529,72 @@ int ref_relax(void* h, const tn_relax_params* prm, tn_nodes_view* nd, tn_relax_s
     }
 }
 
+// the nodes as a tn::Nodes (part3 null: no third partners)
+tn::Nodes nodes_of(int64_t n, const float* P, const uint16_t* lab, const uint8_t* typ, const uint16_t* part,
+                   const uint16_t* part3) {
+    tn::Nodes x;
+    x.P.assign(P, P + 3 * n);
+    x.lab.assign(lab, lab + n);
+    x.typ.assign(typ, typ + n);
+    x.part.assign(part, part + 2 * n);
+    if (part3) {
+        x.part3.assign(part3, part3 + n);
+    } else {
+        x.part3.assign(static_cast<size_t>(n), V2M_NOLAB);
+    }
+    return x;
+}
+
+void nodes_back(const tn::Nodes& x, float* P, uint16_t* lab, uint8_t* typ, uint16_t* part, uint16_t* part3) {
+    std::copy(x.P.begin(), x.P.end(), P);
+    std::copy(x.lab.begin(), x.lab.end(), lab);
+    std::copy(x.typ.begin(), x.typ.end(), typ);
+    std::copy(x.part.begin(), x.part.end(), part);
+    if (part3) {
+        std::copy(x.part3.begin(), x.part3.end(), part3);
+    }
+}
+
+// thin_nodes (src/v2m_particles.cpp:1223) on the given nodes: the kept ones are
+// written back, compacted in order; returns their count (-1 on error)
+int64_t ref_thin_nodes(void* h, const tn_relax_params* prm, int64_t n, float* P, uint16_t* lab, uint8_t* typ,
+                       uint16_t* part, uint16_t* part3) {
+    try {
+        RefGrid* r = static_cast<RefGrid*>(h);
+        tn::Nodes x = nodes_of(n, P, lab, typ, part, part3);
+        tn::thin_nodes(r->g, to_params(prm), x);
+        nodes_back(x, P, lab, typ, part, part3);
+        return static_cast<int64_t>(x.size());
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// protect_features (src/v2m_particles.cpp:1002) with the grid's feature curves, as above
+int64_t ref_protect_features(void* h, int64_t n, float* P, uint16_t* lab, uint8_t* typ, uint16_t* part,
+                             uint16_t* part3) {
+    try {
+        RefGrid* r = static_cast<RefGrid*>(h);
+        tn::Nodes x = nodes_of(n, P, lab, typ, part, part3);
+        tn::protect_features(r->g, x);
+        nodes_back(x, P, lab, typ, part, part3);
+        return static_cast<int64_t>(x.size());
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// the grid's feature curves (Grid::feat): their count, and the values when out is not null
+int64_t ref_grid_feat(void* h, float* out) {
+    const std::vector<float>& f = static_cast<RefGrid*>(h)->g.feat;
+    if (out) {
+        std::copy(f.begin(), f.end(), out);
+    }
+    return static_cast<int64_t>(f.size());
+}
+
 void ref_hash(void* h, const tn_relax_params* prm, tn_hash_info* hi) {
     int nkeys = 0;
     const refh::V2mHash H = hash_of(static_cast<RefGrid*>(h)->g, to_params(prm), &nkeys);

@@ -7,7 +7,8 @@
 // run_pipeline does (src/v2m_pipeline.cpp:497-526). With a device the HIP results
 // must equal the CPU path's bit for bit: build_grid_hip (prob_fields included, on a
 // tissue-probability input) against build_grid_cpu, seed_hip against seed_cpu,
-// relax_hip and relax_hip_batch against relax_cpu.
+// relax_hip and relax_hip_batch against relax_cpu, thin_hip against thin_nodes on the
+// seeds, protect_hip against protect_features (a grid with feature curves).
 //
 // Exit status 0 and a last line "ok <mode>" when every check passed.
 #include <cmath>
@@ -145,6 +146,56 @@ static bool run_case(const char* name, bool tpm, bool device) {
     CHECK(same_nodes(nd, want), "relaxed nodes == relax_cpu");
     CHECK(st.iters == sw.iters && st.rebuilds == sw.rebuilds && st.fire_resets == sw.fire_resets, "stats");
     CHECK(st.n_interior == sw.n_interior && st.n_interface == sw.n_interface && st.n_junction == sw.n_junction, "types");
+
+    // thinning of the seeds, as run_pipeline does before the relaxation
+    for (float B : { 0.6f, 1.1f }) {
+        RelaxParams tp = prm;
+        tp.thin = B;
+        Nodes tw = seeds, th = seeds;
+        const size_t rw = thin_nodes(g, tp, tw);
+        threw = false;
+        size_t rh = 0;
+        try {
+            rh = thin_hip(g, tp, th, -1);
+        } catch (const std::runtime_error&) {
+            threw = true;
+        }
+        if (!device) {
+            CHECK(threw, "thin_hip without a device throws");
+        } else {
+            CHECK(!threw && rh == rw && same_nodes(th, tw) && same(th.part3, tw.part3), "thin_hip == thin_nodes");
+        }
+    }
+    // feature protection: pinned nodes along a segment through the volume, with
+    // interface nodes between them (the grid's own feature curves stand in for a
+    // shape input's)
+    if (device) {
+        Grid gf = g;
+        const float a[3] = { 6.0f, 20.0f, 20.0f }, b[3] = { 34.0f, 20.0f, 20.0f };
+        gf.feat = { 2.0f, a[0], a[1], a[2], b[0], b[1], b[2] };
+        Nodes fn = want;
+        for (int k = 0; k <= 14; ++k) {   // pins every 2 mm along the segment
+            fn.P.insert(fn.P.end(), { a[0] + 2.0f * k, a[1], a[2] });
+            fn.lab.push_back(1);
+            fn.typ.push_back(3);
+            fn.part.insert(fn.part.end(), { 2, 0 });
+            fn.part3.push_back((uint16_t)0xFFFF);
+        }
+        for (int k = 0; k < 14; ++k) {   // and a node between each pair
+            fn.P.insert(fn.P.end(), { a[0] + 2.0f * k + 1.0f, a[1] + 0.3f, a[2] });
+            fn.lab.push_back(1);
+            fn.typ.push_back(1);
+            fn.part.insert(fn.part.end(), { 2, 0 });
+            fn.part3.push_back((uint16_t)0xFFFF);
+        }
+        if (fn.part3.size() != fn.typ.size()) {
+            fn.part3.resize(fn.typ.size(), (uint16_t)0xFFFF);
+        }
+        Nodes fw = fn, fh = fn;
+        const size_t pw = protect_features(gf, fw);
+        const size_t ph = protect_hip(gf, fh, -1);
+        CHECK(pw > 0 && ph == pw && same_nodes(fh, fw) && same(fh.part3, fw.part3), "protect_hip == protect_features");
+    }
 
     // the batch: the same volume twice
     if (device) {

@@ -24,6 +24,7 @@
 
 #include "tn_kernels.hpp"
 #include "tn_seed.hpp"
+#include "tn_thin.hpp"
 
 namespace {
 
@@ -1496,6 +1497,358 @@ void run_grouped(Plans& plans, int64_t& launches, int64_t& steps) {
     TN_HIP(hipStreamSynchronize(bs));
 }
 
+// ---- node thinning and feature protection (tn_thin.hpp) ------------------------------
+
+// the h field of a grid view as a Field (v2m_h_at reads only h and the voxel sizes)
+tnd::Field size_field(const tn_grid_view* g, DevMem& mem) {
+    tnd::Field f{};
+    f.nx = g->nx;
+    f.ny = g->ny;
+    f.nz = g->nz;
+    f.nbx = g->nbx;
+    f.nby = g->nby;
+    f.nbz = g->nbz;
+    f.vx = g->vs[0];
+    f.vy = g->vs[1];
+    f.vz = g->vs[2];
+    set_reciprocals(f);
+    f.h = mem.upload(g->h, (size_t)g->nx * g->ny * g->nz);
+    return f;
+}
+
+// v2m_h_at on the host, src/opencl/v2m_particle_body.cl:30-36
+float host_h_at(const tn_grid_view* g, float px, float py, float pz) {
+    int i = (int)std::floor(px / g->vs[0] + 0.5f), j = (int)std::floor(py / g->vs[1] + 0.5f),
+        k = (int)std::floor(pz / g->vs[2] + 0.5f);
+    i = i < 0 ? 0 : (i >= g->nx ? g->nx - 1 : i);
+    j = j < 0 ? 0 : (j >= g->ny ? g->ny - 1 : j);
+    k = k < 0 ? 0 : (k >= g->nz ? g->nz - 1 : k);
+    return g->h[i + (size_t)g->nx * (j + (size_t)g->ny * k)];
+}
+
+// The nodes of a compaction on the device (null stream), and the kept ones back into
+// the host arrays, in order. keep[0..n) holds the flags; returns the kept count.
+struct NodeArraysDev {
+    float* P = nullptr;
+    uint16_t *lab = nullptr, *part = nullptr, *part3 = nullptr;
+    uint8_t* typ = nullptr;
+};
+
+NodeArraysDev upload_nodes(int n, const float* P, const uint16_t* lab, const uint8_t* typ, const uint16_t* part,
+                           const uint16_t* part3, DevMem& mem) {
+    NodeArraysDev d;
+    d.P = mem.upload(P, (size_t)3 * n);
+    d.lab = mem.upload(lab, (size_t)n);
+    d.typ = mem.upload(typ, (size_t)n);
+    d.part = mem.upload(part, (size_t)2 * n);
+    if (part3) {
+        d.part3 = mem.upload(part3, (size_t)n);
+    } else {   // no third partners
+        const std::vector<uint16_t> none((size_t)n, (uint16_t)TN_NOLAB);
+        d.part3 = mem.upload(none.data(), (size_t)n);
+    }
+    return d;
+}
+
+int64_t compact_back(const NodeArraysDev& d, int n, int32_t* keep, SeedOps& ops, float* P, uint16_t* lab,
+                     uint8_t* typ, uint16_t* part, uint16_t* part3) {
+    DevMem& mem = ops.mem;
+    const int m = ops.scan(keep, n);
+    if (m == n) {
+        return n;
+    }
+    NodeArraysDev o;
+    o.P = mem.alloc<float>((size_t)3 * std::max(m, 1));
+    o.lab = mem.alloc<uint16_t>(std::max(m, 1));
+    o.typ = mem.alloc<uint8_t>(std::max(m, 1));
+    o.part = mem.alloc<uint16_t>((size_t)2 * std::max(m, 1));
+    o.part3 = mem.alloc<uint16_t>(std::max(m, 1));
+    tnthin::k_compact_nodes<<<blocks_for(n, tnk::kBlock), tnk::kBlock>>>(keep, n, d.P, d.lab, d.typ, d.part, d.part3, o.P,
+                                                                         o.lab, o.typ, o.part, o.part3);
+    TN_HIP(hipGetLastError());
+    if (m > 0) {
+        TN_HIP(hipMemcpy(P, o.P, sizeof(float) * 3 * m, hipMemcpyDeviceToHost));
+        TN_HIP(hipMemcpy(lab, o.lab, sizeof(uint16_t) * m, hipMemcpyDeviceToHost));
+        TN_HIP(hipMemcpy(typ, o.typ, m, hipMemcpyDeviceToHost));
+        TN_HIP(hipMemcpy(part, o.part, sizeof(uint16_t) * 2 * m, hipMemcpyDeviceToHost));
+        if (part3) {
+            TN_HIP(hipMemcpy(part3, o.part3, sizeof(uint16_t) * m, hipMemcpyDeviceToHost));
+        }
+    }
+    return m;
+}
+
+// thin_nodes (src/v2m_particles.cpp:1223-1357) with factor B on device dev: the host
+// arrays are compacted in place; returns the kept count
+int64_t thin_on_device(const tn_grid_view* g, float B, int n, float* P, uint16_t* lab, uint8_t* typ, uint16_t* part,
+                       uint16_t* part3, int dev) {
+    TN_HIP(hipSetDevice(dev));
+    DevMem mem;
+    mem.device = dev;
+    SeedOps ops{ mem };
+    const tnd::Field f = size_field(g, mem);
+    const NodeArraysDev d = upload_nodes(n, P, lab, typ, part, part3, mem);
+    float4* X = mem.alloc<float4>(n);
+    int32_t* cand = ops.counts(n);
+    int32_t* hkey = mem.alloc<int32_t>(n);
+    int32_t* state = mem.alloc<int32_t>(n);
+    const unsigned nb = blocks_for(n, tnk::kBlock);
+    tnthin::k_thin_init<<<nb, tnk::kBlock>>>(f, d.P, d.typ, n, X, cand, hkey, state);
+    TN_HIP(hipGetLastError());
+    const int m = ops.scan(cand, n);
+    int32_t* rank = mem.alloc<int32_t>(n);
+    tnk::k_fill_i32<<<nb, tnk::kBlock>>>(rank, n, -1);
+    int32_t* order = nullptr;   // the candidates in the greedy order
+    if (m > 0) {
+        // interfaces first, by h, by index: stable passes over h, then interior-ness
+        int32_t* list = mem.alloc<int32_t>(m);
+        tnk::k_compact_index<<<nb, tnk::kBlock>>>(cand, n, list);
+        int32_t* keys = mem.alloc<int32_t>(m);
+        const unsigned mb = blocks_for(m, tnk::kBlock);
+        tnthin::k_thin_gather_keys<<<mb, tnk::kBlock>>>(hkey, list, m, keys);
+        auto s1 = ops.sort(keys, list, m, 32);
+        tnthin::k_thin_interior<<<nb, tnk::kBlock>>>(d.typ, n, hkey);   // hkey now: interior flags
+        int32_t* keys2 = mem.alloc<int32_t>(m);
+        tnthin::k_thin_gather_keys<<<mb, tnk::kBlock>>>(hkey, s1.second, m, keys2);
+        order = ops.sort(keys2, s1.second, m, 1).second;
+        tnthin::k_thin_rank<<<mb, tnk::kBlock>>>(order, m, rank);
+        TN_HIP(hipGetLastError());
+    }
+    // the hash of src/v2m_particles.cpp:1240-1255, and a superset grid per level that a
+    // candidate can take: nodes of h in [min h, max h] of the grid
+    tnthin::Levels Lv{};
+    Lv.B = B;
+    Lv.c0 = std::max(1e-6f, B * g->hmin);
+    Lv.nlev = 1;
+    while (Lv.nlev < 12 && Lv.c0 * (float)(1 << (Lv.nlev - 1)) < B * g->hmax) {
+        ++Lv.nlev;
+    }
+    float lo[3] = { INFINITY, INFINITY, INFINITY }, hi[3] = { -INFINITY, -INFINITY, -INFINITY };
+    for (int i = 0; i < n; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], P[3 * i + a]);
+            hi[a] = std::max(hi[a], P[3 * i + a]);
+        }
+    }
+    float ext = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        Lv.lo[a] = lo[a];
+        ext = std::max(ext, hi[a] - lo[a]);
+    }
+    const size_t nv = (size_t)g->nx * g->ny * g->nz;
+    float hlo = INFINITY, hhi = 0.0f;
+    for (size_t v = 0; v < nv; ++v) {
+        hlo = std::min(hlo, g->h[v]);
+        hhi = std::max(hhi, g->h[v]);
+    }
+    auto level_of = [&](float r) {
+        int L = 0;
+        while (L + 1 < Lv.nlev && Lv.c0 * (float)(1 << L) < r) {
+            ++L;
+        }
+        return L;
+    };
+    const int Lmin = level_of(B * hlo), Lmax = level_of(B * hhi);
+    int64_t ncells = 0;
+    for (int L = 0; L < Lv.nlev; ++L) {
+        const float c = Lv.c0 * (float)(1 << L);
+        Lv.C[L] = std::max(c, ext / 256.0f);
+        Lv.off[L] = ncells;
+        for (int a = 0; a < 3; ++a) {
+            Lv.dim[L][a] = L < Lmin || L > Lmax ? 1 : (int)std::floor((hi[a] - lo[a]) / Lv.C[L]) + 1;
+        }
+        if (L >= Lmin && L <= Lmax) {
+            ncells += (int64_t)Lv.dim[L][0] * Lv.dim[L][1] * Lv.dim[L][2];
+        }
+    }
+    arg_check(ncells < ((int64_t)1 << 31) - 1, "thin: too many cells");
+    int32_t* cstart = ops.counts(ncells);
+    int32_t* clist = mem.alloc<int32_t>((size_t)n * (Lmax - Lmin + 1));
+    int32_t* fill = ops.counts(ncells);
+    for (int L = Lmin; L <= Lmax; ++L) {
+        tnthin::k_thin_count_cells<<<nb, tnk::kBlock>>>(Lv, X, n, L, cstart);
+    }
+    ops.scan(cstart, ncells);
+    for (int L = Lmin; L <= Lmax; ++L) {
+        tnthin::k_thin_fill_cells<<<nb, tnk::kBlock>>>(Lv, X, n, L, cstart, fill, clist);
+    }
+    TN_HIP(hipGetLastError());
+    if (m > 0) {   // each candidate's earlier conflicts, then the rounds
+        tnthin::Nodes tn{ X, d.lab, d.typ, d.part, d.part3, rank, n };
+        const unsigned mb = blocks_for(m, tnk::kBlock);
+        int32_t* coff = ops.counts(m);
+        tnthin::k_thin_count<<<mb, tnk::kBlock>>>(Lv, tn, cstart, clist, order, m, coff);
+        const int ncon = ops.scan(coff, m);
+        int32_t* con = mem.alloc<int32_t>(std::max(ncon, 1));
+        tnthin::k_thin_fill<<<mb, tnk::kBlock>>>(Lv, tn, cstart, clist, order, m, coff, con);
+        int32_t* left = mem.alloc<int32_t>(1);
+        int32_t open = m;
+        for (int round = 0; open > 0; ++round) {
+            if (round > m) {
+                throw TnError(TN_ERR_INTERNAL, "thin: the rounds do not converge");
+            }
+            tnk::k_fill_i32<<<1, tnk::kBlock>>>(left, 1, 0);
+            tnthin::k_thin_round<<<mb, tnk::kBlock>>>(order, m, coff, con, state, left);
+            TN_HIP(hipGetLastError());
+            TN_HIP(hipMemcpy(&open, left, sizeof open, hipMemcpyDeviceToHost));
+        }
+    }
+    int32_t* keep = ops.counts(n);
+    tnthin::k_thin_keep<<<nb, tnk::kBlock>>>(state, n, keep);
+    return compact_back(d, n, keep, ops, P, lab, typ, part, part3);
+}
+
+// protect_features (src/v2m_particles.cpp:1002-1221) with the feature curves feat on
+// device dev. The balls between neighbouring pinned nodes on each curve are built on
+// the host as the reference builds them (a few per curve); the test of every node
+// against the balls of its cell, and the compaction, run on the device.
+int64_t protect_on_device(const tn_grid_view* g, const float* feat, int64_t nfeat, int n, float* P, uint16_t* lab,
+                          uint8_t* typ, uint16_t* part, uint16_t* part3, int dev) {
+    std::vector<int> pins;
+    for (int i = 0; i < n; ++i) {
+        if (typ[i] == TN_CORNER) {
+            pins.push_back(i);
+        }
+    }
+    // locate (src/v2m_particles.cpp:1018-1084): distance of p from curve `type` q, and
+    // its parameter along it
+    auto seg = [](const float* a, const float* b, const float* p, float& dd, float& tt) {
+        float ab[3], ap[3], ab2 = 0, apab = 0;
+        for (int k = 0; k < 3; ++k) {
+            ab[k] = b[k] - a[k];
+            ap[k] = p[k] - a[k];
+            ab2 += ab[k] * ab[k];
+            apab += ap[k] * ab[k];
+        }
+        tt = ab2 > 0 ? std::min(1.0f, std::max(0.0f, apab / ab2)) : 0.0f;
+        dd = 0;
+        for (int k = 0; k < 3; ++k) {
+            const float e = ap[k] - tt * ab[k];
+            dd += e * e;
+        }
+        dd = std::sqrt(dd);
+    };
+    auto locate = [&](int type, const float* q, const float* p, float& dist, float& t) {
+        if (type == 2) {
+            seg(q, q + 3, p, dist, t);
+        } else if (type == 4) {
+            const int m = (int)q[0];
+            dist = 1e30f;
+            for (int i = 0; i + 1 < m; ++i) {
+                float dd, tt;
+                seg(q + 1 + 3 * i, q + 4 + 3 * i, p, dd, tt);
+                if (dd < dist) {
+                    dist = dd;
+                    t = (float)i + tt;
+                }
+            }
+        } else {
+            const float *c = q, *nn = q + 3;
+            float v[3], h = 0;
+            for (int k = 0; k < 3; ++k) {
+                v[k] = p[k] - c[k];
+                h += v[k] * nn[k];
+            }
+            float rad[3], rho = 0;
+            for (int k = 0; k < 3; ++k) {
+                rad[k] = v[k] - h * nn[k];
+                rho += rad[k] * rad[k];
+            }
+            rho = std::sqrt(rho);
+            dist = std::hypot(h, rho - q[6]);
+            const float a0 = std::fabs(nn[0]) < 0.9f ? 1.0f : 0.0f, a1 = a0 > 0 ? 0.0f : 1.0f;
+            float u[3] = { -nn[2] * a1, nn[2] * a0, nn[0] * a1 - nn[1] * a0 };
+            const float lu = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            for (float& x : u) {
+                x /= lu;
+            }
+            const float w[3] = { nn[1] * u[2] - nn[2] * u[1], nn[2] * u[0] - nn[0] * u[2], nn[0] * u[1] - nn[1] * u[0] };
+            t = std::atan2(rad[0] * w[0] + rad[1] * w[1] + rad[2] * w[2], rad[0] * u[0] + rad[1] * u[1] + rad[2] * u[2]);
+        }
+    };
+    std::vector<float4> balls;   // centre, radius^2
+    for (int64_t k = 0; k < nfeat;) {
+        const int type = (int)feat[k];
+        arg_check(type >= 1 && type <= 4, "features: unknown curve type");
+        const float* q = &feat[k + 1];
+        const int64_t len = type == 1 ? 4 : type == 2 ? 7 : type == 4 ? 2 + 3 * (int64_t)q[0] : 8;
+        arg_check(k + len <= nfeat && (type != 4 || q[0] >= 0), "features: a curve beyond the array");
+        k += len;
+        if (type == 1 || (type == 4 && q[0] < 2)) {
+            continue;
+        }
+        std::vector<std::pair<float, int>> on;
+        for (int j : pins) {
+            const float* p = &P[3 * (size_t)j];
+            float dist, t = 0;
+            locate(type, q, p, dist, t);
+            if (dist < 1e-3f * host_h_at(g, p[0], p[1], p[2])) {
+                on.emplace_back(t, j);
+            }
+        }
+        std::sort(on.begin(), on.end());
+        const size_t m = on.size();
+        for (size_t i = 0; i + 1 < m + (type == 3 && m > 2 ? 1 : 0); ++i) {   // (a circle closes)
+            const float *a = &P[3 * (size_t)on[i].second], *b = &P[3 * (size_t)on[(i + 1) % m].second];
+            const float c[3] = { 0.5f * (a[0] + b[0]), 0.5f * (a[1] + b[1]), 0.5f * (a[2] + b[2]) };
+            const float r2 = 0.25f * ((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) +
+                                      (a[2] - b[2]) * (a[2] - b[2]));
+            const float h = host_h_at(g, c[0], c[1], c[2]);
+            if (r2 > 0 && r2 <= 0.64f * h * h) {
+                balls.push_back(make_float4(c[0], c[1], c[2], r2));
+            }
+        }
+    }
+    if (balls.empty()) {
+        return n;
+    }
+    // the balls by cells of the largest diameter (src/v2m_particles.cpp:1122-1157)
+    float cs = 0;
+    for (const float4& bl : balls) {
+        cs = std::max(cs, 2.0f * std::sqrt(bl.w));
+    }
+    auto key = [](int64_t ix, int64_t iy, int64_t iz) {
+        return (uint64_t)(ix + (1 << 20)) << 42 | (uint64_t)(iy + (1 << 20)) << 21 | (uint64_t)(iz + (1 << 20));
+    };
+    std::vector<std::pair<uint64_t, int32_t>> cells;
+    for (size_t bi = 0; bi < balls.size(); ++bi) {
+        const float bl[4] = { balls[bi].x, balls[bi].y, balls[bi].z, balls[bi].w };
+        const float r = std::sqrt(bl[3]);
+        int64_t clo[3], chi[3];
+        for (int k = 0; k < 3; ++k) {
+            clo[k] = (int64_t)std::floor((bl[k] - r) / cs);
+            chi[k] = (int64_t)std::floor((bl[k] + r) / cs);
+        }
+        for (int64_t x = clo[0]; x <= chi[0]; ++x) {
+            for (int64_t y = clo[1]; y <= chi[1]; ++y) {
+                for (int64_t z = clo[2]; z <= chi[2]; ++z) {
+                    cells.emplace_back(key(x, y, z), (int32_t)bi);
+                }
+            }
+        }
+    }
+    std::sort(cells.begin(), cells.end());
+    std::vector<uint64_t> ckey(cells.size());
+    std::vector<int32_t> cball(cells.size());
+    for (size_t e = 0; e < cells.size(); ++e) {
+        ckey[e] = cells[e].first;
+        cball[e] = cells[e].second;
+    }
+    TN_HIP(hipSetDevice(dev));
+    DevMem mem;
+    mem.device = dev;
+    SeedOps ops{ mem };
+    const NodeArraysDev d = upload_nodes(n, P, lab, typ, part, part3, mem);
+    const uint64_t* dkey = mem.upload(ckey.data(), ckey.size());
+    const int32_t* dball = mem.upload(cball.data(), cball.size());
+    const float4* dballs = mem.upload(balls.data(), balls.size());
+    int32_t* keep = ops.counts(n);
+    tnthin::k_protect<<<blocks_for(n, tnk::kBlock), tnk::kBlock>>>(d.P, d.typ, n, cs, dkey, dball, (int)ckey.size(),
+                                                                   dballs, keep);
+    TN_HIP(hipGetLastError());
+    return compact_back(d, n, keep, ops, P, lab, typ, part, part3);
+}
+
 }  // namespace
 
 // The device memory of tn_relax_noalloc: one array per GPU, held from the library's
@@ -1844,6 +2197,34 @@ int tn_seed(const tn_grid_view* g, const uint8_t* grade, const tn_relax_params* 
         if (part3) {
             TN_HIP(hipMemcpy(part3, s.part3, sizeof(uint16_t) * s.n, hipMemcpyDeviceToHost));
         }
+    });
+}
+
+int tn_thin_nodes(const tn_grid_view* g, const tn_relax_params* prm, int64_t n, float* P, uint16_t* lab, uint8_t* typ,
+                  uint16_t* part, uint16_t* part3, int64_t* n_kept, int32_t device) {
+    return guarded([&] {
+        arg_check(prm != nullptr && n_kept != nullptr, "thin: null argument");
+        arg_check(n >= 0 && n < INT32_MAX && (n == 0 || (P && lab && typ && part)), "thin: bad node arrays");
+        check_grid(g);
+        *n_kept = n;
+        if (!(prm->thin > 0.0f) || n == 0) {   // src/v2m_particles.cpp:1227-1229
+            return;
+        }
+        *n_kept = thin_on_device(g, prm->thin, (int)n, P, lab, typ, part, part3, pick_device(device));
+    });
+}
+
+int tn_protect_features(const tn_grid_view* g, const float* feat, int64_t nfeat, int64_t n, float* P, uint16_t* lab,
+                        uint8_t* typ, uint16_t* part, uint16_t* part3, int64_t* n_kept, int32_t device) {
+    return guarded([&] {
+        arg_check(n_kept != nullptr && nfeat >= 0 && (nfeat == 0 || feat), "features: null argument");
+        arg_check(n >= 0 && n < INT32_MAX && (n == 0 || (P && lab && typ && part)), "features: bad node arrays");
+        check_grid(g);
+        *n_kept = n;
+        if (nfeat == 0 || n == 0) {   // src/v2m_particles.cpp:1005-1007
+            return;
+        }
+        *n_kept = protect_on_device(g, feat, nfeat, (int)n, P, lab, typ, part, part3, pick_device(device));
     });
 }
 

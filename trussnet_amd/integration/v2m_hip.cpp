@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <stdexcept>
+#include <string>
 
 #include "trussnet_amd.hpp"
 
@@ -77,6 +78,29 @@ tn_nodes_view nodes_view(Nodes& nd) {
     return tn_nodes_view{ (int64_t)n, nd.P.data(), nd.lab.data(), nd.typ.data(), nd.part.data() };
 }
 
+// A call that compacts the node arrays in place (thin_nodes, protect_features), then
+// the vectors resized to the nodes kept, as the reference's own functions leave them.
+// Returns the count removed.
+template <class Call>
+size_t compacting(Nodes& nd, const char* what, Call call) {
+    const size_t n = nd.size();
+    if (nd.P.size() != 3 * n || nd.typ.size() != n || nd.part.size() != 2 * n ||
+        (!nd.part3.empty() && nd.part3.size() != n)) {
+        throw std::runtime_error(std::string(what) + ": inconsistent Nodes arrays");
+    }
+    int64_t kept = (int64_t)n;
+    tnamd::check(call((int64_t)n, nd.part3.empty() ? nullptr : nd.part3.data(), &kept), what);
+    const size_t m = (size_t)kept;
+    nd.P.resize(3 * m);
+    nd.lab.resize(m);
+    nd.typ.resize(m);
+    nd.part.resize(2 * m);
+    if (!nd.part3.empty()) {
+        nd.part3.resize(m);
+    }
+    return n - m;
+}
+
 void stats_from(const tn_relax_stats& s, RelaxStats& st) {
     st.iters = s.iters;
     st.rebuilds = s.rebuilds;
@@ -140,6 +164,23 @@ void seed_hip(const Grid& g, const RelaxParams& prm, Nodes& nd, int device) {
                          out.part.data(), out.part3.data()),
                  "tn_seed");
     nd = std::move(out);
+}
+
+size_t thin_hip(const Grid& g, const RelaxParams& prm, Nodes& nd, int device) {
+    const tn_grid_view gv = grid_view(g);
+    const tn_relax_params pc = params(prm);
+    return compacting(nd, "tn_thin_nodes", [&](int64_t n, uint16_t* part3, int64_t* kept) {
+        return tn_thin_nodes(&gv, &pc, n, nd.P.data(), nd.lab.data(), nd.typ.data(), nd.part.data(), part3, kept,
+                             device);
+    });
+}
+
+size_t protect_hip(const Grid& g, Nodes& nd, int device) {
+    const tn_grid_view gv = grid_view(g);
+    return compacting(nd, "tn_protect_features", [&](int64_t n, uint16_t* part3, int64_t* kept) {
+        return tn_protect_features(&gv, g.feat.empty() ? nullptr : g.feat.data(), (int64_t)g.feat.size(), n,
+                                   nd.P.data(), nd.lab.data(), nd.typ.data(), nd.part.data(), part3, kept, device);
+    });
 }
 
 void build_grid_hip(const LabelVolume& lv, const GridParams& p, Grid& g, int device) {

@@ -32,7 +32,7 @@ SYMBOLS = ["tn_relax_params_default", "tn_last_error", "tn_device_count", "tn_re
            "tn_plan_timings", "tn_adj_create", "tn_adj_forward", "tn_adj_backward", "tn_adj_loss_terms",
            "tn_adj_destroy", "tn_grid_params_default", "tn_grid_build", "tn_grid_get_info", "tn_grid_download",
            "tn_grid_destroy", "tn_seed_count_from_grid", "tn_seed_from_grid", "tn_plan_create_from_grid",
-           "tn_plan_field_points"]
+           "tn_plan_field_points", "tn_thin_nodes", "tn_protect_features"]
 
 
 class DeviceError(RuntimeError):
@@ -80,6 +80,8 @@ def lib():
         L.tn_plan_force.argtypes = [vp, vp]
         L.tn_plan_field_points.argtypes = [vp, C.c_int32, C.c_int32, C.c_int64, vp, vp, vp, vp, vp, vp]
         L.tn_plan_step.argtypes = [vp, C.c_int32, C.POINTER(RelaxStatsC)]
+        L.tn_thin_nodes.argtypes = [C.POINTER(GridView), C.POINTER(RelaxParamsC), C.c_int64] + [vp] * 6 + [C.c_int32]
+        L.tn_protect_features.argtypes = [C.POINTER(GridView), vp, C.c_int64, C.c_int64] + [vp] * 6 + [C.c_int32]
         L.tn_plan_set_timing.argtypes = [vp, C.c_int32]
         L.tn_plan_timings.argtypes = [vp, C.POINTER(KernelTimeC), C.c_int32, C.POINTER(C.c_int32)]
         L.tn_seed_count.argtypes = [C.POINTER(GridView), vp, C.POINTER(RelaxParamsC), C.c_int32,
@@ -183,6 +185,31 @@ def seed(grid: Grid, prm: RelaxParams | None = None, device: int = -1) -> Nodes:
     _check(lib().tn_seed(C.byref(gv), _p(grade), C.byref(pc), device, n, _p(P), _p(lab), _p(typ), _p(part),
                          _p(part3)))
     return Nodes(P, lab, typ, part, part3)
+
+
+def _compacted(fn, nodes: Nodes) -> Nodes:
+    x = nodes.copy()
+    part3 = x.part3.copy() if x.part3 is not None else None
+    kept = C.c_int64(0)
+    _check(fn(x.n, _p(x.P), _p(x.lab), _p(x.typ), _p(x.part), _p(part3) if part3 is not None else None,
+              C.byref(kept)))
+    m = int(kept.value)
+    return Nodes(x.P[:m], x.lab[:m], x.typ[:m], x.part[:m], part3[:m] if part3 is not None else None)
+
+
+def thin(grid: Grid, nodes: Nodes, thin: float, device: int = -1) -> Nodes:
+    """thin_nodes on the device (src/v2m_particles.cpp:1223-1357): the nodes kept, in
+    order, with factor ``thin`` (<= 0: all)."""
+    gv, pc = grid.view(), RelaxParams(thin=thin).to_c()
+    return _compacted(lambda *a: lib().tn_thin_nodes(C.byref(gv), C.byref(pc), *a, device), nodes)
+
+
+def protect_features(grid: Grid, feat: np.ndarray, nodes: Nodes, device: int = -1) -> Nodes:
+    """protect_features (src/v2m_particles.cpp:1002-1221) with the shape input's feature
+    curves ``feat`` (Grid::feat): the nodes kept, in order."""
+    gv = grid.view()
+    feat = np.ascontiguousarray(feat, np.float32)
+    return _compacted(lambda *a: lib().tn_protect_features(C.byref(gv), _p(feat), feat.size, *a, device), nodes)
 
 
 class Plan:
