@@ -329,13 +329,10 @@ __global__ __launch_bounds__(kBlock) void k_permute(const int32_t* __restrict__ 
 // before accepting it.)
 
 constexpr int kTrussWave = 64;   // one wave per block
-#ifndef TN_TRUSS_COLLECT
-#define TN_TRUSS_COLLECT 32
-#endif
 #ifndef TN_TRUSS_SPAN
 #define TN_TRUSS_SPAN 16
 #endif
-constexpr int kCollect = TN_TRUSS_COLLECT;   // per-lane candidate buffer (entries)
+constexpr int kCollect = 32;   // per-lane candidate buffer (entries): settled by a 32-key network
 constexpr int kGroupSpan = TN_TRUSS_SPAN;    // cells along a row that one group of lanes may span
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -367,26 +364,39 @@ __device__ __forceinline__ uint64_t list_key(float d, int id) {
     return ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)id;
 }
 
-// Settle entries [lo, cnt) of a lane's buffer (entries e*64 + lane): d^2 -> sqrtf(d^2),
-// then insertion into the ordered prefix [0, lo).
-__device__ __forceinline__ void settle_keys(uint64_t* cb, int lane, int lo, int cnt) {
-    for (int e = lo; e < cnt; ++e) {
-        const uint64_t v = cb[e * kTrussWave + lane];
-        cb[e * kTrussWave + lane] = list_key(sqrtf(__uint_as_float((uint32_t)(v >> 32))), (int)(uint32_t)v);
-    }
-    for (int e = lo > 1 ? lo : 1; e < cnt; ++e) {
-        const uint64_t v = cb[e * kTrussWave + lane];
-        int k = e;
-        while (k > 0) {
-            const uint64_t w = cb[(k - 1) * kTrussWave + lane];
-            if (!(v < w)) {
-                break;
+// Sort 32 keys in registers: a bitonic network, every index known at compile time.
+__device__ __forceinline__ void bitonic32(uint64_t (&k)[32]) {
+#pragma unroll
+    for (int size = 2; size <= 32; size <<= 1) {
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+            for (int i = 0; i < 32; ++i) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const uint64_t x = k[i], y = k[j];
+                    const bool swap = (i & size) == 0 ? x > y : x < y;
+                    k[i] = swap ? y : x;
+                    k[j] = swap ? x : y;
+                }
             }
-            cb[k * kTrussWave + lane] = w;
-            --k;
         }
-        cb[k * kTrussWave + lane] = v;
     }
+}
+
+// A lane's buffer (entries e*64 + lane, kCollect == 32 of them) settled into k, in
+// order: entries [0, lo) hold distances already, [lo, cnt) d^2 (-> sqrtf(d^2)),
+// [cnt, 32) are unused (the largest key).
+__device__ __forceinline__ void settle_keys(const uint64_t* cb, int lane, int lo, int cnt, uint64_t (&k)[32]) {
+#pragma unroll
+    for (int e = 0; e < 32; ++e) {
+        uint64_t v = e < cnt ? cb[e * kTrussWave + lane] : ~0ull;
+        if (e >= lo && e < cnt) {
+            v = list_key(sqrtf(__uint_as_float((uint32_t)(v >> 32))), (int)(uint32_t)v);
+        }
+        k[e] = v;
+    }
+    bitonic32(k);
 }
 
 // ---- the distance prefilter on the matrix cores ------------------------------------------
@@ -601,19 +611,17 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                 }
                 const int rows = nr;
                 nr = 0;
+                int rfirst = 0;   // the row holding the chunk's first candidate
                 for (int base = 0; base < total; base += kTrussWave) {
                     __syncthreads();   // the previous chunk is consumed; the row starts are written
+                    while (rfirst + 1 < rows && r_beg[rfirst + 1] <= base) {
+                        ++rfirst;
+                    }
                     const int p = base + lane;
-                    int lo = 0;   // the candidate's row
-                    if (p < total) {   // the last row that starts at or before p holds it
-                        int up = rows;
-                        while (up - lo > 1) {
-                            const int mid = (lo + up) >> 1;
-                            if (r_beg[mid] <= p) {
-                                lo = mid;
-                            } else {
-                                up = mid;
-                            }
+                    int lo = rfirst;   // the candidate's row: the last that starts at or before p
+                    if (p < total) {
+                        while (lo + 1 < rows && r_beg[lo + 1] <= p) {
+                            ++lo;
                         }
                         const int s = r_sa[lo] + (p - r_beg[lo]);
                         cpos[lane] = in.Ps[s];
@@ -673,7 +681,12 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                             continue;
                         }
                         if (cnt == kCollect) {   // keep the 24 best so far
-                            settle_keys(cb, lane, nval, cnt);
+                            uint64_t kk[32];
+                            settle_keys(cb, lane, nval, cnt, kk);
+#pragma unroll
+                            for (int e = 0; e < kK; ++e) {
+                                cb[e * kTrussWave + lane] = kk[e];
+                            }
                             nval = cnt = kK;
                         }
                         cb[cnt * kTrussWave + lane] = ((uint64_t)__float_as_uint(d2) << 32) | (uint32_t)mt.x;
@@ -747,11 +760,15 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
             }
         }   // the groups
         if (active) {   // (distance, id) order; the list holds slots
-            settle_keys(cb, lane, nval, cnt);
-            const int kk = cnt < kK ? cnt : kK;
-            nnb[q] = kk;
-            for (int k = 0; k < kk; ++k) {
-                nbrT[(size_t)k * n + q] = in.slot_of[(int)(uint32_t)cb[k * kTrussWave + lane]];
+            uint64_t kk[32];
+            settle_keys(cb, lane, nval, cnt, kk);
+            const int m = cnt < kK ? cnt : kK;
+            nnb[q] = m;
+#pragma unroll
+            for (int k = 0; k < kK; ++k) {
+                if (k < m) {
+                    nbrT[(size_t)k * n + q] = in.slot_of[(int)(uint32_t)kk[k]];
+                }
             }
         }
         __syncthreads();   // the buffers are reused by the next block of queries
@@ -973,15 +990,44 @@ __global__ __launch_bounds__(kBlock) void k_order_count(const uint8_t* __restric
 
 // ---- force --------------------------------------------------------------------------
 
-__device__ __forceinline__ void k_force_body(const float4* __restrict__ X, const uint8_t* __restrict__ typ, const int32_t* __restrict__ nbrT, const int32_t* __restrict__ nnb, float fscale, float fsurf, int n, float4* __restrict__ F, int bid, int nblk) {
-    const int i = bid * kBlock + threadIdx.x;
-    if (i < n) {
-        F[i] = bar_force(X, typ, nbrT, nnb, fscale, fsurf, i, n);
+// append i to queue for the lanes with flag set (one atomic on *count per wave)
+__device__ __forceinline__ void wave_append(bool flag, int i, int32_t* count, int32_t* __restrict__ queue) {
+    const uint64_t m = __ballot(flag);
+    if (m) {
+        const int lead = __ffsll((long long)m) - 1;
+        int base = 0;
+        if ((int)(threadIdx.x & (kWave - 1)) == lead) {
+            base = atomicAdd(count, __popcll(m));
+        }
+        base = __shfl(base, lead, kWave);
+        if (flag) {
+            queue[base + __popcll(m & lanemask_lt())] = i;
+        }
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ X, const uint8_t* __restrict__ typ, const int32_t* __restrict__ nbrT, const int32_t* __restrict__ nnb, float fscale, float fsurf, int n, float4* __restrict__ F) {
-    k_force_body(X, typ, nbrT, nnb, fscale, fsurf, n, F, blockIdx.x, gridDim.x);
+
+// The forces; with queue_rest, also the junction nodes queued for the general move
+// pass (their moves need no result of k_move_easy, so that pass can start as soon as
+// the forces are done) and flagged in skip for k_move_easy, which leaves them alone
+// without reading their type (the general pass may be rewriting it).
+__device__ __forceinline__ void k_force_body(const float4* __restrict__ X, const uint8_t* __restrict__ typ, const int32_t* __restrict__ nbrT, const int32_t* __restrict__ nnb, float fscale, float fsurf, int n, float4* __restrict__ F, Ctl* __restrict__ ctl, int32_t* __restrict__ queue_rest, uint8_t* __restrict__ skip, int bid, int nblk) {
+    const int i = bid * kBlock + threadIdx.x;
+    bool junction = false;
+    if (i < n) {
+        F[i] = bar_force(X, typ, nbrT, nnb, fscale, fsurf, i, n);
+        if (queue_rest) {
+            junction = typ[i] == TN_JUNCTION;
+            skip[i] = junction ? 1 : 0;
+        }
+    }
+    if (queue_rest) {
+        wave_append(junction, i, &ctl->nmove_rest, queue_rest);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ X, const uint8_t* __restrict__ typ, const int32_t* __restrict__ nbrT, const int32_t* __restrict__ nnb, float fscale, float fsurf, int n, float4* __restrict__ F, Ctl* __restrict__ ctl, int32_t* __restrict__ queue_rest, uint8_t* __restrict__ skip) {
+    k_force_body(X, typ, nbrT, nnb, fscale, fsurf, n, F, ctl, queue_rest, skip, blockIdx.x, gridDim.x);
 }
 
 // ---- move (v2m_move, src/opencl/v2m_particle_body.cl:844-935) ----------------------
@@ -1080,46 +1126,29 @@ __device__ __forceinline__ void move_store(const MoveArgs& ma, const Ctl* __rest
     mv[i] = m;
 }
 
-// Pass 1 of the move, over every node in slot order: corners, settled nodes and
-// interior nodes whose step stays in their own region with no interface within
-// snap range are finished here, exactly as v2m_move finishes them. Every other node
-// (interface and junction nodes, interior nodes that cross or may snap) is queued,
-// untouched, for k_move. Most nodes are interior, and this pass needs a fraction
-// of the registers of the full trapped step.
-// append i to queue for the lanes with flag set (one atomic on *count per wave)
-__device__ __forceinline__ void wave_append(bool flag, int i, int32_t* count, int32_t* __restrict__ queue) {
-    const uint64_t m = __ballot(flag);
-    if (m) {
-        const int lead = __ffsll((long long)m) - 1;
-        int base = 0;
-        if ((int)(threadIdx.x & (kWave - 1)) == lead) {
-            base = atomicAdd(count, __popcll(m));
-        }
-        base = __shfl(base, lead, kWave);
-        if (flag) {
-            queue[base + __popcll(m & lanemask_lt())] = i;
-        }
-    }
-}
-
+// Pass 1 of the move, over every node in slot order but the junction nodes (queued
+// by k_force for the general pass): corners, settled nodes and interior nodes whose
+// step stays in their own region with no interface within snap range are finished
+// here, exactly as v2m_move finishes them. The interface nodes and the interior
+// nodes that cross or may snap are queued, untouched, for the lean k_move passes.
+// Most nodes are interior, and this pass needs a fraction of the registers of the
+// full trapped step.
 // kPlain: every grid of the launch is a label volume (gm == 0). The compiler then
 // drops the gray-scale and shape-field branches of the field queries, and the hot
 // kernels keep their register budget; other grids take the kPlain = false variants.
 template <bool kPlain>
-__device__ __forceinline__ void k_move_easy_body(Field f, MoveArgs ma, Ctl* __restrict__ ctl, int n, const float4* __restrict__ F, float4* __restrict__ X, float4* __restrict__ V, uint8_t* __restrict__ typ, uint32_t* __restrict__ part, const uint16_t* __restrict__ lab, float* __restrict__ mv, float* __restrict__ pw, int32_t* __restrict__ queue, int32_t* __restrict__ queue_int, int32_t* __restrict__ queue_rest, int bid, int nblk) {
+__device__ __forceinline__ void k_move_easy_body(Field f, MoveArgs ma, Ctl* __restrict__ ctl, int n, const float4* __restrict__ F, float4* __restrict__ X, float4* __restrict__ V, uint8_t* __restrict__ typ, uint32_t* __restrict__ part, const uint16_t* __restrict__ lab, float* __restrict__ mv, float* __restrict__ pw, int32_t* __restrict__ queue, int32_t* __restrict__ queue_int, const uint8_t* __restrict__ skip, int bid, int nblk) {
     if (kPlain) {
         __builtin_assume(f.gm == 0);
     }
     const int g = bid * kBlock + threadIdx.x;
     int i = 0;
-    bool surf = false, inner = false, hard = false;   // the three queues
-    if (g < n) {
+    bool surf = false, inner = false;   // the two lean queues
+    if (g < n && !skip[g]) {   // (junction nodes: queued by k_force for the general pass)
         i = g;
         const int ty0 = typ[i];
         if (ty0 == TN_INTERFACE) {
             surf = true;
-        } else if (ty0 == TN_JUNCTION) {
-            hard = true;
         } else if (ty0 == TN_CORNER) {
             if (ma.fire) {
                 V[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1175,12 +1204,11 @@ __device__ __forceinline__ void k_move_easy_body(Field f, MoveArgs ma, Ctl* __re
     // queue the rest, one atomic per wave and queue
     wave_append(surf, i, &ctl->nmove, queue);
     wave_append(inner, i, &ctl->nmove_int, queue_int);
-    wave_append(hard, i, &ctl->nmove_rest, queue_rest);
 }
 
 template <bool kPlain>
-__global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl* __restrict__ ctl, int n, const float4* __restrict__ F, float4* __restrict__ X, float4* __restrict__ V, uint8_t* __restrict__ typ, uint32_t* __restrict__ part, const uint16_t* __restrict__ lab, float* __restrict__ mv, float* __restrict__ pw, int32_t* __restrict__ queue, int32_t* __restrict__ queue_int, int32_t* __restrict__ queue_rest) {
-    k_move_easy_body<kPlain>(f, ma, ctl, n, F, X, V, typ, part, lab, mv, pw, queue, queue_int, queue_rest, blockIdx.x,
+__global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl* __restrict__ ctl, int n, const float4* __restrict__ F, float4* __restrict__ X, float4* __restrict__ V, uint8_t* __restrict__ typ, uint32_t* __restrict__ part, const uint16_t* __restrict__ lab, float* __restrict__ mv, float* __restrict__ pw, int32_t* __restrict__ queue, int32_t* __restrict__ queue_int, const uint8_t* __restrict__ skip) {
+    k_move_easy_body<kPlain>(f, ma, ctl, n, F, X, V, typ, part, lab, mv, pw, queue, queue_int, skip, blockIdx.x,
                              gridDim.x);
 }
 
@@ -1665,7 +1693,8 @@ __global__ void kg_iter_begin(const PlanIter* __restrict__ tab, const int32_t* _
 __global__ __launch_bounds__(kBlock) void kg_force(const PlanIter* __restrict__ tab, const int32_t* __restrict__ bplan,
                                                    const int32_t* __restrict__ bfirst) {
     TN_PLAN_BLOCK(kShNodes);
-    k_force_body(P.X, P.typ, P.nbrT, P.nnb, P.fscale, P.fsurf, P.n, P.F, bid_, nb_);
+    k_force_body(P.X, P.typ, P.nbrT, P.nnb, P.fscale, P.fsurf, P.n, P.F, P.ctl, P.voxel ? nullptr : P.mq2, P.flag, bid_,
+                 nb_);
 }
 
 template <bool kPlain>
@@ -1677,7 +1706,7 @@ __global__ __launch_bounds__(kBlock) void kg_move_easy(const PlanIter* __restric
         k_move_voxel_body(P.f, P.ma, P.ctl, P.n, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv, P.pw, bid_);
         return;
     }
-    k_move_easy_body<kPlain>(P.f, P.ma, P.ctl, P.n, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv, P.pw, P.mq, P.mqi, P.mq2,
+    k_move_easy_body<kPlain>(P.f, P.ma, P.ctl, P.n, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv, P.pw, P.mq, P.mqi, P.flag,
                              bid_, nb_);
 }
 
