@@ -390,10 +390,11 @@ struct tn_plan {
     // it runs on the null stream with no side streams, events or pinned memory
     bool noalloc = false;
     hipEvent_t ev_iter = nullptr;   // behind each iteration's control-block copy
-    // the general and the lean interior passes run on side streams, beside the lean
-    // interface pass: ev_easy starts them, ev_rest and ev_int join them back
+    // the general pass over the junction nodes runs on side stream 0 from the forces on
+    // (ev_force), beside k_move_easy and the lean interface pass; the lean interior pass
+    // runs on side stream 1 (ev_easy); ev_rest and ev_int join them back
     hipStream_t side[2] = { nullptr, nullptr };
-    hipEvent_t ev_easy = nullptr, ev_rest = nullptr, ev_int = nullptr;
+    hipEvent_t ev_force = nullptr, ev_easy = nullptr, ev_rest = nullptr, ev_int = nullptr;
     // false: the three passes run one after another on the plan's stream (many plans
     // at once keep the device busy by themselves; cross-stream waits of many plans
     // would queue behind each other on the few hardware queues)
@@ -423,7 +424,7 @@ struct tn_plan {
                 (void)hipStreamSynchronize(s);
             }
         }
-        for (hipEvent_t e : { ev_iter, ev_easy, ev_rest, ev_int }) {
+        for (hipEvent_t e : { ev_iter, ev_force, ev_easy, ev_rest, ev_int }) {
             if (e) {
                 (void)hipEventDestroy(e);
             }
@@ -542,7 +543,9 @@ struct tn_plan {
         const unsigned nb = blocks_for(n, tnk::kBlock);
         tnk::k_iter_begin<<<1, 64, 0, stream>>>(dCtl);
         hipEvent_t t = timer.begin();
-        tnk::k_force<<<nb, tnk::kBlock, 0, stream>>>(dX, dTyp, dNbrT, dNnb, prm.fscale, prm.fsurf, n, dF);
+        // (the junction nodes queued for the general pass, unless the move is the voxel trap's)
+        tnk::k_force<<<nb, tnk::kBlock, 0, stream>>>(dX, dTyp, dNbrT, dNnb, prm.fscale, prm.fsurf, n, dF, dCtl,
+                                                     prm.voxel_trap ? nullptr : dMoveQ2, dFlag);
         timer.end(KC_FORCE, t);
         t = timer.begin();
         tnk::MoveArgs ma{ prm.dt, prm.maxstep, prm.snap, fire, rest_lanes };
@@ -556,23 +559,28 @@ struct tn_plan {
         // (label volumes: the move kernels compiled without the other field modes)
         auto moves = [&](auto plain) {
             constexpr bool kP = decltype(plain)::value;
-            tnk::k_move_easy<kP><<<nb, tnk::kBlock, 0, stream>>>(f, ma, dCtl, n, dF, dX, dV, dTyp, dPart, dLab, dMv,
-                                                             dPw, dMoveQ, dMoveQi, dMoveQ2);
-            timer.end(KC_MOVE_EASY, t);
-            // the queued nodes: three passes over disjoint queues at once, each bounded by
-            // its slowest node, then the general pass over the nodes handed back
+            // The queued nodes: the junction nodes' general pass starts on a side stream as
+            // soon as the forces are done, beside k_move_easy; then the two lean passes at
+            // once over the queues k_move_easy fills, each bounded by its slowest node, then
+            // the general pass over the nodes they hand back.
             const unsigned g1 = std::min<unsigned>(nb, tnk::kMoveGrid), g2 = std::min<unsigned>(nb, tnk::kMoveRestGrid);
             const hipStream_t s0 = overlap_moves ? side[0] : stream, s1 = overlap_moves ? side[1] : stream;
             if (overlap_moves) {
-                TN_HIP(hipEventRecord(ev_easy, stream));
-                TN_HIP(hipStreamWaitEvent(s0, ev_easy, 0));
+                TN_HIP(hipEventRecord(ev_force, stream));
+                TN_HIP(hipStreamWaitEvent(s0, ev_force, 0));
             }
-            t = timer.begin(s0);
+            hipEvent_t tr = timer.begin(s0);
             tnk::k_move<tnd::kGeneral, kP><<<g2, tnk::kBlock, 0, s0>>>(f, ma, dCtl, dMoveQ2, dF, dX, dV, dTyp, dPart, dLab, dMv,
                                                                    dPw, &dCtl->nmove_rest, nullptr, nullptr);
-            timer.end(KC_MOVE_REST, t, s0);
+            timer.end(KC_MOVE_REST, tr, s0);
             if (overlap_moves) {
                 TN_HIP(hipEventRecord(ev_rest, s0));
+            }
+            tnk::k_move_easy<kP><<<nb, tnk::kBlock, 0, stream>>>(f, ma, dCtl, n, dF, dX, dV, dTyp, dPart, dLab, dMv,
+                                                             dPw, dMoveQ, dMoveQi, dFlag);
+            timer.end(KC_MOVE_EASY, t);
+            if (overlap_moves) {
+                TN_HIP(hipEventRecord(ev_easy, stream));
                 TN_HIP(hipStreamWaitEvent(s1, ev_easy, 0));
             }
             t = timer.begin(s1);
@@ -784,7 +792,7 @@ tn_plan* plan_build(const tn_relax_params* prm, const tn_nodes_view* seeds, int 
             for (hipStream_t& s : p->side) {
                 TN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
             }
-            for (hipEvent_t* e : { &p->ev_easy, &p->ev_rest, &p->ev_int }) {
+            for (hipEvent_t* e : { &p->ev_force, &p->ev_easy, &p->ev_rest, &p->ev_int }) {
                 TN_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
             }
         }
@@ -2123,7 +2131,8 @@ int tn_plan_force(tn_plan* p, float* F) {
             return;
         }
         tnk::k_force<<<blocks_for(n, tnk::kBlock), tnk::kBlock, 0, p->stream>>>(p->dX, p->dTyp, p->dNbrT, p->dNnb,
-                                                                               p->prm.fscale, p->prm.fsurf, n, p->dF);
+                                                                               p->prm.fscale, p->prm.fsurf, n, p->dF,
+                                                                               p->dCtl, nullptr, nullptr);
         TN_HIP(hipGetLastError());
         TN_HIP(hipStreamSynchronize(p->stream));
         std::vector<float4> Fs(n);
