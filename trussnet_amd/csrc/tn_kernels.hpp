@@ -1133,6 +1133,39 @@ __device__ __forceinline__ void move_store(const MoveArgs& ma, const Ctl* __rest
 // nodes that cross or may snap are queued, untouched, for the lean k_move passes.
 // Most nodes are interior, and this pass needs a fraction of the registers of the
 // full trapped step.
+// Append i to queue a (lanes with fa set) and to queue b (fb), one atomic per block
+// and queue: the 2 M-node pass would otherwise send tens of thousands of atomics to
+// the same two counters. Every thread of the block must arrive here.
+__device__ __forceinline__ void block_append2(bool fa, bool fb, int i, int32_t* ca, int32_t* __restrict__ qa,
+                                              int32_t* cb, int32_t* __restrict__ qb) {
+    constexpr int kW = kBlock / kWave;
+    __shared__ int cnt[2][kW + 1];
+    const uint64_t ma = __ballot(fa), mb = __ballot(fb);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
+    if (lane == 0) {
+        cnt[0][w] = __popcll(ma);
+        cnt[1][w] = __popcll(mb);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {   // the block's offsets of its waves, then one atomic per queue
+        int tot = 0;
+        for (int k = 0; k < kW; ++k) {
+            const int c = cnt[threadIdx.x][k];
+            cnt[threadIdx.x][k] = tot;
+            tot += c;
+        }
+        cnt[threadIdx.x][kW] = tot ? atomicAdd(threadIdx.x == 0 ? ca : cb, tot) : 0;
+    }
+    __syncthreads();
+    const uint64_t below = lanemask_lt();
+    if (fa) {
+        qa[cnt[0][kW] + cnt[0][w] + __popcll(ma & below)] = i;
+    }
+    if (fb) {
+        qb[cnt[1][kW] + cnt[1][w] + __popcll(mb & below)] = i;
+    }
+}
+
 // kPlain: every grid of the launch is a label volume (gm == 0). The compiler then
 // drops the gray-scale and shape-field branches of the field queries, and the hot
 // kernels keep their register budget; other grids take the kPlain = false variants.
@@ -1201,9 +1234,8 @@ __device__ __forceinline__ void k_move_easy_body(Field f, MoveArgs ma, Ctl* __re
             }
         }
     }
-    // queue the rest, one atomic per wave and queue
-    wave_append(surf, i, &ctl->nmove, queue);
-    wave_append(inner, i, &ctl->nmove_int, queue_int);
+    // queue the rest, one atomic per block and queue
+    block_append2(surf, inner, i, &ctl->nmove, queue, &ctl->nmove_int, queue_int);
 }
 
 template <bool kPlain>
