@@ -188,18 +188,75 @@ __device__ __forceinline__ void corners_in_brick(const Field& f, int l, int s, i
 }
 
 // v2m_phi_cell's general case (src/opencl/v2m_grid_body.cl:182-185): a cell across
-// bricks or at the grid's edge, corner by corner
+// bricks or at the grid's edge, corner by corner as v2m_phi_vox reads each. The
+// corners lie in at most eight bricks, mostly two or four; each distinct brick's
+// label list is read once (for both labels of a pair).
 struct Corners {
     float v[8];
 };
 
+struct PairCorners {
+    float a[8], b[8];
+};
+
+template <bool kPair>
+__device__ __forceinline__ void corners_split(const Field& f, int la, int lb, int i0, int j0, int k0, float* ca,
+                                              float* cb) {
+    // the slots of the last two distinct bricks met (a cell across one brick face
+    // alternates between two)
+    int b0 = -1, b1 = -1, sa0 = -1, sb0 = -1, ha0 = 0, hb0 = 0, sa1 = -1, sb1 = -1, ha1 = 0, hb1 = 0;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int i = i0 + (m & 1), j = j0 + ((m >> 1) & 1), k = k0 + (m >> 2);
+        if (i < 0 || j < 0 || k < 0 || i >= f.nx || j >= f.ny || k >= f.nz) {   // outside the grid: label 0
+            ca[m] = la == 0 ? 1.0f : 0.0f;
+            if (kPair) {
+                cb[m] = lb == 0 ? 1.0f : 0.0f;
+            }
+            continue;
+        }
+        const int bk = brick_at(f, i, j, k);
+        if (bk != b0) {
+            if (bk == b1) {   // swap the two entries
+                int t = b0; b0 = b1; b1 = t;
+                t = sa0; sa0 = sa1; sa1 = t;
+                t = sb0; sb0 = sb1; sb1 = t;
+                t = ha0; ha0 = ha1; ha1 = t;
+                t = hb0; hb0 = hb1; hb1 = t;
+            } else {
+                b1 = b0;
+                sa1 = sa0;
+                sb1 = sb0;
+                ha1 = ha0;
+                hb1 = hb0;
+                const BrickList bl = brick_list(f, bk);
+                b0 = bk;
+                sa0 = list_slot(bl, la, &ha0);
+                if (kPair) {
+                    sb0 = list_slot(bl, lb, &hb0);
+                }
+            }
+        }
+        const size_t off = (size_t)((i & 7) + TN_BS * ((j & 7) + TN_BS * (k & 7)));
+        const int lv = (sa0 < 0 || (kPair && sb0 < 0)) ? (int)f.L[vox(f, i, j, k)] : -1;
+        ca[m] = sa0 >= 0 ? f.phi[(size_t)sa0 * TN_SLOT + off] : ((ha0 && lv == la) ? 1.0f : 0.0f);
+        if (kPair) {
+            cb[m] = sb0 >= 0 ? f.phi[(size_t)sb0 * TN_SLOT + off] : ((hb0 && lv == lb) ? 1.0f : 0.0f);
+        }
+    }
+}
+
 __device__ __noinline__ Corners phi_corners_split(const Field* fp, int l, int i0, int j0, int k0) {
     const Field f = *fp;
     Corners c;
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-        c.v[m] = phi_voxel(f, l, i0 + (m & 1), j0 + ((m >> 1) & 1), k0 + ((m >> 2) & 1));
-    }
+    corners_split<false>(f, l, -1, i0, j0, k0, c.v, nullptr);
+    return c;
+}
+
+__device__ __noinline__ PairCorners pair_corners_split(const Field* fp, int a, int b, int i0, int j0, int k0) {
+    const Field f = *fp;
+    PairCorners c;
+    corners_split<true>(f, a, b, i0, j0, k0, c.a, c.b);
     return c;
 }
 
@@ -256,12 +313,11 @@ __device__ __forceinline__ void psi_cell_load(const Field& f, int a, int b, int 
         corners_in_brick(f, a, pc.sa, pc.ha, i0, j0, k0, c);
         corners_in_brick(f, b, pc.sb, pc.hb, i0, j0, k0, e);
     } else {
-        const Corners ca = phi_corners_split(f.self, a, i0, j0, k0);
-        const Corners cb = phi_corners_split(f.self, b, i0, j0, k0);
+        const PairCorners pc2 = pair_corners_split(f.self, a, b, i0, j0, k0);
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
-            c[m] = ca.v[m];
-            e[m] = cb.v[m];
+            c[m] = pc2.a[m];
+            e[m] = pc2.b[m];
         }
     }
 #pragma unroll
