@@ -112,7 +112,7 @@ def pmc_traffic(args) -> dict | None:
         return None
     out = tempfile.mkdtemp(prefix="tn_pmc_", dir="/tmp")
     child = [sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu", "--no-traffic",
-             "--iters", "10", "--workload", args.workload]
+             "--no-extras", "--iters", "10", "--workload", args.workload]
     vals: dict = {}
     copy = {}
     for k, counter in enumerate(("FETCH_SIZE", "WRITE_SIZE")):
@@ -199,7 +199,7 @@ def cpu_baseline(grid, seeds, iters: int, voxel: bool = False) -> dict | None:
                       f"({st['rebuilds']} rebuilds), {dt:.1f} s"}
 
 
-def bench_batch(args) -> int:
+def bench_batch(args, emit: bool = True) -> dict:
     """Many small volumes: tn_relax_batch (all volumes' plans concurrent on their own
     streams) against a loop of tn_relax calls, one volume after another. A step is one
     relaxation of every volume, uploads and downloads included."""
@@ -229,7 +229,7 @@ def bench_batch(args) -> int:
     log(f"batch {value:.4g} node-iterations/s; loop of tn_relax {total * iters / dt_loop:.4g}; identical {same}")
     cpu = None
     if not args.no_cpu:   # the reference on the first four volumes
-        cpu = cpu_baseline_many(grids[:4], seeds[:4], args.cpu_iters)
+        cpu = cpu_baseline_many(grids[:4], seeds[:4], args.cpu_iters or iters)
     line = {
         "metric": "node-iterations/sec, 64 volumes relaxed together (batched)", "value": value,
         "unit": "node-iterations/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -240,11 +240,12 @@ def bench_batch(args) -> int:
                    "nodes": total, "volumes": 64, "iterations": iters, "parallelism": "single GPU"},
         "loop_value": total * iters / dt_loop, "identical_to_loop": same, "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
-    return 0
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
 
 
-def bench_adjoint(args) -> int:
+def bench_adjoint(args, emit: bool = True) -> dict:
     """The differentiable relaxation: forward (iters Jacobi steps of the smoothed
     truss, then the energy + quality loss) and the hand-written adjoint (gradients
     with respect to the start positions and a 32^3 sizing grid, checkpoints every 8
@@ -300,9 +301,8 @@ def bench_adjoint(args) -> int:
         pb1.backward_ptr(td.data_ptr(), gx1.data_ptr(), gt1.data_ptr())
     torch.cuda.synchronize()
     t_bwd1 = (time.perf_counter() - t) / args.steps
-    # the position gradient has no atomics (identical); the grid gradient's double
-    # atomics add in any order
-    same = bool(torch.equal(gx1, gx)) and float((gt1 - gt).abs().max()) <= 1e-6 * float(gt.abs().max())
+    # both gradients are deterministic (the grid's is summed in fixed point)
+    same = bool(torch.equal(gx1, gx)) and bool(torch.equal(gt1, gt))
     log(f"every state kept: forward {1e3 * t_fwd1:.2f} ms, backward {1e3 * t_bwd1:.2f} ms (x{t_bwd1 / t_fwd1:.2f}), "
         f"same gradient as the checkpointed run: {same}")
     pb1.close()
@@ -315,20 +315,32 @@ def bench_adjoint(args) -> int:
         t_cpu = time.perf_counter() - t
         g1, g2 = gx.cpu().numpy().astype(np.float64), gt.cpu().numpy().astype(np.float64)
         rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
-        rng = np.random.default_rng(0)
+        # central differences of the FP64 forward, component by component: the sizing
+        # grid's two largest gradient entries at eps = 1e-3 h, and the start positions'
+        # largest entry at 1e-3 h and 1e-4 h (the 64-step trajectory bends sharply in the
+        # positions: there 1e-3 h is too coarse a step for a central difference)
+        hbar = float(np.mean(theta))
+
+        def central(ex, et, eps_h):
+            eps = eps_h * hbar
+            lp = orc.adj_loss(pc, x0 + eps * ex, theta + eps * et)[0]
+            lm = orc.adj_loss(pc, x0 - eps * ex, theta - eps * et)[0]
+            return (lp - lm) / (2 * eps)
+
         fd = []
-        dx, dt = rng.standard_normal(x0.shape), rng.standard_normal(theta.shape)
-        for eps_h in (1e-3, 1e-6):
-            # central differences of the FP64 forward along one random direction; at
-            # eps = 1e-3 h the smoothed quality's curvature dominates their error
-            eps = eps_h * float(np.mean(theta))
-            lp = orc.adj_loss(pc, x0 + eps * dx, theta + eps * dt)[0]
-            lm = orc.adj_loss(pc, x0 - eps * dx, theta - eps * dt)[0]
-            want = (lp - lm) / (2 * eps)
-            got = float((g1 * dx).sum() + (g2 * dt).sum())
-            got64 = float((gx64 * dx).sum() + (gt64 * dt).sum())
-            fd.append({"eps_over_h": eps_h, "fd": want, "adjoint": got, "rel": abs(got - want) / abs(want),
-                       "fp64_adjoint_rel": abs(got64 - want) / abs(want)})
+        for c in np.argsort(-np.abs(g2.ravel()))[:2]:
+            e = np.zeros(theta.size)
+            e[c] = 1.0
+            want = central(0.0 * x0, e.reshape(theta.shape), 1e-3)
+            fd.append({"component": f"theta[{int(c)}]", "eps_over_h": 1e-3, "fd": want, "adjoint": float(g2.ravel()[c]),
+                       "rel": abs(float(g2.ravel()[c]) - want) / abs(want)})
+        i, a_ = np.unravel_index(int(np.argmax(np.abs(g1))), g1.shape)
+        e = np.zeros(x0.shape)
+        e[i, a_] = 1.0
+        for eps_h in (1e-3, 1e-4):
+            want = central(e, 0.0 * theta, eps_h)
+            fd.append({"component": f"x0[{int(i)},{int(a_)}]", "eps_over_h": eps_h, "fd": want,
+                       "adjoint": float(g1[i, a_]), "rel": abs(float(g1[i, a_]) - want) / abs(want)})
         check = {"loss_fp64": L64, "loss_rel": abs(loss - L64) / abs(L64), "grad_x_rel": rel(g1, gx64),
                  "grad_theta_rel": rel(g2, gt64), "fd": fd, "cpu_grad_s": t_cpu}
         log(f"check: {check}")
@@ -338,7 +350,7 @@ def bench_adjoint(args) -> int:
         "warmup": args.warmup, "ms_per_step": 1e3 * (t_fwd + t_bwd), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "adjoint_volume: 128^3 three nested perturbed ellipsoid labels, h 1.5, device seeding; "
-                               "64 smoothed Jacobi steps, checkpoints every 8, 32^3 trilinear sizing grid, "
+                               "64 smoothed Jacobi steps, checkpoints every 8, 32^3 sizing grid (trilinear, smoothstep fractions), "
                                "loss = mean truss energy + mean (1 - smoothed Joe-Liu) over a Delaunay of the seeds",
                    "nodes": n, "tets": int(pb.tets.shape[0]), "iterations": iters, "parallelism": "single GPU"},
         "forward_ms": 1e3 * t_fwd, "backward_ms": 1e3 * t_bwd, "adjoint_cost_ratio": t_bwd / t_fwd,
@@ -346,9 +358,10 @@ def bench_adjoint(args) -> int:
                              "adjoint_cost_ratio": t_bwd1 / t_fwd1, "identical_gradient": same},
         "loss": loss, "check": check,
     }
-    print(json.dumps(line), flush=True)
+    if emit:
+        print(json.dumps(line), flush=True)
     pb.close()
-    return 0
+    return line
 
 
 def bench_grid(args) -> int:
@@ -411,7 +424,8 @@ def bench_pipeline(args) -> int:
     256^3 volume (hbase 4.5, hmin 1.5, hmax 6 voxels, the reference's defaults
     otherwise), tn_seed_from_grid, tn_plan_create_from_grid, 100 iterations, the
     nodes downloaded. A step is the whole chain; the reference's build_grid_cpu,
-    seed_cpu and relax_cpu (20 iterations, scaled to 100) time the same chain."""
+    seed_cpu and relax_cpu (the same 100 iterations unless --cpu-iters says otherwise,
+    then scaled) time the same chain."""
     from trussnet_amd import gridbuild, volumes
     from trussnet_amd._abi import GridParams, RelaxParams
 
@@ -453,7 +467,7 @@ def bench_pipeline(args) -> int:
             t1 = time.perf_counter()
             sd = r.seed(RelaxParams())
             t2 = time.perf_counter()
-            _, stc = r.relax(sd, RelaxParams(max_iters=args.cpu_iters, dptol=0.0))
+            _, stc = r.relax(sd, RelaxParams(max_iters=args.cpu_iters or iters, dptol=0.0))
             t3 = time.perf_counter()
             total = (t1 - t0) + (t2 - t1) + (t3 - t2) * iters / max(1, stc["iters"])
             cores = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 64))
@@ -508,9 +522,11 @@ def main() -> int:
     ap.add_argument("--workload", choices=["large", "small", "prob", "voxel", "gray", "batch", "adjoint", "grid",
                                            "pipeline"], default="large")
     ap.add_argument("--iters", type=int, default=0, help="iterations per step (default: the workload's)")
-    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--cpu-iters", type=int, default=0, help="iterations of the CPU baseline (default: the GPU's)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 PMC passes")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="large workload: skip the batched and adjoint figures added to its line")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -519,9 +535,11 @@ def main() -> int:
         return 2
 
     if args.workload == "batch":
-        return bench_batch(args)
+        bench_batch(args)
+        return 0
     if args.workload == "adjoint":
-        return bench_adjoint(args)
+        bench_adjoint(args)
+        return 0
     if args.workload == "grid":
         return bench_grid(args)
     if args.workload == "pipeline":
@@ -535,8 +553,10 @@ def main() -> int:
     iters = args.iters or (50 if args.workload == "small" else 100)
     voxel = args.workload == "voxel"
     prm = RelaxParams(max_iters=iters, dptol=0.0, voxel_trap=voxel)
-    plan = relax.Plan(grid, seeds, prm, device=0)
-    log(f"plan: {seeds.n} nodes, hash {plan.hash.nlev} levels / {plan.hash.nkeys} cells")
+    t = time.perf_counter()
+    plan = relax.Plan(grid, seeds, prm, device=0)   # grid and seeds up, first rebuild
+    plan_ms = 1e3 * (time.perf_counter() - t)
+    log(f"plan: {seeds.n} nodes, hash {plan.hash.nlev} levels / {plan.hash.nkeys} cells ({plan_ms:.1f} ms)")
     for w in range(args.warmup):
         st = plan.run()
         log(f"warmup {w}: {st['iters']} iterations, {st['rebuilds']} rebuilds")
@@ -548,6 +568,9 @@ def main() -> int:
     timings = plan.timings()
     if st["iters"] != iters:
         raise RuntimeError(f"a step ran {st['iters']} iterations, not {iters}")
+    t = time.perf_counter()
+    plan.download()   # the relaxed nodes back to the host
+    d2h_ms = 1e3 * (time.perf_counter() - t)
     rb = plan.rebuild()
     kbar = float(rb["nnb"].mean())
     n = seeds.n
@@ -587,7 +610,7 @@ def main() -> int:
                 "bytes_per_launch": nbytes, "avg_launch_us": avg_s * 1e6,
                 "items_per_launch": timings[dom][2] / cnt}
     loop_bytes = iters * n * (83 + 4 * kbar) + st["rebuilds"] * n * (179 + 4 * kbar)
-    cpu = None if args.no_cpu else cpu_baseline(grid, seeds, args.cpu_iters, voxel)
+    cpu = None if args.no_cpu else cpu_baseline(grid, seeds, args.cpu_iters or iters, voxel)
     line = {
         "metric": METRIC, "value": value, "unit": "node-iterations/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
@@ -599,9 +622,30 @@ def main() -> int:
         "hbm_frac_loop": loop_bytes / (dt / args.steps) / HBM_PEAK,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        # the host boundary, outside `value`: plan creation (grid and seeds up, the first
+        # rebuild) and the relaxed nodes down
+        "host_transfer": {"plan_create_ms": plan_ms, "download_ms": d2h_ms,
+                          "pcie_inclusive_value": n * iters / (dt / args.steps + 1e-3 * (plan_ms + d2h_ms))},
     }
-    print(json.dumps(line), flush=True)
     plan.close()
+    if args.workload == "large" and not args.no_extras:
+        # the two added patterns on the same GPU: 64 volumes relaxed together against a
+        # loop of single calls, and the adjoint's cost and accuracy
+        sub = argparse.Namespace(**vars(args))
+        sub.iters, sub.no_cpu, sub.steps, sub.warmup = 0, True, 1, 1
+        b = bench_batch(sub, emit=False)
+        line["batched"] = {"value": b["value"], "loop_value": b["loop_value"], "ratio": b["value"] / b["loop_value"],
+                           "identical_to_loop": b["identical_to_loop"], "ms_per_step": b["ms_per_step"],
+                           "workload": b["config"]["workload"], "nodes": b["config"]["nodes"], "volumes": 64}
+        sub.steps, sub.no_cpu = 3, False
+        a = bench_adjoint(sub, emit=False)
+        ck = a.get("check") or {}
+        line["adjoint"] = {"forward_ms": a["forward_ms"], "backward_ms": a["backward_ms"],
+                           "adjoint_cost_ratio": a["adjoint_cost_ratio"], "nodes": a["config"]["nodes"],
+                           "iterations": a["config"]["iterations"], "checkpoint_every": 8,
+                           "grad_x_rel_fp64": ck.get("grad_x_rel"), "grad_theta_rel_fp64": ck.get("grad_theta_rel"),
+                           "fd": ck.get("fd"), "workload": a["config"]["workload"]}
+    print(json.dumps(line), flush=True)
     return 0
 
 

@@ -300,7 +300,10 @@ int tn_plan_destroy(tn_plan* plan);
  * over it (as tn_plan_create; the plan copies the grid device to device, so the
  * grid may be destroyed afterwards). No counterpart in the reference, whose
  * pipeline hands the Grid from build_grid_cpu to seed_cpu and relax_cl on the host
- * (src/v2m_pipeline.cpp:425-500).
+ * (src/v2m_pipeline.cpp:425-500). The count call keeps its nodes on the device, in
+ * the grid's handle, until the next count call or the grid's destruction; a
+ * tn_seed_from_grid with the same parameters (compared byte for byte) downloads
+ * those instead of seeding again. Calls on one grid from several threads race.
  */
 int tn_seed_count_from_grid(const tn_grid* g, const tn_relax_params* prm, int64_t* n);
 int tn_seed_from_grid(const tn_grid* g, const tn_relax_params* prm, int64_t n, float* P, uint16_t* lab, uint8_t* typ,
@@ -379,7 +382,8 @@ int tn_plan_field_points(tn_plan* plan, int32_t a, int32_t b, int64_t m, const f
  * projections and FIRE resets); for callers that fit a sizing field or seed
  * positions by gradient descent. The surrogate keeps the reference's bar force and
  * Jacobi step (src/opencl/v2m_particle_body.cl:161-184, 844-870) with a softplus
- * bar law, a smooth active-bar count and a trilinear sizing grid; interior nodes
+ * bar law, a smooth active-bar count and a sizing grid (trilinear in smoothstep cell
+ * fractions); interior nodes
  * move, the others stay; lists and tets are fixed. After `iters` steps the loss is
  * the mean truss energy plus wq x the mean (1 - smoothed Joe-Liu quality) of the
  * tets (src/v2m_tetra.cpp:100-120). The adjoint recomputes the steps from

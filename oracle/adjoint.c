@@ -3,7 +3,8 @@
  * in double precision on the CPU, forward and reverse: the FP64 gradient the
  * device adjoint is checked against. TEST INFRASTRUCTURE ONLY.
  *
- * The model (no counterpart in the reference): sizes trilinear in a grid theta
+ * The model (no counterpart in the reference): sizes trilinear in a grid theta, the
+ * cell fractions through smoothstep f^2 (3 - 2 f) (C1 across cell faces)
  * (positions clamped to the grid), bars i -> j of the fixed lists with
  * l0 = c (h_i + h_j) / 2 (c = fsurf if neither end is interior, else fscale),
  * u = l0 - l, F_i = sum sp(u) e / l with sp(u) = log(1 + exp(beta u)) / beta,
@@ -44,7 +45,10 @@ static void tri_at(const tn_adj_problem* p, const double* x, Tri* t) {
             i = g[a] - 2;
         }
         t->i0[a] = i;
-        t->f[a] = u - i;
+        /* the cell fraction through smoothstep: the interpolant is C1 across cell faces */
+        const double f = u - i;
+        t->f[a] = f * f * (3.0 - 2.0 * f);
+        t->gs[a] *= 6.0 * f * (1.0 - f);
     }
 }
 

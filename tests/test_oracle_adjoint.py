@@ -1,9 +1,10 @@
 """The FP64 adjoint of the differentiable relaxation (oracle/adjoint.c) pinned by
 central finite differences of its own double-precision forward, on a small block
 of nodes: directional derivatives along random directions in the positions and
-in the sizing grid, and single components. The step is 1e-6: the smoothed quality
-of near-flat tets is strongly curved, and the central difference's O(eps^2)
-error is 1e-4 relative at eps = 1e-4 but 1e-8 at 1e-6."""
+in the sizing grid (step 1e-6), and single components (step 1e-4: the smallest
+components, ~3e-5, lose 1e-5 of their digits to cancellation at 1e-6, while the
+central difference's O(eps^2) error is ~1e-8 relative at 1e-4). The quality term
+is pinned by closed forms of the Joe-Liu quality."""
 import numpy as np
 import pytest
 
@@ -57,9 +58,46 @@ def test_single_components(case):
         for a in range(3):
             e = np.zeros_like(x0)
             e[i, a] = 1.0
-            assert gx[i, a] == pytest.approx(fd(orc, pc, x0, th, e, 0 * th, 1e-6), rel=1e-5, abs=1e-10)
+            assert gx[i, a] == pytest.approx(fd(orc, pc, x0, th, e, 0 * th, 1e-4), rel=1e-5, abs=1e-10)
     big = np.argsort(-np.abs(gt.ravel()))[:3]
     for c in big:
         e = np.zeros(th.size)
         e[c] = 1.0
-        assert gt.ravel()[c] == pytest.approx(fd(orc, pc, x0, th, 0 * x0, e.reshape(th.shape), 1e-6), rel=1e-5)
+        assert gt.ravel()[c] == pytest.approx(fd(orc, pc, x0, th, 0 * x0, e.reshape(th.shape), 1e-4), rel=1e-5)
+
+
+def quality_loss(orc, pts, tets, eps_v=1e-9):
+    """The loss of nodes pts with no bars and no steps: wq * mean (1 - q) over tets."""
+    n = pts.shape[0]
+    nbr = np.full((n, 24), -1, np.int32)
+    nnb = np.zeros(n, np.int32)
+    typ = np.ones(n, np.uint8)   # fixed
+    theta = np.ones((2, 2, 2), np.float32)
+    prm = AdjParams(iters=0, ckpt_every=1, wq=1.0, eps_v=eps_v)
+    pc, keep = problem_struct(nbr, nnb, typ, np.asarray(tets, np.int32), theta.shape, (0.0, 0.0, 0.0),
+                              (100.0, 100.0, 100.0), prm)
+    L, _, terms = orc.adj_loss(pc, np.asarray(pts, np.float64), theta.astype(np.float64))
+    return L, terms
+
+
+def test_quality_term_closed_forms(oracle_lib):
+    """The quality term is Joe-Liu's 12 (3V)^(2/3) / sum l^2 (src/v2m_tetra.cpp:100-120)
+    as eps_v -> 0: 1 for a regular tet at any scale and orientation, 12 (1/2)^(2/3) / 9
+    for the unit corner tet, 0 for a flat one, and each tet counted once in the mean."""
+    orc = oracle_lib
+    reg = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], np.float64)
+    L, terms = quality_loss(orc, reg, [[0, 1, 2, 3]])
+    assert terms[1] == 0.0 and abs(terms[2]) < 1e-12   # q = 1
+    rng = np.random.default_rng(4)
+    q_, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    L2, t2 = quality_loss(orc, 3.7 * reg @ q_.T + np.array([5.0, -2.0, 9.0]), [[0, 1, 2, 3]])
+    assert abs(t2[2]) < 1e-12
+    corner = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float64)
+    _, t3 = quality_loss(orc, corner, [[0, 1, 2, 3]])
+    assert t3[2] == pytest.approx(1.0 - 12.0 * 0.5 ** (2.0 / 3.0) / 9.0, rel=1e-12)
+    flat = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], np.float64)
+    _, t4 = quality_loss(orc, flat, [[0, 1, 2, 3]])
+    assert t4[2] == pytest.approx(1.0, abs=1e-5)   # q -> 0 (eps_v = 1e-9)
+    both = np.concatenate([reg, corner + 10.0])
+    _, t5 = quality_loss(orc, both, [[0, 1, 2, 3], [4, 5, 6, 7]])
+    assert t5[2] == pytest.approx(0.5 * t3[2], rel=1e-12)

@@ -3,7 +3,8 @@
 No counterpart in the reference, whose relaxation is not differentiable (hinge
 bars, nearest-voxel sizes, projections, FIRE resets). The surrogate keeps its bar
 force and Jacobi step (src/opencl/v2m_particle_body.cl:161-184, 844-870) with a
-softplus bar law, a smooth active-bar count and a trilinear sizing grid theta;
+softplus bar law, a smooth active-bar count and a sizing grid theta interpolated
+trilinearly in smoothstep cell fractions (C1 across cell faces);
 interior nodes move, the others stay, the lists and tets are fixed. After `iters`
 steps the loss is the mean truss energy plus wq times the mean (1 - smoothed
 Joe-Liu quality) of the tets (src/v2m_tetra.cpp:100-120). The gradient with

@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-mllvm", "-amdgpu-mfma-vgpr-form=1", "-Wall", "-Wno-unused-function",
 ]
 SOURCES = ["tn_relax.hip"]
-DEPS = ["tn_relax.hip", "tn_kernels.hpp", "tn_device.hpp", "tn_seed.hpp", "tn_adjoint.hpp", "tn_grid.hpp", os.path.join("..", "..", "include", "trussnet_amd.h")]
+DEPS = ["tn_relax.hip", "tn_kernels.hpp", "tn_device.hpp", "tn_seed.hpp", "tn_thin.hpp", "tn_adjoint.hpp", "tn_grid.hpp", os.path.join("..", "..", "include", "trussnet_amd.h")]
 
 
 def hipcc() -> str:

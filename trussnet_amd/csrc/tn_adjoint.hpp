@@ -7,7 +7,8 @@
 // Jacobi step (src/opencl/v2m_particle_body.cl:844-870, fire off) and smooths what
 // is not differentiable:
 //
-//   h(x)       trilinear in a sizing grid theta (gx x gy x gz, origin o, spacing d),
+//   h(x)       trilinear in a sizing grid theta with smoothstep cell fractions (C1 across
+//              cell faces; gx x gy x gz, origin o, spacing d),
 //              the position clamped to the grid
 //   bar i->j   e = x_i - x_j, l = |e|, l0 = c_ij (h_i + h_j) / 2, u = l0 - l
 //              (c_ij = fsurf if neither end is interior, else fscale)
@@ -75,7 +76,11 @@ __device__ __forceinline__ void tri_axis(float x, float o, float d, int g, int& 
     if (i0 > g - 2) {
         i0 = g - 2;
     }
-    f = u - (float)i0;
+    // the cell fraction through smoothstep: the interpolant is C1 across cell faces, so
+    // the loss has no kinks where nodes cross them
+    const float t = u - (float)i0;
+    f = t * t * (3.0f - 2.0f * t);
+    gs *= 6.0f * t * (1.0f - t);
 }
 
 __device__ __forceinline__ Tri tri_at(const AdjArgs& A, float x, float y, float z) {
@@ -104,9 +109,15 @@ __device__ __forceinline__ float size_of(const AdjArgs& A, float x, float y, flo
     return h;
 }
 
+// The grid gradient is accumulated in fixed point, 2^-44 units in 64-bit integers:
+// integer additions give the same sum in any order, so the gradient is the same on
+// every run (double atomics would round in the order the waves arrive). A term is
+// rounded to the unit once; a cell's sum stays far inside the +-2^19 range.
+constexpr double kThetaScale = 17592186044416.0;   // 2^44
+
 // gtheta[idx] += v for every lane of the wave (all lanes call it): lanes of one
 // run of equal cells (neighbouring nodes share cells) add once
-__device__ __forceinline__ void theta_add(double* __restrict__ gtheta, int idx, double v) {
+__device__ __forceinline__ void theta_add(unsigned long long* __restrict__ gtheta, int idx, double v) {
     const int lane = threadIdx.x & (kWave - 1);
     const int prev = __shfl_up(idx, 1, kWave);
     const bool head = lane == 0 || prev != idx;
@@ -123,14 +134,14 @@ __device__ __forceinline__ void theta_add(double* __restrict__ gtheta, int idx, 
     }
     const int next = __shfl_down(seg, 1, kWave);
     if ((lane == kWave - 1 || next != seg) && v != 0.0) {
-        atomicAdd(&gtheta[idx], v);
+        atomicAdd(&gtheta[idx], (unsigned long long)__double2ll_rn(v * kThetaScale));
     }
 }
 
 // the adjoint hd of h at x: hd * dh/dx into g, hd * w_c into the grid gradient
 // (every lane of the wave calls it; hd = 0 for a lane with nothing to add)
 __device__ __forceinline__ void size_adjoint(const AdjArgs& A, float x, float y, float z, float hd, float3& g,
-                                             double* __restrict__ gtheta) {
+                                             unsigned long long* __restrict__ gtheta) {
     const Tri t = tri_at(A, x, y, z);
     float th[8];
 #pragma unroll
@@ -138,7 +149,7 @@ __device__ __forceinline__ void size_adjoint(const AdjArgs& A, float x, float y,
         th[c] = A.theta[tri_index(A, t, c)];
         theta_add(gtheta, tri_index(A, t, c), (double)(hd * tri_weight(t, c)));
     }
-    // d h / d f per axis (trilinear), times d f / d x
+    // d h / d f per axis (trilinear in the smoothed fractions), times d f / d x
     float hx = 0.0f, hy = 0.0f, hz = 0.0f;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
@@ -376,7 +387,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_loss_sum(AdjArgs A, const double
 // d L / d x_T and the grid gradient of the loss: the energy of the node's own bars
 // and of the bars pointing at it, and the quality of its tets
 __global__ __launch_bounds__(kBlock) void k_adj_loss_grad(AdjArgs A, const float4* __restrict__ X,
-                                                          float4* __restrict__ lam, double* __restrict__ gtheta) {
+                                                          float4* __restrict__ lam, unsigned long long* __restrict__ gtheta) {
     int p = blockIdx.x * kBlock + threadIdx.x;
     const bool valid = p < A.n;   // the others stand in for node n - 1: every lane joins the scatter
     if (!valid) {
@@ -492,9 +503,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_nu(AdjArgs A, const float4* __re
 }
 
 // lambda_k from lambda_{k+1} at the state X_k (nk: each node's (nu, kappa) at X_k)
+// With FDprev, also (nu, kappa) of each node for the step before (k - 1), from the
+// lambda_k just formed and that step's F, D: the next launch needs no separate pass.
 __global__ __launch_bounds__(kBlock) void k_adj_back(AdjArgs A, const float4* __restrict__ X,
                                                      const float4* __restrict__ nk, const float4* __restrict__ lamn,
-                                                     float4* __restrict__ lam, double* __restrict__ gtheta) {
+                                                     float4* __restrict__ lam, unsigned long long* __restrict__ gtheta,
+                                                     const float4* __restrict__ FDprev, float4* __restrict__ nk_prev) {
     int p = blockIdx.x * kBlock + threadIdx.x;
     const bool valid = p < A.n;   // the others stand in for node n - 1: every lane joins the scatter
     if (!valid) {
@@ -541,14 +555,19 @@ __global__ __launch_bounds__(kBlock) void k_adj_back(AdjArgs A, const float4* __
     }
     size_adjoint(A, bp.x + xp.x, bp.y + xp.y, bp.z + xp.z, valid ? hd : 0.0f, g, gtheta);
     if (valid) {
-        lam[p] = make_float4(g.x, g.y, g.z, 0.0f);
+        const float4 l = make_float4(g.x, g.y, g.z, 0.0f);
+        lam[p] = l;
+        if (FDprev) {
+            nk_prev[p] = nu_kappa(A, tp, l, FDprev[p]);
+        }
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_adj_theta_out(int m, const double* __restrict__ g, float* __restrict__ out) {
+__global__ __launch_bounds__(kBlock) void k_adj_theta_out(int m, const unsigned long long* __restrict__ g,
+                                                          float* __restrict__ out) {
     const int c = blockIdx.x * kBlock + threadIdx.x;
     if (c < m) {
-        out[c] = (float)g[c];
+        out[c] = (float)((double)(long long)g[c] * (1.0 / kThetaScale));
     }
 }
 
@@ -596,18 +615,34 @@ struct tn_adj {
     DevMem mem;
     tnadj::AdjArgs A{};
     float4* ckpts = nullptr;   // [nseg * n]: the state at the start of each segment
-    float4* seg = nullptr;     // [ckpt * n]: a segment's states, recomputed
-    float4* fd = nullptr;      // [ckpt * n]: F and D of those states
+    // two segment buffers, [(ckpt + 1) * n] states after a segment's first (which is its
+    // checkpoint) and [ckpt * n] F and D: the backward of one segment runs beside the
+    // recomputation of the one before it, on a second stream
+    float4* seg[2] = { nullptr, nullptr };
+    float4* fd[2] = { nullptr, nullptr };
+    hipStream_t side = nullptr;
+    hipEvent_t ev_start = nullptr, ev_ready[2] = { nullptr, nullptr }, ev_free[2] = { nullptr, nullptr };
     // ckpt_every == 1: every state is a checkpoint and the forward keeps F, D of each
     // ([iters * n]), so the backward recomputes nothing
     float4* fd_all = nullptr;
     float4* last = nullptr;    // the final state of the last forward
-    float4 *Xa = nullptr, *Xb = nullptr, *lamA = nullptr, *lamB = nullptr, *nk = nullptr, *base = nullptr;
-    double* gtheta = nullptr;
+    float4 *Xa = nullptr, *Xb = nullptr, *lamA = nullptr, *lamB = nullptr, *nk = nullptr, *nk2 = nullptr,
+           *base = nullptr;
+    unsigned long long* gtheta = nullptr;   // fixed point (kThetaScale)
     double2* parts = nullptr;
     double* dloss = nullptr;
     bool forward_done = false;
     double loss[3] = { 0, 0, 0 };
+    ~tn_adj() {
+        for (hipEvent_t e : { ev_start, ev_ready[0], ev_ready[1], ev_free[0], ev_free[1] }) {
+            if (e) {
+                (void)hipEventDestroy(e);
+            }
+        }
+        if (side) {
+            (void)hipStreamDestroy(side);
+        }
+    }
 };
 
 namespace {
@@ -726,19 +761,23 @@ int tn_adj_create(const tn_adj_problem* p, int32_t device, tn_adj** out) {
             A.vtet = entries;
         }
         a->ckpts = mem.alloc<float4>((size_t)std::max(a->nseg, 1) * n);
-        a->seg = mem.alloc<float4>((size_t)a->ckpt * n);
-        a->fd = mem.alloc<float4>((size_t)a->ckpt * n);
         if (a->ckpt == 1 && a->iters > 0) {
             a->fd_all = mem.alloc<float4>((size_t)a->iters * n);
+        } else if (a->iters > 0) {
+            for (int b = 0; b < (a->nseg > 1 ? 2 : 1); ++b) {
+                a->seg[b] = mem.alloc<float4>((size_t)(a->ckpt + 1) * n);
+                a->fd[b] = mem.alloc<float4>((size_t)a->ckpt * n);
+            }
         }
         a->Xa = mem.alloc<float4>(n);
         a->Xb = mem.alloc<float4>(n);
         a->lamA = mem.alloc<float4>(n);
         a->lamB = mem.alloc<float4>(n);
         a->nk = mem.alloc<float4>(n);
+        a->nk2 = mem.alloc<float4>(n);
         a->base = mem.alloc<float4>(n);
         A.B = a->base;
-        a->gtheta = mem.alloc<double>(a->gcells);
+        a->gtheta = mem.alloc<unsigned long long>(a->gcells);
         a->nparts = (int)std::min<unsigned>(2048, blocks_for(std::max(n, a->ntet), tnk::kBlock));
         a->parts = mem.alloc<double2>(a->nparts);
         a->dloss = mem.alloc<double>(3);
@@ -793,7 +832,8 @@ int tn_adj_backward(tn_adj* a, const float* theta, float* grad_x0, float* grad_t
         const unsigned nb = blocks_for(n, tnk::kBlock);
         tnadj::AdjArgs A = a->A;
         A.theta = theta;
-        tnadj::k_fill_f64<<<blocks_for(a->gcells, tnk::kBlock), tnk::kBlock, 0, s>>>(a->gtheta, a->gcells);
+        tnadj::k_fill_f64<<<blocks_for(a->gcells, tnk::kBlock), tnk::kBlock, 0, s>>>(
+            reinterpret_cast<double*>(a->gtheta), a->gcells);   // (all-zero bits)
         // the final state: recompute from the last checkpoint
         const int T = a->iters;
         float4* lam = a->lamA;
@@ -803,31 +843,67 @@ int tn_adj_backward(tn_adj* a, const float* theta, float* grad_x0, float* grad_t
         }
         if (a->fd_all) {   // every state and its F, D kept by the forward
             tnadj::k_adj_loss_grad<<<nb, tnk::kBlock, 0, s>>>(A, a->last, lam, a->gtheta);
-            for (int k = T - 1; k >= 0; --k) {
-                tnadj::k_adj_nu<<<nb, tnk::kBlock, 0, s>>>(A, a->fd_all + (size_t)k * n, lam, a->nk);
-                tnadj::k_adj_back<<<nb, tnk::kBlock, 0, s>>>(A, a->ckpts + (size_t)k * n, a->nk, lam, lamo,
-                                                             a->gtheta);
+            float4 *nkc = a->nk, *nkn = a->nk2;
+            if (T > 0) {
+                tnadj::k_adj_nu<<<nb, tnk::kBlock, 0, s>>>(A, a->fd_all + (size_t)(T - 1) * n, lam, nkc);
+            }
+            for (int k = T - 1; k >= 0; --k) {   // each step also forms the next one's (nu, kappa)
+                tnadj::k_adj_back<<<nb, tnk::kBlock, 0, s>>>(A, a->ckpts + (size_t)k * n, nkc, lam, lamo, a->gtheta,
+                                                             k > 0 ? a->fd_all + (size_t)(k - 1) * n : nullptr, nkn);
                 std::swap(lam, lamo);
+                std::swap(nkc, nkn);
             }
         }
-        for (int sg = a->fd_all ? -1 : a->nseg - 1; sg >= 0; --sg) {
-            const int k0 = sg * a->ckpt, k1 = std::min(T, k0 + a->ckpt);   // states k0 .. k1-1 here
-            // recompute the segment: seg[k - k0] = X_k, fd[k - k0] = F, D at X_k
-            TN_HIP(hipMemcpyAsync(a->seg, a->ckpts + (size_t)sg * n, sizeof(float4) * n, hipMemcpyDeviceToDevice, s));
+        if (!a->fd_all && T > 0 && !a->side) {
+            TN_HIP(hipStreamCreateWithFlags(&a->side, hipStreamNonBlocking));
+            for (hipEvent_t* e : { &a->ev_start, &a->ev_ready[0], &a->ev_ready[1], &a->ev_free[0], &a->ev_free[1] }) {
+                TN_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+            }
+        }
+        // segment sg's state k (k0 <= k <= k1): its checkpoint, else buffer sg % 2
+        auto state = [&](int sg, int k) {
+            const int k0 = sg * a->ckpt;
+            return k == k0 ? a->ckpts + (size_t)sg * n : a->seg[sg & 1] + (size_t)(k - k0) * n;
+        };
+        // recompute segment sg on the side stream: its states and each one's F, D
+        bool used[2] = { false, false };
+        auto recompute = [&](int sg) {
+            const int b = sg & 1, k0 = sg * a->ckpt, k1 = std::min(T, k0 + a->ckpt);
+            if (used[b]) {   // the backward that read this buffer last is done with it
+                TN_HIP(hipStreamWaitEvent(a->side, a->ev_free[b], 0));
+            }
+            used[b] = true;
             for (int k = k0; k < k1; ++k) {
-                const int r = k - k0;
-                float4* nxt = (k + 1 < k1) ? a->seg + (size_t)(r + 1) * n : a->Xa;
-                tnadj::k_adj_step<<<nb, tnk::kBlock, 0, s>>>(A, a->seg + (size_t)r * n, nxt, a->fd + (size_t)r * n);
+                tnadj::k_adj_step<<<nb, tnk::kBlock, 0, a->side>>>(A, state(sg, k), state(sg, k + 1),
+                                                                   a->fd[b] + (size_t)(k - k0) * n);
             }
-            if (sg == a->nseg - 1) {   // X_T is in Xa: the loss gradient starts the adjoint
-                tnadj::k_adj_loss_grad<<<nb, tnk::kBlock, 0, s>>>(A, a->Xa, lam, a->gtheta);
+            TN_HIP(hipEventRecord(a->ev_ready[b], a->side));
+        };
+        if (!a->fd_all && T > 0) {
+            TN_HIP(hipEventRecord(a->ev_start, s));   // the side stream starts after the work queued before
+            TN_HIP(hipStreamWaitEvent(a->side, a->ev_start, 0));
+            recompute(a->nseg - 1);
+        }
+        for (int sg = (a->fd_all || T == 0) ? -1 : a->nseg - 1; sg >= 0; --sg) {
+            const int b = sg & 1, k0 = sg * a->ckpt, k1 = std::min(T, k0 + a->ckpt);   // states k0 .. k1-1 here
+            if (sg > 0) {   // the segment before, beside this one's backward
+                recompute(sg - 1);
             }
+            TN_HIP(hipStreamWaitEvent(s, a->ev_ready[b], 0));
+            if (sg == a->nseg - 1) {   // X_T: the loss gradient starts the adjoint
+                tnadj::k_adj_loss_grad<<<nb, tnk::kBlock, 0, s>>>(A, state(sg, k1), lam, a->gtheta);
+            }
+            float4 *nkc = a->nk, *nkn = a->nk2;
+            // the segment's last step; the steps before get theirs from k_adj_back
+            tnadj::k_adj_nu<<<nb, tnk::kBlock, 0, s>>>(A, a->fd[b] + (size_t)(k1 - 1 - k0) * n, lam, nkc);
             for (int k = k1 - 1; k >= k0; --k) {
                 const int r = k - k0;
-                tnadj::k_adj_nu<<<nb, tnk::kBlock, 0, s>>>(A, a->fd + (size_t)r * n, lam, a->nk);
-                tnadj::k_adj_back<<<nb, tnk::kBlock, 0, s>>>(A, a->seg + (size_t)r * n, a->nk, lam, lamo, a->gtheta);
+                tnadj::k_adj_back<<<nb, tnk::kBlock, 0, s>>>(A, state(sg, k), nkc, lam, lamo, a->gtheta,
+                                                             k > k0 ? a->fd[b] + (size_t)(r - 1) * n : nullptr, nkn);
                 std::swap(lam, lamo);
+                std::swap(nkc, nkn);
             }
+            TN_HIP(hipEventRecord(a->ev_free[b], s));
         }
         if (grad_x0) {
             tnadj::k_adj_unload<<<nb, tnk::kBlock, 0, s>>>(n, nullptr, lam, grad_x0);

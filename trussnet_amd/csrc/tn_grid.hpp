@@ -822,6 +822,14 @@ struct tn_grid {
     const float* gI = nullptr;      // gray-scale: the intensity on the device
     const float* dTW = nullptr;     // gray-scale: gTW on the device
     float hbig = 0.0f;              // the largest h
+    // the last tn_seed_count_from_grid's nodes, kept on the device for the
+    // tn_seed_from_grid that follows it with the same parameters
+    struct SeedCache {
+        DevMem mem;
+        tn_relax_params prm{};
+        Seeds s;
+    };
+    mutable std::unique_ptr<SeedCache> seeds;
 };
 
 namespace {
@@ -1286,11 +1294,33 @@ int tn_grid_destroy(tn_grid* g) {
 
 int tn_seed_count_from_grid(const tn_grid* g, const tn_relax_params* prm, int64_t* n) {
     return guarded([&] {
-        arg_check(g != nullptr && n != nullptr, "null argument");
+        arg_check(g != nullptr && n != nullptr && prm != nullptr, "null argument");
         TN_HIP(hipSetDevice(g->mem.device));
+        g->seeds.reset();
         DevMem mem;
         mem.device = g->mem.device;
-        *n = seeding_core(g->f, g->grade, g->info.hmin, g->info.hmax, prm, mem).n;
+        const Seeds s = seeding_core(g->f, g->grade, g->info.hmin, g->info.hmax, prm, mem);
+        // keep the nodes (not the seeding's work arrays) for tn_seed_from_grid
+        auto c = std::make_unique<tn_grid::SeedCache>();
+        c->mem.device = g->mem.device;
+        c->prm = *prm;
+        c->s.n = s.n;
+        auto keep = [&](auto* src, size_t count) {
+            using T = std::remove_pointer_t<decltype(src)>;
+            T* d = c->mem.alloc<T>(std::max<size_t>(count, 1));
+            if (count && src) {
+                TN_HIP(hipMemcpy(d, src, count * sizeof(T), hipMemcpyDeviceToDevice));
+            }
+            return src ? d : nullptr;
+        };
+        const size_t m = (size_t)s.n;
+        c->s.P = keep(s.P, 3 * m);
+        c->s.lab = keep(s.lab, m);
+        c->s.typ = keep(s.typ, m);
+        c->s.part = keep(s.part, 2 * m);
+        c->s.part3 = keep(s.part3, m);
+        g->seeds = std::move(c);
+        *n = s.n;
     });
 }
 
@@ -1299,10 +1329,15 @@ int tn_seed_from_grid(const tn_grid* g, const tn_relax_params* prm, int64_t n, f
     return guarded([&] {
         arg_check(g != nullptr, "null grid");
         arg_check(n >= 0 && (n == 0 || (P && lab && typ && part)), "seeding: null output array");
+        arg_check(prm != nullptr, "null parameters");
         TN_HIP(hipSetDevice(g->mem.device));
         DevMem mem;
         mem.device = g->mem.device;
-        const Seeds s = seeding_core(g->f, g->grade, g->info.hmin, g->info.hmax, prm, mem);
+        // the nodes of the count call with these parameters, else a fresh seeding
+        // (deterministic: the same nodes)
+        std::unique_ptr<tn_grid::SeedCache> cached = std::move(g->seeds);
+        const bool hit = cached && std::memcmp(&cached->prm, prm, sizeof *prm) == 0;
+        const Seeds s = hit ? cached->s : seeding_core(g->f, g->grade, g->info.hmin, g->info.hmax, prm, mem);
         arg_check(s.n == n, "seeding: the output holds " + std::to_string(n) + " nodes, the seeding made " +
                                 std::to_string(s.n) + " (tn_seed_count_from_grid)");
         if (s.n == 0) {

@@ -1248,6 +1248,11 @@ __global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl*
 #define TN_REST_LANES 2
 #endif
 constexpr int kRestLanes = TN_REST_LANES;   // nodes per wave in k_move<kGeneral>
+// a lean pass spreads up to this many nodes per wave before it fills whole waves
+#ifndef TN_LEAN_SPREAD
+#define TN_LEAN_SPREAD 4
+#endif
+constexpr int kLeanSpread = TN_LEAN_SPREAD;
 
 // waves per SIMD the full move kernel is compiled for (its register budget: 512 / waves)
 #ifndef TN_MOVE_WAVES
@@ -1339,8 +1344,8 @@ __device__ __forceinline__ void k_move_body(Field f, MoveArgs ma, Ctl* __restric
     // A fixed grid strides over the queue, whose length only the device knows. A
     // short queue's nodes follow long, divergent paths (projections onto interfaces
     // and junction curves), and a wave executes every path its lanes take, so they
-    // are spread thinly over the grid's waves (lean passes: from one node per wave;
-    // the general pass: from kRestLanes, measured best); a long queue fills whole
+    // are spread thinly over the grid's waves (lean passes: one to kLeanSpread nodes per
+    // wave; the general pass: from kRestLanes, measured best); a long queue fills whole
     // waves, whose lanes then share the issue slots.
     constexpr bool kLean = kMode != kGeneral;
     constexpr int kWaves = kBlock / kWave;
@@ -1349,7 +1354,7 @@ __device__ __forceinline__ void k_move_body(Field f, MoveArgs ma, Ctl* __restric
     const int spread = (count + waves - 1) / waves;
     int lpw;
     if (kLean) {
-        lpw = spread < 1 ? 1 : (spread > kRestLanes ? kWave : spread);
+        lpw = spread < 1 ? 1 : (spread > kLeanSpread ? kWave : spread);
     } else {
         const int rl = ma.lanes > 0 ? ma.lanes : kRestLanes;
         lpw = spread < rl ? rl : (spread > kWave ? kWave : spread);

@@ -1378,13 +1378,16 @@ void run_grouped(Plans& plans, int64_t& launches, int64_t& steps) {
         TN_HIP(hipMemcpyAsync(dmap, hmap, sizeof(int32_t) * o, hipMemcpyHostToDevice, bs));
     };
     build_maps();
-    // Launch-shape tuning: the general move passes' nodes per wave, tried in turn on
-    // the first grouped iterations (kTuneIters each, timed by events around the two
-    // general passes), the fastest kept. The nodes' results do not depend on it.
+    // Launch-shape tuning: the general move passes' nodes per wave. The candidates
+    // take turns (round robin) on grouped iterations 1 .. ncand * kTuneIters (the
+    // first iteration, cold, is skipped), each timed by events around the two general
+    // passes; the one with the least time per queued general-pass node (the plans'
+    // nmove_rest + nmove_def, which shrink as nodes settle) is kept. The nodes'
+    // results do not depend on it.
     static constexpr int kTuneLanes[] = { 1, 2, 4, 8, 16 };
     constexpr int kTuneIters = 2;
     const int ncand = (int)(sizeof kTuneLanes / sizeof kTuneLanes[0]);
-    std::vector<double> tune_ms(ncand, 0.0);
+    std::vector<double> tune_ms(ncand, 0.0), tune_nodes(ncand, 0.0);
     hipEvent_t tune_ev[4];
     for (hipEvent_t& e : tune_ev) {
         TN_HIP(hipEventCreate(&e));
@@ -1397,7 +1400,7 @@ void run_grouped(Plans& plans, int64_t& launches, int64_t& steps) {
             }
         }
     } events_gone{ tune_ev };
-    int tune_step = std::getenv("TN_NO_TUNE") ? ncand * kTuneIters : 0;
+    int tune_step = std::getenv("TN_NO_TUNE") ? ncand * kTuneIters : -1;   // -1: the untimed first iteration
     auto set_lanes = [&](int lanes) {
         for (int k = 0; k < count; ++k) {
             plans[k]->rest_lanes = lanes;
@@ -1410,7 +1413,7 @@ void run_grouped(Plans& plans, int64_t& launches, int64_t& steps) {
     }
     while (!active.empty()) {
         using namespace tnk;
-        const bool tuning = tune_step < ncand * kTuneIters;
+        const bool tuning = tune_step >= 0 && tune_step < ncand * kTuneIters;
         kg_iter_begin<<<nblocks[kShOne], 64, 0, bs>>>(dtab, bplan[kShOne], bfirst[kShOne]);
         kg_force<<<nblocks[kShNodes], kBlock, 0, bs>>>(dtab, bplan[kShNodes], bfirst[kShNodes]);
         auto moves = [&](auto plain) {
@@ -1451,23 +1454,32 @@ void run_grouped(Plans& plans, int64_t& launches, int64_t& steps) {
             float a = 0.0f, b = 0.0f;
             TN_HIP(hipEventElapsedTime(&a, tune_ev[0], tune_ev[1]));
             TN_HIP(hipEventElapsedTime(&b, tune_ev[2], tune_ev[3]));
-            tune_ms[tune_step / kTuneIters] += (double)a + (double)b;
+            double queued = 0.0;
+            for (const int k : active) {
+                queued += (double)hctl[k].nmove_rest + (double)hctl[k].nmove_def;
+            }
+            tune_ms[tune_step % ncand] += (double)a + (double)b;
+            tune_nodes[tune_step % ncand] += queued;
+        }
+        if (tune_step < ncand * kTuneIters) {
             ++tune_step;
-            if (tune_step == ncand * kTuneIters) {   // keep the fastest
+            if (tune_step == ncand * kTuneIters) {   // keep the least time per queued node
+                auto per_node = [&](int c) { return tune_ms[c] / (tune_nodes[c] > 1.0 ? tune_nodes[c] : 1.0); };
                 int best = 0;
                 for (int c = 1; c < ncand; ++c) {
-                    best = tune_ms[c] < tune_ms[best] ? c : best;
+                    best = per_node(c) < per_node(best) ? c : best;
                 }
                 set_lanes(kTuneLanes[best]);
                 if (std::getenv("TN_BATCH_VERBOSE")) {
-                    std::fprintf(stderr, "[tn_relax_batch] general pass lanes:");
+                    std::fprintf(stderr, "[tn_relax_batch] general pass lanes (us per queued node):");
                     for (int c = 0; c < ncand; ++c) {
-                        std::fprintf(stderr, " %d: %.3f ms", kTuneLanes[c], tune_ms[c] / kTuneIters);
+                        std::fprintf(stderr, " %d: %.3f (%.3f ms, %.0f nodes)", kTuneLanes[c], 1e3 * per_node(c),
+                                     tune_ms[c] / kTuneIters, tune_nodes[c] / kTuneIters);
                     }
                     std::fprintf(stderr, " -> %d\n", kTuneLanes[best]);
                 }
-            } else if (tune_step % kTuneIters == 0) {
-                set_lanes(kTuneLanes[tune_step / kTuneIters]);
+            } else {
+                set_lanes(kTuneLanes[tune_step % ncand]);
             }
         }
         bool changed = false;

@@ -59,6 +59,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"trussnet_amd library not built: {LIB_PATH} (python -m trussnet_amd.build)")
+        # torch's wheel bundles a HIP runtime of the same soname as the one the library
+        # links; loaded first, it serves both, where two runtimes in one process leave
+        # torch with no device
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp = C.c_void_p
         L.tn_last_error.restype = C.c_char_p
