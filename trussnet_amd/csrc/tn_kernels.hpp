@@ -335,20 +335,58 @@ constexpr int kTrussWave = 64;   // one wave per block
 constexpr int kCollect = 32;   // per-lane candidate buffer (entries): settled by a 32-key network
 constexpr int kGroupSpan = TN_TRUSS_SPAN;    // cells along a row that one group of lanes may span
 
+// ---- wave-wide reductions and scans on DPP (64 lanes, all active) ----------------
+//
+// Each step combines a lane with a lane of its row (16 lanes) through the data-
+// parallel-primitives operand modifier, then rows with row broadcasts (lane 15 of a
+// row into the next, lane 31 into the upper two), so the reduction never touches LDS.
+// dpp_ctrl: quad_perm [1,0,3,2] 0xB1, [2,3,0,1] 0x4E; row_shr:k 0x110 + k;
+// row_mirror 0x140, row_half_mirror 0x141; row_bcast:15 0x142, row_bcast:31 0x143.
+
+template <int kCtrl, int kRowMask = 0xF>
+__device__ __forceinline__ int dpp(int old, int v) {
+    return __builtin_amdgcn_update_dpp(old, v, kCtrl, kRowMask, 0xF, false);
+}
+
+// the reduction of v over the wave under op (commutative, idempotent), as a uniform value
+template <typename Op>
+__device__ __forceinline__ int wave_reduce(int v, Op op) {
+    v = op(v, dpp<0xB1>(v, v));
+    v = op(v, dpp<0x4E>(v, v));
+    v = op(v, dpp<0x141>(v, v));
+    v = op(v, dpp<0x140>(v, v));
+    v = op(v, dpp<0x142, 0xA>(v, v));
+    v = op(v, dpp<0x143, 0xC>(v, v));
+    return __builtin_amdgcn_readlane(v, kWave - 1);
+}
+
 __device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) {
-        v = min(v, __shfl_xor(v, d, kWave));
-    }
-    return v;
+    return wave_reduce(v, [](int a, int b) { return min(a, b); });
 }
 
 __device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) {
-        v = max(v, __shfl_xor(v, d, kWave));
-    }
-    return v;
+    return wave_reduce(v, [](int a, int b) { return max(a, b); });
+}
+
+// the largest of non-negative floats (their bits order as unsigned integers do)
+__device__ __forceinline__ float wave_max_nonneg(float v) {
+    return __uint_as_float((uint32_t)wave_reduce((int)__float_as_uint(v), [](int a, int b) {
+        return (int)max((uint32_t)a, (uint32_t)b);
+    }));
+}
+
+// exclusive prefix sum over the wave; *total: the sum of all lanes
+__device__ __forceinline__ int wave_excl_scan(int v, int lane, int* total) {
+    (void)lane;
+    int x = v;
+    x += dpp<0x111>(0, x);   // row_shr: a lane with no source adds 0
+    x += dpp<0x112>(0, x);
+    x += dpp<0x114>(0, x);
+    x += dpp<0x118>(0, x);
+    x += dpp<0x142, 0xA>(0, x);
+    x += dpp<0x143, 0xC>(0, x);
+    *total = __builtin_amdgcn_readlane(x, kWave - 1);
+    return x - v;
 }
 
 // (distance, id) order of the K-list (src/opencl/v2m_particle_body.cl:128-141)
@@ -417,13 +455,6 @@ __device__ __forceinline__ void settle_keys(const uint64_t* cb, int lane, int lo
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-__device__ __forceinline__ float wave_maxf(float v) {
-#pragma unroll
-    for (int d = kWave / 2; d > 0; d >>= 1) {
-        v = fmaxf(v, __shfl_xor(v, d, kWave));
-    }
-    return v;
-}
 
 // this lane's B operands: queries 32 c + l % 32 (c = the column tile), three K-steps
 struct QueryOps {
@@ -514,19 +545,6 @@ __device__ __forceinline__ uint64_t chunk_hits(const QueryOps& qo, const float4*
 // rows (and levels) are concatenated into full chunks.
 constexpr int kMaxRanges = 64;
 
-// exclusive prefix sum over the wave; *total: the sum of all lanes
-__device__ __forceinline__ int wave_excl_scan(int v, int lane, int* total) {
-    int x = v;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const int y = __shfl_up(x, d, kWave);
-        if (lane >= d) {
-            x += y;
-        }
-    }
-    *total = __shfl(x, kWave - 1, kWave);
-    return x - v;
-}
 
 // The lists of all nodes at a rebuild: query q is the node in slot q.
 __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n, int32_t* __restrict__ nbrT,
@@ -539,6 +557,9 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
     // are read: the row's first slot and its position in the concatenation
     __shared__ int r_key0[kMaxRanges], r_key1[kMaxRanges], r_lev[kMaxRanges], r_sa[kMaxRanges], r_beg[kMaxRanges],
         r_len[kMaxRanges];
+    // a round's (query lane, candidate) pairs; the row keys are read before any round
+    static_assert(kMaxRanges >= kTrussWave, "the pairs share the row keys' array");
+    int* const pairs = r_key0;
     __shared__ uint64_t r_mask[kMaxRanges];
     __shared__ uint64_t cb[kCollect * kTrussWave];   // the lanes' lists (list_key entries)
     const int lane = threadIdx.x;
@@ -589,7 +610,7 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
             const float qx = px - gox, qy = py - goy, qz = pz - goz;
             const float q2 = qx * qx + qy * qy + qz * qz;
             const QueryOps qops = query_ops(qx, qy, qz, hi, q2 - k2 * hi * hi, lane);
-            const float P2 = wave_maxf(ingroup ? q2 : 0.0f), HP = wave_maxf(ingroup ? hi : 0.0f);
+            const float P2 = wave_max_nonneg(ingroup ? q2 : 0.0f), HP = wave_max_nonneg(ingroup ? hi : 0.0f);
             int nr = 0;   // rows in the table (wave-uniform)
             // Read the cell starts of the table's rows, then test the concatenation of their
             // candidates chunk by chunk: the distance prefilter on the matrix cores, then the
@@ -611,9 +632,13 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                 }
                 const int rows = nr;
                 nr = 0;
-                int rfirst = 0;   // the row holding the chunk's first candidate
-                for (int base = 0; base < total; base += kTrussWave) {
-                    __syncthreads();   // the previous chunk is consumed; the row starts are written
+                __syncthreads();
+                // The candidates of chunk base + 64 are loaded while chunk base is tested.
+                int rfirst = 0;   // the row holding the loaded chunk's first candidate
+                float4 npos;
+                int2 nms;
+                int nrow;
+                auto fetch = [&](int base) {
                     while (rfirst + 1 < rows && r_beg[rfirst + 1] <= base) {
                         ++rfirst;
                     }
@@ -624,22 +649,32 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                             ++lo;
                         }
                         const int s = r_sa[lo] + (p - r_beg[lo]);
-                        cpos[lane] = in.Ps[s];
-                        cms[lane] = in.Ms[s];
-                        crow[lane] = lo;
+                        npos = in.Ps[s];
+                        nms = in.Ms[s];
                     } else {   // past the end: far away, it fails every test
-                        cpos[lane] = make_float4(inf, inf, inf, 0.0f);
-                        cms[lane] = make_int2(-1, 0);
-                        crow[lane] = 0;
+                        npos = make_float4(inf, inf, inf, 0.0f);
+                        nms = make_int2(-1, 0);
                     }
+                    nrow = lo;
+                };
+                fetch(0);
+                for (int base = 0; base < total; base += kTrussWave) {
+                    __syncthreads();   // the previous chunk is consumed
+                    cpos[lane] = npos;
+                    cms[lane] = nms;
+                    crow[lane] = nrow;
+                    const int lo = nrow;
                     __syncthreads();
+                    if (base + kTrussWave < total) {
+                        fetch(base + kTrussWave);
+                    }
                     const int cn = min(kTrussWave, total - base);
                     // the chunk's own bound for the rounding margin
                     const float4 mine_c = cpos[lane];
                     const bool cval = mine_c.x < inf;
                     const float ccx = mine_c.x - gox, ccy = mine_c.y - goy, ccz = mine_c.z - goz;
-                    const float C2 = wave_maxf(cval ? ccx * ccx + ccy * ccy + ccz * ccz : 0.0f);
-                    const float HC = wave_maxf(cval ? mine_c.w : 0.0f);
+                    const float C2 = wave_max_nonneg(cval ? ccx * ccx + ccy * ccy + ccz * ccz : 0.0f);
+                    const float HC = wave_max_nonneg(cval ? mine_c.w : 0.0f);
                     const float hs = HP + HC;
                     const float margin = (2.0f * (P2 + C2) + 2.0f * k2 * hs * hs) * (1.0f / 65536.0f);
                     uint64_t hit = chunk_hits(qops, cpos, gox, goy, goz, k2, margin, lane);
@@ -654,33 +689,13 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                             }
                         }
                     }
-                    if (!ingroup) {
-                        continue;
-                    }
-                    hit &= elig;
-                    while (hit) {   // possible hits: the exact test and the checks, then appended
-                        const int k = __builtin_ctzll(hit);
-                        hit &= hit - 1;
-                        const float4 c = cpos[k];
-                        const float ex = c.x - px, ey = c.y - py, ez = c.z - pz;
-                        const float d2 = ex * ex + ey * ey + ez * ez;
-                        const float rj = half * (hi + c.w);
-                        if (!(d2 < rj * rj)) {
-                            continue;
-                        }
-                        if (!H.prune) {   // the reference's own scan region only
-                            const int L = r_lev[crow[k]];
-                            const float rr = 0.5f * (Ri + H.b0 * (float)(1 << L));
-                            const int cx = cell_coord(H, L, c.x, H.ox, 0);
-                            if (cx < cell_coord(H, L, px - rr, H.ox, 0) || cx > cell_coord(H, L, px + rr, H.ox, 0)) {
-                                continue;
-                            }
-                        }
-                        const int2 mt = cms[k];   // not the node itself; no interior bar across labels
-                        if (mt.x == myid || (ti_int && (mt.y & 0xFF) == TN_INTERIOR && (mt.y >> 8) != li)) {
-                            continue;
-                        }
-                        if (cnt == kCollect) {   // keep the 24 best so far
+                    hit = ingroup ? hit & elig : 0ull;
+                    // The possible hits as (query lane, candidate) pairs, up to 64 a round, one
+                    // a lane: the exact test and the reference's other checks, and the valid
+                    // ones appended to their query's buffer, packed in pair order.
+                    while (__ballot(hit != 0ull)) {
+                        const int pc = __popcll(hit);
+                        if (cnt + pc > kCollect && cnt > kK) {   // no room for them all: keep the 24 best so far
                             uint64_t kk[32];
                             settle_keys(cb, lane, nval, cnt, kk);
 #pragma unroll
@@ -689,8 +704,57 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                             }
                             nval = cnt = kK;
                         }
-                        cb[cnt * kTrussWave + lane] = ((uint64_t)__float_as_uint(d2) << 32) | (uint32_t)mt.x;
-                        ++cnt;
+                        const int want = min(pc, kCollect - cnt);
+                        int npair = 0;
+                        const int s0 = wave_excl_scan(want, lane, &npair);
+                        const int take = max(0, min(want, kTrussWave - s0));
+                        npair = min(npair, kTrussWave);
+                        for (int e = 0; e < take; ++e) {
+                            const int k = __builtin_ctzll(hit);
+                            hit &= hit - 1;
+                            pairs[s0 + e] = (lane << 6) | k;
+                        }
+                        __syncthreads();   // the pairs are written
+                        const int pr = lane < npair ? pairs[lane] : 0;
+                        const int o = pr >> 6, k = pr & 63;
+                        // the owner's query, from its lane (every lane takes part in the shuffles)
+                        const float ox_ = __shfl(px, o, kWave), oy_ = __shfl(py, o, kWave);
+                        const float oz_ = __shfl(pz, o, kWave), oh = __shfl(hi, o, kWave);
+                        const int oid = __shfl(myid, o, kWave), otl = __shfl(ti_int | (li << 1), o, kWave);
+                        const int os0 = __shfl(s0, o, kWave), ocnt = __shfl(cnt, o, kWave);
+                        uint64_t key = 0;
+                        bool valid = false;
+                        if (lane < npair) {
+                            const float4 c = cpos[k];
+                            const int2 mt = cms[k];
+                            const float ex = c.x - ox_, ey = c.y - oy_, ez = c.z - oz_;
+                            const float d2 = ex * ex + ey * ey + ez * ez;
+                            const float rj = half * (oh + c.w);
+                            valid = d2 < rj * rj;
+                            // not the node itself; no interior bar across labels
+                            valid = valid && mt.x != oid &&
+                                    !((otl & 1) && (mt.y & 0xFF) == TN_INTERIOR && (mt.y >> 8) != (otl >> 1));
+                            if (valid && !H.prune) {   // the reference's own scan region only
+                                const int L = r_lev[crow[k]];
+                                const float rr = 0.5f * (tskin * oh + H.b0 * (float)(1 << L));
+                                const int cx = cell_coord(H, L, c.x, H.ox, 0);
+                                valid = cx >= cell_coord(H, L, ox_ - rr, H.ox, 0) &&
+                                        cx <= cell_coord(H, L, ox_ + rr, H.ox, 0);
+                            }
+                            key = ((uint64_t)__float_as_uint(d2) << 32) | (uint32_t)mt.x;
+                        }
+                        const uint64_t vm = __ballot(valid);
+                        // the owner's pairs are [s0, s0 + take) of its round: a valid pair's place is
+                        // the owner's count plus the valid pairs of the owner before it
+                        if (valid) {
+                            const int at = ocnt + __popcll(vm & lanemask_lt() & ~((1ull << (os0 & 63)) - 1));
+                            cb[at * kTrussWave + o] = key;
+                        }
+                        if (take > 0) {
+                            const uint64_t span = take == kTrussWave ? ~0ull : ((1ull << take) - 1);
+                            cnt += __popcll((vm >> s0) & span);
+                        }
+                        __syncthreads();   // the pairs are consumed
                     }
                 }
             };
@@ -713,7 +777,12 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                 // rows beyond reach are skipped (a valid bar is shorter than r; margin for rounding)
                 const float rw = r * 1.0001f + 1e-3f * bL + 2e-6f * (fabsf(px) + fabsf(py) + fabsf(pz));
                 const float rr = rw * rw;
-                const int ny1 = H.dim[L][1] - 1, nz1 = H.dim[L][2] - 1;
+                // the level's grid (uniform: kept in scalar registers through the rows)
+                const int nx = __builtin_amdgcn_readfirstlane(H.dim[L][0]);
+                const int ny = __builtin_amdgcn_readfirstlane(H.dim[L][1]);
+                const int nz1 = __builtin_amdgcn_readfirstlane(H.dim[L][2]) - 1, ny1 = ny - 1;
+                const int koff = __builtin_amdgcn_readfirstlane(H.off[L]);
+                const float inv_bL = 1.0f / bL;
                 for (int cz = Z0; cz <= Z1; ++cz) {
                     const float zlo = cz == 0 ? -inf : H.oz + (float)cz * bL;
                     const float zhi = cz == nz1 ? inf : H.oz + (float)(cz + 1) * bL;
@@ -729,11 +798,18 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                                 xa = x0;
                                 xb = x1;
                             } else {
+                                // The x-extent of the ball of radius rw in this row. A valid
+                                // candidate lies at least (rw - rj) / 2 >= 5e-4 bL inside it (rj <= r,
+                                // the bar's reach), far more than the error of the hardware square
+                                // root and of the multiplication by 1 / bL (below 1e-4 cells), so
+                                // these approximate cells never drop a candidate.
                                 const float rem = rr - dz * dz - dy * dy;
                                 if (rem >= 0.0f) {
-                                    const float w = sqrtf(rem);
-                                    xa = max(x0, cell_coord(H, L, px - w, H.ox, 0));
-                                    xb = min(x1, cell_coord(H, L, px + w, H.ox, 0));
+                                    const float w = __builtin_amdgcn_sqrtf(rem);
+                                    const int ca = (int)floorf((px - w - H.ox) * inv_bL);
+                                    const int cb_ = (int)floorf((px + w - H.ox) * inv_bL);
+                                    xa = max(x0, min(max(ca, 0), nx - 1));
+                                    xb = min(x1, min(max(cb_, 0), nx - 1));
                                 }
                             }
                         }
@@ -743,7 +819,7 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                         }
                         const uint64_t scan = __ballot(xa <= xb);   // the lanes with cells in the row
                         if (lane == 0) {
-                            const int krow = H.off[L] + H.dim[L][0] * (cy + H.dim[L][1] * cz);
+                            const int krow = koff + nx * (cy + ny * cz);
                             r_key0[nr] = krow + XA;
                             r_key1[nr] = krow + XB + 1;
                             r_mask[nr] = scan;
