@@ -1181,6 +1181,7 @@ struct TrussIn {
     const float4* __restrict__ Ps;       // positions + h as of the last rebuild, by slot
     const int2* __restrict__ Ms;         // (id, typ | lab << 8) as of the last rebuild, by slot
     const int32_t* __restrict__ slot_of; // id -> slot
+    const int32_t* __restrict__ order;   // the slots in the K-nearest search's query order
     float t, skin;
 };
 

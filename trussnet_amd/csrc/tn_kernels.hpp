@@ -191,6 +191,58 @@ __global__ __launch_bounds__(kBlock) void k_keys(Hash H, const float4* __restric
     }
 }
 
+// ---- the K-nearest search's query order ------------------------------------------
+//
+// k_truss serves the nodes a wave at a time. In slot order (cells in x-fastest row
+// order) a wave's 64 nodes lie along a row of cells, and the union of their scan
+// boxes is long; taken brick by brick instead (4 x 4 x 4 cells of a level, cells
+// z, y, x within a brick, bricks in x-fastest order) they are compact, and the union,
+// which a wave loads and tests as a whole, is about a quarter smaller. The order is a
+// counting sort of the slots by brick position: a cell's slots stay contiguous.
+
+constexpr int kBrickEdge = 4;
+
+struct BrickMap {
+    int nb[TN_MAXLEV][3];   // bricks per axis at each level
+    int off[TN_MAXLEV];     // each level's first position (64 a brick)
+    int size;               // positions in all
+};
+
+// the brick-order position of cell key k
+__device__ __forceinline__ int brick_pos(const Hash& H, const BrickMap& B, int k) {
+    int L = 0;
+    while (L + 1 < H.nlev && k >= H.off[L + 1]) {
+        ++L;
+    }
+    const int local = k - H.off[L], nx = H.dim[L][0], ny = H.dim[L][1];
+    const int cx = local % nx, cy = (local / nx) % ny, cz = local / (nx * ny);
+    const int brick = (cx / kBrickEdge) + B.nb[L][0] * ((cy / kBrickEdge) + B.nb[L][1] * (cz / kBrickEdge));
+    const int within = (cx % kBrickEdge) + kBrickEdge * ((cy % kBrickEdge) + kBrickEdge * (cz % kBrickEdge));
+    return B.off[L] + brick * (kBrickEdge * kBrickEdge * kBrickEdge) + within;
+}
+
+// the node count of each cell at its brick-order position (positions of no cell stay 0)
+__global__ __launch_bounds__(kBlock) void k_brick_count(Hash H, BrickMap B, const int32_t* __restrict__ cstart,
+                                                        int32_t* __restrict__ cnt) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k < H.nkeys) {
+        cnt[brick_pos(H, B, k)] = cstart[k + 1] - cstart[k];
+    }
+}
+
+// the slots in brick order: slot s goes to its cell's start in that order plus its rank
+// in the cell (start: the exclusive scan of k_brick_count's counts)
+__global__ __launch_bounds__(kBlock) void k_brick_order(Hash H, BrickMap B, const int32_t* __restrict__ key, int n,
+                                                        const int32_t* __restrict__ cstart,
+                                                        const int32_t* __restrict__ start,
+                                                        int32_t* __restrict__ order) {
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s < n) {
+        const int k = key[s];
+        order[start[brick_pos(H, B, k)] + (s - cstart[k])] = s;
+    }
+}
+
 // ---- stable LSD radix sort of (key, id) by key, 8-bit digits --------------------------
 
 constexpr int kRadixItems = 8;
@@ -329,11 +381,7 @@ __global__ __launch_bounds__(kBlock) void k_permute(const int32_t* __restrict__ 
 // before accepting it.)
 
 constexpr int kTrussWave = 64;   // one wave per block
-#ifndef TN_TRUSS_SPAN
-#define TN_TRUSS_SPAN 16
-#endif
 constexpr int kCollect = 32;   // per-lane candidate buffer (entries): settled by a 32-key network
-constexpr int kGroupSpan = TN_TRUSS_SPAN;    // cells along a row that one group of lanes may span
 
 // ---- wave-wide reductions and scans on DPP (64 lanes, all active) ----------------
 //
@@ -568,8 +616,8 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
     const float k2 = half * half;
     const float inf = __builtin_huge_valf();
     for (int qb = blockIdx.x * kTrussWave; qb < n; qb += gridDim.x * kTrussWave) {
-        const int q = qb + lane;
-        const bool active = q < n;
+        const bool active = qb + lane < n;
+        const int q = active ? in.order[qb + lane] : 0;   // the node in slot q
         float px = 0.0f, py = 0.0f, pz = 0.0f, hi = 0.0f, Ri = 0.0f;
         int Li = 0, ti_int = 0, li = 0, myid = -1;
         if (active) {   // the node in slot q (nodes are in cell order)
@@ -585,14 +633,14 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
             ti_int = (mi.y & 0xFF) == TN_INTERIOR;
             li = mi.y >> 8;
         }
-        // The lanes are served in groups that share a cell row of their own level and lie
-        // within kGroupSpan cells of the group's first lane along it, so that a group's
-        // union of scan boxes stays tight (a wave of queries can straddle rows).
+        // The lanes are served in groups that share their level and a row of bricks and lie
+        // within one brick of the group's first lane along it, so that a group's union of
+        // scan boxes stays tight (a wave of queries can straddle bricks).
         int grow = -1, gx = 0;
         if (active) {
             const int cy = cell_coord(H, Li, py, H.oy, 1), cz = cell_coord(H, Li, pz, H.oz, 2);
-            grow = H.off[Li] + H.dim[Li][0] * (cy + H.dim[Li][1] * cz);
-            gx = cell_coord(H, Li, px, H.ox, 0);
+            grow = (Li << 26) | ((cz / kBrickEdge) << 13) | (cy / kBrickEdge);
+            gx = cell_coord(H, Li, px, H.ox, 0) / kBrickEdge;
         }
         bool pending = active;
         int cnt = 0, nval = 0;   // the lane's list: [0, nval) settled, in order; [nval, cnt) appended
@@ -603,7 +651,7 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
             }
             const int lead = __ffsll((long long)pm) - 1;
             const int lrow = __shfl(grow, lead, kWave), lx = __shfl(gx, lead, kWave);
-            const bool ingroup = pending && grow == lrow && abs(gx - lx) <= kGroupSpan;
+            const bool ingroup = pending && grow == lrow && abs(gx - lx) <= 1;
             pending = pending && !ingroup;
             // the prefilter's origin: the group's first query; its query operands
             const float gox = __shfl(px, lead, kWave), goy = __shfl(py, lead, kWave), goz = __shfl(pz, lead, kWave);

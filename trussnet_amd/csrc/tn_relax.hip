@@ -374,6 +374,8 @@ struct tn_plan {
     uint16_t *dLab0 = nullptr, *dLab = nullptr, *dLabb = nullptr;
     int32_t *dId = nullptr, *dIdb = nullptr, *dSlotOf = nullptr;
     const int32_t* dKeySorted = nullptr;   // the keys in slot order after the last rebuild
+    tnk::BrickMap bricks{};                // the K-nearest search's query order (brick positions)
+    int32_t* dBrick = nullptr;             // [bricks.size + 1]: node counts, then starts, by brick position
     float *dMv = nullptr, *dPw = nullptr, *dThr = nullptr;
     int32_t *dKey = nullptr, *dCstart = nullptr, *dSorted = nullptr, *dNbrT = nullptr, *dNnb = nullptr;
     int32_t *dQueue = nullptr, *dBhist = nullptr, *dBsum = nullptr;
@@ -455,6 +457,7 @@ struct tn_plan {
         in.Ps = dPs;
         in.Ms = dMs;
         in.slot_of = dSlotOf;
+        in.order = dSorted;   // (after a rebuild's permutation: the query order)
         in.t = prm.t;
         in.skin = prm.skin;
         return in;
@@ -498,6 +501,12 @@ struct tn_plan {
         std::swap(dId, dIdb);
         timer.end(KC_PERMUTE, t);
         t = timer.begin();
+        // the query order, into dSorted (free once the nodes are permuted)
+        tnk::k_fill_i32<<<blocks_for(bricks.size + 1, tnk::kBlock), tnk::kBlock, 0, stream>>>(dBrick, bricks.size + 1,
+                                                                                             0);
+        tnk::k_brick_count<<<blocks_for(nkeys, tnk::kBlock), tnk::kBlock, 0, stream>>>(H, bricks, dCstart, dBrick);
+        scan(dBrick, dBrick, (int64_t)bricks.size + 1);
+        tnk::k_brick_order<<<nb, tnk::kBlock, 0, stream>>>(H, bricks, dKeySorted, n, dCstart, dBrick, dSorted);
         tnk::k_truss<<<blocks_for(n, tnk::kTrussWave), tnk::kTrussWave, 0, stream>>>(H, truss_in(), n, dNbrT, dNnb);
         timer.end(KC_NEIGHBORS, t);
         TN_HIP(hipGetLastError());
@@ -850,7 +859,23 @@ tn_plan* plan_build(const tn_relax_params* prm, const tn_nodes_view* seeds, int 
         p->radix_passes = (bits + 7) / 8;
         p->radix_nblk = (int)std::max<unsigned>(1, blocks_for(n, tnk::kRadixTile));
         p->dBhist = p->alloc<int32_t>((size_t)256 * p->radix_nblk);
-        const int64_t scan_max = std::max<int64_t>(p->nkeys + 1, (int64_t)256 * p->radix_nblk);
+        {   // the query order's brick positions
+            int64_t pos = 0;
+            for (int L = 0; L < p->H.nlev; ++L) {
+                int64_t nbl = 1;
+                for (int a = 0; a < 3; ++a) {
+                    p->bricks.nb[L][a] = (p->H.dim[L][a] + tnk::kBrickEdge - 1) / tnk::kBrickEdge;
+                    nbl *= p->bricks.nb[L][a];
+                }
+                p->bricks.off[L] = (int)pos;
+                pos += nbl * tnk::kBrickEdge * tnk::kBrickEdge * tnk::kBrickEdge;
+            }
+            arg_check(pos < ((int64_t)1 << 31) - 1, "hash: too many cells for the query order");
+            p->bricks.size = (int)pos;
+            p->dBrick = p->alloc<int32_t>((size_t)pos + 1);
+        }
+        const int64_t scan_max = std::max<int64_t>(std::max<int64_t>(p->nkeys, p->bricks.size) + 1,
+                                                   (int64_t)256 * p->radix_nblk);
         p->dBsum = p->alloc<int32_t>(blocks_for(scan_max, tnk::kScanTile));
         p->nparts = (int)std::max<unsigned>(1, std::min<unsigned>(blocks_for(n, tnk::kBlock), tnk::kStatsBlocks));
         p->refresh_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(n, 2048));   // one wave per listed node
