@@ -1327,6 +1327,9 @@ __device__ __forceinline__ void k_move_easy_body(Field f, MoveArgs ma, Ctl* __re
                 const int a = lab[i];
                 const P3 p = p3(x.x, x.y, x.z);
                 const P3 q = p3(p.x + st.s[0], p.y + st.s[1], p.z + st.s[2]);
+                // h at the target, loaded before the owner query's chain of loads (it is
+                // needed unless the node turns out to cross or to snap)
+                const float hq = size_at(f, q.x, q.y, q.z);
                 const OwnerPsi own = owner_psi_at(f, a, q);
                 if (own.best != a) {
                     inner = true;
@@ -1350,7 +1353,7 @@ __device__ __forceinline__ void k_move_easy_body(Field f, MoveArgs ma, Ctl* __re
                         ns.b = kNoLab;
                     } else {
                         ns.p = q;
-                        ns.hn = size_at(f, q.x, q.y, q.z);
+                        ns.hn = hq;
                         m = sqrtf(m2) / h;
                     }
                     move_store(ma, ctl, i, x, TN_INTERIOR, pp, ns, m, st.pw, X, V, typ, part, mv, pw);
