@@ -1726,6 +1726,11 @@ __device__ float voxel_step(const Field& f, float h, float snap, int a, const fl
         } else if (snap > 0.0f) {   // close to any wall: snap onto it
             const int Rs = 1 + (int)ceilf(snap * h / vmin);
             float best = -1.0f, y[3];
+            // A label met in several directions is scanned once: the scan's result depends
+            // on the label only, and a repeat can never beat the best so far (it ties or
+            // was rejected), so skipping it keeps the reference's choice.
+            constexpr int kSeen = 4;
+            int seen[kSeen] = { 0, 0, 0, 0 }, nseen = 0;
             for (int dz = -1; dz <= 1; ++dz) {
                 for (int dy = -1; dy <= 1; ++dy) {
                     for (int dx = -1; dx <= 1; ++dx) {
@@ -1735,6 +1740,19 @@ __device__ float voxel_step(const Field& f, float h, float snap, int a, const fl
                         if (l2 == a || l2 == b) {
                             continue;
                         }
+                        bool again = false;
+#pragma unroll
+                        for (int m = 0; m < kSeen; ++m) {   // (static indices: the list stays in registers)
+                            again = again || (m < nseen && seen[m] == l2);
+                        }
+                        if (again) {
+                            continue;
+                        }
+#pragma unroll
+                        for (int m = 0; m < kSeen; ++m) {
+                            seen[m] = m == nseen ? l2 : seen[m];
+                        }
+                        nseen += nseen < kSeen ? 1 : 0;
                         const float d2 = vox_nearest(f, a, l2, kNoLab, q, Rs, y);
                         if (d2 >= 0.0f && d2 < snap * snap * h * h && (best < 0.0f || d2 < best)) {
                             best = d2;

@@ -382,6 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_permute(const int32_t* __restrict__ 
 
 constexpr int kTrussWave = 64;   // one wave per block
 constexpr int kCollect = 32;   // per-lane candidate buffer (entries): settled by a 32-key network
+constexpr int kMinRoom = 4;    // a lane with less room than this (and more possible hits) settles first
 
 // ---- wave-wide reductions and scans on DPP (64 lanes, all active) ----------------
 //
@@ -444,8 +445,7 @@ __device__ __forceinline__ bool before(float da, int ja, float db, int jb) {
 
 // A list entry as one 64-bit key: the distance's bits above the id. Distances are
 // >= +0, whose bits order as the floats do, so the keys order as (distance, id), the
-// K-list's order (src/opencl/v2m_particle_body.cl:128-141). While a hit waits in a
-// lane's buffer unsettled, the high word holds d^2 instead.
+// K-list's order (src/opencl/v2m_particle_body.cl:128-141).
 __device__ __forceinline__ uint64_t list_key(float d, int id) {
     return ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)id;
 }
@@ -470,17 +470,12 @@ __device__ __forceinline__ void bitonic32(uint64_t (&k)[32]) {
     }
 }
 
-// A lane's buffer (entries e*64 + lane, kCollect == 32 of them) settled into k, in
-// order: entries [0, lo) hold distances already, [lo, cnt) d^2 (-> sqrtf(d^2)),
-// [cnt, 32) are unused (the largest key).
-__device__ __forceinline__ void settle_keys(const uint64_t* cb, int lane, int lo, int cnt, uint64_t (&k)[32]) {
+// A lane's buffer (entries e*64 + lane, kCollect == 32 of them; [cnt, 32) unused, read
+// as the largest key) settled into k, in order.
+__device__ __forceinline__ void settle_keys(const uint64_t* cb, int lane, int cnt, uint64_t (&k)[32]) {
 #pragma unroll
     for (int e = 0; e < 32; ++e) {
-        uint64_t v = e < cnt ? cb[e * kTrussWave + lane] : ~0ull;
-        if (e >= lo && e < cnt) {
-            v = list_key(sqrtf(__uint_as_float((uint32_t)(v >> 32))), (int)(uint32_t)v);
-        }
-        k[e] = v;
+        k[e] = e < cnt ? cb[e * kTrussWave + lane] : ~0ull;
     }
     bitonic32(k);
 }
@@ -643,7 +638,10 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
             gx = cell_coord(H, Li, px, H.ox, 0) / kBrickEdge;
         }
         bool pending = active;
-        int cnt = 0, nval = 0;   // the lane's list: [0, nval) settled, in order; [nval, cnt) appended
+        // The lane's buffer holds cnt keys; once it has been cut back to the 24 best, kth is
+        // the 24th of them, and a candidate must come before it to enter.
+        int cnt = 0;
+        uint64_t kth = ~0ull;
         for (;;) {
             const uint64_t pm = __ballot(pending);
             if (!pm) {
@@ -743,14 +741,16 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                     // ones appended to their query's buffer, packed in pair order.
                     while (__ballot(hit != 0ull)) {
                         const int pc = __popcll(hit);
-                        if (cnt + pc > kCollect && cnt > kK) {   // no room for them all: keep the 24 best so far
+                        // little room for them: keep the 24 best so far
+                        if (pc > kCollect - cnt && kCollect - cnt < kMinRoom) {
                             uint64_t kk[32];
-                            settle_keys(cb, lane, nval, cnt, kk);
+                            settle_keys(cb, lane, cnt, kk);
 #pragma unroll
                             for (int e = 0; e < kK; ++e) {
                                 cb[e * kTrussWave + lane] = kk[e];
                             }
-                            nval = cnt = kK;
+                            cnt = kK;
+                            kth = kk[kK - 1];
                         }
                         const int want = min(pc, kCollect - cnt);
                         int npair = 0;
@@ -770,6 +770,7 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                         const float oz_ = __shfl(pz, o, kWave), oh = __shfl(hi, o, kWave);
                         const int oid = __shfl(myid, o, kWave), otl = __shfl(ti_int | (li << 1), o, kWave);
                         const int os0 = __shfl(s0, o, kWave), ocnt = __shfl(cnt, o, kWave);
+                        const uint64_t okth = __shfl(kth, o, kWave);
                         uint64_t key = 0;
                         bool valid = false;
                         if (lane < npair) {
@@ -789,7 +790,8 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                                 valid = cx >= cell_coord(H, L, ox_ - rr, H.ox, 0) &&
                                         cx <= cell_coord(H, L, ox_ + rr, H.ox, 0);
                             }
-                            key = ((uint64_t)__float_as_uint(d2) << 32) | (uint32_t)mt.x;
+                            key = list_key(sqrtf(d2), mt.x);
+                            valid = valid && key < okth;
                         }
                         const uint64_t vm = __ballot(valid);
                         // the owner's pairs are [s0, s0 + take) of its round: a valid pair's place is
@@ -885,7 +887,7 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
         }   // the groups
         if (active) {   // (distance, id) order; the list holds slots
             uint64_t kk[32];
-            settle_keys(cb, lane, nval, cnt, kk);
+            settle_keys(cb, lane, cnt, kk);
             const int m = cnt < kK ? cnt : kK;
             nnb[q] = m;
 #pragma unroll
