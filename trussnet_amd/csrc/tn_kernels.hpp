@@ -1049,6 +1049,11 @@ __global__ __launch_bounds__(kWave) void k_truss_node(Hash H, TrussIn in, int n,
 __device__ __forceinline__ void k_refresh_list_body(const uint8_t* __restrict__ flag, int n, Ctl* __restrict__ ctl, int32_t* __restrict__ list, int bid, int nblk) {
     __shared__ int woff[kBlock / kWave];
     __shared__ int base;
+    // k_control has run: a rebuild (which rebuilds every list and sets P0 = P) or the
+    // end of the loop follows, so no refreshed list would ever be read
+    if (ctl->rebuild || ctl->done) {
+        return;
+    }
     const int s = bid * kBlock + threadIdx.x;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
     const bool f = s < n && flag[s];

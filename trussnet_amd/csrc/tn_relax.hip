@@ -630,14 +630,16 @@ struct tn_plan {
         tnk::k_stats<<<nparts, tnk::kBlock, 0, stream>>>(dX, dP0, dMv, dPw, n, prm.skin, fire, dThr, dCtl, dFlag,
                                                          dPartials);
         timer.end(KC_STATS, t);
-        t = timer.begin();
-        tnk::k_refresh_list<<<nb, tnk::kBlock, 0, stream>>>(dFlag, n, dCtl, dQueue);
-        tnk::k_truss_node<<<refresh_blocks, tnk::kWave, 0, stream>>>(H, truss_in(), n, dQueue, dCtl, dNbrT, dNnb, dP0);
-        timer.end(KC_REFRESH, t);
+        // the control before the own-list refreshes, which it does not read: they are
+        // skipped when a rebuild or the loop's end follows
         t = timer.begin();
         tnk::CtlArgs ca{ n, prm.max_iters, fire, rebuild_div, prm.dptol, nparts, force_serial };
         tnk::k_control<<<1, tnk::kBlock, 0, stream>>>(ca, dCtl, dPartials, dPw, dSlotOf);
         timer.end(KC_CONTROL, t);
+        t = timer.begin();
+        tnk::k_refresh_list<<<nb, tnk::kBlock, 0, stream>>>(dFlag, n, dCtl, dQueue);
+        tnk::k_truss_node<<<refresh_blocks, tnk::kWave, 0, stream>>>(H, truss_in(), n, dQueue, dCtl, dNbrT, dNnb, dP0);
+        timer.end(KC_REFRESH, t);
         TN_HIP(hipGetLastError());
         TN_HIP(hipMemcpyAsync(hCtl, dCtl, sizeof(tnk::Ctl), hipMemcpyDeviceToHost, stream));
         if (ev_iter) {
@@ -1467,9 +1469,9 @@ void run_grouped(Plans& plans, int64_t& launches, int64_t& steps) {
             moves(std::false_type{});
         }
         kg_stats<<<nblocks[kShParts], kBlock, 0, bs>>>(dtab, bplan[kShParts], bfirst[kShParts]);
+        kg_control<<<nblocks[kShOne], kBlock, 0, bs>>>(dtab, bplan[kShOne], bfirst[kShOne]);
         kg_refresh_list<<<nblocks[kShNodes], kBlock, 0, bs>>>(dtab, bplan[kShNodes], bfirst[kShNodes]);
         kg_truss_node<<<nblocks[kShRefresh], kWave, 0, bs>>>(dtab, bplan[kShRefresh], bfirst[kShRefresh]);
-        kg_control<<<nblocks[kShOne], kBlock, 0, bs>>>(dtab, bplan[kShOne], bfirst[kShOne]);
         kg_ctl_gather<<<nblocks[kShOne], 64, 0, bs>>>(dtab, bplan[kShOne], bfirst[kShOne], dctl);
         TN_HIP(hipGetLastError());
         TN_HIP(hipMemcpyAsync(hctl, dctl, sizeof(tnk::Ctl) * count, hipMemcpyDeviceToHost, bs));
