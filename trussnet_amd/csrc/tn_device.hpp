@@ -1686,6 +1686,167 @@ __device__ float vox_nearest(const Field& f, int a, int b, int c, const float* q
     return best;
 }
 
+constexpr int kVoxBoxR = 4;   // scans up to this reach read their labels from a box in LDS
+constexpr int kVoxBox = (2 * kVoxBoxR + 3) * (2 * kVoxBoxR + 3) * (2 * kVoxBoxR + 3);
+
+// vox_nearest with the label reads of its (2R+1)^3 voxels split over a wave (every
+// lane active, the same arguments in every lane). Each lane tests one voxel of a
+// group of 64 and marks which of its faces (or edges) qualify. The serial scan
+// measures each candidate from q, and, when out is q itself (as the callers that move
+// the node pass it), from the last point it accepted: then the wave walks the marked
+// ones in the serial scan's order, every lane alike. Otherwise each lane keeps its
+// first best and the wave takes the smallest (d^2, scan position). Either way the
+// serial choice is kept, and the result is the same in every lane.
+
+__device__ float vox_nearest_wave(const Field& f, int a, int b, int c, const float* q, int R, float* out) {
+    __shared__ uint16_t box[4][kVoxBox];   // a wave's labels around its scan (4 waves a block)
+    const int lane = threadIdx.x & 63;
+    const bool chained = out == q;
+    const float vs[3] = { f.vx, f.vy, f.vz };
+    float u[3], qc[3];
+    int ci[3];
+    for (int k = 0; k < 3; ++k) {
+        u[k] = over_v(f, q[k], k) + 0.5f;
+        ci[k] = (int)floorf(u[k]);
+        qc[k] = q[k];
+    }
+    const int W = 2 * R + 1, total = W * W * W, nsub = c == kNoLab ? 2 : 4;
+    // The labels of the scan's voxels and their neighbours, (W + 2)^3 of them, read at
+    // once (independent loads) into LDS; reaches beyond the box read them directly.
+    const bool boxed = R <= kVoxBoxR;
+    const int B = W + 2, x0 = ci[0] - R - 1, y0 = ci[1] - R - 1, z0 = ci[2] - R - 1;
+    uint16_t* bx = box[(threadIdx.x >> 6) & 3];
+    if (boxed) {
+        for (int t = lane; t < B * B * B; t += 64) {
+            bx[t] = (uint16_t)label_or_zero(f, x0 + t % B, y0 + (t / B) % B, z0 + t / (B * B));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    auto lab_at = [&](int i, int j, int k) {
+        return boxed ? (int)bx[(i - x0) + B * ((j - y0) + B * (k - z0))] : label_or_zero(f, i, j, k);
+    };
+    float best = -1.0f;
+    uint64_t lbest = ~0ull;   // not chained: (d^2 bits, scan position) of this lane's best
+    float ly[3] = { 0.0f, 0.0f, 0.0f };
+    for (int base = 0; base < total; base += 64) {
+        const int idx = base + lane;
+        int bits = 0;   // bit ax * 4 + e: the face (edge) e of axis ax qualifies
+        if (idx < total) {
+            const int v[3] = { ci[0] - R + idx % W, ci[1] - R + (idx / W) % W, ci[2] - R + idx / (W * W) };
+            if (lab_at(v[0], v[1], v[2]) == a) {
+                for (int ax = 0; ax < 3; ++ax) {
+                    const int o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
+                    for (int e = 0; e < nsub; ++e) {
+                        bool ok;
+                        if (c == kNoLab) {
+                            int nn[3] = { v[0], v[1], v[2] };
+                            nn[ax] += e ? 1 : -1;
+                            ok = lab_at(nn[0], nn[1], nn[2]) == b;
+                        } else {
+                            const int e1 = e & 1, e2 = e >> 1;
+                            int hb = 0, hc = 0;
+                            for (int m = 0; m < 4; ++m) {
+                                int nn[3] = { v[0], v[1], v[2] };
+                                nn[o1] += e1 - 1 + (m & 1);
+                                nn[o2] += e2 - 1 + (m >> 1);
+                                const int ln = lab_at(nn[0], nn[1], nn[2]);
+                                hb |= ln == b;
+                                hc |= ln == c;
+                            }
+                            ok = hb && hc;
+                        }
+                        bits |= ok ? 1 << (ax * 4 + e) : 0;
+                        if (ok && !chained) {   // measured from q alone: this lane's first best
+                            float y[3];
+                            if (c == kNoLab) {
+                                y[ax] = (float)(v[ax] + e);
+                                y[o1] = fminf(fmaxf(u[o1], (float)v[o1]), (float)(v[o1] + 1));
+                                y[o2] = fminf(fmaxf(u[o2], (float)v[o2]), (float)(v[o2] + 1));
+                            } else {
+                                y[ax] = fminf(fmaxf(u[ax], (float)v[ax]), (float)(v[ax] + 1));
+                                y[o1] = (float)(v[o1] + (e & 1));
+                                y[o2] = (float)(v[o2] + (e >> 1));
+                            }
+                            float d2 = 0.0f;
+                            for (int k = 0; k < 3; ++k) {
+                                y[k] = (y[k] - 0.5f) * vs[k];
+                                d2 += (y[k] - q[k]) * (y[k] - q[k]);
+                            }
+                            const uint64_t key = ((uint64_t)__float_as_uint(d2) << 32) |
+                                                 (uint32_t)(idx * 16 + ax * 4 + e);
+                            if (key < lbest) {
+                                lbest = key;
+                                ly[0] = y[0];
+                                ly[1] = y[1];
+                                ly[2] = y[2];
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        if (!chained) {
+            continue;
+        }
+        // the marked voxels of this group in scan order (uniform across the wave)
+        uint64_t marked = __ballot(bits != 0);
+        while (marked) {
+            const int l = __ffsll((long long)marked) - 1;
+            marked &= marked - 1;
+            const int vb = __shfl(bits, l, 64), vidx = base + l;
+            const int v[3] = { ci[0] - R + vidx % W, ci[1] - R + (vidx / W) % W, ci[2] - R + vidx / (W * W) };
+            for (int ax = 0; ax < 3; ++ax) {
+                const int o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
+                for (int e = 0; e < nsub; ++e) {
+                    if (!((vb >> (ax * 4 + e)) & 1)) {
+                        continue;
+                    }
+                    float y[3];
+                    if (c == kNoLab) {
+                        y[ax] = (float)(v[ax] + e);
+                        y[o1] = fminf(fmaxf(u[o1], (float)v[o1]), (float)(v[o1] + 1));
+                        y[o2] = fminf(fmaxf(u[o2], (float)v[o2]), (float)(v[o2] + 1));
+                    } else {
+                        y[ax] = fminf(fmaxf(u[ax], (float)v[ax]), (float)(v[ax] + 1));
+                        y[o1] = (float)(v[o1] + (e & 1));
+                        y[o2] = (float)(v[o2] + (e >> 1));
+                    }
+                    float d2 = 0.0f;
+                    for (int k = 0; k < 3; ++k) {
+                        y[k] = (y[k] - 0.5f) * vs[k];
+                        d2 += (y[k] - qc[k]) * (y[k] - qc[k]);
+                    }
+                    if (best < 0.0f || d2 < best) {
+                        best = d2;
+                        for (int k = 0; k < 3; ++k) {
+                            out[k] = y[k];
+                            qc[k] = chained ? y[k] : qc[k];
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (!chained) {   // the smallest (d^2, position): the serial scan's first best
+        uint64_t m = lbest;
+        for (int d = 32; d > 0; d >>= 1) {
+            const uint64_t o = __shfl_xor(m, d, 64);
+            m = o < m ? o : m;
+        }
+        if (m != ~0ull) {
+            const int win = __ffsll((long long)__ballot(lbest == m)) - 1;
+            out[0] = __shfl(ly[0], win, 64);
+            out[1] = __shfl(ly[1], win, 64);
+            out[2] = __shfl(ly[2], win, 64);
+            best = __uint_as_float((uint32_t)(m >> 32));
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");   // the box is read before it is reused
+    __builtin_amdgcn_wave_barrier();
+    return best;
+}
+
 // v2m_vox_third, src/opencl/v2m_particle_body.cl:533-545
 __device__ __forceinline__ int vox_third(const Field& f, int a, int b, const float* p) {
     const int i0 = (int)floorf(over_vx(f, p[0])), j0 = (int)floorf(over_vy(f, p[1])),
@@ -1699,10 +1860,28 @@ __device__ __forceinline__ int vox_third(const Field& f, int a, int b, const flo
     return kNoLab;
 }
 
+// which scans the lane pass leaves to the wave pass: an interior node's wall (bit 0)
+// and snap (bit 1) scans, an interface or junction node's face or edge scan (bit 2).
+// The snap scans (reach up to 4 voxels, a 9^3 box per label met) are the long ones; the
+// face scans (reach about 2) run faster one lane per node (measured: wall and snap
+// scans deferred, 5.2 ms an iteration at 2 M nodes; all scans deferred, 6.4 ms).
+constexpr int kDeferWall = 1, kDeferSnap = 2, kDeferFace = 4;
+#ifndef TN_VOX_DEFER
+#define TN_VOX_DEFER 3
+#endif
+constexpr int kVoxDefer = TN_VOX_DEFER;
+
 // v2m_move_voxel, src/opencl/v2m_particle_body.cl:549-650: the capped step s of a node
 // of label a in state ns. Returns |dp| / h (0 and ns untouched when the node's face
-// patch or edge vanished within reach).
+// patch or edge vanished within reach). kWave false: one lane per node, and a node
+// that needs a face or edge scan (an interior node that hits or nears a wall, every
+// interface and junction node) is left untouched, returning kDeferred; kWave true: a
+// wave per node (every lane active, the same node), the scans split over the wave.
+template <bool kWave, int kDefer = kVoxDefer>
 __device__ float voxel_step(const Field& f, float h, float snap, int a, const float* s, NodeState& ns) {
+    auto nearest = [&](int bb, int cc, const float* qq, int RR, float* o) {
+        return kWave ? vox_nearest_wave(f, a, bb, cc, qq, RR, o) : vox_nearest(f, a, bb, cc, qq, RR, o);
+    };
     const float vmin = fminf(fminf(f.vx, f.vy), f.vz);
     const float p[3] = { ns.p.x, ns.p.y, ns.p.z };
     float q[3], x[3];
@@ -1716,8 +1895,11 @@ __device__ float voxel_step(const Field& f, float h, float snap, int a, const fl
         int l;
         const float t = dda_hit(f, a, p, s, x, &l);
         if (t >= 0.0f) {   // hit a wall: trap on it, glide with the rest of the step
+            if (!kWave && (kDefer & kDeferWall)) {
+                return kDeferred;
+            }
             b = l;
-            if (vox_nearest(f, a, b, kNoLab, q, R, q) < 0.0f) {
+            if (nearest(b, kNoLab, q, R, q) < 0.0f) {
                 q[0] = x[0];
                 q[1] = x[1];
                 q[2] = x[2];
@@ -1753,7 +1935,10 @@ __device__ float voxel_step(const Field& f, float h, float snap, int a, const fl
                             seen[m] = m == nseen ? l2 : seen[m];
                         }
                         nseen += nseen < kSeen ? 1 : 0;
-                        const float d2 = vox_nearest(f, a, l2, kNoLab, q, Rs, y);
+                        if (!kWave && (kDefer & kDeferSnap)) {
+                            return kDeferred;
+                        }
+                        const float d2 = nearest(l2, kNoLab, q, Rs, y);
                         if (d2 >= 0.0f && d2 < snap * snap * h * h && (best < 0.0f || d2 < best)) {
                             best = d2;
                             b = l2;
@@ -1771,12 +1956,14 @@ __device__ float voxel_step(const Field& f, float h, float snap, int a, const fl
                 ty = TN_INTERFACE;
             }
         }
+    } else if (!kWave && (kDefer & kDeferFace)) {
+        return kDeferred;
     } else if (ty == TN_INTERFACE) {
-        if (vox_nearest(f, a, b, kNoLab, q, R, q) < 0.0f) {
+        if (nearest(b, kNoLab, q, R, q) < 0.0f) {
             return 0.0f;
         }
     } else {   // junction: the nearest a|b|c voxel edge
-        if (vox_nearest(f, a, b, c, q, R, q) < 0.0f) {
+        if (nearest(b, c, q, R, q) < 0.0f) {
             return 0.0f;
         }
     }
@@ -1786,7 +1973,7 @@ __device__ float voxel_step(const Field& f, float h, float snap, int a, const fl
         const int cc = vox_third(f, a, b, q);
         if (cc != kNoLab) {
             float r[3];
-            const float d2 = vox_nearest(f, a, b, cc, q, 1, r);
+            const float d2 = nearest(b, cc, q, 1, r);
             if (d2 >= 0.0f && d2 < 0.0625f * h * h) {
                 q[0] = r[0];
                 q[1] = r[1];

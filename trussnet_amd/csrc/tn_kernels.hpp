@@ -1424,42 +1424,94 @@ __device__ __forceinline__ bool move_node(const Field& f, const MoveArgs& ma, co
 
 // The whole move in voxel-trap mode (v2m_move with voxmode, src/opencl/v2m_particle_body.cl:
 // 844-935 and 549-650): every node in one pass; corners and settled nodes stay.
+// The voxel-trap move, pass 1, a lane per node: corners, settled nodes and interior
+// nodes whose step meets no wall and no wall within snap range are finished here; the
+// rest (face and edge scans) are queued, untouched, for pass 2.
 __device__ __forceinline__ void k_move_voxel_body(const Field& f, const MoveArgs& ma, const Ctl* __restrict__ ctl,
                                                   int n, const float4* __restrict__ F, float4* __restrict__ X,
                                                   float4* __restrict__ V, uint8_t* __restrict__ typ,
                                                   uint32_t* __restrict__ part, const uint16_t* __restrict__ lab,
-                                                  float* __restrict__ mv, float* __restrict__ pw, int bid) {
+                                                  float* __restrict__ mv, float* __restrict__ pw,
+                                                  int32_t* __restrict__ queue, int32_t* __restrict__ qcount, int bid) {
     const int i = bid * kBlock + threadIdx.x;
-    if (i >= n) {
-        return;
-    }
-    const float4 x = X[i];
-    const float h = x.w;
-    const int ty0 = typ[i];
-    const uint32_t pp = part[i];
-    NodeState ns;
-    ns.p = p3(x.x, x.y, x.z);
-    ns.hn = h;
-    ns.ty = ty0;
-    ns.b = (int)(pp & 0xFFFF);
-    ns.c = (int)(pp >> 16);
-    float m = 0.0f, p_w = 0.0f;
-    if (ty0 != TN_CORNER) {
-        const NodeStep st = node_step(ma, ctl, F[i], V, i, h);
-        p_w = st.pw;
-        if (st.moves) {
-            m = voxel_step(f, h, ma.snap, lab[i], st.s, ns);
+    bool defer = false;
+    if (i < n) {
+        const float4 x = X[i];
+        const float h = x.w;
+        const int ty0 = typ[i];
+        const uint32_t pp = part[i];
+        NodeState ns;
+        ns.p = p3(x.x, x.y, x.z);
+        ns.hn = h;
+        ns.ty = ty0;
+        ns.b = (int)(pp & 0xFFFF);
+        ns.c = (int)(pp >> 16);
+        float m = 0.0f, p_w = 0.0f;
+        if (ty0 != TN_CORNER) {
+            const NodeStep st = node_step(ma, ctl, F[i], V, i, h);
+            p_w = st.pw;
+            if (st.moves) {
+                m = voxel_step<false>(f, h, ma.snap, lab[i], st.s, ns);
+                defer = m == kDeferred;
+            }
+        }
+        if (!defer) {
+            move_store(ma, ctl, i, x, ty0, pp, ns, m, p_w, X, V, typ, part, mv, pw);
         }
     }
-    move_store(ma, ctl, i, x, ty0, pp, ns, m, p_w, X, V, typ, part, mv, pw);
+    wave_append(defer, i, qcount, queue);
 }
 
 __global__ __launch_bounds__(kBlock) void k_move_voxel(Field f, MoveArgs ma, const Ctl* __restrict__ ctl, int n,
                                                        const float4* __restrict__ F, float4* __restrict__ X,
                                                        float4* __restrict__ V, uint8_t* __restrict__ typ,
                                                        uint32_t* __restrict__ part, const uint16_t* __restrict__ lab,
-                                                       float* __restrict__ mv, float* __restrict__ pw) {
-    k_move_voxel_body(f, ma, ctl, n, F, X, V, typ, part, lab, mv, pw, blockIdx.x);
+                                                       float* __restrict__ mv, float* __restrict__ pw,
+                                                       int32_t* __restrict__ queue, int32_t* __restrict__ qcount) {
+    k_move_voxel_body(f, ma, ctl, n, F, X, V, typ, part, lab, mv, pw, queue, qcount, blockIdx.x);
+}
+
+// Pass 2: a wave per queued node, every lane on the same node (the same values in
+// every lane), the face and edge scans split over the wave; lane 0 stores.
+__device__ __forceinline__ void k_move_voxel_wave_body(const Field& f, const MoveArgs& ma, const Ctl* __restrict__ ctl,
+                                                       const int32_t* __restrict__ queue,
+                                                       const int32_t* __restrict__ qcount,
+                                                       const float4* __restrict__ F, float4* __restrict__ X,
+                                                       float4* __restrict__ V, uint8_t* __restrict__ typ,
+                                                       uint32_t* __restrict__ part, const uint16_t* __restrict__ lab,
+                                                       float* __restrict__ mv, float* __restrict__ pw, int bid,
+                                                       int nblk) {
+    constexpr int kWaves = kBlock / kWave;
+    const int count = *qcount;
+    for (int g = bid * kWaves + (int)(threadIdx.x >> 6); g < count; g += nblk * kWaves) {
+        const int i = queue[g];
+        const float4 x = X[i];
+        const float h = x.w;
+        const int ty0 = typ[i];
+        const uint32_t pp = part[i];
+        NodeState ns;
+        ns.p = p3(x.x, x.y, x.z);
+        ns.hn = h;
+        ns.ty = ty0;
+        ns.b = (int)(pp & 0xFFFF);
+        ns.c = (int)(pp >> 16);
+        const NodeStep st = node_step(ma, ctl, F[i], V, i, h);   // queued nodes move
+        const float m = voxel_step<true>(f, h, ma.snap, lab[i], st.s, ns);
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            move_store(ma, ctl, i, x, ty0, pp, ns, m, st.pw, X, V, typ, part, mv, pw);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_move_voxel_wave(Field f, MoveArgs ma, const Ctl* __restrict__ ctl,
+                                                            const int32_t* __restrict__ queue,
+                                                            const int32_t* __restrict__ qcount,
+                                                            const float4* __restrict__ F, float4* __restrict__ X,
+                                                            float4* __restrict__ V, uint8_t* __restrict__ typ,
+                                                            uint32_t* __restrict__ part,
+                                                            const uint16_t* __restrict__ lab, float* __restrict__ mv,
+                                                            float* __restrict__ pw) {
+    k_move_voxel_wave_body(f, ma, ctl, queue, qcount, F, X, V, typ, part, lab, mv, pw, blockIdx.x, gridDim.x);
 }
 
 // Pass 2: v2m_move (src/opencl/v2m_particle_body.cl:844-935) for the queued nodes,
@@ -1873,8 +1925,9 @@ __global__ __launch_bounds__(kBlock) void kg_move_easy(const PlanIter* __restric
                                                        const int32_t* __restrict__ bplan,
                                                        const int32_t* __restrict__ bfirst) {
     TN_PLAN_BLOCK(kShNodes);
-    if (P.voxel) {   // the whole move; its queues stay empty
-        k_move_voxel_body(P.f, P.ma, P.ctl, P.n, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv, P.pw, bid_);
+    if (P.voxel) {   // the voxel-trap move's pass 1; pass 2 takes the interface queue's place
+        k_move_voxel_body(P.f, P.ma, P.ctl, P.n, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv, P.pw, P.mq, &P.ctl->nmove,
+                          bid_);
         return;
     }
     k_move_easy_body<kPlain>(P.f, P.ma, P.ctl, P.n, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv, P.pw, P.mq, P.mqi, P.flag,
@@ -1889,6 +1942,13 @@ __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void kg_move(const PlanIter*
                                                                  const int32_t* __restrict__ bfirst) {
     constexpr int kMode = kQueue == 1 ? kLeanInterior : (kQueue == 2 ? kLeanInterface : kGeneral);
     TN_PLAN_BLOCK(kMode == kGeneral ? kShGen : kShLean);
+    if (P.voxel) {   // the voxel-trap move's pass 2 (its only queue)
+        if (kQueue == 2) {
+            k_move_voxel_wave_body(P.f, P.ma, P.ctl, P.mq, &P.ctl->nmove, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv,
+                                   P.pw, bid_, nb_);
+        }
+        return;
+    }
     const int32_t* queue = kQueue == 0 ? P.mq2 : (kQueue == 1 ? P.mqi : (kQueue == 2 ? P.mq : P.mqd));
     const int32_t* count = kQueue == 0 ? &P.ctl->nmove_rest
                                        : (kQueue == 1 ? &P.ctl->nmove_int

@@ -558,10 +558,14 @@ struct tn_plan {
         timer.end(KC_FORCE, t);
         t = timer.begin();
         tnk::MoveArgs ma{ prm.dt, prm.maxstep, prm.snap, fire, rest_lanes };
-        if (prm.voxel_trap) {   // the staircase move, every node in one pass
+        if (prm.voxel_trap) {   // the staircase move: a lane per node, then a wave per node that scans
             tnk::k_move_voxel<<<nb, tnk::kBlock, 0, stream>>>(f, ma, dCtl, n, dF, dX, dV, dTyp, dPart, dLab, dMv,
-                                                              dPw);
+                                                              dPw, dMoveQ, &dCtl->nmove);
             timer.end(KC_MOVE_EASY, t);
+            t = timer.begin();
+            tnk::k_move_voxel_wave<<<std::min<unsigned>(nb, tnk::kMoveGrid), tnk::kBlock, 0, stream>>>(
+                f, ma, dCtl, dMoveQ, &dCtl->nmove, dF, dX, dV, dTyp, dPart, dLab, dMv, dPw);
+            timer.end(KC_MOVE, t);
             enqueue_iteration_tail(nb);
             return;
         }
