@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define TN_ABI_VERSION 2
+#define TN_ABI_VERSION 3
 
 /* Sizes fixed by the node and field formats. */
 #define TN_K 24          /* truss neighbours per node: V2M_K, src/opencl/v2m_particle_body.cl:18 */
@@ -48,7 +48,7 @@ enum {
     TN_ERR_ARG = 1,         /* a null pointer, a size mismatch, an out-of-range value */
     TN_ERR_NODEVICE = 2,    /* no HIP device, or the device index is out of range */
     TN_ERR_HIP = 3,         /* a HIP runtime call failed */
-    TN_ERR_UNSUPPORTED = 4, /* a mode this build does not run on the device (seeding a shape grid, gm < 0) */
+    TN_ERR_UNSUPPORTED = 4, /* a mode this build does not run on the device */
     TN_ERR_INTERNAL = 5
 };
 
@@ -76,8 +76,13 @@ typedef struct tn_grid_view {
                                  on the host before use */
     int64_t ngTW;             /* values in gTW (>= 2 gm for a gray-scale grid) */
     int32_t gm;               /* 0: label volume; m > 0: gray-scale volume with m thresholds;
-                                 < 0: shape fields (relaxation only: their seeds start with the
-                                 host's feature nodes, so tn_seed returns TN_ERR_UNSUPPORTED) */
+                                 < 0: shape fields */
+    const float* feat;        /* [nfeat] the shape input's feature curves (Grid::feat: 1 x y z
+                                 corners; 2 x0 y0 z0 x1 y1 z1 segments; 3 cx cy cz nx ny nz r
+                                 circles; 4 n then n points, polylines), or null: seeding adds
+                                 their pinned CORNER nodes (add_feature_nodes,
+                                 src/v2m_particles.cpp:304-620) */
+    int64_t nfeat;
 } tn_grid_view;
 
 /* RelaxParams (src/v2m_particles.h:21-41) plus the rebuild divisor that the
@@ -178,9 +183,12 @@ int tn_relax_batch(const tn_grid_view* grids, const tn_relax_params* prms, tn_no
  * one seed per (seed level, cell of jseed x level spacing, label triple) at the
  * 3-label grid vertices, the lattice nodes within 0.4 h of one dropped, the seeds
  * appended in key order. With prm->corners the fixed CORNER nodes at the 4+-label
- * vertices come last (v2m_corner_vertex). grade[nx*ny*nz] is Grid::grade
- * (src/v2m_grid.h:77); the grid's feature curves (add_feature_nodes) are not part
- * of a tn_grid_view. tn_seed_count gives the seed count; tn_seed writes exactly
+ * vertices come last (v2m_corner_vertex). With g->nfeat > 0 (a shape grid's
+ * feature curves) the seeds within 0.4 h of a feature node are dropped and the
+ * feature nodes, pinned CORNER nodes with the labels round them, appended
+ * (add_feature_nodes, src/v2m_particles.cpp:304-620), as seed_cpu does
+ * (:625-635). grade[nx*ny*nz] is Grid::grade (src/v2m_grid.h:77). tn_seed_count
+ * gives the seed count; tn_seed writes exactly
  * that many nodes into the arrays (P[3n], lab[n], typ[n], part[2n], and part3[n],
  * the third partner of a corner node, unless part3 is null).
  */

@@ -195,10 +195,11 @@ class RefGrid:
             gray = (None, cp(v.gTW, v.ngTW, np.float32), int(v.gm))
         else:
             gray = (None, None, 0)
+        feat = cp(v.feat, v.nfeat, np.float32) if v.nfeat > 0 else None
         return Grid(v.nx, v.ny, v.nz, tuple(v.vs), cp(v.L, nv, np.uint16), cp(v.bl_cnt, nb, np.int32),
                     cp(v.bl_lab, nb * 16, np.uint16), cp(v.bl_slot, nb, np.int32),
                     cp(v.phi, v.nslot * 512, np.float32) if v.nslot else np.zeros(0, np.float32),
-                    cp(v.h, nv, np.float32), float(v.hmin), float(v.hmax), gr, *gray)
+                    cp(v.h, nv, np.float32), float(v.hmin), float(v.hmax), gr, *gray, feat)
 
     def seed(self, prm: RelaxParams) -> Nodes:
         """seed_cpu (src/v2m_particles.cpp:625)."""

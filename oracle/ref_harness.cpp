@@ -406,6 +406,10 @@ void* ref_grid_from_view(const tn_grid_view* v) {
         g.gTW.assign(v->gTW, v->gTW + v->ngTW);
     }
 
+    if (v->feat && v->nfeat > 0) {   // the feature curves
+        g.feat.assign(v->feat, v->feat + v->nfeat);
+    }
+
     return r;
 }
 
@@ -452,6 +456,9 @@ void ref_grid_view(void* h, tn_grid_view* v, const uint8_t** grade, float* hbase
         v->gTW = g.gTW.data();
         v->ngTW = static_cast<int64_t>(g.gTW.size());
     }
+
+    v->feat = g.feat.empty() ? nullptr : g.feat.data();
+    v->nfeat = static_cast<int64_t>(g.feat.size());
 
     if (grade) {
         *grade = g.grade.empty() ? nullptr : g.grade.data();

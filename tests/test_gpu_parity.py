@@ -62,14 +62,14 @@ def test_relax_runs(tn, case):
 def test_seeding(tn, case, variant):
     """Device seeding = the reference's seed_cpu: the lattice alone; with junction-line
     seeds on a 0.4-spacing key grid; with the defaults (junction seeds at 0.8, corners);
-    the lattice in voxel-trap mode (each point takes its voxel's label)."""
+    the lattice in voxel-trap mode (each point takes its voxel's label). A shape grid's
+    seeds end with the pinned feature nodes of its curves (add_feature_nodes,
+    src/v2m_particles.cpp:304-620)."""
     from trussnet_amd._abi import Nodes
 
     d = case.d
-    if case.grid.gm < 0:   # shape grids: the feature nodes are seeded on the host
-        with pytest.raises(tn.UnsupportedMode):
-            tn.seed(case.grid, RelaxParams(), device=0)
-        return
+    if case.grid.gm < 0:   # shape grids: every variant ends with the feature nodes
+        assert case.grid.feat is not None and case.grid.feat.size > 0
     hcp = "hcp_" if "hcp_P" in d else "seed_"   # seed_cpu's own nodes (a case may relax other seeds)
     prm, pre = {"lattice": (RelaxParams(jseed=0.0, corners=False), "lat_"),
                 "junctions": (RelaxParams(jseed=0.4, corners=False), "js_"),
@@ -81,8 +81,9 @@ def test_seeding(tn, case, variant):
     assert got.equal(want), (case.name, variant, int((got.P != want.P).any(axis=1).sum()))
     if variant == "default":
         assert np.array_equal(got.part3, d[pre + "part3"])
-    else:
-        assert (got.part3 == 0xFFFF).all()
+    else:   # (no corner seeds; a shape grid's feature nodes may have a fourth label)
+        plain = got.typ != 3 if case.grid.gm < 0 else np.ones(got.n, bool)
+        assert (got.part3[plain] == 0xFFFF).all()
 
 
 def test_generated_workload_vs_oracle(tn, oracle_lib):
@@ -146,3 +147,13 @@ def test_batch_equals_the_reference_runs(tn, which):
         assert got.equal(want), (name, prm.max_iters, prm.dptol, prm.fire)
         for k, v in stats.items():
             assert np.float32(st[k]) == np.float32(v), (name, prm.max_iters, k, st[k], v)
+
+
+def test_shape_grid_seed_then_relax(tn):
+    """A shape input from its grid alone on the device: seeding (its feature nodes
+    included) then the relaxation, equal to the reference's seed_cpu + relax_cpu."""
+    case = Fixture("shapes40")
+    seeds = tn.seed(case.grid, RelaxParams(), device=0)
+    prm, want, _ = case.runs()[0]
+    got, _ = tn.relax(case.grid, seeds, prm)
+    assert got.equal(want)

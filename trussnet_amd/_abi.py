@@ -37,6 +37,7 @@ class GridView(C.Structure):
         ("phi", _f32p), ("nslot", C.c_int64), ("h", _f32p),
         ("hmin", C.c_float), ("hmax", C.c_float),
         ("gI", _f32p), ("gTW", _f32p), ("ngTW", C.c_int64), ("gm", C.c_int32),
+        ("feat", _f32p), ("nfeat", C.c_int64),
     ]
 
 
@@ -183,7 +184,8 @@ class Grid:
     ``L`` and ``h`` are flat arrays of nx*ny*nz; ``phi`` holds nslot*512 values.
     Gray-scale input (gm = m > 0): ``gI`` is the intensity per voxel and ``gTW`` the m
     ascending thresholds followed by their m membership widths (Grid::gI, gray_w).
-    Shape input (gm < 0): ``gTW`` is the labels' compiled analytic-field program.
+    Shape input (gm < 0): ``gTW`` is the labels' compiled analytic-field program and
+    ``feat`` the feature curves (Grid::feat) whose pinned nodes the seeding adds.
     """
     nx: int
     ny: int
@@ -201,6 +203,7 @@ class Grid:
     gI: np.ndarray | None = None
     gTW: np.ndarray | None = None
     gm: int = 0
+    feat: np.ndarray | None = None
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -231,7 +234,8 @@ class Grid:
                     float(self.hmin), float(self.hmax),
                     None if self.grade is None else np.ascontiguousarray(self.grade, np.uint8).ravel(),
                     None if self.gI is None else np.ascontiguousarray(self.gI, np.float32).ravel(),
-                    None if self.gTW is None else np.ascontiguousarray(self.gTW, np.float32).ravel(), int(self.gm))
+                    None if self.gTW is None else np.ascontiguousarray(self.gTW, np.float32).ravel(), int(self.gm),
+                    None if self.feat is None else np.ascontiguousarray(self.feat, np.float32).ravel())
 
     def check(self) -> None:
         nv = self.nx * self.ny * self.nz
@@ -262,6 +266,8 @@ class Grid:
         v.gTW = _ptr(g.gTW, C.c_float)
         v.ngTW = 0 if g.gTW is None else g.gTW.size
         v.gm = int(g.gm)
+        v.feat = _ptr(g.feat, C.c_float)
+        v.nfeat = 0 if g.feat is None else g.feat.size
         if g is not self:
             self._keep.append(g)
         return v
@@ -269,7 +275,7 @@ class Grid:
     def _is_normal(self) -> bool:
         want = [(self.L, np.uint16), (self.bl_cnt, np.int32), (self.bl_lab, np.uint16), (self.bl_slot, np.int32),
                 (self.phi, np.float32), (self.h, np.float32)]
-        want += [(a, np.float32) for a in (self.gI, self.gTW) if a is not None]
+        want += [(a, np.float32) for a in (self.gI, self.gTW, self.feat) if a is not None]
         return all(a.dtype == t and a.flags["C_CONTIGUOUS"] for a, t in want)
 
     def arrays(self) -> dict:
@@ -280,6 +286,8 @@ class Grid:
             d.update(gTW=self.gTW, gm=np.int32(self.gm))
         if self.gI is not None:
             d["gI"] = self.gI
+        if self.feat is not None:
+            d["feat"] = self.feat
         return d
 
     @staticmethod
@@ -290,8 +298,9 @@ class Grid:
         grade = d[prefix + "grade"] if (prefix + "grade") in d else None
         gm = int(d[prefix + "gm"]) if (prefix + "gm") in d else 0
         gI = d[prefix + "gI"] if gm > 0 else None
+        feat = d[prefix + "feat"] if (prefix + "feat") in d else None
         return Grid(nx, ny, nz, tuple(float(x) for x in g("vs")), g("L"), g("bl_cnt"), g("bl_lab"), g("bl_slot"),
-                    g("phi"), g("h"), hmin, hmax, grade, gI, g("gTW") if gm else None, gm).normalized()
+                    g("phi"), g("h"), hmin, hmax, grade, gI, g("gTW") if gm else None, gm, feat).normalized()
 
 
 @dataclass

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
 #include <thread>
 #include <type_traits>
 #include <chrono>
@@ -1231,18 +1233,303 @@ int64_t junction_pass(float ghmax, const tnd::Field& f, const tnk::SeedArgs& sa,
 Seeds seeding_core(const tnd::Field& f, const uint8_t* d_grade, float hmin, float hmax, const tn_relax_params* prm,
                    DevMem& mem);
 
+// v2m_h_at (src/opencl/v2m_particle_body.cl:30-36) on the host's copy of the sizing
+float host_h_at(const tn_grid_view* g, const float* p) {
+    int i = (int)std::floor(p[0] / g->vs[0] + 0.5f), j = (int)std::floor(p[1] / g->vs[1] + 0.5f),
+        k = (int)std::floor(p[2] / g->vs[2] + 0.5f);
+    i = i < 0 ? 0 : (i >= g->nx ? g->nx - 1 : i);
+    j = j < 0 ? 0 : (j >= g->ny ? g->ny - 1 : j);
+    k = k < 0 ? 0 : (k >= g->nz ? g->nz - 1 : k);
+    return g->h[(size_t)i + (size_t)g->nx * ((size_t)j + (size_t)g->ny * k)];
+}
+
+// add_feature_nodes (src/v2m_particles.cpp:304-620): pinned CORNER nodes on the shape
+// input's sharp features (g->feat: corners, segments, circles, polylines), spaced by
+// the sizing along each curve and kept where the labels round them differ; the seeds
+// within 0.4 h of one are dropped and the kept ones appended. The curve geometry, the
+// bisections' bookkeeping, the label counts and the greedy spacing run on the host
+// (a few thousand points, the reference's float and double arithmetic); the label
+// queries and the seeds' drop test run on the device.
+Seeds add_feature_nodes(const tn_grid_view* g, const tnd::Field& f, const Seeds& s, DevMem& mem) {
+    SeedOps ops{ mem };
+    auto hat = [&](const float* p) { return host_h_at(g, p); };
+    // the masks of a batch of points (k_feat_mask), one device round
+    auto masks = [&](const std::vector<float>& X) {
+        const int m = (int)(X.size() / 3);
+        std::vector<unsigned long long> out(m);
+        if (m == 0) {
+            return out;
+        }
+        DevMem tmp;
+        tmp.device = mem.device;
+        const float* dX = tmp.upload(X.data(), X.size());
+        unsigned long long* dm = tmp.alloc<unsigned long long>(m);
+        tnk::k_feat_mask<<<blocks_for(m, tnk::kBlock), tnk::kBlock>>>(f, dX, m, dm);
+        TN_HIP(hipGetLastError());
+        TN_HIP(hipMemcpy(out.data(), dm, sizeof(unsigned long long) * m, hipMemcpyDeviceToHost));
+        return out;
+    };
+    constexpr int nsamp = 256;
+    struct Curve {
+        int type;
+        const float* q;
+        float u[3], w[3];
+        std::vector<float> pts;
+        void at(float t, float* o) const {   // the curve at parameter t in [0, 1]
+            if (type == 4) {   // a polyline: [n, points], by its points' index
+                const int n = (int)q[0];
+                const float uu = t * (float)(n - 1);
+                const int i = std::min(n - 2, std::max(0, (int)uu));
+                const float fr = uu - (float)i;
+                for (int a = 0; a < 3; ++a) {
+                    o[a] = q[1 + 3 * i + a] + fr * (q[1 + 3 * (i + 1) + a] - q[1 + 3 * i + a]);
+                }
+            } else if (type == 2) {
+                for (int a = 0; a < 3; ++a) {
+                    o[a] = q[a] + t * (q[3 + a] - q[a]);
+                }
+            } else {
+                const float th = 6.28318531f * t, c = std::cos(th), sn = std::sin(th);
+                for (int a = 0; a < 3; ++a) {
+                    o[a] = q[a] + q[6] * (c * u[a] + sn * w[a]);
+                }
+            }
+        }
+    };
+    std::vector<float> fp, tp, rp;   // candidates: corners, then the label changes, then the rest
+    std::vector<Curve> curves;
+    for (int64_t k = 0; k < g->nfeat;) {
+        const int type = (int)g->feat[k];
+        arg_check(type >= 1 && type <= 4, "features: unknown curve type");
+        const float* q = &g->feat[k + 1];
+        const int64_t len = type == 1 ? 4 : type == 2 ? 7 : type == 4 ? 2 + 3 * (int64_t)q[0] : 8;
+        arg_check(k + len <= g->nfeat && (type != 4 || q[0] >= 0), "features: a curve beyond the array");
+        if (type == 1) {
+            fp.insert(fp.end(), { q[0], q[1], q[2] });
+        } else if (type != 4 || q[0] >= 2) {
+            Curve cv{};
+            cv.type = type;
+            cv.q = q;
+            if (type == 3) {   // an orthonormal frame about the circle's normal
+                const float* n = q + 3;
+                const float a0 = std::fabs(n[0]) < 0.9f ? 1.0f : 0.0f, a1 = a0 > 0 ? 0.0f : 1.0f;
+                cv.u[0] = n[1] * 0 - n[2] * a1;
+                cv.u[1] = n[2] * a0 - n[0] * 0;
+                cv.u[2] = n[0] * a1 - n[1] * a0;
+                const float lu = std::sqrt(cv.u[0] * cv.u[0] + cv.u[1] * cv.u[1] + cv.u[2] * cv.u[2]);
+                for (float& x : cv.u) {
+                    x /= lu;
+                }
+                cv.w[0] = n[1] * cv.u[2] - n[2] * cv.u[1];
+                cv.w[1] = n[2] * cv.u[0] - n[0] * cv.u[2];
+                cv.w[2] = n[0] * cv.u[1] - n[1] * cv.u[0];
+            }
+            cv.pts.resize(3 * (nsamp + 1));
+            for (int j = 0; j <= nsamp; ++j) {
+                cv.at((float)j / nsamp, &cv.pts[3 * j]);
+            }
+            curves.push_back(std::move(cv));
+        }
+        k += len;
+    }
+    // every curve's sample masks in one round; then the label changes, each bisected
+    // 14 times, all of them a round at a time
+    std::vector<float> all;
+    for (const Curve& cv : curves) {
+        all.insert(all.end(), cv.pts.begin(), cv.pts.end());
+    }
+    const std::vector<unsigned long long> mk_all = masks(all);
+    struct Change {
+        int curve;
+        unsigned long long before;
+        float lo, hi;
+    };
+    std::vector<std::vector<Change>> changes(curves.size());
+    std::vector<Change> flat;
+    for (size_t c = 0, o = 0; c < curves.size(); ++c, o += nsamp + 1) {
+        for (int j = 1; j <= nsamp; ++j) {
+            if (mk_all[o + j] != mk_all[o + j - 1]) {
+                flat.push_back({ (int)c, mk_all[o + j - 1], (float)(j - 1) / nsamp, (float)j / nsamp });
+            }
+        }
+    }
+    for (int it = 0; it < 14 && !flat.empty(); ++it) {
+        std::vector<float> X(3 * flat.size());
+        for (size_t t = 0; t < flat.size(); ++t) {
+            const float mid = 0.5f * (flat[t].lo + flat[t].hi);
+            curves[flat[t].curve].at(mid, &X[3 * t]);
+        }
+        const std::vector<unsigned long long> mm = masks(X);
+        for (size_t t = 0; t < flat.size(); ++t) {
+            const float mid = 0.5f * (flat[t].lo + flat[t].hi);
+            if (mm[t] == flat[t].before) {
+                flat[t].lo = mid;
+            } else {
+                flat[t].hi = mid;
+            }
+        }
+    }
+    for (const Change& ch : flat) {
+        changes[ch.curve].push_back(ch);
+    }
+    for (size_t c = 0; c < curves.size(); ++c) {
+        const Curve& cv = curves[c];
+        for (const Change& ch : changes[c]) {   // pinned first: an edge crossing an interface
+            float x[3];
+            cv.at(0.5f * (ch.lo + ch.hi), x);
+            tp.insert(tp.end(), x, x + 3);
+        }
+        // arc length in units of h: n = round(total) segments (>= 1; a circle >= 6)
+        std::vector<double> acc(nsamp + 1, 0.0);
+        for (int j = 1; j <= nsamp; ++j) {
+            const float *a = &cv.pts[3 * (j - 1)], *b = &cv.pts[3 * j];
+            const float ds = std::sqrt((b[0] - a[0]) * (b[0] - a[0]) + (b[1] - a[1]) * (b[1] - a[1]) +
+                                       (b[2] - a[2]) * (b[2] - a[2]));
+            const float m[3] = { 0.5f * (a[0] + b[0]), 0.5f * (a[1] + b[1]), 0.5f * (a[2] + b[2]) };
+            acc[j] = acc[j - 1] + ds / std::max(hat(m), 1e-6f);
+        }
+        const double per = cv.type == 4 ? 2.0 : 1.0;   // (a polyline twice as dense)
+        const int ns = std::max(cv.type == 3 ? 6 : 1, (int)std::lround(per * acc[nsamp]));
+        for (int sg = cv.type == 3 || cv.type == 4 ? 0 : 1; sg < ns; ++sg) {
+            const double target = acc[nsamp] * sg / ns;
+            int j = 1;
+            while (j < nsamp && acc[j] < target) {
+                ++j;
+            }
+            const double fr = acc[j] > acc[j - 1] ? (target - acc[j - 1]) / (acc[j] - acc[j - 1]) : 0.0;
+            for (int a = 0; a < 3; ++a) {
+                rp.push_back((float)(cv.pts[3 * (j - 1) + a] + fr * (cv.pts[3 * j + a] - cv.pts[3 * (j - 1) + a])));
+            }
+        }
+    }
+    fp.insert(fp.end(), tp.begin(), tp.end());
+    fp.insert(fp.end(), rp.begin(), rp.end());
+    // the labels round each candidate: the 14 axis and diagonal directions and 128 on a
+    // golden spiral, at 0.25 h
+    std::vector<float> dirs = { 1, 0, 0, -1, 0, 0, 0, 1, 0, 0, -1, 0, 0, 0, 1, 0, 0, -1,
+                                .577f, .577f, .577f, -.577f, .577f, .577f, .577f, -.577f, .577f, .577f, .577f, -.577f,
+                                -.577f, -.577f, .577f, -.577f, .577f, -.577f, .577f, -.577f, -.577f, -.577f, -.577f, -.577f };
+    for (int i = 0; i < 128; ++i) {
+        const double zc = 1 - (2 * i + 1) / 128.0, rr = std::sqrt(std::max(0.0, 1 - zc * zc)), ph = 2.39996322972865 * i;
+        dirs.insert(dirs.end(), { (float)(rr * std::cos(ph)), (float)(rr * std::sin(ph)), (float)zc });
+    }
+    const int nd = (int)(dirs.size() / 3), nc = (int)(fp.size() / 3);
+    std::vector<int32_t> labs((size_t)nc * nd);
+    if (nc > 0) {
+        DevMem tmp;
+        tmp.device = mem.device;
+        const float* dX = tmp.upload(fp.data(), fp.size());
+        const float* dD = tmp.upload(dirs.data(), dirs.size());
+        int32_t* dL = tmp.alloc<int32_t>((size_t)nc * nd);
+        tnk::k_feat_labels<<<blocks_for((int64_t)nc * nd, tnk::kBlock), tnk::kBlock>>>(f, dX, nc, dD, nd, dL);
+        TN_HIP(hipGetLastError());
+        TN_HIP(hipMemcpy(labs.data(), dL, sizeof(int32_t) * labs.size(), hipMemcpyDeviceToHost));
+    }
+    std::vector<std::array<int, 4>> ls(nc);
+    std::vector<int> nls(nc, 0);
+    const float xmax = (g->nx - 1) * g->vs[0], ymax = (g->ny - 1) * g->vs[1], zmax = (g->nz - 1) * g->vs[2];
+    for (int c = 0; c < nc; ++c) {
+        const float* x = &fp[3 * (size_t)c];
+        if (x[0] < 0 || x[1] < 0 || x[2] < 0 || x[0] > xmax || x[1] > ymax || x[2] > zmax) {
+            continue;
+        }
+        std::map<int, int> cnt;
+        for (int d = 0; d < nd; ++d) {
+            ++cnt[labs[(size_t)c * nd + d]];
+        }
+        if (cnt.size() < 2) {
+            continue;   // hidden: inside one region
+        }
+        std::vector<std::pair<int, int>> by(cnt.begin(), cnt.end());
+        std::sort(by.begin(), by.end(), [](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+            return a.second > b.second || (a.second == b.second && a.first < b.first);
+        });
+        // (the node's own label a tissue one: 0, the exterior, only as a partner)
+        std::stable_partition(by.begin(), by.end(), [](const std::pair<int, int>& a) { return a.first != 0; });
+        int m = 0;
+        for (const auto& pr : by) {
+            if (m < 4) {
+                ls[c][m++] = pr.first;
+            }
+        }
+        nls[c] = (m > 0 && ls[c][0] == 0) ? 0 : m;
+    }
+    // keep one per 0.3 h, in candidate order (corners first)
+    std::vector<float> keepP;
+    std::vector<std::array<int, 4>> keepL;
+    std::vector<int> keepN;
+    for (int c = 0; c < nc; ++c) {
+        if (nls[c] < 2) {
+            continue;
+        }
+        const float* x = &fp[3 * (size_t)c];
+        const float r = 0.3f * hat(x);
+        bool near = false;
+        for (size_t k = 0; k < keepP.size() / 3 && !near; ++k) {
+            const float dx = keepP[3 * k] - x[0], dy = keepP[3 * k + 1] - x[1], dz = keepP[3 * k + 2] - x[2];
+            near = dx * dx + dy * dy + dz * dz < r * r;
+        }
+        if (!near) {
+            keepP.insert(keepP.end(), x, x + 3);
+            keepL.push_back(ls[c]);
+            keepN.push_back(nls[c]);
+        }
+    }
+    const int nk = (int)(keepP.size() / 3);
+    if (nk == 0) {
+        return s;
+    }
+    // the seeds that stay, in order, then the feature nodes
+    const int n0 = (int)s.n;
+    int32_t* keep = ops.counts(n0);
+    const float* dK = mem.upload(keepP.data(), keepP.size());
+    if (n0 > 0) {
+        tnk::k_feat_drop<<<blocks_for(n0, tnk::kBlock), tnk::kBlock>>>(f, n0, s.P, dK, nk, keep);
+        TN_HIP(hipGetLastError());
+    }
+    const int m = ops.scan(keep, n0);
+    Seeds o;
+    o.n = (int64_t)m + nk;
+    o.P = mem.alloc<float>((size_t)3 * o.n);
+    o.lab = mem.alloc<uint16_t>((size_t)o.n);
+    o.typ = mem.alloc<uint8_t>((size_t)o.n);
+    o.part = mem.alloc<uint16_t>((size_t)2 * o.n);
+    o.part3 = mem.alloc<uint16_t>((size_t)o.n);
+    if (n0 > 0) {
+        tnthin::k_compact_nodes<<<blocks_for(n0, tnk::kBlock), tnk::kBlock>>>(keep, n0, s.P, s.lab, s.typ, s.part,
+                                                                              s.part3, o.P, o.lab, o.typ, o.part,
+                                                                              o.part3);
+        TN_HIP(hipGetLastError());
+    }
+    std::vector<uint16_t> flab(nk), fpart(2 * (size_t)nk), fpart3(nk);
+    std::vector<uint8_t> ftyp(nk, TN_CORNER);
+    for (int k = 0; k < nk; ++k) {
+        const std::array<int, 4>& L = keepL[k];
+        const int mm = keepN[k];
+        flab[k] = (uint16_t)L[0];
+        fpart[2 * k] = (uint16_t)(mm > 1 ? L[1] : TN_NOLAB);
+        fpart[2 * k + 1] = (uint16_t)(mm > 2 ? L[2] : TN_NOLAB);
+        fpart3[k] = (uint16_t)(mm > 3 ? L[3] : TN_NOLAB);
+    }
+    TN_HIP(hipMemcpy(o.P + 3 * (size_t)m, keepP.data(), sizeof(float) * keepP.size(), hipMemcpyHostToDevice));
+    TN_HIP(hipMemcpy(o.lab + m, flab.data(), sizeof(uint16_t) * nk, hipMemcpyHostToDevice));
+    TN_HIP(hipMemcpy(o.typ + m, ftyp.data(), nk, hipMemcpyHostToDevice));
+    TN_HIP(hipMemcpy(o.part + 2 * (size_t)m, fpart.data(), sizeof(uint16_t) * 2 * nk, hipMemcpyHostToDevice));
+    TN_HIP(hipMemcpy(o.part3 + m, fpart3.data(), sizeof(uint16_t) * nk, hipMemcpyHostToDevice));
+    return o;
+}
+
 Seeds run_seeding(const tn_grid_view* g, const uint8_t* grade, const tn_relax_params* prm, int dev, DevMem& mem) {
     check_grid(g);
-    if (g->gm < 0) {
-        throw TnError(TN_ERR_UNSUPPORTED, "seeding a shape grid (gm < 0): its seeds start with the pinned feature "
-                                          "nodes (add_feature_nodes), which are made on the host");
-    }
     arg_check(prm != nullptr && grade != nullptr, "seeding: null argument");
+    arg_check(g->nfeat >= 0 && (g->nfeat == 0 || g->feat), "grid: feat is null");
     TN_HIP(hipSetDevice(dev));
     mem.device = dev;
     const tnd::Field f = upload_grid(g, mem);
     const uint8_t* d_grade = mem.upload(grade, (size_t)g->nx * g->ny * g->nz);
-    return seeding_core(f, d_grade, g->hmin, g->hmax, prm, mem);
+    const Seeds s = seeding_core(f, d_grade, g->hmin, g->hmax, prm, mem);
+    // seed_cpu (src/v2m_particles.cpp:625-635): the shape input's feature nodes last
+    return g->nfeat > 0 ? add_feature_nodes(g, f, s, mem) : s;
 }
 
 // seed_cpu on a grid already on the device (f, its grades d_grade), scratch in mem

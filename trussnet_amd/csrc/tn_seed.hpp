@@ -522,6 +522,63 @@ __global__ __launch_bounds__(kBlock) void k_drop_near_seeds(Field f, int n, cons
     }
 }
 
+// ---- the feature nodes of a shape input (add_feature_nodes, src/v2m_particles.cpp:
+// 304-620). The host samples the curves and keeps the candidates; the device answers
+// its label queries and drops the seeds near the kept ones. ----------------------------
+
+// the labels round each point, as bits: the point and 6 samples at 0.2 h
+// (add_feature_nodes' mask_at, src/v2m_particles.cpp:312-326)
+__global__ __launch_bounds__(kBlock) void k_feat_mask(Field f, const float* __restrict__ X, int m,
+                                                      unsigned long long* __restrict__ mask) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m) {
+        return;
+    }
+    const float x = X[3 * (size_t)i], y = X[3 * (size_t)i + 1], z = X[3 * (size_t)i + 2];
+    const float dir[7][3] = { { 0, 0, 0 }, { 1, 0, 0 }, { -1, 0, 0 }, { 0, 1, 0 }, { 0, -1, 0 }, { 0, 0, 1 }, { 0, 0, -1 } };
+    const float e = 0.2f * size_at(f, x, y, z);
+    unsigned long long bits = 0;
+    for (int s = 0; s < 7; ++s) {
+        const int l = owner_at(f, p3(x + e * dir[s][0], y + e * dir[s][1], z + e * dir[s][2])).x;
+        bits |= l < 64 ? 1ull << l : 0ull;
+    }
+    mask[i] = bits;
+}
+
+// the label at each of nd samples at 0.25 h round each candidate
+// (src/v2m_particles.cpp:482-494); one thread per (candidate, direction)
+__global__ __launch_bounds__(kBlock) void k_feat_labels(Field f, const float* __restrict__ X, int nc,
+                                                        const float* __restrict__ dirs, int nd,
+                                                        int32_t* __restrict__ lab) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= (int64_t)nc * nd) {
+        return;
+    }
+    const int c = (int)(t / nd), d = (int)(t % nd);
+    const float x = X[3 * (size_t)c], y = X[3 * (size_t)c + 1], z = X[3 * (size_t)c + 2];
+    const float e = 0.25f * size_at(f, x, y, z);
+    lab[t] = owner_at(f, p3(x + e * dirs[3 * d], y + e * dirs[3 * d + 1], z + e * dirs[3 * d + 2])).x;
+}
+
+// keep[i] = 0 for a seed closer than 0.4 h to a feature node (src/v2m_particles.cpp:
+// 563-589; the reference's hash only narrows the search to the cells within reach)
+__global__ __launch_bounds__(kBlock) void k_feat_drop(Field f, int n, const float* __restrict__ P,
+                                                      const float* __restrict__ K, int nk,
+                                                      int32_t* __restrict__ keep) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) {
+        return;
+    }
+    const float x = P[3 * (size_t)i], y = P[3 * (size_t)i + 1], z = P[3 * (size_t)i + 2];
+    const float r = 0.4f * size_at(f, x, y, z);
+    int k = 1;
+    for (int j = 0; j < nk && k; ++j) {
+        const float dx = K[3 * (size_t)j] - x, dy = K[3 * (size_t)j + 1] - y, dz = K[3 * (size_t)j + 2] - z;
+        k = dx * dx + dy * dy + dz * dz < r * r ? 0 : 1;
+    }
+    keep[i] = k;
+}
+
 // The node arrays of a seeding (part3: the third partner of a corner node).
 struct SeedNodes {
     float* P;

@@ -43,6 +43,8 @@ tn_grid_view grid_view(const Grid& g) {
     v.gTW = g.gTW.empty() ? nullptr : g.gTW.data();
     v.ngTW = (int64_t)g.gTW.size();
     v.gm = g.gm;
+    v.feat = g.feat.empty() ? nullptr : g.feat.data();
+    v.nfeat = (int64_t)g.feat.size();
     return v;
 }
 
