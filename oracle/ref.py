@@ -77,6 +77,9 @@ def lib():
         L.ref_protect_features.argtypes = [vp, C.c_int64] + [C.c_void_p] * 5
         L.ref_grid_feat.restype = C.c_int64
         L.ref_grid_feat.argtypes = [vp, C.c_void_p]
+        L.ref_tet_quality.argtypes = [C.c_void_p, C.c_void_p]
+        L.ref_delaunay.restype = C.c_int64
+        L.ref_delaunay.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]
         L.ref_classify_points.restype = C.c_int64
         L.ref_classify_points.argtypes = [vp, C.c_int64] + [C.c_void_p] * 5
         _lib = L
@@ -309,3 +312,25 @@ class RefGrid:
         sec = np.zeros(m, np.int32)
         lib().ref_field_points(self.h, a, b, m, _p(X), _p(phi), _p(psi), _p(grad), _p(lab), _p(sec))
         return phi, psi, grad, lab, sec
+
+
+def tet_quality(p) -> tuple[float, float, float]:
+    """tet_quality (src/v2m_tetra.cpp:100-150) of a tet (4 x 3 corners): the minimum
+    dihedral angle in degrees, the Joe-Liu quality and the volume."""
+    q = np.ascontiguousarray(p, np.float64).reshape(12)
+    out = np.zeros(3, np.float64)
+    lib().ref_tet_quality(_p(q), _p(out))
+    return float(out[0]), float(out[1]), float(out[2])
+
+
+def delaunay(X) -> np.ndarray:
+    """TetMesh::tetrahedrize of the points X (n x 3), canonicalized as the reference's
+    pipeline does (tn::canonicalize_tets): the finite tets, (m, 4) vertex indices."""
+    X = np.ascontiguousarray(X, np.float64).reshape(-1, 3)
+    m = lib().ref_delaunay(_p(X), X.shape[0], None, 0)
+    if m < 0:
+        raise RuntimeError("reference delaunay failed")
+    T = np.zeros((m, 4), np.int32)
+    lib().ref_delaunay(_p(X), X.shape[0], _p(T), m)
+    return T
+

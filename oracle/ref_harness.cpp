@@ -22,8 +22,11 @@
 #include <string>
 #include <vector>
 
+#include "delaunay.h"
+#include "v2m_gdel.h"
 #include "v2m_grid.h"
 #include "v2m_particles.h"
+#include "v2m_tetra.h"
 #include "v2m_sdfshape.h"
 #include "v2m_tpm.h"
 #include "v2m_volume.h"
@@ -759,6 +762,41 @@ void ref_field_points(void* h, int a, int b, int64_t m, const float* X, float* p
         label[k] = refh::v2m_label_of(d, g.L->data(), g.bl_cnt.data(), g.bl_lab.data(), g.bl_slot.data(), g.phi.data(),
                                       g.gI, g.gTW.data(), g.gm, 0, x[0], x[1], x[2], &sec, &mg);
         second[k] = sec;
+    }
+}
+
+// tet_quality (src/v2m_tetra.cpp:100-150) of the tet p[12] (its 4 corners): out[3] =
+// (the minimum dihedral in degrees, the Joe-Liu quality, the volume)
+void ref_tet_quality(const double* p, double* out) {
+    const double* q[4] = { p, p + 3, p + 6, p + 9 };
+    tn::tet_quality(q, out[0], out[1], out[2]);
+}
+
+// TetMesh::tetrahedrize (third_party/cdt/src/delaunay.cpp) of n points X[3n], in the
+// canonical form of tn::canonicalize_tets (src/v2m_tetra.cpp:3928): the finite tets'
+// corners, 4 each, into tets when cap >= their count. Returns the finite tet count.
+int64_t ref_delaunay(const double* X, int64_t n, int32_t* tets, int64_t cap) {
+    try {
+        ::TetMesh tm;
+        tm.init_vertices(X, static_cast<uint32_t>(n));
+        tm.tetrahedrize();
+        tn::canonicalize_tets(tm);
+        int64_t m = 0;
+        for (uint64_t t = 0; t < tm.numTets(); ++t) {
+            const uint32_t* v = &tm.tet_node[4 * t];
+            if (v[3] == INFINITE_VERTEX) {
+                continue;
+            }
+            if (tets && m < cap) {
+                for (int k = 0; k < 4; ++k) {
+                    tets[4 * m + k] = static_cast<int32_t>(v[k]);
+                }
+            }
+            ++m;
+        }
+        return m;
+    } catch (...) {
+        return -1;
     }
 }
 

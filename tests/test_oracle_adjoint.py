@@ -101,3 +101,20 @@ def test_quality_term_closed_forms(oracle_lib):
     both = np.concatenate([reg, corner + 10.0])
     _, t5 = quality_loss(orc, both, [[0, 1, 2, 3], [4, 5, 6, 7]])
     assert t5[2] == pytest.approx(0.5 * t3[2], rel=1e-12)
+
+
+def test_quality_matches_the_reference_tet_quality(oracle_lib):
+    """The quality term against the reference's own tet_quality (src/v2m_tetra.cpp:
+    100-150), compiled from its sources: 1 - q per tet equals 1 - its Joe-Liu quality
+    for random tets of every shape, as eps_v -> 0."""
+    from oracle import ref
+
+    if not ref.available():
+        pytest.skip("the reference's sources are not built here (oracle/_ref)")
+    rng = np.random.default_rng(11)
+    for k in range(40):
+        pts = rng.standard_normal((4, 3)) * rng.uniform(0.2, 5.0, size=3) + rng.uniform(-10, 10, size=3)
+        _, terms = quality_loss(oracle_lib, pts, [[0, 1, 2, 3]], eps_v=1e-12)
+        _, jl, vol = ref.tet_quality(pts)
+        assert vol > 1e-6
+        assert terms[2] == pytest.approx(1.0 - jl, rel=1e-9, abs=1e-12), (k, terms[2], 1.0 - jl)
