@@ -66,6 +66,6 @@ def test_autograd_function(torch_dev):
     L = pb.forward_ptr(xd.detach().data_ptr(), td.detach().data_ptr())
     pb.backward_ptr(td.detach().data_ptr(), gx.data_ptr(), gt.data_ptr())
     torch.cuda.synchronize()
-    assert float(loss) == L
+    assert float(loss.detach()) == L
     assert torch.equal(xd.grad, 2.0 * gx) and torch.equal(td.grad, 2.0 * gt)
     pb.close()
