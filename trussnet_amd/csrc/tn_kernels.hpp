@@ -480,6 +480,104 @@ __device__ __forceinline__ void settle_keys(const uint64_t* cb, int lane, int cn
     bitonic32(k);
 }
 
+// ---- settling a few lanes' buffers with the whole wave -----------------------------------
+//
+// When only some lanes' buffers are full, the wave sorts them two at a time, one per
+// half-wave: lane i of a half holds entry i of its buffer and a bitonic network runs
+// across the 32 lanes, each compare-exchange with partner lane i ^ j taken through
+// DPP (quad_perm for j = 1, 2; row_ror:4 and row_ror:12 for j = 4; row_ror:8 for
+// j = 8) or ds_swizzle's bit mode (j = 16, xor within 32 lanes). 15 steps of a few
+// instructions against a 32-key register network per settling lane.
+
+template <int kCtrl>
+__device__ __forceinline__ uint64_t dpp64(uint64_t v) {
+    const int lo = (int)(uint32_t)v, hi = (int)(uint32_t)(v >> 32);
+    return ((uint64_t)(uint32_t)dpp<kCtrl>(hi, hi) << 32) | (uint32_t)dpp<kCtrl>(lo, lo);
+}
+
+// the value of lane (i ^ J) of this lane's half-wave, i = lane % 32
+template <int J>
+__device__ __forceinline__ uint64_t xor_partner(uint64_t v, int i) {
+    if constexpr (J == 1) {
+        return dpp64<0xB1>(v);
+    } else if constexpr (J == 2) {
+        return dpp64<0x4E>(v);
+    } else if constexpr (J == 4) {
+        const uint64_t dn = dpp64<0x124>(v), up = dpp64<0x12C>(v);
+        return (i & 4) ? dn : up;
+    } else if constexpr (J == 8) {
+        return dpp64<0x128>(v);
+    } else {
+        static_assert(J == 16, "xor partners within 32 lanes");
+        const int lo = __builtin_amdgcn_ds_swizzle((int)(uint32_t)v, 0x401F);
+        const int hi = __builtin_amdgcn_ds_swizzle((int)(uint32_t)(v >> 32), 0x401F);
+        return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+    }
+}
+
+template <int K, int J>
+__device__ __forceinline__ uint64_t half_bitonic_step(uint64_t v, int i) {
+    const uint64_t o = xor_partner<J>(v, i);
+    const bool keep_min = ((i & J) == 0) == (K >= 32 || (i & K) == 0);
+    const bool lt = o < v;
+    return (keep_min == lt) ? o : v;
+}
+
+// each half-wave's 32 values (one a lane) in increasing order along its lanes
+__device__ __forceinline__ uint64_t half_wave_sort32(uint64_t v, int lane) {
+    const int i = lane & 31;
+    v = half_bitonic_step<2, 1>(v, i);
+    v = half_bitonic_step<4, 2>(v, i);
+    v = half_bitonic_step<4, 1>(v, i);
+    v = half_bitonic_step<8, 4>(v, i);
+    v = half_bitonic_step<8, 2>(v, i);
+    v = half_bitonic_step<8, 1>(v, i);
+    v = half_bitonic_step<16, 8>(v, i);
+    v = half_bitonic_step<16, 4>(v, i);
+    v = half_bitonic_step<16, 2>(v, i);
+    v = half_bitonic_step<16, 1>(v, i);
+    v = half_bitonic_step<32, 16>(v, i);
+    v = half_bitonic_step<32, 8>(v, i);
+    v = half_bitonic_step<32, 4>(v, i);
+    v = half_bitonic_step<32, 2>(v, i);
+    v = half_bitonic_step<32, 1>(v, i);
+    return v;
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Cut the buffers of the lanes in sm (wave-uniform) back to their 24 best, in order;
+// their cnt becomes 24 and kth the 24th key. All lanes take part.
+__device__ __forceinline__ void settle_lanes(uint64_t* cb, int lane, uint64_t sm, int& cnt, uint64_t& kth) {
+    while (sm) {
+        const int a = __builtin_ctzll(sm);
+        sm &= sm - 1;
+        const int b = sm ? __builtin_ctzll(sm) : a;   // one lane left: both halves sort it
+        sm &= sm ? sm - 1 : 0ull;
+        const int ca = __builtin_amdgcn_readlane(cnt, a), cb_n = __builtin_amdgcn_readlane(cnt, b);
+        const int who = lane < 32 ? a : b, cw = lane < 32 ? ca : cb_n;
+        const int i = lane & 31;
+        uint64_t v = i < cw ? cb[i * kTrussWave + who] : ~0ull;
+        v = half_wave_sort32(v, lane);
+        if (i < kK) {
+            cb[i * kTrussWave + who] = v;
+        }
+        const uint64_t ka = readlane64(v, kK - 1), kb = readlane64(v, 32 + kK - 1);
+        if (lane == a) {
+            cnt = kK;
+            kth = ka;
+        }
+        if (lane == b) {
+            cnt = kK;
+            kth = kb;
+        }
+    }
+}
+
 // ---- the distance prefilter on the matrix cores ------------------------------------------
 //
 // For query p (this lane's node) and candidate c, with both taken relative to a
@@ -742,15 +840,9 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                     while (__ballot(hit != 0ull)) {
                         const int pc = __popcll(hit);
                         // little room for them: keep the 24 best so far
-                        if (pc > kCollect - cnt && kCollect - cnt < kMinRoom) {
-                            uint64_t kk[32];
-                            settle_keys(cb, lane, cnt, kk);
-#pragma unroll
-                            for (int e = 0; e < kK; ++e) {
-                                cb[e * kTrussWave + lane] = kk[e];
-                            }
-                            cnt = kK;
-                            kth = kk[kK - 1];
+                        const uint64_t sm = __ballot(pc > kCollect - cnt && kCollect - cnt < kMinRoom);
+                        if (sm) {
+                            settle_lanes(cb, lane, sm, cnt, kth);
                         }
                         const int want = min(pc, kCollect - cnt);
                         int npair = 0;
