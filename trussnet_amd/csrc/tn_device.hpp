@@ -1907,45 +1907,62 @@ __device__ float voxel_step(const Field& f, float h, float snap, int a, const fl
             ty = TN_INTERFACE;
         } else if (snap > 0.0f) {   // close to any wall: snap onto it
             const int Rs = 1 + (int)ceilf(snap * h / vmin);
+            const int qx = (int)floorf(over_vx(f, q[0]) + 0.5f), qy = (int)floorf(over_vy(f, q[1]) + 0.5f),
+                      qz = (int)floorf(over_vz(f, q[2]) + 0.5f);
+            // the label Rs voxels away in direction d (dz, dy, dx = d / 9, d / 3 % 3, d % 3, less 1)
+            auto dir_label = [&](int d) {
+                return label_or_zero(f, qx + (d % 3 - 1) * Rs, qy + (d / 3 % 3 - 1) * Rs, qz + (d / 9 - 1) * Rs);
+            };
             float best = -1.0f, y[3];
-            // A label met in several directions is scanned once: the scan's result depends
-            // on the label only, and a repeat can never beat the best so far (it ties or
-            // was rejected), so skipping it keeps the reference's choice.
-            constexpr int kSeen = 4;
-            int seen[kSeen] = { 0, 0, 0, 0 }, nseen = 0;
-            for (int dz = -1; dz <= 1; ++dz) {
-                for (int dy = -1; dy <= 1; ++dy) {
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        const int l2 = label_or_zero(f, (int)floorf(over_vx(f, q[0]) + 0.5f) + dx * Rs,
-                                                     (int)floorf(over_vy(f, q[1]) + 0.5f) + dy * Rs,
-                                                     (int)floorf(over_vz(f, q[2]) + 0.5f) + dz * Rs);
-                        if (l2 == a || l2 == b) {
-                            continue;
-                        }
-                        bool again = false;
+            if constexpr (!kWave && (kDefer & kDeferSnap)) {
+                // the lane pass hands the node on at the first label other than a and b;
+                // its 27 reads are issued at once
+                bool other = false;
 #pragma unroll
-                        for (int m = 0; m < kSeen; ++m) {   // (static indices: the list stays in registers)
-                            again = again || (m < nseen && seen[m] == l2);
-                        }
-                        if (again) {
-                            continue;
-                        }
+                for (int d = 0; d < 27; ++d) {
+                    const int l2 = dir_label(d);
+                    other = other | ((l2 != a) & (l2 != b));
+                }
+                if (other) {
+                    return kDeferred;
+                }
+            } else {
+                // the wave reads the 27 labels at once, one a lane
+                int lab27 = 0;
+                if constexpr (kWave) {
+                    const int lane = threadIdx.x & 63;
+                    lab27 = dir_label(lane < 27 ? lane : 0);
+                }
+                // A label met in several directions is scanned once: the scan's result depends
+                // on the label only, and a repeat can never beat the best so far (it ties or
+                // was rejected), so skipping it keeps the reference's choice.
+                constexpr int kSeen = 4;
+                int seen[kSeen] = { 0, 0, 0, 0 }, nseen = 0;
+                for (int d = 0; d < 27; ++d) {
+                    const int l2 = kWave ? __builtin_amdgcn_readlane(lab27, d) : dir_label(d);
+                    if (l2 == a || l2 == b) {
+                        continue;
+                    }
+                    bool again = false;
 #pragma unroll
-                        for (int m = 0; m < kSeen; ++m) {
-                            seen[m] = m == nseen ? l2 : seen[m];
-                        }
-                        nseen += nseen < kSeen ? 1 : 0;
-                        if (!kWave && (kDefer & kDeferSnap)) {
-                            return kDeferred;
-                        }
-                        const float d2 = nearest(l2, kNoLab, q, Rs, y);
-                        if (d2 >= 0.0f && d2 < snap * snap * h * h && (best < 0.0f || d2 < best)) {
-                            best = d2;
-                            b = l2;
-                            x[0] = y[0];
-                            x[1] = y[1];
-                            x[2] = y[2];
-                        }
+                    for (int m = 0; m < kSeen; ++m) {   // (static indices: the list stays in registers)
+                        again = again || (m < nseen && seen[m] == l2);
+                    }
+                    if (again) {
+                        continue;
+                    }
+#pragma unroll
+                    for (int m = 0; m < kSeen; ++m) {
+                        seen[m] = m == nseen ? l2 : seen[m];
+                    }
+                    nseen += nseen < kSeen ? 1 : 0;
+                    const float d2 = nearest(l2, kNoLab, q, Rs, y);
+                    if (d2 >= 0.0f && d2 < snap * snap * h * h && (best < 0.0f || d2 < best)) {
+                        best = d2;
+                        b = l2;
+                        x[0] = y[0];
+                        x[1] = y[1];
+                        x[2] = y[2];
                     }
                 }
             }
