@@ -619,16 +619,13 @@ __device__ __forceinline__ QueryOps query_ops(float qx, float qy, float qz, floa
 }
 
 // The tests of one output tile, dense: bit e for register e (its difference below the
-// margin). Each register costs one compare into VCC and one add-with-carry that
-// doubles the mask and shifts VCC in (the registers are taken from e = 15 down).
+// margin). Plain compares, so that the compiler keeps the wait states the matrix
+// cores' results need before a vector instruction may read them.
 __device__ __forceinline__ uint32_t tile_bits(const f32x16& acc, float margin) {
     uint32_t m = 0;
 #pragma unroll
-    for (int e = 15; e >= 0; --e) {
-        asm("v_cmp_lt_f32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc"
-            : "+v"(m)
-            : "v"(acc[e]), "v"(margin)
-            : "vcc");
+    for (int e = 0; e < 16; ++e) {
+        m |= (acc[e] < margin ? 1u : 0u) << e;
     }
     return m;
 }
