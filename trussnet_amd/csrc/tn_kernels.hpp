@@ -470,24 +470,15 @@ __device__ __forceinline__ void bitonic32(uint64_t (&k)[32]) {
     }
 }
 
-// A lane's buffer (entries e*64 + lane, kCollect == 32 of them; [cnt, 32) unused, read
-// as the largest key) settled into k, in order.
-__device__ __forceinline__ void settle_keys(const uint64_t* cb, int lane, int cnt, uint64_t (&k)[32]) {
-#pragma unroll
-    for (int e = 0; e < 32; ++e) {
-        k[e] = e < cnt ? cb[e * kTrussWave + lane] : ~0ull;
-    }
-    bitonic32(k);
-}
-
 // ---- settling a few lanes' buffers with the whole wave -----------------------------------
 //
 // When only some lanes' buffers are full, the wave sorts them two at a time, one per
-// half-wave: lane i of a half holds entry i of its buffer and a bitonic network runs
-// across the 32 lanes, each compare-exchange with partner lane i ^ j taken through
-// DPP (quad_perm for j = 1, 2; row_ror:4 and row_ror:12 for j = 4; row_ror:8 for
-// j = 8) or ds_swizzle's bit mode (j = 16, xor within 32 lanes). 15 steps of a few
-// instructions against a 32-key register network per settling lane.
+// half-wave: lane i of a half holds entry i of its buffer (the key, with the slot it
+// came from) and a bitonic network runs across the 32 lanes, each compare-exchange
+// with partner lane i ^ j taken through DPP (quad_perm for j = 1, 2; row_ror:4 and
+// row_ror:12 for j = 4; row_ror:8 for j = 8) or ds_swizzle's bit mode (j = 16, xor
+// within 32 lanes). 15 steps of a few instructions against a 32-key register network
+// per settling lane.
 
 template <int kCtrl>
 __device__ __forceinline__ uint64_t dpp64(uint64_t v) {
@@ -515,33 +506,51 @@ __device__ __forceinline__ uint64_t xor_partner(uint64_t v, int i) {
     }
 }
 
-template <int K, int J>
-__device__ __forceinline__ uint64_t half_bitonic_step(uint64_t v, int i) {
-    const uint64_t o = xor_partner<J>(v, i);
-    const bool keep_min = ((i & J) == 0) == (K >= 32 || (i & K) == 0);
-    const bool lt = o < v;
-    return (keep_min == lt) ? o : v;
+// the 32-bit value of lane (i ^ J) of this lane's half-wave
+template <int J>
+__device__ __forceinline__ int xor_partner32(int v, int i) {
+    if constexpr (J == 1) {
+        return dpp<0xB1>(v, v);
+    } else if constexpr (J == 2) {
+        return dpp<0x4E>(v, v);
+    } else if constexpr (J == 4) {
+        const int dn = dpp<0x124>(v, v), up = dpp<0x12C>(v, v);
+        return (i & 4) ? dn : up;
+    } else if constexpr (J == 8) {
+        return dpp<0x128>(v, v);
+    } else {
+        return __builtin_amdgcn_ds_swizzle(v, 0x401F);
+    }
 }
 
-// each half-wave's 32 values (one a lane) in increasing order along its lanes
-__device__ __forceinline__ uint64_t half_wave_sort32(uint64_t v, int lane) {
+// one step of the network on (key, payload); keys are distinct or both padding
+template <int K, int J>
+__device__ __forceinline__ void half_bitonic_step_kv(uint64_t& v, int& pl, int i) {
+    const uint64_t o = xor_partner<J>(v, i);
+    const int op = xor_partner32<J>(pl, i);
+    const bool keep_min = ((i & J) == 0) == (K >= 32 || (i & K) == 0);
+    const bool take = keep_min == (o < v);
+    v = take ? o : v;
+    pl = take ? op : pl;
+}
+
+__device__ __forceinline__ void half_wave_sort32_kv(uint64_t& v, int& pl, int lane) {
     const int i = lane & 31;
-    v = half_bitonic_step<2, 1>(v, i);
-    v = half_bitonic_step<4, 2>(v, i);
-    v = half_bitonic_step<4, 1>(v, i);
-    v = half_bitonic_step<8, 4>(v, i);
-    v = half_bitonic_step<8, 2>(v, i);
-    v = half_bitonic_step<8, 1>(v, i);
-    v = half_bitonic_step<16, 8>(v, i);
-    v = half_bitonic_step<16, 4>(v, i);
-    v = half_bitonic_step<16, 2>(v, i);
-    v = half_bitonic_step<16, 1>(v, i);
-    v = half_bitonic_step<32, 16>(v, i);
-    v = half_bitonic_step<32, 8>(v, i);
-    v = half_bitonic_step<32, 4>(v, i);
-    v = half_bitonic_step<32, 2>(v, i);
-    v = half_bitonic_step<32, 1>(v, i);
-    return v;
+    half_bitonic_step_kv<2, 1>(v, pl, i);
+    half_bitonic_step_kv<4, 2>(v, pl, i);
+    half_bitonic_step_kv<4, 1>(v, pl, i);
+    half_bitonic_step_kv<8, 4>(v, pl, i);
+    half_bitonic_step_kv<8, 2>(v, pl, i);
+    half_bitonic_step_kv<8, 1>(v, pl, i);
+    half_bitonic_step_kv<16, 8>(v, pl, i);
+    half_bitonic_step_kv<16, 4>(v, pl, i);
+    half_bitonic_step_kv<16, 2>(v, pl, i);
+    half_bitonic_step_kv<16, 1>(v, pl, i);
+    half_bitonic_step_kv<32, 16>(v, pl, i);
+    half_bitonic_step_kv<32, 8>(v, pl, i);
+    half_bitonic_step_kv<32, 4>(v, pl, i);
+    half_bitonic_step_kv<32, 2>(v, pl, i);
+    half_bitonic_step_kv<32, 1>(v, pl, i);
 }
 
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
@@ -550,21 +559,46 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Cut the buffers of the lanes in sm (wave-uniform) back to their 24 best, in order;
-// their cnt becomes 24 and kth the 24th key. All lanes take part.
-__device__ __forceinline__ void settle_lanes(uint64_t* cb, int lane, uint64_t sm, int& cnt, uint64_t& kth) {
+// A slot buffer's entry e of lane `who` as its key, recomputed from the candidate's
+// position as the test formed it: (distance from the query at (x, y, z), id).
+__device__ __forceinline__ uint64_t slot_key(const float4* __restrict__ Ps, const int2* __restrict__ Ms, int slot,
+                                             float x, float y, float z) {
+    const float4 c = Ps[slot];
+    const float ex = c.x - x, ey = c.y - y, ez = c.z - z;
+    const float d2 = ex * ex + ey * ey + ez * ez;
+    return list_key(sqrtf(d2), Ms[slot].x);
+}
+
+__device__ __forceinline__ float readlane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Cut the slot buffers of the lanes in sm (wave-uniform) back to their 24 best (in
+// order); their cnt becomes 24 and kth the 24th key. All lanes take part: the keys are
+// recomputed from the slots, relative to the owner's query (px, py, pz).
+__device__ __forceinline__ void settle_slot_lanes(uint32_t* cb, int lane, uint64_t sm, int& cnt, uint64_t& kth,
+                                                  const float4* __restrict__ Ps, const int2* __restrict__ Ms,
+                                                  float px, float py, float pz) {
     while (sm) {
         const int a = __builtin_ctzll(sm);
         sm &= sm - 1;
-        const int b = sm ? __builtin_ctzll(sm) : a;   // one lane left: both halves sort it
+        const int b = sm ? __builtin_ctzll(sm) : a;
         sm &= sm ? sm - 1 : 0ull;
         const int ca = __builtin_amdgcn_readlane(cnt, a), cb_n = __builtin_amdgcn_readlane(cnt, b);
-        const int who = lane < 32 ? a : b, cw = lane < 32 ? ca : cb_n;
+        const float xa = readlane_f(px, a), ya = readlane_f(py, a), za = readlane_f(pz, a);
+        const float xb = readlane_f(px, b), yb = readlane_f(py, b), zb = readlane_f(pz, b);
+        const bool lo = lane < 32;
+        const int who = lo ? a : b, cw = lo ? ca : cb_n;
         const int i = lane & 31;
-        uint64_t v = i < cw ? cb[i * kTrussWave + who] : ~0ull;
-        v = half_wave_sort32(v, lane);
+        int sl = 0;
+        uint64_t v = ~0ull;
+        if (i < cw) {
+            sl = (int)cb[i * kTrussWave + who];
+            v = slot_key(Ps, Ms, sl, lo ? xa : xb, lo ? ya : yb, lo ? za : zb);
+        }
+        half_wave_sort32_kv(v, sl, lane);
         if (i < kK) {
-            cb[i * kTrussWave + who] = v;
+            cb[i * kTrussWave + who] = (uint32_t)sl;
         }
         const uint64_t ka = readlane64(v, kK - 1), kb = readlane64(v, 32 + kK - 1);
         if (lane == a) {
@@ -684,9 +718,12 @@ __device__ __forceinline__ uint64_t chunk_hits(const QueryOps& qo, const float4*
 constexpr int kMaxRanges = 64;
 
 
-// The lists of all nodes at a rebuild: query q is the node in slot q.
-__global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n, int32_t* __restrict__ nbrT,
-                                                      int32_t* __restrict__ nnb) {
+// The lists of all nodes at a rebuild: query q is the node in slot q. The buffers hold
+// the candidates' slots (4 bytes) and the keys are recomputed from them when a buffer
+// is cut back and at the end: 12 KB of LDS a wave (instead of 20 KB with the keys
+// themselves), so that 12 waves fit a CU.
+__global__ __launch_bounds__(kTrussWave, 3) void k_truss(Hash H, TrussIn in, int n, int32_t* __restrict__ nbrT,
+                                                         int32_t* __restrict__ nnb) {
     __shared__ float4 cpos[kTrussWave];   // the chunk's candidates: position, h
     __shared__ int2 cms[kTrussWave];      // their (id, typ | lab << 8)
     __shared__ int crow[kTrussWave];      // their row (index into the row table)
@@ -699,7 +736,8 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
     static_assert(kMaxRanges >= kTrussWave, "the pairs share the row keys' array");
     int* const pairs = r_key0;
     __shared__ uint64_t r_mask[kMaxRanges];
-    __shared__ uint64_t cb[kCollect * kTrussWave];   // the lanes' lists (list_key entries)
+    __shared__ uint32_t cb[kCollect * kTrussWave];   // the lanes' lists (candidate slots)
+    __shared__ int cslot[kTrussWave];                // the chunk's candidates' slots
     const int lane = threadIdx.x;
     const float tskin = in.t + in.skin;
     const float half = tskin * 0.5f;   // (t + skin) * 0.5f * (hi + hj), left to right
@@ -778,7 +816,7 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                 int rfirst = 0;   // the row holding the loaded chunk's first candidate
                 float4 npos;
                 int2 nms;
-                int nrow;
+                int nrow, nslot;
                 auto fetch = [&](int base) {
                     while (rfirst + 1 < rows && r_beg[rfirst + 1] <= base) {
                         ++rfirst;
@@ -792,9 +830,11 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                         const int s = r_sa[lo] + (p - r_beg[lo]);
                         npos = in.Ps[s];
                         nms = in.Ms[s];
+                        nslot = s;
                     } else {   // past the end: far away, it fails every test
                         npos = make_float4(inf, inf, inf, 0.0f);
                         nms = make_int2(-1, 0);
+                        nslot = 0;
                     }
                     nrow = lo;
                 };
@@ -804,6 +844,7 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                     cpos[lane] = npos;
                     cms[lane] = nms;
                     crow[lane] = nrow;
+                    cslot[lane] = nslot;
                     const int lo = nrow;
                     __syncthreads();
                     if (base + kTrussWave < total) {
@@ -839,7 +880,7 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                         // little room for them: keep the 24 best so far
                         const uint64_t sm = __ballot(pc > kCollect - cnt && kCollect - cnt < kMinRoom);
                         if (sm) {
-                            settle_lanes(cb, lane, sm, cnt, kth);
+                            settle_slot_lanes(cb, lane, sm, cnt, kth, in.Ps, in.Ms, px, py, pz);
                         }
                         const int want = min(pc, kCollect - cnt);
                         int npair = 0;
@@ -887,7 +928,7 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
                         // the owner's count plus the valid pairs of the owner before it
                         if (valid) {
                             const int at = ocnt + __popcll(vm & lanemask_lt() & ~((1ull << (os0 & 63)) - 1));
-                            cb[at * kTrussWave + o] = key;
+                            cb[at * kTrussWave + o] = (uint32_t)cslot[k];
                         }
                         if (take > 0) {
                             const uint64_t span = take == kTrussWave ? ~0ull : ((1ull << take) - 1);
@@ -976,7 +1017,11 @@ __global__ __launch_bounds__(kTrussWave) void k_truss(Hash H, TrussIn in, int n,
         }   // the groups
         if (active) {   // (distance, id) order; the list holds slots
             uint64_t kk[32];
-            settle_keys(cb, lane, cnt, kk);
+#pragma unroll
+            for (int e = 0; e < 32; ++e) {
+                kk[e] = e < cnt ? slot_key(in.Ps, in.Ms, (int)cb[e * kTrussWave + lane], px, py, pz) : ~0ull;
+            }
+            bitonic32(kk);
             const int m = cnt < kK ? cnt : kK;
             nnb[q] = m;
 #pragma unroll
