@@ -335,6 +335,7 @@ struct NodeArrays {
     uint32_t* part;
     uint16_t* lab;
     int32_t* id;
+    uint8_t* sticky;   // handed back to the general move pass before (may be null)
 };
 
 __global__ __launch_bounds__(kBlock) void k_permute(const int32_t* __restrict__ sorted, int n, NodeArrays in,
@@ -356,6 +357,9 @@ __global__ __launch_bounds__(kBlock) void k_permute(const int32_t* __restrict__ 
         out.lab[t] = (uint16_t)l;
         const int o = in.id[s];
         out.id[t] = o;
+        if (in.sticky) {
+            out.sticky[t] = in.sticky[s];
+        }
         Ms[t] = make_int2(o, ty | (l << 8));
         slot_of[o] = t;
     }
@@ -1276,13 +1280,17 @@ __device__ __forceinline__ void wave_append(bool flag, int i, int32_t* count, in
 // pass (their moves need no result of k_move_easy, so that pass can start as soon as
 // the forces are done) and flagged in skip for k_move_easy, which leaves them alone
 // without reading their type (the general pass may be rewriting it).
-__device__ __forceinline__ void k_force_body(const float4* __restrict__ X, const uint8_t* __restrict__ typ, const int32_t* __restrict__ nbrT, const int32_t* __restrict__ nnb, float fscale, float fsurf, int n, float4* __restrict__ F, Ctl* __restrict__ ctl, int32_t* __restrict__ queue_rest, uint8_t* __restrict__ skip, int bid, int nblk) {
+// The forces, and the queue of the general move pass that starts beside k_move_easy:
+// the junction nodes and, when `sticky` is given, the nodes a lean pass has handed back
+// before (their move is the same whichever pass computes it; queued here, it leaves the
+// iteration's critical path).
+__device__ __forceinline__ void k_force_body(const float4* __restrict__ X, const uint8_t* __restrict__ typ, const int32_t* __restrict__ nbrT, const int32_t* __restrict__ nnb, float fscale, float fsurf, int n, float4* __restrict__ F, Ctl* __restrict__ ctl, int32_t* __restrict__ queue_rest, uint8_t* __restrict__ skip, const uint8_t* __restrict__ sticky, int bid, int nblk) {
     const int i = bid * kBlock + threadIdx.x;
     bool junction = false;
     if (i < n) {
         F[i] = bar_force(X, typ, nbrT, nnb, fscale, fsurf, i, n);
         if (queue_rest) {
-            junction = typ[i] == TN_JUNCTION;
+            junction = typ[i] == TN_JUNCTION || (sticky && sticky[i]);
             skip[i] = junction ? 1 : 0;
         }
     }
@@ -1291,8 +1299,8 @@ __device__ __forceinline__ void k_force_body(const float4* __restrict__ X, const
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ X, const uint8_t* __restrict__ typ, const int32_t* __restrict__ nbrT, const int32_t* __restrict__ nnb, float fscale, float fsurf, int n, float4* __restrict__ F, Ctl* __restrict__ ctl, int32_t* __restrict__ queue_rest, uint8_t* __restrict__ skip) {
-    k_force_body(X, typ, nbrT, nnb, fscale, fsurf, n, F, ctl, queue_rest, skip, blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(kBlock) void k_force(const float4* __restrict__ X, const uint8_t* __restrict__ typ, const int32_t* __restrict__ nbrT, const int32_t* __restrict__ nnb, float fscale, float fsurf, int n, float4* __restrict__ F, Ctl* __restrict__ ctl, int32_t* __restrict__ queue_rest, uint8_t* __restrict__ skip, const uint8_t* __restrict__ sticky) {
+    k_force_body(X, typ, nbrT, nnb, fscale, fsurf, n, F, ctl, queue_rest, skip, sticky, blockIdx.x, gridDim.x);
 }
 
 // ---- move (v2m_move, src/opencl/v2m_particle_body.cl:844-935) ----------------------
@@ -1657,7 +1665,7 @@ __global__ __launch_bounds__(kBlock) void k_move_voxel_wave(Field f, MoveArgs ma
 // reads and writes only that node's state, so the passes over disjoint queues may
 // run concurrently.
 template <int kMode, bool kPlain>
-__device__ __forceinline__ void k_move_body(Field f, MoveArgs ma, Ctl* __restrict__ ctl, const int32_t* __restrict__ queue, const float4* __restrict__ F, float4* __restrict__ X, float4* __restrict__ V, uint8_t* __restrict__ typ, uint32_t* __restrict__ part, const uint16_t* __restrict__ lab, float* __restrict__ mv, float* __restrict__ pw, const int32_t* __restrict__ qcount, int32_t* __restrict__ defer_count, int32_t* __restrict__ defer_queue, int bid, int nblk) {
+__device__ __forceinline__ void k_move_body(Field f, MoveArgs ma, Ctl* __restrict__ ctl, const int32_t* __restrict__ queue, const float4* __restrict__ F, float4* __restrict__ X, float4* __restrict__ V, uint8_t* __restrict__ typ, uint32_t* __restrict__ part, const uint16_t* __restrict__ lab, float* __restrict__ mv, float* __restrict__ pw, const int32_t* __restrict__ qcount, int32_t* __restrict__ defer_count, int32_t* __restrict__ defer_queue, uint8_t* __restrict__ sticky, int bid, int nblk) {
     if (kPlain) {
         __builtin_assume(f.gm == 0);
     }
@@ -1692,14 +1700,17 @@ __device__ __forceinline__ void k_move_body(Field f, MoveArgs ma, Ctl* __restric
         }
         if (kLean) {   // every lane of the wave is here
             wave_append(defer, i, defer_count, defer_queue);
+            if (defer && sticky) {   // from the next iteration on, queued by k_force
+                sticky[i] = 1;
+            }
         }
     }
 }
 
 template <int kMode, bool kPlain>
-__global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArgs ma, Ctl* __restrict__ ctl, const int32_t* __restrict__ queue, const float4* __restrict__ F, float4* __restrict__ X, float4* __restrict__ V, uint8_t* __restrict__ typ, uint32_t* __restrict__ part, const uint16_t* __restrict__ lab, float* __restrict__ mv, float* __restrict__ pw, const int32_t* __restrict__ qcount, int32_t* __restrict__ defer_count, int32_t* __restrict__ defer_queue) {
+__global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void k_move(Field f, MoveArgs ma, Ctl* __restrict__ ctl, const int32_t* __restrict__ queue, const float4* __restrict__ F, float4* __restrict__ X, float4* __restrict__ V, uint8_t* __restrict__ typ, uint32_t* __restrict__ part, const uint16_t* __restrict__ lab, float* __restrict__ mv, float* __restrict__ pw, const int32_t* __restrict__ qcount, int32_t* __restrict__ defer_count, int32_t* __restrict__ defer_queue, uint8_t* __restrict__ sticky) {
     k_move_body<kMode, kPlain>(f, ma, ctl, queue, F, X, V, typ, part, lab, mv, pw, qcount, defer_count, defer_queue,
-                               blockIdx.x, gridDim.x);
+                               sticky, blockIdx.x, gridDim.x);
 }
 
 constexpr int kMoveGrid = 2048;       // blocks of a lean k_move at most
@@ -2050,7 +2061,7 @@ __global__ void kg_iter_begin(const PlanIter* __restrict__ tab, const int32_t* _
 __global__ __launch_bounds__(kBlock) void kg_force(const PlanIter* __restrict__ tab, const int32_t* __restrict__ bplan,
                                                    const int32_t* __restrict__ bfirst) {
     TN_PLAN_BLOCK(kShNodes);
-    k_force_body(P.X, P.typ, P.nbrT, P.nnb, P.fscale, P.fsurf, P.n, P.F, P.ctl, P.voxel ? nullptr : P.mq2, P.flag, bid_,
+    k_force_body(P.X, P.typ, P.nbrT, P.nnb, P.fscale, P.fsurf, P.n, P.F, P.ctl, P.voxel ? nullptr : P.mq2, P.flag, nullptr, bid_,
                  nb_);
 }
 
@@ -2089,7 +2100,7 @@ __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void kg_move(const PlanIter*
                                                       : (kQueue == 2 ? &P.ctl->nmove : &P.ctl->nmove_def));
     const bool lean = kMode != kGeneral;
     k_move_body<kMode, kPlain>(P.f, P.ma, P.ctl, queue, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv, P.pw, count,
-                       lean ? &P.ctl->nmove_def : nullptr, lean ? P.mqd : nullptr, bid_, nb_);
+                       lean ? &P.ctl->nmove_def : nullptr, lean ? P.mqd : nullptr, nullptr, bid_, nb_);
 }
 
 __global__ __launch_bounds__(kBlock) void kg_stats(const PlanIter* __restrict__ tab, const int32_t* __restrict__ bplan,

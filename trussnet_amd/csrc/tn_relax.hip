@@ -375,6 +375,9 @@ struct tn_plan {
     uint32_t *dPart0 = nullptr, *dPart = nullptr, *dPartb = nullptr;
     uint16_t *dLab0 = nullptr, *dLab = nullptr, *dLabb = nullptr;
     int32_t *dId = nullptr, *dIdb = nullptr, *dSlotOf = nullptr;
+    // nodes a lean move pass has handed back to the general pass in an earlier iteration
+    // of this run (by slot): k_force queues them for the general pass directly
+    uint8_t *dSticky = nullptr, *dStickyb = nullptr;
     const int32_t* dKeySorted = nullptr;   // the keys in slot order after the last rebuild
     tnk::BrickMap bricks{};                // the K-nearest search's query order (brick positions)
     int32_t* dBrick = nullptr;             // [bricks.size + 1]: node counts, then starts, by brick position
@@ -493,7 +496,8 @@ struct tn_plan {
         timer.end(KC_RADIX, t);
         t = timer.begin();
         dKeySorted = radix_passes == 0 ? dKey : dRk[(radix_passes - 1) & 1];
-        const tnk::NodeArrays cur{ dX, dV, dTyp, dPart, dLab, dId }, nxt{ dXb, dVb, dTypb, dPartb, dLabb, dIdb };
+        const tnk::NodeArrays cur{ dX, dV, dTyp, dPart, dLab, dId, dSticky },
+            nxt{ dXb, dVb, dTypb, dPartb, dLabb, dIdb, dStickyb };
         tnk::k_permute<<<nb, tnk::kBlock, 0, stream>>>(dSorted, n, cur, nxt, dPs, dP0, dMs, dSlotOf);
         std::swap(dX, dXb);
         std::swap(dV, dVb);
@@ -501,6 +505,7 @@ struct tn_plan {
         std::swap(dPart, dPartb);
         std::swap(dLab, dLabb);
         std::swap(dId, dIdb);
+        std::swap(dSticky, dStickyb);
         timer.end(KC_PERMUTE, t);
         t = timer.begin();
         // the query order, into dSorted (free once the nodes are permuted)
@@ -526,8 +531,11 @@ struct tn_plan {
         TN_HIP(hipMemcpyAsync(dTyp, dTyp0, n, hipMemcpyDeviceToDevice, stream));
         TN_HIP(hipMemcpyAsync(dPart, dPart0, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, stream));
         TN_HIP(hipMemcpyAsync(dLab, dLab0, sizeof(uint16_t) * n, hipMemcpyDeviceToDevice, stream));
-        if (n > 0) {   // slot = id
+        if (n > 0) {   // slot = id; no node handed back yet (the buffers are padded to 4 bytes)
             k_iota<<<blocks_for(n, tnk::kBlock), tnk::kBlock, 0, stream>>>(dId, n);
+            const int words = (n + 3) / 4;
+            tnk::k_fill_i32<<<blocks_for(words, tnk::kBlock), tnk::kBlock, 0, stream>>>(
+                reinterpret_cast<int32_t*>(dSticky), words, 0);
         }
         tnk::Ctl c;
         std::memset(&c, 0, sizeof c);
@@ -547,7 +555,6 @@ struct tn_plan {
         timer.collect();
     }
 
-    // one iteration (src/v2m_particles.cpp:828-946); returns false once converged
     // one iteration (src/v2m_particles.cpp:828-946), enqueued: the kernels, then the
     // control block's copy to the host and ev_iter behind them
     void enqueue_iteration() {
@@ -556,7 +563,7 @@ struct tn_plan {
         hipEvent_t t = timer.begin();
         // (the junction nodes queued for the general pass, unless the move is the voxel trap's)
         tnk::k_force<<<nb, tnk::kBlock, 0, stream>>>(dX, dTyp, dNbrT, dNnb, prm.fscale, prm.fsurf, n, dF, dCtl,
-                                                     prm.voxel_trap ? nullptr : dMoveQ2, dFlag);
+                                                     prm.voxel_trap ? nullptr : dMoveQ2, dFlag, dSticky);
         timer.end(KC_FORCE, t);
         t = timer.begin();
         tnk::MoveArgs ma{ prm.dt, prm.maxstep, prm.snap, fire, rest_lanes };
@@ -586,7 +593,7 @@ struct tn_plan {
             }
             hipEvent_t tr = timer.begin(s0);
             tnk::k_move<tnd::kGeneral, kP><<<g2, tnk::kBlock, 0, s0>>>(f, ma, dCtl, dMoveQ2, dF, dX, dV, dTyp, dPart, dLab, dMv,
-                                                                   dPw, &dCtl->nmove_rest, nullptr, nullptr);
+                                                                   dPw, &dCtl->nmove_rest, nullptr, nullptr, nullptr);
             timer.end(KC_MOVE_REST, tr, s0);
             if (overlap_moves) {
                 TN_HIP(hipEventRecord(ev_rest, s0));
@@ -601,7 +608,7 @@ struct tn_plan {
             t = timer.begin(s1);
             tnk::k_move<tnd::kLeanInterior, kP><<<g1, tnk::kBlock, 0, s1>>>(f, ma, dCtl, dMoveQi, dF, dX, dV, dTyp, dPart, dLab,
                                                                         dMv, dPw, &dCtl->nmove_int, &dCtl->nmove_def,
-                                                                        dMoveQd);
+                                                                        dMoveQd, dSticky);
             timer.end(KC_MOVE_INT, t, s1);
             if (overlap_moves) {
                 TN_HIP(hipEventRecord(ev_int, s1));
@@ -609,14 +616,14 @@ struct tn_plan {
             t = timer.begin();
             tnk::k_move<tnd::kLeanInterface, kP><<<g1, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQ, dF, dX, dV, dTyp, dPart,
                                                                              dLab, dMv, dPw, &dCtl->nmove, &dCtl->nmove_def,
-                                                                             dMoveQd);
+                                                                             dMoveQd, dSticky);
             timer.end(KC_MOVE, t);
             if (overlap_moves) {
                 TN_HIP(hipStreamWaitEvent(stream, ev_int, 0));
             }
             t = timer.begin();
             tnk::k_move<tnd::kGeneral, kP><<<g2, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQd, dF, dX, dV, dTyp, dPart, dLab,
-                                                                       dMv, dPw, &dCtl->nmove_def, nullptr, nullptr);
+                                                                       dMv, dPw, &dCtl->nmove_def, nullptr, nullptr, nullptr);
             timer.end(KC_MOVE_DEF, t);
             if (overlap_moves) {
                 TN_HIP(hipStreamWaitEvent(stream, ev_rest, 0));
@@ -844,6 +851,8 @@ tn_plan* plan_build(const tn_relax_params* prm, const tn_nodes_view* seeds, int 
         p->dLabb = p->alloc<uint16_t>(n);
         p->dId = p->alloc<int32_t>(n);
         p->dIdb = p->alloc<int32_t>(n);
+        p->dSticky = p->alloc<uint8_t>(n);
+        p->dStickyb = p->alloc<uint8_t>(n);
         p->dSlotOf = p->alloc<int32_t>(n);
         p->dMv = p->alloc<float>(n);
         p->dPw = p->alloc<float>(n);
@@ -2462,7 +2471,7 @@ int tn_plan_force(tn_plan* p, float* F) {
         }
         tnk::k_force<<<blocks_for(n, tnk::kBlock), tnk::kBlock, 0, p->stream>>>(p->dX, p->dTyp, p->dNbrT, p->dNnb,
                                                                                p->prm.fscale, p->prm.fsurf, n, p->dF,
-                                                                               p->dCtl, nullptr, nullptr);
+                                                                               p->dCtl, nullptr, nullptr, nullptr);
         TN_HIP(hipGetLastError());
         TN_HIP(hipStreamSynchronize(p->stream));
         std::vector<float4> Fs(n);
