@@ -15,9 +15,10 @@ ends with a device synchronisation (the plan's stream), so the host clock
 brackets exactly the device work.
 
 roofline: the kernel with the largest device time in the timed region (HIP
-events on the plan's stream), its algorithmic bytes per launch (the per-node
-figures below, times the nodes the launches processed) over its average launch
-time, against 8 TB/s. traffic: its HBM bytes per launch from two rocprofv3 PMC
+events on the stream each launch is made on; the timed classes that launch the
+same kernel merged), its algorithmic bytes per launch (the per-node figures below,
+times the nodes the launches processed) over its average launch time, against
+8 TB/s. rooflines_next: the next three kernels by device time, in the same terms. traffic: its HBM bytes per launch from two rocprofv3 PMC
 passes (FETCH_SIZE, WRITE_SIZE) over a short run of this script made before the
 timed run, each scaled by what the counter reads for the plain device copy of the
 node positions (16 bytes per node) in the same pass.
@@ -100,6 +101,19 @@ CLASS_KERNELS = {
 }
 
 
+def kernel_key(full: str) -> str:
+    """A trace's kernel name as CLASS_KERNELS writes it: no namespace, no arguments, and
+    the label-volume flag of the move kernels' template dropped ("void
+    tnk::k_move<0, true>(...)" -> "k_move<0>", "k_move_easy<true>" -> "k_move_easy")."""
+    name = full.split("(")[0].split("::")[-1]
+    for flag in ("true", "false"):
+        if name.endswith(f", {flag}>"):
+            name = name[: -len(f", {flag}>")] + ">"
+        elif name.endswith(f"<{flag}>"):
+            name = name[: -len(f"<{flag}>")]
+    return name
+
+
 def pmc_traffic(args) -> dict | None:
     """HBM bytes per launch of each kernel from two rocprofv3 PMC passes (FETCH_SIZE, then
     WRITE_SIZE) over a short run of this script, run before this process touches the GPU.
@@ -130,7 +144,7 @@ def pmc_traffic(args) -> dict | None:
             return None
         per: dict = {}
         for row in csv.DictReader(open(files[0])):
-            name = row["Kernel_Name"].split("(")[0].split("::")[-1]
+            name = kernel_key(row["Kernel_Name"])
             per.setdefault(name, []).append(float(row["Counter_Value"]))
         for name, v in per.items():
             vals.setdefault(name, [0.0, 0.0])[k] = sum(v) / len(v)
@@ -582,33 +596,49 @@ def main() -> int:
         if cnt:
             log(f"  {name:<11} {cnt:6d} launches {ms:9.2f} ms total {1e3 * ms / cnt:9.1f} us/launch "
                 f"{items / cnt:11.0f} items/launch")
-    # roofline of the dominant kernel
+    # roofline of the dominant kernel: the timed classes merged by kernel (the general
+    # move pass runs twice an iteration, as two classes), ranked by device time
     passes = 0 if plan.hash.nkeys <= 1 else (int(plan.hash.nkeys - 1).bit_length() + 7) // 8
     per_item = item_bytes(kbar, passes)
-    ranked = [k for k, v in sorted(timings.items(), key=lambda kv: -kv[1][1])
-              if v[0] and launch_bytes(k, timings, per_item) is not None]
-    dom = ranked[0]
-    cnt, ms, _ = timings[dom]
-    avg_s = ms / cnt / 1e3
-    nbytes = launch_bytes(dom, timings, per_item)
-    achieved = nbytes / avg_s
-    traffic = None
-    if pmc and pmc["copy_kib"][0] > 0 and pmc["copy_kib"][1] > 0:
-        # the copy moves 16 n bytes each way: bytes = counter KiB x 1024 x factor
-        fr = 16.0 * n / (pmc["copy_kib"][0] * 1024.0)
-        fw = 16.0 * n / (pmc["copy_kib"][1] * 1024.0)
-        ks = [pmc["kernels"].get(k) for k in CLASS_KERNELS.get(dom, [])]
-        if ks and all(ks):
-            rd = sum(v[0] for v in ks) * 1024.0 * fr
-            wr = sum(v[1] for v in ks) * 1024.0 * fw
-            traffic = rd + wr
-            log(f"traffic of {dom}: {rd / 1e6:.1f} MB read + {wr / 1e6:.1f} MB written per launch "
-                f"(calibration: read x{fr:.3f}, write x{fw:.3f})")
-    kname = "+".join(CLASS_KERNELS.get(dom, [f"k_{dom}"]))
-    roofline = {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
-                "bytes_per_launch": nbytes, "avg_launch_us": avg_s * 1e6,
-                "items_per_launch": timings[dom][2] / cnt}
+    kernels: dict = {}
+    for cls, (cnt, ms, items) in timings.items():
+        b = launch_bytes(cls, timings, per_item) if cnt else None
+        if b is None:
+            continue
+        k = "+".join(CLASS_KERNELS.get(cls, [f"k_{cls}"]))
+        a = kernels.setdefault(k, {"classes": [], "launches": 0, "ms": 0.0, "bytes": 0.0, "items": 0.0})
+        a["classes"].append(cls)
+        a["launches"] += cnt
+        a["ms"] += ms
+        a["bytes"] += b * cnt
+        a["items"] += items
+
+    def kernel_roofline(kname: str) -> dict:
+        a = kernels[kname]
+        avg_s = a["ms"] / a["launches"] / 1e3
+        nbytes = a["bytes"] / a["launches"]
+        achieved = nbytes / avg_s
+        traffic = None
+        if pmc and pmc["copy_kib"][0] > 0 and pmc["copy_kib"][1] > 0:
+            # the copy moves 16 n bytes each way: bytes = counter KiB x 1024 x factor
+            fr = 16.0 * n / (pmc["copy_kib"][0] * 1024.0)
+            fw = 16.0 * n / (pmc["copy_kib"][1] * 1024.0)
+            ks = [pmc["kernels"].get(k) for k in kname.split("+")]
+            if ks and all(ks):
+                rd = sum(v[0] for v in ks) * 1024.0 * fr
+                wr = sum(v[1] for v in ks) * 1024.0 * fw
+                traffic = rd + wr
+                log(f"traffic of {kname}: {rd / 1e6:.1f} MB read + {wr / 1e6:.1f} MB written per launch "
+                    f"(calibration: read x{fr:.3f}, write x{fw:.3f})")
+        return {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic, "bytes_per_launch": nbytes,
+                "avg_launch_us": avg_s * 1e6, "items_per_launch": a["items"] / a["launches"],
+                "ms_per_step": a["ms"] / args.steps, "classes": a["classes"]}
+
+    ranked = sorted(kernels, key=lambda k: -kernels[k]["ms"])
+    roofline = kernel_roofline(ranked[0])
+    # the next kernels by device time, in the same terms (the K-nearest search among them)
+    others = [kernel_roofline(k) for k in ranked[1:4]]
     loop_bytes = iters * n * (83 + 4 * kbar) + st["rebuilds"] * n * (179 + 4 * kbar)
     cpu = None if args.no_cpu else cpu_baseline(grid, seeds, args.cpu_iters or iters, voxel)
     line = {
@@ -621,6 +651,7 @@ def main() -> int:
                    "parallelism": "single GPU"},
         "hbm_frac_loop": loop_bytes / (dt / args.steps) / HBM_PEAK,
         "roofline": roofline,
+        "rooflines_next": others,
         "cpu_baseline": cpu,
         # the host boundary, outside `value`: plan creation (grid and seeds up, the first
         # rebuild) and the relaxed nodes down
