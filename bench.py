@@ -14,14 +14,17 @@ value = nodes x iterations x steps / wall time of the timed steps. Each step
 ends with a device synchronisation (the plan's stream), so the host clock
 brackets exactly the device work.
 
-roofline: the kernel with the largest device time in the timed region (HIP
-events on the stream each launch is made on; the timed classes that launch the
-same kernel merged), its algorithmic bytes per launch (the per-node figures below,
-times the nodes the launches processed) over its average launch time, against
-8 TB/s. rooflines_next: the next three kernels by device time, in the same terms. traffic: its HBM bytes per launch from two rocprofv3 PMC
-passes (FETCH_SIZE, WRITE_SIZE) over a short run of this script made before the
-timed run, each scaled by what the counter reads for the plain device copy of the
-node positions (16 bytes per node) in the same pass.
+roofline: the kernel with the largest device time (the timed classes that launch
+the same kernel merged: the general move pass runs twice an iteration). The warmup
+times every class with HIP events on the stream of each launch and picks it; the
+timed steps record events around its launches only. Its algorithmic bytes per
+launch (the per-node figures below, times the nodes the launches processed) over
+its average launch time in the timed steps, against 8 TB/s. traffic: its HBM bytes
+per launch from two rocprofv3 PMC passes (FETCH_SIZE, WRITE_SIZE) over a short run
+of this script made before the timed run, each scaled by what the counter reads for
+the plain device copy of the node positions (16 bytes per node) in the same pass.
+rooflines_next: the next three kernels by device time, in the same terms, from the
+warmup's events.
 hbm_frac_loop: the whole loop's algorithmic bytes, iterations x n x (83 + 4 K)
 + rebuilds x n x (179 + 4 K), over the step time, against 8 TB/s.
 cpu_baseline: the reference's own relax_cpu (oracle/_ref, compiled from its
@@ -76,13 +79,15 @@ def item_bytes(kbar: float, radix_passes: int) -> dict:
     }
 
 
-def launch_bytes(name: str, timings: dict, per_item: dict) -> float | None:
-    """Algorithmic bytes of one average launch of a kernel class."""
+def launch_bytes(name: str, timings: dict, per_item: dict, queues: dict | None = None) -> float | None:
+    """Algorithmic bytes of one average launch of a kernel class (queues: the timings the
+    queue lengths come from, when the queue passes were not timed themselves)."""
     launches, _, items = timings[name]
     if not launches:
         return None
     if name == "move_easy":   # every node: typ 1; finished nodes 94 more; queued ones 4 (the queue)
-        queued = sum(timings[k][2] / max(timings[k][0], 1) for k in ("move", "move_int", "move_rest") if k in timings)
+        q = queues or timings
+        queued = sum(q[k][2] / max(q[k][0], 1) for k in ("move", "move_int", "move_rest") if k in q)
         n = items / launches
         return 1 * n + 94 * (n - queued) + 4 * queued
     if name not in per_item:
@@ -99,6 +104,24 @@ CLASS_KERNELS = {
     "keys": ["k_fill_i32", "k_keys"],
     "control": ["k_control"],
 }
+
+
+def by_kernel(timings: dict, per_item: dict, queues: dict | None = None) -> dict:
+    """The timed classes merged by the kernel they launch: {kernel: {classes, launches, ms,
+    bytes, items}} (classes whose bytes per item are not known are left out)."""
+    kernels: dict = {}
+    for cls, (cnt, ms, items) in timings.items():
+        b = launch_bytes(cls, timings, per_item, queues) if cnt else None
+        if b is None:
+            continue
+        k = "+".join(CLASS_KERNELS.get(cls, [f"k_{cls}"]))
+        a = kernels.setdefault(k, {"classes": [], "launches": 0, "ms": 0.0, "bytes": 0.0, "items": 0.0})
+        a["classes"].append(cls)
+        a["launches"] += cnt
+        a["ms"] += ms
+        a["bytes"] += b * cnt
+        a["items"] += items
+    return kernels
 
 
 def kernel_key(full: str) -> str:
@@ -571,10 +594,22 @@ def main() -> int:
     plan = relax.Plan(grid, seeds, prm, device=0)   # grid and seeds up, first rebuild
     plan_ms = 1e3 * (time.perf_counter() - t)
     log(f"plan: {seeds.n} nodes, hash {plan.hash.nlev} levels / {plan.hash.nkeys} cells ({plan_ms:.1f} ms)")
+    # The warmup times every kernel class, to find the dominant kernel; the timed steps
+    # record events around that kernel's launches only (every class's events together
+    # cost about 4% of the step).
+    passes = 0 if plan.hash.nkeys <= 1 else (int(plan.hash.nkeys - 1).bit_length() + 7) // 8
+    plan.set_timing(True)
     for w in range(args.warmup):
         st = plan.run()
         log(f"warmup {w}: {st['iters']} iterations, {st['rebuilds']} rebuilds")
-    plan.set_timing(True)
+    warm = plan.timings() if args.warmup else {}
+    if warm:
+        agg = by_kernel(warm, item_bytes(24.0, passes))
+        top = max(agg, key=lambda k: agg[k]["ms"])
+        plan.set_timing(tuple(agg[top]["classes"]))
+        log(f"timed steps: events around {top} only (the dominant kernel of the warmup)")
+    else:
+        plan.set_timing(True)
     t = time.perf_counter()
     for k in range(args.steps):
         st = plan.run()
@@ -592,28 +627,16 @@ def main() -> int:
     ms_step = 1e3 * dt / args.steps
     log(f"timed: {args.steps} steps in {dt:.3f} s -> {value:.4g} node-iterations/s; "
         f"{st['rebuilds']} rebuilds/step, mean list {kbar:.2f}")
-    for name, (cnt, ms, items) in sorted(timings.items(), key=lambda kv: -kv[1][1]):
-        if cnt:
-            log(f"  {name:<11} {cnt:6d} launches {ms:9.2f} ms total {1e3 * ms / cnt:9.1f} us/launch "
-                f"{items / cnt:11.0f} items/launch")
-    # roofline of the dominant kernel: the timed classes merged by kernel (the general
-    # move pass runs twice an iteration, as two classes), ranked by device time
-    passes = 0 if plan.hash.nkeys <= 1 else (int(plan.hash.nkeys - 1).bit_length() + 7) // 8
+    for label, tm, steps in (("timed", timings, args.steps), ("warmup", warm, args.warmup)):
+        for name, (cnt, ms, items) in sorted(tm.items(), key=lambda kv: -kv[1][1]):
+            if cnt:
+                log(f"  {label} {name:<11} {cnt:6d} launches {ms:9.2f} ms total {1e3 * ms / cnt:9.1f} us/launch "
+                    f"{items / cnt:11.0f} items/launch")
+    # roofline of the dominant kernel (the timed classes merged by kernel: the general
+    # move pass runs twice an iteration, as two classes), from the timed steps' events
     per_item = item_bytes(kbar, passes)
-    kernels: dict = {}
-    for cls, (cnt, ms, items) in timings.items():
-        b = launch_bytes(cls, timings, per_item) if cnt else None
-        if b is None:
-            continue
-        k = "+".join(CLASS_KERNELS.get(cls, [f"k_{cls}"]))
-        a = kernels.setdefault(k, {"classes": [], "launches": 0, "ms": 0.0, "bytes": 0.0, "items": 0.0})
-        a["classes"].append(cls)
-        a["launches"] += cnt
-        a["ms"] += ms
-        a["bytes"] += b * cnt
-        a["items"] += items
 
-    def kernel_roofline(kname: str) -> dict:
+    def kernel_roofline(kernels: dict, kname: str, steps: int) -> dict:
         a = kernels[kname]
         avg_s = a["ms"] / a["launches"] / 1e3
         nbytes = a["bytes"] / a["launches"]
@@ -633,12 +656,18 @@ def main() -> int:
         return {"bound": "hbm", "kernel": kname, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic, "bytes_per_launch": nbytes,
                 "avg_launch_us": avg_s * 1e6, "items_per_launch": a["items"] / a["launches"],
-                "ms_per_step": a["ms"] / args.steps, "classes": a["classes"]}
+                "ms_per_step": a["ms"] / max(steps, 1), "classes": a["classes"]}
 
-    ranked = sorted(kernels, key=lambda k: -kernels[k]["ms"])
-    roofline = kernel_roofline(ranked[0])
-    # the next kernels by device time, in the same terms (the K-nearest search among them)
-    others = [kernel_roofline(k) for k in ranked[1:4]]
+    timed_k = by_kernel(timings, per_item, warm or None)
+    dom = max(timed_k, key=lambda k: timed_k[k]["ms"])
+    roofline = kernel_roofline(timed_k, dom, args.steps)
+    # the next kernels by device time, in the same terms, from the warmup's events
+    others = []
+    if warm:
+        warm_k = by_kernel(warm, per_item)
+        for k in sorted(warm_k, key=lambda k: -warm_k[k]["ms"]):
+            if k != dom and len(others) < 3:
+                others.append(dict(kernel_roofline(warm_k, k, args.warmup), timed="warmup"))
     loop_bytes = iters * n * (83 + 4 * kbar) + st["rebuilds"] * n * (179 + 4 * kbar)
     cpu = None if args.no_cpu else cpu_baseline(grid, seeds, args.cpu_iters or iters, voxel)
     line = {

@@ -368,6 +368,13 @@ typedef struct tn_kernel_time {
 } tn_kernel_time;
 
 int tn_plan_set_timing(tn_plan* plan, int32_t on);
+/*
+ * The same for a subset of the classes: bit k of mask times class k in the order
+ * above (keys = bit 0 ... control = bit 13); the others record no events. A caller
+ * that needs one class's times in a timed region pays for that class's events only.
+ * No counterpart in the reference (its profiling is all or nothing).
+ */
+int tn_plan_set_timing_mask(tn_plan* plan, uint32_t mask);
 int tn_plan_timings(const tn_plan* plan, tn_kernel_time* out, int32_t cap, int32_t* count);
 int tn_plan_rebuild(tn_plan* plan, int32_t* key, int32_t* sorted, int32_t* cstart, int32_t* nbr, int32_t* nnb);
 int tn_plan_force(tn_plan* plan, float* F);

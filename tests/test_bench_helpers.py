@@ -38,3 +38,16 @@ def test_launch_bytes_scale_with_items():
     assert b.launch_bytes("neighbors", t, per) == per["neighbors"] * 1000.0
     assert b.launch_bytes("force", t, per) == per["force"] * 1000.0
     assert b.launch_bytes("force", {"force": (0, 0.0, 0.0)}, per) is None
+
+
+def test_classes_of_one_kernel_merge():
+    b = _bench()
+    per = b.item_bytes(21.0, 3)
+    t = {"move_rest": (3, 1.5, 3 * 4000.0), "move_def": (3, 0.3, 3 * 30.0), "force": (3, 0.5, 3 * 1e6),
+         "control": (3, 0.1, 3.0)}
+    k = b.by_kernel(t, per)
+    assert set(k) == {"k_move<0>", "k_force"}
+    m = k["k_move<0>"]
+    assert m["classes"] == ["move_rest", "move_def"] and m["launches"] == 6
+    assert abs(m["ms"] - 1.8) < 1e-12 and m["items"] == 3 * 4030.0
+    assert abs(m["bytes"] - per["move_rest"] * 3 * 4000.0 - per["move_def"] * 3 * 30.0) < 1e-6

@@ -113,6 +113,30 @@ def test_plan_rerun_is_identical(tn):
             assert plan.download().equal(want)
 
 
+def test_timing_classes(tn):
+    """Timing a subset of the kernel classes records only those, leaves the results
+    unchanged, and the class names come back in the order their mask bits take."""
+    case = Fixture("sectors48")
+    prm, want, _ = case.runs()[1]
+    with tn.Plan(case.grid, case.seeds, prm) as plan:
+        plan.set_timing(("neighbors", "move_rest"))
+        plan.run()
+        assert plan.download().equal(want)
+        t = plan.timings()
+        assert tuple(t) == tn.Plan.TIMING_CLASSES
+        assert t["neighbors"][0] > 0 and t["neighbors"][1] > 0.0
+        for name in ("force", "move_easy", "stats", "keys"):
+            assert t[name][0] == 0, name
+        plan.set_timing(True)
+        plan.run()
+        assert plan.download().equal(want)
+        t = plan.timings()
+        assert t["force"][0] == t["stats"][0] > 0
+        plan.set_timing(False)
+        plan.run()
+        assert all(v[0] == 0 for v in plan.timings().values())
+
+
 def test_steps_compose(tn):
     """Iterations run one call at a time reach the same state as one whole run."""
     case = Fixture("spheres64")

@@ -272,15 +272,18 @@ tnd::Field upload_grid(const tn_grid_view* g, DevMem& mem) {
 enum KClass { KC_KEYS, KC_SCAN, KC_RADIX, KC_PERMUTE, KC_NEIGHBORS, KC_FORCE, KC_MOVE_EASY, KC_MOVE, KC_MOVE_INT,
               KC_MOVE_REST, KC_MOVE_DEF,
               KC_STATS, KC_REFRESH, KC_CONTROL, KC_COUNT };
+constexpr uint32_t kAllClasses = (1u << KC_COUNT) - 1u;
+
 const char* const kKernelNames[KC_COUNT] = { "keys", "scan", "radix_sort", "permute", "neighbors",
                                              "force", "move_easy", "move", "move_int", "move_rest", "move_def",
                                              "stats", "refresh", "control" };
 
 // Device time per kernel class from event pairs on the stream of each launch.
 // Elapsed times are read only after the streams have been synchronised (collect),
-// so timing adds event records to the streams but no host waits.
+// so timing adds event records to the streams but no host waits; `mask` selects the
+// classes timed (bit k: class k), so that a few can be timed at a small cost.
 struct KernelTimer {
-    bool on = false;
+    uint32_t mask = 0;
     hipStream_t stream = nullptr;
     std::vector<hipEvent_t> pool;
     struct Rec {
@@ -311,8 +314,8 @@ struct KernelTimer {
         pool.pop_back();
         return e;
     }
-    hipEvent_t begin(hipStream_t s = nullptr) {
-        if (!on) {
+    hipEvent_t begin(int kc, hipStream_t s = nullptr) {
+        if (!((mask >> kc) & 1u)) {
             return nullptr;
         }
         hipEvent_t e = take();
@@ -320,7 +323,7 @@ struct KernelTimer {
         return e;
     }
     void end(int kc, hipEvent_t a, hipStream_t s = nullptr) {
-        if (!on || !a) {
+        if (!a) {
             return;
         }
         hipEvent_t b = take();
@@ -339,7 +342,7 @@ struct KernelTimer {
         pending.clear();
     }
     void count(int kc, int64_t m) {   // nodes (or cells) a timed launch processed
-        if (on) {
+        if ((mask >> kc) & 1u) {
             items[kc] += m;
         }
     }
@@ -472,14 +475,14 @@ struct tn_plan {
     // (src/v2m_gpu.cpp:288-302)
     void rebuild() {
         const unsigned nb = blocks_for(n, tnk::kBlock);
-        hipEvent_t t = timer.begin();
+        hipEvent_t t = timer.begin(KC_KEYS);
         tnk::k_fill_i32<<<blocks_for(nkeys + 1, tnk::kBlock), tnk::kBlock, 0, stream>>>(dCstart, nkeys + 1, 0);
         tnk::k_keys<<<nb, tnk::kBlock, 0, stream>>>(H, dX, n, tskin, dKey, dCstart);
         timer.end(KC_KEYS, t);
-        t = timer.begin();
+        t = timer.begin(KC_SCAN);
         scan(dCstart, dCstart, nkeys + 1);
         timer.end(KC_SCAN, t);
-        t = timer.begin();
+        t = timer.begin(KC_RADIX);
         if (radix_passes == 0) {
             k_iota<<<nb, tnk::kBlock, 0, stream>>>(dSorted, n);
         }
@@ -494,7 +497,7 @@ struct tn_plan {
                                                                           vout);
         }
         timer.end(KC_RADIX, t);
-        t = timer.begin();
+        t = timer.begin(KC_PERMUTE);
         dKeySorted = radix_passes == 0 ? dKey : dRk[(radix_passes - 1) & 1];
         const tnk::NodeArrays cur{ dX, dV, dTyp, dPart, dLab, dId, dSticky },
             nxt{ dXb, dVb, dTypb, dPartb, dLabb, dIdb, dStickyb };
@@ -507,7 +510,7 @@ struct tn_plan {
         std::swap(dId, dIdb);
         std::swap(dSticky, dStickyb);
         timer.end(KC_PERMUTE, t);
-        t = timer.begin();
+        t = timer.begin(KC_NEIGHBORS);
         // the query order, into dSorted (free once the nodes are permuted)
         tnk::k_fill_i32<<<blocks_for(bricks.size + 1, tnk::kBlock), tnk::kBlock, 0, stream>>>(dBrick, bricks.size + 1,
                                                                                              0);
@@ -560,18 +563,18 @@ struct tn_plan {
     void enqueue_iteration() {
         const unsigned nb = blocks_for(n, tnk::kBlock);
         tnk::k_iter_begin<<<1, 64, 0, stream>>>(dCtl);
-        hipEvent_t t = timer.begin();
+        hipEvent_t t = timer.begin(KC_FORCE);
         // (the junction nodes queued for the general pass, unless the move is the voxel trap's)
         tnk::k_force<<<nb, tnk::kBlock, 0, stream>>>(dX, dTyp, dNbrT, dNnb, prm.fscale, prm.fsurf, n, dF, dCtl,
                                                      prm.voxel_trap ? nullptr : dMoveQ2, dFlag, dSticky);
         timer.end(KC_FORCE, t);
-        t = timer.begin();
+        t = timer.begin(KC_MOVE_EASY);
         tnk::MoveArgs ma{ prm.dt, prm.maxstep, prm.snap, fire, rest_lanes };
         if (prm.voxel_trap) {   // the staircase move: a lane per node, then a wave per node that scans
             tnk::k_move_voxel<<<nb, tnk::kBlock, 0, stream>>>(f, ma, dCtl, n, dF, dX, dV, dTyp, dPart, dLab, dMv,
                                                               dPw, dMoveQ, &dCtl->nmove);
             timer.end(KC_MOVE_EASY, t);
-            t = timer.begin();
+            t = timer.begin(KC_MOVE);
             tnk::k_move_voxel_wave<<<std::min<unsigned>(nb, tnk::kMoveGrid), tnk::kBlock, 0, stream>>>(
                 f, ma, dCtl, dMoveQ, &dCtl->nmove, dF, dX, dV, dTyp, dPart, dLab, dMv, dPw);
             timer.end(KC_MOVE, t);
@@ -591,7 +594,7 @@ struct tn_plan {
                 TN_HIP(hipEventRecord(ev_force, stream));
                 TN_HIP(hipStreamWaitEvent(s0, ev_force, 0));
             }
-            hipEvent_t tr = timer.begin(s0);
+            hipEvent_t tr = timer.begin(KC_MOVE_REST, s0);
             tnk::k_move<tnd::kGeneral, kP><<<g2, tnk::kBlock, 0, s0>>>(f, ma, dCtl, dMoveQ2, dF, dX, dV, dTyp, dPart, dLab, dMv,
                                                                    dPw, &dCtl->nmove_rest, nullptr, nullptr, nullptr);
             timer.end(KC_MOVE_REST, tr, s0);
@@ -605,7 +608,7 @@ struct tn_plan {
                 TN_HIP(hipEventRecord(ev_easy, stream));
                 TN_HIP(hipStreamWaitEvent(s1, ev_easy, 0));
             }
-            t = timer.begin(s1);
+            t = timer.begin(KC_MOVE_INT, s1);
             tnk::k_move<tnd::kLeanInterior, kP><<<g1, tnk::kBlock, 0, s1>>>(f, ma, dCtl, dMoveQi, dF, dX, dV, dTyp, dPart, dLab,
                                                                         dMv, dPw, &dCtl->nmove_int, &dCtl->nmove_def,
                                                                         dMoveQd, dSticky);
@@ -613,7 +616,7 @@ struct tn_plan {
             if (overlap_moves) {
                 TN_HIP(hipEventRecord(ev_int, s1));
             }
-            t = timer.begin();
+            t = timer.begin(KC_MOVE);
             tnk::k_move<tnd::kLeanInterface, kP><<<g1, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQ, dF, dX, dV, dTyp, dPart,
                                                                              dLab, dMv, dPw, &dCtl->nmove, &dCtl->nmove_def,
                                                                              dMoveQd, dSticky);
@@ -621,7 +624,7 @@ struct tn_plan {
             if (overlap_moves) {
                 TN_HIP(hipStreamWaitEvent(stream, ev_int, 0));
             }
-            t = timer.begin();
+            t = timer.begin(KC_MOVE_DEF);
             tnk::k_move<tnd::kGeneral, kP><<<g2, tnk::kBlock, 0, stream>>>(f, ma, dCtl, dMoveQd, dF, dX, dV, dTyp, dPart, dLab,
                                                                        dMv, dPw, &dCtl->nmove_def, nullptr, nullptr, nullptr);
             timer.end(KC_MOVE_DEF, t);
@@ -639,17 +642,17 @@ struct tn_plan {
 
     // stats, own-list refreshes, control, and the control block's copy to the host
     void enqueue_iteration_tail(unsigned nb) {
-        hipEvent_t t = timer.begin();
+        hipEvent_t t = timer.begin(KC_STATS);
         tnk::k_stats<<<nparts, tnk::kBlock, 0, stream>>>(dX, dP0, dMv, dPw, n, prm.skin, fire, dThr, dCtl, dFlag,
                                                          dPartials);
         timer.end(KC_STATS, t);
         // the control before the own-list refreshes, which it does not read: they are
         // skipped when a rebuild or the loop's end follows
-        t = timer.begin();
+        t = timer.begin(KC_CONTROL);
         tnk::CtlArgs ca{ n, prm.max_iters, fire, rebuild_div, prm.dptol, nparts, force_serial };
         tnk::k_control<<<1, tnk::kBlock, 0, stream>>>(ca, dCtl, dPartials, dPw, dSlotOf);
         timer.end(KC_CONTROL, t);
-        t = timer.begin();
+        t = timer.begin(KC_REFRESH);
         tnk::k_refresh_list<<<nb, tnk::kBlock, 0, stream>>>(dFlag, n, dCtl, dQueue);
         tnk::k_truss_node<<<refresh_blocks, tnk::kWave, 0, stream>>>(H, truss_in(), n, dQueue, dCtl, dNbrT, dNnb, dP0);
         timer.end(KC_REFRESH, t);
@@ -821,7 +824,7 @@ tn_plan* plan_build(const tn_relax_params* prm, const tn_nodes_view* seeds, int 
             }
         }
         p->timer.stream = p->stream;
-        p->timer.on = !p->noalloc && (prm->verbose || std::getenv("TN_KERNEL_TIMING") != nullptr);
+        p->timer.mask = !p->noalloc && (prm->verbose || std::getenv("TN_KERNEL_TIMING") != nullptr) ? kAllClasses : 0u;
         {
             const char* e = std::getenv("TN_SERIAL_POWER");
             p->force_serial = e && e[0] && e[0] != '0' ? 1 : 0;
@@ -2302,13 +2305,17 @@ int tn_plan_create(const tn_grid_view* g, const tn_relax_params* prm, const tn_n
 }
 
 int tn_plan_set_timing(tn_plan* p, int32_t on) {
+    return tn_plan_set_timing_mask(p, on != 0 ? kAllClasses : 0u);
+}
+
+int tn_plan_set_timing_mask(tn_plan* p, uint32_t mask) {
     return guarded([&] {
         arg_check(p != nullptr, "null plan");
         TN_HIP(hipSetDevice(p->device));
         TN_HIP(hipStreamSynchronize(p->stream));
         p->timer.collect();
         p->timer.clear();
-        p->timer.on = on != 0;
+        p->timer.mask = mask & kAllClasses;
     });
 }
 

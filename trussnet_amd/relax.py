@@ -29,6 +29,7 @@ SYMBOLS = ["tn_relax_params_default", "tn_last_error", "tn_device_count", "tn_re
            "tn_seed_count", "tn_seed",
            "tn_plan_create", "tn_plan_run", "tn_plan_download", "tn_plan_destroy", "tn_plan_hash_info",
            "tn_plan_reset", "tn_plan_rebuild", "tn_plan_force", "tn_plan_step", "tn_plan_set_timing",
+           "tn_plan_set_timing_mask",
            "tn_plan_timings", "tn_adj_create", "tn_adj_forward", "tn_adj_backward", "tn_adj_loss_terms",
            "tn_adj_destroy", "tn_grid_params_default", "tn_grid_build", "tn_grid_get_info", "tn_grid_download",
            "tn_grid_destroy", "tn_seed_count_from_grid", "tn_seed_from_grid", "tn_plan_create_from_grid",
@@ -90,6 +91,7 @@ def lib():
         L.tn_thin_nodes.argtypes = [C.POINTER(GridView), C.POINTER(RelaxParamsC), C.c_int64] + [vp] * 6 + [C.c_int32]
         L.tn_protect_features.argtypes = [C.POINTER(GridView), vp, C.c_int64, C.c_int64] + [vp] * 6 + [C.c_int32]
         L.tn_plan_set_timing.argtypes = [vp, C.c_int32]
+        L.tn_plan_set_timing_mask.argtypes = [vp, C.c_uint32]
         L.tn_plan_timings.argtypes = [vp, C.POINTER(KernelTimeC), C.c_int32, C.POINTER(C.c_int32)]
         L.tn_seed_count.argtypes = [C.POINTER(GridView), vp, C.POINTER(RelaxParamsC), C.c_int32,
                                     C.POINTER(C.c_int64)]
@@ -310,9 +312,21 @@ class Plan:
                                           _p(sec)))
         return phi, psi, grad, lab, sec
 
-    def set_timing(self, on: bool = True):
-        """Per-kernel device timing (HIP events on the plan's stream); clears the totals."""
-        _check(lib().tn_plan_set_timing(self._h, 1 if on else 0))
+    # the kernel classes, in the order of their bits in a timing mask
+    TIMING_CLASSES = ("keys", "scan", "radix_sort", "permute", "neighbors", "force", "move_easy", "move", "move_int",
+                      "move_rest", "move_def", "stats", "refresh", "control")
+
+    def set_timing(self, on=True):
+        """Per-kernel device timing (HIP events on the stream of each launch); clears the
+        totals. on: True / False for every class, or an iterable of class names to time
+        only those (the others record no events)."""
+        if isinstance(on, bool):
+            _check(lib().tn_plan_set_timing(self._h, 1 if on else 0))
+            return
+        mask = 0
+        for name in on:
+            mask |= 1 << self.TIMING_CLASSES.index(name)
+        _check(lib().tn_plan_set_timing_mask(self._h, mask))
 
     def timings(self) -> dict:
         """{kernel class: (launches, total ms, items)} since timing was switched on (items: the
