@@ -2043,6 +2043,7 @@ struct PlanIter {
     double2* partials;
     int32_t *rqueue, *mq, *mqi, *mq2, *mqd;
     const int32_t* slot_of;
+    uint8_t* sticky;   // handed back to the general pass before (k_force queues them there)
     int nblk[kShapes];
 };
 
@@ -2061,8 +2062,8 @@ __global__ void kg_iter_begin(const PlanIter* __restrict__ tab, const int32_t* _
 __global__ __launch_bounds__(kBlock) void kg_force(const PlanIter* __restrict__ tab, const int32_t* __restrict__ bplan,
                                                    const int32_t* __restrict__ bfirst) {
     TN_PLAN_BLOCK(kShNodes);
-    k_force_body(P.X, P.typ, P.nbrT, P.nnb, P.fscale, P.fsurf, P.n, P.F, P.ctl, P.voxel ? nullptr : P.mq2, P.flag, nullptr, bid_,
-                 nb_);
+    k_force_body(P.X, P.typ, P.nbrT, P.nnb, P.fscale, P.fsurf, P.n, P.F, P.ctl, P.voxel ? nullptr : P.mq2, P.flag,
+                 P.sticky, bid_, nb_);
 }
 
 template <bool kPlain>
@@ -2100,7 +2101,8 @@ __global__ __launch_bounds__(kBlock, TN_MOVE_WAVES) void kg_move(const PlanIter*
                                                       : (kQueue == 2 ? &P.ctl->nmove : &P.ctl->nmove_def));
     const bool lean = kMode != kGeneral;
     k_move_body<kMode, kPlain>(P.f, P.ma, P.ctl, queue, P.F, P.X, P.V, P.typ, P.part, P.lab, P.mv, P.pw, count,
-                       lean ? &P.ctl->nmove_def : nullptr, lean ? P.mqd : nullptr, nullptr, bid_, nb_);
+                       lean ? &P.ctl->nmove_def : nullptr, lean ? P.mqd : nullptr, lean ? P.sticky : nullptr, bid_,
+                       nb_);
 }
 
 __global__ __launch_bounds__(kBlock) void kg_stats(const PlanIter* __restrict__ tab, const int32_t* __restrict__ bplan,

@@ -724,6 +724,7 @@ struct tn_plan {
         a.mq2 = dMoveQ2;
         a.mqd = dMoveQd;
         a.slot_of = dSlotOf;
+        a.sticky = dSticky;
         const int nb = (int)blocks_for(n, tnk::kBlock);
         a.nblk[tnk::kShNodes] = nb;
         a.nblk[tnk::kShLean] = std::min(nb, tnk::kMoveGrid);
@@ -1730,6 +1731,9 @@ void run_grouped(Plans& plans, int64_t& launches, int64_t& steps) {
             }
         }
     } events_gone{ tune_ev };
+    // the side stream of the junction queues' general pass, and its fork and join
+    hipStream_t gs = plans[0]->side[0];
+    hipEvent_t fork_ev = plans[0]->ev_force, join_ev = plans[0]->ev_rest;
     int tune_step = std::getenv("TN_NO_TUNE") ? ncand * kTuneIters : -1;   // -1: the untimed first iteration
     auto set_lanes = [&](int lanes) {
         for (int k = 0; k < count; ++k) {
@@ -1746,16 +1750,22 @@ void run_grouped(Plans& plans, int64_t& launches, int64_t& steps) {
         const bool tuning = tune_step >= 0 && tune_step < ncand * kTuneIters;
         kg_iter_begin<<<nblocks[kShOne], 64, 0, bs>>>(dtab, bplan[kShOne], bfirst[kShOne]);
         kg_force<<<nblocks[kShNodes], kBlock, 0, bs>>>(dtab, bplan[kShNodes], bfirst[kShNodes]);
+        // The general pass over every volume's junction queue runs on a side stream from
+        // the forces on, beside the easy and lean passes (they move disjoint nodes, as in
+        // a single plan's iteration); the pass over the handed-back nodes follows them.
         auto moves = [&](auto plain) {
             constexpr bool kP = decltype(plain)::value;
+            TN_HIP(hipEventRecord(fork_ev, bs));
+            TN_HIP(hipStreamWaitEvent(gs, fork_ev, 0));
+            if (tuning) {
+                TN_HIP(hipEventRecord(tune_ev[0], gs));
+            }
+            kg_move<0, kP><<<nblocks[kShGen], kBlock, 0, gs>>>(dtab, bplan[kShGen], bfirst[kShGen]);
+            if (tuning) {
+                TN_HIP(hipEventRecord(tune_ev[1], gs));
+            }
+            TN_HIP(hipEventRecord(join_ev, gs));
             kg_move_easy<kP><<<nblocks[kShNodes], kBlock, 0, bs>>>(dtab, bplan[kShNodes], bfirst[kShNodes]);
-            if (tuning) {
-                TN_HIP(hipEventRecord(tune_ev[0], bs));
-            }
-            kg_move<0, kP><<<nblocks[kShGen], kBlock, 0, bs>>>(dtab, bplan[kShGen], bfirst[kShGen]);
-            if (tuning) {
-                TN_HIP(hipEventRecord(tune_ev[1], bs));
-            }
             kg_move<1, kP><<<nblocks[kShLean], kBlock, 0, bs>>>(dtab, bplan[kShLean], bfirst[kShLean]);
             kg_move<2, kP><<<nblocks[kShLean], kBlock, 0, bs>>>(dtab, bplan[kShLean], bfirst[kShLean]);
             if (tuning) {
@@ -1765,6 +1775,7 @@ void run_grouped(Plans& plans, int64_t& launches, int64_t& steps) {
             if (tuning) {
                 TN_HIP(hipEventRecord(tune_ev[3], bs));
             }
+            TN_HIP(hipStreamWaitEvent(bs, join_ev, 0));
         };
         if (all_plain) {
             moves(std::true_type{});
