@@ -150,15 +150,60 @@ __global__ void k_iota(int32_t* __restrict__ a, int n) {
     }
 }
 
-// Device buffers freed together (a plan's, or one seeding call's).
-// Device memory owned by one object and freed with it. Small buffers are carved
-// from shared 16 MB chunks (256-byte aligned): a plan holds a few dozen buffers,
-// and every hipMalloc / hipFree costs tens of microseconds and a device-wide sync.
-// With an arena (set_arena), every buffer is carved from that caller-given device
-// memory instead and nothing is allocated or freed; a request beyond its end throws.
+// Freed device buffers kept for reuse, by device and size, so that creating and
+// destroying plans (a batch of 64 volumes, a loop of single calls) rarely calls
+// hipMalloc / hipFree, each of which costs tens of microseconds to milliseconds
+// and a device-wide sync. At most TN_MEM_CACHE_MB (default 8192; 0: off) are kept;
+// the buffers are never returned to the driver before the process ends.
+struct MemCache {
+    std::mutex mu;
+    std::multimap<std::pair<int, size_t>, void*> free;
+    size_t bytes = 0, cap = 0;
+    MemCache() {
+        const char* e = std::getenv("TN_MEM_CACHE_MB");
+        cap = (e && e[0] ? (size_t)std::strtoull(e, nullptr, 10) : (size_t)8192) << 20;
+    }
+    void* take(int dev, size_t b) {
+        std::lock_guard<std::mutex> g(mu);
+        const auto it = free.find({ dev, b });
+        if (it == free.end()) {
+            return nullptr;
+        }
+        void* p = it->second;
+        free.erase(it);
+        bytes -= b;
+        return p;
+    }
+    bool give(int dev, size_t b, void* p) {
+        std::lock_guard<std::mutex> g(mu);
+        if (bytes + b > cap) {
+            return false;
+        }
+        free.emplace(std::make_pair(dev, b), p);
+        bytes += b;
+        return true;
+    }
+};
+
+MemCache& mem_cache() {
+    static MemCache* c = new MemCache;   // (never destroyed: the runtime may be gone at exit)
+    return *c;
+}
+
+// Device memory owned by one object and freed with it (a plan's, or one seeding
+// call's). Small buffers are carved from shared 16 MB chunks (256-byte aligned): a
+// plan holds a few dozen buffers. Buffers come from and go back to the MemCache; the
+// device is synchronised before they go back, as hipFree would. With an arena
+// (set_arena), every buffer is carved from that caller-given device memory instead
+// and nothing is allocated or freed; a request beyond its end throws.
 struct DevMem {
     static constexpr size_t kChunk = (size_t)16 << 20, kAlign = 256;
-    std::vector<void*> owned;
+    struct Owned {
+        void* p;
+        size_t bytes;
+        int dev;
+    };
+    std::vector<Owned> owned;
     char* chunk = nullptr;
     size_t chunk_left = 0;
     int device = -1;
@@ -168,12 +213,29 @@ struct DevMem {
     DevMem(const DevMem&) = delete;
     DevMem& operator=(const DevMem&) = delete;
     ~DevMem() {
+        if (owned.empty()) {
+            return;
+        }
         if (device >= 0) {
             (void)hipSetDevice(device);
         }
-        for (void* p : owned) {
-            (void)hipFree(p);
+        (void)hipDeviceSynchronize();   // no work in flight may still use them
+        for (const Owned& o : owned) {
+            if (!mem_cache().give(o.dev, o.bytes, o.p)) {
+                (void)hipFree(o.p);
+            }
         }
+    }
+    // a buffer of exactly `bytes` on the current device: a cached one, else a new one
+    void* fresh(size_t bytes) {
+        int dev = 0;
+        TN_HIP(hipGetDevice(&dev));
+        void* p = mem_cache().take(dev, bytes);
+        if (!p) {
+            TN_HIP(hipMalloc(&p, bytes));
+        }
+        owned.push_back({ p, bytes, dev });
+        return p;
     }
     void set_arena(void* base, size_t cap) {
         arena = static_cast<char*>(base);
@@ -194,16 +256,10 @@ struct DevMem {
             return p;
         }
         if (bytes > kChunk / 4) {   // a buffer of its own
-            void* p = nullptr;
-            TN_HIP(hipMalloc(&p, bytes));
-            owned.push_back(p);
-            return p;
+            return fresh(bytes);
         }
         if (bytes > chunk_left) {
-            void* p = nullptr;
-            TN_HIP(hipMalloc(&p, kChunk));
-            owned.push_back(p);
-            chunk = static_cast<char*>(p);
+            chunk = static_cast<char*>(fresh(kChunk));
             chunk_left = kChunk;
         }
         void* p = chunk;
