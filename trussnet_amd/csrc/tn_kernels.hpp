@@ -1521,7 +1521,7 @@ __global__ __launch_bounds__(kBlock) void k_move_easy(Field f, MoveArgs ma, Ctl*
 }
 
 #ifndef TN_REST_LANES
-#define TN_REST_LANES 2
+#define TN_REST_LANES 4
 #endif
 constexpr int kRestLanes = TN_REST_LANES;   // nodes per wave in k_move<kGeneral>
 // a lean pass spreads up to this many nodes per wave before it fills whole waves
