@@ -1324,8 +1324,10 @@ struct NodeStep {
     bool moves;
 };
 
-__device__ __forceinline__ NodeStep node_step(const MoveArgs& ma, const Ctl* __restrict__ ctl, float4 Fi,
-                                              const float4* __restrict__ V, int i, float h) {
+// (v_at(): the node's velocity, read only when FIRE uses it)
+template <class VAt>
+__device__ __forceinline__ NodeStep node_step_with(const MoveArgs& ma, const Ctl* __restrict__ ctl, float4 Fi,
+                                                   VAt v_at, float h) {
     NodeStep r;
     r.pw = 0.0f;
     const float stiff = Fi.w > 1.0f ? Fi.w : 1.0f;
@@ -1334,7 +1336,7 @@ __device__ __forceinline__ NodeStep node_step(const MoveArgs& ma, const Ctl* __r
         const float fdt = ctl->dt, falpha = ctl->alpha;
         float v[3] = { 0.0f, 0.0f, 0.0f };
         if (!ctl->zero_v) {
-            const float4 v4 = V[i];
+            const float4 v4 = v_at();
             v[0] = v4.x;
             v[1] = v4.y;
             v[2] = v4.z;
@@ -1369,6 +1371,11 @@ __device__ __forceinline__ NodeStep node_step(const MoveArgs& ma, const Ctl* __r
         }
     }
     return r;
+}
+
+__device__ __forceinline__ NodeStep node_step(const MoveArgs& ma, const Ctl* __restrict__ ctl, float4 Fi,
+                                              const float4* __restrict__ V, int i, float h) {
+    return node_step_with(ma, ctl, Fi, [&] { return V[i]; }, h);
 }
 
 // the writes that end v2m_move for node i (src/opencl/v2m_particle_body.cl:921-934)
@@ -1450,9 +1457,18 @@ __device__ __forceinline__ void k_move_easy_body(Field f, MoveArgs ma, Ctl* __re
     const int g = bid * kBlock + threadIdx.x;
     int i = 0;
     bool surf = false, inner = false;   // the two lean queues
-    if (g < n && !skip[g]) {   // (junction nodes: queued by k_force for the general pass)
+    // The node's own values, read at once rather than each behind the test that needs
+    // it (a node another pass moves reads values it then leaves unused).
+    const bool in = g < n;
+    const int sk = in ? skip[g] : 1;
+    const int ty0 = in ? typ[g] : 0;
+    const float4 x = in ? X[g] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    const float4 Fi = in ? F[g] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float4 Vi = in && ma.fire ? V[g] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const int a = in ? lab[g] : 0;
+    const uint32_t pp = in ? part[g] : 0u;
+    if (in && !sk) {   // (junction nodes: queued by k_force for the general pass)
         i = g;
-        const int ty0 = typ[i];
         if (ty0 == TN_INTERFACE) {
             surf = true;
         } else if (ty0 == TN_CORNER) {
@@ -1462,9 +1478,8 @@ __device__ __forceinline__ void k_move_easy_body(Field f, MoveArgs ma, Ctl* __re
             }
             mv[i] = 0.0f;
         } else {
-            const float4 x = X[i];
             const float h = x.w;
-            const NodeStep st = node_step(ma, ctl, F[i], V, i, h);
+            const NodeStep st = node_step_with(ma, ctl, Fi, [&] { return Vi; }, h);
             if (!st.moves) {
                 if (ma.fire) {
                     V[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1473,7 +1488,6 @@ __device__ __forceinline__ void k_move_easy_body(Field f, MoveArgs ma, Ctl* __re
                 mv[i] = 0.0f;
             } else {
                 // trapped_step's interior branch up to its first decision
-                const int a = lab[i];
                 const P3 p = p3(x.x, x.y, x.z);
                 const P3 q = p3(p.x + st.s[0], p.y + st.s[1], p.z + st.s[2]);
                 // h at the target, loaded before the owner query's chain of loads (it is
@@ -1488,7 +1502,6 @@ __device__ __forceinline__ void k_move_easy_body(Field f, MoveArgs ma, Ctl* __re
                     inner = gn > 1e-9f && gq.w / gn < ma.snap * h;
                 }
                 if (!inner) {   // the rest of trapped_step for an interior node that stays interior
-                    const uint32_t pp = part[i];
                     NodeState ns;
                     ns.p = p;
                     ns.hn = h;
