@@ -1195,22 +1195,27 @@ __device__ __forceinline__ float4 bar_force(const float4* __restrict__ X, const 
     const int ti = typ[i];
     float F0 = 0.0f, F1 = 0.0f, F2 = 0.0f, F3 = 0.0f;
     const int m = nnb[i];
-    // eight bars at a time: their loads are issued together, the sum keeps list order
-    // (entries past the list read the node itself and are not summed)
+    // the whole list read at once, then eight bars at a time: their loads are issued
+    // together, the sum keeps list order (entries past the list read the node itself and
+    // are not summed)
     constexpr int kB = 8;
     static_assert(TN_K % kB == 0, "the list length is a multiple of the batch");
-    for (int k0 = 0; k0 < m; k0 += kB) {
-        int j[kB];
+    int jl[TN_K];
 #pragma unroll
-        for (int u = 0; u < kB; ++u) {
-            j[u] = k0 + u < m ? nbrT[(size_t)(k0 + u) * n_nodes + i] : i;
+    for (int u = 0; u < TN_K; ++u) {
+        jl[u] = u < m ? nbrT[(size_t)u * n_nodes + i] : i;
+    }
+#pragma unroll
+    for (int k0 = 0; k0 < TN_K; k0 += kB) {
+        if (k0 >= m) {
+            break;
         }
         float4 xj[kB];
         int tj[kB];
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
-            xj[u] = X[j[u]];
-            tj[u] = typ[j[u]];
+            xj[u] = X[jl[k0 + u]];
+            tj[u] = typ[jl[k0 + u]];
         }
 #pragma unroll
         for (int u = 0; u < kB; ++u) {
